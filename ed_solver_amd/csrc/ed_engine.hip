@@ -1,0 +1,865 @@
+// H.v kernels and the device-resident Lanczos engine.
+//
+// H.v (HAIMupdo_multr, HAIMupdo.f90:677-1195), gather form, on the column-major
+// d_up x d_dn coefficient matrix V (istate = iup + ido*d_up):
+//
+//   W(:, ido) = [diag_up + diag_dn(ido) + Uimp(imp_up, imp_dn(ido))] o V(:, ido)   (i)-(iii)
+//             + T_up V(:, ido)                                                   (iv) up hops
+//             + sum_k off_dn(ido, k) V(:, jdo_k)                                 (v)  down hops
+//             + sum_t J_t V(swap_up(:, cu_t), swap_dn(ido, cd_t))                (vii)-(viii) Hund
+//
+// One workgroup owns one column (or several short columns): the column is staged in
+// LDS once, the up hops gather from LDS through an ELL table (coalesced over rows),
+// the down hops stream whole neighbour columns (coalesced, row-contiguous), the Hund
+// terms gather single elements.  No atomics: every output element is written by
+// exactly one thread.
+//
+// Lanczos (lanc_opt, Lanczos_plain.h:66-173) runs as two kernels per step on
+// device-resident scalars (LzState), so a chunk of steps needs no host round trip:
+//   k_hv<MODE=1>:   w = H(u_j)/beta_j - (beta_j/beta_{j-1}) u_{j-1};  alpha_j = w.v_j
+//   k_lz_update:    w -= alpha_j v_j;  beta_{j+1} = ||w||;  j += 1
+// with u_j the un-normalised Lanczos vectors (v_j = u_j / beta_j) kept in a Krylov
+// basis on the device, and every reduction finished by the last workgroup in a fixed
+// order (deterministic).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ed_internal.h"
+
+namespace ed {
+
+#define ED_TRY(x)                     \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != ED_OK) return rc_; \
+    } while (0)
+
+// ----------------------------------------------------------- reductions
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum in a fixed order; the result is valid in thread 0.
+__device__ inline double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+    return t;
+}
+
+// Publishes this workgroup's partial (thread 0's value) and lets the last workgroup to
+// arrive reduce all partials in index order.  Returns true in that last workgroup,
+// with the total in *total (thread 0).  The ticket is reset by the last workgroup.
+__device__ bool last_block_total(double part, double *partials, unsigned *ticket, double *red, int *flag,
+                                 double *total)
+{
+    const int nblk = gridDim.x * gridDim.y;
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0) {
+        partials[bid] = part;
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        *flag = (t == (unsigned)(nblk - 1));
+    }
+    __syncthreads();
+    if (!*flag) return false;
+    __threadfence();
+    const volatile double *vp = partials;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) v += vp[i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) {
+        *total = v;
+        *ticket = 0u;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ H.v kernel
+
+// MODE 0: y = H x for grid.y vectors (leading dimension ld).
+// MODE 1: Lanczos phase 1 on the Krylov slots of `lz` (x = u_j, y = u_{j+1}).
+// ONECOL: one column per workgroup (d_up >= workgroup size), column index uniform.
+template <typename T, int MODE, bool ONECOL>
+__global__ __launch_bounds__(kBlock) void k_hv(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+                                                int64_t ld, LzArgs<T> lz)
+{
+    extern __shared__ double smem_d[];
+    T *col = reinterpret_cast<T *>(smem_d);
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+
+    const int dup = a.dup, ncfg = a.ncfg;
+    const int c0 = ONECOL ? blockIdx.x : blockIdx.x * a.cpb;
+    const int ncol = ONECOL ? 1 : min(a.cpb, a.ddn - c0);
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    const T *__restrict__ prev = nullptr;
+    double sj = 1.0, cprev = 0.0;
+    int j = 0;
+    if (MODE == 1) {
+        if (lz.st->status) return;
+        j = lz.st->j;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+        const double bj = lz.st->beta[j - 1];
+        sj = 1.0 / bj;
+        if (j > 1) {
+            prev = lz.slot(j - 1);
+            cprev = bj / lz.st->beta[j - 2];
+        }
+    } else {
+        x = xin + (int64_t)blockIdx.y * ld;
+        y = yout + (int64_t)blockIdx.y * ld;
+    }
+    const int64_t base = (int64_t)c0 * dup;
+    const int nel = ncol * dup;
+    for (int e = threadIdx.x; e < nel; e += kBlock) col[e] = x[base + e];
+    __syncthreads();
+
+    double dot = 0.0;
+    for (int e = threadIdx.x; e < nel; e += kBlock) {
+        const int cl = ONECOL ? 0 : e / dup;
+        const int r = ONECOL ? e : e - cl * dup;
+        const int ido = c0 + cl;
+        const T *cv = col + cl * dup;
+        const int cu = a.imp_up[r], cd = a.imp_dn[ido];
+        const T xv = cv[r];
+        T acc = (a.diag_up[r] + a.diag_dn[ido] + a.uimp[cu * ncfg + cd]) * xv;
+        for (int k = 0; k < a.kup; ++k) acc += a.ell_val_up[k * dup + r] * cv[a.ell_rk_up[k * dup + r]];
+        const int p0 = a.ptr_dn[ido], p1 = a.ptr_dn[ido + 1];
+        for (int k = p0; k < p1; ++k) acc += a.off_dn[k] * x[r + (int64_t)a.rk_dn[k] * dup];
+        if (a.hund) {
+            const int h0 = a.hund_ptr[cu * ncfg + cd], h1 = a.hund_ptr[cu * ncfg + cd + 1];
+            for (int t = h0; t < h1; ++t)
+                acc += a.hund_val[t] *
+                       x[a.swap_up[r * ncfg + a.hund_cu[t]] + (int64_t)a.swap_dn[ido * ncfg + a.hund_cd[t]] * dup];
+        }
+        const int64_t idx = base + e;
+        if (MODE == 1) {
+            double yv = (double)acc * sj;
+            if (j > 1) yv -= cprev * (double)prev[idx];
+            const T yt = (T)yv;
+            y[idx] = yt;
+            dot += (double)yt * ((double)xv * sj);
+        } else {
+            y[idx] = acc;
+        }
+    }
+    if (MODE == 1) {
+        __syncthreads();
+        const double p = block_sum(dot, red);
+        double tot;
+        if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0)
+            lz.st->alpha[j - 1] = tot;
+    }
+}
+
+// Lanczos phase 2: w -= alpha_j v_j, beta_{j+1} = ||w||; advances j when `advance`.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+    if (lz.st->status) return;
+    const int j = lz.st->j;
+    T *__restrict__ y = lz.slot(j + 1);
+    const T *__restrict__ x = lz.slot(j);
+    const double c = lz.st->alpha[j - 1] / lz.st->beta[j - 1];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const T v = (T)((double)y[i] - c * (double)x[i]);
+        y[i] = v;
+        acc += (double)v * (double)v;
+    }
+    const double p = block_sum(acc, red);
+    double tot;
+    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
+        const double b = sqrt(tot);
+        lz.st->beta[j] = b;
+        if (advance) {
+            if (b < kMinLanczosNorm) lz.st->status = 1;
+            lz.st->j = j + 1;
+        }
+    }
+}
+
+// Full reorthogonalisation, classical Gram-Schmidt pass 1/2: coef[k] = w . v_k, k <= j.
+// One workgroup per slice of the vector; partials [k][block]; the last workgroup
+// reduces them in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *coef)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+    if (lz.st->status) return;
+    const int j = lz.st->j;
+    const T *w = lz.slot(j + 1);
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+    for (int k = 1; k <= j; ++k) {
+        const T *v = lz.slot(k);
+        double acc = 0.0;
+        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += (double)w[i] * (double)v[i];
+        acc = block_sum(acc, red);
+        if (threadIdx.x == 0) lz.partials[(int64_t)(k - 1) * gridDim.x + blockIdx.x] = acc / lz.st->beta[k - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned t = atomicAdd(&lz.st->ticket, 1u);
+        flag = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!flag) return;
+    __threadfence();
+    const volatile double *vp = lz.partials;
+    for (int k = 1 + threadIdx.x; k <= j; k += kBlock) {
+        double s = 0.0;
+        for (int b = 0; b < (int)gridDim.x; ++b) s += vp[(int64_t)(k - 1) * gridDim.x + b];
+        coef[k - 1] = s;
+    }
+    if (threadIdx.x == 0) lz.st->ticket = 0u;
+}
+
+// w -= sum_k coef[k] v_k; beta_{j+1} = ||w||; advances j when `advance`.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+    if (lz.st->status) return;
+    const int j = lz.st->j;
+    T *w = lz.slot(j + 1);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double v = (double)w[i];
+        for (int k = 1; k <= j; ++k) v -= coef[k - 1] / lz.st->beta[k - 1] * (double)lz.slot(k)[i];
+        const T vt = (T)v;
+        w[i] = vt;
+        acc += (double)vt * (double)vt;
+    }
+    const double p = block_sum(acc, red);
+    double tot;
+    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
+        const double b = sqrt(tot);
+        lz.st->beta[j] = b;
+        if (advance) {
+            if (b < kMinLanczosNorm) lz.st->status = 1;
+            lz.st->j = j + 1;
+        }
+    }
+}
+
+// u_1 = start (converted to T), beta_1 = ||u_1||, j = 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *start, int64_t n)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+    T *u = lz.slot(1);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const T v = (T)start[i];
+        u[i] = v;
+        acc += (double)v * (double)v;
+    }
+    const double p = block_sum(acc, red);
+    double tot;
+    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
+        const double b = sqrt(tot);
+        lz.st->beta[0] = b;
+        lz.st->j = 1;
+        lz.st->status = (b < kMinLanczosNorm) ? 2 : 0;
+    }
+}
+
+// psi = sum_{k<m} coef[k] u_{k+1} / beta_{k+1} (FP64 output).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_combine(LzArgs<T> lz, int m, const double *coef, double *psi, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double s = 0.0;
+        for (int k = 0; k < m; ++k) s += coef[k] / lz.st->beta[k] * (double)lz.slot(k + 1)[i];
+        psi[i] = s;
+    }
+}
+
+// out[0] = x . y  (deterministic; ticket/partials are the caller's scratch).
+__global__ __launch_bounds__(kBlock) void k_dot(const double *x, const double *y, int64_t n, double *partials,
+                                                unsigned *ticket, double *out)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int flag;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        acc += x[i] * y[i];
+    const double p = block_sum(acc, red);
+    double tot;
+    if (last_block_total(p, partials, ticket, red, &flag, &tot) && threadIdx.x == 0) *out = tot;
+}
+
+__global__ void k_scale(double *x, int64_t n, const double *nrm2)
+{
+    const double s = 1.0 / sqrt(*nrm2);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= s;
+}
+
+__global__ void k_start_vector(int64_t n, double *out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t x = ((uint64_t)(i % 10000) + 1ull) * 0x9E3779B97F4A7C15ull;
+        x ^= x >> 29;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 32;
+        const uint64_t m = x >> 11;
+        const uint64_t k = (m * 1000ull) >> 53;
+        out[i] = (double)(k + 1ull) / 1000.0;
+    }
+}
+
+// ------------------------------------------------------------------ host side
+
+HvArgs<double> hv_args_f64(const ed_sector *s)
+{
+    HvArgs<double> a;
+    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kmax;
+    a.cpb = s->up.d >= kBlock ? 1 : kBlock / std::max(1, s->up.d);
+    a.hund = s->hund;
+    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = s->up.ell_val;
+    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = s->dn.off;
+    a.diag_up = s->up.diag; a.diag_dn = s->dn.diag;
+    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
+    a.uimp = s->d_uimp;
+    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd; a.hund_val = s->d_hund_val;
+    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    return a;
+}
+
+HvArgs<float> hv_args_f32(const ed_sector *s)
+{
+    HvArgs<float> a;
+    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kmax;
+    a.cpb = s->up.d >= kBlock ? 1 : kBlock / std::max(1, s->up.d);
+    a.hund = s->hund;
+    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = s->up.ell_val32;
+    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = s->dn.off32;
+    a.diag_up = s->up.diag32; a.diag_dn = s->dn.diag32;
+    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
+    a.uimp = s->d_uimp32;
+    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd; a.hund_val = s->d_hund_val32;
+    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    return a;
+}
+
+template <typename T>
+static void hv_launch(const HvArgs<T> &a, int mode, const T *x, T *y, int64_t ld, int nvec, const LzArgs<T> &lz,
+                      hipStream_t st)
+{
+    const int nblk = (a.ddn + a.cpb - 1) / a.cpb;
+    const size_t lds = (size_t)a.cpb * a.dup * sizeof(T);
+    dim3 grid(nblk, mode ? 1 : nvec);
+    if (a.cpb == 1) {
+        if (mode) hipLaunchKernelGGL((k_hv<T, 1, true>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+        else hipLaunchKernelGGL((k_hv<T, 0, true>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+    } else {
+        if (mode) hipLaunchKernelGGL((k_hv<T, 1, false>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+        else hipLaunchKernelGGL((k_hv<T, 0, false>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+    }
+}
+
+static int elem_grid(int64_t n)
+{
+    int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st)
+{
+    if (prec == ED_PREC_F64) {
+        LzArgs<double> lz{};
+        hv_launch(hv_args_f64(s), 0, (const double *)x, (double *)y, ld, nvec, lz, st);
+    } else {
+        LzArgs<float> lz{};
+        hv_launch(hv_args_f32(s), 0, (const float *)x, (float *)y, ld, nvec, lz, st);
+    }
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+int start_vector_dev(int64_t n, double *d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_start_vector, dim3(elem_grid(n)), dim3(kBlock), 0, st, n, d_out);
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+static int ensure_state(ed_sector *s)
+{
+    if (!s->d_state) {
+        ED_HIP_CHECK(hipMalloc((void **)&s->d_state, sizeof(LzState)));
+        ED_HIP_CHECK(hipMemset(s->d_state, 0, sizeof(LzState)));
+        ED_HIP_CHECK(hipMalloc((void **)&s->d_partials, sizeof(double) * (size_t)kMaxIter * 2048));
+    }
+    return ED_OK;
+}
+
+static int ensure_io(ed_sector *s)
+{
+    if (!s->d_io_a) {
+        ED_HIP_CHECK(hipMalloc((void **)&s->d_io_a, sizeof(double) * std::max<int64_t>(1, s->dim)));
+        ED_HIP_CHECK(hipMalloc((void **)&s->d_io_b, sizeof(double) * std::max<int64_t>(1, s->dim)));
+    }
+    return ED_OK;
+}
+
+// Krylov basis with nslots vectors of precision prec (leading dimension padded to 64).
+static int ensure_basis(ed_sector *s, int prec, int nslots)
+{
+    const size_t esz = prec == ED_PREC_F64 ? sizeof(double) : sizeof(float);
+    const int64_t ld = (s->dim + 63) / 64 * 64;
+    const size_t need = esz * (size_t)ld * nslots;
+    if (s->d_basis && s->basis_bytes >= need) {
+        s->basis_prec = prec;
+        s->basis_nslots = nslots;
+        s->ld = ld;
+        return ED_OK;
+    }
+    if (s->d_basis) (void)hipFree(s->d_basis);
+    s->d_basis = nullptr;
+    s->basis_bytes = 0;
+    if (s->graph.exec) {
+        (void)hipGraphExecDestroy(s->graph.exec);
+        (void)hipGraphDestroy(s->graph.graph);
+        s->graph = GraphCache();
+    }
+    hipError_t e = hipMalloc(&s->d_basis, need);
+    if (e != hipSuccess)
+        return fail(ED_ERR_NOMEM, "Krylov basis of " + std::to_string(need) + " bytes: " + hipGetErrorString(e));
+    s->basis_bytes = need;
+    s->basis_prec = prec;
+    s->basis_nslots = nslots;
+    s->ld = ld;
+    return ED_OK;
+}
+
+template <typename T>
+static LzArgs<T> lz_args(ed_sector *s)
+{
+    LzArgs<T> lz;
+    lz.base = (T *)s->d_basis;
+    lz.ld = s->ld;
+    lz.nslots = s->basis_nslots;
+    lz.st = s->d_state;
+    lz.partials = s->d_partials;
+    return lz;
+}
+
+// Device scratch for reorthogonalisation coefficients lives after the partials.
+static double *reorth_coef(ed_sector *s) { return s->d_partials + (size_t)(kMaxIter - 1) * 2048; }
+
+template <typename T>
+static void launch_steps_impl(ed_sector *s, const HvArgs<T> &a, int nsteps, int reorth, hipStream_t st)
+{
+    const LzArgs<T> lz = lz_args<T>(s);
+    const int64_t n = s->dim;
+    const int g = elem_grid(n);
+    const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + kBlock * 16 - 1) / (kBlock * 16)));
+    for (int i = 0; i < nsteps; ++i) {
+        hv_launch(a, 1, (const T *)nullptr, (T *)nullptr, 0, 1, lz, st);
+        hipLaunchKernelGGL(k_lz_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, reorth ? 0 : 1);
+        if (reorth) {
+            double *coef = reorth_coef(s);
+            hipLaunchKernelGGL(k_reorth_dots<T>, dim3(gr), dim3(kBlock), 0, st, lz, n, coef);
+            hipLaunchKernelGGL(k_reorth_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, (const double *)coef, 0);
+            hipLaunchKernelGGL(k_reorth_dots<T>, dim3(gr), dim3(kBlock), 0, st, lz, n, coef);
+            hipLaunchKernelGGL(k_reorth_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, (const double *)coef, 1);
+        }
+    }
+}
+
+static void run_steps(ed_sector *s, int prec, int nsteps, int reorth, hipStream_t st)
+{
+    if (prec == ED_PREC_F64) launch_steps_impl<double>(s, hv_args_f64(s), nsteps, reorth, st);
+    else launch_steps_impl<float>(s, hv_args_f32(s), nsteps, reorth, st);
+}
+
+// Runs nsteps Lanczos steps; chunks of exactly `chunk` steps replay a captured graph.
+static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
+{
+    hipStream_t st = s->stream;
+    if (nsteps == chunk && chunk > 1) {
+        GraphCache &G = s->graph;
+        const int key_prec = prec * 2 + (reorth ? 1 : 0);
+        if (!(G.exec && G.prec == key_prec && G.chunk == chunk && G.nslots == s->basis_nslots &&
+              G.base == s->d_basis)) {
+            if (G.exec) {
+                (void)hipGraphExecDestroy(G.exec);
+                (void)hipGraphDestroy(G.graph);
+                G = GraphCache();
+            }
+            ED_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            run_steps(s, prec, nsteps, reorth, st);
+            ED_HIP_CHECK(hipStreamEndCapture(st, &G.graph));
+            ED_HIP_CHECK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+            G.prec = key_prec;
+            G.chunk = chunk;
+            G.nslots = s->basis_nslots;
+            G.base = s->d_basis;
+        }
+        ED_HIP_CHECK(hipGraphLaunch(G.exec, st));
+    } else {
+        run_steps(s, prec, nsteps, reorth, st);
+    }
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+static int lz_start(ed_sector *s, int prec, const double *h_start, const double *d_start)
+{
+    hipStream_t st = s->stream;
+    const double *src = d_start;
+    if (!src) {
+        ED_TRY(ensure_io(s));
+        if (h_start)
+            ED_HIP_CHECK(hipMemcpyAsync(s->d_io_a, h_start, sizeof(double) * s->dim, hipMemcpyHostToDevice, st));
+        else
+            ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
+        src = s->d_io_a;
+    }
+    const int g = elem_grid(s->dim);
+    if (prec == ED_PREC_F64)
+        hipLaunchKernelGGL(k_lz_init<double>, dim3(g), dim3(kBlock), 0, st, lz_args<double>(s), src, s->dim);
+    else
+        hipLaunchKernelGGL(k_lz_init<float>, dim3(g), dim3(kBlock), 0, st, lz_args<float>(s), src, s->dim);
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+struct HostLz {
+    int j, status;
+    std::vector<double> alpha, beta;
+};
+
+static int read_state(ed_sector *s, HostLz &h, int upto)
+{
+    // copies j, status, alpha[0..upto), beta[0..upto]
+    LzState *d = s->d_state;
+    int hdr[2];
+    h.alpha.resize(std::max(upto, 1));
+    h.beta.resize(upto + 1);
+    ED_HIP_CHECK(hipMemcpyAsync(hdr, d, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (upto > 0)
+        ED_HIP_CHECK(hipMemcpyAsync(h.alpha.data(), (char *)d + offsetof(LzState, alpha), sizeof(double) * upto,
+                                    hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipMemcpyAsync(h.beta.data(), (char *)d + offsetof(LzState, beta), sizeof(double) * (upto + 1),
+                                hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipStreamSynchronize(s->stream));
+    h.j = hdr[0];
+    h.status = hdr[1];
+    return ED_OK;
+}
+
+static int check_common(ed_sector *s, int prec)
+{
+    if (!s) return fail(ED_ERR_ARG, "NULL sector");
+    if (prec != ED_PREC_F64 && prec != ED_PREC_F32) return fail(ED_ERR_ARG, "precision must be ED_PREC_F64 or ED_PREC_F32");
+    ED_HIP_CHECK(hipSetDevice(s->device));
+    return ED_OK;
+}
+
+// Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134).
+static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const double *start, int *iters_out,
+                               int *neig_out, double *evals, double *vecp, double *diag, double *subdiag)
+{
+    ED_TRY(check_common(s, o ? o->precision : 0));
+    if (!o || !iters_out) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: NULL argument");
+    const int64_t D = s->dim;
+    const int niter = (int)std::min<int64_t>(D - 1, o->nitermax);
+    if (niter < 1) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: sector dimension 1 has no Lanczos step (full ED case)");
+    if (niter > kMaxIter - 2) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: nitermax above " + std::to_string(kMaxIter - 2));
+    int neig = (int)std::min<int64_t>(o->neigen, D);
+    neig = std::min(neig, niter);
+    if (neig < 1) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: neigen < 1");
+    const int chunk = std::max(1, o->check_every);
+    ED_TRY(ensure_state(s));
+    ED_TRY(ensure_basis(s, o->precision, niter + 1));
+    s->basis_valid_iters = 0;
+    ED_TRY(lz_start(s, o->precision, start, nullptr));
+    std::vector<double> last(neig, 1e300), w(neig);
+    std::vector<int> conv(neig, 0);
+    HostLz h;
+    int done = 0, final_iter = 0;
+    while (done < niter && !final_iter) {
+        const int c = std::min(chunk, niter - done);
+        ED_TRY(steps(s, o->precision, c, chunk, o->reorth));
+        ED_TRY(read_state(s, h, done + c));
+        if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
+        const int completed = h.j - 1;   // steps whose beta_{j+1} is known
+        for (int it = done + 1; it <= done + c; ++it) {
+            if (it > completed)   // the reference STOPs at the start of this step (Lanczos_plain.h:85-92)
+                return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(it));
+            tridiag_lowest(it, h.alpha.data(), h.beta.data(), neig, w.data());
+            const int m = std::min(neig, it);
+            bool allc = true;
+            for (int k = 0; k < neig; ++k) {
+                conv[k] = (k < m) && std::fabs(w[k] - last[k]) < o->tolerance && it > 2;
+                if (!conv[k]) allc = false;
+            }
+            const bool one_step = (it == 1) && std::fabs(h.beta[1]) < 1e-24;
+            if (allc || it == niter || one_step) {
+                final_iter = it;
+                break;
+            }
+            for (int k = 0; k < m; ++k) last[k] = w[k];
+        }
+        done += c;
+    }
+    const int it = final_iter;
+    std::vector<double> ev(it), z((size_t)it * it);
+    ED_TRY(tridiag_eig(it, h.alpha.data(), h.beta.data(), ev.data(), z.data()));
+    for (int k = 0; k < neig; ++k) {
+        if (evals) evals[k] = ev[k];
+        if (vecp)
+            for (int r = 0; r < it; ++r) vecp[(size_t)k * it + r] = z[(size_t)k * it + r];
+    }
+    if (diag) std::memcpy(diag, h.alpha.data(), sizeof(double) * it);
+    if (subdiag) std::memcpy(subdiag, h.beta.data(), sizeof(double) * it);
+    *iters_out = it;
+    if (neig_out) *neig_out = neig;
+    s->basis_valid_iters = it;
+    return ED_OK;
+}
+
+static int lanczos_run(ed_sector *s, int niter, const double *h_start, const double *d_start, int prec, int chunk,
+                       double *diag, double *subdiag, int *breakdown_at)
+{
+    ED_TRY(check_common(s, prec));
+    if (niter < 1 || niter > kMaxIter - 2) return fail(ED_ERR_ARG, "ed_lanczos_run: niter outside [1, 2046]");
+    ED_TRY(ensure_state(s));
+    ED_TRY(ensure_basis(s, prec, niter + 1));
+    s->basis_valid_iters = 0;
+    ED_TRY(lz_start(s, prec, h_start, d_start));
+    int done = 0;
+    while (done < niter) {
+        const int c = std::min(chunk, niter - done);
+        ED_TRY(steps(s, prec, c, chunk, 0));
+        done += c;
+    }
+    HostLz h;
+    ED_TRY(read_state(s, h, niter));
+    if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
+    int reached = h.j - 1;   // completed steps
+    if (breakdown_at) *breakdown_at = (h.status == 1) ? reached : 0;
+    for (int i = 0; i < niter; ++i) {
+        if (diag) diag[i] = i < reached ? h.alpha[i] : 0.0;
+        if (subdiag) subdiag[i] = i <= reached ? h.beta[i] : 0.0;
+    }
+    s->basis_valid_iters = reached;
+    if (h.status == 1 && reached < niter)
+        return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(reached + 1) +
+                                          " (Lanczos_plain.h:85-92)");
+    return ED_OK;
+}
+
+static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_psi)
+{
+    ED_TRY(check_common(s, 0));
+    if (!vecp || !d_psi || iters < 1) return fail(ED_ERR_ARG, "ed_lanczos_gs_vector: bad argument");
+    if (iters > s->basis_valid_iters || !s->d_basis)
+        return fail(ED_ERR_STATE, "ed_lanczos_gs_vector: no Krylov basis of " + std::to_string(iters) +
+                                      " vectors kept on this sector (run a Lanczos call first)");
+    hipStream_t st = s->stream;
+    double *d_coef = s->d_partials + (size_t)(kMaxIter - 2) * 2048;
+    ED_HIP_CHECK(hipMemcpyAsync(d_coef, vecp, sizeof(double) * iters, hipMemcpyHostToDevice, st));
+    const int g = elem_grid(s->dim);
+    if (s->basis_prec == ED_PREC_F64)
+        hipLaunchKernelGGL(k_combine<double>, dim3(g), dim3(kBlock), 0, st, lz_args<double>(s), iters,
+                           (const double *)d_coef, d_psi, s->dim);
+    else
+        hipLaunchKernelGGL(k_combine<float>, dim3(g), dim3(kBlock), 0, st, lz_args<float>(s), iters,
+                           (const double *)d_coef, d_psi, s->dim);
+    double *d_nrm = d_coef + iters;
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)d_psi, (const double *)d_psi, s->dim,
+                       s->d_partials, &s->d_state->ticket, d_nrm);
+    hipLaunchKernelGGL(k_scale, dim3(g), dim3(kBlock), 0, st, d_psi, s->dim, (const double *)d_nrm);
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" {
+
+int ed_hmult_f64_dev(ed_sector *sec, int64_t n, double *d_out, const double *d_in, void *stream)
+{
+    ED_TRY(check_common(sec, 0));
+    if (n != sec->dim || !d_out || !d_in) return fail(ED_ERR_ARG, "ed_hmult_f64_dev: size mismatch or NULL vector");
+    return launch_hv(sec, ED_PREC_F64, d_in, d_out, 0, 1, stream ? (hipStream_t)stream : sec->stream);
+}
+
+int ed_hmult_f32_dev(ed_sector *sec, int64_t n, float *d_out, const float *d_in, void *stream)
+{
+    ED_TRY(check_common(sec, 0));
+    if (n != sec->dim || !d_out || !d_in) return fail(ED_ERR_ARG, "ed_hmult_f32_dev: size mismatch or NULL vector");
+    return launch_hv(sec, ED_PREC_F32, d_in, d_out, 0, 1, stream ? (hipStream_t)stream : sec->stream);
+}
+
+int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, double *d_out, const double *d_in,
+                           void *stream)
+{
+    ED_TRY(check_common(sec, 0));
+    if (n != sec->dim || nvec < 1 || nvec > 65535 || ld < n || !d_out || !d_in)
+        return fail(ED_ERR_ARG, "ed_hmult_block_f64_dev: bad sizes");
+    return launch_hv(sec, ED_PREC_F64, d_in, d_out, ld, nvec, stream ? (hipStream_t)stream : sec->stream);
+}
+
+int ed_sector_synchronize(ed_sector *sec)
+{
+    ED_TRY(check_common(sec, 0));
+    ED_HIP_CHECK(hipStreamSynchronize(sec->stream));
+    return ED_OK;
+}
+
+int ed_hmult_f64(ed_sector *sec, int64_t n, double *vec_out, const double *vec_in)
+{
+    ED_TRY(check_common(sec, 0));
+    if (n != sec->dim || !vec_out || !vec_in) return fail(ED_ERR_ARG, "ed_hmult_f64: n differs from dimen_H()");
+    ED_TRY(ensure_io(sec));
+    hipStream_t st = sec->stream;
+    ED_HIP_CHECK(hipMemcpyAsync(sec->d_io_a, vec_in, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
+    ED_HIP_CHECK(hipMemcpyAsync(vec_out, sec->d_io_b, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
+int ed_start_vector(int64_t n, double *out)
+{
+    if (n < 1 || !out) return fail(ED_ERR_ARG, "ed_start_vector: bad argument");
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t x = ((uint64_t)(i % 10000) + 1ull) * 0x9E3779B97F4A7C15ull;
+        x ^= x >> 29;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 32;
+        const uint64_t m = x >> 11;
+        out[i] = (double)(((m * 1000ull) >> 53) + 1ull) / 1000.0;
+    }
+    return ED_OK;
+}
+
+int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const double *start, int *iters,
+                           int *neig_out, double *evals, double *vecp, double *diag, double *subdiag)
+{
+    return lanczos_diagonalize(sec, opts, start, iters, neig_out, evals, vecp, diag, subdiag);
+}
+
+int ed_lanczos_run(ed_sector *sec, int niter, const double *start, int precision, double *diag, double *subdiag)
+{
+    return lanczos_run(sec, niter, start, nullptr, precision, 8, diag, subdiag, nullptr);
+}
+
+int ed_lanczos_gs_vector_dev(ed_sector *sec, int iters, const double *vecp, double *d_psi)
+{
+    return gs_vector_dev(sec, iters, vecp, d_psi);
+}
+
+int ed_lanczos_gs_vector(ed_sector *sec, int iters, const double *vecp, double *psi)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!psi) return fail(ED_ERR_ARG, "ed_lanczos_gs_vector: NULL psi");
+    ED_TRY(ensure_io(sec));
+    ED_TRY(gs_vector_dev(sec, iters, vecp, sec->d_io_b));
+    ED_HIP_CHECK(hipMemcpyAsync(psi, sec->d_io_b, sizeof(double) * sec->dim, hipMemcpyDeviceToHost, sec->stream));
+    ED_HIP_CHECK(hipStreamSynchronize(sec->stream));
+    return ED_OK;
+}
+
+int ed_rayleigh_f64(ed_sector *sec, int64_t n, const double *psi, double *energy)
+{
+    ED_TRY(check_common(sec, 0));
+    if (n != sec->dim || !psi || !energy) return fail(ED_ERR_ARG, "ed_rayleigh_f64: bad argument");
+    ED_TRY(ensure_io(sec));
+    ED_TRY(ensure_state(sec));
+    hipStream_t st = sec->stream;
+    ED_HIP_CHECK(hipMemcpyAsync(sec->d_io_a, psi, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
+    double *d_r = sec->d_partials + (size_t)(kMaxIter - 3) * 2048;
+    const int g = elem_grid(n);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_a, (const double *)sec->d_io_b,
+                       n, sec->d_partials, &sec->d_state->ticket, d_r);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_a, (const double *)sec->d_io_a,
+                       n, sec->d_partials, &sec->d_state->ticket, d_r + 1);
+    double h[2];
+    ED_HIP_CHECK(hipMemcpyAsync(h, d_r, sizeof(h), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    *energy = h[0] / h[1];
+    return ED_OK;
+}
+
+int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, double *bytes_per_launch)
+{
+    ED_TRY(check_common(sec, precision));
+    if (reps < 1 || reps > kMaxIter - 2 || !avg_ms) return fail(ED_ERR_ARG, "ed_hmult_benchmark: bad argument");
+    if (sec->dim < 2) return fail(ED_ERR_ARG, "ed_hmult_benchmark: sector of dimension 1");
+    hipStream_t st = sec->stream;
+    ED_TRY(ensure_state(sec));
+    ED_TRY(ensure_basis(sec, precision, reps + 1));
+    sec->basis_valid_iters = 0;
+    ED_TRY(lz_start(sec, precision, nullptr, nullptr));
+    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+    for (auto &e : ev) ED_HIP_CHECK(hipEventCreate(&e));
+    const int64_t n = sec->dim;
+    const int g = elem_grid(n);
+    for (int r = 0; r < reps; ++r) {
+        ED_HIP_CHECK(hipEventRecord(ev[2 * r], st));
+        if (precision == ED_PREC_F64) {
+            const LzArgs<double> lz = lz_args<double>(sec);
+            hv_launch(hv_args_f64(sec), 1, (const double *)nullptr, (double *)nullptr, 0, 1, lz, st);
+            ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
+            hipLaunchKernelGGL(k_lz_update<double>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
+        } else {
+            const LzArgs<float> lz = lz_args<float>(sec);
+            hv_launch(hv_args_f32(sec), 1, (const float *)nullptr, (float *)nullptr, 0, 1, lz, st);
+            ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
+            hipLaunchKernelGGL(k_lz_update<float>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
+        }
+    }
+    ED_HIP_CHECK(hipGetLastError());
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    double tot = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        float ms = 0.f;
+        ED_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+        tot += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    const size_t esz = precision == ED_PREC_F64 ? sizeof(double) : sizeof(float);
+    *avg_ms = tot / reps;
+    if (bytes_per_launch) *bytes_per_launch = 3.0 * esz * (double)n;
+    return ED_OK;
+}
+
+}  // extern "C"
+
+// Internal entry points used by the Green's-function translation unit.
+namespace ed {
+int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, double *diag, double *subdiag,
+                    int *breakdown_at)
+{
+    return lanczos_run(s, niter, nullptr, d_start, prec, 8, diag, subdiag, breakdown_at);
+}
+}  // namespace ed

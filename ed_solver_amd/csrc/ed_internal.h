@@ -1,0 +1,137 @@
+// Internal definitions shared by the ed_hip translation units (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ed_hip.h"
+
+namespace ed {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define ED_HIP_CHECK(expr)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::ed::fail(ED_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
+constexpr int kMaxPartials = 4096;   // per-block partial sums of one reduction
+constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
+
+// ------------------------------------------------------------ device state
+// Recurrence scalars of one Lanczos run, resident on the device so that a chunk of
+// iterations runs (and can be replayed from a captured graph) without the host.
+struct LzState {
+    int j;             // current iteration (1-based): the next kernel pair computes v_{j+1}
+    int status;        // 0 ok, 1 zero-norm vector appeared, 2 zero-norm start vector
+    unsigned ticket;   // last-block-done counter of the running reduction (reset by its owner)
+    int pad;
+    double alpha[kMaxIter];      // alpha_j       -> alpha[j-1]
+    double beta[kMaxIter + 1];   // beta_1=||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
+};
+
+// One spin's tables on the device (tab_HAIM2 output plus derived layouts).
+struct SpinTables {
+    int d = 0;              // spin-sector dimension
+    int64_t nnz = 0;        // tabulated off-diagonal entries
+    int kmax = 0;           // max entries per state (ELL width)
+    int32_t *states = nullptr;   // [d]
+    double *diag = nullptr;      // [d]
+    int32_t *noff = nullptr;     // [d]
+    int32_t *ptr = nullptr;      // [d+1] exclusive prefix sum of noff
+    int32_t *rk = nullptr;       // [nnz] 0-based target rank (CSR order = reference order)
+    double *off = nullptr;       // [nnz]
+    int32_t *ell_rk = nullptr;   // [kmax][d], padded with the row itself
+    double *ell_val = nullptr;   // [kmax][d], padded with 0
+    int32_t *imp = nullptr;      // [d] impurity bits (state >> nb)
+    int32_t *swap = nullptr;     // [d][ncfg] rank of the state with its impurity bits replaced
+    // FP32 copies for the mixed-precision kernels
+    float *diag32 = nullptr, *off32 = nullptr, *ell_val32 = nullptr;
+};
+
+// Kernel-side view of a sector's Hamiltonian for precision T.
+template <typename T>
+struct HvArgs {
+    int dup, ddn, ncfg, kup, cpb, hund;
+    const int32_t *ell_rk_up;
+    const T *ell_val_up;
+    const int32_t *ptr_dn, *rk_dn;
+    const T *off_dn;
+    const T *diag_up, *diag_dn;
+    const int32_t *imp_up, *imp_dn;
+    const T *uimp;                 // [ncfg][ncfg] interaction diagonal per impurity pair
+    const int32_t *hund_ptr;       // [ncfg*ncfg + 1]
+    const int32_t *hund_cu, *hund_cd;
+    const T *hund_val;
+    const int32_t *swap_up, *swap_dn;
+};
+
+// Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1.
+template <typename T>
+struct LzArgs {
+    T *base;
+    int64_t ld;
+    int nslots;
+    LzState *st;
+    double *partials;
+    __host__ __device__ T *slot(int j) const { return base + (int64_t)((j - 1) % nslots) * ld; }
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int prec = -1, chunk = 0, nslots = 0;
+    void *base = nullptr;
+};
+
+}  // namespace ed
+
+struct ed_sector {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nc = 0, nb = 0, ns = 0, nup = 0, ndn = 0, ncfg = 0;
+    int64_t dim = 0;
+    std::vector<double> ec, eb, vbc, u, jj;
+    double jhund = 0.0, cutoff = 0.0;
+    int32_t *d_rank = nullptr;       // [2^ns] 0-based rank inside the popcount bucket
+    ed::SpinTables up, dn;
+    double *d_uimp = nullptr;
+    float *d_uimp32 = nullptr;
+    int hund = 0, nhund = 0;
+    int32_t *d_hund_ptr = nullptr, *d_hund_cu = nullptr, *d_hund_cd = nullptr;
+    double *d_hund_val = nullptr;
+    float *d_hund_val32 = nullptr;
+    // Lanczos work space
+    void *d_basis = nullptr;
+    size_t basis_bytes = 0;
+    int basis_prec = -1, basis_nslots = 0, basis_valid_iters = 0;
+    int64_t ld = 0;
+    ed::LzState *d_state = nullptr;
+    double *d_partials = nullptr;
+    double *d_io_a = nullptr, *d_io_b = nullptr;   // host-API staging vectors [dim]
+    ed::GraphCache graph;
+    // timing of the H.v kernel inside Lanczos chunks
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double hv_ms_sum = 0.0;
+    int hv_count = 0;
+    int hv_prec = 0;
+};
+
+namespace ed {
+// tridiagonal eigen-solver (host, FP64): implicit QL with Wilkinson shifts
+int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z);
+void tridiag_lowest(int n, const double *diag, const double *off, int k, double *w);
+// launch helpers implemented in ed_engine.hip
+int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st);
+HvArgs<double> hv_args_f64(const ed_sector *s);
+HvArgs<float> hv_args_f32(const ed_sector *s);
+int start_vector_dev(int64_t n, double *d_out, hipStream_t st);
+}  // namespace ed

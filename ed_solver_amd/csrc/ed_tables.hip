@@ -1,0 +1,488 @@
+// Sector construction on the device: Fock-sector enumeration, rank tables and the
+// quadratic-part tables of tab_HAIM2, all as integer kernels, bit-exact with the
+// reference.
+//
+//   k_enum_states   quantum_fermion_hilbert_class.f90:346-421 (states of popcount N in
+//                   increasing integer order) -- here by combinatorial unranking: the
+//                   k-th N-subset in colexicographic order, which is increasing order.
+//   k_rank_table    rank(state) -- here the combinatorial number system
+//                   sum_i C(c_i, i) over the set bits c_1 < c_2 < ...
+//   k_tab_count /   tab_HAIM2 count and fill passes (HAIM_offdiag_arrays.f90:89-148,
+//   k_tab_fill      210-606), same entry order: impurity hops, bath hops, V b+c, V* c+b.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ed_internal.h"
+
+namespace ed {
+
+__host__ __device__ inline int binom(int n, int k)
+{
+    if (k < 0 || n < k) return 0;
+    if (k > n - k) k = n - k;
+    int r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Sign of c+/c on `bit` = (-1)^{# occupied orbitals above bit} of the resulting state
+// (Fermion_sign, quantum_fermion_ket_class.f90:605-618).  Bits above `bit` do not
+// change under the operator, so the input state can be used.
+__device__ inline int sign_above(uint32_t s, int bit)
+{
+    return (__popc(s >> (bit + 1)) & 1) ? -1 : 1;
+}
+
+// hop(out, to, from, in) = c+_to c_from: destroy then create (ket_class.f90:483-492).
+__device__ inline bool hop(uint32_t s, int to, int from, uint32_t *out, int *sg)
+{
+    if (!((s >> from) & 1u)) return false;
+    uint32_t t = s & ~(1u << from);
+    int g = sign_above(t, from);
+    if ((t >> to) & 1u) return false;
+    uint32_t o = t | (1u << to);
+    g *= sign_above(o, to);
+    *out = o;
+    *sg = g;
+    return true;
+}
+
+__global__ void k_enum_states(int ns, int n, int d, int32_t *states)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d) return;
+    int r = t;
+    uint32_t s = 0;
+    int c = ns - 1;
+    for (int i = n; i >= 1; --i) {
+        while (binom(c, i) > r) --c;
+        s |= 1u << c;
+        r -= binom(c, i);
+        --c;
+    }
+    states[t] = (int32_t)s;
+}
+
+__global__ void k_rank_table(int ns, int32_t *rank)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (1 << ns)) return;
+    int r = 0, i = 0;
+    for (int c = 0; c < ns; ++c)
+        if ((s >> c) & 1) r += binom(c, ++i);
+    rank[s] = r;
+}
+
+// Model parameters of one spin, device copies (row-major).
+struct SpinModel {
+    int nc, nb;
+    const double *ec, *eb, *vbc;
+    double cut;
+};
+
+template <bool FILL>
+__global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *rank,
+                      const int32_t *ptr, int32_t *noff, double *diag, int32_t *rk, double *off)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    const int nc = m.nc, nb = m.nb;
+    const uint32_t s = (uint32_t)states[i];
+    int k = FILL ? ptr[i] : 0;
+    const int k0 = k;
+    uint32_t o;
+    int sg;
+    if (FILL) {
+        double dg = 0.0;
+        for (int c = 0; c < nc; ++c)
+            if ((s >> (nb + c)) & 1u) dg += m.ec[c * nc + c];
+        for (int b = 0; b < nb; ++b)
+            if ((s >> b) & 1u) dg += m.eb[b * nb + b];
+        diag[i] = dg;
+    }
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            double t = m.ec[a * nc + b];
+            if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
+            if (hop(s, nb + a, nb + b, &o, &sg)) {
+                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
+                ++k;
+            }
+        }
+    for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) {
+            double t = m.eb[a * nb + b];
+            if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
+            if (hop(s, a, b, &o, &sg)) {
+                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
+                ++k;
+            }
+        }
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < nc; ++c) {
+            double t = m.vbc[b * nc + c];
+            if (t == 0.0 || !(fabs(t) > m.cut)) continue;
+            if (hop(s, b, nb + c, &o, &sg)) {
+                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
+                ++k;
+            }
+            if (hop(s, nb + c, b, &o, &sg)) {
+                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
+                ++k;
+            }
+        }
+    if (!FILL) noff[i] = k - k0;
+}
+
+// Exclusive prefix sum of noff[0..d) into ptr[0..d], one workgroup; also the row max.
+__global__ void k_scan(const int32_t *noff, int d, int32_t *ptr, int32_t *kmax)
+{
+    __shared__ int32_t part[kBlock], mx[kBlock];
+    const int t = threadIdx.x;
+    const int per = (d + kBlock - 1) / kBlock;
+    const int lo = min(d, t * per), hi = min(d, lo + per);
+    int32_t s = 0, m = 0;
+    for (int i = lo; i < hi; ++i) { s += noff[i]; m = max(m, noff[i]); }
+    part[t] = s;
+    mx[t] = m;
+    __syncthreads();
+    if (t == 0) {
+        int32_t acc = 0, mm = 0;
+        for (int i = 0; i < kBlock; ++i) { int32_t v = part[i]; part[i] = acc; acc += v; mm = max(mm, mx[i]); }
+        ptr[d] = acc;
+        *kmax = mm;
+    }
+    __syncthreads();
+    int32_t acc = part[t];
+    for (int i = lo; i < hi; ++i) { ptr[i] = acc; acc += noff[i]; }
+}
+
+__global__ void k_ell(int d, int kmax, const int32_t *ptr, const int32_t *rk, const double *off,
+                      int32_t *ell_rk, double *ell_val)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    int p = ptr[i], n = ptr[i + 1] - p;
+    for (int k = 0; k < kmax; ++k) {
+        ell_rk[(int64_t)k * d + i] = k < n ? rk[p + k] : i;
+        ell_val[(int64_t)k * d + i] = k < n ? off[p + k] : 0.0;
+    }
+}
+
+// imp[i] = impurity bits of state i; swap[i][c] = rank of the state whose bath bits
+// are those of state i and whose impurity bits are c (-1 if the popcount differs).
+__global__ void k_imp_swap(const int32_t *states, int d, int nb, int ncfg, const int32_t *rank,
+                           int32_t *imp, int32_t *swp)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d * ncfg) return;
+    int i = t / ncfg, c = t - i * ncfg;
+    uint32_t s = (uint32_t)states[i];
+    uint32_t bath = s & ((1u << nb) - 1u);
+    uint32_t s2 = bath | ((uint32_t)c << nb);
+    swp[t] = (__popc(s2) == __popc(s)) ? rank[s2] : -1;
+    if (c == 0) imp[i] = (int32_t)(s >> nb);
+}
+
+__global__ void k_to_f32(const double *in, float *out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(ED_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return ED_OK;
+}
+
+#define ED_TRY(x)              \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_ != ED_OK) return rc_; \
+    } while (0)
+
+static void free_spin(SpinTables &t)
+{
+    void *ps[] = {t.states, t.diag, t.noff, t.ptr, t.rk, t.off, t.ell_rk, t.ell_val, t.imp, t.swap,
+                  t.diag32, t.off32, t.ell_val32};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    t = SpinTables();
+}
+
+static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
+{
+    const int ns = S->ns, nc = S->nc, nb = S->nb;
+    hipStream_t st = S->stream;
+    T.d = binom(ns, n);
+    const int d = T.d;
+    ED_TRY(dalloc(&T.states, d));
+    ED_TRY(dalloc(&T.diag, d));
+    ED_TRY(dalloc(&T.noff, d));
+    ED_TRY(dalloc(&T.ptr, d + 1));
+    hipLaunchKernelGGL(k_enum_states, dim3(nblocks(d)), dim3(kBlock), 0, st, ns, n, d, T.states);
+    ED_HIP_CHECK(hipGetLastError());
+
+    double *d_ec = nullptr, *d_eb = nullptr, *d_vbc = nullptr;
+    ED_TRY(dalloc(&d_ec, (size_t)nc * nc));
+    ED_TRY(dalloc(&d_eb, (size_t)nb * nb));
+    ED_TRY(dalloc(&d_vbc, (size_t)nb * nc));
+    ED_HIP_CHECK(hipMemcpyAsync(d_ec, S->ec.data() + (size_t)spin * nc * nc, sizeof(double) * nc * nc,
+                                hipMemcpyHostToDevice, st));
+    if (nb) {
+        ED_HIP_CHECK(hipMemcpyAsync(d_eb, S->eb.data() + (size_t)spin * nb * nb, sizeof(double) * nb * nb,
+                                    hipMemcpyHostToDevice, st));
+        ED_HIP_CHECK(hipMemcpyAsync(d_vbc, S->vbc.data() + (size_t)spin * nb * nc, sizeof(double) * nb * nc,
+                                    hipMemcpyHostToDevice, st));
+    }
+    SpinModel m{nc, nb, d_ec, d_eb, d_vbc, S->cutoff};
+    hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
+                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+    int32_t *d_kmax = nullptr;
+    ED_TRY(dalloc(&d_kmax, 1));
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kBlock), 0, st, T.noff, d, T.ptr, d_kmax);
+    ED_HIP_CHECK(hipGetLastError());
+    int32_t h[2];
+    ED_HIP_CHECK(hipMemcpyAsync(&h[0], T.ptr + d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipMemcpyAsync(&h[1], d_kmax, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    T.nnz = h[0];
+    T.kmax = h[1];
+    ED_TRY(dalloc(&T.rk, (size_t)T.nnz));
+    ED_TRY(dalloc(&T.off, (size_t)T.nnz));
+    hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
+                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off);
+    ED_HIP_CHECK(hipGetLastError());
+    const int kw = std::max(T.kmax, 1);
+    ED_TRY(dalloc(&T.ell_rk, (size_t)kw * d));
+    ED_TRY(dalloc(&T.ell_val, (size_t)kw * d));
+    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, T.kmax, T.ptr, T.rk, T.off, T.ell_rk,
+                       T.ell_val);
+    ED_TRY(dalloc(&T.imp, d));
+    ED_TRY(dalloc(&T.swap, (size_t)d * S->ncfg));
+    hipLaunchKernelGGL(k_imp_swap, dim3(nblocks((int64_t)d * S->ncfg)), dim3(kBlock), 0, st, T.states, d, nb,
+                       S->ncfg, S->d_rank, T.imp, T.swap);
+    ED_TRY(dalloc(&T.diag32, d));
+    ED_TRY(dalloc(&T.off32, (size_t)T.nnz));
+    ED_TRY(dalloc(&T.ell_val32, (size_t)kw * d));
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(d)), dim3(kBlock), 0, st, T.diag, T.diag32, (int64_t)d);
+    if (T.nnz)
+        hipLaunchKernelGGL(k_to_f32, dim3(nblocks(T.nnz)), dim3(kBlock), 0, st, T.off, T.off32, (int64_t)T.nnz);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks((int64_t)kw * d)), dim3(kBlock), 0, st, T.ell_val, T.ell_val32,
+                       (int64_t)kw * d);
+    ED_HIP_CHECK(hipGetLastError());
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(d_ec);
+    (void)hipFree(d_eb);
+    (void)hipFree(d_vbc);
+    (void)hipFree(d_kmax);
+    return ED_OK;
+}
+
+// Interaction diagonal and Hund transitions per impurity configuration pair.  Both act
+// on impurity bits only, and the Hund signs are parities of impurity bits strictly
+// between two impurity orbitals, so they tabulate over (cu, cd) in [0, 2^nc)^2
+// (HAIMupdo.f90:785-829 density terms, :1017-1129 spin flip and pair hopping).
+static int build_interaction(ed_sector *S)
+{
+    const int nc = S->nc, ncfg = S->ncfg;
+    const double *U = S->u.data(), *JJ = S->jj.data();
+    bool offc = false;
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b)
+            if (a != b && U[a * nc + b] != 0.0) offc = true;
+    S->hund = (std::fabs(S->jhund) > 1e-5 && offc) ? 1 : 0;
+    std::vector<double> uimp((size_t)ncfg * ncfg, 0.0);
+    std::vector<int32_t> hptr((size_t)ncfg * ncfg + 1, 0), hcu, hcd;
+    std::vector<double> hval;
+    auto par = [](uint32_t x, int a, int b) {   // parity of bits strictly between a and b
+        int lo = std::min(a, b), hi = std::max(a, b);
+        uint32_t mask = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+        return (__builtin_popcount(x & mask) & 1) ? -1 : 1;
+    };
+    for (int cu = 0; cu < ncfg; ++cu)
+        for (int cd = 0; cd < ncfg; ++cd) {
+            double e = 0.0;
+            for (int c = 0; c < nc; ++c)
+                if (U[c * nc + c] != 0.0 && ((cu >> c) & 1) && ((cd >> c) & 1)) e += U[c * nc + c];
+            if (offc)
+                for (int c1 = 0; c1 < nc - 1; ++c1) {
+                    int u1 = (cu >> c1) & 1, d1 = (cd >> c1) & 1;
+                    if (!(u1 + d1)) continue;
+                    for (int c2 = c1 + 1; c2 < nc; ++c2) {
+                        if (U[c1 * nc + c2] == 0.0) continue;
+                        int u2 = (cu >> c2) & 1, d2 = (cd >> c2) & 1;
+                        if (!(u2 + d2)) continue;
+                        e += U[c1 * nc + c2] * (u1 + d1) * (u2 + d2);
+                        if (u1 && u2) e -= JJ[c1 * nc + c2];
+                        if (d1 && d2) e -= JJ[c1 * nc + c2];
+                    }
+                }
+            uimp[(size_t)cu * ncfg + cd] = e;
+            if (S->hund) {
+                for (int c1 = 0; c1 < nc; ++c1) {
+                    int u1 = (cu >> c1) & 1, d1 = (cd >> c1) & 1;
+                    if (u1 + d1 != 1) continue;
+                    for (int c2 = c1 + 1; c2 < nc; ++c2) {
+                        if (U[c1 * nc + c2] == 0.0) continue;
+                        int u2 = (cu >> c2) & 1, d2 = (cd >> c2) & 1;
+                        if (u2 + d2 != 1) continue;
+                        if (u1 && d2) {   // up c1 -> c2, down c2 -> c1
+                            uint32_t ou = (cu & ~(1u << c1)) | (1u << c2), od = (cd & ~(1u << c2)) | (1u << c1);
+                            hcu.push_back(ou); hcd.push_back(od);
+                            hval.push_back(JJ[c1 * nc + c2] * par(ou, c1, c2) * par(od, c1, c2));
+                        }
+                        if (d1 && u2) {   // up c2 -> c1, down c1 -> c2
+                            uint32_t ou = (cu & ~(1u << c2)) | (1u << c1), od = (cd & ~(1u << c1)) | (1u << c2);
+                            hcu.push_back(ou); hcd.push_back(od);
+                            hval.push_back(JJ[c1 * nc + c2] * par(ou, c1, c2) * par(od, c1, c2));
+                        }
+                    }
+                }
+                for (int c1 = 0; c1 < nc; ++c1) {
+                    if (!(((cu >> c1) & 1) && ((cd >> c1) & 1))) continue;
+                    for (int c2 = 0; c2 < nc; ++c2) {
+                        if (c2 == c1 || U[c1 * nc + c2] == 0.0) continue;
+                        if (((cu >> c2) & 1) || ((cd >> c2) & 1)) continue;
+                        uint32_t ou = (cu & ~(1u << c1)) | (1u << c2), od = (cd & ~(1u << c1)) | (1u << c2);
+                        hcu.push_back(ou); hcd.push_back(od);
+                        hval.push_back(JJ[c1 * nc + c2] * par(ou, c1, c2) * par(od, c1, c2));
+                    }
+                }
+            }
+            hptr[(size_t)cu * ncfg + cd + 1] = (int32_t)hcu.size();
+        }
+    S->nhund = (int)hcu.size();
+    std::vector<float> uimp32(uimp.begin(), uimp.end()), hval32(hval.begin(), hval.end());
+    ED_TRY(dalloc(&S->d_uimp, uimp.size()));
+    ED_TRY(dalloc(&S->d_uimp32, uimp.size()));
+    ED_TRY(dalloc(&S->d_hund_ptr, hptr.size()));
+    ED_TRY(dalloc(&S->d_hund_cu, hcu.size()));
+    ED_TRY(dalloc(&S->d_hund_cd, hcd.size()));
+    ED_TRY(dalloc(&S->d_hund_val, hval.size()));
+    ED_TRY(dalloc(&S->d_hund_val32, hval.size()));
+    ED_HIP_CHECK(hipMemcpy(S->d_uimp, uimp.data(), uimp.size() * sizeof(double), hipMemcpyHostToDevice));
+    ED_HIP_CHECK(hipMemcpy(S->d_uimp32, uimp32.data(), uimp32.size() * sizeof(float), hipMemcpyHostToDevice));
+    ED_HIP_CHECK(hipMemcpy(S->d_hund_ptr, hptr.data(), hptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!hcu.empty()) {
+        ED_HIP_CHECK(hipMemcpy(S->d_hund_cu, hcu.data(), hcu.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        ED_HIP_CHECK(hipMemcpy(S->d_hund_cd, hcd.data(), hcd.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        ED_HIP_CHECK(hipMemcpy(S->d_hund_val, hval.data(), hval.size() * sizeof(double), hipMemcpyHostToDevice));
+        ED_HIP_CHECK(hipMemcpy(S->d_hund_val32, hval32.data(), hval32.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return ED_OK;
+}
+
+static void free_sector(ed_sector *S)
+{
+    free_spin(S->up);
+    free_spin(S->dn);
+    void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val,
+                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (S->graph.exec) (void)hipGraphExecDestroy(S->graph.exec);
+    if (S->graph.graph) (void)hipGraphDestroy(S->graph.graph);
+    if (S->ev0) (void)hipEventDestroy(S->ev0);
+    if (S->ev1) (void)hipEventDestroy(S->ev1);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+    delete S;
+}
+
+static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
+{
+    if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create: NULL argument");
+    *out = nullptr;
+    const int nc = m->nc, nb = m->nb, ns = nc + nb;
+    if (nc < 1 || nb < 0 || ns > 15)
+        return fail(ED_ERR_ARG, "ed_sector_create: need nc >= 1, nb >= 0, nc + nb <= 15");
+    if (nup < 0 || nup > ns || ndn < 0 || ndn > ns)
+        return fail(ED_ERR_ARG, "ed_sector_create: particle numbers outside [0, Ns]");
+    if (!m->ec || !m->u || !m->jj || (nb && (!m->eb || !m->vbc)))
+        return fail(ED_ERR_ARG, "ed_sector_create: NULL model array");
+    ed_sector *S = new ed_sector();
+    ED_HIP_CHECK(hipGetDevice(&S->device));
+    S->nc = nc; S->nb = nb; S->ns = ns; S->nup = nup; S->ndn = ndn; S->ncfg = 1 << nc;
+    S->ec.assign(m->ec, m->ec + 2 * nc * nc);
+    if (nb) {
+        S->eb.assign(m->eb, m->eb + 2 * nb * nb);
+        S->vbc.assign(m->vbc, m->vbc + 2 * nb * nc);
+    }
+    S->u.assign(m->u, m->u + nc * nc);
+    S->jj.assign(m->jj, m->jj + nc * nc);
+    S->jhund = m->jhund;
+    S->cutoff = m->cutoff;
+    int rc = ED_OK;
+    auto bail = [&](int code) { free_sector(S); return code; };
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    if ((rc = dalloc(&S->d_rank, (size_t)1 << ns)) != ED_OK) return bail(rc);
+    hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
+    if ((rc = build_spin(S, 0, nup, S->up)) != ED_OK) return bail(rc);
+    if ((rc = build_spin(S, 1, ndn, S->dn)) != ED_OK) return bail(rc);
+    if ((rc = build_interaction(S)) != ED_OK) return bail(rc);
+    S->dim = (int64_t)S->up.d * S->dn.d;
+    if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
+        return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
+    if (hipStreamSynchronize(S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "sector build failed"));
+    *out = S;
+    return ED_OK;
+}
+
+}  // namespace ed
+
+extern "C" {
+
+int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out)
+{
+    return ed::create_sector(model, nup, ndn, out);
+}
+
+int ed_sector_destroy(ed_sector *sec)
+{
+    if (!sec) return ED_OK;
+    (void)hipStreamSynchronize(sec->stream);
+    ed::free_sector(sec);
+    return ED_OK;
+}
+
+int64_t ed_sector_dim(const ed_sector *sec) { return sec ? sec->dim : 0; }
+
+int ed_sector_spin_dims(const ed_sector *sec, int32_t *d_up, int32_t *d_dn)
+{
+    if (!sec) return ed::fail(ED_ERR_ARG, "ed_sector_spin_dims: NULL sector");
+    if (d_up) *d_up = sec->up.d;
+    if (d_dn) *d_dn = sec->dn.d;
+    return ED_OK;
+}
+
+int ed_sector_nnz(const ed_sector *sec, int spin, int64_t *nnz)
+{
+    if (!sec || !nnz || (spin != 0 && spin != 1)) return ed::fail(ED_ERR_ARG, "ed_sector_nnz: bad argument");
+    *nnz = spin ? sec->dn.nnz : sec->up.nnz;
+    return ED_OK;
+}
+
+int ed_sector_tables(const ed_sector *sec, int spin, int32_t *states, double *diag, int32_t *noff,
+                     int32_t *rankoff, double *offdiag)
+{
+    if (!sec || (spin != 0 && spin != 1)) return ed::fail(ED_ERR_ARG, "ed_sector_tables: bad argument");
+    const ed::SpinTables &T = spin ? sec->dn : sec->up;
+    if (states) ED_HIP_CHECK(hipMemcpy(states, T.states, sizeof(int32_t) * T.d, hipMemcpyDeviceToHost));
+    if (diag) ED_HIP_CHECK(hipMemcpy(diag, T.diag, sizeof(double) * T.d, hipMemcpyDeviceToHost));
+    if (noff) ED_HIP_CHECK(hipMemcpy(noff, T.noff, sizeof(int32_t) * T.d, hipMemcpyDeviceToHost));
+    if (rankoff && T.nnz) ED_HIP_CHECK(hipMemcpy(rankoff, T.rk, sizeof(int32_t) * T.nnz, hipMemcpyDeviceToHost));
+    if (offdiag && T.nnz) ED_HIP_CHECK(hipMemcpy(offdiag, T.off, sizeof(double) * T.nnz, hipMemcpyDeviceToHost));
+    return ED_OK;
+}
+
+}  // extern "C"

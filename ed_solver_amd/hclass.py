@@ -1,0 +1,309 @@
+"""Python host mirror of the reference's Hamiltonian boundary and the solvers that call it.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``Sector`` is one (N_up, N_dn) sector resident on the GPU -- the state that
+  new_H / delete_H manage in H_class.f90:28-58 (module-global ``sector_h``) and
+  HAIMupdo.f90:53-121 (tables), made a handle so several sectors can coexist.
+* ``new_H`` / ``delete_H`` / ``Hmult__`` / ``dimen_H`` keep the one-global-sector
+  surface of H_class.f90:13-152 for code written against it.
+* ``Lanczos_fast_diagonalize`` / ``Lanczos_get_GS_sector`` mirror
+  Lanczos_plain.f90:33-229, ``compute_dynamic`` mirrors
+  green_class_compute_dynamic.f90:191-407, ``apply_C`` mirrors apply_C.f90:497-598,
+  ``invert_zmtridiag`` mirrors matrices_tridiag.f90:163-198.
+
+Where the reference STOPs, these raise ``EDError`` (``BreakdownError`` for a
+zero-norm Lanczos vector).  Every call goes to libed_hip.so; nothing here computes
+on the CPU beyond argument checks.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from ._lib import (ED_PREC_F32, ED_PREC_F64, EDError, LanczosOpts, check, dptr, iptr, lib)
+from .aim import AIMParams, CModel
+
+__all__ = ["Sector", "LanczosResult", "new_H", "delete_H", "Hmult__", "dimen_H", "Lanczos_fast_diagonalize",
+           "Lanczos_get_GS_sector", "apply_C", "compute_dynamic", "compute_dynamic_batched", "invert_zmtridiag",
+           "matsubara", "start_vector", "tridiag_eig"]
+
+_PREC = {"f64": ED_PREC_F64, "fp64": ED_PREC_F64, "f32": ED_PREC_F32, "fp32": ED_PREC_F32,
+         ED_PREC_F64: ED_PREC_F64, ED_PREC_F32: ED_PREC_F32}
+
+
+def _prec(p) -> int:
+    try:
+        return _PREC[p]
+    except KeyError:
+        raise ValueError(f"precision must be 'f64' or 'f32', got {p!r}") from None
+
+
+@dataclasses.dataclass
+class LanczosResult:
+    iters: int
+    evals: np.ndarray        # lowest eigenvalues of T_iters
+    vecp: np.ndarray         # (neig, iters) Lanczos coefficients
+    diag: np.ndarray         # alpha_1..alpha_iters
+    subdiag: np.ndarray      # ||v1||, beta_2..beta_iters
+    precision: str
+
+
+class Sector:
+    """One (N_up, N_dn) sector of an Anderson impurity model on the current GPU."""
+
+    def __init__(self, model: AIMParams, nup: int, ndn: int):
+        self.model = model
+        self.nup, self.ndn = int(nup), int(ndn)
+        self._cm = CModel.from_params(model)
+        h = ctypes.c_void_p()
+        check(lib().ed_sector_create(ctypes.byref(self._cm), self.nup, self.ndn, ctypes.byref(h)))
+        self._h = h
+        self.dim = int(lib().ed_sector_dim(h))
+        du, dd = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ed_sector_spin_dims(h, ctypes.byref(du), ctypes.byref(dd)))
+        self.dup, self.ddn = du.value, dd.value
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise EDError(-5, "sector already destroyed (delete_H)")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().ed_sector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ----------------------------------------------------------------- tables
+    def nnz(self, spin: int) -> int:
+        n = ctypes.c_int64()
+        check(lib().ed_sector_nnz(self.handle, spin, ctypes.byref(n)))
+        return n.value
+
+    def tables(self, spin: int) -> dict:
+        """tab_HAIM2 output of one spin (states, diag, noff, 0-based rankoff, offdiag)."""
+        d = self.ddn if spin else self.dup
+        nnz = self.nnz(spin)
+        st = np.zeros(d, np.int32); dg = np.zeros(d); no = np.zeros(d, np.int32)
+        rk = np.zeros(max(nnz, 1), np.int32); of = np.zeros(max(nnz, 1))
+        check(lib().ed_sector_tables(self.handle, spin, iptr(st), dptr(dg), iptr(no), iptr(rk), dptr(of)))
+        return {"states": st, "diag": dg, "noff": no, "rankoff": rk[:nnz], "offdiag": of[:nnz]}
+
+    # -------------------------------------------------------------------- H.v
+    def hmult(self, v: np.ndarray) -> np.ndarray:
+        """Hmultr: host vectors in, host vector out (FP64)."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if v.shape != (self.dim,):
+            raise EDError(-1, f"Hmult: vector of length {v.size}, sector dimension {self.dim}")
+        out = np.empty_like(v)
+        check(lib().ed_hmult_f64(self.handle, self.dim, dptr(out), dptr(v)))
+        return out
+
+    def hmult_dev(self, out_ptr: int, in_ptr: int, precision="f64", stream: int | None = None) -> None:
+        """H.v on device pointers (e.g. torch tensor.data_ptr()); asynchronous."""
+        p = _prec(precision)
+        f = lib().ed_hmult_f64_dev if p == ED_PREC_F64 else lib().ed_hmult_f32_dev
+        check(f(self.handle, self.dim, ctypes.c_void_p(out_ptr), ctypes.c_void_p(in_ptr),
+                ctypes.c_void_p(stream) if stream else None))
+
+    def synchronize(self) -> None:
+        check(lib().ed_sector_synchronize(self.handle))
+
+    def hmult_block_dev(self, out_ptr: int, in_ptr: int, nvec: int, ld: int, stream: int | None = None) -> None:
+        check(lib().ed_hmult_block_f64_dev(self.handle, self.dim, nvec, ld, ctypes.c_void_p(out_ptr),
+                                           ctypes.c_void_p(in_ptr), ctypes.c_void_p(stream) if stream else None))
+
+    def hmult_benchmark(self, precision="f64", reps: int = 50) -> tuple[float, float]:
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        check(lib().ed_hmult_benchmark(self.handle, _prec(precision), reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    # ---------------------------------------------------------------- Lanczos
+    def lanczos(self, nitermax: int = 128, neigen: int = 1, tolerance: float = 1e-12, precision="f64",
+                start: np.ndarray | None = None, check_every: int = 8, reorth: bool = False) -> LanczosResult:
+        """Lanczos_fast_diagonalize on the device (convergence test as the reference)."""
+        o = LanczosOpts(int(nitermax), int(neigen), float(tolerance), _prec(precision), int(check_every),
+                        1 if reorth else 0)
+        n = max(1, min(self.dim - 1, nitermax))
+        neig_cap = max(1, min(neigen, self.dim, n))
+        ev = np.zeros(neig_cap); vecp = np.zeros(neig_cap * n); dg = np.zeros(n); sd = np.zeros(n)
+        it, ne = ctypes.c_int(), ctypes.c_int()
+        sp = None
+        if start is not None:
+            start = np.ascontiguousarray(start, np.float64)
+            if start.shape != (self.dim,):
+                raise EDError(-1, "start vector length differs from the sector dimension")
+            sp = dptr(start)
+        check(lib().ed_lanczos_diagonalize(self.handle, ctypes.byref(o), sp, ctypes.byref(it), ctypes.byref(ne),
+                                           dptr(ev), dptr(vecp), dptr(dg), dptr(sd)))
+        k, m = ne.value, it.value
+        return LanczosResult(m, ev[:k].copy(), vecp[:k * m].reshape(k, m).copy(), dg[:m].copy(), sd[:m].copy(),
+                             "f64" if o.precision == ED_PREC_F64 else "f32")
+
+    def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
+        """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""
+        psi = np.zeros(self.dim)
+        row = np.ascontiguousarray(res.vecp[k], np.float64)
+        check(lib().ed_lanczos_gs_vector(self.handle, res.iters, dptr(row), dptr(psi)))
+        return psi
+
+    def lanczos_run(self, niter: int, start: np.ndarray | None = None, precision="f64"):
+        """Lanczos_GPU_updo: exactly niter steps, returns (diag, subdiag)."""
+        dg = np.zeros(niter); sd = np.zeros(niter)
+        sp = None
+        if start is not None:
+            start = np.ascontiguousarray(start, np.float64)
+            sp = dptr(start)
+        check(lib().ed_lanczos_run(self.handle, niter, sp, _prec(precision), dptr(dg), dptr(sd)))
+        return dg, sd
+
+    def rayleigh(self, psi: np.ndarray) -> float:
+        psi = np.ascontiguousarray(psi, np.float64)
+        e = ctypes.c_double()
+        check(lib().ed_rayleigh_f64(self.handle, self.dim, dptr(psi), ctypes.byref(e)))
+        return e.value
+
+
+# --------------------------------------------------------------------------------
+# One-global-sector surface of H_class.f90 (sector_h, :11)
+
+_sector_h: Sector | None = None
+
+
+def new_H(model: AIMParams, nup: int, ndn: int) -> Sector:
+    """new_H (H_class.f90:28-37): builds the sector tables and makes it current."""
+    global _sector_h
+    delete_H()
+    _sector_h = Sector(model, nup, ndn)
+    return _sector_h
+
+
+def delete_H() -> None:
+    """delete_H (H_class.f90:48-58): no-op if H was never created."""
+    global _sector_h
+    if _sector_h is not None:
+        _sector_h.close()
+        _sector_h = None
+
+
+def _current() -> Sector:
+    if _sector_h is None:
+        raise EDError(-5, "Hmultr, no array are associated!")   # H_class.f90:132
+    return _sector_h
+
+
+def Hmult__(n: int, vec_out: np.ndarray, vec_in: np.ndarray) -> None:
+    """Hmultr (H_class.f90:109-136): vec_out(1:n) = H vec_in(1:n), in place."""
+    s = _current()
+    if n != s.dim or vec_out.shape != (n,) or vec_in.shape != (n,):
+        raise EDError(-1, f"Hmult__: n={n} differs from dimen_H()={s.dim}")
+    if vec_out.dtype != np.float64 or not vec_out.flags.c_contiguous:
+        raise EDError(-1, "Hmult__: vec_out must be a contiguous float64 array")
+    vin = np.ascontiguousarray(vec_in, np.float64)
+    check(lib().ed_hmult_f64(s.handle, n, dptr(vec_out), dptr(vin)))
+
+
+def dimen_H() -> int:
+    """dimen_H (H_class.f90:149-152)."""
+    return _current().dim
+
+
+def Lanczos_fast_diagonalize(neigen: int = 1, nitermax: int = 128, tolerance: float = 1e-12, precision="f64",
+                             start=None, sector: Sector | None = None, **kw) -> LanczosResult:
+    """Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134) on the current sector."""
+    s = sector or _current()
+    return s.lanczos(nitermax=nitermax, neigen=neigen, tolerance=tolerance, precision=precision, start=start, **kw)
+
+
+def Lanczos_get_GS_sector(res: LanczosResult, k: int = 0, sector: Sector | None = None) -> np.ndarray:
+    """Lanczos_get_GS_sector (Lanczos_plain.f90:156-229)."""
+    s = sector or _current()
+    return s.ground_state(res, k)
+
+
+def start_vector(n: int) -> np.ndarray:
+    out = np.zeros(n)
+    check(lib().ed_start_vector(n, dptr(out)))
+    return out
+
+
+def tridiag_eig(diag, subdiag):
+    d = np.ascontiguousarray(diag, np.float64); b = np.ascontiguousarray(subdiag, np.float64)
+    n = len(d)
+    w = np.zeros(n); z = np.zeros(n * n)
+    check(lib().ed_tridiag_eig(n, dptr(d), dptr(b), dptr(w), dptr(z)))
+    return w, z.reshape(n, n)
+
+
+def apply_C(src: Sector, dst: Sector, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
+    """apply_Cupdo (apply_C.f90:497-598): c+ (pm=+1) / c (pm=-1) on impurity orbital `site`."""
+    psi = np.ascontiguousarray(psi, np.float64)
+    if psi.shape != (src.dim,):
+        raise EDError(-1, "apply_C: psi length differs from the source sector dimension")
+    phi = np.zeros(dst.dim)
+    check(lib().ed_apply_c(src.handle, dst.handle, site, spin, pm, dptr(psi), dptr(phi)))
+    return phi
+
+
+def matsubara(beta: float, nw: int) -> np.ndarray:
+    """w_n = (2n-1) pi / beta, n = 1..nw (frequency_axis.f90:165-170)."""
+    n = np.arange(1, nw + 1, dtype=np.float64)
+    return (2.0 * np.pi / beta) * n - np.pi / beta
+
+
+def invert_zmtridiag(a, b, wn) -> np.ndarray:
+    """<0|(i wn - T)^-1|0> for every wn (matrices_tridiag.f90:163-198), on the device."""
+    a = np.ascontiguousarray(a, np.float64); b = np.ascontiguousarray(b, np.float64)
+    wn = np.ascontiguousarray(wn, np.float64)
+    out = np.zeros(2 * len(wn))
+    check(lib().ed_continued_fraction(len(a), dptr(a), dptr(b), len(wn), dptr(wn), dptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def compute_dynamic(sector: Sector, iph: int, energy: float, phi: np.ndarray, nitergreen: int, wn: np.ndarray,
+                    cutoff_dynamic: float = 1e-8, precision="f64"):
+    """compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407): returns (dyn, a, b)."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    wn = np.ascontiguousarray(wn, np.float64)
+    if phi.shape != (sector.dim,):
+        raise EDError(-1, "compute_dynamic inconsistent dimensions, CRITICAL")
+    n = max(1, min(nitergreen, sector.dim))
+    dyn = np.zeros(2 * len(wn)); a = np.zeros(n); b = np.zeros(n)
+    check(lib().ed_dynamic(sector.handle, iph, energy, dptr(phi), nitergreen, cutoff_dynamic, _prec(precision),
+                           len(wn), dptr(wn), dptr(dyn), dptr(a), dptr(b)))
+    return dyn[0::2] + 1j * dyn[1::2], a, b
+
+
+def compute_dynamic_batched(sectors, phis, iphs, energies, nitergreen: int, wn: np.ndarray,
+                            cutoff_dynamic: float = 1e-8, precision="f64"):
+    """All excitation runs in one call; returns dyn (njobs, nw) complex."""
+    nj = len(sectors)
+    phis = [np.ascontiguousarray(p, np.float64) for p in phis]
+    for s, p in zip(sectors, phis):
+        if p.shape != (s.dim,):
+            raise EDError(-1, "compute_dynamic_batched: phi length differs from its sector dimension")
+    wn = np.ascontiguousarray(wn, np.float64)
+    hs = (ctypes.c_void_p * nj)(*[s.handle.value for s in sectors])
+    ps = (ctypes.POINTER(ctypes.c_double) * nj)(*[dptr(p) for p in phis])
+    ip = (ctypes.c_int * nj)(*[int(i) for i in iphs])
+    en = np.ascontiguousarray(energies, np.float64)
+    dyn = np.zeros(nj * 2 * len(wn)); a = np.zeros(nj * nitergreen); b = np.zeros(nj * nitergreen)
+    check(lib().ed_dynamic_batched(nj, hs, ps, ip, dptr(en), nitergreen, cutoff_dynamic, _prec(precision), len(wn),
+                                   dptr(wn), dptr(dyn), dptr(a), dptr(b)))
+    d = dyn.reshape(nj, len(wn), 2)
+    return d[..., 0] + 1j * d[..., 1], a.reshape(nj, nitergreen), b.reshape(nj, nitergreen)

@@ -1,0 +1,226 @@
+/*
+ * ed_hip.h -- C-ABI of the MI355X exact-diagonalisation engine (libed_hip.so).
+ *
+ * Plain C types only (no HIP or torch types): Fortran binds it through
+ * ISO_C_BINDING, Python through ctypes.  Every entry point returns an int status
+ * (ED_OK = 0); ed_last_error() gives the message of the last failure on the
+ * calling thread.  The reference STOPs where this library returns an error.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to
+ * the reference repository, directory ed_solver/).
+ *
+ * Conventions shared with the reference:
+ *   per-spin bit layout     bath site k -> bit k, impurity orbital c -> bit nb+c
+ *                           (quantum_algebra_Anderson_Impurity_class.f90:73-86)
+ *   spin-sector ordering    states with popcount N in increasing integer order
+ *                           (quantum_fermion_hilbert_class.f90:385-404)
+ *   vector index            istate = iup + ido * d_up, 0-based (the reference's
+ *                           iup + (ido-1)*d_up with 1-based iup, ido)
+ *   table ranks             rankoff is 0-based (reference value minus one)
+ *   tridiagonal storage     diag[0..n-1]; subdiag[0] = ||v1||, subdiag[1..n-1]
+ *                           the off-diagonal (Dagotto storage, matrices_tridiag.f90:55-63)
+ */
+#ifndef ED_HIP_H
+#define ED_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ED_OK 0
+#define ED_ERR_ARG (-1)       /* invalid argument (size mismatch, bad sector ...) */
+#define ED_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define ED_ERR_NOMEM (-3)     /* device allocation failed */
+#define ED_ERR_BREAKDOWN (-4) /* zero-norm Lanczos vector: Lanczos_plain.h:70-92 STOPs */
+#define ED_ERR_STATE (-5)     /* call out of order (e.g. no Krylov basis kept) */
+#define ED_ERR_CF (-6)        /* continued-fraction denominator < eps (matrices_tridiag.f90:174-186) */
+
+#define ED_PREC_F64 0         /* vectors and H.v in FP64 */
+#define ED_PREC_F32 1         /* vectors and H.v in FP32, dot products / recurrence scalars in FP64 */
+
+/* Anderson impurity model, one real (FP64) parameter set.  All arrays row-major.
+ * Replaces the fields of AIM_type / AIM2_type / impurity_type the Hamiltonian
+ * reads (quantum_algebra_Anderson_Impurity_class2.f90:62-135, HAIMupdo.f90:53-121). */
+typedef struct ed_model {
+    int32_t nc;          /* impurity orbitals per spin (Nc)                           */
+    int32_t nb;          /* bath sites per spin (Nb); Ns = nc + nb <= 15              */
+    const double *ec;    /* [2][nc][nc]  Ec(spin)                                     */
+    const double *eb;    /* [2][nb][nb]  Eb(spin)                                     */
+    const double *vbc;   /* [2][nb][nc]  Vbc(spin)(k, c)                              */
+    const double *u;     /* [nc][nc]     U_ii, and U_ij already reduced by 2J (i != j)*/
+    const double *jj;    /* [nc][nc]     JJmatrix                                     */
+    double jhund;        /* Jhund: spin flip + pair hopping are on when |Jhund| > 1e-5 */
+    double cutoff;       /* cutoff_hamilt_param                                       */
+} ed_model;
+
+/* One (N_up, N_dn) sector resident on one GPU: integer tables, H.v kernels and
+ * Lanczos work space.  Opaque. */
+typedef struct ed_sector ed_sector;
+
+/* Error text of the last failing call on this thread (never NULL). */
+const char *ed_last_error(void);
+
+/* Select the device for subsequent calls on this thread.
+ * Replaces init_gpu_device / choose_gpu (core_functions_initsim_closesim.f90:131). */
+int ed_init(int device);
+
+/* Library version string, e.g. "ed_hip 0.1 gfx950". */
+const char *ed_version(void);
+
+/* ------------------------------------------------------------------ sectors */
+
+/* new_H(AIM, sector) -> new_HAIMupdo (H_class.f90:28-37, HAIMupdo.f90:53-121):
+ * enumerates both spin sectors, builds rank tables and tab_HAIM2 tables
+ * (HAIM_offdiag_arrays.f90:89-148, 210-606) with integer kernels on the device. */
+int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out);
+
+/* delete_H (H_class.f90:48-58).  NULL is accepted. */
+int ed_sector_destroy(ed_sector *sec);
+
+/* dimen_H (H_class.f90:149-152) = d_up * d_dn. */
+int64_t ed_sector_dim(const ed_sector *sec);
+
+/* Spin-sector dimensions d_up, d_dn (fermion_sector2 up%dimen, down%dimen). */
+int ed_sector_spin_dims(const ed_sector *sec, int32_t *d_up, int32_t *d_dn);
+
+/* Number of tabulated off-diagonal entries of one spin (noffdiagup / noffdiagdo,
+ * HAIM_offdiag_arrays.f90:37). */
+int ed_sector_nnz(const ed_sector *sec, int spin, int64_t *nnz);
+
+/* Copy one spin's tables to host memory (any pointer may be NULL):
+ * states[d], diag[d], noff[d], rankoff[nnz] (0-based), offdiag[nnz]
+ * (sector%state and diagup/noffup/rankoffup/offdiagup, HAIM_offdiag_arrays.f90:20-44). */
+int ed_sector_tables(const ed_sector *sec, int spin, int32_t *states, double *diag,
+                     int32_t *noff, int32_t *rankoff, double *offdiag);
+
+/* ---------------------------------------------------------------------- H.v */
+
+/* Hmultr (H_class.f90:109-136) -> HAIMupdo_multr (HAIMupdo.f90:677-1195):
+ * vec_out = H vec_in on host arrays of length n = dimen_H().  Synchronous; copies
+ * through device buffers owned by the sector. */
+int ed_hmult_f64(ed_sector *sec, int64_t n, double *vec_out, const double *vec_in);
+
+/* Device-resident H.v (the fast path): d_out / d_in are device pointers of length
+ * n; `stream` is a hipStream_t or NULL for the sector's own stream.  Asynchronous. */
+int ed_hmult_f64_dev(ed_sector *sec, int64_t n, double *d_out, const double *d_in, void *stream);
+
+/* FP32 H.v on device vectors (tables rounded to FP32). */
+int ed_hmult_f32_dev(ed_sector *sec, int64_t n, float *d_out, const float *d_in, void *stream);
+
+/* Waits for all work queued on the sector's own stream (the asynchronous device
+ * entry points above use it when called with stream == NULL). */
+int ed_sector_synchronize(ed_sector *sec);
+
+/* Block H.v on nvec device vectors stored column by column with leading dimension
+ * ld (>= n): V(:, 1:nvec) = H U(:, 1:nvec), the LFLAG=1 callback of
+ * Block_Lanczos_diagonalize (Block_Lanczos.f90:189-259).  FP64. */
+int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, double *d_out,
+                           const double *d_in, void *stream);
+
+/* ------------------------------------------------------------------ Lanczos */
+
+/* Deterministic start vector (period 10000, values k/1000 in (0, 1]) used when a
+ * Lanczos entry point gets start == NULL.  It takes the place of
+ * create_fix_initial_vector (rcvector_class.f90:43-60), whose gfortran random
+ * stream cannot be reproduced; Fortran callers may pass their own vector. */
+int ed_start_vector(int64_t n, double *out);
+
+typedef struct ed_lanczos_opts {
+    int nitermax;        /* Nitermax (default 128)                                    */
+    int neigen;          /* Neigen (default 1)                                        */
+    double tolerance;    /* convergence of |lambda_k(T_j) - lambda_k(T_j-1)| (1e-12)  */
+    int precision;       /* ED_PREC_F64 or ED_PREC_F32                                */
+    int check_every;     /* iterations between host convergence checks (default 8)    */
+    int reorth;          /* 0: plain (the reference); 1: full reorthogonalisation (CGS2) */
+} ed_lanczos_opts;
+
+/* Fills *opts with the reference defaults (dmft_ed_solver_variables.h:9-20). */
+int ed_lanczos_default_opts(ed_lanczos_opts *opts);
+
+/* Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134): plain Lanczos from `start`
+ * (host, length dimen_H(), or NULL for ed_start_vector) with the reference's
+ * convergence test.  Outputs (host): *iters; evals[neigen'] (neigen' = min(neigen,
+ * dim, niter)); vecp[neigen'][*iters] Lanczos coefficients (row k = eigenvector
+ * k of T); diag/subdiag[*iters] (may be NULL, capacity nitermax).  The Krylov basis
+ * stays on the device for ed_lanczos_gs_vector. */
+int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const double *start,
+                           int *iters, int *neig_out, double *evals, double *vecp,
+                           double *diag, double *subdiag);
+
+/* Lanczos_GPU_updo (Lanczos_gpu.f90:106): exactly `niter` Lanczos steps (no
+ * convergence test) from `start` (host or NULL); writes diag[niter], subdiag[niter].
+ * Returns ED_ERR_BREAKDOWN when a zero-norm vector appears before niter. */
+int ed_lanczos_run(ed_sector *sec, int niter, const double *start, int precision,
+                   double *diag, double *subdiag);
+
+/* Lanczos_GPU_get_GS_updo (Lanczos_gpu.f90:48) / Lanczos_get_GS_sector
+ * (Lanczos_plain.f90:156-229): psi = normalised sum_j vecp[j] v_j over the Krylov
+ * basis kept by the last Lanczos call on this sector.  psi is a host array of
+ * length dimen_H() (FP64). */
+int ed_lanczos_gs_vector(ed_sector *sec, int iters, const double *vecp, double *psi);
+
+/* Same, result left on the device (d_psi, FP64). */
+int ed_lanczos_gs_vector_dev(ed_sector *sec, int iters, const double *vecp, double *d_psi);
+
+/* Rayleigh quotient <psi|H|psi>/<psi|psi> in FP64 on a host vector (the FP64
+ * polish of the mixed-precision path). */
+int ed_rayleigh_f64(ed_sector *sec, int64_t n, const double *psi, double *energy);
+
+/* Eigenpairs of the symmetric tridiagonal T (diag[n], subdiag[1..n-1]; subdiag[0]
+ * unused), ascending: w[n], z[k*n + r] = component r of eigenvector k (sign: largest
+ * component positive).  The DSTEDC call of diagonalize_tridiag
+ * (matrices_tridiag.f90:210-259), host FP64. */
+int ed_tridiag_eig(int n, const double *diag, const double *subdiag, double *w, double *z);
+
+/* --------------------------------------------------- Green's functions */
+
+/* apply_Cupdo (apply_C.f90:497-598): phi = c+ (pm = +1) or c (pm = -1) of impurity
+ * orbital `site` (0-based), spin 0 (up) / 1 (down), applied to psi of sector `src`;
+ * phi lives in sector `dst` ((nup+-1, ndn) or (nup, ndn+-1)).  Host arrays. */
+int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm,
+               const double *psi, double *phi);
+
+/* invert_zmtridiag (matrices_tridiag.f90:163-198) for nw points z = i*wn[k]:
+ * out[2k], out[2k+1] = Re, Im of <0|(z - T)^-1|0>; T = (a[0..n-1], b[1..n-1]).
+ * Evaluated on the device, one thread per frequency. */
+int ed_continued_fraction(int n, const double *a, const double *b, int nw, const double *wn,
+                          double *out);
+
+/* compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407), fermionic, real:
+ * niter = min(nitergreen, dim) Lanczos steps from phi/||phi|| in sector `sec`,
+ * then dyn(i wn) = ||phi||^2 CF(i wn) of T' = ph (T - energy), ph = +1 for
+ * iph = 1 (particle), -1 for iph = 2 (hole).  dyn is complex interleaved [2*nw].
+ * ||phi||^2 < cutoff_dynamic gives dyn = 0.  a/b (may be NULL, capacity niter)
+ * receive the raw tridiagonal. */
+int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int nitergreen,
+               double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn,
+               double *a, double *b);
+
+/* Batched excitation Lanczos (no reference counterpart as one call: it fuses the
+ * per-(state, orbital, iph) calls of compute_dynamic made by
+ * green_class_computeAA.h:101-302): njobs independent runs, job k in sector
+ * secs[k] from the host vector phis[k] (length dimen of that sector), all advanced
+ * together, one batched kernel per recurrence phase.  Per job: iph[k], energy[k];
+ * output dyn[k][2*nw], a[k][nitergreen], b[k][nitergreen]. */
+int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis,
+                       const int *iph, const double *energy, int nitergreen,
+                       double cutoff_dynamic, int precision, int nw, const double *wn,
+                       double *dyn, double *a, double *b);
+
+/* ----------------------------------------------------------- diagnostics */
+
+/* Runs a fresh FP64 (ED_PREC_F64) or FP32 (ED_PREC_F32) Lanczos recurrence of
+ * `reps` steps from the default start vector, launching its kernels one by one with
+ * a HIP event pair around every fused H.v kernel on the sector's stream.
+ * *avg_ms = average duration of that kernel; *bytes_per_launch = its algorithmic
+ * HBM bytes 3 * sizeof(T) * dimen_H() (read v_j and v_{j-1}, write w).  No
+ * reference counterpart: it is the measurement hook of the engine. */
+int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ED_HIP_H */

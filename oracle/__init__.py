@@ -112,8 +112,9 @@ def start_vector(dim: int) -> np.ndarray:
     x ^= x >> np.uint64(29)
     x *= np.uint64(0xBF58476D1CE4E5B9)
     x ^= x >> np.uint64(32)
-    r = (x >> np.uint64(11)).astype(np.float64) / float(1 << 53)
-    return np.floor(r * 1000.0 + 1.0) / 1000.0   # in (0, 1], never all-zero
+    m = x >> np.uint64(11)                                   # 53 random bits
+    k = (m * np.uint64(1000)) >> np.uint64(53)               # floor(1000 * m / 2^53) in [0, 999]
+    return (k + np.uint64(1)).astype(np.float64) / 1000.0    # in (0, 1], never all-zero
 
 
 class OracleSector:

@@ -1,0 +1,97 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol include/ed_hip.h
+declares, and the host-side argument checks raise before any device work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ed_hip.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ed_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_expected_surface():
+    syms = declared_symbols()
+    for must in ("ed_sector_create", "ed_sector_destroy", "ed_hmult_f64", "ed_sector_dim", "ed_lanczos_diagonalize",
+                 "ed_lanczos_gs_vector", "ed_lanczos_run", "ed_dynamic", "ed_continued_fraction", "ed_apply_c"):
+        assert must in syms
+
+
+def test_library_exports_every_declared_symbol():
+    import ed_solver_amd as ed
+    L = ed.lib()
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from ed_solver_amd._lib import SIGNATURES
+    assert sorted(SIGNATURES) == declared_symbols()
+
+
+def test_version_and_defaults():
+    import ed_solver_amd as ed
+    from ed_solver_amd._lib import LanczosOpts
+    assert ed.lib().ed_version().startswith(b"ed_hip")
+    o = LanczosOpts()
+    assert ed.lib().ed_lanczos_default_opts(ctypes.byref(o)) == 0
+    assert (o.nitermax, o.neigen, o.tolerance, o.precision) == (128, 1, 1e-12, 0)
+
+
+def test_start_vector_host_entry_matches_oracle():
+    import ed_solver_amd as ed
+    import oracle
+    for n in (1, 7, 10000, 25001):
+        a = ed.start_vector(n)
+        b = oracle.start_vector(n)
+        assert np.array_equal(a, b)
+        assert a.min() > 0 and a.max() <= 1.0
+
+
+def test_tridiag_eig_vs_scipy():
+    import scipy.linalg as la
+    import ed_solver_amd as ed
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 5, 40, 128):
+        a = rng.uniform(-3, 3, n)
+        b = np.concatenate([[7.0], rng.uniform(0.01, 1.5, n - 1)])   # b[0] is ||v1||, not referenced
+        w, z = ed.tridiag_eig(a, b)
+        if n == 1:
+            assert w[0] == a[0]
+            continue
+        w_ref, z_ref = la.eigh_tridiagonal(a, b[1:])
+        assert np.max(np.abs(w - w_ref)) < 1e-12 * max(1, np.abs(w_ref).max())
+        T = np.diag(a) + np.diag(b[1:], 1) + np.diag(b[1:], -1)
+        assert np.max(np.abs(T @ z.T - z.T * w)) < 1e-11
+        assert np.max(np.abs(z @ z.T - np.eye(n))) < 1e-12
+
+
+def test_errors_raise_without_device_work():
+    import ed_solver_amd as ed
+    from ed_solver_amd._lib import LanczosOpts, check
+    with pytest.raises(ed.EDError):
+        check(ed.lib().ed_lanczos_default_opts(None))
+    with pytest.raises(ed.EDError):
+        check(ed.lib().ed_start_vector(0, None))
+    with pytest.raises(ed.EDError):
+        ed.invert_zmtridiag(np.zeros(0), np.zeros(0), np.zeros(3))
+    with pytest.raises(ValueError):
+        ed.aim.AIMParams(nc=3, nb=13, ec=np.zeros((2, 3, 3)), eb=np.zeros((2, 13, 13)), vbc=np.zeros((2, 13, 3)),
+                         u=np.zeros((3, 3)), jj=np.zeros((3, 3)), jhund=0.0)
+
+
+def test_package_has_no_cpu_fallback():
+    """The product never imports the oracle."""
+    pkg = os.path.join(ROOT, "ed_solver_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith(".py"):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in src and "from oracle" not in src, f

@@ -427,7 +427,7 @@ static int ensure_io(ed_sector *s)
 // Krylov basis with nslots vectors of precision prec (leading dimension padded to 64).
 static int ensure_basis(ed_sector *s, int prec, int nslots)
 {
-    const size_t esz = prec == ED_PREC_F64 ? sizeof(double) : sizeof(float);
+    const size_t esz = sizeof(double);   // sized for FP64 so FP32 / FP64 runs share one allocation
     const int64_t ld = (s->dim + 63) / 64 * 64;
     const size_t need = esz * (size_t)ld * nslots;
     if (s->d_basis && s->basis_bytes >= need) {
@@ -439,11 +439,12 @@ static int ensure_basis(ed_sector *s, int prec, int nslots)
     if (s->d_basis) (void)hipFree(s->d_basis);
     s->d_basis = nullptr;
     s->basis_bytes = 0;
-    if (s->graph.exec) {
-        (void)hipGraphExecDestroy(s->graph.exec);
-        (void)hipGraphDestroy(s->graph.graph);
-        s->graph = GraphCache();
-    }
+    for (GraphCache &G : s->graphs)
+        if (G.exec) {
+            (void)hipGraphExecDestroy(G.exec);
+            (void)hipGraphDestroy(G.graph);
+            G = GraphCache();
+        }
     hipError_t e = hipMalloc(&s->d_basis, need);
     if (e != hipSuccess)
         return fail(ED_ERR_NOMEM, "Krylov basis of " + std::to_string(need) + " bytes: " + hipGetErrorString(e));
@@ -500,10 +501,9 @@ static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
 {
     hipStream_t st = s->stream;
     if (nsteps == chunk && chunk > 1) {
-        GraphCache &G = s->graph;
-        const int key_prec = prec * 2 + (reorth ? 1 : 0);
-        if (!(G.exec && G.prec == key_prec && G.chunk == chunk && G.nslots == s->basis_nslots &&
-              G.base == s->d_basis)) {
+        const int key = prec * 2 + (reorth ? 1 : 0);
+        GraphCache &G = s->graphs[key];
+        if (!(G.exec && G.chunk == chunk && G.nslots == s->basis_nslots && G.base == s->d_basis)) {
             if (G.exec) {
                 (void)hipGraphExecDestroy(G.exec);
                 (void)hipGraphDestroy(G.graph);
@@ -513,7 +513,7 @@ static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
             run_steps(s, prec, nsteps, reorth, st);
             ED_HIP_CHECK(hipStreamEndCapture(st, &G.graph));
             ED_HIP_CHECK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
-            G.prec = key_prec;
+            G.prec = key;
             G.chunk = chunk;
             G.nslots = s->basis_nslots;
             G.base = s->d_basis;
@@ -579,7 +579,62 @@ static int check_common(ed_sector *s, int prec)
     return ED_OK;
 }
 
-// Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134).
+struct PhaseOut {
+    int iters = 0;
+    std::vector<double> alpha, beta;
+};
+
+// One Lanczos_fast_diagonalize loop (Lanczos_plain.f90:33-134) in precision `prec`
+// from a host or device start vector: chunks of `chunk` steps on the device, then the
+// reference's convergence test on the host for every step of the chunk.
+static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig, int chunk, int reorth,
+                         const double *h_start, const double *d_start, PhaseOut &out)
+{
+    ED_TRY(ensure_state(s));
+    ED_TRY(ensure_basis(s, prec, niter + 1));
+    s->basis_valid_iters = 0;
+    ED_TRY(lz_start(s, prec, h_start, d_start));
+    std::vector<double> last(neig, 1e300), w(neig);
+    HostLz h;
+    int done = 0, final_iter = 0;
+    while (done < niter && !final_iter) {
+        const int c = std::min(chunk, niter - done);
+        ED_TRY(steps(s, prec, c, chunk, reorth));
+        ED_TRY(read_state(s, h, done + c));
+        if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
+        const int completed = h.j - 1;   // steps whose beta_{j+1} is known
+        for (int it = done + 1; it <= done + c; ++it) {
+            if (it > completed)   // the reference STOPs at the start of this step (Lanczos_plain.h:85-92)
+                return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(it));
+            tridiag_lowest(it, h.alpha.data(), h.beta.data(), neig, w.data());
+            const int m = std::min(neig, it);
+            bool allc = true;
+            for (int k = 0; k < neig; ++k)
+                if (!((k < m) && std::fabs(w[k] - last[k]) < tol && it > 2)) allc = false;
+            const bool one_step = (it == 1) && std::fabs(h.beta[1]) < 1e-24;
+            if (allc || it == niter || one_step) {
+                final_iter = it;
+                break;
+            }
+            for (int k = 0; k < m; ++k) last[k] = w[k];
+        }
+        done += c;
+    }
+    out.iters = final_iter;
+    out.alpha.assign(h.alpha.begin(), h.alpha.begin() + final_iter);
+    out.beta.assign(h.beta.begin(), h.beta.begin() + final_iter);
+    s->basis_valid_iters = final_iter;
+    s->basis_prec = prec;
+    return ED_OK;
+}
+
+static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_psi);
+
+// Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134).  ED_PREC_F64 is the reference
+// algorithm.  ED_PREC_F32 is the mixed-precision form: the loop runs on FP32 vectors
+// and FP32 H.v (FP64 dots and scalars) to |dlambda| < max(tol, 1e-6), then an FP64
+// Lanczos restarted on the device from that Ritz vector finishes to `tol`; the
+// returned T, eigenpairs and kept basis are those of the FP64 phase.
 static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const double *start, int *iters_out,
                                int *neig_out, double *evals, double *vecp, double *diag, double *subdiag)
 {
@@ -593,52 +648,34 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
     neig = std::min(neig, niter);
     if (neig < 1) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: neigen < 1");
     const int chunk = std::max(1, o->check_every);
-    ED_TRY(ensure_state(s));
-    ED_TRY(ensure_basis(s, o->precision, niter + 1));
-    s->basis_valid_iters = 0;
-    ED_TRY(lz_start(s, o->precision, start, nullptr));
-    std::vector<double> last(neig, 1e300), w(neig);
-    std::vector<int> conv(neig, 0);
-    HostLz h;
-    int done = 0, final_iter = 0;
-    while (done < niter && !final_iter) {
-        const int c = std::min(chunk, niter - done);
-        ED_TRY(steps(s, o->precision, c, chunk, o->reorth));
-        ED_TRY(read_state(s, h, done + c));
-        if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
-        const int completed = h.j - 1;   // steps whose beta_{j+1} is known
-        for (int it = done + 1; it <= done + c; ++it) {
-            if (it > completed)   // the reference STOPs at the start of this step (Lanczos_plain.h:85-92)
-                return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(it));
-            tridiag_lowest(it, h.alpha.data(), h.beta.data(), neig, w.data());
-            const int m = std::min(neig, it);
-            bool allc = true;
-            for (int k = 0; k < neig; ++k) {
-                conv[k] = (k < m) && std::fabs(w[k] - last[k]) < o->tolerance && it > 2;
-                if (!conv[k]) allc = false;
-            }
-            const bool one_step = (it == 1) && std::fabs(h.beta[1]) < 1e-24;
-            if (allc || it == niter || one_step) {
-                final_iter = it;
-                break;
-            }
-            for (int k = 0; k < m; ++k) last[k] = w[k];
-        }
-        done += c;
+    PhaseOut ph;
+    s->last_f32_iters = 0;
+    if (o->precision == ED_PREC_F32) {
+        PhaseOut p32;
+        ED_TRY(lanczos_phase(s, ED_PREC_F32, std::max(o->tolerance, 1e-6), niter, 1, chunk, o->reorth, start, nullptr,
+                             p32));
+        std::vector<double> ev(p32.iters), z((size_t)p32.iters * p32.iters);
+        ED_TRY(tridiag_eig(p32.iters, p32.alpha.data(), p32.beta.data(), ev.data(), z.data()));
+        ED_TRY(ensure_io(s));
+        ED_TRY(gs_vector_dev(s, p32.iters, z.data(), s->d_io_b));
+        s->last_f32_iters = p32.iters;
+        ED_TRY(lanczos_phase(s, ED_PREC_F64, o->tolerance, niter, neig, chunk, o->reorth, nullptr, s->d_io_b, ph));
+    } else {
+        ED_TRY(lanczos_phase(s, ED_PREC_F64, o->tolerance, niter, neig, chunk, o->reorth, start, nullptr, ph));
     }
-    const int it = final_iter;
+    const int it = ph.iters;
+    neig = std::min(neig, it);
     std::vector<double> ev(it), z((size_t)it * it);
-    ED_TRY(tridiag_eig(it, h.alpha.data(), h.beta.data(), ev.data(), z.data()));
+    ED_TRY(tridiag_eig(it, ph.alpha.data(), ph.beta.data(), ev.data(), z.data()));
     for (int k = 0; k < neig; ++k) {
         if (evals) evals[k] = ev[k];
         if (vecp)
             for (int r = 0; r < it; ++r) vecp[(size_t)k * it + r] = z[(size_t)k * it + r];
     }
-    if (diag) std::memcpy(diag, h.alpha.data(), sizeof(double) * it);
-    if (subdiag) std::memcpy(subdiag, h.beta.data(), sizeof(double) * it);
+    if (diag) std::memcpy(diag, ph.alpha.data(), sizeof(double) * it);
+    if (subdiag) std::memcpy(subdiag, ph.beta.data(), sizeof(double) * it);
     *iters_out = it;
     if (neig_out) *neig_out = neig;
-    s->basis_valid_iters = it;
     return ED_OK;
 }
 
@@ -765,6 +802,13 @@ int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const do
                            int *neig_out, double *evals, double *vecp, double *diag, double *subdiag)
 {
     return lanczos_diagonalize(sec, opts, start, iters, neig_out, evals, vecp, diag, subdiag);
+}
+
+int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase)
+{
+    ED_TRY(check_common(sec, 0));
+    if (iters_f32_phase) *iters_f32_phase = sec->last_f32_iters;
+    return ED_OK;
 }
 
 int ed_lanczos_run(ed_sector *sec, int niter, const double *start, int precision, double *diag, double *subdiag)

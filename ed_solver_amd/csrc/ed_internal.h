@@ -117,7 +117,8 @@ struct ed_sector {
     ed::LzState *d_state = nullptr;
     double *d_partials = nullptr;
     double *d_io_a = nullptr, *d_io_b = nullptr;   // host-API staging vectors [dim]
-    ed::GraphCache graph;
+    ed::GraphCache graphs[4];      // captured Lanczos chunks, key = precision * 2 + reorth
+    int last_f32_iters = 0;
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double hv_ms_sum = 0.0;

@@ -390,8 +390,10 @@ static void free_sector(ed_sector *S)
                   S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    if (S->graph.exec) (void)hipGraphExecDestroy(S->graph.exec);
-    if (S->graph.graph) (void)hipGraphDestroy(S->graph.graph);
+    for (ed::GraphCache &G : S->graphs) {
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.graph) (void)hipGraphDestroy(G.graph);
+    }
     if (S->ev0) (void)hipEventDestroy(S->ev0);
     if (S->ev1) (void)hipEventDestroy(S->ev1);
     if (S->stream) (void)hipStreamDestroy(S->stream);

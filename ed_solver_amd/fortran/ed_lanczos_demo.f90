@@ -1,0 +1,85 @@
+! Fortran host driver over H_class_hip.
+!
+! Reads a model file (first line: nc nb nup ndn nitermax; then Ec(2*nc*nc),
+! Eb(2*nb*nb), Vbc(2*nb*nc), U(nc*nc), JJ(nc*nc) in C row-major order, then Jhund
+! and cutoff) and runs:
+!   1. the plain Lanczos loop of Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134
+!      with lanc_opt, Lanczos_plain.h:66-173) on the host, calling Hmult__ for every
+!      H.v -- the reference's own solver shape on top of the drop-in Hmult__;
+!   2. the coarse device path Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo.
+! Prints E0 from both, the iteration count and the Rayleigh quotient of the device
+! ground state.
+program ed_lanczos_demo
+  use iso_c_binding
+  use ed_hip
+  use H_class_hip
+  implicit none
+  integer :: nc, nb, nup, ndn, nitermax, n, iter, niter, k
+  real(8), allocatable :: ec(:), eb(:), vbc(:), u(:), jj(:)
+  real(8) :: jhund, cutoff
+  real(8), allocatable :: vin(:), vout(:), tmp(:), diag(:), subdiag(:), w(:), z(:), psi(:), hpsi(:)
+  real(8) :: normv_in, normv_out, aa, ddiag, last, e0_host, e0_dev, rq
+  character(len=512) :: fname
+  integer :: u_in
+
+  call get_command_argument(1, fname)
+  open(newunit=u_in, file=trim(fname), status='old', action='read')
+  read(u_in, *) nc, nb, nup, ndn, nitermax
+  allocate(ec(2*nc*nc), eb(max(1, 2*nb*nb)), vbc(max(1, 2*nb*nc)), u(nc*nc), jj(nc*nc))
+  eb = 0; vbc = 0
+  read(u_in, *) ec
+  if (nb > 0) read(u_in, *) eb(1:2*nb*nb)
+  if (nb > 0) read(u_in, *) vbc(1:2*nb*nc)
+  read(u_in, *) u
+  read(u_in, *) jj
+  read(u_in, *) jhund, cutoff
+  close(u_in)
+
+  call ed_check(ed_init(0_c_int), 'ed_init')
+  call new_H(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, nup, ndn)
+  n = dimen_H()
+  niter = min(n - 1, nitermax)
+  allocate(vin(n), vout(n), tmp(n), diag(niter), subdiag(niter), w(niter), z(niter*niter))
+
+  ! ---- 1. host Lanczos loop over Hmult__ (one_step_Lanczos_fast + lanc_opt)
+  call ed_check(ed_start_vector(int(n, c_int64_t), vin), 'ed_start_vector')
+  tmp = 0; diag = 0; subdiag = 0
+  normv_in = sqrt(sum(vin*vin)); normv_out = 0; subdiag(1) = normv_in
+  last = huge(1d0); e0_host = 0
+  do iter = 1, niter
+    if (iter == 1) then
+      aa = 1d0 / normv_in
+    else
+      if (abs(normv_out) < 1d-25) stop '0 norm vector in lanczos, critical'
+      aa = 1d0 / normv_out
+    end if
+    vin = vin * aa
+    call Hmult__(n, vout, vin)
+    if (iter > 1) vout = vout - subdiag(iter) * tmp
+    ddiag = sum(vout * vin)
+    diag(iter) = ddiag
+    vout = vout - ddiag * vin
+    normv_out = sqrt(sum(vout * vout))
+    if (iter < niter) subdiag(iter + 1) = normv_out
+    tmp = vin
+    vin = vout
+    call ed_check(ed_tridiag_eig(int(iter, c_int), diag, subdiag, w, z), 'ed_tridiag_eig')
+    e0_host = w(1)
+    if ((abs(w(1) - last) < 1d-12 .and. iter > 2) .or. iter == niter .or. (iter == 1 .and. normv_out < 1d-24)) exit
+    last = w(1)
+  end do
+  k = min(iter, niter)
+  write(*, '(a,f22.14,a,i0)') 'E0_host_loop ', e0_host, ' iters ', k
+
+  ! ---- 2. coarse device path: Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo
+  call Lanczos_GPU_updo(k, n, diag(1:k), subdiag(1:k))
+  call ed_check(ed_tridiag_eig(int(k, c_int), diag, subdiag, w, z), 'ed_tridiag_eig')
+  e0_dev = w(1)
+  allocate(psi(n), hpsi(n))
+  call Lanczos_GPU_get_GS_updo(k, n, z(1:k), psi)
+  call Hmult__(n, hpsi, psi)
+  rq = sum(psi * hpsi) / sum(psi * psi)
+  write(*, '(a,f22.14)') 'E0_gpu_updo ', e0_dev
+  write(*, '(a,f22.14)') 'E0_rayleigh ', rq
+  call delete_H()
+end program ed_lanczos_demo

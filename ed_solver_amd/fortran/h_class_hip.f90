@@ -1,0 +1,108 @@
+! H_class surface over the MI355X engine.
+!
+! Keeps the interface the reference's solvers call (H_class.f90:13-152):
+!   new_H / delete_H / Hmult__ (= Hmultr) / dimen_H
+! and the coarse GPU entry points the reference sketches for which_lanczos='GPU'
+! (Lanczos_gpu.f90:48,106; green_class_compute_dynamic.f90:468):
+!   Lanczos_GPU_updo / Lanczos_GPU_get_GS_updo / Lanczos_dynamic_GPU_updo
+! so Lanczos_plain, ED_ARPACK, Block_Lanczos and green_class_compute_dynamic keep
+! working unchanged on top of it.
+!
+! new_H takes the model arrays the reference reads from AIM_type
+! (quantum_algebra_Anderson_Impurity_class2.f90:62-135: Ec, Eb, Vbc per spin; the
+! impurity U and JJmatrix; Jhund; cutoff_hamilt_param) as plain arrays, in C row-major
+! order, so the caller's new_H(AIM, sector) wrapper only gathers them.
+module H_class_hip
+  use iso_c_binding
+  use ed_hip
+  implicit none
+  private
+
+  type(c_ptr), save :: sector_h = c_null_ptr        ! H_class.f90:11
+  real(c_double), allocatable, target, save :: m_ec(:), m_eb(:), m_vbc(:), m_u(:), m_jj(:)
+
+  interface Hmult__
+    module procedure Hmultr
+  end interface
+
+  public :: new_H, delete_H, Hmult__, Hmultr, dimen_H, sector_handle
+  public :: Lanczos_GPU_updo, Lanczos_GPU_get_GS_updo, Lanczos_dynamic_GPU_updo
+
+contains
+
+  subroutine new_H(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, nup, ndn)
+    integer, intent(in)        :: nc, nb, nup, ndn
+    real(8), intent(in)        :: ec(2*nc*nc), eb(max(1, 2*nb*nb)), vbc(max(1, 2*nb*nc)), u(nc*nc), jj(nc*nc)
+    real(8), intent(in)        :: jhund, cutoff
+    type(ed_model) :: m
+    call delete_H()
+    m_ec = ec; m_eb = eb; m_vbc = vbc; m_u = u; m_jj = jj
+    m%nc = nc; m%nb = nb
+    m%ec = c_loc(m_ec); m%eb = c_loc(m_eb); m%vbc = c_loc(m_vbc); m%u = c_loc(m_u); m%jj = c_loc(m_jj)
+    m%jhund = jhund; m%cutoff = cutoff
+    call ed_check(ed_sector_create(m, int(nup, c_int), int(ndn, c_int), sector_h), 'new_H')
+  end subroutine
+
+  subroutine delete_H()
+    integer(c_int) :: rc
+    if (c_associated(sector_h)) then           ! no-op if H was never created (H_class.f90:48-58)
+      rc = ed_sector_destroy(sector_h)
+      sector_h = c_null_ptr
+    end if
+  end subroutine
+
+  function sector_handle() result(p)
+    type(c_ptr) :: p
+    p = sector_h
+  end function
+
+  subroutine Hmultr(n, vec_out, vec_in)
+    integer                :: n
+    real(8), intent(inout) :: vec_out(n)
+    real(8), intent(in)    :: vec_in(n)
+    if (.not. c_associated(sector_h)) stop 'Hmultr, no array are associated!'
+    call ed_check(ed_hmult_f64(sector_h, int(n, c_int64_t), vec_out, vec_in), 'Hmultr')
+  end subroutine
+
+  pure integer function dimen_H()
+    interface
+      pure integer(c_int64_t) function dim_pure(sec) bind(C, name="ed_sector_dim")
+        import :: c_int64_t, c_ptr
+        type(c_ptr), value :: sec
+      end function
+    end interface
+    dimen_H = int(dim_pure(sector_h))
+  end function
+
+  ! Lanczos_GPU_updo(Niter, sizevec, diag, subdiag) (Lanczos_gpu.f90:106): Niter steps
+  ! from the default start vector; subdiag(1) = ||v1|| as in the reference's storage.
+  subroutine Lanczos_GPU_updo(Niter, sizevec, diag, subdiag)
+    integer, intent(in)  :: Niter, sizevec
+    real(8), intent(out) :: diag(Niter), subdiag(Niter)
+    if (sizevec /= dimen_H()) stop 'Lanczos_GPU_updo: sizevec /= dimen_H()'
+    call ed_check(ed_lanczos_run(sector_h, int(Niter, c_int), c_null_ptr, ED_PREC_F64, diag, subdiag), &
+                  'Lanczos_GPU_updo')
+  end subroutine
+
+  ! Lanczos_GPU_get_GS_updo(iter, sizevec, vecp, vec) (Lanczos_gpu.f90:48).
+  subroutine Lanczos_GPU_get_GS_updo(iter, sizevec, vecp, vec)
+    integer, intent(in)  :: iter, sizevec
+    real(8), intent(in)  :: vecp(iter)
+    real(8), intent(out) :: vec(sizevec)
+    real(8), allocatable :: tmp(:)
+    tmp = vecp
+    call ed_check(ed_lanczos_gs_vector(sector_h, int(iter, c_int), tmp, vec), 'Lanczos_GPU_get_GS_updo')
+  end subroutine
+
+  ! Lanczos_dynamic_GPU_updo(Niter, sizevec, diag, subdiag, vec)
+  ! (green_class_compute_dynamic.f90:468): Niter steps from the excitation vector.
+  subroutine Lanczos_dynamic_GPU_updo(Niter, sizevec, diag, subdiag, vec)
+    integer, intent(in)         :: Niter, sizevec
+    real(8), intent(out)        :: diag(Niter), subdiag(Niter)
+    real(8), intent(in), target :: vec(sizevec)
+    if (sizevec /= dimen_H()) stop 'Lanczos_dynamic_GPU_updo: sizevec /= dimen_H()'
+    call ed_check(ed_lanczos_run(sector_h, int(Niter, c_int), c_loc(vec), ED_PREC_F64, diag, subdiag), &
+                  'Lanczos_dynamic_GPU_updo')
+  end subroutine
+
+end module H_class_hip

@@ -49,6 +49,7 @@ class LanczosResult:
     diag: np.ndarray         # alpha_1..alpha_iters
     subdiag: np.ndarray      # ||v1||, beta_2..beta_iters
     precision: str
+    iters_f32: int = 0       # FP32 phase of the mixed form (0 for FP64)
 
 
 class Sector:
@@ -152,8 +153,10 @@ class Sector:
         check(lib().ed_lanczos_diagonalize(self.handle, ctypes.byref(o), sp, ctypes.byref(it), ctypes.byref(ne),
                                            dptr(ev), dptr(vecp), dptr(dg), dptr(sd)))
         k, m = ne.value, it.value
+        i32 = ctypes.c_int()
+        check(lib().ed_lanczos_info(self.handle, ctypes.byref(i32)))
         return LanczosResult(m, ev[:k].copy(), vecp[:k * m].reshape(k, m).copy(), dg[:m].copy(), sd[:m].copy(),
-                             "f64" if o.precision == ED_PREC_F64 else "f32")
+                             "f64" if o.precision == ED_PREC_F64 else "mixed", i32.value)
 
     def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
         """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""

@@ -144,10 +144,17 @@ int ed_lanczos_default_opts(ed_lanczos_opts *opts);
  * convergence test.  Outputs (host): *iters; evals[neigen'] (neigen' = min(neigen,
  * dim, niter)); vecp[neigen'][*iters] Lanczos coefficients (row k = eigenvector
  * k of T); diag/subdiag[*iters] (may be NULL, capacity nitermax).  The Krylov basis
- * stays on the device for ed_lanczos_gs_vector. */
+ * stays on the device for ed_lanczos_gs_vector.  precision ED_PREC_F32 selects the
+ * mixed form: FP32-vector Lanczos to |dlambda| < max(tolerance, 1e-6), then an FP64
+ * Lanczos restarted from that Ritz vector to `tolerance`; the outputs describe the
+ * FP64 phase (ed_lanczos_info gives the FP32 phase's iteration count). */
 int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const double *start,
                            int *iters, int *neig_out, double *evals, double *vecp,
                            double *diag, double *subdiag);
+
+/* Iterations of the FP32 phase of the last mixed-precision ed_lanczos_diagonalize
+ * on this sector (0 after an FP64 call).  No reference counterpart. */
+int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase);
 
 /* Lanczos_GPU_updo (Lanczos_gpu.f90:106): exactly `niter` Lanczos steps (no
  * convergence test) from `start` (host or NULL); writes diag[niter], subdiag[niter].

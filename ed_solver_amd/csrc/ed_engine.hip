@@ -8,19 +8,24 @@
 //             + sum_k off_dn(ido, k) V(:, jdo_k)                                 (v)  down hops
 //             + sum_t J_t V(swap_up(:, cu_t), swap_dn(ido, cd_t))                (vii)-(viii) Hund
 //
-// One workgroup owns one column (or several short columns): the column is staged in
-// LDS once, the up hops gather from LDS through an ELL table (coalesced over rows),
-// the down hops stream whole neighbour columns (coalesced, row-contiguous), the Hund
-// terms gather single elements.  No atomics: every output element is written by
-// exactly one thread.
+// A workgroup owns CPB whole columns: they are staged in LDS once; each thread owns
+// rows of those columns, loads the row's up-hop entries (ELL, width unrolled at compile
+// time) once and applies them to all CPB columns from LDS; the down hops stream whole
+// neighbour columns (uniform per column, coalesced over rows); the Hund terms gather
+// single elements.  No atomics: every output element is written by exactly one thread.
+// CPB is picked per sector by a small autotuner (1, 2 or 4 columns per workgroup).
 //
 // Lanczos (lanc_opt, Lanczos_plain.h:66-173) runs as two kernels per step on
-// device-resident scalars (LzState), so a chunk of steps needs no host round trip:
-//   k_hv<MODE=1>:   w = H(u_j)/beta_j - (beta_j/beta_{j-1}) u_{j-1};  alpha_j = w.v_j
-//   k_lz_update:    w -= alpha_j v_j;  beta_{j+1} = ||w||;  j += 1
-// with u_j the un-normalised Lanczos vectors (v_j = u_j / beta_j) kept in a Krylov
-// basis on the device, and every reduction finished by the last workgroup in a fixed
-// order (deterministic).
+// device-resident scalars (LzState), so a chunk of steps runs from a captured graph
+// without the host:
+//   k_hv<MODE=1>:   beta_j = ||u_j||; w = H(u_j)/beta_j - (beta_j/beta_{j-1}) u_{j-1};
+//                   partial sums of w.v_j
+//   k_lz_update:    alpha_j = w.v_j; w -= alpha_j v_j; partial sums of ||w||^2
+// with u_j the un-normalised Lanczos vectors (v_j = u_j / beta_j) kept in a Krylov basis
+// on the device.  Each kernel writes one partial sum per workgroup; the NEXT kernel
+// finishes that reduction in every workgroup, in the same fixed order (so every
+// workgroup obtains the same bits).  This avoids cross-XCD atomics and release fences,
+// which cost more than the reduction itself on MI355X.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -45,107 +50,192 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block sum in a fixed order; the result is valid in thread 0.
-__device__ inline double block_sum(double v, double *red)
+// Block sum in a fixed order, returned to every thread.  `red` holds blockDim/64 + 1.
+__device__ inline double block_sum_all(double v, double *red)
 {
     v = wave_sum(v);
+    const int nw = blockDim.x >> 6;
+    __syncthreads();   // red may still be read from a previous call
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
-    return t;
-}
-
-// Publishes this workgroup's partial (thread 0's value) and lets the last workgroup to
-// arrive reduce all partials in index order.  Returns true in that last workgroup,
-// with the total in *total (thread 0).  The ticket is reset by the last workgroup.
-__device__ bool last_block_total(double part, double *partials, unsigned *ticket, double *red, int *flag,
-                                 double *total)
-{
-    const int nblk = gridDim.x * gridDim.y;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0) {
-        partials[bid] = part;
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        *flag = (t == (unsigned)(nblk - 1));
+        double t = 0.0;
+        for (int i = 0; i < nw; ++i) t += red[i];
+        red[nw] = t;
     }
     __syncthreads();
-    if (!*flag) return false;
-    __threadfence();
-    const volatile double *vp = partials;
+    return red[nw];
+}
+
+// Sum of n partials in a fixed order (thread-strided, then the block tree); every
+// workgroup that calls it on the same data obtains the same value.
+__device__ inline double sum_partials(const double *p, int n, double *red)
+{
     double v = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) v += vp[i];
-    v = block_sum(v, red);
-    if (threadIdx.x == 0) {
-        *total = v;
-        *ticket = 0u;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    return block_sum_all(v, red);
+}
+
+__device__ inline bool block0() { return blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0; }
+
+// Prologue of a Lanczos H.v phase: beta_j from the update phase's partials, breakdown
+// test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
+template <typename T>
+__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, int &j, double &sj, double &cprev)
+{
+    if (lz.st->status) return false;
+    j = lz.st->j_k1;
+    const double b2 = sum_partials(lz.partB, lz.nB, red);
+    const double bj = sqrt(b2);
+    if (bj < kMinLanczosNorm) {
+        if (block0()) lz.st->status = (j == 1) ? 2 : 1;
+        return false;
     }
+    if (block0()) {
+        lz.st->beta[j - 1] = bj;
+        lz.st->j_k2 = j;
+    }
+    sj = 1.0 / bj;
+    cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
     return true;
 }
 
-// ------------------------------------------------------------------ H.v kernel
+// ------------------------------------------------------------------ H.v kernels
 
+// Row-owner kernel: CPB columns per workgroup, ELL width KU unrolled.
 // MODE 0: y = H x for grid.y vectors (leading dimension ld).
 // MODE 1: Lanczos phase 1 on the Krylov slots of `lz` (x = u_j, y = u_{j+1}).
-// ONECOL: one column per workgroup (d_up >= workgroup size), column index uniform.
-template <typename T, int MODE, bool ONECOL>
-__global__ __launch_bounds__(kBlock) void k_hv(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
-                                                int64_t ld, LzArgs<T> lz)
+template <typename T, int MODE, int CPB, int KU>
+__global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+                                                     int64_t ld, LzArgs<T> lz)
 {
     extern __shared__ double smem_d[];
     T *col = reinterpret_cast<T *>(smem_d);
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
+    __shared__ double red[kBlock / 64 + 1];
 
     const int dup = a.dup, ncfg = a.ncfg;
-    const int c0 = ONECOL ? blockIdx.x : blockIdx.x * a.cpb;
-    const int ncol = ONECOL ? 1 : min(a.cpb, a.ddn - c0);
+    const int c0 = blockIdx.x * CPB;
     const T *__restrict__ x;
     T *__restrict__ y;
     const T *__restrict__ prev = nullptr;
     double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (lz.st->status) return;
-        j = lz.st->j;
+        if (!lz_prologue(lz, red, j, sj, cprev)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
-        const double bj = lz.st->beta[j - 1];
-        sj = 1.0 / bj;
-        if (j > 1) {
-            prev = lz.slot(j - 1);
-            cprev = bj / lz.st->beta[j - 2];
-        }
+        if (j > 1) prev = lz.slot(j - 1);
     } else {
         x = xin + (int64_t)blockIdx.y * ld;
         y = yout + (int64_t)blockIdx.y * ld;
     }
-    const int64_t base = (int64_t)c0 * dup;
-    const int nel = ncol * dup;
-    for (int e = threadIdx.x; e < nel; e += kBlock) col[e] = x[base + e];
+    // per-column uniform data
+    int ido[CPB], cd[CPB], p0[CPB], p1[CPB];
+    T ddg[CPB];
+    bool ok[CPB];
+#pragma unroll
+    for (int c = 0; c < CPB; ++c) {
+        ok[c] = c0 + c < a.ddn;
+        ido[c] = ok[c] ? c0 + c : c0;
+        cd[c] = a.imp_dn[ido[c]];
+        p0[c] = a.ptr_dn[ido[c]];
+        p1[c] = ok[c] ? a.ptr_dn[ido[c] + 1] : p0[c];
+        ddg[c] = a.diag_dn[ido[c]];
+    }
+    const int ncol = min(CPB, a.ddn - c0);
+    const T *xc = x + (int64_t)c0 * dup;
+    for (int e = threadIdx.x; e < ncol * dup; e += kBlock) col[e] = xc[e];
     __syncthreads();
 
     double dot = 0.0;
+    for (int r = threadIdx.x; r < dup; r += kBlock) {
+        int rk[KU];
+        T v[KU];
+#pragma unroll
+        for (int k = 0; k < KU; ++k) {
+            rk[k] = a.ell_rk_up[k * dup + r];
+            v[k] = a.ell_val_up[k * dup + r];
+        }
+        const T du = a.diag_up[r];
+        const int cu = a.imp_up[r];
+#pragma unroll
+        for (int c = 0; c < CPB; ++c) {
+            if (!ok[c]) continue;
+            const T *cv = col + c * dup;
+            const T xv = cv[r];
+            T acc = (du + ddg[c] + a.uimp[cu * ncfg + cd[c]]) * xv;
+#pragma unroll
+            for (int k = 0; k < KU; ++k) acc += v[k] * cv[rk[k]];
+#pragma unroll 4
+            for (int k = p0[c]; k < p1[c]; ++k) acc += a.off_dn[k] * x[r + a.rk_dn[k] * dup];
+            if (a.hund) {
+                const int h0 = a.hund_ptr[cu * ncfg + cd[c]], h1 = a.hund_ptr[cu * ncfg + cd[c] + 1];
+                for (int t = h0; t < h1; ++t)
+                    acc += a.hund_val[t] * x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido[c] * ncfg + a.hund_cd[t]] * dup];
+            }
+            const int idx = (c0 + c) * dup + r;
+            if (MODE == 1) {
+                double yv = (double)acc * sj;
+                if (j > 1) yv -= cprev * (double)prev[idx];
+                const T yt = (T)yv;
+                y[idx] = yt;
+                dot += (double)yt * ((double)xv * sj);
+            } else {
+                y[idx] = acc;
+            }
+        }
+    }
+    if (MODE == 1) {
+        const double p = block_sum_all(dot, red);
+        if (threadIdx.x == 0) lz.partA[blockIdx.x] = p;
+    }
+}
+
+// Generic kernel for short columns (d_up < 128: several columns per workgroup, element-
+// strided) and for ELL widths above the compiled ones.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+                                                        int64_t ld, LzArgs<T> lz)
+{
+    extern __shared__ double smem_d[];
+    T *col = reinterpret_cast<T *>(smem_d);
+    __shared__ double red[kBlock / 64 + 1];
+    const int dup = a.dup, ncfg = a.ncfg;
+    const int c0 = blockIdx.x * a.cpb;
+    const int ncol = min(a.cpb, a.ddn - c0);
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    const T *__restrict__ prev = nullptr;
+    double sj = 1.0, cprev = 0.0;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_prologue(lz, red, j, sj, cprev)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+        if (j > 1) prev = lz.slot(j - 1);
+    } else {
+        x = xin + (int64_t)blockIdx.y * ld;
+        y = yout + (int64_t)blockIdx.y * ld;
+    }
+    const int base = c0 * dup;
+    const int nel = ncol * dup;
+    for (int e = threadIdx.x; e < nel; e += kBlock) col[e] = x[base + e];
+    __syncthreads();
+    double dot = 0.0;
     for (int e = threadIdx.x; e < nel; e += kBlock) {
-        const int cl = ONECOL ? 0 : e / dup;
-        const int r = ONECOL ? e : e - cl * dup;
-        const int ido = c0 + cl;
+        const int cl = e / dup, r = e - cl * dup, ido = c0 + cl;
         const T *cv = col + cl * dup;
         const int cu = a.imp_up[r], cd = a.imp_dn[ido];
         const T xv = cv[r];
         T acc = (a.diag_up[r] + a.diag_dn[ido] + a.uimp[cu * ncfg + cd]) * xv;
         for (int k = 0; k < a.kup; ++k) acc += a.ell_val_up[k * dup + r] * cv[a.ell_rk_up[k * dup + r]];
-        const int p0 = a.ptr_dn[ido], p1 = a.ptr_dn[ido + 1];
-        for (int k = p0; k < p1; ++k) acc += a.off_dn[k] * x[r + (int64_t)a.rk_dn[k] * dup];
+        for (int k = a.ptr_dn[ido]; k < a.ptr_dn[ido + 1]; ++k) acc += a.off_dn[k] * x[r + a.rk_dn[k] * dup];
         if (a.hund) {
             const int h0 = a.hund_ptr[cu * ncfg + cd], h1 = a.hund_ptr[cu * ncfg + cd + 1];
             for (int t = h0; t < h1; ++t)
-                acc += a.hund_val[t] *
-                       x[a.swap_up[r * ncfg + a.hund_cu[t]] + (int64_t)a.swap_dn[ido * ncfg + a.hund_cd[t]] * dup];
+                acc += a.hund_val[t] * x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido * ncfg + a.hund_cd[t]] * dup];
         }
-        const int64_t idx = base + e;
+        const int idx = base + e;
         if (MODE == 1) {
             double yv = (double)acc * sj;
             if (j > 1) yv -= cprev * (double)prev[idx];
@@ -157,116 +247,18 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvArgs<T> a, const T *__restrict_
         }
     }
     if (MODE == 1) {
-        __syncthreads();
-        const double p = block_sum(dot, red);
-        double tot;
-        if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0)
-            lz.st->alpha[j - 1] = tot;
+        const double p = block_sum_all(dot, red);
+        if (threadIdx.x == 0) lz.partA[blockIdx.x] = p;
     }
 }
 
-// Lanczos phase 2: w -= alpha_j v_j, beta_{j+1} = ||w||; advances j when `advance`.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
-{
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
-    if (lz.st->status) return;
-    const int j = lz.st->j;
-    T *__restrict__ y = lz.slot(j + 1);
-    const T *__restrict__ x = lz.slot(j);
-    const double c = lz.st->alpha[j - 1] / lz.st->beta[j - 1];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const T v = (T)((double)y[i] - c * (double)x[i]);
-        y[i] = v;
-        acc += (double)v * (double)v;
-    }
-    const double p = block_sum(acc, red);
-    double tot;
-    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
-        const double b = sqrt(tot);
-        lz.st->beta[j] = b;
-        if (advance) {
-            if (b < kMinLanczosNorm) lz.st->status = 1;
-            lz.st->j = j + 1;
-        }
-    }
-}
+// ------------------------------------------------------------- Lanczos kernels
 
-// Full reorthogonalisation, classical Gram-Schmidt pass 1/2: coef[k] = w . v_k, k <= j.
-// One workgroup per slice of the vector; partials [k][block]; the last workgroup
-// reduces them in a fixed order.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *coef)
-{
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
-    if (lz.st->status) return;
-    const int j = lz.st->j;
-    const T *w = lz.slot(j + 1);
-    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
-    for (int k = 1; k <= j; ++k) {
-        const T *v = lz.slot(k);
-        double acc = 0.0;
-        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += (double)w[i] * (double)v[i];
-        acc = block_sum(acc, red);
-        if (threadIdx.x == 0) lz.partials[(int64_t)(k - 1) * gridDim.x + blockIdx.x] = acc / lz.st->beta[k - 1];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned t = atomicAdd(&lz.st->ticket, 1u);
-        flag = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!flag) return;
-    __threadfence();
-    const volatile double *vp = lz.partials;
-    for (int k = 1 + threadIdx.x; k <= j; k += kBlock) {
-        double s = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) s += vp[(int64_t)(k - 1) * gridDim.x + b];
-        coef[k - 1] = s;
-    }
-    if (threadIdx.x == 0) lz.st->ticket = 0u;
-}
-
-// w -= sum_k coef[k] v_k; beta_{j+1} = ||w||; advances j when `advance`.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
-{
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
-    if (lz.st->status) return;
-    const int j = lz.st->j;
-    T *w = lz.slot(j + 1);
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        double v = (double)w[i];
-        for (int k = 1; k <= j; ++k) v -= coef[k - 1] / lz.st->beta[k - 1] * (double)lz.slot(k)[i];
-        const T vt = (T)v;
-        w[i] = vt;
-        acc += (double)vt * (double)vt;
-    }
-    const double p = block_sum(acc, red);
-    double tot;
-    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
-        const double b = sqrt(tot);
-        lz.st->beta[j] = b;
-        if (advance) {
-            if (b < kMinLanczosNorm) lz.st->status = 1;
-            lz.st->j = j + 1;
-        }
-    }
-}
-
-// u_1 = start (converted to T), beta_1 = ||u_1||, j = 1.
+// u_1 = start (converted to T); partial sums of ||u_1||^2; j = 1.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *start, int64_t n)
 {
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
+    __shared__ double red[kBlock / 64 + 1];
     T *u = lz.slot(1);
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -274,14 +266,106 @@ __global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *
         u[i] = v;
         acc += (double)v * (double)v;
     }
-    const double p = block_sum(acc, red);
-    double tot;
-    if (last_block_total(p, lz.partials, &lz.st->ticket, red, &flag, &tot) && threadIdx.x == 0) {
-        const double b = sqrt(tot);
-        lz.st->beta[0] = b;
-        lz.st->j = 1;
-        lz.st->status = (b < kMinLanczosNorm) ? 2 : 0;
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
+    if (block0()) {
+        lz.st->j_k1 = 1;
+        lz.st->j_k2 = 0;
+        lz.st->status = 0;
     }
+}
+
+// Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
+{
+    __shared__ double red[kBlock / 64 + 1];
+    if (lz.st->status) return;
+    const int j = lz.st->j_k2;
+    const double alpha = sum_partials(lz.partA, lz.nA, red);
+    if (block0()) lz.st->alpha[j - 1] = alpha;
+    T *__restrict__ y = lz.slot(j + 1);
+    const T *__restrict__ x = lz.slot(j);
+    const double c = alpha / lz.st->beta[j - 1];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const T v = (T)((double)y[i] - c * (double)x[i]);
+        y[i] = v;
+        acc += (double)v * (double)v;
+    }
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
+    if (advance && block0()) lz.st->j_k1 = j + 1;
+}
+
+// End of a chunk: beta_{j} of the next step from the update partials, so the host sees
+// it (the next H.v phase recomputes the same value from the same partials).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
+{
+    __shared__ double red[kBlock / 64 + 1];
+    if (lz.st->status) return;
+    const int j = lz.st->j_k1;
+    const double b = sqrt(sum_partials(lz.partB, lz.nB, red));
+    if (threadIdx.x == 0) {
+        lz.st->beta[j - 1] = b;
+        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+    }
+}
+
+// Full reorthogonalisation (classical Gram-Schmidt, applied twice): partial dots of
+// w = u_{j+1} with every kept u_k, k <= j.  One workgroup per slice of the vectors.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *partR)
+{
+    __shared__ double red[kBlock / 64 + 1];
+    if (lz.st->status) return;
+    const int j = lz.st->j_k2;
+    const T *w = lz.slot(j + 1);
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+    for (int k = 1; k <= j; ++k) {
+        const T *v = lz.slot(k);
+        double acc = 0.0;
+        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += (double)w[i] * (double)v[i];
+        acc = block_sum_all(acc, red);
+        if (threadIdx.x == 0) partR[(int64_t)(k - 1) * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+// coef[k-1] = (w . u_k) / beta_k^2, one workgroup, fixed summation order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reorth_coef(LzArgs<T> lz, const double *partR, int nr, double *coef)
+{
+    if (lz.st->status) return;
+    const int j = lz.st->j_k2;
+    for (int k = 1 + threadIdx.x; k <= j; k += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nr; ++b) s += partR[(int64_t)(k - 1) * nr + b];
+        const double bk = lz.st->beta[k - 1];
+        coef[k - 1] = s / (bk * bk);
+    }
+}
+
+// w -= sum_k coef[k] u_k; partial ||w||^2; advances j when `advance`.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
+{
+    __shared__ double red[kBlock / 64 + 1];
+    if (lz.st->status) return;
+    const int j = lz.st->j_k2;
+    T *w = lz.slot(j + 1);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double v = (double)w[i];
+        for (int k = 1; k <= j; ++k) v -= coef[k - 1] * (double)lz.slot(k)[i];
+        const T vt = (T)v;
+        w[i] = vt;
+        acc += (double)vt * (double)vt;
+    }
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
+    if (advance && block0()) lz.st->j_k1 = j + 1;
 }
 
 // psi = sum_{k<m} coef[k] u_{k+1} / beta_{k+1} (FP64 output).
@@ -295,18 +379,22 @@ __global__ __launch_bounds__(kBlock) void k_combine(LzArgs<T> lz, int m, const d
     }
 }
 
-// out[0] = x . y  (deterministic; ticket/partials are the caller's scratch).
-__global__ __launch_bounds__(kBlock) void k_dot(const double *x, const double *y, int64_t n, double *partials,
-                                                unsigned *ticket, double *out)
+// Two-kernel deterministic dot product: partial sums, then one workgroup sums them.
+__global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const double *y, int64_t n, double *partials)
 {
-    __shared__ double red[kBlock / 64];
-    __shared__ int flag;
+    __shared__ double red[kBlock / 64 + 1];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         acc += x[i] * y[i];
-    const double p = block_sum(acc, red);
-    double tot;
-    if (last_block_total(p, partials, ticket, red, &flag, &tot) && threadIdx.x == 0) *out = tot;
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_part(const double *partials, int n, double *out)
+{
+    __shared__ double red[kBlock / 64 + 1];
+    const double s = sum_partials(partials, n, red);
+    if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void k_scale(double *x, int64_t n, const double *nrm2)
@@ -331,51 +419,81 @@ __global__ void k_start_vector(int64_t n, double *out)
 
 // ------------------------------------------------------------------ host side
 
-HvArgs<double> hv_args_f64(const ed_sector *s)
-{
-    HvArgs<double> a;
-    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kmax;
-    a.cpb = s->up.d >= kBlock ? 1 : kBlock / std::max(1, s->up.d);
-    a.hund = s->hund;
-    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = s->up.ell_val;
-    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = s->dn.off;
-    a.diag_up = s->up.diag; a.diag_dn = s->dn.diag;
-    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
-    a.uimp = s->d_uimp;
-    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd; a.hund_val = s->d_hund_val;
-    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
-    return a;
-}
-
-HvArgs<float> hv_args_f32(const ed_sector *s)
-{
-    HvArgs<float> a;
-    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kmax;
-    a.cpb = s->up.d >= kBlock ? 1 : kBlock / std::max(1, s->up.d);
-    a.hund = s->hund;
-    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = s->up.ell_val32;
-    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = s->dn.off32;
-    a.diag_up = s->up.diag32; a.diag_dn = s->dn.diag32;
-    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
-    a.uimp = s->d_uimp32;
-    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd; a.hund_val = s->d_hund_val32;
-    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
-    return a;
-}
-
 template <typename T>
-static void hv_launch(const HvArgs<T> &a, int mode, const T *x, T *y, int64_t ld, int nvec, const LzArgs<T> &lz,
-                      hipStream_t st)
+static HvArgs<T> hv_args(const ed_sector *s, int cpb_rows)
 {
-    const int nblk = (a.ddn + a.cpb - 1) / a.cpb;
-    const size_t lds = (size_t)a.cpb * a.dup * sizeof(T);
-    dim3 grid(nblk, mode ? 1 : nvec);
-    if (a.cpb == 1) {
-        if (mode) hipLaunchKernelGGL((k_hv<T, 1, true>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
-        else hipLaunchKernelGGL((k_hv<T, 0, true>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+    const bool f64 = sizeof(T) == sizeof(double);
+    HvArgs<T> a;
+    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kpad;
+    a.cpb = cpb_rows;
+    a.hund = s->hund;
+    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = (const T *)(f64 ? (void *)s->up.ell_val : (void *)s->up.ell_val32);
+    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = (const T *)(f64 ? (void *)s->dn.off : (void *)s->dn.off32);
+    a.diag_up = (const T *)(f64 ? (void *)s->up.diag : (void *)s->up.diag32);
+    a.diag_dn = (const T *)(f64 ? (void *)s->dn.diag : (void *)s->dn.diag32);
+    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
+    a.uimp = (const T *)(f64 ? (void *)s->d_uimp : (void *)s->d_uimp32);
+    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd;
+    a.hund_val = (const T *)(f64 ? (void *)s->d_hund_val : (void *)s->d_hund_val32);
+    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    return a;
+}
+
+HvArgs<double> hv_args_f64(const ed_sector *s) { return hv_args<double>(s, 1); }
+HvArgs<float> hv_args_f32(const ed_sector *s) { return hv_args<float>(s, 1); }
+
+// Launch shape of one H.v: which kernel, columns per workgroup, grid.
+struct HvShape {
+    bool rows;   // row-owner kernel (else generic)
+    int cpb;
+    int nblk;
+};
+
+static HvShape hv_shape(const ed_sector *s, int cpb_pref)
+{
+    HvShape h;
+    const int dup = s->up.d, ddn = s->dn.d;
+    h.rows = dup >= 128 && s->up.kpad <= 32;
+    if (h.rows) {
+        h.cpb = (cpb_pref == 2 || cpb_pref == 4) ? cpb_pref : 1;
+        // LDS holds CPB FP64 columns; stay within 64 KB
+        while (h.cpb > 1 && (size_t)h.cpb * dup * sizeof(double) > 65536) h.cpb /= 2;
     } else {
-        if (mode) hipLaunchKernelGGL((k_hv<T, 1, false>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
-        else hipLaunchKernelGGL((k_hv<T, 0, false>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+        h.cpb = dup >= kBlock ? 1 : std::max(1, kBlock / std::max(1, dup));
+    }
+    h.nblk = (ddn + h.cpb - 1) / h.cpb;
+    return h;
+}
+
+template <typename T, int MODE, int CPB>
+static void launch_rows_ku(const HvArgs<T> &a, int ku, dim3 grid, size_t lds, hipStream_t st, const T *x, T *y,
+                           int64_t ld, const LzArgs<T> &lz)
+{
+    switch (ku) {
+    case 4: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 4>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    case 8: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 8>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    case 12: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 12>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    case 16: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 16>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    case 24: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 24>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    default: hipLaunchKernelGGL((k_hv_rows<T, MODE, CPB, 32>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz); break;
+    }
+}
+
+template <typename T, int MODE>
+static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *y, int64_t ld, int nvec,
+                           const LzArgs<T> &lz, hipStream_t st)
+{
+    HvArgs<T> a = hv_args<T>(s, h.cpb);
+    dim3 grid(h.nblk, MODE ? 1 : nvec);
+    const size_t lds = (size_t)h.cpb * a.dup * sizeof(T);
+    if (!h.rows) {
+        hipLaunchKernelGGL((k_hv_generic<T, MODE>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
+        return;
+    }
+    switch (h.cpb) {
+    case 2: launch_rows_ku<T, MODE, 2>(a, a.kup, grid, lds, st, x, y, ld, lz); break;
+    case 4: launch_rows_ku<T, MODE, 4>(a, a.kup, grid, lds, st, x, y, ld, lz); break;
+    default: launch_rows_ku<T, MODE, 1>(a, a.kup, grid, lds, st, x, y, ld, lz); break;
     }
 }
 
@@ -387,12 +505,13 @@ static int elem_grid(int64_t n)
 
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st)
 {
+    const HvShape h = hv_shape(s, s->cpb);
     if (prec == ED_PREC_F64) {
         LzArgs<double> lz{};
-        hv_launch(hv_args_f64(s), 0, (const double *)x, (double *)y, ld, nvec, lz, st);
+        hv_launch_mode<double, 0>(s, h, (const double *)x, (double *)y, ld, nvec, lz, st);
     } else {
         LzArgs<float> lz{};
-        hv_launch(hv_args_f32(s), 0, (const float *)x, (float *)y, ld, nvec, lz, st);
+        hv_launch_mode<float, 0>(s, h, (const float *)x, (float *)y, ld, nvec, lz, st);
     }
     ED_HIP_CHECK(hipGetLastError());
     return ED_OK;
@@ -410,7 +529,8 @@ static int ensure_state(ed_sector *s)
     if (!s->d_state) {
         ED_HIP_CHECK(hipMalloc((void **)&s->d_state, sizeof(LzState)));
         ED_HIP_CHECK(hipMemset(s->d_state, 0, sizeof(LzState)));
-        ED_HIP_CHECK(hipMalloc((void **)&s->d_partials, sizeof(double) * (size_t)kMaxIter * 2048));
+        ED_HIP_CHECK(hipMalloc((void **)&s->d_partials, sizeof(double) * (size_t)kScratchDoubles));
+        ED_HIP_CHECK(hipMemset(s->d_partials, 0, sizeof(double) * (size_t)kScratchDoubles));
     }
     return ED_OK;
 }
@@ -424,12 +544,22 @@ static int ensure_io(ed_sector *s)
     return ED_OK;
 }
 
-// Krylov basis with nslots vectors of precision prec (leading dimension padded to 64).
+static void drop_graphs(ed_sector *s)
+{
+    for (GraphCache &G : s->graphs)
+        if (G.exec) {
+            (void)hipGraphExecDestroy(G.exec);
+            (void)hipGraphDestroy(G.graph);
+            G = GraphCache();
+        }
+}
+
+// Krylov basis with nslots vectors (leading dimension padded to 64 elements), sized for
+// FP64 so that FP32 and FP64 runs share one allocation.
 static int ensure_basis(ed_sector *s, int prec, int nslots)
 {
-    const size_t esz = sizeof(double);   // sized for FP64 so FP32 / FP64 runs share one allocation
     const int64_t ld = (s->dim + 63) / 64 * 64;
-    const size_t need = esz * (size_t)ld * nslots;
+    const size_t need = sizeof(double) * (size_t)ld * nslots;
     if (s->d_basis && s->basis_bytes >= need) {
         s->basis_prec = prec;
         s->basis_nslots = nslots;
@@ -439,12 +569,7 @@ static int ensure_basis(ed_sector *s, int prec, int nslots)
     if (s->d_basis) (void)hipFree(s->d_basis);
     s->d_basis = nullptr;
     s->basis_bytes = 0;
-    for (GraphCache &G : s->graphs)
-        if (G.exec) {
-            (void)hipGraphExecDestroy(G.exec);
-            (void)hipGraphDestroy(G.graph);
-            G = GraphCache();
-        }
+    drop_graphs(s);
     hipError_t e = hipMalloc(&s->d_basis, need);
     if (e != hipSuccess)
         return fail(ED_ERR_NOMEM, "Krylov basis of " + std::to_string(need) + " bytes: " + hipGetErrorString(e));
@@ -463,38 +588,44 @@ static LzArgs<T> lz_args(ed_sector *s)
     lz.ld = s->ld;
     lz.nslots = s->basis_nslots;
     lz.st = s->d_state;
-    lz.partials = s->d_partials;
+    lz.partA = s->d_partials + kPartA;
+    lz.partB = s->d_partials + kPartB;
+    lz.nA = hv_shape(s, s->cpb).nblk;
+    lz.nB = elem_grid(s->dim);
     return lz;
 }
 
-// Device scratch for reorthogonalisation coefficients lives after the partials.
-static double *reorth_coef(ed_sector *s) { return s->d_partials + (size_t)(kMaxIter - 1) * 2048; }
-
 template <typename T>
-static void launch_steps_impl(ed_sector *s, const HvArgs<T> &a, int nsteps, int reorth, hipStream_t st)
+static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t st)
 {
     const LzArgs<T> lz = lz_args<T>(s);
+    const HvShape h = hv_shape(s, s->cpb);
     const int64_t n = s->dim;
     const int g = elem_grid(n);
-    const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + kBlock * 16 - 1) / (kBlock * 16)));
+    const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(kReorthGrid, (n + kBlock * 16 - 1) / (kBlock * 16)));
+    double *partR = s->d_partials + kPartR, *coef = s->d_partials + kCoef;
     for (int i = 0; i < nsteps; ++i) {
-        hv_launch(a, 1, (const T *)nullptr, (T *)nullptr, 0, 1, lz, st);
+        hv_launch_mode<T, 1>(s, h, (const T *)nullptr, (T *)nullptr, 0, 1, lz, st);
         hipLaunchKernelGGL(k_lz_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, reorth ? 0 : 1);
         if (reorth) {
-            double *coef = reorth_coef(s);
-            hipLaunchKernelGGL(k_reorth_dots<T>, dim3(gr), dim3(kBlock), 0, st, lz, n, coef);
-            hipLaunchKernelGGL(k_reorth_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, (const double *)coef, 0);
-            hipLaunchKernelGGL(k_reorth_dots<T>, dim3(gr), dim3(kBlock), 0, st, lz, n, coef);
-            hipLaunchKernelGGL(k_reorth_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, (const double *)coef, 1);
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL(k_reorth_dots<T>, dim3(gr), dim3(kBlock), 0, st, lz, n, partR);
+                hipLaunchKernelGGL(k_reorth_coef<T>, dim3(1), dim3(kBlock), 0, st, lz, (const double *)partR, gr, coef);
+                hipLaunchKernelGGL(k_reorth_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, (const double *)coef,
+                                   pass == 1 ? 1 : 0);
+            }
         }
     }
+    hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
 }
 
 static void run_steps(ed_sector *s, int prec, int nsteps, int reorth, hipStream_t st)
 {
-    if (prec == ED_PREC_F64) launch_steps_impl<double>(s, hv_args_f64(s), nsteps, reorth, st);
-    else launch_steps_impl<float>(s, hv_args_f32(s), nsteps, reorth, st);
+    if (prec == ED_PREC_F64) launch_steps_impl<double>(s, nsteps, reorth, st);
+    else launch_steps_impl<float>(s, nsteps, reorth, st);
 }
+
+static int tune(ed_sector *s);
 
 // Runs nsteps Lanczos steps; chunks of exactly `chunk` steps replay a captured graph.
 static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
@@ -529,6 +660,7 @@ static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
 static int lz_start(ed_sector *s, int prec, const double *h_start, const double *d_start)
 {
     hipStream_t st = s->stream;
+    if (!s->cpb) ED_TRY(tune(s));
     const double *src = d_start;
     if (!src) {
         ED_TRY(ensure_io(s));
@@ -552,14 +684,14 @@ struct HostLz {
     std::vector<double> alpha, beta;
 };
 
+// Copies j (next step), status, alpha[0..upto), beta[0..upto] to the host.
 static int read_state(ed_sector *s, HostLz &h, int upto)
 {
-    // copies j, status, alpha[0..upto), beta[0..upto]
     LzState *d = s->d_state;
-    int hdr[2];
+    int hdr[3];
     h.alpha.resize(std::max(upto, 1));
     h.beta.resize(upto + 1);
-    ED_HIP_CHECK(hipMemcpyAsync(hdr, d, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipMemcpyAsync(hdr, d, 3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     if (upto > 0)
         ED_HIP_CHECK(hipMemcpyAsync(h.alpha.data(), (char *)d + offsetof(LzState, alpha), sizeof(double) * upto,
                                     hipMemcpyDeviceToHost, s->stream));
@@ -567,7 +699,7 @@ static int read_state(ed_sector *s, HostLz &h, int upto)
                                 hipMemcpyDeviceToHost, s->stream));
     ED_HIP_CHECK(hipStreamSynchronize(s->stream));
     h.j = hdr[0];
-    h.status = hdr[1];
+    h.status = hdr[2];
     return ED_OK;
 }
 
@@ -576,6 +708,44 @@ static int check_common(ed_sector *s, int prec)
     if (!s) return fail(ED_ERR_ARG, "NULL sector");
     if (prec != ED_PREC_F64 && prec != ED_PREC_F32) return fail(ED_ERR_ARG, "precision must be ED_PREC_F64 or ED_PREC_F32");
     ED_HIP_CHECK(hipSetDevice(s->device));
+    return ED_OK;
+}
+
+// Autotuner: times the FP64 H.v for 1, 2 and 4 columns per workgroup (3 launches each
+// after one warm-up, HIP events on the sector's stream) and keeps the fastest.
+static int tune(ed_sector *s)
+{
+    ED_TRY(ensure_io(s));
+    hipStream_t st = s->stream;
+    const HvShape h1 = hv_shape(s, 1);
+    if (!h1.rows || s->dim < 4096) {
+        s->cpb = 1;
+        return ED_OK;
+    }
+    ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
+    float best = 1e30f;
+    int best_c = 1;
+    const int cands[3] = {1, 2, 4};
+    for (int i = 0; i < 3; ++i) {
+        const HvShape h = hv_shape(s, cands[i]);
+        if (h.cpb != cands[i]) continue;
+        LzArgs<double> lz{};
+        hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
+        ED_HIP_CHECK(hipEventRecord(s->ev0, st));
+        for (int r = 0; r < 3; ++r) hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
+        ED_HIP_CHECK(hipEventRecord(s->ev1, st));
+        ED_HIP_CHECK(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        ED_HIP_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->tune_ms[i] = ms / 3;
+        if (ms < best) {
+            best = ms;
+            best_c = cands[i];
+        }
+    }
+    ED_HIP_CHECK(hipGetLastError());
+    s->cpb = best_c;
+    drop_graphs(s);
     return ED_OK;
 }
 
@@ -697,17 +867,27 @@ static int lanczos_run(ed_sector *s, int niter, const double *h_start, const dou
     HostLz h;
     ED_TRY(read_state(s, h, niter));
     if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
-    int reached = h.j - 1;   // completed steps
+    const int reached = h.j - 1;   // completed steps
     if (breakdown_at) *breakdown_at = (h.status == 1) ? reached : 0;
     for (int i = 0; i < niter; ++i) {
         if (diag) diag[i] = i < reached ? h.alpha[i] : 0.0;
         if (subdiag) subdiag[i] = i <= reached ? h.beta[i] : 0.0;
     }
     s->basis_valid_iters = reached;
+    s->basis_prec = prec;
     if (h.status == 1 && reached < niter)
         return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(reached + 1) +
                                           " (Lanczos_plain.h:85-92)");
     return ED_OK;
+}
+
+// x.y into *out (device scalar), deterministic.
+static void dot_dev(ed_sector *s, const double *x, const double *y, int64_t n, double *out, hipStream_t st)
+{
+    const int g = elem_grid(n);
+    double *part = s->d_partials + kPartR;   // free outside a reorthogonalised step
+    hipLaunchKernelGGL(k_dot_part, dim3(g), dim3(kBlock), 0, st, x, y, n, part);
+    hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(kBlock), 0, st, (const double *)part, g, out);
 }
 
 static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_psi)
@@ -718,7 +898,7 @@ static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_
         return fail(ED_ERR_STATE, "ed_lanczos_gs_vector: no Krylov basis of " + std::to_string(iters) +
                                       " vectors kept on this sector (run a Lanczos call first)");
     hipStream_t st = s->stream;
-    double *d_coef = s->d_partials + (size_t)(kMaxIter - 2) * 2048;
+    double *d_coef = s->d_partials + kMisc;
     ED_HIP_CHECK(hipMemcpyAsync(d_coef, vecp, sizeof(double) * iters, hipMemcpyHostToDevice, st));
     const int g = elem_grid(s->dim);
     if (s->basis_prec == ED_PREC_F64)
@@ -728,8 +908,7 @@ static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_
         hipLaunchKernelGGL(k_combine<float>, dim3(g), dim3(kBlock), 0, st, lz_args<float>(s), iters,
                            (const double *)d_coef, d_psi, s->dim);
     double *d_nrm = d_coef + iters;
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)d_psi, (const double *)d_psi, s->dim,
-                       s->d_partials, &s->d_state->ticket, d_nrm);
+    dot_dev(s, d_psi, d_psi, s->dim, d_nrm, st);
     hipLaunchKernelGGL(k_scale, dim3(g), dim3(kBlock), 0, st, d_psi, s->dim, (const double *)d_nrm);
     ED_HIP_CHECK(hipGetLastError());
     return ED_OK;
@@ -781,6 +960,16 @@ int ed_hmult_f64(ed_sector *sec, int64_t n, double *vec_out, const double *vec_i
     ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
     ED_HIP_CHECK(hipMemcpyAsync(vec_out, sec->d_io_b, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
+int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
+{
+    ED_TRY(check_common(sec, 0));
+    ED_TRY(tune(sec));
+    if (cpb) *cpb = sec->cpb;
+    if (ms_per_choice)
+        for (int i = 0; i < 3; ++i) ms_per_choice[i] = sec->tune_ms[i];
     return ED_OK;
 }
 
@@ -841,12 +1030,9 @@ int ed_rayleigh_f64(ed_sector *sec, int64_t n, const double *psi, double *energy
     hipStream_t st = sec->stream;
     ED_HIP_CHECK(hipMemcpyAsync(sec->d_io_a, psi, sizeof(double) * n, hipMemcpyHostToDevice, st));
     ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
-    double *d_r = sec->d_partials + (size_t)(kMaxIter - 3) * 2048;
-    const int g = elem_grid(n);
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_a, (const double *)sec->d_io_b,
-                       n, sec->d_partials, &sec->d_state->ticket, d_r);
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_a, (const double *)sec->d_io_a,
-                       n, sec->d_partials, &sec->d_state->ticket, d_r + 1);
+    double *d_r = sec->d_partials + kMisc + kMaxIter;
+    dot_dev(sec, sec->d_io_a, sec->d_io_b, n, d_r, st);
+    dot_dev(sec, sec->d_io_a, sec->d_io_a, n, d_r + 1, st);
     double h[2];
     ED_HIP_CHECK(hipMemcpyAsync(h, d_r, sizeof(h), hipMemcpyDeviceToHost, st));
     ED_HIP_CHECK(hipStreamSynchronize(st));
@@ -868,16 +1054,17 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     for (auto &e : ev) ED_HIP_CHECK(hipEventCreate(&e));
     const int64_t n = sec->dim;
     const int g = elem_grid(n);
+    const HvShape h = hv_shape(sec, sec->cpb);
     for (int r = 0; r < reps; ++r) {
         ED_HIP_CHECK(hipEventRecord(ev[2 * r], st));
         if (precision == ED_PREC_F64) {
             const LzArgs<double> lz = lz_args<double>(sec);
-            hv_launch(hv_args_f64(sec), 1, (const double *)nullptr, (double *)nullptr, 0, 1, lz, st);
+            hv_launch_mode<double, 1>(sec, h, (const double *)nullptr, (double *)nullptr, 0, 1, lz, st);
             ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
             hipLaunchKernelGGL(k_lz_update<double>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
         } else {
             const LzArgs<float> lz = lz_args<float>(sec);
-            hv_launch(hv_args_f32(sec), 1, (const float *)nullptr, (float *)nullptr, 0, 1, lz, st);
+            hv_launch_mode<float, 1>(sec, h, (const float *)nullptr, (float *)nullptr, 0, 1, lz, st);
             ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
             hipLaunchKernelGGL(k_lz_update<float>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
         }

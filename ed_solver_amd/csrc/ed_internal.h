@@ -23,34 +23,46 @@ int fail(int code, const std::string &msg);
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
-constexpr int kMaxPartials = 4096;   // per-block partial sums of one reduction
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
+
+// Layout of the per-sector scratch array of doubles (offsets in doubles).
+constexpr int kPartA = 0;                          // partial sums written by the H.v phase
+constexpr int kPartB = kPartA + 8192;              // partial sums written by the update phase
+constexpr int kPartR = kPartB + 8192;              // reorthogonalisation dots [k][workgroup]
+constexpr int kReorthGrid = 256;                   // workgroups of the reorthogonalisation dots
+constexpr int kCoef = kPartR + kMaxIter * kReorthGrid;   // reorthogonalisation coefficients
+constexpr int kMisc = kCoef + kMaxIter;            // Ritz coefficients, norms, Rayleigh quotient
+constexpr int kScratchDoubles = kMisc + kMaxIter + 64;
 
 // ------------------------------------------------------------ device state
 // Recurrence scalars of one Lanczos run, resident on the device so that a chunk of
 // iterations runs (and can be replayed from a captured graph) without the host.
+// The iteration index is double-buffered between the two phases of a step so that no
+// kernel ever writes a word that its own workgroups read: the H.v phase reads j_k1
+// and writes j_k2, the update phase reads j_k2 and writes j_k1.
 struct LzState {
-    int j;             // current iteration (1-based): the next kernel pair computes v_{j+1}
+    int j_k1;          // step index seen by the H.v phase (1-based)
+    int j_k2;          // step index seen by the update phase
     int status;        // 0 ok, 1 zero-norm vector appeared, 2 zero-norm start vector
-    unsigned ticket;   // last-block-done counter of the running reduction (reset by its owner)
     int pad;
-    double alpha[kMaxIter];      // alpha_j       -> alpha[j-1]
-    double beta[kMaxIter + 1];   // beta_1=||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
+    double alpha[kMaxIter];      // alpha_j        -> alpha[j-1]
+    double beta[kMaxIter + 1];   // beta_1 = ||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
 };
 
 // One spin's tables on the device (tab_HAIM2 output plus derived layouts).
 struct SpinTables {
     int d = 0;              // spin-sector dimension
     int64_t nnz = 0;        // tabulated off-diagonal entries
-    int kmax = 0;           // max entries per state (ELL width)
+    int kmax = 0;           // max entries per state
+    int kpad = 0;           // ELL width (kmax rounded up to a compiled width)
     int32_t *states = nullptr;   // [d]
     double *diag = nullptr;      // [d]
     int32_t *noff = nullptr;     // [d]
     int32_t *ptr = nullptr;      // [d+1] exclusive prefix sum of noff
     int32_t *rk = nullptr;       // [nnz] 0-based target rank (CSR order = reference order)
     double *off = nullptr;       // [nnz]
-    int32_t *ell_rk = nullptr;   // [kmax][d], padded with the row itself
-    double *ell_val = nullptr;   // [kmax][d], padded with 0
+    int32_t *ell_rk = nullptr;   // [kpad][d], padded with the row itself
+    double *ell_val = nullptr;   // [kpad][d], padded with 0
     int32_t *imp = nullptr;      // [d] impurity bits (state >> nb)
     int32_t *swap = nullptr;     // [d][ncfg] rank of the state with its impurity bits replaced
     // FP32 copies for the mixed-precision kernels
@@ -74,14 +86,16 @@ struct HvArgs {
     const int32_t *swap_up, *swap_dn;
 };
 
-// Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1.
+// Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1, and
+// where the per-workgroup partial sums of the two phases go.
 template <typename T>
 struct LzArgs {
     T *base;
     int64_t ld;
     int nslots;
     LzState *st;
-    double *partials;
+    double *partA, *partB;
+    int nA, nB;        // workgroups that wrote partA (H.v phase) / partB (update phase)
     __host__ __device__ T *slot(int j) const { return base + (int64_t)((j - 1) % nslots) * ld; }
 };
 
@@ -119,6 +133,8 @@ struct ed_sector {
     double *d_io_a = nullptr, *d_io_b = nullptr;   // host-API staging vectors [dim]
     ed::GraphCache graphs[4];      // captured Lanczos chunks, key = precision * 2 + reorth
     int last_f32_iters = 0;
+    int cpb = 0;                   // H.v columns per workgroup chosen by the autotuner (0 = not tuned)
+    double tune_ms[3] = {0, 0, 0}; // measured FP64 H.v time for 1, 2, 4 columns per workgroup
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double hv_ms_sum = 0.0;

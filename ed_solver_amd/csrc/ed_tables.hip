@@ -262,10 +262,16 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
     hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
                        (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off);
     ED_HIP_CHECK(hipGetLastError());
-    const int kw = std::max(T.kmax, 1);
+    // ELL width: the next compiled width of the unrolled H.v kernel, or kmax itself
+    // above the largest one (the generic kernel then loops over it)
+    const int widths[] = {4, 8, 12, 16, 24, 32};
+    T.kpad = std::max(T.kmax, 1);
+    for (int w : widths)
+        if (T.kmax <= w) { T.kpad = w; break; }
+    const int kw = T.kpad;
     ED_TRY(dalloc(&T.ell_rk, (size_t)kw * d));
     ED_TRY(dalloc(&T.ell_val, (size_t)kw * d));
-    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, T.kmax, T.ptr, T.rk, T.off, T.ell_rk,
+    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk,
                        T.ell_val);
     ED_TRY(dalloc(&T.imp, d));
     ED_TRY(dalloc(&T.swap, (size_t)d * S->ncfg));
@@ -405,8 +411,8 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
     if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create: NULL argument");
     *out = nullptr;
     const int nc = m->nc, nb = m->nb, ns = nc + nb;
-    if (nc < 1 || nb < 0 || ns > 15)
-        return fail(ED_ERR_ARG, "ed_sector_create: need nc >= 1, nb >= 0, nc + nb <= 15");
+    if (nc < 1 || nc > 6 || nb < 0 || ns > 15)
+        return fail(ED_ERR_ARG, "ed_sector_create: need 1 <= nc <= 6, nb >= 0, nc + nb <= 15");
     if (nup < 0 || nup > ns || ndn < 0 || ndn > ns)
         return fail(ED_ERR_ARG, "ed_sector_create: particle numbers outside [0, Ns]");
     if (!m->ec || !m->u || !m->jj || (nb && (!m->eb || !m->vbc)))

@@ -129,6 +129,13 @@ class Sector:
         check(lib().ed_hmult_block_f64_dev(self.handle, self.dim, nvec, ld, ctypes.c_void_p(out_ptr),
                                            ctypes.c_void_p(in_ptr), ctypes.c_void_p(stream) if stream else None))
 
+    def tune(self) -> tuple[int, list]:
+        """Runs the H.v schedule autotuner; returns (columns per workgroup, ms per choice)."""
+        c = ctypes.c_int()
+        ms = np.zeros(3)
+        check(lib().ed_hmult_tune(self.handle, ctypes.byref(c), dptr(ms)))
+        return c.value, ms.tolist()
+
     def hmult_benchmark(self, precision="f64", reps: int = 50) -> tuple[float, float]:
         ms, by = ctypes.c_double(), ctypes.c_double()
         check(lib().ed_hmult_benchmark(self.handle, _prec(precision), reps, ctypes.byref(ms), ctypes.byref(by)))

@@ -119,6 +119,13 @@ int ed_sector_synchronize(ed_sector *sec);
 int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, double *d_out,
                            const double *d_in, void *stream);
 
+/* Autotuner of the H.v launch schedule (no reference counterpart): times the FP64
+ * H.v with 1, 2 and 4 columns per workgroup on this sector and keeps the fastest for
+ * every later H.v and Lanczos call on it.  *cpb = the choice; ms_per_choice[3] (may be
+ * NULL) = measured times (0 where a choice does not apply).  Lanczos calls run it on
+ * first use. */
+int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice);
+
 /* ------------------------------------------------------------------ Lanczos */
 
 /* Deterministic start vector (period 10000, values k/1000 in (0, 1]) used when a

@@ -85,6 +85,8 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"sector (6,6) dim {sec.dim} built in {t_build*1e3:.1f} ms")
     D = sec.dim
+    sched, sched_ms = sec.tune()
+    log(f"H.v schedule {sched} (ms for row kernel x1/x2/x4, fast kernel: {sched_ms})")
 
     def rate(prec):
         for _ in range(args.warmup):
@@ -154,6 +156,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": None, "kernel": "k_hv<double,1,true> (fused Lanczos H.v)", "avg_us": avg_ms * 1e3,
                 "bytes_per_launch": bytes_l, "fp32_avg_us": avg32 * 1e3,
+                "schedule": sched, "schedule_ms": sched_ms,
                 "fp32_achieved": bytes32 / (avg32 * 1e-3) / 1e9}
 
     cpu = None

@@ -61,6 +61,7 @@ SIGNATURES = {
     "ed_sector_synchronize": (ctypes.c_int, [_vp]),
     "ed_hmult_block_f64_dev": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp]),
     "ed_hmult_tune": (ctypes.c_int, [_vp, _c_int_p, _dp]),
+    "ed_hmult_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ed_start_vector": (ctypes.c_int, [ctypes.c_int64, _dp]),
     "ed_lanczos_default_opts": (ctypes.c_int, [ctypes.POINTER(LanczosOpts)]),
     "ed_lanczos_diagonalize": (ctypes.c_int, [_vp, ctypes.POINTER(LanczosOpts), _dp, _c_int_p, _c_int_p, _dp, _dp,

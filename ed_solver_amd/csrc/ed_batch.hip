@@ -1,0 +1,346 @@
+// Batched excitation Lanczos: the per-(state, orbital, iph) compute_dynamic runs of a
+// Green's function (green_class_computeAA.h:101-302 -> green_class_compute_dynamic.f90:
+// 191-407) advanced together.  Every job (one excitation vector in its own N+-1 sector,
+// sectors of uneven dimension) owns a contiguous range of workgroups in ONE launch per
+// recurrence phase:
+//
+//   k_hv_batch      H.v + beta-subtraction + alpha partials of every job
+//   k_update_batch  alpha-subtraction + beta partials of every job
+//
+// so a Green's function with J excitation runs costs 2 launches per Lanczos step instead
+// of 2J, and small sectors fill the GPU together.  A workgroup finds its job by a scan
+// of the (uniform) job table.  The continued fractions of all jobs and frequencies run
+// in one more launch (k_cfrac).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ed_kernels.h"
+
+namespace ed {
+
+int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, double *diag, double *subdiag,
+                    int *breakdown_at);
+int cfrac_batch_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
+                    const double *d_scale, double *d_out, hipStream_t st);
+
+#define ED_TRY(x)                     \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != ED_OK) return rc_; \
+    } while (0)
+
+template <typename T>
+struct JobDesc {
+    HvArgs<T> a;
+    LzArgs<T> lz;
+    const double *start;   // normalised excitation vector (device, FP64)
+    int64_t n;
+    int blk0_hv, nblk_hv;  // workgroups of this job in the batched H.v grid
+    int blk0_el, nblk_el;  // workgroups of this job in the batched element-wise grid
+};
+
+template <typename T>
+__device__ inline int find_job_hv(const JobDesc<T> *jobs, int njobs)
+{
+    int j = 0;
+    while (j + 1 < njobs && (int)blockIdx.x >= jobs[j + 1].blk0_hv) ++j;
+    return j;
+}
+
+template <typename T>
+__device__ inline int find_job_el(const JobDesc<T> *jobs, int njobs)
+{
+    int j = 0;
+    while (j + 1 < njobs && (int)blockIdx.x >= jobs[j + 1].blk0_el) ++j;
+    return j;
+}
+
+template <typename T, int KU>
+__global__ __launch_bounds__(kBlock) void k_hv_batch(const JobDesc<T> *jobs, int njobs)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double red[kRed];
+    const int j = find_job_hv(jobs, njobs);
+    const HvArgs<T> a = jobs[j].a;
+    const LzArgs<T> lz = jobs[j].lz;
+    hv_rows_body<T, 1, 1, KU, true>(a, (const T *)nullptr, (T *)nullptr, lz, blockIdx.x - jobs[j].blk0_hv,
+                                    reinterpret_cast<T *>(smem_d), red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_batch(const JobDesc<T> *jobs, int njobs)
+{
+    __shared__ double red[kRed];
+    const int j = find_job_el(jobs, njobs);
+    const LzArgs<T> lz = jobs[j].lz;
+    lz_update_body(lz, jobs[j].n, 1, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_init_batch(const JobDesc<T> *jobs, int njobs)
+{
+    __shared__ double red[kRed];
+    const int j = find_job_el(jobs, njobs);
+    const LzArgs<T> lz = jobs[j].lz;
+    lz_init_body(lz, jobs[j].start, jobs[j].n, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finish_batch(const JobDesc<T> *jobs)
+{
+    __shared__ double red[kRed];
+    const LzArgs<T> lz = jobs[blockIdx.x].lz;
+    lz_finish_body(lz, red);
+}
+
+// phi_k / ||phi_k|| for every job into its start area.
+__global__ void k_scale_into(const double *x, double *y, int64_t n, double s)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = s * x[i];
+}
+
+static int egrid(int64_t n)
+{
+    int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+// Grow-only device work space of the batched runs (one per process; calls serialise).
+struct BatchWS {
+    std::mutex mu;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+};
+static BatchWS g_ws;
+
+static int ws_reserve(size_t bytes, int device)
+{
+    if (g_ws.ptr && g_ws.bytes >= bytes && g_ws.device == device) return ED_OK;
+    if (g_ws.ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_ws.ptr);
+        g_ws.ptr = nullptr;
+        g_ws.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&g_ws.ptr, bytes);
+    if (e != hipSuccess) return fail(ED_ERR_NOMEM, "batched Lanczos work space: " + std::string(hipGetErrorString(e)));
+    g_ws.bytes = bytes;
+    g_ws.device = device;
+    return ED_OK;
+}
+
+static bool rows_capable(const ed_sector *s) { return s->up.d >= 128 && s->up.kpad <= 32; }
+
+template <typename T>
+static HvArgs<T> hv_args_of(const ed_sector *s)
+{
+    const bool f64 = sizeof(T) == sizeof(double);
+    HvArgs<T> a;
+    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kpad; a.cpb = 1; a.hund = s->hund;
+    a.ell_rk_up = s->up.ell_rk; a.ell_val_up = (const T *)(f64 ? (void *)s->up.ell_val : (void *)s->up.ell_val32);
+    a.ptr_dn = s->dn.ptr; a.rk_dn = s->dn.rk; a.off_dn = (const T *)(f64 ? (void *)s->dn.off : (void *)s->dn.off32);
+    a.diag_up = (const T *)(f64 ? (void *)s->up.diag : (void *)s->up.diag32);
+    a.diag_dn = (const T *)(f64 ? (void *)s->dn.diag : (void *)s->dn.diag32);
+    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
+    a.uimp = (const T *)(f64 ? (void *)s->d_uimp : (void *)s->d_uimp32);
+    a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd;
+    a.hund_val = (const T *)(f64 ? (void *)s->d_hund_val : (void *)s->d_hund_val32);
+    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    return a;
+}
+
+template <typename T, int KU>
+static void launch_hv_batch(int total, size_t lds, hipStream_t st, const JobDesc<T> *d_jobs, int nj)
+{
+    hipLaunchKernelGGL((k_hv_batch<T, KU>), dim3(total), dim3(kBlock), lds, st, d_jobs, nj);
+}
+
+// Runs `niter` Lanczos steps for every job together; fills a/b [nj][niter].
+template <typename T>
+static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<const double *> &h_phi,
+                     const std::vector<double> &nrm2, int niter, hipStream_t st, std::vector<double> &a_out,
+                     std::vector<double> &b_out)
+{
+    const int nj = (int)secs.size();
+    // ---- layout of the work space
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    std::vector<size_t> o_basis(nj), o_start(nj), o_part(nj);
+    std::vector<int64_t> ld(nj);
+    const size_t o_state = take(sizeof(LzState) * nj);
+    const size_t o_jobs = take(sizeof(JobDesc<T>) * nj);
+    for (int k = 0; k < nj; ++k) {
+        ld[k] = (secs[k]->dim + 63) / 64 * 64;
+        o_basis[k] = take(sizeof(T) * (size_t)ld[k] * (niter + 1));
+        o_start[k] = take(sizeof(double) * (size_t)secs[k]->dim);
+        o_part[k] = take(sizeof(double) * (8192 + 2048));
+    }
+    ED_TRY(ws_reserve(off, secs[0]->device));
+    char *base = (char *)g_ws.ptr;
+    // ---- job table
+    std::vector<JobDesc<T>> jobs(nj);
+    int blk_hv = 0, blk_el = 0, ku = 4;
+    size_t lds = 0;
+    for (int k = 0; k < nj; ++k) {
+        ed_sector *s = secs[k];
+        JobDesc<T> &J = jobs[k];
+        J.a = hv_args_of<T>(s);
+        J.n = s->dim;
+        J.nblk_hv = s->dn.d;
+        J.nblk_el = egrid(s->dim);
+        J.blk0_hv = blk_hv;
+        J.blk0_el = blk_el;
+        blk_hv += J.nblk_hv;
+        blk_el += J.nblk_el;
+        J.start = (const double *)(base + o_start[k]);
+        J.lz.base = (T *)(base + o_basis[k]);
+        J.lz.ld = ld[k];
+        J.lz.nslots = niter + 1;
+        J.lz.st = (LzState *)(base + o_state) + k;
+        J.lz.partA = (double *)(base + o_part[k]);
+        J.lz.partB = J.lz.partA + 8192;
+        J.lz.nA = J.nblk_hv;
+        J.lz.nB = J.nblk_el;
+        ku = std::max(ku, s->up.kpad);
+        lds = std::max(lds, (size_t)s->up.d * sizeof(T));
+        ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k] + 0, h_phi[k], sizeof(double) * s->dim, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scale_into, dim3(egrid(s->dim)), dim3(kBlock), 0, st, (const double *)(base + o_start[k]),
+                           (double *)(base + o_start[k]), s->dim, 1.0 / std::sqrt(nrm2[k]));
+    }
+    const JobDesc<T> *d_jobs = (const JobDesc<T> *)(base + o_jobs);
+    ED_HIP_CHECK(hipMemcpyAsync(base + o_jobs, jobs.data(), sizeof(JobDesc<T>) * nj, hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemsetAsync(base + o_state, 0, sizeof(LzState) * nj, st));
+    hipLaunchKernelGGL(k_init_batch<T>, dim3(blk_el), dim3(kBlock), 0, st, d_jobs, nj);
+    for (int it = 0; it < niter; ++it) {
+        switch (ku) {
+        case 4: launch_hv_batch<T, 4>(blk_hv, lds, st, d_jobs, nj); break;
+        case 8: launch_hv_batch<T, 8>(blk_hv, lds, st, d_jobs, nj); break;
+        case 12: launch_hv_batch<T, 12>(blk_hv, lds, st, d_jobs, nj); break;
+        case 16: launch_hv_batch<T, 16>(blk_hv, lds, st, d_jobs, nj); break;
+        case 24: launch_hv_batch<T, 24>(blk_hv, lds, st, d_jobs, nj); break;
+        default: launch_hv_batch<T, 32>(blk_hv, lds, st, d_jobs, nj); break;
+        }
+        hipLaunchKernelGGL(k_update_batch<T>, dim3(blk_el), dim3(kBlock), 0, st, d_jobs, nj);
+    }
+    hipLaunchKernelGGL(k_finish_batch<T>, dim3(nj), dim3(kBlock), 0, st, d_jobs);
+    ED_HIP_CHECK(hipGetLastError());
+    std::vector<LzState> hs(nj);
+    ED_HIP_CHECK(hipMemcpyAsync(hs.data(), base + o_state, sizeof(LzState) * nj, hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    a_out.assign((size_t)nj * niter, 0.0);
+    b_out.assign((size_t)nj * niter, 0.0);
+    for (int k = 0; k < nj; ++k) {
+        const LzState &h = hs[k];
+        const int reached = h.j_k1 - 1;
+        if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "batched Lanczos: zero-norm start vector in job " + std::to_string(k));
+        if (h.status == 1 && reached < niter)
+            return fail(ED_ERR_BREAKDOWN, "batched Lanczos: zero-norm vector at iteration " + std::to_string(reached + 1) +
+                                              " of job " + std::to_string(k) + " (Lanczos_plain.h:85-92)");
+        for (int i = 0; i < niter; ++i) {
+            a_out[(size_t)k * niter + i] = h.alpha[i];
+            b_out[(size_t)k * niter + i] = h.beta[i];
+        }
+    }
+    return ED_OK;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int nitergreen,
+                          double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn, double *a,
+                          double *b);
+
+extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
+                                  const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
+                                  const double *wn, double *dyn, double *a, double *b)
+{
+    if (njobs < 1 || !secs || !phis || !iph || !energy || !wn || !dyn || nw < 1 || nitergreen < 1)
+        return fail(ED_ERR_ARG, "ed_dynamic_batched: bad argument");
+    if (precision != ED_PREC_F64 && precision != ED_PREC_F32) return fail(ED_ERR_ARG, "ed_dynamic_batched: bad precision");
+    std::lock_guard<std::mutex> lock(g_ws.mu);
+    for (int k = 0; k < njobs; ++k) {
+        if (!secs[k] || !phis[k]) return fail(ED_ERR_ARG, "ed_dynamic_batched: NULL sector or vector");
+        if (iph[k] != 1 && iph[k] != 2) return fail(ED_ERR_ARG, "ed_dynamic_batched: iph must be 1 or 2");
+        if (secs[k]->device != secs[0]->device) return fail(ED_ERR_ARG, "ed_dynamic_batched: sectors on different devices");
+    }
+    ED_HIP_CHECK(hipSetDevice(secs[0]->device));
+    std::fill(dyn, dyn + 2 * (size_t)njobs * nw, 0.0);
+    if (a) std::fill(a, a + (size_t)njobs * nitergreen, 0.0);
+    if (b) std::fill(b, b + (size_t)njobs * nitergreen, 0.0);
+    // Group: all jobs with the same Lanczos length that the batched kernel can take run
+    // together; the others (short columns, very wide tables) run one by one.
+    std::vector<double> nrm2(njobs, 0.0);
+    for (int k = 0; k < njobs; ++k) {
+        double s = 0.0;
+        for (int64_t i = 0; i < secs[k]->dim; ++i) s += phis[k][i] * phis[k][i];
+        nrm2[k] = s;
+    }
+    std::vector<int> batch;
+    for (int k = 0; k < njobs; ++k) {
+        if (nrm2[k] < cutoff_dynamic) continue;   // dyn = 0 (green_class_compute_dynamic.f90:263-265)
+        const int niter = (int)std::min<int64_t>(nitergreen, secs[k]->dim);
+        if (rows_capable(secs[k]) && niter == nitergreen) {
+            batch.push_back(k);
+        } else {
+            int rc = ed_dynamic(secs[k], iph[k], energy[k], phis[k], nitergreen, cutoff_dynamic, precision, nw, wn,
+                                dyn + (size_t)k * 2 * nw, a ? a + (size_t)k * nitergreen : nullptr,
+                                b ? b + (size_t)k * nitergreen : nullptr);
+            if (rc != ED_OK) return rc;
+        }
+    }
+    if (batch.empty()) return ED_OK;
+    const int nb = (int)batch.size();
+    std::vector<ed_sector *> bs(nb);
+    std::vector<const double *> bp(nb);
+    std::vector<double> bn(nb);
+    for (int i = 0; i < nb; ++i) {
+        bs[i] = secs[batch[i]];
+        bp[i] = phis[batch[i]];
+        bn[i] = nrm2[batch[i]];
+    }
+    hipStream_t st = bs[0]->stream;
+    const int niter = nitergreen;
+    std::vector<double> ab, bb;
+    int rc = precision == ED_PREC_F64 ? run_batch<double>(bs, bp, bn, niter, st, ab, bb)
+                                      : run_batch<float>(bs, bp, bn, niter, st, ab, bb);
+    if (rc != ED_OK) return rc;
+    // shifted tridiagonals: ph (alpha - E) (green_class_compute_dynamic.f90:310-311)
+    std::vector<double> sa((size_t)nb * niter), scale(nb);
+    for (int i = 0; i < nb; ++i) {
+        const int k = batch[i];
+        const double ph = iph[k] == 1 ? 1.0 : -1.0;
+        for (int t = 0; t < niter; ++t) sa[(size_t)i * niter + t] = ph * (ab[(size_t)i * niter + t] - energy[k]);
+        scale[i] = nrm2[k];
+        if (a) std::copy(ab.begin() + (size_t)i * niter, ab.begin() + (size_t)(i + 1) * niter, a + (size_t)k * nitergreen);
+        if (b) std::copy(bb.begin() + (size_t)i * niter, bb.begin() + (size_t)(i + 1) * niter, b + (size_t)k * nitergreen);
+    }
+    double *d = nullptr;
+    const size_t tot = 2 * (size_t)nb * niter + nb + nw + 2 * (size_t)nb * nw;
+    ED_HIP_CHECK(hipMallocAsync((void **)&d, sizeof(double) * tot, st));
+    double *d_a = d, *d_b = d + (size_t)nb * niter, *d_sc = d_b + (size_t)nb * niter, *d_wn = d_sc + nb,
+           *d_out = d_wn + nw;
+    ED_HIP_CHECK(hipMemcpyAsync(d_a, sa.data(), sizeof(double) * nb * niter, hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(d_b, bb.data(), sizeof(double) * nb * niter, hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(d_sc, scale.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(d_wn, wn, sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    rc = cfrac_batch_dev(niter, d_a, d_b, niter, nb, nw, d_wn, d_sc, d_out, st);
+    std::vector<double> out(2 * (size_t)nb * nw);
+    if (rc == ED_OK) {
+        ED_HIP_CHECK(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < nb; ++i)
+            std::copy(out.begin() + (size_t)i * 2 * nw, out.begin() + (size_t)(i + 1) * 2 * nw,
+                      dyn + (size_t)batch[i] * 2 * nw);
+    }
+    (void)hipFreeAsync(d, st);
+    (void)hipStreamSynchronize(st);
+    return rc;
+}

@@ -32,7 +32,7 @@
 #include <string>
 #include <vector>
 
-#include "ed_internal.h"
+#include "ed_kernels.h"
 
 namespace ed {
 
@@ -42,63 +42,7 @@ namespace ed {
         if (rc_ != ED_OK) return rc_; \
     } while (0)
 
-// ----------------------------------------------------------- reductions
-
-__device__ inline double wave_sum(double v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Block sum in a fixed order, returned to every thread.  `red` holds blockDim/64 + 1.
-__device__ inline double block_sum_all(double v, double *red)
-{
-    v = wave_sum(v);
-    const int nw = blockDim.x >> 6;
-    __syncthreads();   // red may still be read from a previous call
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < nw; ++i) t += red[i];
-        red[nw] = t;
-    }
-    __syncthreads();
-    return red[nw];
-}
-
-// Sum of n partials in a fixed order (thread-strided, then the block tree); every
-// workgroup that calls it on the same data obtains the same value.
-__device__ inline double sum_partials(const double *p, int n, double *red)
-{
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
-    return block_sum_all(v, red);
-}
-
 __device__ inline bool block0() { return blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0; }
-
-// Prologue of a Lanczos H.v phase: beta_j from the update phase's partials, breakdown
-// test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
-template <typename T>
-__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, int &j, double &sj, double &cprev)
-{
-    if (lz.st->status) return false;
-    j = lz.st->j_k1;
-    const double b2 = sum_partials(lz.partB, lz.nB, red);
-    const double bj = sqrt(b2);
-    if (bj < kMinLanczosNorm) {
-        if (block0()) lz.st->status = (j == 1) ? 2 : 1;
-        return false;
-    }
-    if (block0()) {
-        lz.st->beta[j - 1] = bj;
-        lz.st->j_k2 = j;
-    }
-    sj = 1.0 / bj;
-    cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
-    return true;
-}
 
 // ------------------------------------------------------------------ H.v kernels
 
@@ -110,85 +54,22 @@ __global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__rest
                                                      int64_t ld, LzArgs<T> lz)
 {
     extern __shared__ double smem_d[];
-    T *col = reinterpret_cast<T *>(smem_d);
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_rows_body<T, MODE, CPB, KU, false>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
+                                          reinterpret_cast<T *>(smem_d), red);
+}
 
-    const int dup = a.dup, ncfg = a.ncfg;
-    const int c0 = blockIdx.x * CPB;
-    const T *__restrict__ x;
-    T *__restrict__ y;
-    const T *__restrict__ prev = nullptr;
-    double sj = 1.0, cprev = 0.0;
-    int j = 0;
-    if (MODE == 1) {
-        if (!lz_prologue(lz, red, j, sj, cprev)) return;
-        x = lz.slot(j);
-        y = lz.slot(j + 1);
-        if (j > 1) prev = lz.slot(j - 1);
-    } else {
-        x = xin + (int64_t)blockIdx.y * ld;
-        y = yout + (int64_t)blockIdx.y * ld;
-    }
-    // per-column uniform data
-    int ido[CPB], cd[CPB], p0[CPB], p1[CPB];
-    T ddg[CPB];
-    bool ok[CPB];
-#pragma unroll
-    for (int c = 0; c < CPB; ++c) {
-        ok[c] = c0 + c < a.ddn;
-        ido[c] = ok[c] ? c0 + c : c0;
-        cd[c] = a.imp_dn[ido[c]];
-        p0[c] = a.ptr_dn[ido[c]];
-        p1[c] = ok[c] ? a.ptr_dn[ido[c] + 1] : p0[c];
-        ddg[c] = a.diag_dn[ido[c]];
-    }
-    const int ncol = min(CPB, a.ddn - c0);
-    const T *xc = x + (int64_t)c0 * dup;
-    for (int e = threadIdx.x; e < ncol * dup; e += kBlock) col[e] = xc[e];
-    __syncthreads();
-
-    double dot = 0.0;
-    for (int r = threadIdx.x; r < dup; r += kBlock) {
-        int rk[KU];
-        T v[KU];
-#pragma unroll
-        for (int k = 0; k < KU; ++k) {
-            rk[k] = a.ell_rk_up[k * dup + r];
-            v[k] = a.ell_val_up[k * dup + r];
-        }
-        const T du = a.diag_up[r];
-        const int cu = a.imp_up[r];
-#pragma unroll
-        for (int c = 0; c < CPB; ++c) {
-            if (!ok[c]) continue;
-            const T *cv = col + c * dup;
-            const T xv = cv[r];
-            T acc = (du + ddg[c] + a.uimp[cu * ncfg + cd[c]]) * xv;
-#pragma unroll
-            for (int k = 0; k < KU; ++k) acc += v[k] * cv[rk[k]];
-#pragma unroll 4
-            for (int k = p0[c]; k < p1[c]; ++k) acc += a.off_dn[k] * x[r + a.rk_dn[k] * dup];
-            if (a.hund) {
-                const int h0 = a.hund_ptr[cu * ncfg + cd[c]], h1 = a.hund_ptr[cu * ncfg + cd[c] + 1];
-                for (int t = h0; t < h1; ++t)
-                    acc += a.hund_val[t] * x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido[c] * ncfg + a.hund_cd[t]] * dup];
-            }
-            const int idx = (c0 + c) * dup + r;
-            if (MODE == 1) {
-                double yv = (double)acc * sj;
-                if (j > 1) yv -= cprev * (double)prev[idx];
-                const T yt = (T)yv;
-                y[idx] = yt;
-                dot += (double)yt * ((double)xv * sj);
-            } else {
-                y[idx] = acc;
-            }
-        }
-    }
-    if (MODE == 1) {
-        const double p = block_sum_all(dot, red);
-        if (threadIdx.x == 0) lz.partA[blockIdx.x] = p;
-    }
+// Fast kernel (one column per workgroup, blockDim.x up to 1024), see hv_fast_body.
+template <typename T, int MODE, int K>
+__global__ __launch_bounds__(1024) void k_hv_fast(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+                                                   int64_t ld, LzArgs<T> lz)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_fast_body<T, MODE, K>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
+                             reinterpret_cast<unsigned char *>(smem_d), red);
 }
 
 // Generic kernel for short columns (d_up < 128: several columns per workgroup, element-
@@ -199,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
 {
     extern __shared__ double smem_d[];
     T *col = reinterpret_cast<T *>(smem_d);
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
     const int dup = a.dup, ncfg = a.ncfg;
     const int c0 = blockIdx.x * a.cpb;
     const int ncol = min(a.cpb, a.ddn - c0);
@@ -209,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, j, sj, cprev)) return;
+        if (!lz_prologue(lz, red, block0(), j, sj, cprev)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
         if (j > 1) prev = lz.slot(j - 1);
@@ -254,63 +135,25 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
 
 // ------------------------------------------------------------- Lanczos kernels
 
-// u_1 = start (converted to T); partial sums of ||u_1||^2; j = 1.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *start, int64_t n)
 {
-    __shared__ double red[kBlock / 64 + 1];
-    T *u = lz.slot(1);
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const T v = (T)start[i];
-        u[i] = v;
-        acc += (double)v * (double)v;
-    }
-    const double p = block_sum_all(acc, red);
-    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
-    if (block0()) {
-        lz.st->j_k1 = 1;
-        lz.st->j_k2 = 0;
-        lz.st->status = 0;
-    }
+    __shared__ double red[kRed];
+    lz_init_body(lz, start, n, blockIdx.x, gridDim.x, red);
 }
 
-// Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
 {
-    __shared__ double red[kBlock / 64 + 1];
-    if (lz.st->status) return;
-    const int j = lz.st->j_k2;
-    const double alpha = sum_partials(lz.partA, lz.nA, red);
-    if (block0()) lz.st->alpha[j - 1] = alpha;
-    T *__restrict__ y = lz.slot(j + 1);
-    const T *__restrict__ x = lz.slot(j);
-    const double c = alpha / lz.st->beta[j - 1];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const T v = (T)((double)y[i] - c * (double)x[i]);
-        y[i] = v;
-        acc += (double)v * (double)v;
-    }
-    const double p = block_sum_all(acc, red);
-    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
-    if (advance && block0()) lz.st->j_k1 = j + 1;
+    __shared__ double red[kRed];
+    lz_update_body(lz, n, advance, blockIdx.x, gridDim.x, red);
 }
 
-// End of a chunk: beta_{j} of the next step from the update partials, so the host sees
-// it (the next H.v phase recomputes the same value from the same partials).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
 {
-    __shared__ double red[kBlock / 64 + 1];
-    if (lz.st->status) return;
-    const int j = lz.st->j_k1;
-    const double b = sqrt(sum_partials(lz.partB, lz.nB, red));
-    if (threadIdx.x == 0) {
-        lz.st->beta[j - 1] = b;
-        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
-    }
+    __shared__ double red[kRed];
+    lz_finish_body(lz, red);
 }
 
 // Full reorthogonalisation (classical Gram-Schmidt, applied twice): partial dots of
@@ -318,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *partR)
 {
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     const T *w = lz.slot(j + 1);
@@ -351,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_reorth_coef(LzArgs<T> lz, const doub
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
 {
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *w = lz.slot(j + 1);
@@ -382,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(LzArgs<T> lz, int m, const d
 // Two-kernel deterministic dot product: partial sums, then one workgroup sums them.
 __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const double *y, int64_t n, double *partials)
 {
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         acc += x[i] * y[i];
@@ -392,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const doub
 
 __global__ __launch_bounds__(kBlock) void k_sum_part(const double *partials, int n, double *out)
 {
-    __shared__ double red[kBlock / 64 + 1];
+    __shared__ double red[kRed];
     const double s = sum_partials(partials, n, red);
     if (threadIdx.x == 0) *out = s;
 }
@@ -436,33 +279,85 @@ static HvArgs<T> hv_args(const ed_sector *s, int cpb_rows)
     a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd;
     a.hund_val = (const T *)(f64 ? (void *)s->d_hund_val : (void *)s->d_hund_val32);
     a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    a.ell_pk_up = s->up.ell_pk;
+    a.coef_up = (const T *)(f64 ? (void *)s->up.coef : (void *)s->up.coef32);
+    a.ncoef_up = s->up.ncoef;
+    a.kdn = s->dn.kpad;
+    a.nhund = s->nhund;
+    a.hund_tr = s->d_hund_tr;
     return a;
 }
 
 HvArgs<double> hv_args_f64(const ed_sector *s) { return hv_args<double>(s, 1); }
 HvArgs<float> hv_args_f32(const ed_sector *s) { return hv_args<float>(s, 1); }
 
-// Launch shape of one H.v: which kernel, columns per workgroup, grid.
+// Launch shape of one H.v: which kernel, columns per workgroup, threads, grid.
 struct HvShape {
-    bool rows;   // row-owner kernel (else generic)
+    int kind;    // 0 generic, 1 row kernel, 2 fast kernel
     int cpb;
     int nblk;
+    int nt;      // threads per workgroup
+    int k;       // compiled table width
+    size_t lds;
 };
 
-static HvShape hv_shape(const ed_sector *s, int cpb_pref)
+static int width_of(int kmax)
+{
+    const int widths[] = {4, 8, 12, 16, 24, 32};
+    for (int w : widths)
+        if (kmax <= w) return w;
+    return 0;
+}
+
+static bool fast_ok(const ed_sector *s)
+{
+    const int k = width_of(std::max(s->up.kpad, s->dn.kpad));
+    // widths above 16 exceed the 128-VGPR budget of a 1024-thread workgroup in FP64
+    return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 &&
+           hv_fast_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(double)) <= 60 * 1024;
+}
+
+// sched: 1, 2, 4 = row kernel with that many columns per workgroup; 8 = fast kernel.
+static HvShape hv_shape(const ed_sector *s, int sched)
 {
     HvShape h;
     const int dup = s->up.d, ddn = s->dn.d;
-    h.rows = dup >= 128 && s->up.kpad <= 32;
-    if (h.rows) {
-        h.cpb = (cpb_pref == 2 || cpb_pref == 4) ? cpb_pref : 1;
+    h.nt = kBlock;
+    h.k = s->up.kpad;
+    if (sched == 8 && fast_ok(s)) {
+        h.kind = 2;
+        h.cpb = 1;
+        h.nt = std::min(1024, std::max(64, (dup + 63) / 64 * 64));
+        h.k = width_of(std::max(s->up.kpad, s->dn.kpad));
+        h.nblk = ddn;
+        h.lds = 0;   // set per precision at launch
+        return h;
+    }
+    h.kind = (dup >= 128 && s->up.kpad <= 32) ? 1 : 0;
+    if (h.kind == 1) {
+        h.cpb = (sched == 2 || sched == 4) ? sched : 1;
         // LDS holds CPB FP64 columns; stay within 64 KB
         while (h.cpb > 1 && (size_t)h.cpb * dup * sizeof(double) > 65536) h.cpb /= 2;
     } else {
         h.cpb = dup >= kBlock ? 1 : std::max(1, kBlock / std::max(1, dup));
     }
     h.nblk = (ddn + h.cpb - 1) / h.cpb;
+    h.lds = 0;
     return h;
+}
+
+template <typename T, int MODE>
+static void launch_fast(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x, T *y,
+                        int64_t ld, const LzArgs<T> &lz)
+{
+    switch (k) {
+    case 4: hipLaunchKernelGGL((k_hv_fast<T, MODE, 4>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 8: hipLaunchKernelGGL((k_hv_fast<T, MODE, 8>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 12: hipLaunchKernelGGL((k_hv_fast<T, MODE, 12>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 16: hipLaunchKernelGGL((k_hv_fast<T, MODE, 16>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 24: hipLaunchKernelGGL((k_hv_fast<T, MODE, 24>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    default: hipLaunchKernelGGL((k_hv_fast<T, MODE, 32>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    }
 }
 
 template <typename T, int MODE, int CPB>
@@ -485,8 +380,13 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
 {
     HvArgs<T> a = hv_args<T>(s, h.cpb);
     dim3 grid(h.nblk, MODE ? 1 : nvec);
+    if (h.kind == 2) {
+        const size_t lds = hv_fast_lds(a.dup, a.ncoef_up, a.ncfg, a.nhund, sizeof(T));
+        launch_fast<T, MODE>(a, h.k, grid, h.nt, lds, st, x, y, ld, lz);
+        return;
+    }
     const size_t lds = (size_t)h.cpb * a.dup * sizeof(T);
-    if (!h.rows) {
+    if (h.kind == 0) {
         hipLaunchKernelGGL((k_hv_generic<T, MODE>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
         return;
     }
@@ -711,24 +611,26 @@ static int check_common(ed_sector *s, int prec)
     return ED_OK;
 }
 
-// Autotuner: times the FP64 H.v for 1, 2 and 4 columns per workgroup (3 launches each
-// after one warm-up, HIP events on the sector's stream) and keeps the fastest.
+// Autotuner: times the FP64 H.v under every applicable schedule (row kernel with 1, 2
+// or 4 columns per workgroup; fast kernel), 3 launches each after one warm-up, HIP
+// events on the sector's stream, and keeps the fastest.
 static int tune(ed_sector *s)
 {
     ED_TRY(ensure_io(s));
     hipStream_t st = s->stream;
-    const HvShape h1 = hv_shape(s, 1);
-    if (!h1.rows || s->dim < 4096) {
-        s->cpb = 1;
+    for (double &t : s->tune_ms) t = 0.0;
+    if (s->dim < 4096) {
+        s->cpb = fast_ok(s) ? 8 : 1;
         return ED_OK;
     }
     ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
     float best = 1e30f;
     int best_c = 1;
-    const int cands[3] = {1, 2, 4};
-    for (int i = 0; i < 3; ++i) {
+    const int cands[4] = {1, 2, 4, 8};
+    for (int i = 0; i < 4; ++i) {
         const HvShape h = hv_shape(s, cands[i]);
-        if (h.cpb != cands[i]) continue;
+        const bool applies = (cands[i] == 8) ? h.kind == 2 : (h.kind == 1 && h.cpb == cands[i]) || (h.kind == 0 && i == 0);
+        if (!applies) continue;
         LzArgs<double> lz{};
         hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
         ED_HIP_CHECK(hipEventRecord(s->ev0, st));
@@ -969,7 +871,18 @@ int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
     ED_TRY(tune(sec));
     if (cpb) *cpb = sec->cpb;
     if (ms_per_choice)
-        for (int i = 0; i < 3; ++i) ms_per_choice[i] = sec->tune_ms[i];
+        for (int i = 0; i < 4; ++i) ms_per_choice[i] = sec->tune_ms[i];
+    return ED_OK;
+}
+
+int ed_hmult_set_schedule(ed_sector *sec, int choice)
+{
+    ED_TRY(check_common(sec, 0));
+    if (choice != 1 && choice != 2 && choice != 4 && choice != 8)
+        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4 or 8");
+    if (choice == 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");
+    sec->cpb = choice;
+    drop_graphs(sec);
     return ED_OK;
 }
 

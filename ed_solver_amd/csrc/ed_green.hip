@@ -8,8 +8,7 @@
 //   k_cfrac              invert_zmtridiag (matrices_tridiag.f90:163-198), one thread
 //                        per Matsubara frequency (and per job when batched).
 //   ed_dynamic           compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407).
-//   ed_dynamic_batched   the per-(state, orbital, iph) calls of compute_dynamic made
-//                        by green_class_computeAA.h:101-302, advanced together.
+//   (the batched form of ed_dynamic lives in ed_batch.hip)
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -146,8 +145,8 @@ int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, 
     return ED_OK;
 }
 
-static int cfrac_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
-                     const double *d_scale, double *d_out, hipStream_t st)
+int cfrac_batch_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
+                    const double *d_scale, double *d_out, hipStream_t st)
 {
     int *d_err = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_err, sizeof(int), st));
@@ -198,7 +197,7 @@ static int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_p
     ED_HIP_CHECK(hipMemcpyAsync(d_ab + niter, b.data(), sizeof(double) * niter, hipMemcpyHostToDevice, st));
     // dyn = -fermion_sign * CF * ||phi||^2 with fermion_sign = -1 (:224-230, :324)
     ED_HIP_CHECK(hipMemcpyAsync(d_sc, &nrm2, sizeof(double), hipMemcpyHostToDevice, st));
-    rc = cfrac_dev(niter, d_ab, d_ab + niter, niter, 1, nw, d_wn, d_sc, d_out, st);
+    rc = cfrac_batch_dev(niter, d_ab, d_ab + niter, niter, 1, nw, d_wn, d_sc, d_out, st);
     if (rc == ED_OK) {
         ED_HIP_CHECK(hipMemcpyAsync(dyn, d_out, sizeof(double) * 2 * nw, hipMemcpyDeviceToHost, st));
         ED_HIP_CHECK(hipStreamSynchronize(st));
@@ -244,7 +243,7 @@ int ed_continued_fraction(int n, const double *a, const double *b, int nw, const
     ED_HIP_CHECK(hipMemcpy(d_a, a, sizeof(double) * n, hipMemcpyHostToDevice));
     ED_HIP_CHECK(hipMemcpy(d_b, b, sizeof(double) * n, hipMemcpyHostToDevice));
     ED_HIP_CHECK(hipMemcpy(d_wn, wn, sizeof(double) * nw, hipMemcpyHostToDevice));
-    int rc = cfrac_dev(n, d_a, d_b, n, 1, nw, d_wn, nullptr, d_out, st);
+    int rc = cfrac_batch_dev(n, d_a, d_b, n, 1, nw, d_wn, nullptr, d_out, st);
     if (rc == ED_OK) ED_HIP_CHECK(hipMemcpy(out, d_out, sizeof(double) * 2 * nw, hipMemcpyDeviceToHost));
     (void)hipFree(d);
     return rc;
@@ -269,21 +268,6 @@ int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int ni
     (void)hipFreeAsync(d_wn, st);
     (void)hipStreamSynchronize(st);
     return rc;
-}
-
-int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
-                       const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
-                       const double *wn, double *dyn, double *a, double *b)
-{
-    if (njobs < 1 || !secs || !phis || !iph || !energy || !wn || !dyn)
-        return fail(ED_ERR_ARG, "ed_dynamic_batched: bad argument");
-    for (int k = 0; k < njobs; ++k) {
-        int rc = ed_dynamic(secs[k], iph[k], energy[k], phis[k], nitergreen, cutoff_dynamic, precision, nw, wn,
-                            dyn + (size_t)k * 2 * nw, a ? a + (size_t)k * nitergreen : nullptr,
-                            b ? b + (size_t)k * nitergreen : nullptr);
-        if (rc != ED_OK) return rc;
-    }
-    return ED_OK;
 }
 
 }  // extern "C"

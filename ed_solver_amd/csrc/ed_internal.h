@@ -22,6 +22,7 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kRed = 17;             // reduction scratch: one slot per wave of a 1024-thread workgroup + 1
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
 
@@ -67,6 +68,12 @@ struct SpinTables {
     int32_t *swap = nullptr;     // [d][ncfg] rank of the state with its impurity bits replaced
     // FP32 copies for the mixed-precision kernels
     float *diag32 = nullptr, *off32 = nullptr, *ell_val32 = nullptr;
+    // packed ELL: entry = target rank (low 16 bits) | coefficient index (high 16 bits)
+    // into the table of distinct coefficients coef[ncoef] (coef[0] = 0 pads short rows)
+    uint32_t *ell_pk = nullptr;  // [kpad][d]
+    double *coef = nullptr;
+    float *coef32 = nullptr;
+    int ncoef = 0;
 };
 
 // Kernel-side view of a sector's Hamiltonian for precision T.
@@ -84,6 +91,11 @@ struct HvArgs {
     const int32_t *hund_cu, *hund_cd;
     const T *hund_val;
     const int32_t *swap_up, *swap_dn;
+    // packed layouts of the fast kernel
+    const uint32_t *ell_pk_up;     // [kup][dup]
+    const T *coef_up;              // [ncoef_up]
+    int ncoef_up, kdn, nhund;
+    const int32_t *hund_tr;        // [nhund] cu' | cd' << 8
 };
 
 // Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1, and
@@ -120,7 +132,7 @@ struct ed_sector {
     double *d_uimp = nullptr;
     float *d_uimp32 = nullptr;
     int hund = 0, nhund = 0;
-    int32_t *d_hund_ptr = nullptr, *d_hund_cu = nullptr, *d_hund_cd = nullptr;
+    int32_t *d_hund_ptr = nullptr, *d_hund_cu = nullptr, *d_hund_cd = nullptr, *d_hund_tr = nullptr;
     double *d_hund_val = nullptr;
     float *d_hund_val32 = nullptr;
     // Lanczos work space
@@ -133,8 +145,9 @@ struct ed_sector {
     double *d_io_a = nullptr, *d_io_b = nullptr;   // host-API staging vectors [dim]
     ed::GraphCache graphs[4];      // captured Lanczos chunks, key = precision * 2 + reorth
     int last_f32_iters = 0;
-    int cpb = 0;                   // H.v columns per workgroup chosen by the autotuner (0 = not tuned)
-    double tune_ms[3] = {0, 0, 0}; // measured FP64 H.v time for 1, 2, 4 columns per workgroup
+    int cpb = 0;                   // H.v schedule chosen by the autotuner: 1/2/4 = row kernel with that many
+                                   // columns per workgroup, 8 = fast kernel; 0 = not tuned
+    double tune_ms[4] = {0, 0, 0, 0};   // measured FP64 H.v time per schedule (1, 2, 4, fast)
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double hv_ms_sum = 0.0;

@@ -1,0 +1,335 @@
+// Device-side bodies of the H.v and Lanczos kernels, shared by the single-sector
+// kernels (ed_engine.hip) and the batched ones (ed_batch.hip).  A body gets the
+// index of its workgroup inside its own job (`blk`) and the number of workgroups the
+// job owns (`nblk`); the single-sector wrappers pass blockIdx.x / gridDim.x.
+#pragma once
+
+#include "ed_internal.h"
+
+namespace ed {
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum in a fixed order, returned to every thread.  `red` holds blockDim/64 + 1.
+__device__ inline double block_sum_all(double v, double *red)
+{
+    v = wave_sum(v);
+    const int nw = blockDim.x >> 6;
+    __syncthreads();   // red may still be read from a previous call
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < nw; ++i) t += red[i];
+        red[nw] = t;
+    }
+    __syncthreads();
+    return red[nw];
+}
+
+// Sum of n partials in a fixed order (thread-strided, then the block tree); every
+// workgroup that calls it on the same data obtains the same value.
+__device__ inline double sum_partials(const double *p, int n, double *red)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    return block_sum_all(v, red);
+}
+
+// Prologue of a Lanczos H.v phase: beta_j from the update phase's partials, breakdown
+// test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
+// `lead` is true in exactly one thread of the job (it publishes beta_j and j).
+template <typename T>
+__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, bool lead, int &j, double &sj, double &cprev)
+{
+    if (lz.st->status) return false;
+    j = lz.st->j_k1;
+    const double bj = sqrt(sum_partials(lz.partB, lz.nB, red));
+    if (bj < kMinLanczosNorm) {
+        if (lead) lz.st->status = (j == 1) ? 2 : 1;
+        return false;
+    }
+    if (lead) {
+        lz.st->beta[j - 1] = bj;
+        lz.st->j_k2 = j;
+    }
+    sj = 1.0 / bj;
+    cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
+    return true;
+}
+
+// H.v for CPB columns starting at column blk*CPB, rows owned by threads.
+// MODE 0: y = H x.  MODE 1: Lanczos phase 1 (x, y from the Krylov slots).
+// CLAMP: the ELL width of this job may be below KU (batched kernels); entries k >=
+// a.kup reread entry a.kup-1 with a zero coefficient, so no load leaves the table.
+template <typename T, int MODE, int CPB, int KU, bool CLAMP>
+__device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
+                                    const LzArgs<T> &lz, int blk, T *col, double *red)
+{
+    const int dup = a.dup, ncfg = a.ncfg;
+    const int c0 = blk * CPB;
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    const T *__restrict__ prev = nullptr;
+    double sj = 1.0, cprev = 0.0;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_prologue(lz, red, blk == 0 && threadIdx.x == 0, j, sj, cprev)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+        if (j > 1) prev = lz.slot(j - 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    int ido[CPB], cd[CPB], p0[CPB], p1[CPB];
+    T ddg[CPB];
+    bool ok[CPB];
+#pragma unroll
+    for (int c = 0; c < CPB; ++c) {
+        ok[c] = c0 + c < a.ddn;
+        ido[c] = ok[c] ? c0 + c : c0;
+        cd[c] = a.imp_dn[ido[c]];
+        p0[c] = a.ptr_dn[ido[c]];
+        p1[c] = ok[c] ? a.ptr_dn[ido[c] + 1] : p0[c];
+        ddg[c] = a.diag_dn[ido[c]];
+    }
+    const int ncol = min(CPB, a.ddn - c0);
+    const T *xc = x + (int64_t)c0 * dup;
+    for (int e = threadIdx.x; e < ncol * dup; e += blockDim.x) col[e] = xc[e];
+    __syncthreads();
+
+    double dot = 0.0;
+    for (int r = threadIdx.x; r < dup; r += blockDim.x) {
+        int rk[KU];
+        T v[KU];
+#pragma unroll
+        for (int k = 0; k < KU; ++k) {
+            const int kk = CLAMP ? min(k, a.kup - 1) : k;
+            rk[k] = a.ell_rk_up[kk * dup + r];
+            const T vv = a.ell_val_up[kk * dup + r];
+            v[k] = (CLAMP && k >= a.kup) ? (T)0 : vv;
+        }
+        const T du = a.diag_up[r];
+        const int cu = a.imp_up[r];
+#pragma unroll
+        for (int c = 0; c < CPB; ++c) {
+            if (!ok[c]) continue;
+            const T *cv = col + c * dup;
+            const T xv = cv[r];
+            T acc = (du + ddg[c] + a.uimp[cu * ncfg + cd[c]]) * xv;
+#pragma unroll
+            for (int k = 0; k < KU; ++k) acc += v[k] * cv[rk[k]];
+#pragma unroll 4
+            for (int k = p0[c]; k < p1[c]; ++k) acc += a.off_dn[k] * x[r + a.rk_dn[k] * dup];
+            if (a.hund) {
+                const int h0 = a.hund_ptr[cu * ncfg + cd[c]], h1 = a.hund_ptr[cu * ncfg + cd[c] + 1];
+                for (int t = h0; t < h1; ++t)
+                    acc += a.hund_val[t] *
+                           x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido[c] * ncfg + a.hund_cd[t]] * dup];
+            }
+            const int idx = (c0 + c) * dup + r;
+            if (MODE == 1) {
+                double yv = (double)acc * sj;
+                if (j > 1) yv -= cprev * (double)prev[idx];
+                const T yt = (T)yv;
+                y[idx] = yt;
+                dot += (double)yt * ((double)xv * sj);
+            } else {
+                y[idx] = acc;
+            }
+        }
+    }
+    if (MODE == 1) {
+        const double p = block_sum_all(dot, red);
+        if (threadIdx.x == 0) lz.partA[blk] = p;
+    }
+}
+
+// Bytes of dynamic LDS the fast H.v needs (column + coefficient/interaction tables).
+__host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int nhund, size_t tsize)
+{
+    const size_t nq = (size_t)ncfg * ncfg;
+    return tsize * ((size_t)dup + ncoef + nq + nhund) + sizeof(int) * (nq + 1 + nhund + ncfg);
+}
+
+// Fast H.v: one column per workgroup, blockDim.x threads own rows r = tid + i*blockDim.x.
+// All loads that do not depend on the staged column (packed up-hop entries, down-hop
+// neighbour values, diagonal, previous Lanczos vector) are issued before the LDS barrier,
+// so the column staging and the row gathers overlap.  Up-hop coefficients, interaction
+// diagonal and Hund transitions are read from LDS.  K bounds the entries per row of both
+// spin tables.  Dynamic LDS: col[dup] coef[ncoef] uimp[ncfg^2] hval[nhund] (T), then
+// hptr[ncfg^2+1] htr[nhund] swdn[ncfg] (int).
+template <typename T, int MODE, int K>
+__device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
+                                    const LzArgs<T> &lz, int blk, unsigned char *smem, double *red)
+{
+    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund;
+    T *col = reinterpret_cast<T *>(smem);
+    T *coef = col + dup;
+    T *uimp = coef + a.ncoef_up;
+    T *hval = uimp + nq;
+    int *hptr = reinterpret_cast<int *>(hval + nh);
+    int *htr = hptr + nq + 1;
+    int *swdn = htr + nh;
+    const int ido = blk, NT = blockDim.x, tid = threadIdx.x;
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    const T *__restrict__ prev = nullptr;
+    double sj = 1.0, cprev = 0.0;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_prologue(lz, red, blk == 0 && tid == 0, j, sj, cprev)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+        if (j > 1) prev = lz.slot(j - 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int cd = a.imp_dn[ido];
+    const int p0 = a.ptr_dn[ido], nd = a.ptr_dn[ido + 1] - p0;
+    const T ddg = a.diag_dn[ido];
+    T od[K];
+    int jd[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        od[k] = k < nd ? a.off_dn[p0 + k] : (T)0;
+        jd[k] = k < nd ? a.rk_dn[p0 + k] * dup : ido * dup;
+    }
+    const int base = ido * dup;
+
+    // ---- loads of the first row, issued before the column is staged
+    int r = tid;
+    uint32_t pk[K];
+    T xd[K];
+    T du = 0, pv = 0;
+    int cu = 0;
+    if (r < dup) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
+#pragma unroll
+        for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
+        du = a.diag_up[r];
+        cu = a.imp_up[r];
+        if (MODE == 1 && j > 1) pv = prev[base + r];
+    }
+    // ---- stage the column and the small tables in LDS
+    for (int e = tid; e < dup; e += NT) col[e] = x[base + e];
+    for (int e = tid; e < a.ncoef_up; e += NT) coef[e] = a.coef_up[e];
+    for (int e = tid; e < nq; e += NT) uimp[e] = a.uimp[e];
+    if (a.hund) {
+        for (int e = tid; e <= nq; e += NT) hptr[e] = a.hund_ptr[e];
+        for (int e = tid; e < nh; e += NT) {
+            htr[e] = a.hund_tr[e];
+            hval[e] = a.hund_val[e];
+        }
+        for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
+    }
+    __syncthreads();
+
+    double dot = 0.0;
+    while (r < dup) {
+        const T xv = col[r];
+        T acc = (du + ddg + uimp[cu * ncfg + cd]) * xv;
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc += coef[pk[k] >> 16] * col[pk[k] & 0xFFFFu];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc += od[k] * xd[k];
+        if (a.hund) {
+            const int h0 = hptr[cu * ncfg + cd], h1 = hptr[cu * ncfg + cd + 1];
+            for (int t = h0; t < h1; ++t) {
+                const int tr = htr[t];
+                acc += hval[t] * x[a.swap_up[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup];
+            }
+        }
+        const int idx = base + r;
+        if (MODE == 1) {
+            double yv = (double)acc * sj;
+            if (j > 1) yv -= cprev * (double)pv;
+            const T yt = (T)yv;
+            y[idx] = yt;
+            dot += (double)yt * ((double)xv * sj);
+        } else {
+            y[idx] = acc;
+        }
+        // next row of this thread
+        r += NT;
+        if (r < dup) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
+#pragma unroll
+            for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
+            du = a.diag_up[r];
+            cu = a.imp_up[r];
+            if (MODE == 1 && j > 1) pv = prev[base + r];
+        }
+    }
+    if (MODE == 1) {
+        const double p = block_sum_all(dot, red);
+        if (tid == 0) lz.partA[blk] = p;
+    }
+}
+
+// u_1 = start (converted to T); partial sums of ||u_1||^2; j = 1.
+template <typename T>
+__device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, int64_t n, int blk, int nblk,
+                                    double *red)
+{
+    T *u = lz.slot(1);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)nblk * blockDim.x) {
+        const T v = (T)start[i];
+        u[i] = v;
+        acc += (double)v * (double)v;
+    }
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) lz.partB[blk] = p;
+    if (blk == 0 && threadIdx.x == 0) {
+        lz.st->j_k1 = 1;
+        lz.st->j_k2 = 0;
+        lz.st->status = 0;
+    }
+}
+
+// Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
+template <typename T>
+__device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advance, int blk, int nblk, double *red)
+{
+    if (lz.st->status) return;
+    const int j = lz.st->j_k2;
+    const double alpha = sum_partials(lz.partA, lz.nA, red);
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    if (lead) lz.st->alpha[j - 1] = alpha;
+    T *__restrict__ y = lz.slot(j + 1);
+    const T *__restrict__ x = lz.slot(j);
+    const double c = alpha / lz.st->beta[j - 1];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)nblk * blockDim.x) {
+        const T v = (T)((double)y[i] - c * (double)x[i]);
+        y[i] = v;
+        acc += (double)v * (double)v;
+    }
+    const double p = block_sum_all(acc, red);
+    if (threadIdx.x == 0) lz.partB[blk] = p;
+    if (advance && lead) lz.st->j_k1 = j + 1;
+}
+
+// End of a chunk: beta_j of the next step from the update partials (one workgroup).
+template <typename T>
+__device__ inline void lz_finish_body(const LzArgs<T> &lz, double *red)
+{
+    if (lz.st->status) return;
+    const int j = lz.st->j_k1;
+    const double b = sqrt(sum_partials(lz.partB, lz.nB, red));
+    if (threadIdx.x == 0) {
+        lz.st->beta[j - 1] = b;
+        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+    }
+}
+
+}  // namespace ed

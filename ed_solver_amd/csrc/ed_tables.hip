@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ed_internal.h"
@@ -213,10 +214,62 @@ static int dalloc(T **p, size_t n)
 static void free_spin(SpinTables &t)
 {
     void *ps[] = {t.states, t.diag, t.noff, t.ptr, t.rk, t.off, t.ell_rk, t.ell_val, t.imp, t.swap,
-                  t.diag32, t.off32, t.ell_val32};
+                  t.diag32, t.off32, t.ell_val32, t.ell_pk, t.coef, t.coef32};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     t = SpinTables();
+}
+
+// Packed ELL of one spin: entry (k, i) = rank | cidx << 16, where cidx indexes the
+// table of the distinct coefficient values of the spin (index 0 holds 0.0 and pads rows
+// shorter than kpad).  The values are copies of tab_HAIM2's, so the products are the
+// same bits as with the plain ELL.  Built on the host from the device tables.
+static int build_packed(SpinTables &T, hipStream_t st)
+{
+    const int d = T.d, kw = T.kpad;
+    std::vector<int32_t> ptr(d + 1), rk(std::max<int64_t>(1, T.nnz));
+    std::vector<double> off(std::max<int64_t>(1, T.nnz));
+    ED_HIP_CHECK(hipMemcpy(ptr.data(), T.ptr, sizeof(int32_t) * (d + 1), hipMemcpyDeviceToHost));
+    if (T.nnz) {
+        ED_HIP_CHECK(hipMemcpy(rk.data(), T.rk, sizeof(int32_t) * T.nnz, hipMemcpyDeviceToHost));
+        ED_HIP_CHECK(hipMemcpy(off.data(), T.off, sizeof(double) * T.nnz, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> coef(1, 0.0);
+    std::unordered_map<uint64_t, uint32_t> index;
+    index[0] = 0;
+    std::vector<uint32_t> pk((size_t)kw * d);
+    T.ncoef = 0;
+    if (d > 65535) return ED_OK;   // ranks do not fit 16 bits: the packed kernel is not used
+    for (int i = 0; i < d; ++i) {
+        const int n = ptr[i + 1] - ptr[i];
+        for (int k = 0; k < kw; ++k) {
+            uint32_t r = (uint32_t)i, c = 0;
+            if (k < n) {
+                const double v = off[ptr[i] + k];
+                uint64_t b;
+                std::memcpy(&b, &v, sizeof b);
+                auto it = index.find(b);
+                if (it == index.end()) {
+                    if (coef.size() > 65535) return ED_OK;
+                    it = index.emplace(b, (uint32_t)coef.size()).first;
+                    coef.push_back(v);
+                }
+                c = it->second;
+                r = (uint32_t)rk[ptr[i] + k];
+            }
+            pk[(size_t)k * d + i] = r | (c << 16);
+        }
+    }
+    std::vector<float> coef32(coef.begin(), coef.end());
+    ED_TRY(dalloc(&T.ell_pk, pk.size()));
+    ED_TRY(dalloc(&T.coef, coef.size()));
+    ED_TRY(dalloc(&T.coef32, coef.size()));
+    ED_HIP_CHECK(hipMemcpyAsync(T.ell_pk, pk.data(), sizeof(uint32_t) * pk.size(), hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(T.coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(T.coef32, coef32.data(), sizeof(float) * coef32.size(), hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    T.ncoef = (int)coef.size();
+    return ED_OK;
 }
 
 static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
@@ -287,6 +340,7 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
                        (int64_t)kw * d);
     ED_HIP_CHECK(hipGetLastError());
     ED_HIP_CHECK(hipStreamSynchronize(st));
+    ED_TRY(build_packed(T, st));
     (void)hipFree(d_ec);
     (void)hipFree(d_eb);
     (void)hipFree(d_vbc);
@@ -308,7 +362,7 @@ static int build_interaction(ed_sector *S)
             if (a != b && U[a * nc + b] != 0.0) offc = true;
     S->hund = (std::fabs(S->jhund) > 1e-5 && offc) ? 1 : 0;
     std::vector<double> uimp((size_t)ncfg * ncfg, 0.0);
-    std::vector<int32_t> hptr((size_t)ncfg * ncfg + 1, 0), hcu, hcd;
+    std::vector<int32_t> hptr((size_t)ncfg * ncfg + 1, 0), hcu, hcd, htr;
     std::vector<double> hval;
     auto par = [](uint32_t x, int a, int b) {   // parity of bits strictly between a and b
         int lo = std::min(a, b), hi = std::max(a, b);
@@ -368,6 +422,7 @@ static int build_interaction(ed_sector *S)
             hptr[(size_t)cu * ncfg + cd + 1] = (int32_t)hcu.size();
         }
     S->nhund = (int)hcu.size();
+    for (size_t t = 0; t < hcu.size(); ++t) htr.push_back(hcu[t] | (hcd[t] << 8));
     std::vector<float> uimp32(uimp.begin(), uimp.end()), hval32(hval.begin(), hval.end());
     ED_TRY(dalloc(&S->d_uimp, uimp.size()));
     ED_TRY(dalloc(&S->d_uimp32, uimp.size()));
@@ -375,6 +430,7 @@ static int build_interaction(ed_sector *S)
     ED_TRY(dalloc(&S->d_hund_cu, hcu.size()));
     ED_TRY(dalloc(&S->d_hund_cd, hcd.size()));
     ED_TRY(dalloc(&S->d_hund_val, hval.size()));
+    ED_TRY(dalloc(&S->d_hund_tr, htr.size()));
     ED_TRY(dalloc(&S->d_hund_val32, hval.size()));
     ED_HIP_CHECK(hipMemcpy(S->d_uimp, uimp.data(), uimp.size() * sizeof(double), hipMemcpyHostToDevice));
     ED_HIP_CHECK(hipMemcpy(S->d_uimp32, uimp32.data(), uimp32.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -383,6 +439,7 @@ static int build_interaction(ed_sector *S)
         ED_HIP_CHECK(hipMemcpy(S->d_hund_cu, hcu.data(), hcu.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         ED_HIP_CHECK(hipMemcpy(S->d_hund_cd, hcd.data(), hcd.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         ED_HIP_CHECK(hipMemcpy(S->d_hund_val, hval.data(), hval.size() * sizeof(double), hipMemcpyHostToDevice));
+        ED_HIP_CHECK(hipMemcpy(S->d_hund_tr, htr.data(), htr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         ED_HIP_CHECK(hipMemcpy(S->d_hund_val32, hval32.data(), hval32.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     return ED_OK;
@@ -392,7 +449,7 @@ static void free_sector(ed_sector *S)
 {
     free_spin(S->up);
     free_spin(S->dn);
-    void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val,
+    void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
                   S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b};
     for (void *p : ps)
         if (p) (void)hipFree(p);

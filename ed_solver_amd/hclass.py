@@ -130,11 +130,14 @@ class Sector:
                                            ctypes.c_void_p(in_ptr), ctypes.c_void_p(stream) if stream else None))
 
     def tune(self) -> tuple[int, list]:
-        """Runs the H.v schedule autotuner; returns (columns per workgroup, ms per choice)."""
+        """Runs the H.v schedule autotuner; returns (choice 1/2/4 row kernel or 8 fast kernel, ms per choice)."""
         c = ctypes.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         check(lib().ed_hmult_tune(self.handle, ctypes.byref(c), dptr(ms)))
         return c.value, ms.tolist()
+
+    def set_schedule(self, choice: int) -> None:
+        check(lib().ed_hmult_set_schedule(self.handle, int(choice)))
 
     def hmult_benchmark(self, precision="f64", reps: int = 50) -> tuple[float, float]:
         ms, by = ctypes.c_double(), ctypes.c_double()

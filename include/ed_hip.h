@@ -120,11 +120,17 @@ int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, doub
                            const double *d_in, void *stream);
 
 /* Autotuner of the H.v launch schedule (no reference counterpart): times the FP64
- * H.v with 1, 2 and 4 columns per workgroup on this sector and keeps the fastest for
- * every later H.v and Lanczos call on it.  *cpb = the choice; ms_per_choice[3] (may be
- * NULL) = measured times (0 where a choice does not apply).  Lanczos calls run it on
- * first use. */
-int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice);
+ * H.v on this sector under each applicable schedule -- the row kernel with 1, 2 or 4
+ * columns per workgroup, and the fast kernel (one column per workgroup, up to 1024
+ * threads, packed tables in LDS) -- and keeps the fastest for every later H.v and
+ * Lanczos call on the sector.  *choice = 1, 2, 4 (row kernel) or 8 (fast kernel);
+ * ms_per_choice[4] (may be NULL) = measured times in that order (0 where a schedule
+ * does not apply).  Lanczos calls run it on first use. */
+int ed_hmult_tune(ed_sector *sec, int *choice, double *ms_per_choice);
+
+/* Forces one H.v schedule (1, 2, 4 or 8 as above) on this sector, e.g. to test or time
+ * a schedule the autotuner did not pick.  No reference counterpart. */
+int ed_hmult_set_schedule(ed_sector *sec, int choice);
 
 /* ------------------------------------------------------------------ Lanczos */
 

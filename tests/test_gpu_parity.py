@@ -107,6 +107,26 @@ def test_hmult_workloads_fp64(gpu, name):
     s.close()
 
 
+@pytest.mark.parametrize("name", ["B", "C", "E"])
+def test_hmult_every_schedule(gpu, name):
+    """Every H.v schedule the autotuner can pick gives the oracle's H.v (FP64 1e-13)."""
+    m, (nu, nd) = golden(name)
+    s = gpu.Sector(m, nu, nd)
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    v = np.random.default_rng(5).uniform(-1, 1, s.dim)
+    ref = o.hmult(v)
+    choice, ms = s.tune()
+    print(f"{name}: tuned schedule {choice}, ms per schedule {ms}", flush=True)
+    for c in (1, 2, 4, 8):
+        try:
+            s.set_schedule(c)
+        except gpu.EDError:
+            assert c == 8
+            continue
+        assert rel(s.hmult(v), ref) < 1e-13, c
+    s.close()
+
+
 def test_hmult_fp32_and_block(gpu):
     from hipmem import DevBuf
     m, (nu, nd) = golden("B")

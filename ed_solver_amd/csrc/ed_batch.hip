@@ -207,6 +207,8 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         J.lz.partB = J.lz.partA + 8192;
         J.lz.nA = J.nblk_hv;
         J.lz.nB = J.nblk_el;
+        J.lz.colmap = 0;
+        J.lz.dup = s->up.d;
         ku = std::max(ku, s->up.kpad);
         lds = std::max(lds, (size_t)s->up.d * sizeof(T));
         ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k] + 0, h_phi[k], sizeof(double) * s->dim, hipMemcpyHostToDevice, st));

@@ -61,15 +61,15 @@ __global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__rest
 }
 
 // Fast kernel (one column per workgroup, blockDim.x up to 1024), see hv_fast_body.
-template <typename T, int MODE, int K>
+template <typename T, int MODE, int K, bool ONE>
 __global__ __launch_bounds__(1024) void k_hv_fast(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
                                                    int64_t ld, LzArgs<T> lz)
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
-    hv_fast_body<T, MODE, K>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
-                             reinterpret_cast<unsigned char *>(smem_d), red);
+    hv_fast_body<T, MODE, K, ONE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
+                                  reinterpret_cast<unsigned char *>(smem_d), red);
 }
 
 // Generic kernel for short columns (d_up < 128: several columns per workgroup, element-
@@ -127,10 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
             y[idx] = acc;
         }
     }
-    if (MODE == 1) {
-        const double p = block_sum_all(dot, red);
-        if (threadIdx.x == 0) lz.partA[blockIdx.x] = p;
-    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blockIdx.x);
 }
 
 // ------------------------------------------------------------- Lanczos kernels
@@ -198,17 +195,18 @@ __global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t 
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *w = lz.slot(j + 1);
+    int64_t i0, i1, st;
+    elem_range(lz, n, blockIdx.x, gridDim.x, i0, i1, st);
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    for (int64_t i = i0; i < i1; i += st) {
         double v = (double)w[i];
         for (int k = 1; k <= j; ++k) v -= coef[k - 1] * (double)lz.slot(k)[i];
         const T vt = (T)v;
         w[i] = vt;
         acc += (double)vt * (double)vt;
     }
-    const double p = block_sum_all(acc, red);
-    if (threadIdx.x == 0) lz.partB[blockIdx.x] = p;
-    if (advance && block0()) lz.st->j_k1 = j + 1;
+    block_partial(acc, red, lz.partB + blockIdx.x);
+    if (advance && blockIdx.x == 0 && threadIdx.x == 0) lz.st->j_k1 = j + 1;
 }
 
 // psi = sum_{k<m} coef[k] u_{k+1} / beta_{k+1} (FP64 output).
@@ -346,18 +344,24 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     return h;
 }
 
+template <typename T, int MODE, bool ONE>
+static void launch_fast_one(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x,
+                            T *y, int64_t ld, const LzArgs<T> &lz)
+{
+    switch (k) {
+    case 4: hipLaunchKernelGGL((k_hv_fast<T, MODE, 4, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 8: hipLaunchKernelGGL((k_hv_fast<T, MODE, 8, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 12: hipLaunchKernelGGL((k_hv_fast<T, MODE, 12, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    default: hipLaunchKernelGGL((k_hv_fast<T, MODE, 16, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    }
+}
+
 template <typename T, int MODE>
 static void launch_fast(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x, T *y,
                         int64_t ld, const LzArgs<T> &lz)
 {
-    switch (k) {
-    case 4: hipLaunchKernelGGL((k_hv_fast<T, MODE, 4>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 8: hipLaunchKernelGGL((k_hv_fast<T, MODE, 8>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 12: hipLaunchKernelGGL((k_hv_fast<T, MODE, 12>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 16: hipLaunchKernelGGL((k_hv_fast<T, MODE, 16>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 24: hipLaunchKernelGGL((k_hv_fast<T, MODE, 24>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    default: hipLaunchKernelGGL((k_hv_fast<T, MODE, 32>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    }
+    if (a.dup <= nt) launch_fast_one<T, MODE, true>(a, k, grid, nt, lds, st, x, y, ld, lz);
+    else launch_fast_one<T, MODE, false>(a, k, grid, nt, lds, st, x, y, ld, lz);
 }
 
 template <typename T, int MODE, int CPB>
@@ -491,7 +495,9 @@ static LzArgs<T> lz_args(ed_sector *s)
     lz.partA = s->d_partials + kPartA;
     lz.partB = s->d_partials + kPartB;
     lz.nA = hv_shape(s, s->cpb).nblk;
-    lz.nB = elem_grid(s->dim);
+    lz.dup = s->up.d;
+    lz.colmap = hv_shape(s, s->cpb).kind == 2 ? 1 : 0;
+    lz.nB = lz.colmap ? s->dn.d : elem_grid(s->dim);
     return lz;
 }
 
@@ -501,7 +507,7 @@ static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t 
     const LzArgs<T> lz = lz_args<T>(s);
     const HvShape h = hv_shape(s, s->cpb);
     const int64_t n = s->dim;
-    const int g = elem_grid(n);
+    const int g = lz.nB;   // element-wise kernels: one workgroup per partial of partB
     const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(kReorthGrid, (n + kBlock * 16 - 1) / (kBlock * 16)));
     double *partR = s->d_partials + kPartR, *coef = s->d_partials + kCoef;
     for (int i = 0; i < nsteps; ++i) {
@@ -570,7 +576,7 @@ static int lz_start(ed_sector *s, int prec, const double *h_start, const double 
             ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
         src = s->d_io_a;
     }
-    const int g = elem_grid(s->dim);
+    const int g = lz_args<double>(s).nB;
     if (prec == ED_PREC_F64)
         hipLaunchKernelGGL(k_lz_init<double>, dim3(g), dim3(kBlock), 0, st, lz_args<double>(s), src, s->dim);
     else
@@ -966,7 +972,7 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     for (auto &e : ev) ED_HIP_CHECK(hipEventCreate(&e));
     const int64_t n = sec->dim;
-    const int g = elem_grid(n);
+    const int g = lz_args<double>(sec).nB;
     const HvShape h = hv_shape(sec, sec->cpb);
     for (int r = 0; r < reps; ++r) {
         ED_HIP_CHECK(hipEventRecord(ev[2 * r], st));

@@ -108,6 +108,9 @@ struct LzArgs {
     LzState *st;
     double *partA, *partB;
     int nA, nB;        // workgroups that wrote partA (H.v phase) / partB (update phase)
+    int colmap, dup;   // colmap: element-wise kernels give workgroup b the column b (d_up
+                       // elements), the column the H.v workgroup b reads and writes, so it
+                       // stays in the same XCD's L2; else a grid-stride split
     __host__ __device__ T *slot(int j) const { return base + (int64_t)((j - 1) % nslots) * ld; }
 };
 

@@ -40,6 +40,32 @@ __device__ inline double sum_partials(const double *p, int n, double *red)
     return block_sum_all(v, red);
 }
 
+// Sum of n partials by ONE wave (lane-strided, then the butterfly): the same fixed order
+// in every workgroup and every kernel that uses it, so all of them obtain the same bits.
+// Valid in every lane of the calling wave.
+__device__ inline double wave_sum_partials(const double *p, int n)
+{
+    double v = 0.0;
+    for (int i = (int)(threadIdx.x & 63); i < n; i += 64) v += p[i];
+    return wave_sum(v);
+}
+
+// Element range of workgroup blk of an element-wise Lanczos kernel (see LzArgs::colmap).
+template <typename T>
+__device__ inline void elem_range(const LzArgs<T> &lz, int64_t n, int blk, int nblk, int64_t &i0, int64_t &i1,
+                                  int64_t &step)
+{
+    if (lz.colmap) {
+        i0 = (int64_t)blk * lz.dup + threadIdx.x;
+        i1 = min(n, (int64_t)(blk + 1) * lz.dup);
+        step = blockDim.x;
+    } else {
+        i0 = (int64_t)blk * blockDim.x + threadIdx.x;
+        i1 = n;
+        step = (int64_t)nblk * blockDim.x;
+    }
+}
+
 // Prologue of a Lanczos H.v phase: beta_j from the update phase's partials, breakdown
 // test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
 // `lead` is true in exactly one thread of the job (it publishes beta_j and j).
@@ -48,7 +74,12 @@ __device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, bool lead, 
 {
     if (lz.st->status) return false;
     j = lz.st->j_k1;
-    const double bj = sqrt(sum_partials(lz.partB, lz.nB, red));
+    if (threadIdx.x < 64) {
+        const double b2 = wave_sum_partials(lz.partB, lz.nB);
+        if (threadIdx.x == 0) red[kRed - 1] = b2;
+    }
+    __syncthreads();
+    const double bj = sqrt(red[kRed - 1]);
     if (bj < kMinLanczosNorm) {
         if (lead) lz.st->status = (j == 1) ? 2 : 1;
         return false;
@@ -60,6 +91,21 @@ __device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, bool lead, 
     sj = 1.0 / bj;
     cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
     return true;
+}
+
+// Deterministic block sum of one value per thread into `out` (thread 0 only): one
+// butterfly per wave, one barrier, waves added in order by thread 0.
+__device__ inline void block_partial(double v, double *red, double *out)
+{
+    v = wave_sum(v);
+    __syncthreads();   // red may still be in use
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        *out = t;
+    }
 }
 
 // H.v for CPB columns starting at column blk*CPB, rows owned by threads.
@@ -144,10 +190,7 @@ __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xi
             }
         }
     }
-    if (MODE == 1) {
-        const double p = block_sum_all(dot, red);
-        if (threadIdx.x == 0) lz.partA[blk] = p;
-    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }
 
 // Bytes of dynamic LDS the fast H.v needs (column + coefficient/interaction tables).
@@ -157,14 +200,15 @@ __host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int 
     return tsize * ((size_t)dup + ncoef + nq + nhund) + sizeof(int) * (nq + 1 + nhund + ncfg);
 }
 
-// Fast H.v: one column per workgroup, blockDim.x threads own rows r = tid + i*blockDim.x.
-// All loads that do not depend on the staged column (packed up-hop entries, down-hop
-// neighbour values, diagonal, previous Lanczos vector) are issued before the LDS barrier,
-// so the column staging and the row gathers overlap.  Up-hop coefficients, interaction
+// Fast H.v: one column per workgroup, blockDim.x threads own rows r = tid + i*blockDim.x
+// (ONE: at most one row per thread).  All loads that do not depend on the staged column
+// (packed up-hop entries, down-hop neighbour values, diagonal, previous Lanczos vector)
+// are issued before the one LDS barrier, together with the column staging and, in
+// Lanczos mode, the reduction of beta_j by wave 0.  Up-hop coefficients, interaction
 // diagonal and Hund transitions are read from LDS.  K bounds the entries per row of both
 // spin tables.  Dynamic LDS: col[dup] coef[ncoef] uimp[ncfg^2] hval[nhund] (T), then
 // hptr[ncfg^2+1] htr[nhund] swdn[ncfg] (int).
-template <typename T, int MODE, int K>
+template <typename T, int MODE, int K, bool ONE>
 __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
                                     const LzArgs<T> &lz, int blk, unsigned char *smem, double *red)
 {
@@ -180,10 +224,10 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     const T *__restrict__ x;
     T *__restrict__ y;
     const T *__restrict__ prev = nullptr;
-    double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, blk == 0 && tid == 0, j, sj, cprev)) return;
+        if (lz.st->status) return;
+        j = lz.st->j_k1;
         x = lz.slot(j);
         y = lz.slot(j + 1);
         if (j > 1) prev = lz.slot(j - 1);
@@ -218,7 +262,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         cu = a.imp_up[r];
         if (MODE == 1 && j > 1) pv = prev[base + r];
     }
-    // ---- stage the column and the small tables in LDS
+    // ---- stage the column and the small tables in LDS; wave 0 sums beta_j^2
     for (int e = tid; e < dup; e += NT) col[e] = x[base + e];
     for (int e = tid; e < a.ncoef_up; e += NT) coef[e] = a.coef_up[e];
     for (int e = tid; e < nq; e += NT) uimp[e] = a.uimp[e];
@@ -230,7 +274,25 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         }
         for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
     }
+    if (MODE == 1 && tid < 64) {
+        const double b2 = wave_sum_partials(lz.partB, lz.nB);
+        if (tid == 0) red[kRed - 1] = b2;
+    }
     __syncthreads();
+    double sj = 1.0, cprev = 0.0;
+    if (MODE == 1) {
+        const double bj = sqrt(red[kRed - 1]);
+        if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+            if (blk == 0 && tid == 0) lz.st->status = (j == 1) ? 2 : 1;
+            return;
+        }
+        if (blk == 0 && tid == 0) {
+            lz.st->beta[j - 1] = bj;
+            lz.st->j_k2 = j;
+        }
+        sj = 1.0 / bj;
+        cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
+    }
 
     double dot = 0.0;
     while (r < dup) {
@@ -257,6 +319,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         } else {
             y[idx] = acc;
         }
+        if (ONE) break;
         // next row of this thread
         r += NT;
         if (r < dup) {
@@ -269,10 +332,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
             if (MODE == 1 && j > 1) pv = prev[base + r];
         }
     }
-    if (MODE == 1) {
-        const double p = block_sum_all(dot, red);
-        if (tid == 0) lz.partA[blk] = p;
-    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }
 
 // u_1 = start (converted to T); partial sums of ||u_1||^2; j = 1.
@@ -281,14 +341,15 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
                                     double *red)
 {
     T *u = lz.slot(1);
+    int64_t i0, i1, st;
+    elem_range(lz, n, blk, nblk, i0, i1, st);
     double acc = 0.0;
-    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)nblk * blockDim.x) {
+    for (int64_t i = i0; i < i1; i += st) {
         const T v = (T)start[i];
         u[i] = v;
         acc += (double)v * (double)v;
     }
-    const double p = block_sum_all(acc, red);
-    if (threadIdx.x == 0) lz.partB[blk] = p;
+    block_partial(acc, red, lz.partB + blk);
     if (blk == 0 && threadIdx.x == 0) {
         lz.st->j_k1 = 1;
         lz.st->j_k2 = 0;
@@ -297,38 +358,58 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
 }
 
 // Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
+// The first element pair of every thread is loaded before alpha_j is known.
 template <typename T>
 __device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advance, int blk, int nblk, double *red)
 {
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
-    const double alpha = sum_partials(lz.partA, lz.nA, red);
-    const bool lead = blk == 0 && threadIdx.x == 0;
-    if (lead) lz.st->alpha[j - 1] = alpha;
     T *__restrict__ y = lz.slot(j + 1);
     const T *__restrict__ x = lz.slot(j);
+    int64_t i0, i1, st;
+    elem_range(lz, n, blk, nblk, i0, i1, st);
+    T y0 = 0, x0 = 0;
+    if (i0 < i1) {
+        y0 = y[i0];
+        x0 = x[i0];
+    }
+    if (threadIdx.x < 64) {
+        const double al = wave_sum_partials(lz.partA, lz.nA);
+        if (threadIdx.x == 0) red[kRed - 1] = al;
+    }
+    __syncthreads();
+    const double alpha = red[kRed - 1];
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    if (lead) lz.st->alpha[j - 1] = alpha;
     const double c = alpha / lz.st->beta[j - 1];
     double acc = 0.0;
-    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)nblk * blockDim.x) {
+    if (i0 < i1) {
+        const T v = (T)((double)y0 - c * (double)x0);
+        y[i0] = v;
+        acc += (double)v * (double)v;
+    }
+    for (int64_t i = i0 + st; i < i1; i += st) {
         const T v = (T)((double)y[i] - c * (double)x[i]);
         y[i] = v;
         acc += (double)v * (double)v;
     }
-    const double p = block_sum_all(acc, red);
-    if (threadIdx.x == 0) lz.partB[blk] = p;
+    block_partial(acc, red, lz.partB + blk);
     if (advance && lead) lz.st->j_k1 = j + 1;
 }
 
-// End of a chunk: beta_j of the next step from the update partials (one workgroup).
+// End of a chunk: beta_j of the next step from the update partials (one workgroup), the
+// same reduction the next H.v phase performs.
 template <typename T>
 __device__ inline void lz_finish_body(const LzArgs<T> &lz, double *red)
 {
     if (lz.st->status) return;
     const int j = lz.st->j_k1;
-    const double b = sqrt(sum_partials(lz.partB, lz.nB, red));
-    if (threadIdx.x == 0) {
-        lz.st->beta[j - 1] = b;
-        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+    if (threadIdx.x < 64) {
+        const double b = sqrt(wave_sum_partials(lz.partB, lz.nB));
+        if (threadIdx.x == 0) {
+            lz.st->beta[j - 1] = b;
+            if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+        }
     }
 }
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "ed_lanczos_diagonalize": (ctypes.c_int, [_vp, ctypes.POINTER(LanczosOpts), _dp, _c_int_p, _c_int_p, _dp, _dp,
                                               _dp, _dp]),
     "ed_lanczos_info": (ctypes.c_int, [_vp, _c_int_p]),
+    "ed_full_ed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_gs_vector": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_gs_vector_dev": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _vp]),

@@ -64,26 +64,29 @@ __global__ __launch_bounds__(kBlock) void k_hv_batch(const JobDesc<T> *jobs, int
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int j = find_job_hv(jobs, njobs);
     const HvArgs<T> a = jobs[j].a;
     const LzArgs<T> lz = jobs[j].lz;
     hv_rows_body<T, 1, 1, KU, true>(a, (const T *)nullptr, (T *)nullptr, lz, blockIdx.x - jobs[j].blk0_hv,
-                                    reinterpret_cast<T *>(smem_d), red);
+                                    reinterpret_cast<T *>(smem_d), red, grp);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_batch(const JobDesc<T> *jobs, int njobs)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int j = find_job_el(jobs, njobs);
     const LzArgs<T> lz = jobs[j].lz;
-    lz_update_body(lz, jobs[j].n, 1, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red);
+    lz_update_body(lz, jobs[j].n, 1, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red, grp);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_init_batch(const JobDesc<T> *jobs, int njobs)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int j = find_job_el(jobs, njobs);
     const LzArgs<T> lz = jobs[j].lz;
     lz_init_body(lz, jobs[j].start, jobs[j].n, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red);
@@ -93,8 +96,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finish_batch(const JobDesc<T> *jobs)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const LzArgs<T> lz = jobs[blockIdx.x].lz;
-    lz_finish_body(lz, red);
+    lz_finish_body(lz, grp);
 }
 
 // phi_k / ||phi_k|| for every job into its start area.

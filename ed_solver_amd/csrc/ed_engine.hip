@@ -55,9 +55,10 @@ __global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__rest
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
     hv_rows_body<T, MODE, CPB, KU, false>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
-                                          reinterpret_cast<T *>(smem_d), red);
+                                          reinterpret_cast<T *>(smem_d), red, grp);
 }
 
 // Fast kernel (one column per workgroup, blockDim.x up to 1024), see hv_fast_body.
@@ -67,9 +68,10 @@ __global__ __launch_bounds__(1024) void k_hv_fast(HvArgs<T> a, const T *__restri
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
     hv_fast_body<T, MODE, K, ONE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
-                                  reinterpret_cast<unsigned char *>(smem_d), red);
+                                  reinterpret_cast<unsigned char *>(smem_d), red, grp);
 }
 
 // Generic kernel for short columns (d_up < 128: several columns per workgroup, element-
@@ -81,6 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     extern __shared__ double smem_d[];
     T *col = reinterpret_cast<T *>(smem_d);
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const int dup = a.dup, ncfg = a.ncfg;
     const int c0 = blockIdx.x * a.cpb;
     const int ncol = min(a.cpb, a.ddn - c0);
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, block0(), j, sj, cprev)) return;
+        if (!lz_prologue(lz, red, grp, block0(), j, sj, cprev)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
         if (j > 1) prev = lz.slot(j - 1);
@@ -136,6 +139,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *start, int64_t n)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     lz_init_body(lz, start, n, blockIdx.x, gridDim.x, red);
 }
 
@@ -143,14 +147,16 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
 {
     __shared__ double red[kRed];
-    lz_update_body(lz, n, advance, blockIdx.x, gridDim.x, red);
+    __shared__ double grp[kMaxGroups];
+    lz_update_body(lz, n, advance, blockIdx.x, gridDim.x, red, grp);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
 {
     __shared__ double red[kRed];
-    lz_finish_body(lz, red);
+    __shared__ double grp[kMaxGroups];
+    lz_finish_body(lz, grp);
 }
 
 // Full reorthogonalisation (classical Gram-Schmidt, applied twice): partial dots of
@@ -159,6 +165,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *partR)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     const T *w = lz.slot(j + 1);
@@ -192,6 +199,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *w = lz.slot(j + 1);
@@ -224,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(LzArgs<T> lz, int m, const d
 __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const double *y, int64_t n, double *partials)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         acc += x[i] * y[i];
@@ -234,6 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const doub
 __global__ __launch_bounds__(kBlock) void k_sum_part(const double *partials, int n, double *out)
 {
     __shared__ double red[kRed];
+    __shared__ double grp[kMaxGroups];
     const double s = sum_partials(partials, n, red);
     if (threadIdx.x == 0) *out = s;
 }
@@ -312,7 +322,7 @@ static bool fast_ok(const ed_sector *s)
     const int k = width_of(std::max(s->up.kpad, s->dn.kpad));
     // widths above 16 exceed the 128-VGPR budget of a 1024-thread workgroup in FP64
     return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 &&
-           hv_fast_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(double)) <= 60 * 1024;
+           hv_fast_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(double)) <= 56 * 1024;   // + 4.2 KB static
 }
 
 // sched: 1, 2, 4 = row kernel with that many columns per workgroup; 8 = fast kernel.
@@ -335,7 +345,7 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     if (h.kind == 1) {
         h.cpb = (sched == 2 || sched == 4) ? sched : 1;
         // LDS holds CPB FP64 columns; stay within 64 KB
-        while (h.cpb > 1 && (size_t)h.cpb * dup * sizeof(double) > 65536) h.cpb /= 2;
+        while (h.cpb > 1 && (size_t)h.cpb * dup * sizeof(double) > 56 * 1024) h.cpb /= 2;
     } else {
         h.cpb = dup >= kBlock ? 1 : std::max(1, kBlock / std::max(1, dup));
     }
@@ -1007,6 +1017,97 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
 
 // Internal entry points used by the Green's-function translation unit.
 namespace ed {
+// ED_diago (ED_ARPACK.f90:590-687): the dense H of a small sector, column i = H e_i
+// from the block H.v kernel, then all eigenpairs on the host by cyclic Jacobi rotations
+// (Golub and Van Loan, Matrix Computations, sec. 8.5), ascending.
+static int jacobi_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V)
+{
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) {
+                const double a = A[(size_t)q * n + p];
+                tot += a * a;
+                if (p != q) off += a * a;
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[(size_t)q * n + p];
+                if (apq == 0.0) continue;
+                const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {   // columns p, q
+                    const double akp = A[(size_t)p * n + k], akq = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * akp - s * akq;
+                    A[(size_t)q * n + k] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {   // rows p, q
+                    const double apk = A[(size_t)k * n + p], aqk = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * apk - s * aqk;
+                    A[(size_t)k * n + q] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[(size_t)p * n + k], vkq = V[(size_t)q * n + k];
+                    V[(size_t)p * n + k] = c * vkp - s * vkq;
+                    V[(size_t)q * n + k] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+    w.resize(n);
+    std::vector<double> Vs((size_t)n * n);
+    for (int k = 0; k < n; ++k) {
+        w[k] = A[(size_t)order[k] * n + order[k]];
+        const double *src = V.data() + (size_t)order[k] * n;
+        int imax = 0;
+        for (int r = 1; r < n; ++r)
+            if (std::fabs(src[r]) > std::fabs(src[imax])) imax = r;
+        const double sg = src[imax] < 0 ? -1.0 : 1.0;
+        for (int r = 0; r < n; ++r) Vs[(size_t)k * n + r] = sg * src[r];
+    }
+    V.swap(Vs);
+    return ED_OK;
+}
+
+}  // namespace ed
+
+extern "C" int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs)
+{
+    using namespace ed;
+    ED_TRY(check_common(sec, 0));
+    const int64_t n = sec->dim;
+    if (n < 1 || n > 1024) return fail(ED_ERR_ARG, "ed_full_ed: sector dimension must be in [1, 1024]");
+    if (neig < 1 || neig > n || !evals) return fail(ED_ERR_ARG, "ed_full_ed: neig outside [1, dim] or NULL evals");
+    hipStream_t st = sec->stream;
+    std::vector<double> eye((size_t)n * n, 0.0), H((size_t)n * n);
+    for (int64_t i = 0; i < n; ++i) eye[(size_t)i * n + i] = 1.0;
+    double *d = nullptr;
+    ED_HIP_CHECK(hipMallocAsync((void **)&d, sizeof(double) * 2 * n * n, st));
+    ED_HIP_CHECK(hipMemcpyAsync(d, eye.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, st));
+    int rc = launch_hv(sec, ED_PREC_F64, d, d + n * n, n, (int)n, st);
+    if (rc == ED_OK) ED_HIP_CHECK(hipMemcpyAsync(H.data(), d + n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFreeAsync(d, st);
+    if (rc != ED_OK) return rc;
+    std::vector<double> w, V;
+    ED_TRY(jacobi_eig((int)n, H, w, V));
+    for (int k = 0; k < neig; ++k) {
+        evals[k] = w[k];
+        if (evecs) std::memcpy(evecs + (size_t)k * n, V.data() + (size_t)k * n, sizeof(double) * n);
+    }
+    return ED_OK;
+}
+
+namespace ed {
+
 int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, double *diag, double *subdiag,
                     int *breakdown_at)
 {

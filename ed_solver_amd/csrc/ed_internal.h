@@ -23,6 +23,8 @@ int fail(int code, const std::string &msg);
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kRed = 17;             // reduction scratch: one slot per wave of a 1024-thread workgroup + 1
+constexpr int kGroup = 16;           // partial sums are added in groups of 16 consecutive entries
+constexpr int kMaxGroups = 512;      // LDS slots for group sums (up to 8192 partials)
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
 

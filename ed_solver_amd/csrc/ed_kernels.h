@@ -40,14 +40,36 @@ __device__ inline double sum_partials(const double *p, int n, double *red)
     return block_sum_all(v, red);
 }
 
-// Sum of n partials by ONE wave (lane-strided, then the butterfly): the same fixed order
-// in every workgroup and every kernel that uses it, so all of them obtain the same bits.
-// Valid in every lane of the calling wave.
-__device__ inline double wave_sum_partials(const double *p, int n)
+// Fixed-order sum of n <= kGroup*kMaxGroups partial sums, in two steps so that the
+// loads are issued together and the order does not depend on the workgroup size:
+//   grp_issue:  group g = entries [16g, 16g+16) summed left to right into grp[g]
+//               (threads take groups g = tid, tid + blockDim.x, ...);
+//   (barrier)
+//   grp_total:  the groups summed left to right (every thread, LDS broadcast reads).
+// Every kernel that reduces the same partials therefore obtains the same bits.
+__device__ inline void grp_issue(const double *p, int n, double *grp)
 {
-    double v = 0.0;
-    for (int i = (int)(threadIdx.x & 63); i < n; i += 64) v += p[i];
-    return wave_sum(v);
+    const int ng = (n + kGroup - 1) / kGroup;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+        double v[kGroup];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) {
+            const int i = g * kGroup + q;
+            v[q] = i < n ? p[i] : 0.0;
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) s += v[q];
+        grp[g] = s;
+    }
+}
+
+__device__ inline double grp_total(const double *grp, int n)
+{
+    const int ng = (n + kGroup - 1) / kGroup;
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += grp[g];
+    return s;
 }
 
 // Element range of workgroup blk of an element-wise Lanczos kernel (see LzArgs::colmap).
@@ -70,16 +92,14 @@ __device__ inline void elem_range(const LzArgs<T> &lz, int64_t n, int blk, int n
 // test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
 // `lead` is true in exactly one thread of the job (it publishes beta_j and j).
 template <typename T>
-__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, bool lead, int &j, double &sj, double &cprev)
+__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, double *grp, bool lead, int &j, double &sj,
+                                   double &cprev)
 {
     if (lz.st->status) return false;
     j = lz.st->j_k1;
-    if (threadIdx.x < 64) {
-        const double b2 = wave_sum_partials(lz.partB, lz.nB);
-        if (threadIdx.x == 0) red[kRed - 1] = b2;
-    }
+    grp_issue(lz.partB, lz.nB, grp);
     __syncthreads();
-    const double bj = sqrt(red[kRed - 1]);
+    const double bj = sqrt(grp_total(grp, lz.nB));
     if (bj < kMinLanczosNorm) {
         if (lead) lz.st->status = (j == 1) ? 2 : 1;
         return false;
@@ -114,7 +134,7 @@ __device__ inline void block_partial(double v, double *red, double *out)
 // a.kup reread entry a.kup-1 with a zero coefficient, so no load leaves the table.
 template <typename T, int MODE, int CPB, int KU, bool CLAMP>
 __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
-                                    const LzArgs<T> &lz, int blk, T *col, double *red)
+                                    const LzArgs<T> &lz, int blk, T *col, double *red, double *grp)
 {
     const int dup = a.dup, ncfg = a.ncfg;
     const int c0 = blk * CPB;
@@ -124,7 +144,7 @@ __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xi
     double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, blk == 0 && threadIdx.x == 0, j, sj, cprev)) return;
+        if (!lz_prologue(lz, red, grp, blk == 0 && threadIdx.x == 0, j, sj, cprev)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
         if (j > 1) prev = lz.slot(j - 1);
@@ -210,7 +230,7 @@ __host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int 
 // hptr[ncfg^2+1] htr[nhund] swdn[ncfg] (int).
 template <typename T, int MODE, int K, bool ONE>
 __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
-                                    const LzArgs<T> &lz, int blk, unsigned char *smem, double *red)
+                                    const LzArgs<T> &lz, int blk, unsigned char *smem, double *red, double *grp)
 {
     const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund;
     T *col = reinterpret_cast<T *>(smem);
@@ -274,14 +294,11 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         }
         for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
     }
-    if (MODE == 1 && tid < 64) {
-        const double b2 = wave_sum_partials(lz.partB, lz.nB);
-        if (tid == 0) red[kRed - 1] = b2;
-    }
+    if (MODE == 1) grp_issue(lz.partB, lz.nB, grp);
     __syncthreads();
     double sj = 1.0, cprev = 0.0;
     if (MODE == 1) {
-        const double bj = sqrt(red[kRed - 1]);
+        const double bj = sqrt(grp_total(grp, lz.nB));
         if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
             if (blk == 0 && tid == 0) lz.st->status = (j == 1) ? 2 : 1;
             return;
@@ -358,37 +375,42 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
 }
 
 // Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
-// The first element pair of every thread is loaded before alpha_j is known.
+// The first kPre element pairs of every thread are loaded before alpha_j is known.
 template <typename T>
-__device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advance, int blk, int nblk, double *red)
+__device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advance, int blk, int nblk, double *red,
+                                      double *grp)
 {
+    constexpr int kPre = 4;
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *__restrict__ y = lz.slot(j + 1);
     const T *__restrict__ x = lz.slot(j);
     int64_t i0, i1, st;
     elem_range(lz, n, blk, nblk, i0, i1, st);
-    T y0 = 0, x0 = 0;
-    if (i0 < i1) {
-        y0 = y[i0];
-        x0 = x[i0];
+    T yv[kPre], xv[kPre];
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+        const int64_t i = i0 + q * st;
+        yv[q] = i < i1 ? y[i] : (T)0;
+        xv[q] = i < i1 ? x[i] : (T)0;
     }
-    if (threadIdx.x < 64) {
-        const double al = wave_sum_partials(lz.partA, lz.nA);
-        if (threadIdx.x == 0) red[kRed - 1] = al;
-    }
+    grp_issue(lz.partA, lz.nA, grp);
     __syncthreads();
-    const double alpha = red[kRed - 1];
+    const double alpha = grp_total(grp, lz.nA);
     const bool lead = blk == 0 && threadIdx.x == 0;
     if (lead) lz.st->alpha[j - 1] = alpha;
     const double c = alpha / lz.st->beta[j - 1];
     double acc = 0.0;
-    if (i0 < i1) {
-        const T v = (T)((double)y0 - c * (double)x0);
-        y[i0] = v;
-        acc += (double)v * (double)v;
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+        const int64_t i = i0 + q * st;
+        if (i < i1) {
+            const T v = (T)((double)yv[q] - c * (double)xv[q]);
+            y[i] = v;
+            acc += (double)v * (double)v;
+        }
     }
-    for (int64_t i = i0 + st; i < i1; i += st) {
+    for (int64_t i = i0 + kPre * st; i < i1; i += st) {
         const T v = (T)((double)y[i] - c * (double)x[i]);
         y[i] = v;
         acc += (double)v * (double)v;
@@ -400,16 +422,16 @@ __device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advanc
 // End of a chunk: beta_j of the next step from the update partials (one workgroup), the
 // same reduction the next H.v phase performs.
 template <typename T>
-__device__ inline void lz_finish_body(const LzArgs<T> &lz, double *red)
+__device__ inline void lz_finish_body(const LzArgs<T> &lz, double *grp)
 {
     if (lz.st->status) return;
     const int j = lz.st->j_k1;
-    if (threadIdx.x < 64) {
-        const double b = sqrt(wave_sum_partials(lz.partB, lz.nB));
-        if (threadIdx.x == 0) {
-            lz.st->beta[j - 1] = b;
-            if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
-        }
+    grp_issue(lz.partB, lz.nB, grp);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = sqrt(grp_total(grp, lz.nB));
+        lz.st->beta[j - 1] = b;
+        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
     }
 }
 

@@ -175,6 +175,13 @@ class Sector:
         check(lib().ed_lanczos_gs_vector(self.handle, res.iters, dptr(row), dptr(psi)))
         return psi
 
+    def full_ed(self, neig: int = 1):
+        """ED_diago: lowest neig eigenpairs of the dense sector H (dim <= 1024)."""
+        neig = int(min(neig, self.dim))
+        ev = np.zeros(neig); vec = np.zeros(neig * self.dim)
+        check(lib().ed_full_ed(self.handle, neig, dptr(ev), dptr(vec)))
+        return ev, vec.reshape(neig, self.dim)
+
     def lanczos_run(self, niter: int, start: np.ndarray | None = None, precision="f64"):
         """Lanczos_GPU_updo: exactly niter steps, returns (diag, subdiag)."""
         dg = np.zeros(niter); sd = np.zeros(niter)

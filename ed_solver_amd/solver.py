@@ -1,0 +1,122 @@
+"""Sector scan, Boltzmann weights and the Green's function over many states.
+
+Host orchestration that mirrors the reference's solver layer on top of the device
+entry points (every H.v, Lanczos step, excitation and continued fraction runs in
+libed_hip.so):
+
+* ``get_eigen_energies``  solver.h:98-237 -- per sector: dense ED when the dimension
+  is <= FULL_ED_SWITCH (Global_Variables.f90:23), else plain Lanczos (or block Lanczos
+  when ``block_size`` > 0); then the per-sector window filter [E_s, E_s + dEmax]
+  (filter_eigen) and the global one [E0, E0 + dEmax];
+* ``partition``           Z and the Boltzmann weights e^{-beta (E - E0)} / Z
+  (eigen_sector_class.f90:431-444, green_class_computeAA.h:107);
+* ``compute_green``       compute_greenAA (green_class_computeAA.f90:40-185,
+  green_class_computeAA.h:101-302) for the diagonal impurity Green's function:
+  G(i w_n) = sum_states boltz * (particle + hole), all excitation runs of a state set
+  in one batched device call.
+"""
+from __future__ import annotations
+
+import dataclasses
+from math import comb
+
+import numpy as np
+
+from .aim import AIMParams
+from .hclass import Sector, apply_C, compute_dynamic_batched
+
+FULL_ED_SWITCH = 40   # Global_Variables.f90:23
+
+
+@dataclasses.dataclass
+class EigenState:
+    sector: tuple[int, int]
+    energy: float
+    vector: np.ndarray
+
+
+def all_sectors(ns: int) -> list[tuple[int, int]]:
+    return [(nu, nd) for nu in range(ns + 1) for nd in range(ns + 1)]
+
+
+def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128,
+                       tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
+                       precision="f64", keep_sectors: bool = False):
+    """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax]."""
+    sectors = all_sectors(model.ns) if sectors is None else sectors
+    states: list[EigenState] = []
+    for (nu, nd) in sectors:
+        dim = comb(model.ns, nu) * comb(model.ns, nd)
+        with Sector(model, nu, nd) as sec:
+            if dim <= full_ed_switch:
+                ev, vec = sec.full_ed(min(neigen, dim))
+                found = [(ev[k], vec[k]) for k in range(len(ev))]
+            elif block_size > 0 and nu not in (0, 1, model.ns) and nd not in (0, 1, model.ns):
+                from .block import block_lanczos
+                ev, vec = block_lanczos(sec, nblock=block_size, neig=neigen, nitermax=nitermax, tolerance=tolerance)
+                found = [(ev[k], vec[k]) for k in range(len(ev))]
+            else:
+                res = sec.lanczos(nitermax=nitermax, neigen=neigen, tolerance=tolerance, precision=precision)
+                found = [(res.evals[k], sec.ground_state(res, k)) for k in range(len(res.evals))]
+        if not found:
+            continue
+        e_min = min(e for e, _ in found)
+        for e, v in found:   # filter_eigen(lowest, [dE, dE + dEmax]) per sector
+            if e <= e_min + dEmax:
+                states.append(EigenState((nu, nd), float(e), v))
+    if not states:
+        return []
+    e0 = min(s.energy for s in states)
+    return [s for s in states if s.energy <= e0 + dEmax]
+
+
+def partition(states: list[EigenState], beta: float) -> np.ndarray:
+    """Boltzmann weights e^{-beta (E - E0)} / Z of the kept states."""
+    e = np.array([s.energy for s in states])
+    w = np.exp(-beta * (e - e.min()))
+    return w / w.sum()
+
+
+def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
+                  spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
+                  precision="f64") -> np.ndarray:
+    """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
+
+    For every state with Boltzmann weight above weight_cutoff and every orbital: the
+    particle run from c+|psi> in (N+1) and the hole run from c|psi> in (N-1); all of
+    them advance together in one batched device call."""
+    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+    w = partition(states, beta)
+    G = np.zeros((len(orbitals), len(wn)), complex)
+    sectors: dict[tuple[int, int], Sector] = {}
+
+    def sector(nu, nd):
+        key = (nu, nd)
+        if key not in sectors:
+            sectors[key] = Sector(model, nu, nd)
+        return sectors[key]
+
+    jobs = []   # (orbital index, weight, sector, phi, iph, energy)
+    try:
+        for st, wt in zip(states, w):
+            if wt <= weight_cutoff:
+                continue
+            nu, nd = st.sector
+            src = sector(nu, nd)
+            for io, c in enumerate(orbitals):
+                for iph, pm in ((1, +1), (2, -1)):
+                    tn = (nu + pm, nd) if spin == 0 else (nu, nd + pm)
+                    if not (0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns):
+                        continue
+                    dst = sector(*tn)
+                    phi = apply_C(src, dst, c, spin, pm, st.vector)
+                    jobs.append((io, wt, dst, phi, iph, st.energy))
+        if jobs:
+            dyn, _, _ = compute_dynamic_batched([j[2] for j in jobs], [j[3] for j in jobs], [j[4] for j in jobs],
+                                                [j[5] for j in jobs], nitergreen, wn, cutoff_dynamic, precision)
+            for k, (io, wt, *_rest) in enumerate(jobs):
+                G[io] += wt * dyn[k]
+    finally:
+        for s in sectors.values():
+            s.close()
+    return G

@@ -169,6 +169,11 @@ int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const do
  * on this sector (0 after an FP64 call).  No reference counterpart. */
 int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase);
 
+/* ED_diago (ED_ARPACK.f90:590-687), the path of sectors of dimension <= FULL_ED_SWITCH
+ * (solver.h:156-180): dense H from H e_i on the device, all eigenpairs on the host
+ * (cyclic Jacobi), ascending.  evals[neig]; evecs[neig][dim] (may be NULL).  dim <= 1024. */
+int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs);
+
 /* Lanczos_GPU_updo (Lanczos_gpu.f90:106): exactly `niter` Lanczos steps (no
  * convergence test) from `start` (host or NULL); writes diag[niter], subdiag[niter].
  * Returns ED_ERR_BREAKDOWN when a zero-norm vector appears before niter. */

@@ -86,7 +86,7 @@ def main():
     log(f"sector (6,6) dim {sec.dim} built in {t_build*1e3:.1f} ms")
     D = sec.dim
     sched, sched_ms = sec.tune()
-    log(f"H.v schedule {sched} (ms for row kernel x1/x2/x4, fast kernel: {sched_ms})")
+    log(f"H.v schedule {sched} (ms for row kernel x1/x2/x4, fast kernel, fast kernel XCD-grouped: {sched_ms})")
 
     def rate(prec):
         for _ in range(args.warmup):

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_batch(const JobDesc<T> *jobs, int
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int j = find_job_hv(jobs, njobs);
     const HvArgs<T> a = jobs[j].a;
     const LzArgs<T> lz = jobs[j].lz;
@@ -76,7 +76,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_batch(const JobDesc<T> *jobs, int njobs)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int j = find_job_el(jobs, njobs);
     const LzArgs<T> lz = jobs[j].lz;
     lz_update_body(lz, jobs[j].n, 1, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red, grp);
@@ -86,7 +86,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_init_batch(const JobDesc<T> *jobs, int njobs)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int j = find_job_el(jobs, njobs);
     const LzArgs<T> lz = jobs[j].lz;
     lz_init_body(lz, jobs[j].start, jobs[j].n, blockIdx.x - jobs[j].blk0_el, jobs[j].nblk_el, red);
@@ -96,7 +96,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finish_batch(const JobDesc<T> *jobs)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const LzArgs<T> lz = jobs[blockIdx.x].lz;
     lz_finish_body(lz, grp);
 }
@@ -183,7 +183,7 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         ld[k] = (secs[k]->dim + 63) / 64 * 64;
         o_basis[k] = take(sizeof(T) * (size_t)ld[k] * (niter + 1));
         o_start[k] = take(sizeof(double) * (size_t)secs[k]->dim);
-        o_part[k] = take(sizeof(double) * (8192 + 2048));
+        o_part[k] = take(sizeof(double) * (8192 + 4 * 2048));
     }
     ED_TRY(ws_reserve(off, secs[0]->device));
     char *base = (char *)g_ws.ptr;
@@ -209,9 +209,12 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         J.lz.st = (LzState *)(base + o_state) + k;
         J.lz.partA = (double *)(base + o_part[k]);
         J.lz.partB = J.lz.partA + 8192;
+        J.lz.partC = J.lz.partB + 2 * 2048;
+        J.lz.pstride = 2048;
         J.lz.nA = J.nblk_hv;
         J.lz.nB = J.nblk_el;
         J.lz.colmap = 0;
+        J.lz.colord = nullptr;
         J.lz.dup = s->up.d;
         ku = std::max(ku, s->up.kpad);
         lds = std::max(lds, (size_t)s->up.d * sizeof(T));

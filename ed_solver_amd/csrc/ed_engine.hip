@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__rest
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
     hv_rows_body<T, MODE, CPB, KU, false>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
                                           reinterpret_cast<T *>(smem_d), red, grp);
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(1024) void k_hv_fast(HvArgs<T> a, const T *__restri
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
     hv_fast_body<T, MODE, K, ONE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
                                   reinterpret_cast<unsigned char *>(smem_d), red, grp);
@@ -83,20 +83,17 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     extern __shared__ double smem_d[];
     T *col = reinterpret_cast<T *>(smem_d);
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const int dup = a.dup, ncfg = a.ncfg;
     const int c0 = blockIdx.x * a.cpb;
     const int ncol = min(a.cpb, a.ddn - c0);
     const T *__restrict__ x;
     T *__restrict__ y;
-    const T *__restrict__ prev = nullptr;
-    double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, grp, block0(), j, sj, cprev)) return;
+        if (!lz_hv_begin(lz, block0(), j)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
-        if (j > 1) prev = lz.slot(j - 1);
     } else {
         x = xin + (int64_t)blockIdx.y * ld;
         y = yout + (int64_t)blockIdx.y * ld;
@@ -119,16 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
             for (int t = h0; t < h1; ++t)
                 acc += a.hund_val[t] * x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido * ncfg + a.hund_cd[t]] * dup];
         }
-        const int idx = base + e;
-        if (MODE == 1) {
-            double yv = (double)acc * sj;
-            if (j > 1) yv -= cprev * (double)prev[idx];
-            const T yt = (T)yv;
-            y[idx] = yt;
-            dot += (double)yt * ((double)xv * sj);
-        } else {
-            y[idx] = acc;
-        }
+        y[base + e] = acc;
+        if (MODE == 1) dot += (double)acc * (double)xv;
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blockIdx.x);
 }
@@ -139,7 +128,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_init(LzArgs<T> lz, const double *start, int64_t n)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     lz_init_body(lz, start, n, blockIdx.x, gridDim.x, red);
 }
 
@@ -147,7 +136,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_update(LzArgs<T> lz, int64_t n, int advance)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     lz_update_body(lz, n, advance, blockIdx.x, gridDim.x, red, grp);
 }
 
@@ -155,7 +144,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     lz_finish_body(lz, grp);
 }
 
@@ -165,7 +154,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_dots(LzArgs<T> lz, int64_t n, double *partR)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     const T *w = lz.slot(j + 1);
@@ -194,26 +183,31 @@ __global__ __launch_bounds__(kBlock) void k_reorth_coef(LzArgs<T> lz, const doub
     }
 }
 
-// w -= sum_k coef[k] u_k; partial ||w||^2; advances j when `advance`.
+// w -= sum_k coef[k] u_k; partial sums of ||w||^2 and <w, u_j> for step j+1; advances
+// j when `advance`.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reorth_update(LzArgs<T> lz, int64_t n, const double *coef, int advance)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ int cnt;
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *w = lz.slot(j + 1);
+    const T *uj = lz.slot(j);
     int64_t i0, i1, st;
     elem_range(lz, n, blockIdx.x, gridDim.x, i0, i1, st);
-    double acc = 0.0;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    double nn = 0.0, cr = 0.0;
     for (int64_t i = i0; i < i1; i += st) {
         double v = (double)w[i];
         for (int k = 1; k <= j; ++k) v -= coef[k - 1] * (double)lz.slot(k)[i];
         const T vt = (T)v;
         w[i] = vt;
-        acc += (double)vt * (double)vt;
+        nn += (double)vt * (double)vt;
+        cr += (double)vt * (double)uj[i];
     }
-    block_partial(acc, red, lz.partB + blockIdx.x);
+    block_partial_last(nn, cr, red, &cnt, lz.nrm(j + 1) + blockIdx.x, lz.crs(j + 1) + blockIdx.x);
     if (advance && blockIdx.x == 0 && threadIdx.x == 0) lz.st->j_k1 = j + 1;
 }
 
@@ -232,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(LzArgs<T> lz, int m, const d
 __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const double *y, int64_t n, double *partials)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         acc += x[i] * y[i];
@@ -243,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_part(const double *x, const doub
 __global__ __launch_bounds__(kBlock) void k_sum_part(const double *partials, int n, double *out)
 {
     __shared__ double red[kRed];
-    __shared__ double grp[kMaxGroups];
+    __shared__ double grp[kSums];
     const double s = sum_partials(partials, n, red);
     if (threadIdx.x == 0) *out = s;
 }
@@ -293,6 +287,7 @@ static HvArgs<T> hv_args(const ed_sector *s, int cpb_rows)
     a.kdn = s->dn.kpad;
     a.nhund = s->nhund;
     a.hund_tr = s->d_hund_tr;
+    a.colord = nullptr;
     return a;
 }
 
@@ -307,6 +302,7 @@ struct HvShape {
     int nt;      // threads per workgroup
     int k;       // compiled table width
     size_t lds;
+    int xcd;     // fast kernel over the XCD-grouped column order (s->d_colord)
 };
 
 static int width_of(int kmax)
@@ -321,23 +317,26 @@ static bool fast_ok(const ed_sector *s)
 {
     const int k = width_of(std::max(s->up.kpad, s->dn.kpad));
     // widths above 16 exceed the 128-VGPR budget of a 1024-thread workgroup in FP64
-    return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 &&
+    return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 && s->dn.d <= 8184 &&
            hv_fast_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(double)) <= 56 * 1024;   // + 4.2 KB static
 }
 
-// sched: 1, 2, 4 = row kernel with that many columns per workgroup; 8 = fast kernel.
+// sched: 1, 2, 4 = row kernel with that many columns per workgroup; 8 = fast kernel;
+// 16 = fast kernel with the XCD-grouped column order (needs s->d_colord).
 static HvShape hv_shape(const ed_sector *s, int sched)
 {
     HvShape h;
     const int dup = s->up.d, ddn = s->dn.d;
     h.nt = kBlock;
     h.k = s->up.kpad;
-    if (sched == 8 && fast_ok(s)) {
+    h.xcd = 0;
+    if ((sched == 8 || sched == 16) && fast_ok(s)) {
         h.kind = 2;
         h.cpb = 1;
         h.nt = std::min(1024, std::max(64, (dup + 63) / 64 * 64));
         h.k = width_of(std::max(s->up.kpad, s->dn.kpad));
-        h.nblk = ddn;
+        h.xcd = sched == 16 && s->d_colord;
+        h.nblk = h.xcd ? s->colord_blocks : ddn;
         h.lds = 0;   // set per precision at launch
         return h;
     }
@@ -393,6 +392,7 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
                            const LzArgs<T> &lz, hipStream_t st)
 {
     HvArgs<T> a = hv_args<T>(s, h.cpb);
+    if (h.xcd) a.colord = s->d_colord;
     dim3 grid(h.nblk, MODE ? 1 : nvec);
     if (h.kind == 2) {
         const size_t lds = hv_fast_lds(a.dup, a.ncoef_up, a.ncfg, a.nhund, sizeof(T));
@@ -504,10 +504,14 @@ static LzArgs<T> lz_args(ed_sector *s)
     lz.st = s->d_state;
     lz.partA = s->d_partials + kPartA;
     lz.partB = s->d_partials + kPartB;
-    lz.nA = hv_shape(s, s->cpb).nblk;
+    lz.partC = s->d_partials + kPartC;
+    lz.pstride = 8192;
+    const HvShape h = hv_shape(s, s->cpb);
+    lz.nA = h.nblk;
     lz.dup = s->up.d;
-    lz.colmap = hv_shape(s, s->cpb).kind == 2 ? 1 : 0;
-    lz.nB = lz.colmap ? s->dn.d : elem_grid(s->dim);
+    lz.colmap = h.kind == 2 ? 1 : 0;
+    lz.colord = h.xcd ? s->d_colord : nullptr;
+    lz.nB = lz.colmap ? h.nblk : elem_grid(s->dim);
     return lz;
 }
 
@@ -627,9 +631,58 @@ static int check_common(ed_sector *s, int prec)
     return ED_OK;
 }
 
+// XCD-grouped column order of the fast H.v.  Workgroups are dispatched to the 8 XCDs
+// round-robin (workgroup b runs on XCD b mod 8) and every XCD has an L2 of its own.  The
+// down hops of column ido read whole neighbour columns, so the columns one XCD owns
+// should be neighbours of each other: the down-hop graph is ordered breadth-first,
+// component by component, each component from a column of least degree; the order is
+// cut into 8 contiguous chunks and workgroup b takes entry b / 8 of chunk b mod 8.
+static int ensure_colord(ed_sector *s)
+{
+    if (s->d_colord) return ED_OK;
+    const int d = s->dn.d;
+    std::vector<int32_t> ptr(d + 1), rk(std::max<int64_t>(1, s->dn.nnz));
+    ED_HIP_CHECK(hipMemcpy(ptr.data(), s->dn.ptr, sizeof(int32_t) * (d + 1), hipMemcpyDeviceToHost));
+    if (s->dn.nnz > 0)
+        ED_HIP_CHECK(hipMemcpy(rk.data(), s->dn.rk, sizeof(int32_t) * s->dn.nnz, hipMemcpyDeviceToHost));
+    std::vector<int> by_deg(d), order;
+    order.reserve(d);
+    for (int i = 0; i < d; ++i) by_deg[i] = i;
+    std::stable_sort(by_deg.begin(), by_deg.end(),
+                     [&](int a, int b) { return ptr[a + 1] - ptr[a] < ptr[b + 1] - ptr[b]; });
+    std::vector<char> seen(d, 0);
+    for (int root : by_deg) {
+        if (seen[root]) continue;
+        size_t head = order.size();
+        seen[root] = 1;
+        order.push_back(root);
+        while (head < order.size()) {
+            const int u = order[head++];
+            for (int k = ptr[u]; k < ptr[u + 1]; ++k) {
+                const int v = rk[k];
+                if (v >= 0 && v < d && !seen[v]) {
+                    seen[v] = 1;
+                    order.push_back(v);
+                }
+            }
+        }
+    }
+    const int q = (d + 7) / 8, nblk = 8 * q;
+    std::vector<int32_t> map(nblk);
+    for (int b = 0; b < nblk; ++b) {
+        const int pos = (b % 8) * q + b / 8;
+        map[b] = pos < d ? order[pos] : -1;
+    }
+    ED_HIP_CHECK(hipMalloc((void **)&s->d_colord, sizeof(int32_t) * nblk));
+    ED_HIP_CHECK(hipMemcpy(s->d_colord, map.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice));
+    s->colord_blocks = nblk;
+    return ED_OK;
+}
+
 // Autotuner: times the FP64 H.v under every applicable schedule (row kernel with 1, 2
-// or 4 columns per workgroup; fast kernel), 3 launches each after one warm-up, HIP
-// events on the sector's stream, and keeps the fastest.
+// or 4 columns per workgroup; fast kernel in natural and in XCD-grouped column order),
+// 3 launches each after one warm-up, HIP events on the sector's stream, and keeps the
+// fastest.
 static int tune(ed_sector *s)
 {
     ED_TRY(ensure_io(s));
@@ -639,13 +692,16 @@ static int tune(ed_sector *s)
         s->cpb = fast_ok(s) ? 8 : 1;
         return ED_OK;
     }
+    if (fast_ok(s)) ED_TRY(ensure_colord(s));
     ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
     float best = 1e30f;
     int best_c = 1;
-    const int cands[4] = {1, 2, 4, 8};
-    for (int i = 0; i < 4; ++i) {
+    const int cands[5] = {1, 2, 4, 8, 16};
+    for (int i = 0; i < 5; ++i) {
         const HvShape h = hv_shape(s, cands[i]);
-        const bool applies = (cands[i] == 8) ? h.kind == 2 : (h.kind == 1 && h.cpb == cands[i]) || (h.kind == 0 && i == 0);
+        const bool applies = (cands[i] == 16) ? h.kind == 2 && h.xcd
+                             : (cands[i] == 8) ? h.kind == 2
+                                               : (h.kind == 1 && h.cpb == cands[i]) || (h.kind == 0 && i == 0);
         if (!applies) continue;
         LzArgs<double> lz{};
         hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
@@ -887,16 +943,17 @@ int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
     ED_TRY(tune(sec));
     if (cpb) *cpb = sec->cpb;
     if (ms_per_choice)
-        for (int i = 0; i < 4; ++i) ms_per_choice[i] = sec->tune_ms[i];
+        for (int i = 0; i < 5; ++i) ms_per_choice[i] = sec->tune_ms[i];
     return ED_OK;
 }
 
 int ed_hmult_set_schedule(ed_sector *sec, int choice)
 {
     ED_TRY(check_common(sec, 0));
-    if (choice != 1 && choice != 2 && choice != 4 && choice != 8)
-        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4 or 8");
-    if (choice == 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");
+    if (choice != 1 && choice != 2 && choice != 4 && choice != 8 && choice != 16)
+        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4, 8 or 16");
+    if (choice >= 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");
+    if (choice == 16) ED_TRY(ensure_colord(sec));
     sec->cpb = choice;
     drop_graphs(sec);
     return ED_OK;
@@ -1009,7 +1066,7 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     for (auto &e : ev) (void)hipEventDestroy(e);
     const size_t esz = precision == ED_PREC_F64 ? sizeof(double) : sizeof(float);
     *avg_ms = tot / reps;
-    if (bytes_per_launch) *bytes_per_launch = 3.0 * esz * (double)n;
+    if (bytes_per_launch) *bytes_per_launch = 2.0 * esz * (double)n;
     return ED_OK;
 }
 

@@ -22,16 +22,16 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
-constexpr int kRed = 17;             // reduction scratch: one slot per wave of a 1024-thread workgroup + 1
-constexpr int kGroup = 16;           // partial sums are added in groups of 16 consecutive entries
-constexpr int kMaxGroups = 512;      // LDS slots for group sums (up to 8192 partials)
+constexpr int kRed = 34;             // reduction scratch: two slots per wave of a 1024-thread workgroup + 2
+constexpr int kSums = 4;             // LDS slots for the reduced recurrence sums
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
 
 // Layout of the per-sector scratch array of doubles (offsets in doubles).
 constexpr int kPartA = 0;                          // partial sums written by the H.v phase
-constexpr int kPartB = kPartA + 8192;              // partial sums written by the update phase
-constexpr int kPartR = kPartB + 8192;              // reorthogonalisation dots [k][workgroup]
+constexpr int kPartB = kPartA + 8192;              // update phase, ||u_j||^2, [parity of j][8192]
+constexpr int kPartC = kPartB + 2 * 8192;          // update phase, <u_j, u_{j-1}>, [parity of j][8192]
+constexpr int kPartR = kPartC + 2 * 8192;          // reorthogonalisation dots [k][workgroup]
 constexpr int kReorthGrid = 256;                   // workgroups of the reorthogonalisation dots
 constexpr int kCoef = kPartR + kMaxIter * kReorthGrid;   // reorthogonalisation coefficients
 constexpr int kMisc = kCoef + kMaxIter;            // Ritz coefficients, norms, Rayleigh quotient
@@ -98,6 +98,7 @@ struct HvArgs {
     const T *coef_up;              // [ncoef_up]
     int ncoef_up, kdn, nhund;
     const int32_t *hund_tr;        // [nhund] cu' | cd' << 8
+    const int32_t *colord;         // fast kernel: workgroup -> column (XCD-grouped order, -1 = idle), or null
 };
 
 // Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1, and
@@ -108,12 +109,18 @@ struct LzArgs {
     int64_t ld;
     int nslots;
     LzState *st;
-    double *partA, *partB;
-    int nA, nB;        // workgroups that wrote partA (H.v phase) / partB (update phase)
+    double *partA;     // [nA] H.v phase: <H u_j, u_j>
+    double *partB;     // [2][pstride] update phase, by parity of j: ||u_j||^2
+    double *partC;     // [2][pstride] update phase, by parity of j: <u_j, u_{j-1}>
+    int pstride;
+    int nA, nB;        // workgroups that wrote partA (H.v phase) / partB, partC (update phase)
     int colmap, dup;   // colmap: element-wise kernels give workgroup b the column b (d_up
                        // elements), the column the H.v workgroup b reads and writes, so it
                        // stays in the same XCD's L2; else a grid-stride split
+    const int32_t *colord;   // with colmap: workgroup -> column as in the H.v (null = identity)
     __host__ __device__ T *slot(int j) const { return base + (int64_t)((j - 1) % nslots) * ld; }
+    __host__ __device__ double *nrm(int j) const { return partB + (j & 1) * pstride; }
+    __host__ __device__ double *crs(int j) const { return partC + (j & 1) * pstride; }
 };
 
 struct GraphCache {
@@ -151,8 +158,11 @@ struct ed_sector {
     ed::GraphCache graphs[4];      // captured Lanczos chunks, key = precision * 2 + reorth
     int last_f32_iters = 0;
     int cpb = 0;                   // H.v schedule chosen by the autotuner: 1/2/4 = row kernel with that many
-                                   // columns per workgroup, 8 = fast kernel; 0 = not tuned
-    double tune_ms[4] = {0, 0, 0, 0};   // measured FP64 H.v time per schedule (1, 2, 4, fast)
+                                   // columns per workgroup, 8 = fast kernel, 16 = fast kernel with the
+                                   // XCD-grouped column order; 0 = not tuned
+    double tune_ms[5] = {0, 0, 0, 0, 0};   // measured FP64 H.v time per schedule (1, 2, 4, 8, 16)
+    int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
+    int colord_blocks = 0;
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double hv_ms_sum = 0.0;

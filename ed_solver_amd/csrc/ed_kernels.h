@@ -40,36 +40,26 @@ __device__ inline double sum_partials(const double *p, int n, double *red)
     return block_sum_all(v, red);
 }
 
-// Fixed-order sum of n <= kGroup*kMaxGroups partial sums, in two steps so that the
-// loads are issued together and the order does not depend on the workgroup size:
-//   grp_issue:  group g = entries [16g, 16g+16) summed left to right into grp[g]
-//               (threads take groups g = tid, tid + blockDim.x, ...);
-//   (barrier)
-//   grp_total:  the groups summed left to right (every thread, LDS broadcast reads).
-// Every kernel that reduces the same partials therefore obtains the same bits.
-__device__ inline void grp_issue(const double *p, int n, double *grp)
+// Fixed-order sum of n partial sums by ONE whole wave (64 lanes): lane l adds entries
+// l, l+64, l+128, ... left to right (eight coalesced loads in flight per batch), then
+// the xor butterfly.  Addition is commutative, so the butterfly leaves the same bits in
+// every lane, and the order depends only on n: every kernel that reduces the same
+// partials with this function obtains the same value.
+__device__ inline double wave_part_sum(const double *p, int n)
 {
-    const int ng = (n + kGroup - 1) / kGroup;
-    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
-        double v[kGroup];
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int b0 = 0; b0 < n; b0 += 512) {
+        double v[8];
 #pragma unroll
-        for (int q = 0; q < kGroup; ++q) {
-            const int i = g * kGroup + q;
+        for (int q = 0; q < 8; ++q) {
+            const int i = b0 + q * 64 + lane;
             v[q] = i < n ? p[i] : 0.0;
         }
-        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < kGroup; ++q) s += v[q];
-        grp[g] = s;
+        for (int q = 0; q < 8; ++q) s += v[q];
     }
-}
-
-__device__ inline double grp_total(const double *grp, int n)
-{
-    const int ng = (n + kGroup - 1) / kGroup;
-    double s = 0.0;
-    for (int g = 0; g < ng; ++g) s += grp[g];
-    return s;
+    return wave_sum(s);
 }
 
 // Element range of workgroup blk of an element-wise Lanczos kernel (see LzArgs::colmap).
@@ -78,9 +68,15 @@ __device__ inline void elem_range(const LzArgs<T> &lz, int64_t n, int blk, int n
                                   int64_t &step)
 {
     if (lz.colmap) {
-        i0 = (int64_t)blk * lz.dup + threadIdx.x;
-        i1 = min(n, (int64_t)(blk + 1) * lz.dup);
+        const int c = lz.colord ? lz.colord[blk] : blk;
         step = blockDim.x;
+        if (c < 0) {   // idle workgroup of the XCD-grouped order
+            i0 = 0;
+            i1 = 0;
+            return;
+        }
+        i0 = (int64_t)c * lz.dup + threadIdx.x;
+        i1 = min(n, (int64_t)(c + 1) * lz.dup);
     } else {
         i0 = (int64_t)blk * blockDim.x + threadIdx.x;
         i1 = n;
@@ -88,28 +84,14 @@ __device__ inline void elem_range(const LzArgs<T> &lz, int64_t n, int blk, int n
     }
 }
 
-// Prologue of a Lanczos H.v phase: beta_j from the update phase's partials, breakdown
-// test (Lanczos_plain.h:70-92), scale factors.  Returns false when the step is skipped.
-// `lead` is true in exactly one thread of the job (it publishes beta_j and j).
+// Start of a Lanczos H.v phase: skipped once the run has stopped; j = step index.
+// `lead` (one thread of the job) hands j on to the update phase.
 template <typename T>
-__device__ inline bool lz_prologue(const LzArgs<T> &lz, double *red, double *grp, bool lead, int &j, double &sj,
-                                   double &cprev)
+__device__ inline bool lz_hv_begin(const LzArgs<T> &lz, bool lead, int &j)
 {
     if (lz.st->status) return false;
     j = lz.st->j_k1;
-    grp_issue(lz.partB, lz.nB, grp);
-    __syncthreads();
-    const double bj = sqrt(grp_total(grp, lz.nB));
-    if (bj < kMinLanczosNorm) {
-        if (lead) lz.st->status = (j == 1) ? 2 : 1;
-        return false;
-    }
-    if (lead) {
-        lz.st->beta[j - 1] = bj;
-        lz.st->j_k2 = j;
-    }
-    sj = 1.0 / bj;
-    cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
+    if (lead) lz.st->j_k2 = j;
     return true;
 }
 
@@ -128,6 +110,33 @@ __device__ inline void block_partial(double v, double *red, double *out)
     }
 }
 
+// The same fixed-order block sum without a workgroup barrier: every wave stores its
+// butterfly sum and bumps an LDS counter (`cnt`, zeroed before an earlier barrier); the
+// wave that arrives last adds the wave sums in index order and stores the result.  Two
+// values (v0 -> out0, v1 -> out1, out1 may be null) share one pass.
+__device__ inline void block_partial_last(double v0, double v1, double *red, int *cnt, double *out0, double *out1)
+{
+    v0 = wave_sum(v0);
+    if (out1) v1 = wave_sum(v1);
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        red[2 * w] = v0;
+        red[2 * w + 1] = v1;
+        __threadfence_block();
+        if (atomicAdd(cnt, 1) == nw - 1) {
+            __threadfence_block();
+            const volatile double *rv = red;
+            double t0 = 0.0, t1 = 0.0;
+            for (int i = 0; i < nw; ++i) {
+                t0 += rv[2 * i];
+                t1 += rv[2 * i + 1];
+            }
+            *out0 = t0;
+            if (out1) *out1 = t1;
+        }
+    }
+}
+
 // H.v for CPB columns starting at column blk*CPB, rows owned by threads.
 // MODE 0: y = H x.  MODE 1: Lanczos phase 1 (x, y from the Krylov slots).
 // CLAMP: the ELL width of this job may be below KU (batched kernels); entries k >=
@@ -140,14 +149,11 @@ __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xi
     const int c0 = blk * CPB;
     const T *__restrict__ x;
     T *__restrict__ y;
-    const T *__restrict__ prev = nullptr;
-    double sj = 1.0, cprev = 0.0;
     int j = 0;
     if (MODE == 1) {
-        if (!lz_prologue(lz, red, grp, blk == 0 && threadIdx.x == 0, j, sj, cprev)) return;
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
         x = lz.slot(j);
         y = lz.slot(j + 1);
-        if (j > 1) prev = lz.slot(j - 1);
     } else {
         x = xin;
         y = yout;
@@ -198,16 +204,8 @@ __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xi
                     acc += a.hund_val[t] *
                            x[a.swap_up[r * ncfg + a.hund_cu[t]] + a.swap_dn[ido[c] * ncfg + a.hund_cd[t]] * dup];
             }
-            const int idx = (c0 + c) * dup + r;
-            if (MODE == 1) {
-                double yv = (double)acc * sj;
-                if (j > 1) yv -= cprev * (double)prev[idx];
-                const T yt = (T)yv;
-                y[idx] = yt;
-                dot += (double)yt * ((double)xv * sj);
-            } else {
-                y[idx] = acc;
-            }
+            y[(c0 + c) * dup + r] = acc;
+            if (MODE == 1) dot += (double)acc * (double)xv;
         }
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blk);
@@ -240,17 +238,18 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     int *hptr = reinterpret_cast<int *>(hval + nh);
     int *htr = hptr + nq + 1;
     int *swdn = htr + nh;
-    const int ido = blk, NT = blockDim.x, tid = threadIdx.x;
+    const int ido = a.colord ? a.colord[blk] : blk, NT = blockDim.x, tid = threadIdx.x;
     const T *__restrict__ x;
     T *__restrict__ y;
-    const T *__restrict__ prev = nullptr;
     int j = 0;
+    if (MODE == 1 && !lz_hv_begin(lz, blk == 0 && tid == 0, j)) return;
+    if (ido < 0) {   // idle workgroup of the XCD-grouped order: contributes a zero partial
+        if (MODE == 1 && tid == 0) lz.partA[blk] = 0.0;
+        return;
+    }
     if (MODE == 1) {
-        if (lz.st->status) return;
-        j = lz.st->j_k1;
         x = lz.slot(j);
         y = lz.slot(j + 1);
-        if (j > 1) prev = lz.slot(j - 1);
     } else {
         x = xin;
         y = yout;
@@ -271,7 +270,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     int r = tid;
     uint32_t pk[K];
     T xd[K];
-    T du = 0, pv = 0;
+    T du = 0;
     int cu = 0;
     if (r < dup) {
 #pragma unroll
@@ -280,9 +279,8 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
         du = a.diag_up[r];
         cu = a.imp_up[r];
-        if (MODE == 1 && j > 1) pv = prev[base + r];
     }
-    // ---- stage the column and the small tables in LDS; wave 0 sums beta_j^2
+    // ---- stage the column and the small tables in LDS
     for (int e = tid; e < dup; e += NT) col[e] = x[base + e];
     for (int e = tid; e < a.ncoef_up; e += NT) coef[e] = a.coef_up[e];
     for (int e = tid; e < nq; e += NT) uimp[e] = a.uimp[e];
@@ -294,22 +292,9 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         }
         for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
     }
-    if (MODE == 1) grp_issue(lz.partB, lz.nB, grp);
+    __shared__ int cnt;
+    if (MODE == 1 && tid == 0) cnt = 0;
     __syncthreads();
-    double sj = 1.0, cprev = 0.0;
-    if (MODE == 1) {
-        const double bj = sqrt(grp_total(grp, lz.nB));
-        if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
-            if (blk == 0 && tid == 0) lz.st->status = (j == 1) ? 2 : 1;
-            return;
-        }
-        if (blk == 0 && tid == 0) {
-            lz.st->beta[j - 1] = bj;
-            lz.st->j_k2 = j;
-        }
-        sj = 1.0 / bj;
-        cprev = (j > 1) ? bj / lz.st->beta[j - 2] : 0.0;
-    }
 
     double dot = 0.0;
     while (r < dup) {
@@ -326,16 +311,8 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
                 acc += hval[t] * x[a.swap_up[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup];
             }
         }
-        const int idx = base + r;
-        if (MODE == 1) {
-            double yv = (double)acc * sj;
-            if (j > 1) yv -= cprev * (double)pv;
-            const T yt = (T)yv;
-            y[idx] = yt;
-            dot += (double)yt * ((double)xv * sj);
-        } else {
-            y[idx] = acc;
-        }
+        y[base + r] = acc;
+        if (MODE == 1) dot += (double)acc * (double)xv;
         if (ONE) break;
         // next row of this thread
         r += NT;
@@ -346,13 +323,12 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
             for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
             du = a.diag_up[r];
             cu = a.imp_up[r];
-            if (MODE == 1 && j > 1) pv = prev[base + r];
         }
     }
-    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+    if (MODE == 1) block_partial_last(dot, 0.0, red, &cnt, lz.partA + blk, nullptr);
 }
 
-// u_1 = start (converted to T); partial sums of ||u_1||^2; j = 1.
+// u_1 = start (converted to T); partial sums of ||u_1||^2 (and zero cross sums); j = 1.
 template <typename T>
 __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, int64_t n, int blk, int nblk,
                                     double *red)
@@ -366,7 +342,8 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
         u[i] = v;
         acc += (double)v * (double)v;
     }
-    block_partial(acc, red, lz.partB + blk);
+    block_partial(acc, red, lz.nrm(1) + blk);
+    if (threadIdx.x == 0) lz.crs(1)[blk] = 0.0;
     if (blk == 0 && threadIdx.x == 0) {
         lz.st->j_k1 = 1;
         lz.st->j_k2 = 0;
@@ -374,64 +351,108 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
     }
 }
 
-// Lanczos phase 2: alpha_j from the H.v partials, w -= alpha_j v_j, partial ||w||^2.
-// The first kPre element pairs of every thread are loaded before alpha_j is known.
+// Recurrence scalars of step j, identical in every workgroup of the update phase.  From
+// the fixed-order sums (one wave each, in parallel) of
+//   N = ||u_j||^2,  P = <u_j, u_{j-1}>   (update phase of step j-1, or initialisation)
+//   Q = <z, u_j>,   z = H u_j            (H.v phase of step j)
+// with v_j = u_j / beta_j:
+//   beta_j  = sqrt(N)                     (breakdown below kMinLanczosNorm, Lanczos_plain.h:70-92)
+//   alpha_j = <H v_j - beta_j v_{j-1}, v_j> = Q / beta_j^2 - P / beta_{j-1}
+// which is lanc_opt's alpha (taken after the beta-subtraction, Lanczos_plain.h:116-140)
+// rearranged so that the H.v phase needs no scalar of step j.  sh[] receives N, P, Q.
+template <typename T>
+__device__ inline void lz_scalars(const LzArgs<T> &lz, int j, double *sh)
+{
+    const int w = threadIdx.x >> 6;
+    if (w == 0) {
+        const double t = wave_part_sum(lz.nrm(j), lz.nB);
+        if (threadIdx.x == 0) sh[0] = t;
+    } else if (w == 1) {
+        const double t = j > 1 ? wave_part_sum(lz.crs(j), lz.nB) : 0.0;
+        if (threadIdx.x == 64) sh[1] = t;
+    } else if (w == 2) {
+        const double t = wave_part_sum(lz.partA, lz.nA);
+        if (threadIdx.x == 128) sh[2] = t;
+    }
+}
+
+// Lanczos phase 2 of step j (lanc_opt, Lanczos_plain.h:66-173): with z = H u_j in slot
+// j+1,  u_{j+1} = z / beta_j - (beta_j / beta_{j-1}) u_{j-1} - (alpha_j / beta_j) u_j,
+// written over z; partial sums of ||u_{j+1}||^2 and <u_{j+1}, u_j> for step j+1.  The
+// first kPre elements of every thread are loaded before the scalars are known.
 template <typename T>
 __device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advance, int blk, int nblk, double *red,
-                                      double *grp)
+                                      double *sh)
 {
     constexpr int kPre = 4;
     if (lz.st->status) return;
     const int j = lz.st->j_k2;
     T *__restrict__ y = lz.slot(j + 1);
     const T *__restrict__ x = lz.slot(j);
+    const T *__restrict__ pr = lz.slot(j > 1 ? j - 1 : j);
     int64_t i0, i1, st;
     elem_range(lz, n, blk, nblk, i0, i1, st);
-    T yv[kPre], xv[kPre];
+    T yv[kPre], xv[kPre], pv[kPre];
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
         const int64_t i = i0 + q * st;
         yv[q] = i < i1 ? y[i] : (T)0;
         xv[q] = i < i1 ? x[i] : (T)0;
+        pv[q] = (i < i1 && j > 1) ? pr[i] : (T)0;
     }
-    grp_issue(lz.partA, lz.nA, grp);
+    const double bprev = j > 1 ? lz.st->beta[j - 2] : 1.0;
+    lz_scalars(lz, j, sh);
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
-    const double alpha = grp_total(grp, lz.nA);
+    const double bj = sqrt(sh[0]);
     const bool lead = blk == 0 && threadIdx.x == 0;
-    if (lead) lz.st->alpha[j - 1] = alpha;
-    const double c = alpha / lz.st->beta[j - 1];
-    double acc = 0.0;
+    if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+        if (lead) lz.st->status = (j == 1) ? 2 : 1;
+        return;
+    }
+    const double alpha = sh[2] / (bj * bj) - (j > 1 ? sh[1] / bprev : 0.0);
+    if (lead) {
+        lz.st->beta[j - 1] = bj;
+        lz.st->alpha[j - 1] = alpha;
+    }
+    const double cz = 1.0 / bj, cp = j > 1 ? bj / bprev : 0.0, cx = alpha / bj;
+    double nn = 0.0, cr = 0.0;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
         const int64_t i = i0 + q * st;
         if (i < i1) {
-            const T v = (T)((double)yv[q] - c * (double)xv[q]);
+            const T v = (T)(cz * (double)yv[q] - cp * (double)pv[q] - cx * (double)xv[q]);
             y[i] = v;
-            acc += (double)v * (double)v;
+            nn += (double)v * (double)v;
+            cr += (double)v * (double)xv[q];
         }
     }
     for (int64_t i = i0 + kPre * st; i < i1; i += st) {
-        const T v = (T)((double)y[i] - c * (double)x[i]);
+        const double xi = (double)x[i];
+        const T v = (T)(cz * (double)y[i] - cp * (j > 1 ? (double)pr[i] : 0.0) - cx * xi);
         y[i] = v;
-        acc += (double)v * (double)v;
+        nn += (double)v * (double)v;
+        cr += (double)v * xi;
     }
-    block_partial(acc, red, lz.partB + blk);
+    block_partial_last(nn, cr, red, &cnt, lz.nrm(j + 1) + blk, lz.crs(j + 1) + blk);
     if (advance && lead) lz.st->j_k1 = j + 1;
 }
 
-// End of a chunk: beta_j of the next step from the update partials (one workgroup), the
-// same reduction the next H.v phase performs.
+// End of a chunk: beta_j of the next step j from the update partials (one workgroup),
+// the same reduction the next update phase performs, so the host sees those bits.
 template <typename T>
-__device__ inline void lz_finish_body(const LzArgs<T> &lz, double *grp)
+__device__ inline void lz_finish_body(const LzArgs<T> &lz, double *sh)
 {
     if (lz.st->status) return;
     const int j = lz.st->j_k1;
-    grp_issue(lz.partB, lz.nB, grp);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double b = sqrt(grp_total(grp, lz.nB));
-        lz.st->beta[j - 1] = b;
-        if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+    if (threadIdx.x < 64) {
+        const double t = wave_part_sum(lz.nrm(j), lz.nB);
+        if (threadIdx.x == 0) {
+            const double b = sqrt(t);
+            lz.st->beta[j - 1] = b;
+            if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
+        }
     }
 }
 

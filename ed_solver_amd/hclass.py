@@ -130,9 +130,10 @@ class Sector:
                                            ctypes.c_void_p(in_ptr), ctypes.c_void_p(stream) if stream else None))
 
     def tune(self) -> tuple[int, list]:
-        """Runs the H.v schedule autotuner; returns (choice 1/2/4 row kernel or 8 fast kernel, ms per choice)."""
+        """Runs the H.v schedule autotuner; returns (choice, ms per choice) with choices 1/2/4 (row
+        kernel), 8 (fast kernel) and 16 (fast kernel, XCD-grouped column order)."""
         c = ctypes.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         check(lib().ed_hmult_tune(self.handle, ctypes.byref(c), dptr(ms)))
         return c.value, ms.tolist()
 

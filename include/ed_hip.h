@@ -121,14 +121,16 @@ int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, doub
 
 /* Autotuner of the H.v launch schedule (no reference counterpart): times the FP64
  * H.v on this sector under each applicable schedule -- the row kernel with 1, 2 or 4
- * columns per workgroup, and the fast kernel (one column per workgroup, up to 1024
- * threads, packed tables in LDS) -- and keeps the fastest for every later H.v and
- * Lanczos call on the sector.  *choice = 1, 2, 4 (row kernel) or 8 (fast kernel);
- * ms_per_choice[4] (may be NULL) = measured times in that order (0 where a schedule
- * does not apply).  Lanczos calls run it on first use. */
+ * columns per workgroup, the fast kernel (one column per workgroup, up to 1024
+ * threads, packed tables in LDS) and the fast kernel with its workgroups mapped to
+ * columns in an XCD-grouped order (neighbouring columns of the down-hop graph run on
+ * the same XCD and share its L2) -- and keeps the fastest for every later H.v and
+ * Lanczos call on the sector.  *choice = 1, 2, 4 (row kernel), 8 (fast kernel) or 16
+ * (fast kernel, XCD-grouped order); ms_per_choice[5] (may be NULL) = measured times in
+ * that order (0 where a schedule does not apply).  Lanczos calls run it on first use. */
 int ed_hmult_tune(ed_sector *sec, int *choice, double *ms_per_choice);
 
-/* Forces one H.v schedule (1, 2, 4 or 8 as above) on this sector, e.g. to test or time
+/* Forces one H.v schedule (1, 2, 4, 8 or 16 as above) on this sector, e.g. to test or time
  * a schedule the autotuner did not pick.  No reference counterpart. */
 int ed_hmult_set_schedule(ed_sector *sec, int choice);
 
@@ -240,7 +242,7 @@ int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *p
  * `reps` steps from the default start vector, launching its kernels one by one with
  * a HIP event pair around every fused H.v kernel on the sector's stream.
  * *avg_ms = average duration of that kernel; *bytes_per_launch = its algorithmic
- * HBM bytes 3 * sizeof(T) * dimen_H() (read v_j and v_{j-1}, write w).  No
+ * HBM bytes 2 * sizeof(T) * dimen_H() (read u_j, write H u_j).  No
  * reference counterpart: it is the measurement hook of the engine. */
 int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, double *bytes_per_launch);
 

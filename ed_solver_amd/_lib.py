@@ -34,6 +34,11 @@ class LanczosOpts(ctypes.Structure):
                 ("precision", ctypes.c_int), ("check_every", ctypes.c_int), ("reorth", ctypes.c_int)]
 
 
+class BlockOpts(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int), ("nev", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("tolerance", ctypes.c_double)]
+
+
 class EDError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[ed_hip status {code}] {msg}")
@@ -73,6 +78,16 @@ SIGNATURES = {
     "ed_lanczos_gs_vector_dev": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _vp]),
     "ed_rayleigh_f64": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp]),
     "ed_tridiag_eig": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "ed_band_eig": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
+    "ed_block_default_opts": (ctypes.c_int, [ctypes.POINTER(BlockOpts)]),
+    "ed_block_lanczos": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _dp, ctypes.c_int64, _c_int_p,
+                                        _c_int_p]),
+    "ed_block_lanczos_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _vp, ctypes.c_int64, _c_int_p,
+                                            _c_int_p]),
+    "ed_gram_f64_dev": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp,
+                                       ctypes.c_int64, _dp, _vp]),
+    "ed_xc_f64_dev": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _dp,
+                                     ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int64, _vp]),
     "ed_apply_c": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_continued_fraction": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_dynamic": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,

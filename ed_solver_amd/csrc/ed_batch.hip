@@ -27,12 +27,6 @@ int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, do
 int cfrac_batch_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
                     const double *d_scale, double *d_out, hipStream_t st);
 
-#define ED_TRY(x)                     \
-    do {                              \
-        int rc_ = (x);                \
-        if (rc_ != ED_OK) return rc_; \
-    } while (0)
-
 template <typename T>
 struct JobDesc {
     HvArgs<T> a;

@@ -1,0 +1,564 @@
+// Block Lanczos (Block_Lanczos_diagonalize, Block_Lanczos.f90:37-355) on the device, and
+// the FP64 MFMA tall-skinny contractions its orthogonalisation is built from.
+//
+// The reference hands H.U products to BLZPACK (BLZDRD, reverse communication: LFLAG = 1
+// asks for V = H U on NVOPU <= Block_size vectors, LFLAG = 4 for start vectors, LFLAG = 0
+// returns converged eigenpairs VALP/VECP, Block_Lanczos.f90:189-259).  BLZPACK is not
+// part of the reference sources; this driver is a block Lanczos of its own, written from
+// the textbook method (Golub and Underwood 1977; Golub and Van Loan, Matrix
+// Computations, sec. 10.3.6) with full reorthogonalisation:
+//
+//   Q_1 R_0 = start block (n x b, Cholesky-QR2)
+//   for j = 1, 2, ...:
+//     W  = H Q_j                                   block H.v (b vectors per launch)
+//     W -= Q_{1..j} (Q_{1..j}^T W), twice          classical Gram-Schmidt, CGS2
+//     A_j = the Q_j rows of the two projections     diagonal block of T
+//     Q_{j+1} B_{j+1} = W                           Cholesky-QR2; B_{j+1} upper triangular
+//     T = block tridiagonal (A_j, B_j), half-bandwidth b; its lowest Ritz pairs from the
+//     band eigen-solver on the host; stop when every wanted Ritz value has moved by less
+//     than `tolerance` since the previous test and its residual bound
+//     ||B_{j+1} s_k(last b rows)|| is below sqrt(tolerance) * max(1, |theta_k|), or when
+//     the block steps or the Krylov space run out.
+//   Ritz vectors X = Q_{1..m} S, residuals ||H x_k - theta_k x_k|| on the device.
+//
+// Contractions (FP64, v_mfma_f64_16x16x4f64, one wave per 16 x 16 output tile):
+//   Gram    G = X^T Y   (X n x p, Y n x q, q <= 16): the reduction over n is split in
+//           row blocks; each workgroup writes its partial tile sums, a second kernel adds
+//           them in row-block order (deterministic);
+//   update  Y = beta Y + alpha X C  (C p x q): each wave owns 64 rows of Y and runs the
+//           reduction over p in MFMA steps of 4.
+// Both are HBM-bound (about b/4 flop per byte); MFMA keeps the instruction count low.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ed_kernels.h"
+
+namespace ed {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16) in FP64 on the matrix cores.  Lane l (0..63) supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; it holds D[4 * r + l / 16][l % 16] in
+// element r of the accumulator (layout checked against NumPy by the GPU tests).
+__device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kGramTiles = 4;    // 16-column tiles of X per Gram workgroup
+constexpr int kGramRowBlocks = 256;
+
+// 16 consecutive rows [r, r + 16) of column `col` of a column-major matrix, zero
+// outside [0, n) or when !ok.
+__device__ inline void load_rows16(const double *m, int64_t ld, bool ok, int col, int64_t r, int64_t n, double *v)
+{
+    if (ok && r + 16 <= n) {
+        const double2 *p = reinterpret_cast<const double2 *>(m + (int64_t)col * ld + r);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const double2 t = p[s];
+            v[2 * s] = t.x;
+            v[2 * s + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) v[s] = (ok && r + s < n) ? m[(int64_t)col * ld + r + s] : 0.0;
+    }
+}
+
+// Partial Gram sums: workgroup (tile group, row block); part[rb][p16][16].  Row block rb
+// covers rows [rb * rows, (rb + 1) * rows), rows a multiple of 256; wave w takes the
+// 64-row chunks w, w + 4, ...; lane group g (l / 16) takes rows 16 g .. 16 g + 15 of a
+// chunk, one row per MFMA step, so each lane reads 128 contiguous bytes of a column.
+__global__ __launch_bounds__(256) void k_gram_part(int64_t n, int p, int q, const double *X, int64_t ldx,
+                                                   const double *Y, int64_t ldy, int64_t rows, int p16, double *part)
+{
+    __shared__ double red[4][kGramTiles][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i16 = lane & 15, g = lane >> 4;
+    const int ntile = p16 / 16, t0 = blockIdx.x * kGramTiles;
+    const int64_t r_begin = (int64_t)blockIdx.y * rows, r_end = min(n, r_begin + rows);
+    dbl4 acc[kGramTiles];
+#pragma unroll
+    for (int t = 0; t < kGramTiles; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t c0 = r_begin + 64 * w; c0 < r_end; c0 += 256) {
+        const int64_t r = c0 + 16 * g;
+        double yv[16];
+        load_rows16(Y, ldy, i16 < q, i16, r, n, yv);
+#pragma unroll
+        for (int t = 0; t < kGramTiles; ++t) {
+            const int tile = t0 + t;
+            if (tile >= ntile) break;
+            const int xc = tile * 16 + i16;
+            double xv[16];
+            load_rows16(X, ldx, xc < p, xc, r, n, xv);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) acc[t] = mfma_f64_16x16x4(xv[s], yv[s], acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kGramTiles; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[w][t][lane * 4 + r] = acc[t][r];
+    __syncthreads();
+    const int e = threadIdx.x, el = e >> 2, er = e & 3;
+    const int i = 4 * er + (el >> 4), jj = el & 15;
+    for (int t = 0; t < kGramTiles; ++t) {
+        const int tile = t0 + t;
+        if (tile >= ntile) break;
+        const double s = ((red[0][t][e] + red[1][t][e]) + red[2][t][e]) + red[3][t][e];
+        part[((int64_t)blockIdx.y * p16 + tile * 16 + i) * 16 + jj] = s;
+    }
+}
+
+// G[i][j] (row-major p x q) = sum over row blocks, in order, of the partial tiles.
+__global__ void k_gram_sum(int nrb, int p, int q, int p16, const double *part, double *G)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p * q) return;
+    const int i = e / q, j = e - i * q;
+    double s = 0.0;
+    for (int b = 0; b < nrb; ++b) s += part[((int64_t)b * p16 + i) * 16 + j];
+    G[e] = s;
+}
+
+// Y[:, 0:q] = beta Y + alpha X[:, 0:p] C (C row-major p x q, q <= 16).  Each wave owns
+// rows [r0, r0 + 64) as four 16-row tiles; MFMA step k covers columns k0 .. k0 + 3 of X:
+// lane l reads X[r0 + 16 t + l % 16][k0 + l / 16] (16 contiguous rows per lane group)
+// and C[k0 + l / 16][l % 16].  Y may alias X when p == q (every wave reads its rows of X
+// before it writes them).
+__global__ __launch_bounds__(256) void k_xc(int64_t n, int p, int q, const double *X, int64_t ldx, const double *C,
+                                            double alpha, double beta, double *Y, int64_t ldy)
+{
+    const int lane = threadIdx.x & 63, i16 = lane & 15, g = lane >> 4;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (r0 >= n) return;
+    dbl4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const bool full = r0 + 64 <= n;
+    for (int k0 = 0; k0 < p; k0 += 16) {
+        double a[4][4], bcol[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 4 * u + g;
+            const bool kok = k < p;
+            bcol[u] = (kok && i16 < q) ? C[(int64_t)k * q + i16] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int64_t r = r0 + 16 * t + i16;
+                a[u][t] = (kok && (full || r < n)) ? X[(int64_t)k * ldx + r] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma_f64_16x16x4(a[u][t], bcol[u], acc[t]);
+    }
+    if (i16 >= q) return;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = r0 + 16 * t + 4 * r + g;
+            if (row < n) {
+                double *y = Y + (int64_t)i16 * ldy + row;
+                const double v = alpha * acc[t][r];
+                *y = beta == 0.0 ? v : beta * *y + v;
+            }
+        }
+}
+
+// Deterministic start block in (-1, 1): the hash of (row mod 10000, column).
+__global__ void k_block_start(int64_t n, int b, double *Q, int64_t ld)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < b; ++c) {
+            uint64_t x = ((uint64_t)(i % 10000) + 1ull + 10007ull * (uint64_t)c) * 0x9E3779B97F4A7C15ull;
+            x ^= x >> 29;
+            x *= 0xBF58476D1CE4E5B9ull;
+            x ^= x >> 32;
+            Q[(int64_t)c * ld + i] = (double)(x >> 11) / 4503599627370496.0 - 1.0;
+        }
+}
+
+// ---------------------------------------------------------------- host side
+
+// Work space of the contractions: Gram partials and the small matrices, per device,
+// grown on demand (the block driver and the C-ABI contraction calls share it).
+struct BlockWS {
+    double *part = nullptr, *small = nullptr;
+    size_t part_n = 0, small_n = 0;
+    int device = -1;
+};
+static BlockWS g_bws[16];
+
+static int bws_get(int dev, size_t part_n, size_t small_n, BlockWS *&w)
+{
+    if (dev < 0 || dev >= 16) return fail(ED_ERR_ARG, "device index out of range");
+    w = &g_bws[dev];
+    if (w->part_n < part_n) {
+        if (w->part) (void)hipFree(w->part);
+        w->part = nullptr;
+        w->part_n = 0;
+        ED_HIP_CHECK(hipMalloc((void **)&w->part, sizeof(double) * part_n));
+        w->part_n = part_n;
+    }
+    if (w->small_n < small_n) {
+        if (w->small) (void)hipFree(w->small);
+        w->small = nullptr;
+        w->small_n = 0;
+        ED_HIP_CHECK(hipMalloc((void **)&w->small, sizeof(double) * small_n));
+        w->small_n = small_n;
+    }
+    return ED_OK;
+}
+
+static int gram_rows(int64_t n)
+{
+    const int64_t per = (n + kGramRowBlocks - 1) / kGramRowBlocks;
+    return (int)std::max<int64_t>(256, (per + 255) / 256 * 256);
+}
+
+// G (host, row-major p x q) = X^T Y.
+static int gram(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                double *G, hipStream_t st)
+{
+    const int p16 = (p + 15) / 16 * 16;
+    const int64_t rows = gram_rows(n);
+    const int nrb = (int)((n + rows - 1) / rows);
+    BlockWS *w;
+    ED_TRY(bws_get(dev, (size_t)nrb * p16 * 16, (size_t)p * q + 256, w));
+    const dim3 grid((p16 / 16 + kGramTiles - 1) / kGramTiles, nrb);
+    hipLaunchKernelGGL(k_gram_part, grid, dim3(256), 0, st, n, p, q, X, ldx, Y, ldy, rows, p16, w->part);
+    hipLaunchKernelGGL(k_gram_sum, dim3((p * q + 255) / 256), dim3(256), 0, st, nrb, p, q, p16,
+                       (const double *)w->part, w->small);
+    ED_HIP_CHECK(hipGetLastError());
+    ED_HIP_CHECK(hipMemcpyAsync(G, w->small, sizeof(double) * p * q, hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
+// Y = beta Y + alpha X C with C given on the host (row-major p x q).
+static int xc(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *C, double alpha,
+              double beta, double *Y, int64_t ldy, hipStream_t st)
+{
+    BlockWS *w;
+    ED_TRY(bws_get(dev, 16, (size_t)p * q + 256, w));
+    ED_HIP_CHECK(hipMemcpyAsync(w->small, C, sizeof(double) * p * q, hipMemcpyHostToDevice, st));
+    const int64_t waves = (n + 63) / 64;
+    hipLaunchKernelGGL(k_xc, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, n, p, q, X, ldx,
+                       (const double *)w->small, alpha, beta, Y, ldy);
+    ED_HIP_CHECK(hipGetLastError());
+    // the small buffer is reused by the next call: finish this one first
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
+// Upper-triangular R with G = R^T R (G b x b row-major, symmetric); false when G is not
+// numerically positive definite relative to its largest diagonal entry.
+static bool cholesky_upper(int b, const std::vector<double> &G, std::vector<double> &R)
+{
+    R.assign((size_t)b * b, 0.0);
+    double gmax = 0.0;
+    for (int i = 0; i < b; ++i) gmax = std::max(gmax, G[(size_t)i * b + i]);
+    if (!(gmax > 0.0)) return false;
+    for (int i = 0; i < b; ++i) {
+        double s = G[(size_t)i * b + i];
+        for (int k = 0; k < i; ++k) s -= R[(size_t)k * b + i] * R[(size_t)k * b + i];
+        if (!(s > 1e-26 * gmax)) return false;
+        const double rii = std::sqrt(s);
+        R[(size_t)i * b + i] = rii;
+        for (int j = i + 1; j < b; ++j) {
+            double t = G[(size_t)i * b + j];
+            for (int k = 0; k < i; ++k) t -= R[(size_t)k * b + i] * R[(size_t)k * b + j];
+            R[(size_t)i * b + j] = t / rii;
+        }
+    }
+    return true;
+}
+
+// Inverse of an upper-triangular b x b matrix (row-major).
+static std::vector<double> upper_inverse(int b, const std::vector<double> &R)
+{
+    std::vector<double> Ri((size_t)b * b, 0.0);
+    for (int j = 0; j < b; ++j) {
+        Ri[(size_t)j * b + j] = 1.0 / R[(size_t)j * b + j];
+        for (int i = j - 1; i >= 0; --i) {
+            double s = 0.0;
+            for (int k = i + 1; k <= j; ++k) s += R[(size_t)i * b + k] * Ri[(size_t)k * b + j];
+            Ri[(size_t)i * b + j] = -s / R[(size_t)i * b + i];
+        }
+    }
+    return Ri;
+}
+
+// In place W <- W R^{-1} with W^T W = R^T R (Cholesky-QR, twice); Rtot = R2 R1.
+// Returns false when the block is numerically rank deficient.
+static int chol_qr2(int dev, int64_t n, int b, double *W, int64_t ld, std::vector<double> &Rtot, bool &ok,
+                    hipStream_t st)
+{
+    std::vector<double> G((size_t)b * b), R1, R2;
+    ED_TRY(gram(dev, n, b, b, W, ld, W, ld, G.data(), st));
+    ok = cholesky_upper(b, G, R1);
+    if (!ok) return ED_OK;
+    ED_TRY(xc(dev, n, b, b, W, ld, upper_inverse(b, R1).data(), 1.0, 0.0, W, ld, st));
+    ED_TRY(gram(dev, n, b, b, W, ld, W, ld, G.data(), st));
+    ok = cholesky_upper(b, G, R2);
+    if (!ok) return ED_OK;
+    ED_TRY(xc(dev, n, b, b, W, ld, upper_inverse(b, R2).data(), 1.0, 0.0, W, ld, st));
+    Rtot.assign((size_t)b * b, 0.0);
+    for (int i = 0; i < b; ++i)
+        for (int j = i; j < b; ++j) {
+            double s = 0.0;
+            for (int k = i; k <= j; ++k) s += R2[(size_t)i * b + k] * R1[(size_t)k * b + j];
+            Rtot[(size_t)i * b + j] = s;
+        }
+    return ED_OK;
+}
+
+int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st);
+
+struct BlockResult {
+    std::vector<double> evals, resid;
+    int steps = 0, nconv = 0;
+};
+
+// The block Lanczos driver; Ritz vectors go to d_out (column-major n x nev, leading
+// dimension ldo) when d_out is not null.
+static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, int64_t ldo, BlockResult &res)
+{
+    const int64_t n = s->dim;
+    const int b = o.block_size, nev = o.nev, dev = s->device;
+    hipStream_t st = s->stream;
+    if (!s->cpb) {   // H.v schedule: the autotuner runs with the first Lanczos call
+        int choice;
+        double ms[5];
+        ED_TRY(ed_hmult_tune(s, &choice, ms));
+    }
+    const int64_t ld = (n + 63) / 64 * 64;
+    const int maxsteps = (int)std::max<int64_t>(1, std::min<int64_t>(o.max_steps, (n + b - 1) / b));
+    if ((int64_t)maxsteps * b < nev) return fail(ED_ERR_ARG, "block Lanczos: max_steps * block_size < nev");
+    const size_t nvec = (size_t)(maxsteps + 1) * b;
+    double *Q = nullptr, *HX = nullptr;
+    std::string err;
+    {
+        hipError_t e = hipMalloc((void **)&Q, sizeof(double) * ld * nvec);
+        if (e != hipSuccess)
+            return fail(ED_ERR_NOMEM, "block Lanczos basis of " + std::to_string(nvec) + " vectors: " + hipGetErrorString(e));
+    }
+    struct Guard {
+        double *&a, *&b;
+        ~Guard()
+        {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        }
+    } guard{Q, HX};
+    hipLaunchKernelGGL(k_block_start, dim3(1024), dim3(256), 0, st, n, b, Q, ld);
+    ED_HIP_CHECK(hipGetLastError());
+    std::vector<double> R0;
+    bool ok = true;
+    ED_TRY(chol_qr2(dev, n, b, Q, ld, R0, ok, st));
+    if (!ok) return fail(ED_ERR_BREAKDOWN, "block Lanczos: start block is rank deficient");
+
+    const int nmax = maxsteps * b;
+    std::vector<double> T((size_t)nmax * nmax, 0.0);
+    auto Tat = [&](int i, int j) -> double & { return T[(size_t)i * nmax + j]; };
+    std::vector<double> G1, G2, Bnext, prev(nev, 1e300), w(nev), Z;
+    int steps = 0, done = 0;
+    bool exhausted = false;
+    std::vector<double> lastB;
+    for (int j = 0; j < maxsteps && !done; ++j) {
+        const int p = (j + 1) * b;   // columns of Q_{1..j+1}
+        double *Qj = Q + (size_t)j * b * ld, *W = Q + (size_t)(j + 1) * b * ld;
+        ED_TRY(launch_hv(s, ED_PREC_F64, Qj, W, ld, b, st));
+        G1.assign((size_t)p * b, 0.0);
+        G2.assign((size_t)p * b, 0.0);
+        ED_TRY(gram(dev, n, p, b, Q, ld, W, ld, G1.data(), st));
+        ED_TRY(xc(dev, n, p, b, Q, ld, G1.data(), -1.0, 1.0, W, ld, st));
+        ED_TRY(gram(dev, n, p, b, Q, ld, W, ld, G2.data(), st));
+        ED_TRY(xc(dev, n, p, b, Q, ld, G2.data(), -1.0, 1.0, W, ld, st));
+        for (int r = 0; r < b; ++r)   // A_j, symmetrised
+            for (int c = 0; c < b; ++c) {
+                const double a1 = G1[(size_t)(j * b + r) * b + c] + G2[(size_t)(j * b + r) * b + c];
+                const double a2 = G1[(size_t)(j * b + c) * b + r] + G2[(size_t)(j * b + c) * b + r];
+                Tat(j * b + r, j * b + c) = 0.5 * (a1 + a2);
+            }
+        steps = j + 1;
+        bool last = (j + 1 == maxsteps) || (int64_t)p + b > n;
+        if (!last) {
+            ED_TRY(chol_qr2(dev, n, b, W, ld, Bnext, ok, st));
+            if (!ok) {
+                exhausted = true;   // the Krylov space is invariant: T is exact on it
+                last = true;
+            } else {
+                for (int r = 0; r < b; ++r)
+                    for (int c = 0; c < b; ++c) {
+                        Tat((j + 1) * b + r, j * b + c) = Bnext[(size_t)r * b + c];
+                        Tat(j * b + c, (j + 1) * b + r) = Bnext[(size_t)r * b + c];
+                    }
+                lastB = Bnext;
+            }
+        }
+        const int m = p;
+        if (m < nev) continue;
+        if (!last && (steps % std::max(1, o.check_every)) != 0) continue;
+        // Ritz values and vectors of T_j (leading m x m block)
+        std::vector<double> Tm((size_t)m * m);
+        for (int r = 0; r < m; ++r)
+            for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
+        Z.assign((size_t)m * nev, 0.0);
+        ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), Z.data()));
+        if (last) {
+            done = 1;
+            break;
+        }
+        bool conv = steps > 1;
+        for (int k = 0; k < nev && conv; ++k) {
+            // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
+            double rr = 0.0;
+            for (int r = 0; r < b; ++r) {
+                double t = 0.0;
+                for (int c = r; c < b; ++c) t += lastB[(size_t)r * b + c] * Z[(size_t)k * m + (m - b + c)];
+                rr += t * t;
+            }
+            const double rnorm = std::sqrt(rr);
+            if (!(std::fabs(w[k] - prev[k]) < o.tolerance && rnorm < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))))
+                conv = false;
+        }
+        for (int k = 0; k < nev; ++k) prev[k] = w[k];
+        if (conv) done = 1;
+    }
+    const int m = steps * b;
+    (void)exhausted;
+    // Ritz vectors X = Q_{1..m} S and the true residuals
+    res.evals.assign(w.begin(), w.end());
+    res.steps = steps;
+    res.resid.assign(nev, 0.0);
+    {
+        hipError_t e = hipMalloc((void **)&HX, sizeof(double) * ld * 2 * nev);
+        if (e != hipSuccess) return fail(ED_ERR_NOMEM, std::string("block Lanczos Ritz vectors: ") + hipGetErrorString(e));
+    }
+    double *X = HX + (size_t)nev * ld;
+    std::vector<double> S((size_t)m * nev);   // row-major m x nev
+    for (int k = 0; k < nev; ++k)
+        for (int r = 0; r < m; ++r) S[(size_t)r * nev + k] = Z[(size_t)k * m + r];
+    ED_TRY(xc(dev, n, m, nev, Q, ld, S.data(), 1.0, 0.0, X, ld, st));
+    ED_TRY(launch_hv(s, ED_PREC_F64, X, HX, ld, nev, st));
+    std::vector<double> D((size_t)nev * nev, 0.0), RtR((size_t)nev * nev);
+    for (int k = 0; k < nev; ++k) D[(size_t)k * nev + k] = w[k];
+    ED_TRY(xc(dev, n, nev, nev, X, ld, D.data(), -1.0, 1.0, HX, ld, st));
+    ED_TRY(gram(dev, n, nev, nev, HX, ld, HX, ld, RtR.data(), st));
+    int nconv = 0;
+    for (int k = 0; k < nev; ++k) {
+        res.resid[k] = std::sqrt(std::max(0.0, RtR[(size_t)k * nev + k]));
+        if (res.resid[k] < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))) ++nconv;
+    }
+    res.nconv = nconv;
+    if (d_out)
+        ED_HIP_CHECK(hipMemcpy2DAsync(d_out, sizeof(double) * ldo, X, sizeof(double) * ld, sizeof(double) * n, nev,
+                                      hipMemcpyDeviceToDevice, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
+}  // namespace ed
+
+extern "C" {
+
+int ed_block_default_opts(ed_block_opts *o)
+{
+    if (!o) return ed::fail(ED_ERR_ARG, "ed_block_default_opts: NULL");
+    o->block_size = 8;
+    o->nev = 8;
+    o->max_steps = 128;
+    o->check_every = 2;
+    o->tolerance = 1e-12;
+    return ED_OK;
+}
+
+static int block_check(ed_sector *sec, const ed_block_opts *o)
+{
+    if (!sec || !o) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: NULL argument");
+    if (o->block_size < 1 || o->block_size > 16) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: block_size must be 1..16");
+    if (o->nev < 1 || o->nev > 16 || o->nev > sec->dim)
+        return ed::fail(ED_ERR_ARG, "ed_block_lanczos: nev must be 1..16 and at most the sector dimension");
+    if (o->max_steps < 1) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: max_steps < 1");
+    if (!(o->tolerance > 0.0)) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: tolerance must be positive");
+    if ((int64_t)o->block_size > sec->dim) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: block_size exceeds the dimension");
+    ED_HIP_CHECK(hipSetDevice(sec->device));
+    return ED_OK;
+}
+
+int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *evecs,
+                     int64_t ldv, int *nconv, int *steps)
+{
+    ED_TRY(block_check(sec, opts));
+    if (!evals) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: NULL evals");
+    if (evecs && ldv < sec->dim) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: ldv < dimension");
+    ed::BlockResult r;
+    double *d_out = nullptr;
+    if (evecs) {
+        hipError_t e = hipMalloc((void **)&d_out, sizeof(double) * sec->dim * opts->nev);
+        if (e != hipSuccess) return ed::fail(ED_ERR_NOMEM, "ed_block_lanczos: eigenvector staging");
+    }
+    int rc = ed::block_lanczos(sec, *opts, d_out, sec->dim, r);
+    if (rc == ED_OK && evecs) {
+        hipError_t e = hipMemcpy2D(evecs, sizeof(double) * ldv, d_out, sizeof(double) * sec->dim,
+                                   sizeof(double) * sec->dim, opts->nev, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = ed::fail(ED_ERR_HIP, std::string("ed_block_lanczos: copy-out: ") + hipGetErrorString(e));
+    }
+    if (d_out) (void)hipFree(d_out);
+    if (rc != ED_OK) return rc;
+    for (int k = 0; k < opts->nev; ++k) {
+        evals[k] = r.evals[k];
+        if (resid) resid[k] = r.resid[k];
+    }
+    if (nconv) *nconv = r.nconv;
+    if (steps) *steps = r.steps;
+    return ED_OK;
+}
+
+int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *d_evecs,
+                         int64_t ldv, int *nconv, int *steps)
+{
+    ED_TRY(block_check(sec, opts));
+    if (!evals || !d_evecs || ldv < sec->dim) return ed::fail(ED_ERR_ARG, "ed_block_lanczos_dev: bad argument");
+    ed::BlockResult r;
+    ED_TRY(ed::block_lanczos(sec, *opts, d_evecs, ldv, r));
+    for (int k = 0; k < opts->nev; ++k) {
+        evals[k] = r.evals[k];
+        if (resid) resid[k] = r.resid[k];
+    }
+    if (nconv) *nconv = r.nconv;
+    if (steps) *steps = r.steps;
+    return ED_OK;
+}
+
+int ed_gram_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, const double *d_y, int64_t ldy, double *g,
+                    void *stream)
+{
+    if (n < 1 || p < 1 || q < 1 || q > 16 || !d_x || !d_y || !g || ldx < n || ldy < n)
+        return ed::fail(ED_ERR_ARG, "ed_gram_f64_dev: bad argument");
+    if ((ldx & 1) || (ldy & 1) || ((uintptr_t)d_x & 15) || ((uintptr_t)d_y & 15))
+        return ed::fail(ED_ERR_ARG, "ed_gram_f64_dev: pointers must be 16-byte aligned and leading dimensions even");
+    int dev = 0;
+    ED_HIP_CHECK(hipGetDevice(&dev));
+    return ed::gram(dev, n, p, q, d_x, ldx, d_y, ldy, g, (hipStream_t)stream);
+}
+
+int ed_xc_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, const double *c, double alpha, double beta,
+                  double *d_y, int64_t ldy, void *stream)
+{
+    if (n < 1 || p < 1 || q < 1 || q > 16 || !d_x || !d_y || !c || ldx < n || ldy < n)
+        return ed::fail(ED_ERR_ARG, "ed_xc_f64_dev: bad argument");
+    if (d_x == d_y && (p != q || ldx != ldy)) return ed::fail(ED_ERR_ARG, "ed_xc_f64_dev: in place needs p == q");
+    int dev = 0;
+    ED_HIP_CHECK(hipGetDevice(&dev));
+    return ed::xc(dev, n, p, q, d_x, ldx, c, alpha, beta, d_y, ldy, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -36,12 +36,6 @@
 
 namespace ed {
 
-#define ED_TRY(x)                     \
-    do {                              \
-        int rc_ = (x);                \
-        if (rc_ != ED_OK) return rc_; \
-    } while (0)
-
 __device__ inline bool block0() { return blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0; }
 
 // ------------------------------------------------------------------ H.v kernels

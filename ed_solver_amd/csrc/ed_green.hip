@@ -22,12 +22,6 @@ namespace ed {
 int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, double *diag, double *subdiag,
                     int *breakdown_at);
 
-#define ED_TRY(x)                     \
-    do {                              \
-        int rc_ = (x);                \
-        if (rc_ != ED_OK) return rc_; \
-    } while (0)
-
 // For every destination spin state t: src[t] = source rank (or -1), sg[t] = operator sign.
 // c+ (pm > 0): t must hold `bit`, source = t without it; c (pm < 0): t must not hold it,
 // source = t with it.  The sign counts occupied orbitals above `bit` in t

@@ -21,6 +21,12 @@ int fail(int code, const std::string &msg);
             return ::ed::fail(ED_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define ED_TRY(x)                     \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != ED_OK) return rc_; \
+    } while (0)
+
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kRed = 34;             // reduction scratch: two slots per wave of a 1024-thread workgroup + 2
 constexpr int kSums = 4;             // LDS slots for the reduced recurrence sums
@@ -174,6 +180,9 @@ namespace ed {
 // tridiagonal eigen-solver (host, FP64): implicit QL with Wilkinson shifts
 int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z);
 void tridiag_lowest(int n, const double *diag, const double *off, int k, double *w);
+// nev lowest eigenpairs of a symmetric band matrix (n x n row-major, half-bandwidth b);
+// z (n x nev, column-major) may be null
+int band_lowest(int n, int b, const double *a, int nev, double *w, double *z);
 // launch helpers implemented in ed_engine.hip
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st);
 HvArgs<double> hv_args_f64(const ed_sector *s);

@@ -205,12 +205,6 @@ static int dalloc(T **p, size_t n)
     return ED_OK;
 }
 
-#define ED_TRY(x)              \
-    do {                       \
-        int rc_ = (x);         \
-        if (rc_ != ED_OK) return rc_; \
-    } while (0)
-
 static void free_spin(SpinTables &t)
 {
     void *ps[] = {t.states, t.diag, t.noff, t.ptr, t.rk, t.off, t.ell_rk, t.ell_val, t.imp, t.swap,

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "ed_internal.h"
@@ -109,9 +110,193 @@ void tridiag_lowest(int n, const double *diag, const double *off, int k, double 
     for (int i = n; i < k; ++i) w[i] = 1e300;
 }
 
+// ---------------------------------------------------------------- band matrices
+// Lowest eigenpairs of the symmetric band matrix of a block Lanczos run (half-bandwidth
+// b = block size), which takes the role of BLZPACK's Ritz computation inside BLZDRD
+// (Block_Lanczos.f90:132,195; the library is not part of the reference sources).
+//  1. Reduction to tridiagonal form by Givens rotations that annihilate the band column
+//     by column and chase each bulge down the band (Schwarz's method; Golub and Van
+//     Loan, Matrix Computations, sec. 8.3): O(n^2 b) operations, rotations logged.
+//  2. Eigenvalues of the tridiagonal matrix by implicit QL.
+//  3. Eigenvectors of the tridiagonal matrix by inverse iteration with partial pivoting
+//     (Wilkinson's method; Gram-Schmidt inside clusters of close eigenvalues), carried
+//     back through the logged rotations.
+
+namespace {
+
+struct Rot {
+    int p;
+    double c, s;
+};
+
+// A (n x n, row-major, both triangles) -> tridiagonal by similarity rotations G on the
+// planes (p, p+1), G = [[c, s], [-s, c]]; the rotations are appended to `log` in order.
+void band_to_tridiag(int n, int b, std::vector<double> &A, std::vector<Rot> &log)
+{
+    auto at = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+    auto rotate = [&](int p, double c, double s) {
+        const int q = p + 1, lo = std::max(0, p - b - 2), hi = std::min(n - 1, q + b + 2);
+        for (int j = lo; j <= hi; ++j) {
+            const double t1 = at(p, j), t2 = at(q, j);
+            at(p, j) = c * t1 + s * t2;
+            at(q, j) = -s * t1 + c * t2;
+        }
+        for (int i = lo; i <= hi; ++i) {
+            const double t1 = at(i, p), t2 = at(i, q);
+            at(i, p) = c * t1 + s * t2;
+            at(i, q) = -s * t1 + c * t2;
+        }
+    };
+    for (int k = 0; k + 2 < n; ++k) {
+        for (int i = std::min(n - 1, k + b); i >= k + 2; --i) {
+            int q = i, col = k;
+            while (true) {
+                const int p = q - 1;
+                const double x = at(p, col), y = at(q, col);
+                if (y == 0.0) break;
+                const double r = std::hypot(x, y), c = x / r, s = y / r;
+                rotate(p, c, s);
+                at(q, col) = 0.0;
+                at(col, q) = 0.0;
+                log.push_back({p, c, s});
+                // the rotation filled (q + b, p): chase it one band further down
+                if (q + b >= n) break;
+                col = p;
+                q = q + b;
+            }
+        }
+    }
+}
+
+// Solves (T - lam) x = rhs in place for the tridiagonal (d, e), e[i] coupling i and i+1,
+// by Gaussian elimination with partial pivoting; zero pivots are replaced by `tiny`.
+void tri_shift_solve(int n, const double *d, const double *e, double lam, double tiny, double *x)
+{
+    std::vector<double> dl(n, 0.0), dd(n), du(n, 0.0), du2(n, 0.0);
+    std::vector<char> swp(n, 0);
+    for (int i = 0; i < n; ++i) dd[i] = d[i] - lam;
+    for (int i = 0; i + 1 < n; ++i) {
+        dl[i] = e[i];
+        du[i] = e[i];
+    }
+    for (int i = 0; i + 1 < n; ++i) {
+        if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+            if (dd[i] == 0.0) dd[i] = tiny;
+            const double f = dl[i] / dd[i];
+            dl[i] = f;
+            dd[i + 1] -= f * du[i];
+        } else {   // interchange rows i and i+1
+            const double f = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = f;
+            const double t = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = t - f * dd[i + 1];
+            if (i + 2 < n) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -f * du[i + 1];
+            }
+            swp[i] = 1;
+        }
+    }
+    if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
+    for (int i = 0; i + 1 < n; ++i) {
+        if (!swp[i]) {
+            x[i + 1] -= dl[i] * x[i];
+        } else {
+            const double t = x[i];
+            x[i] = x[i + 1];
+            x[i + 1] = t - dl[i] * x[i];
+        }
+    }
+    x[n - 1] /= dd[n - 1];
+    if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+}
+
+}  // namespace
+
+int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
+{
+    if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
+    std::vector<double> A(a, a + (size_t)n * n);
+    std::vector<Rot> log;
+    if (b > 1) band_to_tridiag(n, b, A, log);
+    std::vector<double> d(n), e(n, 0.0);
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        d[i] = A[(size_t)i * n + i];
+        if (i + 1 < n) e[i] = A[(size_t)(i + 1) * n + i];
+        tnorm = std::max(tnorm, std::fabs(d[i]) + std::fabs(e[i]) + (i ? std::fabs(e[i - 1]) : 0.0));
+    }
+    {
+        std::vector<double> dq(d), eq(e);
+        if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0)
+            return fail(ED_ERR_ARG, "band_lowest: QL iteration did not converge");
+        std::sort(dq.begin(), dq.end());
+        for (int k = 0; k < nev; ++k) w[k] = dq[k];
+    }
+    if (!z) return ED_OK;
+    const double eps = DBL_EPSILON, tiny = eps * std::max(tnorm, 1e-300), ortol = 1e-3 * tnorm;
+    int cluster0 = 0;
+    for (int k = 0; k < nev; ++k) {
+        double *x = z + (size_t)k * n;
+        if (k > 0 && w[k] - w[k - 1] > ortol) cluster0 = k;
+        // a perturbed copy of a cluster's eigenvalue keeps the factorisation regular
+        const double lam = w[k] + (k - cluster0) * 10.0 * eps * std::max(tnorm, 1.0);
+        uint64_t h = 0x2545F4914F6CDD1Dull + (uint64_t)k * 0x9E3779B97F4A7C15ull;
+        for (int i = 0; i < n; ++i) {   // deterministic start vector in (-1, 1)
+            h ^= h << 13;
+            h ^= h >> 7;
+            h ^= h << 17;
+            x[i] = (double)(h >> 11) / 4503599627370496.0 - 1.0;
+        }
+        for (int it = 0; it < 4; ++it) {
+            tri_shift_solve(n, d.data(), e.data(), lam, tiny, x);
+            for (int c = cluster0; c < k; ++c) {   // Gram-Schmidt against the cluster
+                const double *v = z + (size_t)c * n;
+                double dot = 0.0;
+                for (int i = 0; i < n; ++i) dot += v[i] * x[i];
+                for (int i = 0; i < n; ++i) x[i] -= dot * v[i];
+            }
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+            nrm = std::sqrt(nrm);
+            if (!(nrm > 0.0)) return fail(ED_ERR_ARG, "band_lowest: inverse iteration failed");
+            for (int i = 0; i < n; ++i) x[i] /= nrm;
+        }
+    }
+    // back through the rotations: x = G_1^T ... G_N^T y
+    for (size_t r = log.size(); r-- > 0;) {
+        const Rot &g = log[r];
+        for (int k = 0; k < nev; ++k) {
+            double *x = z + (size_t)k * n;
+            const double yp = x[g.p], yq = x[g.p + 1];
+            x[g.p] = g.c * yp - g.s * yq;
+            x[g.p + 1] = g.s * yp + g.c * yq;
+        }
+    }
+    for (int k = 0; k < nev; ++k) {   // sign: largest-magnitude component positive
+        double *x = z + (size_t)k * n;
+        int imax = 0;
+        for (int i = 1; i < n; ++i)
+            if (std::fabs(x[i]) > std::fabs(x[imax])) imax = i;
+        if (x[imax] < 0)
+            for (int i = 0; i < n; ++i) x[i] = -x[i];
+    }
+    return ED_OK;
+}
+
 }  // namespace ed
 
 extern "C" {
+
+// Lowest eigenpairs of a symmetric band matrix (the Ritz step of the block Lanczos
+// driver, exposed for its host-side tests).
+int ed_band_eig(int n, int b, const double *a, int nev, double *w, double *z)
+{
+    return ed::band_lowest(n, b, a, nev, w, z);
+}
 
 // Exposed for the parity tests of the host eigen-solver (no reference counterpart as a
 // separate symbol: it is the DSTEDC call inside diagonalize_tridiag).

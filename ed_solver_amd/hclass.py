@@ -271,6 +271,17 @@ def tridiag_eig(diag, subdiag):
     return w, z.reshape(n, n)
 
 
+def band_eig(a: np.ndarray, b: int, nev: int, vectors: bool = True):
+    """nev lowest eigenpairs of a symmetric band matrix (half-bandwidth b): the Ritz step
+    of the block Lanczos driver.  Returns (w, z) with z of shape (nev, n)."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    w = np.zeros(nev)
+    z = np.zeros((nev, n)) if vectors else None
+    check(lib().ed_band_eig(n, int(b), dptr(a), int(nev), dptr(w), dptr(z) if vectors else None))
+    return w, z
+
+
 def apply_C(src: Sector, dst: Sector, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
     """apply_Cupdo (apply_C.f90:497-598): c+ (pm=+1) / c (pm=-1) on impurity orbital `site`."""
     psi = np.ascontiguousarray(psi, np.float64)

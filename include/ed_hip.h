@@ -201,6 +201,55 @@ int ed_rayleigh_f64(ed_sector *sec, int64_t n, const double *psi, double *energy
  * (matrices_tridiag.f90:210-259), host FP64. */
 int ed_tridiag_eig(int n, const double *diag, const double *subdiag, double *w, double *z);
 
+/* nev lowest eigenpairs of a symmetric band matrix a (n x n, row-major, both
+ * triangles, half-bandwidth b): w[nev] ascending, z (n x nev, column-major, may be
+ * NULL; sign: largest component positive).  The Ritz step of the block Lanczos driver,
+ * standing in for the one inside BLZPACK's BLZDRD (Block_Lanczos.f90:132,195), host
+ * FP64: band reduction by Givens rotations, implicit QL, inverse iteration. */
+int ed_band_eig(int n, int b, const double *a, int nev, double *w, double *z);
+
+/* ----------------------------------------------------------- block Lanczos */
+
+/* Block_Lanczos_diagonalize (Block_Lanczos.f90:37-355): the Neigen lowest eigenpairs of
+ * the sector by a block Lanczos of block size NVB (Block_size, :64-68), at most
+ * max_steps block steps (ISTOR(6) = min(Nitermax, dimen_H), :63).  BLZPACK (BLZDRD) is
+ * replaced by a device driver: block H.v, full reorthogonalisation (classical
+ * Gram-Schmidt twice) and Cholesky-QR2 from FP64 MFMA contractions, Ritz values of the
+ * band matrix T on the host.  Converged when every wanted Ritz value moved by less than
+ * `tolerance` (RSTOR(3), :82) between two tests `check_every` block steps apart and its
+ * residual bound is below sqrt(tolerance) * max(1, |E|). */
+typedef struct ed_block_opts {
+    int block_size;      /* NVB, 1..16                                           */
+    int nev;             /* Neigen, 1..16                                        */
+    int max_steps;       /* block steps                                          */
+    int check_every;     /* Ritz test every this many block steps                */
+    double tolerance;
+} ed_block_opts;
+
+int ed_block_default_opts(ed_block_opts *opts);   /* 8, 8, 128, 2, 1e-12 */
+
+/* evals[nev] ascending (VALP(:,1)), resid[nev] = ||H x_k - E_k x_k|| (VALP(:,2), may be
+ * NULL), evecs = host D x nev column-major with leading dimension ldv (VECP, may be
+ * NULL), *nconv = eigenpairs whose residual meets the test (ISTOR(67)), *steps = block
+ * steps (ISTOR(19)). */
+int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *evecs,
+                     int64_t ldv, int *nconv, int *steps);
+/* Same, eigenvectors written to device memory d_evecs (leading dimension ldv). */
+int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *d_evecs,
+                         int64_t ldv, int *nconv, int *steps);
+
+/* The contractions of the block orthogonalisation, on device matrices (column-major):
+ *   ed_gram_f64_dev:  g (host, row-major p x q) = X^T Y, X n x p, Y n x q, q <= 16
+ *                     (16-byte aligned pointers, even leading dimensions);
+ *   ed_xc_f64_dev:    Y = beta Y + alpha X C, C host row-major p x q, q <= 16
+ *                     (Y may be X when p == q).
+ * FP64 MFMA; synchronous.  No reference counterpart (they are the dense
+ * products inside BLZDRD). */
+int ed_gram_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, const double *d_y, int64_t ldy, double *g,
+                    void *stream);
+int ed_xc_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, const double *c, double alpha, double beta,
+                  double *d_y, int64_t ldy, void *stream);
+
 /* --------------------------------------------------- Green's functions */
 
 /* apply_Cupdo (apply_C.f90:497-598): phi = c+ (pm = +1) or c (pm = -1) of impurity

@@ -197,6 +197,23 @@ class OracleSector:
         return re + 1j * im, a[:st], b[:st]
 
 
+def lowest_eigs(sector: OracleSector, k: int = 8, tol: float = 1e-14):
+    """The k lowest eigenpairs of the restated H: LAPACK (numpy eigh) on the dense matrix
+    built column by column from the restated H.v when dim <= 3000, else ARPACK (scipy
+    eigsh, which='SA') on the H.v as a linear operator.  The library routine is the
+    ground truth for Block_Lanczos_diagonalize and ARPACK_diago (ED_ARPACK.f90:475-570)."""
+    import scipy.sparse.linalg as sla
+    n = sector.dim
+    if n <= 3000:
+        H = np.stack([sector.hmult(e) for e in np.eye(n)], axis=1)
+        w, v = np.linalg.eigh(0.5 * (H + H.T))
+        return w[:k], v[:, :k].T.copy()
+    op = sla.LinearOperator((n, n), matvec=sector.hmult, dtype=np.float64)
+    w, v = sla.eigsh(op, k=k, which="SA", tol=tol, v0=start_vector(n), ncv=max(3 * k, 40))
+    order = np.argsort(w)
+    return w[order], v[:, order].T.copy()
+
+
 def apply_c(src: OracleSector, dst: OracleSector, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
     psi = np.ascontiguousarray(psi, np.float64)
     phi = np.zeros(dst.dim)

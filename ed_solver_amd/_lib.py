@@ -88,6 +88,10 @@ SIGNATURES = {
                                        ctypes.c_int64, _dp, _vp]),
     "ed_xc_f64_dev": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _dp,
                                      ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int64, _vp]),
+    "ed_hf_gradient": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "ed_hf_gradient_dev": (ctypes.c_int, [_vp, _vp, _dp, _dp, _dp]),
+    "ed_bath_distance": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
     "ed_apply_c": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_continued_fraction": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_dynamic": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,

@@ -250,6 +250,33 @@ int ed_gram_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, con
 int ed_xc_f64_dev(int64_t n, int p, int q, const double *d_x, int64_t ldx, const double *c, double alpha, double beta,
                   double *d_y, int64_t ldy, void *stream);
 
+/* ---------------------------------------------------------- derivatives */
+
+/* Hellmann-Feynman gradient dE/dtheta = <psi|dH/dtheta|psi> / <psi|psi> for an
+ * eigenvector psi of the sector (FP64, host vector of length n = dimen_H(), or device
+ * vector for the _dev form).  d_level[ns] = derivatives in the site levels (bit a:
+ * bath site a < nb -> Eb(a,a), impurity orbital c = a - nb -> Ec(c,c); both spins),
+ * d_v[nb*nc] (row-major) = derivatives in the real hybridisations Vbc(k,c) (both spins;
+ * the term Vbc b+_k c_c + h.c. of HAIM_offdiag_arrays.f90:483-527), *d_u = derivative in
+ * the interaction scale U of U_ii = U, U_ij = U - 2J at fixed J (HAIMupdo.f90:785-829).
+ * Any output may be NULL.  No reference counterpart: the reference has no derivative
+ * of the energy; it stands beside the finite differences of the bath fit. */
+int ed_hf_gradient(ed_sector *sec, int64_t n, const double *psi, double *d_level, double *d_v, double *d_u);
+int ed_hf_gradient_dev(ed_sector *sec, const double *d_psi, double *d_level, double *d_v, double *d_u);
+
+/* distance_func / diff_correl (bath_class_hybrid.f90:1151-1204, correl_class.f90:
+ * 1061-1160) for a diagonal bath eps[nb] with real couplings vbc[nb*nc] (row-major,
+ * Vbc(k,c)), shared by both spins: Delta_ij(i w_n) = sum_k Vbc(k,i) Vbc(k,j)/(i w_n - eps_k)
+ * (bath2hybrid, bath_class_hybrid.f90:587-903, normal state), target T[nw][2][nc][nc]
+ * complex (re, im interleaved; spin up, spin down), wn[nw] the Matsubara frequencies:
+ *   S = sum_n sum_{sigma,i,j} |Delta_ij - T^sigma_ij|^weight_expo / (|w_n - w_1| + shift)^fit_weight_power
+ *   *dist = |S| / (nw (2 nc)^2),  shift = fit_shift, or |w_1| when |fit_shift| < 1e-10.
+ * grad_eps[nb], grad_v[nb*nc] (may be NULL) = the analytic gradient of *dist (the
+ * reference takes finite differences, conj_grad_civelli.f90:282-303).  Device kernel. */
+int ed_bath_distance(int nc, int nb, const double *eps, const double *vbc, int nw, const double *wn,
+                     const double *target, double weight_expo, double fit_weight_power, double fit_shift, double *dist,
+                     double *grad_eps, double *grad_v);
+
 /* --------------------------------------------------- Green's functions */
 
 /* apply_Cupdo (apply_C.f90:497-598): phi = c+ (pm = +1) or c (pm = -1) of impurity

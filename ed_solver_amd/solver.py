@@ -5,9 +5,9 @@ entry points (every H.v, Lanczos step, excitation and continued fraction runs in
 libed_hip.so):
 
 * ``get_eigen_energies``  solver.h:98-237 -- per sector: dense ED when the dimension
-  is <= FULL_ED_SWITCH (Global_Variables.f90:23), else plain Lanczos (or block Lanczos
-  when ``block_size`` > 0); then the per-sector window filter [E_s, E_s + dEmax]
-  (filter_eigen) and the global one [E0, E0 + dEmax];
+  is <= FULL_ED_SWITCH (Global_Variables.f90:23), else plain Lanczos, ARPACK or dense ED
+  by ``which_lanczos`` (or block Lanczos when ``block_size`` > 0); then the per-sector
+  window filter [E_s, E_s + dEmax] (filter_eigen) and the global one [E0, E0 + dEmax];
 * ``partition``           Z and the Boltzmann weights e^{-beta (E - E0)} / Z
   (eigen_sector_class.f90:431-444, green_class_computeAA.h:107);
 * ``compute_green``       compute_greenAA (green_class_computeAA.f90:40-185,
@@ -41,23 +41,36 @@ def all_sectors(ns: int) -> list[tuple[int, int]]:
 
 def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128,
                        tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
-                       precision="f64", keep_sectors: bool = False):
-    """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax]."""
+                       precision="f64", which_lanczos: str = "GPU", neigen_arpack: int | None = None,
+                       arpack_ncv: int | None = None):
+    """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax].
+
+    Per sector (solver.h:150-200): block Lanczos when block_size > 0, unless N_up or N_dn
+    is 0 or 1 or both are Ns; otherwise dense ED at dimension <= full_ed_switch, else
+    which_lanczos = 'NORMAL' / 'GPU' (the device Lanczos), 'FULL_ED' or 'ARPACK'."""
     sectors = all_sectors(model.ns) if sectors is None else sectors
     states: list[EigenState] = []
+    ns = model.ns
     for (nu, nd) in sectors:
-        dim = comb(model.ns, nu) * comb(model.ns, nd)
+        dim = comb(ns, nu) * comb(ns, nd)
+        plain = block_size == 0 or nu in (0, 1) or nd in (0, 1) or (nu == ns and nd == ns)
         with Sector(model, nu, nd) as sec:
-            if dim <= full_ed_switch:
+            if dim <= full_ed_switch or (plain and which_lanczos == "FULL_ED"):
                 ev, vec = sec.full_ed(min(neigen, dim))
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
-            elif block_size > 0 and nu not in (0, 1, model.ns) and nd not in (0, 1, model.ns):
+            elif not plain:
                 from .block import block_lanczos
                 ev, vec = block_lanczos(sec, nblock=block_size, neig=neigen, nitermax=nitermax, tolerance=tolerance)
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
-            else:
+            elif which_lanczos == "ARPACK":
+                from .arpack import arpack_diago
+                r = arpack_diago(sec, neigen_arpack or max(2 * neigen, 2) + 1, tolerance, dEmax, ncv=arpack_ncv)
+                found = [(r.evals[k], r.vectors[k]) for k in range(r.nconv)]
+            elif which_lanczos in ("GPU", "NORMAL"):
                 res = sec.lanczos(nitermax=nitermax, neigen=neigen, tolerance=tolerance, precision=precision)
                 found = [(res.evals[k], sec.ground_state(res, k)) for k in range(len(res.evals))]
+            else:
+                raise ValueError(f"which_lanczos {which_lanczos!r}: expected NORMAL, GPU, FULL_ED or ARPACK")
         if not found:
             continue
         e_min = min(e for e, _ in found)

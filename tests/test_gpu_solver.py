@@ -70,3 +70,44 @@ def test_thermal_green_function_vs_lehmann(gpu):
                 Gref += amp[b] * (np.exp(-beta * (em[a] - e0)) + np.exp(-beta * (en[b] - e0))) / (1j * wn - (en[b] - em[a]))
     Gref /= Z
     assert np.max(np.abs(G - Gref)) < 1e-9
+
+
+@pytest.mark.parametrize("name,neigen_arpack,ncv", [("A", 9, None), ("B", 7, 24)])
+def test_arpack_diago_with_device_hmult(gpu, name, neigen_arpack, ncv):
+    """ARPACK_diago mirror: ARPACK (which='BE', nev = Neigen_arpack - 1, maxitr 300; ncv =
+    Neigen_arpack as in the reference, or larger) on the device H.v; the low end against
+    LAPACK / ARPACK on the oracle H.v."""
+    import json
+    import os
+    import oracle
+    from ed_solver_amd.aim import AIMParams
+    from ed_solver_amd.arpack import arpack_diago
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")) as f:
+        d = json.load(f)[name]
+    m, sec = AIMParams.from_json(d["model"]), tuple(d["sector"])
+    s = gpu.Sector(m, *sec)
+    r = arpack_diago(s, neigen_arpack, tolerance=1e-12, ncv=ncv)
+    o = oracle.OracleSector(m, *sec, nthreads=16)
+    ref, _ = oracle.lowest_eigs(o, 3)
+    o.close()
+    print(f"{name}: nconv {r.nconv} matvecs {r.matvecs} E {r.evals[:3]} rel resid {r.rel_resid.max():.1e}", flush=True)
+    assert r.nconv >= 2
+    assert abs(r.evals[0] - ref[0]) < 1e-10
+    assert np.max(r.rel_resid) < 1e-8
+    assert np.max(np.abs(r.vectors @ r.vectors.T - np.eye(r.nconv))) < 1e-8
+    s.close()
+
+
+def test_solver_which_lanczos_options(gpu):
+    """get_eigen_energies with which_lanczos = GPU, FULL_ED and ARPACK agree sector by sector."""
+    from ed_solver_amd.solver import get_eigen_energies
+    m = small_model(1, 5, 2.0, 0.0, seed=6)   # Ns=6
+    secs = [(3, 3), (2, 3), (3, 4)]
+    e = {}
+    for how in ("GPU", "FULL_ED", "ARPACK"):
+        st = get_eigen_energies(m, sectors=secs, neigen=1, which_lanczos=how, arpack_ncv=20,
+                                full_ed_switch=0 if how != "FULL_ED" else 40)
+        e[how] = {x.sector: x.energy for x in sorted(st, key=lambda x: -x.energy)}
+    for sec in secs:
+        assert abs(e["GPU"][sec] - e["FULL_ED"][sec]) < 1e-10
+        assert abs(e["ARPACK"][sec] - e["FULL_ED"][sec]) < 1e-10

@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--niter", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-green", action="store_true")
+    ap.add_argument("--no-scan", action="store_true", help="skip the whole-Fock-space sector scan (block Lanczos)")
+    ap.add_argument("--no-grad", action="store_true", help="skip the Ns=14 gradient workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one sector (or one batch of sectors) per GPU; "
@@ -148,6 +150,81 @@ def main():
         log(f"G(iw): plain {len(secs)/t_plain:.1f} solves/s, batched {len(secs)/t_batch:.1f} solves/s")
         s_p.close(); s_h.close()
 
+    # mixed precision at the accuracy the target is stated for (|dE0| < 1e-8): both
+    # ground-state solves with the convergence tolerance 1e-9 (GS vector included)
+    t0 = time.perf_counter()
+    r64b = sec.lanczos(nitermax=128, neigen=1, tolerance=1e-9, precision="f64")
+    sec.ground_state(r64b)
+    t_gs64b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r32b = sec.lanczos(nitermax=128, neigen=1, tolerance=1e-9, precision="f32")
+    sec.ground_state(r32b)
+    t_gs32b = time.perf_counter() - t0
+    mixed_1e8 = {"tolerance": 1e-9, "gs_ms_fp64": t_gs64b * 1e3, "gs_ms_mixed": t_gs32b * 1e3,
+                 "gs_speedup": t_gs64b / t_gs32b, "abs_dE0_fp64": abs(float(r64b.evals[0]) - float(r64.evals[0])),
+                 "abs_dE0_mixed": abs(float(r32b.evals[0]) - float(r64.evals[0])), "iters_fp64": r64b.iters,
+                 "iters_mixed_fp32_phase": r32b.iters_f32, "iters_mixed_fp64_phase": r32b.iters}
+    log(f"GS at tolerance 1e-9: FP64 {t_gs64b*1e3:.1f} ms, mixed {t_gs32b*1e3:.1f} ms "
+        f"(|dE0| {mixed_1e8['abs_dE0_mixed']:.1e})")
+
+    scan = None
+    if not args.no_scan:
+        # every sector of the Ns=12 model with Boltzmann weight > 1e-6 at beta = 100, 8 states
+        # each by block Lanczos of width 8, then G(i w_n) of all of them in one batched run
+        from ed_solver_amd.solver import compute_green, sector_scan
+        t0 = time.perf_counter()
+        states, info = sector_scan(model, beta=100.0, weight_cutoff=1e-6, block_size=8, neigen=8)
+        t_scan = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        G = compute_green(model, states, 100.0, ed.matsubara(100.0, 2048), nitergreen=36)
+        t_g = time.perf_counter() - t0
+        scan = {"sectors": info["sectors"], "kept_sectors": [list(k) for k in info["kept_sectors"]],
+                "states": info["states"], "E0": info["E0"], "t_lowest_all_sectors_s": info["t_pass1_s"],
+                "t_block_lanczos_s": info["t_pass2_s"], "t_total_s": t_scan, "t_green_s": t_g,
+                "G_iw0": [float(G[0, 0].real), float(G[0, 0].imag)]}
+        log(f"sector scan: {info['sectors']} sectors in {info['t_pass1_s']:.2f} s, block Lanczos on "
+            f"{len(info['kept_sectors'])} sectors ({info['states']} states) in {info['t_pass2_s']:.2f} s, "
+            f"G in {t_g:.2f} s")
+
+    grad = None
+    if not args.no_grad:
+        # Ns=14: 2 orbitals + 12 bath sites, sector (7,7), dim 11,778,624: ground state and
+        # dE0/d(eps_k, V_k, U) by Hellmann-Feynman; bath-distance gradient on 4096 w_n
+        from ed_solver_amd.grad import bath_distance, hf_gradient
+        mE, secE = ed.synthetic("E")
+        t0 = time.perf_counter()
+        sE = ed.Sector(mE, *secE)
+        t_b = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rE = sE.lanczos(nitermax=300, neigen=1, tolerance=1e-12)
+        psiE = sE.ground_state(rE)
+        t_gsE = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gE = hf_gradient(sE, psiE)
+        t_hf = time.perf_counter() - t0
+        dimE = sE.dim
+        sE.close()
+        import numpy as np
+        nbE, ncE = mE.nb, mE.nc
+        wn4 = ed.matsubara(100.0, 4096)
+        epsE = np.diag(mE.eb[0]).copy()
+        vE = mE.vbc[0].copy()
+        tgt = np.zeros((4096, 2, ncE, ncE), complex)
+        for k in range(nbE):   # target: the bath with every level shifted by 0.1
+            c = k % ncE
+            tgt[:, :, c, c] += (vE[k, c] ** 2 / (1j * wn4 - epsE[k] - 0.1))[:, None]
+        bath_distance(epsE, vE, wn4, tgt)   # warm-up
+        t0 = time.perf_counter()
+        dist, ge, gv = bath_distance(epsE, vE, wn4, tgt)
+        t_bd = time.perf_counter() - t0
+        grad = {"workload": "Ns=14: 2-orbital Kanamori (U=3, J=0.5) + 12 bath sites, sector (7,7), "
+                            f"dim {dimE}", "t_build_s": t_b, "t_gs_s": t_gsE, "gs_iters": rE.iters,
+                "E0": float(rE.evals[0]), "t_hf_gradient_s": t_hf, "n_params": 2 * nbE + 1,
+                "dE0_dU": gE.u, "bath_distance": dist, "t_bath_distance_and_gradient_ms": t_bd * 1e3,
+                "n_matsubara": 4096}
+        log(f"Ns=14 (7,7): GS {t_gsE:.2f} s ({rE.iters} it), HF gradient {t_hf*1e3:.1f} ms, "
+            f"bath distance + gradient (4096 w_n) {t_bd*1e3:.2f} ms")
+
     # roofline of the dominant kernel: the fused Lanczos H.v, events around each launch
     avg_ms, bytes_l = sec.hmult_benchmark("f64", reps=64)
     ach = bytes_l / (avg_ms * 1e-3) / 1e9
@@ -178,9 +255,19 @@ def main():
                             "E0_fp64": float(r64.evals[0]), "E0_mixed": e_mixed,
                             "abs_dE0": de0, "abs_dE0_rayleigh": de0_rq, "iters_fp64": r64.iters,
                             "iters_mixed_fp32_phase": r32.iters_f32, "iters_mixed_fp64_phase": r32.iters},
+        "mixed_precision_at_1e-8": mixed_1e8,
         "green": green,
         "speedup_vs_cpu": (it64 / cpu["value"]) if cpu else None,
+        "sector_scan": scan,
+        "gradients_ns14": grad,
     }
+    if cpu and green:
+        # ground state + the 6 excitation runs, against the CPU restatement's iteration rate
+        # (its excited-sector iterations scaled by their smaller dimension)
+        t_gpu = t_gs64 + green["solves"] / green["batched_solves_per_s"]
+        exc = green["excited_dim"] / D
+        t_cpu = (r64.iters + green["solves"] * green["nitergreen"] * exc) / cpu["value"]
+        out["gs_plus_green"] = {"t_gpu_ms": t_gpu * 1e3, "t_cpu_est_ms": t_cpu * 1e3, "speedup": t_cpu / t_gpu}
     sec.close()
     print(json.dumps(out), flush=True)
 

@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "ed_internal.h"
@@ -27,7 +28,7 @@ static int ql_implicit(int n, double *d, double *e, double *z)
         do {
             for (m = l; m < n - 1; ++m) {
                 const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= DBL_EPSILON * dd) break;
+                if (std::fabs(e[m]) <= DBL_EPSILON * dd || std::fabs(e[m]) <= DBL_MIN) break;
             }
             if (m != l) {
                 if (++iter > 100) return -1;
@@ -73,16 +74,76 @@ static int ql_implicit(int n, double *d, double *e, double *z)
     return 0;
 }
 
+static void tri_vectors(int n, const double *d, const double *e, int nev, const double *w, double *z);
+
+// Number of eigenvalues of (d, e) below x (Sturm count of the LDL^T pivots).
+static int sturm_count(int n, const double *d, const double *e, double x)
+{
+    int cnt = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+        q = d[i] - x - (i ? e[i - 1] * e[i - 1] / q : 0.0);
+        if (q == 0.0) q = -DBL_EPSILON * (std::fabs(x) + 1e-300);
+        if (q < 0.0) ++cnt;
+    }
+    return cnt;
+}
+
+// All eigenvalues of (d, e) by bisection on the Sturm count (Golub and Van Loan, sec. 8.4):
+// the fallback when the QL iteration does not converge.
+static void sturm_all(int n, const double *d, const double *e, double *w)
+{
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+        lo = std::min(lo, d[i] - r);
+        hi = std::max(hi, d[i] + r);
+    }
+    const double span = std::max(hi - lo, 1e-300);
+    lo -= 1e-12 * span;
+    hi += 1e-12 * span;
+    for (int k = 0; k < n; ++k) {
+        double a = lo, b = hi;
+        for (int it = 0; it < 200 && b - a > 2.0 * DBL_EPSILON * std::max(std::fabs(a), std::fabs(b)); ++it) {
+            const double mid = 0.5 * (a + b);
+            if (sturm_count(n, d, e, mid) > k) b = mid;
+            else a = mid;
+        }
+        w[k] = 0.5 * (a + b);
+    }
+}
+
 // All eigenpairs of T, ascending; z[k*n + r] = component r of eigenvector k, with the
 // sign chosen so that the largest-magnitude component is positive.
 int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z)
 {
     if (n < 1) return fail(ED_ERR_ARG, "tridiag_eig: n < 1");
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(diag[i]) || (i > 0 && !std::isfinite(off[i])))
+            return fail(ED_ERR_ARG, "tridiag_eig: non-finite entry in row " + std::to_string(i) + " of " +
+                                        std::to_string(n));
+        amax = std::max(amax, std::fabs(diag[i]) + (i > 0 ? std::fabs(off[i]) : 0.0));
+    }
     std::vector<double> d(diag, diag + n), e(n, 0.0), q((size_t)n * n, 0.0);
     for (int i = 0; i + 1 < n; ++i) e[i] = off[i + 1];
     for (int i = 0; i < n; ++i) q[(size_t)i * n + i] = 1.0;
-    if (ql_implicit(n, d.data(), e.data(), q.data()) != 0)
-        return fail(ED_ERR_ARG, "tridiag_eig: QL iteration did not converge");
+    if (ql_implicit(n, d.data(), e.data(), q.data()) != 0) {
+        // bisection + inverse iteration (the dstebz / dstein pair) on the original matrix
+        std::vector<double> d0(diag, diag + n), e0(n, 0.0);
+        for (int i = 0; i + 1 < n; ++i) e0[i] = off[i + 1];
+        sturm_all(n, d0.data(), e0.data(), w);
+        tri_vectors(n, d0.data(), e0.data(), n, w, z);
+        for (int k = 0; k < n; ++k) {   // sign: largest-magnitude component positive
+            double *col = z + (size_t)k * n;
+            int imax = 0;
+            for (int r = 1; r < n; ++r)
+                if (std::fabs(col[r]) > std::fabs(col[imax])) imax = r;
+            if (col[imax] < 0)
+                for (int r = 0; r < n; ++r) col[r] = -col[r];
+        }
+        return ED_OK;
+    }
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
@@ -104,7 +165,11 @@ void tridiag_lowest(int n, const double *diag, const double *off, int k, double 
 {
     std::vector<double> d(diag, diag + n), e(n, 0.0);
     for (int i = 0; i + 1 < n; ++i) e[i] = off[i + 1];
-    ql_implicit(n, d.data(), e.data(), nullptr);
+    if (ql_implicit(n, d.data(), e.data(), nullptr) != 0) {
+        std::vector<double> e0(n, 0.0);
+        for (int i = 0; i + 1 < n; ++i) e0[i] = off[i + 1];
+        sturm_all(n, diag, e0.data(), d.data());
+    }
     std::sort(d.begin(), d.end());
     for (int i = 0; i < k && i < n; ++i) w[i] = d[i];
     for (int i = n; i < k; ++i) w[i] = 1e300;
@@ -216,27 +281,14 @@ void tri_shift_solve(int n, const double *d, const double *e, double lam, double
 
 }  // namespace
 
-int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
+// Eigenvectors z (n x nev, column-major) of the tridiagonal (d, e), e[i] coupling i and
+// i+1, for the ascending eigenvalues w[0..nev): inverse iteration with partial pivoting,
+// Gram-Schmidt inside clusters of close eigenvalues (Wilkinson; LAPACK dstein's scheme).
+static void tri_vectors(int n, const double *d, const double *e, int nev, const double *w, double *z)
 {
-    if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
-    std::vector<double> A(a, a + (size_t)n * n);
-    std::vector<Rot> log;
-    if (b > 1) band_to_tridiag(n, b, A, log);
-    std::vector<double> d(n), e(n, 0.0);
     double tnorm = 0.0;
-    for (int i = 0; i < n; ++i) {
-        d[i] = A[(size_t)i * n + i];
-        if (i + 1 < n) e[i] = A[(size_t)(i + 1) * n + i];
-        tnorm = std::max(tnorm, std::fabs(d[i]) + std::fabs(e[i]) + (i ? std::fabs(e[i - 1]) : 0.0));
-    }
-    {
-        std::vector<double> dq(d), eq(e);
-        if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0)
-            return fail(ED_ERR_ARG, "band_lowest: QL iteration did not converge");
-        std::sort(dq.begin(), dq.end());
-        for (int k = 0; k < nev; ++k) w[k] = dq[k];
-    }
-    if (!z) return ED_OK;
+    for (int i = 0; i < n; ++i)
+        tnorm = std::max(tnorm, std::fabs(d[i]) + (i + 1 < n ? std::fabs(e[i]) : 0.0) + (i ? std::fabs(e[i - 1]) : 0.0));
     const double eps = DBL_EPSILON, tiny = eps * std::max(tnorm, 1e-300), ortol = 1e-3 * tnorm;
     int cluster0 = 0;
     for (int k = 0; k < nev; ++k) {
@@ -252,7 +304,7 @@ int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
             x[i] = (double)(h >> 11) / 4503599627370496.0 - 1.0;
         }
         for (int it = 0; it < 4; ++it) {
-            tri_shift_solve(n, d.data(), e.data(), lam, tiny, x);
+            tri_shift_solve(n, d, e, lam, tiny, x);
             for (int c = cluster0; c < k; ++c) {   // Gram-Schmidt against the cluster
                 const double *v = z + (size_t)c * n;
                 double dot = 0.0;
@@ -262,10 +314,31 @@ int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
             double nrm = 0.0;
             for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
             nrm = std::sqrt(nrm);
-            if (!(nrm > 0.0)) return fail(ED_ERR_ARG, "band_lowest: inverse iteration failed");
             for (int i = 0; i < n; ++i) x[i] /= nrm;
         }
     }
+}
+
+int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
+{
+    if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
+    std::vector<double> A(a, a + (size_t)n * n);
+    std::vector<Rot> log;
+    if (b > 1) band_to_tridiag(n, b, A, log);
+    std::vector<double> d(n), e(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        d[i] = A[(size_t)i * n + i];
+        if (i + 1 < n) e[i] = A[(size_t)(i + 1) * n + i];
+        if (!std::isfinite(d[i]) || !std::isfinite(e[i])) return fail(ED_ERR_ARG, "band_lowest: non-finite entry");
+    }
+    {
+        std::vector<double> dq(d), eq(e);
+        if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0) sturm_all(n, d.data(), e.data(), dq.data());
+        std::sort(dq.begin(), dq.end());
+        for (int k = 0; k < nev; ++k) w[k] = dq[k];
+    }
+    if (!z) return ED_OK;
+    tri_vectors(n, d.data(), e.data(), nev, w, z);
     // back through the rotations: x = G_1^T ... G_N^T y
     for (size_t r = log.size(); r-- > 0;) {
         const Rot &g = log[r];

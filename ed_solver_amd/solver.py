@@ -83,6 +83,39 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
     return [s for s in states if s.energy <= e0 + dEmax]
 
 
+def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, block_size: int = 8, neigen: int = 8,
+                nitermax: int = 128, tolerance: float = 1e-12, full_ed_switch: int = FULL_ED_SWITCH):
+    """Every (N_up, N_dn) sector with Boltzmann weight above weight_cutoff, neigen lowest
+    states each (solver.h:98-237 over the whole Fock space).
+
+    Pass 1: the lowest energy of every sector (dense ED or device Lanczos).  Pass 2: the
+    sectors with e^{-beta (E_s - E0)} > weight_cutoff get `neigen` states by block Lanczos
+    of width `block_size` (plain Lanczos in the sectors solver.h:152 excludes); states
+    below the weight cutoff are dropped.  Returns (states, info)."""
+    import time
+    ns = model.ns
+    t0 = time.perf_counter()
+    e_low = {}
+    for (nu, nd) in all_sectors(ns):
+        dim = comb(ns, nu) * comb(ns, nd)
+        with Sector(model, nu, nd) as sec:
+            if dim <= full_ed_switch:
+                e_low[(nu, nd)] = float(sec.full_ed(1)[0][0])
+            else:
+                e_low[(nu, nd)] = float(sec.lanczos(nitermax=nitermax, neigen=1, tolerance=tolerance).evals[0])
+    t_pass1 = time.perf_counter() - t0
+    e0 = min(e_low.values())
+    kept = sorted(s for s, e in e_low.items() if np.exp(-beta * (e - e0)) > weight_cutoff)
+    t0 = time.perf_counter()
+    states = get_eigen_energies(model, sectors=kept, neigen=neigen, block_size=block_size, nitermax=nitermax,
+                                tolerance=tolerance, full_ed_switch=full_ed_switch)
+    t_pass2 = time.perf_counter() - t0
+    states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
+    info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,
+            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2}
+    return states, info
+
+
 def partition(states: list[EigenState], beta: float) -> np.ndarray:
     """Boltzmann weights e^{-beta (E - E0)} / Z of the kept states."""
     e = np.array([s.energy for s in states])

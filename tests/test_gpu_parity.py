@@ -127,6 +127,31 @@ def test_hmult_every_schedule(gpu, name):
     s.close()
 
 
+@pytest.mark.parametrize("sector", [(6, 5), (6, 9), (5, 6)])
+def test_hmult_and_lanczos_unequal_table_widths(gpu, sector):
+    """Sectors of workload C whose down table is wider than the up table ((6,5), (6,9)) or
+    narrower ((5,6)): H.v on every schedule and the Lanczos ground state against the
+    oracle (the fast kernel unrolls to the wider width and must not read past the up table)."""
+    m, _ = golden("C")
+    s = gpu.Sector(m, *sector)
+    o = oracle.OracleSector(m, *sector, nthreads=16)
+    print(f"sector {sector} dim {s.dim}", flush=True)
+    v = np.random.default_rng(6).uniform(-1, 1, s.dim)
+    ref = o.hmult(v)
+    s.tune()
+    for c in (1, 2, 4, 8, 16):
+        try:
+            s.set_schedule(c)
+        except gpu.EDError:
+            assert c >= 8
+            continue
+        assert rel(s.hmult(v), ref) < 1e-13, c
+        r = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+        ro = o.lanczos(nitermax=128, neigen=1, tol=1e-12)
+        assert abs(r.evals[0] - ro["evals"][0]) < 1e-10, c
+    s.close()
+
+
 def test_hmult_fp32_and_block(gpu):
     from hipmem import DevBuf
     m, (nu, nd) = golden("B")

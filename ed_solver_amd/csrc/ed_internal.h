@@ -78,7 +78,7 @@ struct SpinTables {
     float *diag32 = nullptr, *off32 = nullptr, *ell_val32 = nullptr;
     // packed ELL: entry = target rank (low 16 bits) | coefficient index (high 16 bits)
     // into the table of distinct coefficients coef[ncoef] (coef[0] = 0 pads short rows)
-    uint32_t *ell_pk = nullptr;  // [kpad][d]
+    uint32_t *ell_pk = nullptr;  // [max(kpad of both spins)][d]
     double *coef = nullptr;
     float *coef32 = nullptr;
     int ncoef = 0;
@@ -100,7 +100,7 @@ struct HvArgs {
     const T *hund_val;
     const int32_t *swap_up, *swap_dn;
     // packed layouts of the fast kernel
-    const uint32_t *ell_pk_up;     // [kup][dup]
+    const uint32_t *ell_pk_up;     // [max(kup, kdn)][dup]: as many rows as the fast kernel unrolls
     const T *coef_up;              // [ncoef_up]
     int ncoef_up, kdn, nhund;
     const int32_t *hund_tr;        // [nhund] cu' | cd' << 8

@@ -224,14 +224,14 @@ __host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int 
 // are issued before the one LDS barrier, together with the column staging and, in
 // Lanczos mode, the reduction of beta_j by wave 0.  Up-hop coefficients, interaction
 // diagonal and Hund transitions are read from LDS.  K bounds the entries per row of both
-// spin tables; the up table holds a.kup <= K rows, entries beyond it read as the zero
-// coefficient.  Dynamic LDS: col[dup] coef[ncoef] uimp[ncfg^2] hval[nhund] (T), then
+// spin tables; the packed up table has K rows (padded with the zero coefficient when the
+// down table is the wider one).  Dynamic LDS: col[dup] coef[ncoef] uimp[ncfg^2] hval[nhund] (T), then
 // hptr[ncfg^2+1] htr[nhund] swdn[ncfg] (int).
 template <typename T, int MODE, int K, bool ONE>
 __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
                                     const LzArgs<T> &lz, int blk, unsigned char *smem, double *red, double *grp)
 {
-    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund, kup = a.kup;
+    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund;
     T *col = reinterpret_cast<T *>(smem);
     T *coef = col + dup;
     T *uimp = coef + a.ncoef_up;
@@ -275,7 +275,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     int cu = 0;
     if (r < dup) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) pk[k] = k < kup ? a.ell_pk_up[k * dup + r] : 0u;   // 0: coef[0] = 0
+        for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
 #pragma unroll
         for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
         du = a.diag_up[r];
@@ -319,7 +319,7 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         r += NT;
         if (r < dup) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) pk[k] = k < kup ? a.ell_pk_up[k * dup + r] : 0u;   // 0: coef[0] = 0
+            for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
 #pragma unroll
             for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
             du = a.diag_up[r];

@@ -218,9 +218,11 @@ static void free_spin(SpinTables &t)
 // table of the distinct coefficient values of the spin (index 0 holds 0.0 and pads rows
 // shorter than kpad).  The values are copies of tab_HAIM2's, so the products are the
 // same bits as with the plain ELL.  Built on the host from the device tables.
-static int build_packed(SpinTables &T, hipStream_t st)
+// Packed ELL of one spin with `rows` >= kpad rows (the unrolled width of the fast H.v,
+// the wider of the two spins' widths; rows beyond kpad hold the zero coefficient).
+static int build_packed(SpinTables &T, int rows, hipStream_t st)
 {
-    const int d = T.d, kw = T.kpad;
+    const int d = T.d, kw = rows;
     std::vector<int32_t> ptr(d + 1), rk(std::max<int64_t>(1, T.nnz));
     std::vector<double> off(std::max<int64_t>(1, T.nnz));
     ED_HIP_CHECK(hipMemcpy(ptr.data(), T.ptr, sizeof(int32_t) * (d + 1), hipMemcpyDeviceToHost));
@@ -334,7 +336,6 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
                        (int64_t)kw * d);
     ED_HIP_CHECK(hipGetLastError());
     ED_HIP_CHECK(hipStreamSynchronize(st));
-    ED_TRY(build_packed(T, st));
     (void)hipFree(d_ec);
     (void)hipFree(d_eb);
     (void)hipFree(d_vbc);
@@ -488,6 +489,11 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
     hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
     if ((rc = build_spin(S, 0, nup, S->up)) != ED_OK) return bail(rc);
     if ((rc = build_spin(S, 1, ndn, S->dn)) != ED_OK) return bail(rc);
+    {
+        const int rows = std::max(S->up.kpad, S->dn.kpad);
+        if ((rc = build_packed(S->up, rows, S->stream)) != ED_OK) return bail(rc);
+        if ((rc = build_packed(S->dn, rows, S->stream)) != ED_OK) return bail(rc);
+    }
     if ((rc = build_interaction(S)) != ED_OK) return bail(rc);
     S->dim = (int64_t)S->up.d * S->dn.d;
     if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)

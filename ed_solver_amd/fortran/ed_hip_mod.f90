@@ -17,6 +17,12 @@ module ed_hip
     real(c_double)     :: jhund, cutoff
   end type
 
+  ! ed_block_opts (Block_Lanczos_diagonalize parameters)
+  type, bind(C) :: ed_block_opts_t
+    integer(c_int) :: block_size, nev, max_steps, check_every
+    real(c_double) :: tolerance
+  end type
+
   interface
     function ed_last_error() bind(C, name="ed_last_error")
       import :: c_ptr
@@ -74,6 +80,36 @@ module ed_hip
       import :: c_int, c_double
       integer(c_int), value :: n, nw
       real(c_double)        :: a(*), b(*), wn(*), out(*)
+    end function
+    integer(c_int) function ed_block_default_opts(opts) bind(C, name="ed_block_default_opts")
+      import :: c_int, ed_block_opts_t
+      type(ed_block_opts_t), intent(out) :: opts
+    end function
+    integer(c_int) function ed_block_lanczos(sec, opts, evals, resid, evecs, ldv, nconv, steps) &
+        bind(C, name="ed_block_lanczos")
+      import :: c_int, c_int64_t, c_ptr, c_double, ed_block_opts_t
+      type(c_ptr), value                :: sec
+      type(ed_block_opts_t), intent(in) :: opts
+      real(c_double)                    :: evals(*), resid(*), evecs(*)
+      integer(c_int64_t), value         :: ldv
+      integer(c_int), intent(out)       :: nconv, steps
+    end function
+    integer(c_int) function ed_hf_gradient(sec, n, psi, d_level, d_v, d_u) bind(C, name="ed_hf_gradient")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value          :: sec
+      integer(c_int64_t), value   :: n
+      real(c_double), intent(in)  :: psi(*)
+      real(c_double)              :: d_level(*), d_v(*)
+      real(c_double), intent(out) :: d_u
+    end function
+    integer(c_int) function ed_bath_distance(nc, nb, eps, vbc, nw, wn, target, weight_expo, fit_weight_power, &
+        fit_shift, dist, grad_eps, grad_v) bind(C, name="ed_bath_distance")
+      import :: c_int, c_double
+      integer(c_int), value       :: nc, nb, nw
+      real(c_double), intent(in)  :: eps(*), vbc(*), wn(*), target(*)
+      real(c_double), value       :: weight_expo, fit_weight_power, fit_shift
+      real(c_double), intent(out) :: dist
+      real(c_double)              :: grad_eps(*), grad_v(*)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr

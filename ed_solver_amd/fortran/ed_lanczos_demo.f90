@@ -6,8 +6,10 @@
 !   1. the plain Lanczos loop of Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134
 !      with lanc_opt, Lanczos_plain.h:66-173) on the host, calling Hmult__ for every
 !      H.v -- the reference's own solver shape on top of the drop-in Hmult__;
-!   2. the coarse device path Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo.
-! Prints E0 from both, the iteration count and the Rayleigh quotient of the device
+!   2. the coarse device path Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo;
+!   3. Block_Lanczos_diagonalize through ed_block_lanczos (block size 4, 4 states) when
+!      the sector has at least 16 states.
+! Prints E0 from each, the iteration count and the Rayleigh quotient of the device
 ! ground state.
 program ed_lanczos_demo
   use iso_c_binding
@@ -19,6 +21,10 @@ program ed_lanczos_demo
   real(8) :: jhund, cutoff
   real(8), allocatable :: vin(:), vout(:), tmp(:), diag(:), subdiag(:), w(:), z(:), psi(:), hpsi(:)
   real(8) :: normv_in, normv_out, aa, ddiag, last, e0_host, e0_dev, rq
+  type(ed_block_opts_t) :: bo
+  real(8) :: valp(4, 2)
+  real(8), allocatable :: vecp(:, :)
+  integer(c_int) :: nconv, nsteps
   character(len=512) :: fname
   integer :: u_in
 
@@ -81,5 +87,15 @@ program ed_lanczos_demo
   rq = sum(psi * hpsi) / sum(psi * psi)
   write(*, '(a,f22.14)') 'E0_gpu_updo ', e0_dev
   write(*, '(a,f22.14)') 'E0_rayleigh ', rq
+
+  ! ---- 3. Block_Lanczos_diagonalize (Block_Lanczos.f90:37-355) on the device
+  if (n >= 16) then
+    call ed_check(ed_block_default_opts(bo), 'ed_block_default_opts')
+    bo%block_size = 4; bo%nev = 4; bo%max_steps = min(nitermax, n / 4); bo%tolerance = 1d-12
+    allocate(vecp(n, 4))
+    call ed_check(ed_block_lanczos(sector_handle(), bo, valp(:, 1), valp(:, 2), vecp, int(n, c_int64_t), &
+                                   nconv, nsteps), 'ed_block_lanczos')
+    write(*, '(a,4f22.14,a,i0,a,i0)') 'E_block ', valp(1:4, 1), ' nconv ', nconv, ' steps ', nsteps
+  end if
   call delete_H()
 end program ed_lanczos_demo

@@ -48,3 +48,8 @@ def test_fortran_demo_matches_oracle(tmp_path, name):
     assert abs(vals["E0_host_loop"] - e0) < 1e-10
     assert abs(vals["E0_gpu_updo"] - e0) < 1e-10
     assert abs(vals["E0_rayleigh"] - e0) < 1e-9
+    # Block_Lanczos_diagonalize through ed_block_lanczos (4 lowest states)
+    line = [l for l in r.stdout.splitlines() if l.startswith("E_block")][0].split()
+    eb = np.array([float(x) for x in line[1:5]])
+    ref, _ = oracle.lowest_eigs(oracle.OracleSector(m, nu, nd, nthreads=8), 4)
+    assert np.max(np.abs(eb - ref)) < 1e-10

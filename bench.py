@@ -88,7 +88,7 @@ def main():
     log(f"sector (6,6) dim {sec.dim} built in {t_build*1e3:.1f} ms")
     D = sec.dim
     sched, sched_ms = sec.tune()
-    log(f"H.v schedule {sched} (ms for row kernel x1/x2/x4, fast kernel, fast kernel XCD-grouped: {sched_ms})")
+    log(f"H.v schedule {sched} (ms for schedules 1/2/4 row kernel, 8 fast, 16 fast XCD-grouped, 18/19/20 fast XCD-grouped with 2/3/4 rows per thread: {sched_ms})")
 
     def rate(prec):
         for _ in range(args.warmup):

@@ -335,7 +335,7 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     hipStream_t st = s->stream;
     if (!s->cpb) {   // H.v schedule: the autotuner runs with the first Lanczos call
         int choice;
-        double ms[5];
+        double ms[kSchedules];
         ED_TRY(ed_hmult_tune(s, &choice, ms));
     }
     const int64_t ld = (n + 63) / 64 * 64;

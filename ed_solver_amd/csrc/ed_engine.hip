@@ -56,15 +56,19 @@ __global__ __launch_bounds__(kBlock) void k_hv_rows(HvArgs<T> a, const T *__rest
 }
 
 // Fast kernel (one column per workgroup, blockDim.x up to 1024), see hv_fast_body.
-template <typename T, int MODE, int K, bool ONE>
-__global__ __launch_bounds__(1024) void k_hv_fast(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+// Largest workgroup of the fast kernel with R rows per thread: 1024 / R threads (R >= 2
+// keeps the register budget of 1024 / R threads, so R rows of loads stay in registers).
+__host__ __device__ constexpr int fast_max_threads(int R) { return R > 1 ? (1024 / R + 63) / 64 * 64 : 1024; }
+
+template <typename T, int MODE, int K, int R>
+__global__ __launch_bounds__(fast_max_threads(R)) void k_hv_fast(HvArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
                                                    int64_t ld, LzArgs<T> lz)
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
     __shared__ double grp[kSums];
     const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
-    hv_fast_body<T, MODE, K, ONE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
+    hv_fast_body<T, MODE, K, R>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
                                   reinterpret_cast<unsigned char *>(smem_d), red, grp);
 }
 
@@ -297,6 +301,7 @@ struct HvShape {
     int k;       // compiled table width
     size_t lds;
     int xcd;     // fast kernel over the XCD-grouped column order (s->d_colord)
+    int rows;    // fast kernel: rows per thread with their loads in flight together (0 = loop)
 };
 
 static int width_of(int kmax)
@@ -316,7 +321,13 @@ static bool fast_ok(const ed_sector *s)
 }
 
 // sched: 1, 2, 4 = row kernel with that many columns per workgroup; 8 = fast kernel;
-// 16 = fast kernel with the XCD-grouped column order (needs s->d_colord).
+// 16 = fast kernel with the XCD-grouped column order (needs s->d_colord); 18, 19, 20 =
+// that kernel with 2, 3, 4 rows per thread (smaller workgroups, all rows' loads in flight).
+static bool sched_valid(int sched)
+{
+    return sched == 1 || sched == 2 || sched == 4 || sched == 8 || sched == 16 || (sched >= 18 && sched <= 20);
+}
+
 static HvShape hv_shape(const ed_sector *s, int sched)
 {
     HvShape h;
@@ -324,12 +335,16 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     h.nt = kBlock;
     h.k = s->up.kpad;
     h.xcd = 0;
-    if ((sched == 8 || sched == 16) && fast_ok(s)) {
+    h.rows = 0;
+    const int rows = sched >= 18 ? sched - 16 : 1;
+    const int nt_rows = std::max(64, ((dup + rows - 1) / rows + 63) / 64 * 64);
+    if ((sched == 8 || sched >= 16) && fast_ok(s) && (rows == 1 || nt_rows <= fast_max_threads(rows))) {
         h.kind = 2;
         h.cpb = 1;
-        h.nt = std::min(1024, std::max(64, (dup + 63) / 64 * 64));
+        h.rows = rows == 1 ? (dup <= 1024 ? 1 : 0) : rows;
+        h.nt = rows == 1 ? std::min(1024, std::max(64, (dup + 63) / 64 * 64)) : nt_rows;
         h.k = width_of(std::max(s->up.kpad, s->dn.kpad));
-        h.xcd = sched == 16 && s->d_colord;
+        h.xcd = sched >= 16 && s->d_colord;
         h.nblk = h.xcd ? s->colord_blocks : ddn;
         h.lds = 0;   // set per precision at launch
         return h;
@@ -347,24 +362,29 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     return h;
 }
 
-template <typename T, int MODE, bool ONE>
-static void launch_fast_one(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x,
-                            T *y, int64_t ld, const LzArgs<T> &lz)
+template <typename T, int MODE, int R>
+static void launch_fast_rows(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x,
+                             T *y, int64_t ld, const LzArgs<T> &lz)
 {
     switch (k) {
-    case 4: hipLaunchKernelGGL((k_hv_fast<T, MODE, 4, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 8: hipLaunchKernelGGL((k_hv_fast<T, MODE, 8, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    case 12: hipLaunchKernelGGL((k_hv_fast<T, MODE, 12, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
-    default: hipLaunchKernelGGL((k_hv_fast<T, MODE, 16, ONE>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 4: hipLaunchKernelGGL((k_hv_fast<T, MODE, 4, R>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 8: hipLaunchKernelGGL((k_hv_fast<T, MODE, 8, R>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    case 12: hipLaunchKernelGGL((k_hv_fast<T, MODE, 12, R>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
+    default: hipLaunchKernelGGL((k_hv_fast<T, MODE, 16, R>), grid, dim3(nt), lds, st, a, x, y, ld, lz); break;
     }
 }
 
 template <typename T, int MODE>
-static void launch_fast(const HvArgs<T> &a, int k, dim3 grid, int nt, size_t lds, hipStream_t st, const T *x, T *y,
-                        int64_t ld, const LzArgs<T> &lz)
+static void launch_fast(const HvArgs<T> &a, int k, int rows, dim3 grid, int nt, size_t lds, hipStream_t st,
+                        const T *x, T *y, int64_t ld, const LzArgs<T> &lz)
 {
-    if (a.dup <= nt) launch_fast_one<T, MODE, true>(a, k, grid, nt, lds, st, x, y, ld, lz);
-    else launch_fast_one<T, MODE, false>(a, k, grid, nt, lds, st, x, y, ld, lz);
+    switch (rows) {
+    case 1: launch_fast_rows<T, MODE, 1>(a, k, grid, nt, lds, st, x, y, ld, lz); break;
+    case 2: launch_fast_rows<T, MODE, 2>(a, k, grid, nt, lds, st, x, y, ld, lz); break;
+    case 3: launch_fast_rows<T, MODE, 3>(a, k, grid, nt, lds, st, x, y, ld, lz); break;
+    case 4: launch_fast_rows<T, MODE, 4>(a, k, grid, nt, lds, st, x, y, ld, lz); break;
+    default: launch_fast_rows<T, MODE, 0>(a, k, grid, nt, lds, st, x, y, ld, lz); break;
+    }
 }
 
 template <typename T, int MODE, int CPB>
@@ -390,7 +410,7 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
     dim3 grid(h.nblk, MODE ? 1 : nvec);
     if (h.kind == 2) {
         const size_t lds = hv_fast_lds(a.dup, a.ncoef_up, a.ncfg, a.nhund, sizeof(T));
-        launch_fast<T, MODE>(a, h.k, grid, h.nt, lds, st, x, y, ld, lz);
+        launch_fast<T, MODE>(a, h.k, h.rows, grid, h.nt, lds, st, x, y, ld, lz);
         return;
     }
     const size_t lds = (size_t)h.cpb * a.dup * sizeof(T);
@@ -411,9 +431,16 @@ static int elem_grid(int64_t n)
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
+// Schedule of the kernels of precision T: FP32 has its own when the autotuner set one.
+template <typename T>
+static int sched_of(const ed_sector *s)
+{
+    return (sizeof(T) == sizeof(float) && s->cpb32) ? s->cpb32 : s->cpb;
+}
+
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st)
 {
-    const HvShape h = hv_shape(s, s->cpb);
+    const HvShape h = hv_shape(s, prec == ED_PREC_F32 ? sched_of<float>(s) : sched_of<double>(s));
     if (prec == ED_PREC_F64) {
         LzArgs<double> lz{};
         hv_launch_mode<double, 0>(s, h, (const double *)x, (double *)y, ld, nvec, lz, st);
@@ -500,7 +527,7 @@ static LzArgs<T> lz_args(ed_sector *s)
     lz.partB = s->d_partials + kPartB;
     lz.partC = s->d_partials + kPartC;
     lz.pstride = 8192;
-    const HvShape h = hv_shape(s, s->cpb);
+    const HvShape h = hv_shape(s, sched_of<T>(s));
     lz.nA = h.nblk;
     lz.dup = s->up.d;
     lz.colmap = h.kind == 2 ? 1 : 0;
@@ -513,7 +540,7 @@ template <typename T>
 static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t st)
 {
     const LzArgs<T> lz = lz_args<T>(s);
-    const HvShape h = hv_shape(s, s->cpb);
+    const HvShape h = hv_shape(s, sched_of<T>(s));
     const int64_t n = s->dim;
     const int g = lz.nB;   // element-wise kernels: one workgroup per partial of partB
     const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(kReorthGrid, (n + kBlock * 16 - 1) / (kBlock * 16)));
@@ -584,11 +611,12 @@ static int lz_start(ed_sector *s, int prec, const double *h_start, const double 
             ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
         src = s->d_io_a;
     }
-    const int g = lz_args<double>(s).nB;
     if (prec == ED_PREC_F64)
-        hipLaunchKernelGGL(k_lz_init<double>, dim3(g), dim3(kBlock), 0, st, lz_args<double>(s), src, s->dim);
+        hipLaunchKernelGGL(k_lz_init<double>, dim3(lz_args<double>(s).nB), dim3(kBlock), 0, st, lz_args<double>(s), src,
+                           s->dim);
     else
-        hipLaunchKernelGGL(k_lz_init<float>, dim3(g), dim3(kBlock), 0, st, lz_args<float>(s), src, s->dim);
+        hipLaunchKernelGGL(k_lz_init<float>, dim3(lz_args<float>(s).nB), dim3(kBlock), 0, st, lz_args<float>(s), src,
+                           s->dim);
     ED_HIP_CHECK(hipGetLastError());
     return ED_OK;
 }
@@ -677,42 +705,62 @@ static int ensure_colord(ed_sector *s)
 // or 4 columns per workgroup; fast kernel in natural and in XCD-grouped column order),
 // 3 launches each after one warm-up, HIP events on the sector's stream, and keeps the
 // fastest.
-static int tune(ed_sector *s)
+template <typename T>
+static int tune_prec(ed_sector *s, double *ms_out, int &choice)
 {
-    ED_TRY(ensure_io(s));
     hipStream_t st = s->stream;
-    for (double &t : s->tune_ms) t = 0.0;
-    if (s->dim < 4096) {
-        s->cpb = fast_ok(s) ? 8 : 1;
-        return ED_OK;
-    }
-    if (fast_ok(s)) ED_TRY(ensure_colord(s));
-    ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
+    const T *x = (const T *)s->d_io_a;   // timing only: the contents do not matter
+    T *y = (T *)s->d_io_b;
     float best = 1e30f;
     int best_c = 1;
-    const int cands[5] = {1, 2, 4, 8, 16};
-    for (int i = 0; i < 5; ++i) {
+    const int cands[kSchedules] = {1, 2, 4, 8, 16, 18, 19, 20};
+    for (int i = 0; i < kSchedules; ++i) {
         const HvShape h = hv_shape(s, cands[i]);
-        const bool applies = (cands[i] == 16) ? h.kind == 2 && h.xcd
+        const bool applies = (cands[i] >= 16) ? h.kind == 2 && h.xcd && (cands[i] == 16 || h.rows == cands[i] - 16)
                              : (cands[i] == 8) ? h.kind == 2
                                                : (h.kind == 1 && h.cpb == cands[i]) || (h.kind == 0 && i == 0);
         if (!applies) continue;
-        LzArgs<double> lz{};
-        hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
+        LzArgs<T> lz{};
+        hv_launch_mode<T, 0>(s, h, x, y, 0, 1, lz, st);
         ED_HIP_CHECK(hipEventRecord(s->ev0, st));
-        for (int r = 0; r < 3; ++r) hv_launch_mode<double, 0>(s, h, s->d_io_a, s->d_io_b, 0, 1, lz, st);
+        for (int r = 0; r < 3; ++r) hv_launch_mode<T, 0>(s, h, x, y, 0, 1, lz, st);
         ED_HIP_CHECK(hipEventRecord(s->ev1, st));
         ED_HIP_CHECK(hipEventSynchronize(s->ev1));
         float ms = 0.f;
         ED_HIP_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        s->tune_ms[i] = ms / 3;
+        ms_out[i] = ms / 3;
         if (ms < best) {
             best = ms;
             best_c = cands[i];
         }
     }
     ED_HIP_CHECK(hipGetLastError());
-    s->cpb = best_c;
+    choice = best_c;
+    return ED_OK;
+}
+
+// Autotuner: times the H.v under every applicable schedule (row kernel with 1, 2 or 4
+// columns per workgroup; fast kernel in natural and XCD-grouped column order, with 1-4
+// rows per thread), 3 launches each after one warm-up, HIP events on the sector's
+// stream, separately for the FP64 and the FP32 kernels, and keeps the fastest of each.
+static int tune(ed_sector *s)
+{
+    ED_TRY(ensure_io(s));
+    hipStream_t st = s->stream;
+    for (double &t : s->tune_ms) t = 0.0;
+    for (double &t : s->tune_ms32) t = 0.0;
+    if (s->dim < 4096) {
+        s->cpb = fast_ok(s) ? 8 : 1;
+        s->cpb32 = 0;
+        return ED_OK;
+    }
+    if (fast_ok(s)) ED_TRY(ensure_colord(s));
+    ED_TRY(start_vector_dev(s->dim, s->d_io_a, st));
+    int c64 = 1, c32 = 1;
+    ED_TRY(tune_prec<double>(s, s->tune_ms, c64));
+    ED_TRY(tune_prec<float>(s, s->tune_ms32, c32));
+    s->cpb = c64;
+    s->cpb32 = c32;
     drop_graphs(s);
     return ED_OK;
 }
@@ -937,18 +985,21 @@ int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
     ED_TRY(tune(sec));
     if (cpb) *cpb = sec->cpb;
     if (ms_per_choice)
-        for (int i = 0; i < 5; ++i) ms_per_choice[i] = sec->tune_ms[i];
+        for (int i = 0; i < kSchedules; ++i) ms_per_choice[i] = sec->tune_ms[i];
     return ED_OK;
 }
 
 int ed_hmult_set_schedule(ed_sector *sec, int choice)
 {
     ED_TRY(check_common(sec, 0));
-    if (choice != 1 && choice != 2 && choice != 4 && choice != 8 && choice != 16)
-        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4, 8 or 16");
+    if (!sched_valid(choice))
+        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4, 8, 16, 18, 19 or 20");
     if (choice >= 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");
-    if (choice == 16) ED_TRY(ensure_colord(sec));
+    if (choice >= 16) ED_TRY(ensure_colord(sec));
+    if (choice >= 18 && hv_shape(sec, choice).rows != choice - 16)
+        return fail(ED_ERR_ARG, "ed_hmult_set_schedule: that many rows per thread do not apply here");
     sec->cpb = choice;
+    sec->cpb32 = 0;   // FP32 follows the forced schedule too
     drop_graphs(sec);
     return ED_OK;
 }
@@ -1033,8 +1084,9 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     for (auto &e : ev) ED_HIP_CHECK(hipEventCreate(&e));
     const int64_t n = sec->dim;
-    const int g = lz_args<double>(sec).nB;
-    const HvShape h = hv_shape(sec, sec->cpb);
+    const bool f64 = precision == ED_PREC_F64;
+    const int g = f64 ? lz_args<double>(sec).nB : lz_args<float>(sec).nB;
+    const HvShape h = hv_shape(sec, f64 ? sched_of<double>(sec) : sched_of<float>(sec));
     for (int r = 0; r < reps; ++r) {
         ED_HIP_CHECK(hipEventRecord(ev[2 * r], st));
         if (precision == ED_PREC_F64) {

@@ -30,6 +30,7 @@ int fail(int code, const std::string &msg);
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kRed = 34;             // reduction scratch: two slots per wave of a 1024-thread workgroup + 2
 constexpr int kSums = 4;             // LDS slots for the reduced recurrence sums
+constexpr int kSchedules = 8;        // H.v schedules the autotuner times
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
 
@@ -166,7 +167,9 @@ struct ed_sector {
     int cpb = 0;                   // H.v schedule chosen by the autotuner: 1/2/4 = row kernel with that many
                                    // columns per workgroup, 8 = fast kernel, 16 = fast kernel with the
                                    // XCD-grouped column order; 0 = not tuned
-    double tune_ms[5] = {0, 0, 0, 0, 0};   // measured FP64 H.v time per schedule (1, 2, 4, 8, 16)
+    double tune_ms[ed::kSchedules] = {};   // measured FP64 H.v time per schedule (1, 2, 4, 8, 16, 18, 19, 20)
+    int cpb32 = 0;                 // schedule of the FP32 kernels (tuned on their own); 0 = as cpb
+    double tune_ms32[ed::kSchedules] = {};
     int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
     int colord_blocks = 0;
     // timing of the H.v kernel inside Lanczos chunks

@@ -219,7 +219,8 @@ __host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int 
 }
 
 // Fast H.v: one column per workgroup, blockDim.x threads own rows r = tid + i*blockDim.x
-// (ONE: at most one row per thread).  All loads that do not depend on the staged column
+// (R >= 1: exactly the R rows i < R, all their loads in flight together; R == 0: a loop
+// over the rows, each row's loads issued after the previous row is done).  All loads that do not depend on the staged column
 // (packed up-hop entries, down-hop neighbour values, diagonal, previous Lanczos vector)
 // are issued before the one LDS barrier, together with the column staging and, in
 // Lanczos mode, the reduction of beta_j by wave 0.  Up-hop coefficients, interaction
@@ -227,7 +228,7 @@ __host__ __device__ inline size_t hv_fast_lds(int dup, int ncoef, int ncfg, int 
 // spin tables; the packed up table has K rows (padded with the zero coefficient when the
 // down table is the wider one).  Dynamic LDS: col[dup] coef[ncoef] uimp[ncfg^2] hval[nhund] (T), then
 // hptr[ncfg^2+1] htr[nhund] swdn[ncfg] (int).
-template <typename T, int MODE, int K, bool ONE>
+template <typename T, int MODE, int K, int R>
 __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
                                     const LzArgs<T> &lz, int blk, unsigned char *smem, double *red, double *grp)
 {
@@ -267,19 +268,26 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     }
     const int base = ido * dup;
 
-    // ---- loads of the first row, issued before the column is staged
-    int r = tid;
-    uint32_t pk[K];
-    T xd[K];
-    T du = 0;
-    int cu = 0;
-    if (r < dup) {
+    // ---- per-row loads issued before the column is staged: all R rows of the thread
+    // (R >= 1), or the first of a loop over rows tid, tid + NT, ... (R == 0)
+    constexpr int RR = R > 0 ? R : 1;
+    uint32_t pk[RR][K];
+    T xd[RR][K];
+    T du[RR];
+    int cu[RR];
 #pragma unroll
-        for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
+    for (int i = 0; i < RR; ++i) {
+        const int r = tid + i * NT;
+        du[i] = (T)0;
+        cu[i] = 0;
+        if (r < dup) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
-        du = a.diag_up[r];
-        cu = a.imp_up[r];
+            for (int k = 0; k < K; ++k) pk[i][k] = a.ell_pk_up[k * dup + r];
+#pragma unroll
+            for (int k = 0; k < K; ++k) xd[i][k] = k < nd ? x[r + jd[k]] : (T)0;
+            du[i] = a.diag_up[r];
+            cu[i] = a.imp_up[r];
+        }
     }
     // ---- stage the column and the small tables in LDS
     for (int e = tid; e < dup; e += NT) col[e] = x[base + e];
@@ -298,15 +306,15 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
     __syncthreads();
 
     double dot = 0.0;
-    while (r < dup) {
+    auto row = [&](int r, const uint32_t *pr, const T *xr, T dur, int cur) {
         const T xv = col[r];
-        T acc = (du + ddg + uimp[cu * ncfg + cd]) * xv;
+        T acc = (dur + ddg + uimp[cur * ncfg + cd]) * xv;
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc += coef[pk[k] >> 16] * col[pk[k] & 0xFFFFu];
+        for (int k = 0; k < K; ++k) acc += coef[pr[k] >> 16] * col[pr[k] & 0xFFFFu];
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc += od[k] * xd[k];
+        for (int k = 0; k < K; ++k) acc += od[k] * xr[k];
         if (a.hund) {
-            const int h0 = hptr[cu * ncfg + cd], h1 = hptr[cu * ncfg + cd + 1];
+            const int h0 = hptr[cur * ncfg + cd], h1 = hptr[cur * ncfg + cd + 1];
             for (int t = h0; t < h1; ++t) {
                 const int tr = htr[t];
                 acc += hval[t] * x[a.swap_up[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup];
@@ -314,16 +322,26 @@ __device__ inline void hv_fast_body(const HvArgs<T> &a, const T *__restrict__ xi
         }
         y[base + r] = acc;
         if (MODE == 1) dot += (double)acc * (double)xv;
-        if (ONE) break;
-        // next row of this thread
-        r += NT;
-        if (r < dup) {
+    };
+    if constexpr (R > 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) pk[k] = a.ell_pk_up[k * dup + r];
+        for (int i = 0; i < R; ++i) {
+            const int r = tid + i * NT;
+            if (r < dup) row(r, pk[i], xd[i], du[i], cu[i]);
+        }
+    } else {
+        for (int r = tid; r < dup;) {
+            row(r, pk[0], xd[0], du[0], cu[0]);
+            // next row of this thread
+            r += NT;
+            if (r < dup) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) xd[k] = k < nd ? x[r + jd[k]] : (T)0;
-            du = a.diag_up[r];
-            cu = a.imp_up[r];
+                for (int k = 0; k < K; ++k) pk[0][k] = a.ell_pk_up[k * dup + r];
+#pragma unroll
+                for (int k = 0; k < K; ++k) xd[0][k] = k < nd ? x[r + jd[k]] : (T)0;
+                du[0] = a.diag_up[r];
+                cu[0] = a.imp_up[r];
+            }
         }
     }
     if (MODE == 1) block_partial_last(dot, 0.0, red, &cnt, lz.partA + blk, nullptr);

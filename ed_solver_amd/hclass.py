@@ -131,9 +131,10 @@ class Sector:
 
     def tune(self) -> tuple[int, list]:
         """Runs the H.v schedule autotuner; returns (choice, ms per choice) with choices 1/2/4 (row
-        kernel), 8 (fast kernel) and 16 (fast kernel, XCD-grouped column order)."""
+        kernel), 8 (fast kernel), 16 (fast kernel, XCD-grouped column order) and 18/19/20 (the
+        latter with 2/3/4 rows per thread)."""
         c = ctypes.c_int()
-        ms = np.zeros(5)
+        ms = np.zeros(8)
         check(lib().ed_hmult_tune(self.handle, ctypes.byref(c), dptr(ms)))
         return c.value, ms.tolist()
 

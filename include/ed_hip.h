@@ -124,13 +124,15 @@ int ed_hmult_block_f64_dev(ed_sector *sec, int64_t n, int nvec, int64_t ld, doub
  * columns per workgroup, the fast kernel (one column per workgroup, up to 1024
  * threads, packed tables in LDS) and the fast kernel with its workgroups mapped to
  * columns in an XCD-grouped order (neighbouring columns of the down-hop graph run on
- * the same XCD and share its L2) -- and keeps the fastest for every later H.v and
- * Lanczos call on the sector.  *choice = 1, 2, 4 (row kernel), 8 (fast kernel) or 16
- * (fast kernel, XCD-grouped order); ms_per_choice[5] (may be NULL) = measured times in
- * that order (0 where a schedule does not apply).  Lanczos calls run it on first use. */
+ * the same XCD and share its L2), and that kernel with 2, 3 or 4 rows per thread
+ * (smaller workgroups with every row's loads in flight) -- and keeps the fastest for
+ * every later H.v and Lanczos call on the sector.  *choice = 1, 2, 4 (row kernel), 8
+ * (fast kernel), 16 (fast kernel, XCD-grouped order) or 18, 19, 20 (XCD-grouped, 2, 3, 4
+ * rows per thread); ms_per_choice[8] (may be NULL) = measured times in that order (0
+ * where a schedule does not apply).  Lanczos calls run it on first use. */
 int ed_hmult_tune(ed_sector *sec, int *choice, double *ms_per_choice);
 
-/* Forces one H.v schedule (1, 2, 4, 8 or 16 as above) on this sector, e.g. to test or time
+/* Forces one H.v schedule (1, 2, 4, 8, 16, 18, 19 or 20 as above) on this sector, e.g. to test or time
  * a schedule the autotuner did not pick.  No reference counterpart. */
 int ed_hmult_set_schedule(ed_sector *sec, int choice);
 

@@ -117,7 +117,7 @@ def test_hmult_every_schedule(gpu, name):
     ref = o.hmult(v)
     choice, ms = s.tune()
     print(f"{name}: tuned schedule {choice}, ms per schedule {ms}", flush=True)
-    for c in (1, 2, 4, 8, 16):
+    for c in (1, 2, 4, 8, 16, 18, 19, 20):
         try:
             s.set_schedule(c)
         except gpu.EDError:
@@ -139,7 +139,7 @@ def test_hmult_and_lanczos_unequal_table_widths(gpu, sector):
     v = np.random.default_rng(6).uniform(-1, 1, s.dim)
     ref = o.hmult(v)
     s.tune()
-    for c in (1, 2, 4, 8, 16):
+    for c in (1, 2, 4, 8, 16, 18, 19, 20):
         try:
             s.set_schedule(c)
         except gpu.EDError:

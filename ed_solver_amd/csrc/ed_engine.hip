@@ -840,8 +840,8 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
         PhaseOut p32;
         ED_TRY(lanczos_phase(s, ED_PREC_F32, std::max(o->tolerance, 1e-6), niter, 1, chunk, o->reorth, start, nullptr,
                              p32));
-        std::vector<double> ev(p32.iters), z((size_t)p32.iters * p32.iters);
-        ED_TRY(tridiag_eig(p32.iters, p32.alpha.data(), p32.beta.data(), ev.data(), z.data()));
+        std::vector<double> ev(1), z((size_t)p32.iters);
+        ED_TRY(tridiag_lowest_pairs(p32.iters, p32.alpha.data(), p32.beta.data(), 1, ev.data(), z.data()));
         ED_TRY(ensure_io(s));
         ED_TRY(gs_vector_dev(s, p32.iters, z.data(), s->d_io_b));
         s->last_f32_iters = p32.iters;
@@ -851,8 +851,8 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
     }
     const int it = ph.iters;
     neig = std::min(neig, it);
-    std::vector<double> ev(it), z((size_t)it * it);
-    ED_TRY(tridiag_eig(it, ph.alpha.data(), ph.beta.data(), ev.data(), z.data()));
+    std::vector<double> ev(neig), z((size_t)it * neig);
+    ED_TRY(tridiag_lowest_pairs(it, ph.alpha.data(), ph.beta.data(), neig, ev.data(), z.data()));
     for (int k = 0; k < neig; ++k) {
         if (evals) evals[k] = ev[k];
         if (vecp)

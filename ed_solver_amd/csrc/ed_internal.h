@@ -183,6 +183,7 @@ namespace ed {
 // tridiagonal eigen-solver (host, FP64): implicit QL with Wilkinson shifts
 int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z);
 void tridiag_lowest(int n, const double *diag, const double *off, int k, double *w);
+int tridiag_lowest_pairs(int n, const double *diag, const double *off, int k, double *w, double *z);
 // nev lowest eigenpairs of a symmetric band matrix (n x n row-major, half-bandwidth b);
 // z (n x nev, column-major) may be null
 int band_lowest(int n, int b, const double *a, int nev, double *w, double *z);

@@ -222,23 +222,52 @@ static int gram_rows(int64_t n)
     return (int)std::max<int64_t>(256, (per + 255) / 256 * 256);
 }
 
-// G (host, row-major p x q) = X^T Y.
-static int gram(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
-                double *G, hipStream_t st)
+// Partial-sum workspace of a Gram product with p columns of X over n rows.
+static size_t gram_part_size(int64_t n, int p)
+{
+    const int p16 = (p + 15) / 16 * 16;
+    const int64_t rows = gram_rows(n);
+    return (size_t)((n + rows - 1) / rows) * p16 * 16;
+}
+
+// d_G (device, row-major p x q) = X^T Y, queued on st (no host wait).  The partial-sum
+// workspace must not be reallocated while a queued Gram still uses it: callers that
+// queue several size it first (bws_get with the largest p).
+static int gram_dev(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                    double *d_G, hipStream_t st)
 {
     const int p16 = (p + 15) / 16 * 16;
     const int64_t rows = gram_rows(n);
     const int nrb = (int)((n + rows - 1) / rows);
     BlockWS *w;
-    ED_TRY(bws_get(dev, (size_t)nrb * p16 * 16, (size_t)p * q + 256, w));
+    ED_TRY(bws_get(dev, gram_part_size(n, p), 256, w));
     const dim3 grid((p16 / 16 + kGramTiles - 1) / kGramTiles, nrb);
     hipLaunchKernelGGL(k_gram_part, grid, dim3(256), 0, st, n, p, q, X, ldx, Y, ldy, rows, p16, w->part);
     hipLaunchKernelGGL(k_gram_sum, dim3((p * q + 255) / 256), dim3(256), 0, st, nrb, p, q, p16,
-                       (const double *)w->part, w->small);
+                       (const double *)w->part, d_G);
     ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+// G (host, row-major p x q) = X^T Y.
+static int gram(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                double *G, hipStream_t st)
+{
+    BlockWS *w;
+    ED_TRY(bws_get(dev, gram_part_size(n, p), (size_t)p * q + 256, w));
+    ED_TRY(gram_dev(dev, n, p, q, X, ldx, Y, ldy, w->small, st));
     ED_HIP_CHECK(hipMemcpyAsync(G, w->small, sizeof(double) * p * q, hipMemcpyDeviceToHost, st));
     ED_HIP_CHECK(hipStreamSynchronize(st));
     return ED_OK;
+}
+
+// Y = beta Y + alpha X C with C on the device (row-major p x q), queued on st.
+static void xc_dev(int64_t n, int p, int q, const double *X, int64_t ldx, const double *d_C, double alpha,
+                   double beta, double *Y, int64_t ldy, hipStream_t st)
+{
+    const int64_t waves = (n + 63) / 64;
+    hipLaunchKernelGGL(k_xc, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta,
+                       Y, ldy);
 }
 
 // Y = beta Y + alpha X C with C given on the host (row-major p x q).
@@ -248,9 +277,7 @@ static int xc(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, co
     BlockWS *w;
     ED_TRY(bws_get(dev, 16, (size_t)p * q + 256, w));
     ED_HIP_CHECK(hipMemcpyAsync(w->small, C, sizeof(double) * p * q, hipMemcpyHostToDevice, st));
-    const int64_t waves = (n + 63) / 64;
-    hipLaunchKernelGGL(k_xc, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, n, p, q, X, ldx,
-                       (const double *)w->small, alpha, beta, Y, ldy);
+    xc_dev(n, p, q, X, ldx, w->small, alpha, beta, Y, ldy, st);
     ED_HIP_CHECK(hipGetLastError());
     // the small buffer is reused by the next call: finish this one first
     ED_HIP_CHECK(hipStreamSynchronize(st));
@@ -342,21 +369,31 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     const int maxsteps = (int)std::max<int64_t>(1, std::min<int64_t>(o.max_steps, (n + b - 1) / b));
     if ((int64_t)maxsteps * b < nev) return fail(ED_ERR_ARG, "block Lanczos: max_steps * block_size < nev");
     const size_t nvec = (size_t)(maxsteps + 1) * b;
-    double *Q = nullptr, *HX = nullptr;
+    double *Q = nullptr, *HX = nullptr, *dG = nullptr;
     std::string err;
     {
         hipError_t e = hipMalloc((void **)&Q, sizeof(double) * ld * nvec);
         if (e != hipSuccess)
             return fail(ED_ERR_NOMEM, "block Lanczos basis of " + std::to_string(nvec) + " vectors: " + hipGetErrorString(e));
+        e = hipMalloc((void **)&dG, sizeof(double) * 2 * nvec * b);
+        if (e != hipSuccess) {
+            (void)hipFree(Q);
+            return fail(ED_ERR_NOMEM, std::string("block Lanczos projections: ") + hipGetErrorString(e));
+        }
     }
     struct Guard {
-        double *&a, *&b;
+        double *&a, *&b, *&c;
         ~Guard()
         {
             if (a) (void)hipFree(a);
             if (b) (void)hipFree(b);
+            if (c) (void)hipFree(c);
         }
-    } guard{Q, HX};
+    } guard{Q, HX, dG};
+    {   // size the Gram workspace once for the widest product, so queued Grams never move it
+        BlockWS *w;
+        ED_TRY(bws_get(dev, gram_part_size(n, (int)nvec), 256, w));
+    }
     hipLaunchKernelGGL(k_block_start, dim3(1024), dim3(256), 0, st, n, b, Q, ld);
     ED_HIP_CHECK(hipGetLastError());
     std::vector<double> R0;
@@ -377,20 +414,32 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         ED_TRY(launch_hv(s, ED_PREC_F64, Qj, W, ld, b, st));
         G1.assign((size_t)p * b, 0.0);
         G2.assign((size_t)p * b, 0.0);
-        ED_TRY(gram(dev, n, p, b, Q, ld, W, ld, G1.data(), st));
-        ED_TRY(xc(dev, n, p, b, Q, ld, G1.data(), -1.0, 1.0, W, ld, st));
-        ED_TRY(gram(dev, n, p, b, Q, ld, W, ld, G2.data(), st));
-        ED_TRY(xc(dev, n, p, b, Q, ld, G2.data(), -1.0, 1.0, W, ld, st));
-        for (int r = 0; r < b; ++r)   // A_j, symmetrised
-            for (int c = 0; c < b; ++c) {
-                const double a1 = G1[(size_t)(j * b + r) * b + c] + G2[(size_t)(j * b + r) * b + c];
-                const double a2 = G1[(size_t)(j * b + c) * b + r] + G2[(size_t)(j * b + c) * b + r];
-                Tat(j * b + r, j * b + c) = 0.5 * (a1 + a2);
-            }
+        // CGS2 entirely on the device: the projections stay there between the products
+        double *dG1 = dG, *dG2 = dG + nvec * b;
+        ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
+        xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
+        ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
+        xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
+        ED_HIP_CHECK(hipGetLastError());
+        ED_HIP_CHECK(hipMemcpyAsync(G1.data(), dG1, sizeof(double) * p * b, hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipMemcpyAsync(G2.data(), dG2, sizeof(double) * p * b, hipMemcpyDeviceToHost, st));
         steps = j + 1;
         bool last = (j + 1 == maxsteps) || (int64_t)p + b > n;
+        auto diag_block = [&]() {   // A_j, symmetrised (G1, G2 are on the host after a stream wait)
+            for (int r = 0; r < b; ++r)
+                for (int c = 0; c < b; ++c) {
+                    const double a1 = G1[(size_t)(j * b + r) * b + c] + G2[(size_t)(j * b + r) * b + c];
+                    const double a2 = G1[(size_t)(j * b + c) * b + r] + G2[(size_t)(j * b + c) * b + r];
+                    Tat(j * b + r, j * b + c) = 0.5 * (a1 + a2);
+                }
+        };
+        if (last) {
+            ED_HIP_CHECK(hipStreamSynchronize(st));
+            diag_block();
+        }
         if (!last) {
-            ED_TRY(chol_qr2(dev, n, b, W, ld, Bnext, ok, st));
+            ED_TRY(chol_qr2(dev, n, b, W, ld, Bnext, ok, st));   // waits on the stream: G1, G2 are in
+            diag_block();
             if (!ok) {
                 exhausted = true;   // the Krylov space is invariant: T is exact on it
                 last = true;
@@ -405,7 +454,9 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         }
         const int m = p;
         if (m < nev) continue;
-        if (!last && (steps % std::max(1, o.check_every)) != 0) continue;
+        // Ritz test every check_every steps, thinned to every steps / 8 as T grows (the
+        // band solver costs O(m^2 b) per test)
+        if (!last && (steps % std::max(std::max(1, o.check_every), steps / 8)) != 0) continue;
         // Ritz values and vectors of T_j (leading m x m block)
         std::vector<double> Tm((size_t)m * m);
         for (int r = 0; r < m; ++r)

@@ -254,35 +254,63 @@ struct Rot {
     double c, s;
 };
 
-// A (n x n, row-major, both triangles) -> tridiagonal by similarity rotations G on the
+// Symmetric band storage with room for the bulge: L[c * (b + 2) + (r - c)] = A[r][c] for
+// 0 <= r - c <= b + 1 (the band plus the one diagonal a bulge occupies while it is chased).
+struct Band {
+    int n, b, w;
+    std::vector<double> L;
+    Band(int n_, int b_) : n(n_), b(b_), w(b_ + 2), L((size_t)n_ * (b_ + 2), 0.0) {}
+    bool in(int r, int c) const { return std::abs(r - c) <= b + 1; }
+    double get(int r, int c) const
+    {
+        if (r < c) std::swap(r, c);
+        return r - c <= b + 1 ? L[(size_t)c * w + (r - c)] : 0.0;
+    }
+    void set(int r, int c, double v)
+    {
+        if (r < c) std::swap(r, c);
+        L[(size_t)c * w + (r - c)] = v;
+    }
+};
+
+// A (symmetric band, half-bandwidth b) -> tridiagonal by similarity rotations G on the
 // planes (p, p+1), G = [[c, s], [-s, c]]; the rotations are appended to `log` in order.
-void band_to_tridiag(int n, int b, std::vector<double> &A, std::vector<Rot> &log)
+// A rotation mixes rows (and columns) p and q = p + 1 at the columns x within b + 1 of
+// either; the 2 x 2 block transforms as G B G^T.
+void band_to_tridiag(Band &A, std::vector<Rot> &log)
 {
-    auto at = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+    const int n = A.n, b = A.b;
+    double *L = A.L.data();
+    const int w = A.w;
     auto rotate = [&](int p, double c, double s) {
-        const int q = p + 1, lo = std::max(0, p - b - 2), hi = std::min(n - 1, q + b + 2);
-        for (int j = lo; j <= hi; ++j) {
-            const double t1 = at(p, j), t2 = at(q, j);
-            at(p, j) = c * t1 + s * t2;
-            at(q, j) = -s * t1 + c * t2;
+        const int q = p + 1, lo = std::max(0, q - b - 1), hi = std::min(n - 1, p + b + 1);
+        for (int x = lo; x < p; ++x) {   // A[p][x], A[q][x]: adjacent entries of column x
+            double *colx = L + (size_t)x * w;
+            const double t1 = colx[p - x], t2 = colx[q - x];
+            colx[p - x] = c * t1 + s * t2;
+            colx[q - x] = -s * t1 + c * t2;
         }
-        for (int i = lo; i <= hi; ++i) {
-            const double t1 = at(i, p), t2 = at(i, q);
-            at(i, p) = c * t1 + s * t2;
-            at(i, q) = -s * t1 + c * t2;
+        double *colp = L + (size_t)p * w, *colq = L + (size_t)q * w;
+        for (int x = q + 1; x <= hi; ++x) {   // A[x][p], A[x][q]: runs down columns p and q
+            const double t1 = colp[x - p], t2 = colq[x - q];
+            colp[x - p] = c * t1 + s * t2;
+            colq[x - q] = -s * t1 + c * t2;
         }
+        const double app = A.get(p, p), apq = A.get(p, q), aqq = A.get(q, q);
+        A.set(p, p, c * c * app + 2.0 * c * s * apq + s * s * aqq);
+        A.set(q, q, s * s * app - 2.0 * c * s * apq + c * c * aqq);
+        A.set(p, q, -c * s * app + (c * c - s * s) * apq + c * s * aqq);
     };
     for (int k = 0; k + 2 < n; ++k) {
         for (int i = std::min(n - 1, k + b); i >= k + 2; --i) {
             int q = i, col = k;
             while (true) {
                 const int p = q - 1;
-                const double x = at(p, col), y = at(q, col);
+                const double x = A.get(p, col), y = A.get(q, col);
                 if (y == 0.0) break;
                 const double r = std::hypot(x, y), c = x / r, s = y / r;
                 rotate(p, c, s);
-                at(q, col) = 0.0;
-                at(col, q) = 0.0;
+                A.set(q, col, 0.0);
                 log.push_back({p, c, s});
                 // the rotation filled (q + b, p): chase it one band further down
                 if (q + b >= n) break;
@@ -382,16 +410,20 @@ static void tri_vectors(int n, const double *d, const double *e, int nev, const 
 int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
 {
     if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
-    std::vector<double> A(a, a + (size_t)n * n);
+    Band A(n, std::max(b, 1));
+    for (int c = 0; c < n; ++c)
+        for (int r = c; r < std::min(n, c + std::max(b, 1) + 1); ++r) A.set(r, c, a[(size_t)r * n + c]);
     std::vector<Rot> log;
-    if (b > 1) band_to_tridiag(n, b, A, log);
+    if (b > 1) band_to_tridiag(A, log);
     std::vector<double> d(n), e(n, 0.0);
     for (int i = 0; i < n; ++i) {
-        d[i] = A[(size_t)i * n + i];
-        if (i + 1 < n) e[i] = A[(size_t)(i + 1) * n + i];
+        d[i] = A.get(i, i);
+        if (i + 1 < n) e[i] = A.get(i + 1, i);
         if (!std::isfinite(d[i]) || !std::isfinite(e[i])) return fail(ED_ERR_ARG, "band_lowest: non-finite entry");
     }
-    {
+    if (nev <= 32 && nev < n) {
+        sturm_lowest(n, d.data(), e.data(), nev, w);
+    } else {
         std::vector<double> dq(d), eq(e);
         if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0) sturm_all(n, d.data(), e.data(), dq.data());
         std::sort(dq.begin(), dq.end());

@@ -462,10 +462,13 @@ int start_vector_dev(int64_t n, double *d_out, hipStream_t st)
 static int ensure_state(ed_sector *s)
 {
     if (!s->d_state) {
+        // Cleared on the sector's own stream: that stream is non-blocking, so a clear on
+        // the null stream (hipMemset) is not ordered before the kernels queued on it and
+        // could land after the first Lanczos initialisation has written its partials.
         ED_HIP_CHECK(hipMalloc((void **)&s->d_state, sizeof(LzState)));
-        ED_HIP_CHECK(hipMemset(s->d_state, 0, sizeof(LzState)));
+        ED_HIP_CHECK(hipMemsetAsync(s->d_state, 0, sizeof(LzState), s->stream));
         ED_HIP_CHECK(hipMalloc((void **)&s->d_partials, sizeof(double) * (size_t)kScratchDoubles));
-        ED_HIP_CHECK(hipMemset(s->d_partials, 0, sizeof(double) * (size_t)kScratchDoubles));
+        ED_HIP_CHECK(hipMemsetAsync(s->d_partials, 0, sizeof(double) * (size_t)kScratchDoubles, s->stream));
     }
     return ED_OK;
 }

@@ -16,6 +16,7 @@ restatement of the reference algorithm timed on this host.
 """
 import argparse
 import json
+from math import comb
 import os
 import sys
 import time
@@ -25,6 +26,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Lanczos iters/s (Ns=12 Sz=0) + batched G(iω) solves/s vs plain, |ΔE0|<1e-8"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP64 fused H.v in the (6,6) sector (dim 853,776), by H.v
+# schedule: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
+# `bench.py --steps 1 --warmup 0 --no-cpu --no-green --no-scan --no-grad`, FETCH_SIZE scaled
+# by 65536 / 32780.3 (a 64 MiB device copy reads as 32,780.3 KiB), WRITE_SIZE by 1.0 (it
+# reads 65,536 KiB).  Summary: profiles/hv_fused_ns12_66_pmc.md.  Schedules 8 and 16 share
+# one kernel instance and were not separated; they report null.
+HV_TRAFFIC_F64 = {18: 22816400, 19: 22834620, 20: 22827147}
 
 
 def log(msg):
@@ -231,16 +239,34 @@ def main():
         log(f"Ns=14 (7,7): GS {t_gsE:.2f} s ({rE.iters} it), HF gradient {t_hf*1e3:.1f} ms, "
             f"bath distance + gradient (4096 w_n) {t_bd*1e3:.2f} ms")
 
-    # roofline of the dominant kernel: the fused Lanczos H.v, events around each launch
-    avg_ms, bytes_l = sec.hmult_benchmark("f64", reps=64)
-    ach = bytes_l / (avg_ms * 1e-3) / 1e9
-    avg32, bytes32 = sec.hmult_benchmark("f32", reps=64)
-    log(f"fused H.v FP64 {avg_ms*1e3:.2f} us -> {ach:.0f} GB/s; FP32 {avg32*1e3:.2f} us")
+    # roofline of the dominant kernel of the timed region: the fixed-length FP64 run of
+    # args.niter steps (the persistent Lanczos kernel where it applies: one launch per run),
+    # a HIP event pair around the run on the sector's stream; algorithmic bytes 40 D per
+    # step (H.v: read u_j, write H u_j; recurrence: read u_{j-1}, u_j, write u_{j+1})
+    run_ms, run_bytes, persistent = sec.lanczos_benchmark("f64", niter=args.niter, reps=5)
+    ach = run_bytes / (run_ms * 1e-3) / 1e9
+    # the fused H.v of the two-kernel step (ground-state runs with a convergence test),
+    # 256 launches back to back between one event pair
+    avg_ms, bytes_l = sec.hmult_benchmark("f64", reps=256)
+    ach_hv = bytes_l / (avg_ms * 1e-3) / 1e9
+    avg32, bytes32 = sec.hmult_benchmark("f32", reps=256)
+    log(f"Lanczos run of {args.niter} steps: {run_ms:.3f} ms ({'persistent kernel' if persistent else 'two-kernel steps'}) "
+        f"-> {ach:.0f} GB/s; fused H.v FP64 {avg_ms*1e3:.2f} us -> {ach_hv:.0f} GB/s; FP32 {avg32*1e3:.2f} us")
+    rows = {8: 1, 16: 1, 18: 2, 19: 3, 20: 4}.get(sched)
+    kname_hv = (f"k_hv_fast<double, 1, 8, {rows}> (fused Lanczos H.v, schedule {sched})" if rows
+                else f"k_hv_rows<double, 1, ...> (fused Lanczos H.v, schedule {sched})")
+    d_up = comb(model.ns, nu)
+    pk_rows = 1 if d_up <= 256 else 2 if d_up <= 512 else 4
+    kname = (f"k_lz_persist<double, 8, {pk_rows}> (persistent Lanczos, {args.niter} steps per launch)" if persistent
+             else f"{kname_hv} + k_lz_update<double>")
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_hv<double,1,true> (fused Lanczos H.v)", "avg_us": avg_ms * 1e3,
-                "bytes_per_launch": bytes_l, "fp32_avg_us": avg32 * 1e3,
-                "schedule": sched, "schedule_ms": sched_ms,
-                "fp32_achieved": bytes32 / (avg32 * 1e-3) / 1e9}
+                "traffic": None, "kernel": kname, "avg_us": run_ms * 1e3, "bytes_per_launch": run_bytes,
+                "steps_per_launch": args.niter,
+                "hv_fused": {"kernel": kname_hv, "achieved": ach_hv, "frac": ach_hv / HBM_PEAK_GBS,
+                             "avg_us": avg_ms * 1e3, "bytes_per_launch": bytes_l,
+                             "traffic": HV_TRAFFIC_F64.get(sched), "fp32_avg_us": avg32 * 1e3,
+                             "fp32_achieved": bytes32 / (avg32 * 1e-3) / 1e9},
+                "schedule": sched, "schedule_ms": sched_ms}
 
     cpu = None
     if not args.no_cpu:

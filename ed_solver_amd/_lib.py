@@ -100,6 +100,10 @@ SIGNATURES = {
                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                           _dp]),
     "ed_hmult_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "ed_lanczos_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ed_lanczos_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                            ctypes.POINTER(ctypes.c_int)]),
+    "ed_lanczos_persistent_active": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }
 
 _lib = None

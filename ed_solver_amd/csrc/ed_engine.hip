@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ed_kernels.h"
@@ -144,6 +145,31 @@ __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
     __shared__ double red[kRed];
     __shared__ double grp[kSums];
     lz_finish_body(lz, grp);
+}
+
+// Persistent Lanczos: nsteps steps of the fast H.v and the update in one cooperative
+// launch (see lz_persist_body).  256 threads, at most 4 workgroups per CU resident.
+template <typename T, int K, int R>
+__global__ __launch_bounds__(kBlock, 4) void k_lz_persist(HvArgs<T> a, LzArgs<T> lz, int nsteps, int spin_limit)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double red[kRed];
+    __shared__ double grp[kSums];
+    __shared__ int flag;
+    lz_persist_body<T, K, R>(a, lz, nsteps, spin_limit, reinterpret_cast<unsigned char *>(smem_d), red, grp, &flag);
+}
+
+__global__ void k_bar_reset(LzState *st)
+{
+    for (int g = threadIdx.x; g < kBarGroups; g += blockDim.x) {
+        st->bar_grp[16 * g] = 0;
+        st->bar_rel[16 * g] = 0;
+    }
+    if (threadIdx.x == 0) {
+        st->bar_count = 0;
+        st->bar_top = 0;
+        st->bar_err = 0;
+    }
 }
 
 // Full reorthogonalisation (classical Gram-Schmidt, applied twice): partial dots of
@@ -563,6 +589,107 @@ static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t 
     hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
 }
 
+// ---------------------------------------------------------- persistent Lanczos
+
+// Wait rounds of a grid-barrier poll before the kernel gives up (a few seconds).
+constexpr int kBarrierSpins = 1 << 21;
+// Shortest run given to the persistent kernel: below it the launch and barrier set-up
+// cost more than the captured graph of two-kernel steps.
+constexpr int kPersistMinSteps = 16;
+
+template <typename T, int K, int R>
+static const void *persist_fn()
+{
+    return reinterpret_cast<const void *>(&k_lz_persist<T, K, R>);
+}
+
+// The instances compiled: those whose rows fit the 128-register budget of a co-resident
+// grid (4 workgroups of 256 threads per CU) without spilling in the step loop; wider
+// tables with more rows per thread keep the two-kernel step.
+template <typename T>
+static const void *persist_kernel(int k, int rows)
+{
+    constexpr bool f32 = std::is_same<T, float>::value;
+    switch (k * 8 + rows) {
+    case 4 * 8 + 1: return persist_fn<T, 4, 1>();
+    case 4 * 8 + 2: return persist_fn<T, 4, 2>();
+    case 4 * 8 + 4: return persist_fn<T, 4, 4>();
+    case 8 * 8 + 1: return persist_fn<T, 8, 1>();
+    case 8 * 8 + 2: return persist_fn<T, 8, 2>();
+    case 8 * 8 + 4: return persist_fn<T, 8, 4>();
+    case 12 * 8 + 1: return persist_fn<T, 12, 1>();
+    case 12 * 8 + 2: return f32 ? persist_fn<float, 12, 2>() : nullptr;
+    case 16 * 8 + 1: return persist_fn<T, 16, 1>();
+    default: return nullptr;
+    }
+}
+
+struct PersistShape {
+    const void *fn = nullptr;
+    int nblk = 0, rows = 0;
+    size_t lds = 0;
+};
+
+// The persistent kernel applies when the sector's schedule is a fast-kernel one (one
+// column per workgroup, so its partial sums have the fast kernel's layout), d_up fits 4
+// rows per thread of 256, and every workgroup of the grid can be resident at once.
+template <typename T>
+static bool persist_shape(ed_sector *s, PersistShape &p)
+{
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    if (!s->persist || s->persist_ok[pi] == 0) return false;
+    const HvShape h = hv_shape(s, sched_of<T>(s));
+    if (h.kind != 2 || s->up.d > 4 * kBlock) return false;
+    p.rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
+    p.fn = persist_kernel<T>(h.k, p.rows);
+    p.nblk = h.nblk;
+    p.lds = lz_persist_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(T));
+    if (!p.fn) return false;
+    if (s->persist_ok[pi] < 0) {
+        int dev = 0, per_cu = 0, ncu = 0, coop = 0;
+        bool ok = hipGetDevice(&dev) == hipSuccess &&
+                  hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
+                  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, kBlock, p.lds) == hipSuccess &&
+                  (int64_t)per_cu * ncu >= p.nblk && p.nblk <= kBarGroups * kBarGroupSize && p.lds <= 64 * 1024;
+        (void)hipGetLastError();
+        s->persist_ok[pi] = ok ? 1 : 0;
+    }
+    return s->persist_ok[pi] == 1;
+}
+
+template <typename T>
+static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipStream_t st)
+{
+    HvArgs<T> a = hv_args<T>(s, 1);
+    LzArgs<T> lz = lz_args<T>(s);
+    const HvShape h = hv_shape(s, sched_of<T>(s));
+    if (h.xcd) a.colord = s->d_colord;
+    lz.nA = lz.nB = p.nblk;
+    int spins = kBarrierSpins;
+    void *args[] = {&a, &lz, &nsteps, &spins};
+    hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, st, lz.st);
+    ED_HIP_CHECK(hipLaunchCooperativeKernel(p.fn, dim3(p.nblk), dim3(kBlock), args, (unsigned)p.lds, st));
+    hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
+    return ED_OK;
+}
+
+// Runs nsteps steps with the persistent kernel when it applies; *done = false otherwise.
+static int run_persist(ed_sector *s, int prec, int nsteps, hipStream_t st, bool &done)
+{
+    PersistShape p;
+    done = false;
+    if (prec == ED_PREC_F64) {
+        if (!persist_shape<double>(s, p)) return ED_OK;
+        ED_TRY(launch_persist<double>(s, p, nsteps, st));
+    } else {
+        if (!persist_shape<float>(s, p)) return ED_OK;
+        ED_TRY(launch_persist<float>(s, p, nsteps, st));
+    }
+    done = true;
+    return ED_OK;
+}
+
 static void run_steps(ed_sector *s, int prec, int nsteps, int reorth, hipStream_t st)
 {
     if (prec == ED_PREC_F64) launch_steps_impl<double>(s, nsteps, reorth, st);
@@ -575,6 +702,14 @@ static int tune(ed_sector *s);
 static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
 {
     hipStream_t st = s->stream;
+    if (!reorth && nsteps >= kPersistMinSteps) {
+        bool done = false;
+        ED_TRY(run_persist(s, prec, nsteps, st, done));
+        if (done) {
+            ED_HIP_CHECK(hipGetLastError());
+            return ED_OK;
+        }
+    }
     if (nsteps == chunk && chunk > 1) {
         const int key = prec * 2 + (reorth ? 1 : 0);
         GraphCache &G = s->graphs[key];
@@ -645,6 +780,12 @@ static int read_state(ed_sector *s, HostLz &h, int upto)
     ED_HIP_CHECK(hipStreamSynchronize(s->stream));
     h.j = hdr[0];
     h.status = hdr[2];
+    if (h.status == 3) {   // the persistent kernel's barrier timed out: never use it here again
+        hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s->stream, d);
+        (void)hipStreamSynchronize(s->stream);
+        s->persist = 0;
+        return fail(ED_ERR_HIP, "persistent Lanczos kernel: grid barrier wait timed out (run stopped)");
+    }
     return ED_OK;
 }
 
@@ -764,6 +905,7 @@ static int tune(ed_sector *s)
     ED_TRY(tune_prec<float>(s, s->tune_ms32, c32));
     s->cpb = c64;
     s->cpb32 = c32;
+    s->persist_ok[0] = s->persist_ok[1] = -1;
     drop_graphs(s);
     return ED_OK;
 }
@@ -877,9 +1019,13 @@ static int lanczos_run(ed_sector *s, int niter, const double *h_start, const dou
     ED_TRY(ensure_basis(s, prec, niter + 1));
     s->basis_valid_iters = 0;
     ED_TRY(lz_start(s, prec, h_start, d_start));
+    // no test between chunks here: the persistent kernel runs all steps in one launch
+    PersistShape ps;
+    const bool one = niter >= kPersistMinSteps &&
+                     (prec == ED_PREC_F64 ? persist_shape<double>(s, ps) : persist_shape<float>(s, ps));
     int done = 0;
     while (done < niter) {
-        const int c = std::min(chunk, niter - done);
+        const int c = one ? niter - done : std::min(chunk, niter - done);
         ED_TRY(steps(s, prec, c, chunk, 0));
         done += c;
     }
@@ -1003,7 +1149,65 @@ int ed_hmult_set_schedule(ed_sector *sec, int choice)
         return fail(ED_ERR_ARG, "ed_hmult_set_schedule: that many rows per thread do not apply here");
     sec->cpb = choice;
     sec->cpb32 = 0;   // FP32 follows the forced schedule too
+    sec->persist_ok[0] = sec->persist_ok[1] = -1;
     drop_graphs(sec);
+    return ED_OK;
+}
+
+int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, double *avg_ms, double *bytes_per_run,
+                         int *persistent)
+{
+    ED_TRY(check_common(sec, precision));
+    if (niter < 1 || niter > kMaxIter - 2 || reps < 1 || !avg_ms) return fail(ED_ERR_ARG, "ed_lanczos_benchmark: bad argument");
+    if (sec->dim < 2) return fail(ED_ERR_ARG, "ed_lanczos_benchmark: sector of dimension 1");
+    hipStream_t st = sec->stream;
+    ED_TRY(ensure_state(sec));
+    ED_TRY(ensure_basis(sec, precision, niter + 1));
+    sec->basis_valid_iters = 0;
+    PersistShape ps;
+    const bool one = niter >= kPersistMinSteps &&
+                     (precision == ED_PREC_F64 ? persist_shape<double>(sec, ps) : persist_shape<float>(sec, ps));
+    double tot = 0.0;
+    for (int r = 0; r <= reps; ++r) {   // r = 0: warm-up (graph capture, first launch)
+        ED_TRY(lz_start(sec, precision, nullptr, nullptr));
+        ED_HIP_CHECK(hipEventRecord(sec->ev0, st));
+        if (one) {
+            ED_TRY(steps(sec, precision, niter, 8, 0));
+        } else {
+            for (int done = 0; done < niter; done += 8) ED_TRY(steps(sec, precision, std::min(8, niter - done), 8, 0));
+        }
+        ED_HIP_CHECK(hipEventRecord(sec->ev1, st));
+        ED_HIP_CHECK(hipEventSynchronize(sec->ev1));
+        float ms = 0.f;
+        ED_HIP_CHECK(hipEventElapsedTime(&ms, sec->ev0, sec->ev1));
+        if (r > 0) tot += ms;
+    }
+    HostLz h;
+    ED_TRY(read_state(sec, h, niter));
+    if (h.status != 0) return fail(ED_ERR_BREAKDOWN, "ed_lanczos_benchmark: the recurrence broke down");
+    const double s = precision == ED_PREC_F64 ? 8.0 : 4.0;
+    *avg_ms = tot / reps;
+    if (bytes_per_run) *bytes_per_run = 5.0 * s * (double)sec->dim * niter;
+    if (persistent) *persistent = one ? 1 : 0;
+    return ED_OK;
+}
+
+int ed_lanczos_set_persistent(ed_sector *sec, int enable)
+{
+    ED_TRY(check_common(sec, 0));
+    if (enable != 0 && enable != 1) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0 or 1");
+    sec->persist = enable;
+    sec->persist_ok[0] = sec->persist_ok[1] = -1;
+    return ED_OK;
+}
+
+int ed_lanczos_persistent_active(ed_sector *sec, int precision, int *active)
+{
+    ED_TRY(check_common(sec, precision));
+    if (!active) return fail(ED_ERR_ARG, "ed_lanczos_persistent_active: NULL output");
+    if (!sec->cpb) ED_TRY(tune(sec));
+    PersistShape p;
+    *active = precision == ED_PREC_F64 ? persist_shape<double>(sec, p) : persist_shape<float>(sec, p);
     return ED_OK;
 }
 
@@ -1084,37 +1288,32 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     ED_TRY(ensure_basis(sec, precision, reps + 1));
     sec->basis_valid_iters = 0;
     ED_TRY(lz_start(sec, precision, nullptr, nullptr));
-    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
-    for (auto &e : ev) ED_HIP_CHECK(hipEventCreate(&e));
     const int64_t n = sec->dim;
     const bool f64 = precision == ED_PREC_F64;
-    const int g = f64 ? lz_args<double>(sec).nB : lz_args<float>(sec).nB;
     const HvShape h = hv_shape(sec, f64 ? sched_of<double>(sec) : sched_of<float>(sec));
-    for (int r = 0; r < reps; ++r) {
-        ED_HIP_CHECK(hipEventRecord(ev[2 * r], st));
-        if (precision == ED_PREC_F64) {
+    // Phase 1 of step 1 (z = H u_1 and the partials of <z, u_1>), launched reps times back
+    // to back between one event pair: every launch reads u_1 and writes z and the
+    // partials again, the same traffic as a step of the recurrence, so the average is the
+    // kernel's duration plus the dispatch gap (no per-launch event overhead).
+    auto launch = [&]() {
+        if (f64) {
             const LzArgs<double> lz = lz_args<double>(sec);
             hv_launch_mode<double, 1>(sec, h, (const double *)nullptr, (double *)nullptr, 0, 1, lz, st);
-            ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
-            hipLaunchKernelGGL(k_lz_update<double>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
         } else {
             const LzArgs<float> lz = lz_args<float>(sec);
             hv_launch_mode<float, 1>(sec, h, (const float *)nullptr, (float *)nullptr, 0, 1, lz, st);
-            ED_HIP_CHECK(hipEventRecord(ev[2 * r + 1], st));
-            hipLaunchKernelGGL(k_lz_update<float>, dim3(g), dim3(kBlock), 0, st, lz, n, 1);
         }
-    }
+    };
+    launch();   // warm-up
+    ED_HIP_CHECK(hipEventRecord(sec->ev0, st));
+    for (int r = 0; r < reps; ++r) launch();
+    ED_HIP_CHECK(hipEventRecord(sec->ev1, st));
     ED_HIP_CHECK(hipGetLastError());
-    ED_HIP_CHECK(hipStreamSynchronize(st));
-    double tot = 0.0;
-    for (int r = 0; r < reps; ++r) {
-        float ms = 0.f;
-        ED_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
-        tot += ms;
-    }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    ED_HIP_CHECK(hipEventSynchronize(sec->ev1));
+    float ms = 0.f;
+    ED_HIP_CHECK(hipEventElapsedTime(&ms, sec->ev0, sec->ev1));
     const size_t esz = precision == ED_PREC_F64 ? sizeof(double) : sizeof(float);
-    *avg_ms = tot / reps;
+    *avg_ms = (double)ms / reps;
     if (bytes_per_launch) *bytes_per_launch = 2.0 * esz * (double)n;
     return ED_OK;
 }

@@ -28,6 +28,8 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kBarGroups = 64;        // grid-barrier groups of the persistent Lanczos kernel
+constexpr int kBarGroupSize = 32;     // workgroups per group
 constexpr int kRed = 34;             // reduction scratch: two slots per wave of a 1024-thread workgroup + 2
 constexpr int kSums = 4;             // LDS slots for the reduced recurrence sums
 constexpr int kSchedules = 8;        // H.v schedules the autotuner times
@@ -53,10 +55,18 @@ constexpr int kScratchDoubles = kMisc + kMaxIter + 64;
 struct LzState {
     int j_k1;          // step index seen by the H.v phase (1-based)
     int j_k2;          // step index seen by the update phase
-    int status;        // 0 ok, 1 zero-norm vector appeared, 2 zero-norm start vector
+    int status;        // 0 ok, 1 zero-norm vector appeared, 2 zero-norm start vector,
+                       // 3 grid barrier of the persistent Lanczos kernel timed out
     int pad;
+    unsigned bar_count;   // persistent Lanczos kernel: group leaders' arrivals since its launch
+    int bar_err;          // a barrier wait exceeded its spin limit
+    unsigned bar_top;     // barriers released since the launch
+    int pad3;
     double alpha[kMaxIter];      // alpha_j        -> alpha[j-1]
     double beta[kMaxIter + 1];   // beta_1 = ||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
+    // grid barrier groups of the persistent Lanczos kernel, one 64-byte line per word
+    unsigned bar_grp[kBarGroups * 16];   // arrivals in group g: bar_grp[16 g]
+    unsigned bar_rel[kBarGroups * 16];   // barriers released in group g: bar_rel[16 g]
 };
 
 // One spin's tables on the device (tab_HAIM2 output plus derived layouts).
@@ -172,6 +182,8 @@ struct ed_sector {
     double tune_ms32[ed::kSchedules] = {};
     int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
     int colord_blocks = 0;
+    int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
+    int persist_ok[2] = {-1, -1};  // per precision (F64, F32): -1 not checked, 0 no, 1 co-resident grid
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double hv_ms_sum = 0.0;

@@ -41,23 +41,38 @@ __device__ inline double sum_partials(const double *p, int n, double *red)
 }
 
 // Fixed-order sum of n partial sums by ONE whole wave (64 lanes): lane l adds entries
-// l, l+64, l+128, ... left to right (eight coalesced loads in flight per batch), then
+// l, l+64, l+128, ... left to right (sixteen coalesced loads in flight per batch), then
 // the xor butterfly.  Addition is commutative, so the butterfly leaves the same bits in
 // every lane, and the order depends only on n: every kernel that reduces the same
 // partials with this function obtains the same value.
+// Device-scope coherent access (agent-scope relaxed atomics: no cache fences needed) for
+// data another workgroup of the same kernel writes.
+template <typename T>
+__device__ inline T ld_agent(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void st_agent(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// COH: the partials come from other workgroups of the running kernel (coherent loads).
+template <bool COH = false>
 __device__ inline double wave_part_sum(const double *p, int n)
 {
     const int lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int b0 = 0; b0 < n; b0 += 512) {
-        double v[8];
+    for (int b0 = 0; b0 < n; b0 += 1024) {
+        double v[16];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 16; ++q) {
             const int i = b0 + q * 64 + lane;
-            v[q] = i < n ? p[i] : 0.0;
+            v[q] = i < n ? (COH ? ld_agent(p + i) : p[i]) : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s += v[q];
+        for (int q = 0; q < 16; ++q) s += v[q];
     }
     return wave_sum(s);
 }
@@ -107,6 +122,35 @@ __device__ inline void block_partial(double v, double *red, double *out)
         double t = 0.0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
         *out = t;
+    }
+}
+
+// Two deterministic block sums in one pass, each in the order of block_partial (out1 may
+// be null).  COH: stored with device-scope coherent stores.
+template <bool COH = false>
+__device__ inline void block_partial2(double v0, double v1, double *red, double *out0, double *out1)
+{
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    __syncthreads();   // red may still be in use
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = v0;
+        red[2 * (threadIdx.x >> 6) + 1] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            t0 += red[2 * i];
+            t1 += red[2 * i + 1];
+        }
+        if (COH) {
+            st_agent(out0, t0);
+            if (out1) st_agent(out1, t1);
+        } else {
+            *out0 = t0;
+            if (out1) *out1 = t1;
+        }
     }
 }
 
@@ -379,18 +423,18 @@ __device__ inline void lz_init_body(const LzArgs<T> &lz, const double *start, in
 //   alpha_j = <H v_j - beta_j v_{j-1}, v_j> = Q / beta_j^2 - P / beta_{j-1}
 // which is lanc_opt's alpha (taken after the beta-subtraction, Lanczos_plain.h:116-140)
 // rearranged so that the H.v phase needs no scalar of step j.  sh[] receives N, P, Q.
-template <typename T>
+template <typename T, bool COH = false>
 __device__ inline void lz_scalars(const LzArgs<T> &lz, int j, double *sh)
 {
     const int w = threadIdx.x >> 6;
     if (w == 0) {
-        const double t = wave_part_sum(lz.nrm(j), lz.nB);
+        const double t = wave_part_sum<COH>(lz.nrm(j), lz.nB);
         if (threadIdx.x == 0) sh[0] = t;
     } else if (w == 1) {
-        const double t = j > 1 ? wave_part_sum(lz.crs(j), lz.nB) : 0.0;
+        const double t = j > 1 ? wave_part_sum<COH>(lz.crs(j), lz.nB) : 0.0;
         if (threadIdx.x == 64) sh[1] = t;
     } else if (w == 2) {
-        const double t = wave_part_sum(lz.partA, lz.nA);
+        const double t = wave_part_sum<COH>(lz.partA, lz.nA);
         if (threadIdx.x == 128) sh[2] = t;
     }
 }
@@ -472,6 +516,245 @@ __device__ inline void lz_finish_body(const LzArgs<T> &lz, double *sh)
             lz.st->beta[j - 1] = b;
             if (b < kMinLanczosNorm) lz.st->status = (j == 1) ? 2 : 1;
         }
+    }
+}
+
+// ------------------------------------------------ persistent Lanczos (cooperative launch)
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident at once).  The
+// data the workgroups exchange inside the kernel (Krylov vector, partial sums) is written
+// and read with device-scope coherent accesses (ld_agent / st_agent), so the barrier
+// needs no cache write-back or invalidation: every wave waits for its own accesses
+// (s_waitcnt 0) and the workgroup meets.  Then a two-level tree on counters that only
+// grow during the launch (zeroed before it): thread 0 adds one to its group's counter
+// (kBarGroupSize workgroups per group); the group's last arrival adds one to
+// st->bar_count, and the last of those stores the barrier number `b` (1, 2, ... in the
+// launch) in st->bar_top.  Group leaders poll bar_top and then release their group by
+// storing b in its word; the other workgroups poll that word.  So no word has more than
+// kBarGroupSize writers or readers.  A poll that exceeds `spin_limit` rounds (or sees
+// another workgroup's timeout) sets st->bar_err and status 3 and leaves, so the kernel
+// always ends; returns true then.
+__device__ inline bool bar_wait(unsigned *word, unsigned b, LzState *st, int spin_limit)
+{
+    for (int spins = 0;; ++spins) {
+        if (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= b) return false;
+        if (spins >= spin_limit ||
+            ((spins & 63) == 63 && __hip_atomic_load(&st->bar_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            return true;
+    }
+}
+
+__device__ inline bool grid_barrier(LzState *st, unsigned b, int spin_limit, int *flag)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nblk = gridDim.x, g = blockIdx.x / kBarGroupSize;
+        const unsigned ng = (nblk + kBarGroupSize - 1) / kBarGroupSize;
+        const unsigned sz = min((unsigned)kBarGroupSize, nblk - g * kBarGroupSize);
+        bool err = false;
+        const unsigned c = __hip_atomic_fetch_add(&st->bar_grp[16 * g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == sz * b - 1) {   // last of the group
+            const unsigned t = __hip_atomic_fetch_add(&st->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t == ng * b - 1) __hip_atomic_store(&st->bar_top, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else err = bar_wait(&st->bar_top, b, st, spin_limit);
+            __hip_atomic_store(&st->bar_rel[16 * g], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            err = bar_wait(&st->bar_rel[16 * g], b, st, spin_limit);
+        }
+        if (err) {
+            __hip_atomic_store(&st->bar_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st->status = 3;
+        }
+        *flag = err ? 1 : 0;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
+// Bytes of dynamic LDS of the persistent Lanczos kernel: the fast H.v's layout, then
+// u_{j-1} and the diagonal of the workgroup's rows (T[dup] each, 16-byte aligned).
+__host__ __device__ inline size_t lz_persist_lds(int dup, int ncoef, int ncfg, int nhund, size_t tsize)
+{
+    return (hv_fast_lds(dup, ncoef, ncfg, nhund, tsize) + 15) / 16 * 16 + 2 * tsize * (size_t)dup;
+}
+
+// `nsteps` Lanczos steps in one kernel: the fast H.v (one column per workgroup, 256
+// threads, rows r = tid + i*256, i < R) and the update of the same column, separated by
+// grid barriers.  The static data of a workgroup (its column's down-hop entries, the
+// packed up-hop entries of its rows in registers, the diagonal and the LDS tables) are
+// loaded once per launch; u_j and u_{j-1} of the workgroup's column stay in LDS and
+// z = H u_j in registers across the steps, so a step reads from memory only the
+// neighbour columns (and Hund targets) of u_j and the partial sums.  Per step:
+//   H.v:     z = H u_j, partial <z, u_j> -> partA
+//   barrier
+//   update:  N, P, Q as in lz_scalars; beta_j, alpha_j; u_{j+1} = z/beta_j -
+//            (beta_j/beta_{j-1}) u_{j-1} - (alpha_j/beta_j) u_j -> slot(j+1) and LDS;
+//            partials of ||u_{j+1}||^2, <u_{j+1}, u_j>
+//   barrier
+// Every sum is formed in the order of the two-kernel step with the same thread layout
+// (the fast kernel at 256 threads and the update kernel), so alpha and beta carry the
+// same bits.  Breakdown (beta_j below kMinLanczosNorm) is seen by every workgroup alike
+// and ends the loop everywhere.  The caller launches k_lz_finish afterwards.
+template <typename T, int K, int R>
+__device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, int nsteps, int spin_limit,
+                                       unsigned char *smem, double *red, double *sh, int *flag)
+{
+    constexpr int NT = kBlock;
+    LzState *st = lz.st;
+    if (st->status) return;   // written by an earlier kernel: the same for every workgroup
+    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund;
+    T *col = reinterpret_cast<T *>(smem);
+    T *coef = col + dup;
+    T *uimp = coef + a.ncoef_up;
+    T *hval = uimp + nq;
+    int *hptr = reinterpret_cast<int *>(hval + nh);
+    int *htr = hptr + nq + 1;
+    int *swdn = htr + nh;
+    T *colm = reinterpret_cast<T *>(smem + (hv_fast_lds(dup, a.ncoef_up, ncfg, nh, sizeof(T)) + 15) / 16 * 16);
+    T *dgl = colm + dup;
+    const int blk = blockIdx.x, tid = threadIdx.x;
+    const bool lead = blk == 0 && tid == 0;
+    unsigned nbar = 0;   // barriers passed in this launch
+    int j = st->j_k1;
+    const int ido = lz.colord ? lz.colord[blk] : blk;
+    const bool act = ido >= 0;
+    // ---- static data of the column and of the rows
+    int cd = 0, nd = 0;
+    T ddg = (T)0;
+    T od[K];
+    int jd[K];
+    if (act) {
+        cd = a.imp_dn[ido];
+        const int p0 = a.ptr_dn[ido];
+        nd = a.ptr_dn[ido + 1] - p0;
+        ddg = a.diag_dn[ido];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        od[k] = (act && k < nd) ? a.off_dn[a.ptr_dn[ido] + k] : (T)0;
+        jd[k] = (act && k < nd) ? a.rk_dn[a.ptr_dn[ido] + k] * dup : 0;
+    }
+    const int base = act ? ido * dup : 0;
+    uint32_t pk[R][K];
+    int cu[R];
+    {
+        const T *x0 = lz.slot(j);
+        const T *xm = lz.slot(j > 1 ? j - 1 : j);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = tid + i * NT;
+            const bool ok = act && r < dup;
+            cu[i] = ok ? a.imp_up[r] : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) pk[i][k] = ok ? a.ell_pk_up[k * dup + r] : 0u;
+            if (ok) {
+                dgl[r] = (a.diag_up[r] + ddg) + a.uimp[cu[i] * ncfg + cd];
+                col[r] = x0[base + r];
+                colm[r] = j > 1 ? xm[base + r] : (T)0;
+            }
+        }
+    }
+    for (int e = tid; e < a.ncoef_up; e += NT) coef[e] = a.coef_up[e];
+    for (int e = tid; e < nq; e += NT) uimp[e] = a.uimp[e];
+    if (a.hund && act) {
+        for (int e = tid; e <= nq; e += NT) hptr[e] = a.hund_ptr[e];
+        for (int e = tid; e < nh; e += NT) {
+            htr[e] = a.hund_tr[e];
+            hval[e] = a.hund_val[e];
+        }
+        for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
+    }
+    double bprev = j > 1 ? st->beta[j - 2] : 1.0;
+    __syncthreads();
+
+    for (int s = 0; s < nsteps; ++s) {
+        // ---- H.v phase of step j (u_j of the own column is in col)
+        const T *__restrict__ x = lz.slot(j);
+        T *__restrict__ y = lz.slot(j + 1);
+        T z[R];
+        double dot = 0.0;
+        {
+            // neighbour columns through uniform per-hop pointers: one 32-bit row offset
+            // per row serves every hop.  Rows go in batches of RB (the loads of a batch
+            // in flight together) to stay within the register budget of a co-resident grid.
+            const T *xk[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) xk[k] = x + jd[k];
+            constexpr int RB = R < 2 ? R : 2;
+#pragma unroll
+            for (int i0 = 0; i0 < R; i0 += RB) {
+                T xd[RB][K];
+#pragma unroll
+                for (int b = 0; b < RB; ++b) {
+                    const unsigned r = (unsigned)(tid + (i0 + b) * NT);
+                    const bool ok = act && (int)r < dup;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) xd[b][k] = (ok && k < nd) ? ld_agent(xk[k] + r) : (T)0;
+                }
+#pragma unroll
+                for (int b = 0; b < RB; ++b) {
+                    const int i = i0 + b, r = tid + i * NT;
+                    z[i] = (T)0;
+                    if (act && r < dup) {
+                        const T xv = col[r];
+                        T acc = dgl[r] * xv;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) acc += coef[pk[i][k] >> 16] * col[pk[i][k] & 0xFFFFu];
+#pragma unroll
+                        for (int k = 0; k < K; ++k) acc += od[k] * xd[b][k];
+                        if (a.hund) {
+                            const int h0 = hptr[cu[i] * ncfg + cd], h1 = hptr[cu[i] * ncfg + cd + 1];
+                            for (int t = h0; t < h1; ++t) {
+                                const int tr = htr[t];
+                                acc += hval[t] * ld_agent(x + a.swap_up[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup);
+                            }
+                        }
+                        z[i] = acc;
+                        dot += (double)acc * (double)xv;
+                    }
+                }
+            }
+        }
+        block_partial2<true>(dot, 0.0, red, lz.partA + blk, nullptr);
+        if (grid_barrier(st, ++nbar, spin_limit, flag)) return;
+
+        // ---- update phase of step j
+        lz_scalars<T, true>(lz, j, sh);
+        __syncthreads();
+        const double bj = sqrt(sh[0]);
+        if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+            if (lead) st->status = (j == 1) ? 2 : 1;
+            return;
+        }
+        const double alpha = sh[2] / (bj * bj) - (j > 1 ? sh[1] / bprev : 0.0);
+        if (lead) {
+            st->beta[j - 1] = bj;
+            st->alpha[j - 1] = alpha;
+        }
+        const double cz = 1.0 / bj, cp = j > 1 ? bj / bprev : 0.0, cx = alpha / bj;
+        double nn = 0.0, cr = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = tid + i * NT;
+            if (act && r < dup) {
+                const T xv = col[r];
+                const T v = (T)(cz * (double)z[i] - cp * (double)colm[r] - cx * (double)xv);
+                st_agent(y + base + r, v);
+                nn += (double)v * (double)v;
+                cr += (double)v * (double)xv;
+                colm[r] = xv;
+                col[r] = v;
+            }
+        }
+        block_partial2<true>(nn, cr, red, lz.nrm(j + 1) + blk, lz.crs(j + 1) + blk);
+        bprev = bj;
+        ++j;
+        if (lead) {
+            st->j_k1 = j;
+            st->j_k2 = j - 1;
+        }
+        if (grid_barrier(st, ++nbar, spin_limit, flag)) return;
     }
 }
 

@@ -141,6 +141,23 @@ class Sector:
     def set_schedule(self, choice: int) -> None:
         check(lib().ed_hmult_set_schedule(self.handle, int(choice)))
 
+    def lanczos_benchmark(self, precision="f64", niter: int = 128, reps: int = 5) -> tuple[float, float, bool]:
+        """(ms per fixed-length run of niter steps, algorithmic bytes per run, persistent kernel used)."""
+        ms, by, pk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().ed_lanczos_benchmark(self.handle, _prec(precision), int(niter), int(reps), ctypes.byref(ms),
+                                         ctypes.byref(by), ctypes.byref(pk)))
+        return ms.value, by.value, bool(pk.value)
+
+    def set_persistent(self, enable: bool) -> None:
+        """Use the persistent Lanczos kernel where it applies (True, the default) or never."""
+        check(lib().ed_lanczos_set_persistent(self.handle, 1 if enable else 0))
+
+    def persistent_active(self, precision="f64") -> bool:
+        """Whether the next Lanczos run in `precision` uses the persistent kernel."""
+        v = ctypes.c_int()
+        check(lib().ed_lanczos_persistent_active(self.handle, _prec(precision), ctypes.byref(v)))
+        return bool(v.value)
+
     def hmult_benchmark(self, precision="f64", reps: int = 50) -> tuple[float, float]:
         ms, by = ctypes.c_double(), ctypes.c_double()
         check(lib().ed_hmult_benchmark(self.handle, _prec(precision), reps, ctypes.byref(ms), ctypes.byref(by)))

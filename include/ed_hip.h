@@ -136,6 +136,32 @@ int ed_hmult_tune(ed_sector *sec, int *choice, double *ms_per_choice);
  * a schedule the autotuner did not pick.  No reference counterpart. */
 int ed_hmult_set_schedule(ed_sector *sec, int choice);
 
+/* Times `reps` fixed-length runs of `niter` Lanczos steps (after one warm-up run), each
+ * from the default start vector: a HIP event pair on the sector's stream around the
+ * steps of a run (the persistent kernel's one launch, or the captured two-kernel chunks).
+ * *avg_ms = average per run; *bytes_per_run = its algorithmic HBM bytes 5 * sizeof(T) *
+ * dimen_H() * niter (per step: read u_j and write H u_j in the H.v, read u_{j-1}, u_j and
+ * write u_{j+1} in the recurrence); *persistent = 1 when the persistent kernel ran.  No
+ * reference counterpart: a measurement hook. */
+int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, double *avg_ms, double *bytes_per_run,
+                         int *persistent);
+
+/* Persistent Lanczos kernel (no reference counterpart; an engine option).  Where the
+ * sector's H.v schedule is a fast-kernel one (8, 16, 18-20), d_up <= 1024 and every
+ * workgroup of the grid fits on the device at once, fixed-length runs of at least 16
+ * steps (ed_lanczos_run, the excitation runs of ed_dynamic) execute in one cooperative
+ * launch: H.v and update separated by grid barriers, the static row data in registers
+ * and LDS, the vectors exchanged with device-scope coherent loads and stores.  The
+ * arithmetic and the order of every sum are those of the two-kernel step, so alpha and
+ * beta are the same numbers.  enable = 1 (default): use it where it applies; 0: never.
+ * A barrier wait that exceeds a few seconds stops the run with ED_ERR_HIP and turns
+ * the kernel off for the sector. */
+int ed_lanczos_set_persistent(ed_sector *sec, int enable);
+
+/* *active = 1 when a fixed-length Lanczos run of at least 16 steps in `precision` on
+ * this sector uses the persistent kernel (tunes the sector first if needed), else 0. */
+int ed_lanczos_persistent_active(ed_sector *sec, int precision, int *active);
+
 /* ------------------------------------------------------------------ Lanczos */
 
 /* Deterministic start vector (period 10000, values k/1000 in (0, 1]) used when a
@@ -316,12 +342,13 @@ int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *p
 
 /* ----------------------------------------------------------- diagnostics */
 
-/* Runs a fresh FP64 (ED_PREC_F64) or FP32 (ED_PREC_F32) Lanczos recurrence of
- * `reps` steps from the default start vector, launching its kernels one by one with
- * a HIP event pair around every fused H.v kernel on the sector's stream.
- * *avg_ms = average duration of that kernel; *bytes_per_launch = its algorithmic
- * HBM bytes 2 * sizeof(T) * dimen_H() (read u_j, write H u_j).  No
- * reference counterpart: it is the measurement hook of the engine. */
+/* Starts an FP64 (ED_PREC_F64) or FP32 (ED_PREC_F32) Lanczos recurrence from the
+ * default start vector and launches the fused H.v of its first step (z = H u_1 and
+ * the partial sums of <z, u_1>) `reps` times back to back, after one warm-up launch,
+ * between one HIP event pair on the sector's stream.  *avg_ms = elapsed / reps (the
+ * kernel's duration plus the dispatch gap); *bytes_per_launch = its algorithmic HBM
+ * bytes 2 * sizeof(T) * dimen_H() (read u_j, write H u_j).  No reference
+ * counterpart: it is the measurement hook of the engine. */
 int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, double *bytes_per_launch);
 
 #ifdef __cplusplus

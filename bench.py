@@ -26,13 +26,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Lanczos iters/s (Ns=12 Sz=0) + batched G(iω) solves/s vs plain, |ΔE0|<1e-8"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the FP64 fused H.v in the (6,6) sector (dim 853,776), by H.v
-# schedule: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
-# `bench.py --steps 1 --warmup 0 --no-cpu --no-green --no-scan --no-grad`, FETCH_SIZE scaled
-# by 65536 / 32780.3 (a 64 MiB device copy reads as 32,780.3 KiB), WRITE_SIZE by 1.0 (it
-# reads 65,536 KiB).  Summary: profiles/hv_fused_ns12_66_pmc.md.  Schedules 8 and 16 share
-# one kernel instance and were not separated; they report null.
+# Measured HBM bytes in the (6,6) sector (dim 853,776): rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE passes over `bench.py --steps 1 --warmup 0 --no-cpu --no-green --no-scan
+# --no-grad`, FETCH_SIZE scaled by 65536 / 32780.3 (a 64 MiB device copy reads as 32,780.3
+# KiB), WRITE_SIZE by 1.0 (it reads 65,536 KiB); summary in profiles/lanczos_ns12_66_pmc.md.
+# FP64 fused H.v per launch, by H.v schedule (8 and 16 share one kernel instance and were
+# not separated: null):
 HV_TRAFFIC_F64 = {18: 22816400, 19: 22834620, 20: 22827147}
+# persistent Lanczos kernel k_lz_persist<double, 8, 4>, per step
+PERSIST_TRAFFIC_F64_STEP = 24405937
 
 
 def log(msg):
@@ -259,8 +261,9 @@ def main():
     pk_rows = 1 if d_up <= 256 else 2 if d_up <= 512 else 4
     kname = (f"k_lz_persist<double, 8, {pk_rows}> (persistent Lanczos, {args.niter} steps per launch)" if persistent
              else f"{kname_hv} + k_lz_update<double>")
+    traffic = PERSIST_TRAFFIC_F64_STEP * args.niter if (persistent and pk_rows == 4) else None
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": kname, "avg_us": run_ms * 1e3, "bytes_per_launch": run_bytes,
+                "traffic": traffic, "kernel": kname, "avg_us": run_ms * 1e3, "bytes_per_launch": run_bytes,
                 "steps_per_launch": args.niter,
                 "hv_fused": {"kernel": kname_hv, "achieved": ach_hv, "frac": ach_hv / HBM_PEAK_GBS,
                              "avg_us": avg_ms * 1e3, "bytes_per_launch": bytes_l,

@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "ed_internal.h"
+#include "ed_kernels.h"
 
 namespace ed {
 
@@ -202,6 +202,36 @@ static int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_p
     return rc;
 }
 
+// |<n|phi>|^2 for the nst eigenvectors (rows of vec, dim each) of a small sector: one
+// workgroup per eigenvector, fixed-order block sum.
+__global__ __launch_bounds__(kBlock) void k_overlap2(int dim, const double *vec, const double *phi, double *w)
+{
+    __shared__ double red[kRed];
+    const double *v = vec + (size_t)blockIdx.x * dim;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < dim; i += kBlock) s += v[i] * phi[i];
+    s = block_sum_all(s, red);
+    if (threadIdx.x == 0) w[blockIdx.x] = s * s;
+}
+
+// Lehmann sum of compute_dynamic_full_ed: dyn(i w_k) = sum_n w_n / (i w_k - ph (E_n - E)),
+// poles in index order, one thread per frequency.
+__global__ void k_lehmann(int nst, const double *e, const double *w, double e0, double ph, int nw, const double *wn,
+                          double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nw) return;
+    double re = 0.0, im = 0.0;
+    for (int n = 0; n < nst; ++n) {
+        // w / (x + i y) with x = -ph (E_n - E), y = w_k
+        const double x = -ph * (e[n] - e0), y = wn[k], m2 = x * x + y * y;
+        re += w[n] * x / m2;
+        im -= w[n] * y / m2;
+    }
+    out[2 * k] = re;
+    out[2 * k + 1] = im;
+}
+
 }  // namespace ed
 
 using namespace ed;
@@ -223,6 +253,43 @@ int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, i
     (void)hipFreeAsync(d_psi, st);
     (void)hipFreeAsync(d_phi, st);
     ED_HIP_CHECK(hipStreamSynchronize(st));
+    return rc;
+}
+
+int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
+                       const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn)
+{
+    if (dim < 1 || nst < 0 || !phi || (nst > 0 && (!evals || !evecs)) || nw < 1 || !wn || !dyn)
+        return fail(ED_ERR_ARG, "ed_dynamic_full_ed: bad argument");
+    if (iph != 1 && iph != 2) return fail(ED_ERR_ARG, "ed_dynamic_full_ed: iph must be 1 (particle) or 2 (hole)");
+    std::fill(dyn, dyn + 2 * (size_t)nw, 0.0);
+    double nrm2 = 0.0;
+    for (int i = 0; i < dim; ++i) nrm2 += phi[i] * phi[i];
+    // green_class_compute_dynamic.f90:117-119: below the cutoff, or no stored state of the
+    // target sector (iisector = 0, :135), the contribution is zero
+    if (nrm2 < cutoff_dynamic || nst == 0) return ED_OK;
+    hipStream_t st = nullptr;
+    double *d = nullptr;
+    const size_t tot = (size_t)nst * dim + dim + 2 * (size_t)nst + nw + 2 * (size_t)nw;
+    ED_HIP_CHECK(hipMalloc((void **)&d, sizeof(double) * tot));
+    double *d_vec = d, *d_phi = d_vec + (size_t)nst * dim, *d_e = d_phi + dim, *d_w = d_e + nst, *d_wn = d_w + nst,
+           *d_out = d_wn + nw;
+    int rc = ED_OK;
+    if (hipMemcpy(d_vec, evecs, sizeof(double) * nst * dim, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_phi, phi, sizeof(double) * dim, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_e, evals, sizeof(double) * nst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_wn, wn, sizeof(double) * nw, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(ED_ERR_HIP, "ed_dynamic_full_ed: copy to the device failed");
+    } else {
+        hipLaunchKernelGGL(k_overlap2, dim3(nst), dim3(kBlock), 0, st, dim, (const double *)d_vec,
+                           (const double *)d_phi, d_w);
+        hipLaunchKernelGGL(k_lehmann, dim3((nw + kBlock - 1) / kBlock), dim3(kBlock), 0, st, nst, (const double *)d_e,
+                           (const double *)d_w, energy, iph == 1 ? 1.0 : -1.0, nw, (const double *)d_wn, d_out);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(dyn, d_out, sizeof(double) * 2 * nw, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ED_ERR_HIP, "ed_dynamic_full_ed: kernel or copy back failed");
+    }
+    (void)hipFree(d);
     return rc;
 }
 

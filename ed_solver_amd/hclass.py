@@ -339,6 +339,22 @@ def compute_dynamic(sector: Sector, iph: int, energy: float, phi: np.ndarray, ni
     return dyn[0::2] + 1j * dyn[1::2], a, b
 
 
+def dynamic_full_ed(iph: int, energy: float, phi: np.ndarray, evals: np.ndarray, evecs: np.ndarray, wn: np.ndarray,
+                    cutoff_dynamic: float = 1e-8) -> np.ndarray:
+    """compute_dynamic_full_ed: Lehmann sum over the given eigenpairs of a small target
+    sector (evecs one state per row); returns dyn(i w_n) complex."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    ev = np.ascontiguousarray(evals, np.float64).reshape(-1)
+    vec = np.ascontiguousarray(evecs, np.float64).reshape(len(ev), -1) if len(ev) else np.zeros((0, phi.size))
+    if len(ev) and vec.shape[1] != phi.size:
+        raise EDError(-1, "dynamic_full_ed: eigenvector length differs from phi")
+    wn = np.ascontiguousarray(wn, np.float64)
+    out = np.zeros(2 * len(wn))
+    check(lib().ed_dynamic_full_ed(phi.size, int(iph), float(energy), dptr(phi), len(ev), dptr(ev) if len(ev) else None,
+                                   dptr(vec) if len(ev) else None, float(cutoff_dynamic), len(wn), dptr(wn), dptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
 def compute_dynamic_batched(sectors, phis, iphs, energies, nitergreen: int, wn: np.ndarray,
                             cutoff_dynamic: float = 1e-8, precision="f64"):
     """All excitation runs in one call; returns dyn (njobs, nw) complex."""

@@ -23,7 +23,7 @@ from math import comb
 import numpy as np
 
 from .aim import AIMParams
-from .hclass import Sector, apply_C, compute_dynamic_batched
+from .hclass import Sector, apply_C, compute_dynamic_batched, dynamic_full_ed
 
 FULL_ED_SWITCH = 40   # Global_Variables.f90:23
 
@@ -42,12 +42,15 @@ def all_sectors(ns: int) -> list[tuple[int, int]]:
 def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128,
                        tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
                        precision="f64", which_lanczos: str = "GPU", neigen_arpack: int | None = None,
-                       arpack_ncv: int | None = None):
+                       arpack_ncv: int | None = None, full_ed_green: bool = False):
     """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax].
 
     Per sector (solver.h:150-200): block Lanczos when block_size > 0, unless N_up or N_dn
     is 0 or 1 or both are Ns; otherwise dense ED at dimension <= full_ed_switch, else
-    which_lanczos = 'NORMAL' / 'GPU' (the device Lanczos), 'FULL_ED' or 'ARPACK'."""
+    which_lanczos = 'NORMAL' / 'GPU' (the device Lanczos), 'FULL_ED' or 'ARPACK'.
+    full_ed_green (FLAG_FULL_ED_GREEN): dense-ED sectors keep all their states
+    (ED_ARPACK.f90:652-665) and neither window filter runs (solver.h:211,
+    solver.f90:93), so compute_green's Lehmann sums over small sectors are complete."""
     sectors = all_sectors(model.ns) if sectors is None else sectors
     states: list[EigenState] = []
     ns = model.ns
@@ -56,7 +59,7 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
         plain = block_size == 0 or nu in (0, 1) or nd in (0, 1) or (nu == ns and nd == ns)
         with Sector(model, nu, nd) as sec:
             if dim <= full_ed_switch or (plain and which_lanczos == "FULL_ED"):
-                ev, vec = sec.full_ed(min(neigen, dim))
+                ev, vec = sec.full_ed(dim if full_ed_green else min(neigen, dim))
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
             elif not plain:
                 from .block import block_lanczos
@@ -75,10 +78,12 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
             continue
         e_min = min(e for e, _ in found)
         for e, v in found:   # filter_eigen(lowest, [dE, dE + dEmax]) per sector
-            if e <= e_min + dEmax:
+            if full_ed_green or e <= e_min + dEmax:
                 states.append(EigenState((nu, nd), float(e), v))
     if not states:
         return []
+    if full_ed_green:
+        return states
     e0 = min(s.energy for s in states)
     return [s for s in states if s.energy <= e0 + dEmax]
 
@@ -125,12 +130,15 @@ def partition(states: list[EigenState], beta: float) -> np.ndarray:
 
 def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
                   spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
-                  precision="f64") -> np.ndarray:
+                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
     """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
 
     For every state with Boltzmann weight above weight_cutoff and every orbital: the
-    particle run from c+|psi> in (N+1) and the hole run from c|psi> in (N-1); all of
-    them advance together in one batched device call."""
+    particle run from c+|psi> in (N+1) and the hole run from c|psi> in (N-1).  As
+    compute_dynamic dispatches (green_class_compute_dynamic.f90:44-62): a target sector
+    of dimension <= full_ed_switch takes the Lehmann sum over the states `states` holds
+    for it (compute_dynamic_full_ed, :77-165; none held gives zero), every other run is
+    an excitation Lanczos, and those all advance together in one batched device call."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     w = partition(states, beta)
     G = np.zeros((len(orbitals), len(wn)), complex)
@@ -143,6 +151,14 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
         return sectors[key]
 
     jobs = []   # (orbital index, weight, sector, phi, iph, energy)
+    held: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
+
+    def held_states(key):
+        if key not in held:
+            sel = [x for x in states if x.sector == key]
+            held[key] = (np.array([x.energy for x in sel]), np.array([x.vector for x in sel]).reshape(len(sel), -1))
+        return held[key]
+
     try:
         for st, wt in zip(states, w):
             if wt <= weight_cutoff:
@@ -156,7 +172,11 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
                         continue
                     dst = sector(*tn)
                     phi = apply_C(src, dst, c, spin, pm, st.vector)
-                    jobs.append((io, wt, dst, phi, iph, st.energy))
+                    if dst.dim <= full_ed_switch:
+                        ev, vec = held_states(tn)
+                        G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
+                    else:
+                        jobs.append((io, wt, dst, phi, iph, st.energy))
         if jobs:
             dyn, _, _ = compute_dynamic_batched([j[2] for j in jobs], [j[3] for j in jobs], [j[4] for j in jobs],
                                                 [j[5] for j in jobs], nitergreen, wn, cutoff_dynamic, precision)

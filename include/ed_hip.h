@@ -329,6 +329,17 @@ int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int ni
                double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn,
                double *a, double *b);
 
+/* compute_dynamic_full_ed (green_class_compute_dynamic.f90:77-165), the dynamic
+ * correlation of a target sector of dimension dim <= FULL_ED_SWITCH (40,
+ * Global_Variables.f90:23) from the nst eigenpairs stored for that sector (evals[nst],
+ * evecs[nst][dim] row per state; the reference uses the states its eigen list holds,
+ * which are all of them only with FLAG_FULL_ED_GREEN):
+ *   dyn(i wn) = sum_n |<n|phi>|^2 / (i wn - ph (E_n - energy)),  ph = +1 (iph = 1) / -1.
+ * ||phi||^2 < cutoff_dynamic or nst = 0 gives dyn = 0.  Overlaps and the Lehmann sum on
+ * the device; dyn complex interleaved [2*nw]. */
+int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
+                       const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn);
+
 /* Batched excitation Lanczos (no reference counterpart as one call: it fuses the
  * per-(state, orbital, iph) calls of compute_dynamic made by
  * green_class_computeAA.h:101-302): njobs independent runs, job k in sector

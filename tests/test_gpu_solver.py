@@ -111,3 +111,54 @@ def test_solver_which_lanczos_options(gpu):
     for sec in secs:
         assert abs(e["GPU"][sec] - e["FULL_ED"][sec]) < 1e-10
         assert abs(e["ARPACK"][sec] - e["FULL_ED"][sec]) < 1e-10
+
+
+def test_dynamic_full_ed_lehmann(gpu):
+    """compute_dynamic_full_ed (target dimension <= 40): the device Lehmann sum over all
+    states of the target sector equals the dense resolvent <phi|(iw -+ (H - E))^-1|phi>,
+    and equals the excitation-Lanczos continued fraction once the Krylov space is the
+    whole sector."""
+    from ed_solver_amd.hclass import apply_C, compute_dynamic, dynamic_full_ed
+    m = small_model(1, 3, 2.0, 0.0, seed=6)   # Ns=4: every sector has dimension <= 36
+    beta = 20.0
+    wn = np.pi / beta * (2 * np.arange(1, 65) - 1)
+    s0 = gpu.Sector(m, 2, 2)
+    ev0, vec0 = s0.full_ed(1)
+    for pm, iph, tn in ((+1, 1, (3, 2)), (-1, 2, (1, 2))):
+        dst = gpu.Sector(m, *tn)
+        phi = apply_C(s0, dst, 0, 0, pm, vec0[0])
+        ev, vec = dst.full_ed(dst.dim)
+        dyn = dynamic_full_ed(iph, ev0[0], phi, ev, vec, wn)
+        H = bf.sector_hamiltonian(m, *tn).toarray()
+        ph = 1.0 if iph == 1 else -1.0
+        ref = np.array([phi @ np.linalg.solve(1j * w * np.eye(dst.dim) - ph * (H - ev0[0] * np.eye(dst.dim)), phi)
+                        for w in wn])
+        assert np.max(np.abs(dyn - ref)) < 1e-12
+        cf = compute_dynamic(dst, iph, ev0[0], phi, dst.dim, wn)[0]
+        assert np.max(np.abs(dyn - cf)) < 1e-9
+        # only the states held for the target sector enter (the reference's eigen list)
+        part = dynamic_full_ed(iph, ev0[0], phi, ev[:3], vec[:3], wn)
+        amp = (vec[:3] @ phi) ** 2
+        ref3 = np.array([np.sum(amp / (1j * w - ph * (ev[:3] - ev0[0]))) for w in wn])
+        assert np.max(np.abs(part - ref3)) < 1e-13
+        assert np.all(dynamic_full_ed(iph, ev0[0], phi, ev[:0], vec[:0], wn) == 0)
+        dst.close()
+    s0.close()
+
+
+def test_green_full_ed_green_flag(gpu):
+    """With FLAG_FULL_ED_GREEN every small sector keeps all its states and G through the
+    Lehmann path equals G through excitation Lanczos (full_ed_switch = 0, 36 >= dim
+    steps)."""
+    from ed_solver_amd.solver import compute_green, get_eigen_energies
+    m = small_model(1, 3, 2.5, 0.0, seed=7)   # Ns=4
+    beta = 15.0
+    wn = np.pi / beta * (2 * np.arange(1, 33) - 1)
+    states = get_eigen_energies(m, neigen=2, dEmax=0.5, full_ed_green=True)
+    assert len(states) == 4 ** 4
+    g_led = compute_green(m, states, beta, wn, nitergreen=36, weight_cutoff=1e-12)
+    g_cf = compute_green(m, states, beta, wn, nitergreen=36, weight_cutoff=1e-12, full_ed_switch=0)
+    assert np.max(np.abs(g_led - g_cf)) < 1e-9
+    # without the flag the windows filter the states and the Lehmann sums are partial
+    kept = get_eigen_energies(m, neigen=2, dEmax=0.5)
+    assert len(kept) < len(states)

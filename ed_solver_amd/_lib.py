@@ -54,6 +54,10 @@ SIGNATURES = {
     "ed_last_error": (ctypes.c_char_p, []),
     "ed_version": (ctypes.c_char_p, []),
     "ed_init": (ctypes.c_int, [ctypes.c_int]),
+    "ed_sector_create_sz": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(_vp)]),
+    "ed_sector_energy_shift": (ctypes.c_int, [_vp, _dp]),
+    "ed_sector_quart_sz": (ctypes.c_int, [_vp, _dp]),
     "ed_sector_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "ed_sector_destroy": (ctypes.c_int, [_vp]),
     "ed_sector_dim": (ctypes.c_int64, [_vp]),

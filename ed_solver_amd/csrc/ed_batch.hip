@@ -133,7 +133,7 @@ static int ws_reserve(size_t bytes, int device)
     return ED_OK;
 }
 
-static bool rows_capable(const ed_sector *s) { return s->up.d >= 128 && s->up.kpad <= 32; }
+static bool rows_capable(const ed_sector *s) { return !s->sz && s->up.d >= 128 && s->up.kpad <= 32; }
 
 template <typename T>
 static HvArgs<T> hv_args_of(const ed_sector *s)

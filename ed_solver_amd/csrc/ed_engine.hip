@@ -121,6 +121,16 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     if (MODE == 1) block_partial(dot, red, lz.partA + blockIdx.x);
 }
 
+// S_z sectors: HAIMsz_multr, one thread per row.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_sz(SzArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
+                                                   int64_t ld, LzArgs<T> lz)
+{
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_sz_body<T, MODE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
+}
+
 // ------------------------------------------------------------- Lanczos kernels
 
 template <typename T>
@@ -340,6 +350,7 @@ static int width_of(int kmax)
 
 static bool fast_ok(const ed_sector *s)
 {
+    if (s->sz) return false;
     const int k = width_of(std::max(s->up.kpad, s->dn.kpad));
     // widths above 16 exceed the 128-VGPR budget of a 1024-thread workgroup in FP64
     return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 && s->dn.d <= 8184 &&
@@ -362,6 +373,13 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     h.k = s->up.kpad;
     h.xcd = 0;
     h.rows = 0;
+    if (s->sz) {   // S_z sector: one row per thread
+        h.kind = 3;
+        h.cpb = 1;
+        h.nblk = (int)((s->dim + kBlock - 1) / kBlock);
+        h.lds = 0;
+        return h;
+    }
     const int rows = sched >= 18 ? sched - 16 : 1;
     const int nt_rows = std::max(64, ((dup + rows - 1) / rows + 63) / 64 * 64);
     if ((sched == 8 || sched >= 16) && fast_ok(s) && (rows == 1 || nt_rows <= fast_max_threads(rows))) {
@@ -431,6 +449,18 @@ template <typename T, int MODE>
 static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *y, int64_t ld, int nvec,
                            const LzArgs<T> &lz, hipStream_t st)
 {
+    if (h.kind == 3) {
+        const bool f64 = std::is_same<T, double>::value;
+        SzArgs<T> za;
+        za.d = s->dim;
+        za.kw = s->up.kpad;
+        za.rk = s->up.ell_rk;
+        za.val = (const T *)(f64 ? (const void *)s->up.ell_val : (const void *)s->up.ell_val32);
+        za.diag = (const T *)(f64 ? (const void *)s->up.diag : (const void *)s->up.diag32);
+        za.quart = (const T *)(f64 ? (const void *)s->d_quart : (const void *)s->d_quart32);
+        hipLaunchKernelGGL((k_hv_sz<T, MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za, x, y, ld, lz);
+        return;
+    }
     HvArgs<T> a = hv_args<T>(s, h.cpb);
     if (h.xcd) a.colord = s->d_colord;
     dim3 grid(h.nblk, MODE ? 1 : nvec);
@@ -893,7 +923,7 @@ static int tune(ed_sector *s)
     hipStream_t st = s->stream;
     for (double &t : s->tune_ms) t = 0.0;
     for (double &t : s->tune_ms32) t = 0.0;
-    if (s->dim < 4096) {
+    if (s->sz || s->dim < 4096) {
         s->cpb = fast_ok(s) ? 8 : 1;
         s->cpb32 = 0;
         return ED_OK;
@@ -1141,6 +1171,7 @@ int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
 int ed_hmult_set_schedule(ed_sector *sec, int choice)
 {
     ED_TRY(check_common(sec, 0));
+    if (sec->sz) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: an S_z sector has one H.v schedule");
     if (!sched_valid(choice))
         return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4, 8, 16, 18, 19 or 20");
     if (choice >= 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");

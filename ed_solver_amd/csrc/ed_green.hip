@@ -114,6 +114,7 @@ static int grid_for(int64_t n)
 static int check_pair(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm)
 {
     if (!src || !dst) return fail(ED_ERR_ARG, "ed_apply_c: NULL sector");
+    if (src->sz || dst->sz) return fail(ED_ERR_ARG, "ed_apply_c: (N_up, N_dn) sectors only (apply_Cupdo)");
     if (src->nc != dst->nc || src->nb != dst->nb) return fail(ED_ERR_ARG, "ed_apply_c: sectors of different models");
     if (site < 0 || site >= src->nc) return fail(ED_ERR_ARG, "ed_apply_c: site outside [0, nc)");
     if ((spin != 0 && spin != 1) || (pm != 1 && pm != -1)) return fail(ED_ERR_ARG, "ed_apply_c: spin 0/1, pm +1/-1");

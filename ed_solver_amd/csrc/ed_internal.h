@@ -182,6 +182,13 @@ struct ed_sector {
     double tune_ms32[ed::kSchedules] = {};
     int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
     int colord_blocks = 0;
+    // S_z (Nambu) sector: one table over 2 Ns bits in `up` (dn.d = 1), the quartic
+    // diagonal QUART_INT_SZ in d_quart, and the constant tr(Ec_dn) + tr(Eb_dn) that the
+    // particle-hole transformation moves out of H
+    int sz = 0;
+    double *d_quart = nullptr;
+    float *d_quart32 = nullptr;
+    double energy_shift = 0.0;
     int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
     int persist_ok[2] = {-1, -1};  // per precision (F64, F32): -1 not checked, 0 no, 1 co-resident grid
     // timing of the H.v kernel inside Lanczos chunks

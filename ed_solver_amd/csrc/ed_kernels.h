@@ -519,6 +519,51 @@ __device__ inline void lz_finish_body(const LzArgs<T> &lz, double *sh)
     }
 }
 
+// ------------------------------------------------------------- S_z (Nambu) sectors
+
+// Tables of an S_z sector: ELL [kw][d] of tab_HAIM2 over the Nambu orbitals, the
+// quadratic diagonal and QUART_INT_SZ.
+template <typename T>
+struct SzArgs {
+    int64_t d;
+    int kw;
+    const int32_t *rk;
+    const T *val;
+    const T *diag;
+    const T *quart;
+};
+
+// HAIMsz_multr (HAIMsz.f90:453-560, USE_CC gather form): one thread per row,
+//   w_i = QUART_i v_i + diag_i v_i + sum_k off_ik v_{rank_ik}   in this order.
+// MODE 1: Lanczos phase 1 on the Krylov slots with the partial <z, u_j> per workgroup.
+template <typename T, int MODE>
+__device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
+                                  const LzArgs<T> &lz, int blk, double *red)
+{
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int64_t i = (int64_t)blk * blockDim.x + threadIdx.x, d = a.d;
+    double dot = 0.0;
+    if (i < d) {
+        const T xv = x[i];
+        T acc = a.quart[i] * xv;
+        acc += a.diag[i] * xv;
+        for (int k = 0; k < a.kw; ++k) acc += a.val[k * d + i] * x[a.rk[k * d + i]];
+        y[i] = acc;
+        dot = (double)acc * (double)xv;
+    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+}
+
 // ------------------------------------------------ persistent Lanczos (cooperative launch)
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident at once).  The

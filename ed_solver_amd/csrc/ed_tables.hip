@@ -77,12 +77,33 @@ __global__ void k_rank_table(int ns, int32_t *rank)
     rank[s] = r;
 }
 
-// Model parameters of one spin, device copies (row-major).
+// Quadratic model tabulated by tab_HAIM2: nimp impurity and nbath bath orbitals, their
+// bit positions in the state word, and device copies of the row-major matrices
+// ec[nimp][nimp], eb[nbath][nbath], vbc[nbath][nimp].  One spin of the (N_up, N_dn)
+// basis: nimp = Nc at bits nb + c, nbath = Nb at bits b.  The Nambu basis of the S_z
+// sectors: 2 Nc and 2 Nb orbitals over 2 Ns bits (AIM2%IMPiorb / BATHiorb,
+// quantum_algebra_Anderson_Impurity_class2.f90:83-89).
+constexpr int kMaxImpOrb = 12, kMaxBathOrb = 28;
 struct SpinModel {
-    int nc, nb;
+    int nimp, nbath;
+    int ibit[kMaxImpOrb], bbit[kMaxBathOrb];
     const double *ec, *eb, *vbc;
     double cut;
 };
+
+static SpinModel spin_model(int nc, int nb, const double *ec, const double *eb, const double *vbc, double cut)
+{
+    SpinModel m{};
+    m.nimp = nc;
+    m.nbath = nb;
+    for (int c = 0; c < nc; ++c) m.ibit[c] = nb + c;
+    for (int b = 0; b < nb; ++b) m.bbit[b] = b;
+    m.ec = ec;
+    m.eb = eb;
+    m.vbc = vbc;
+    m.cut = cut;
+    return m;
+}
 
 template <bool FILL>
 __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *rank,
@@ -90,7 +111,7 @@ __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
-    const int nc = m.nc, nb = m.nb;
+    const int nc = m.nimp, nb = m.nbath;
     const uint32_t s = (uint32_t)states[i];
     int k = FILL ? ptr[i] : 0;
     const int k0 = k;
@@ -99,16 +120,16 @@ __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *
     if (FILL) {
         double dg = 0.0;
         for (int c = 0; c < nc; ++c)
-            if ((s >> (nb + c)) & 1u) dg += m.ec[c * nc + c];
+            if ((s >> m.ibit[c]) & 1u) dg += m.ec[c * nc + c];
         for (int b = 0; b < nb; ++b)
-            if ((s >> b) & 1u) dg += m.eb[b * nb + b];
+            if ((s >> m.bbit[b]) & 1u) dg += m.eb[b * nb + b];
         diag[i] = dg;
     }
     for (int a = 0; a < nc; ++a)
         for (int b = 0; b < nc; ++b) {
             double t = m.ec[a * nc + b];
             if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, nb + a, nb + b, &o, &sg)) {
+            if (hop(s, m.ibit[a], m.ibit[b], &o, &sg)) {
                 if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
                 ++k;
             }
@@ -117,7 +138,7 @@ __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *
         for (int b = 0; b < nb; ++b) {
             double t = m.eb[a * nb + b];
             if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, a, b, &o, &sg)) {
+            if (hop(s, m.bbit[a], m.bbit[b], &o, &sg)) {
                 if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
                 ++k;
             }
@@ -126,11 +147,11 @@ __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *
         for (int c = 0; c < nc; ++c) {
             double t = m.vbc[b * nc + c];
             if (t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, b, nb + c, &o, &sg)) {
+            if (hop(s, m.bbit[b], m.ibit[c], &o, &sg)) {
                 if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
                 ++k;
             }
-            if (hop(s, nb + c, b, &o, &sg)) {
+            if (hop(s, m.ibit[c], m.bbit[b], &o, &sg)) {
                 if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
                 ++k;
             }
@@ -293,7 +314,7 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
         ED_HIP_CHECK(hipMemcpyAsync(d_vbc, S->vbc.data() + (size_t)spin * nb * nc, sizeof(double) * nb * nc,
                                     hipMemcpyHostToDevice, st));
     }
-    SpinModel m{nc, nb, d_ec, d_eb, d_vbc, S->cutoff};
+    const SpinModel m = spin_model(nc, nb, d_ec, d_eb, d_vbc, S->cutoff);
     hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
                        (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr);
     int32_t *d_kmax = nullptr;
@@ -445,7 +466,8 @@ static void free_sector(ed_sector *S)
     free_spin(S->up);
     free_spin(S->dn);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
-                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord};
+                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord,
+                  S->d_quart, S->d_quart32};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (ed::GraphCache &G : S->graphs) {
@@ -503,9 +525,226 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
     return ED_OK;
 }
 
+// QUART_INT_SZ (HAIMsz.f90:129-177): the impurity interaction of every Nambu state,
+// with the actual occupations n_up = bit of IMPiorb(c), n_dn = NOT bit of IMPiorb(c + Nc):
+//   sum_c U_cc [n_c = 2]  +  sum_{c < c', U_cc' != 0} U_cc' n_c n_c'.
+// (The reference gates the second sum with a flag computed from an uninitialised
+// diagonal mask, HAIMsz.f90:75-79; every term it can add has U_cc' != 0, so the sum over
+// the nonzero off-diagonal U is the same result.)  J = 0 here: the real-valued S_z
+// H.v stops for |Jhund| > 1e-5 (HAIMsz.f90:473-476).
+__global__ void k_quart_sz(const int32_t *states, int d, int nc, int nb, int ns, const double *u, double *quart)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    const uint32_t s = (uint32_t)states[i];
+    double q = 0.0;
+    for (int c1 = 0; c1 < nc; ++c1) {
+        const int n1 = (int)((s >> (nb + c1)) & 1u) + (int)(((s >> (ns + nb + c1)) & 1u) ^ 1u);
+        if (n1 == 2 && u[c1 * nc + c1] != 0.0) q += u[c1 * nc + c1];
+        if (n1 != 0)
+            for (int c2 = c1 + 1; c2 < nc; ++c2) {
+                if (u[c1 * nc + c2] == 0.0) continue;
+                const int n2 = (int)((s >> (nb + c2)) & 1u) + (int)(((s >> (ns + nb + c2)) & 1u) ^ 1u);
+                q += u[c1 * nc + c2] * n1 * n2;
+            }
+    }
+    quart[i] = q;
+}
+
+// Nambu matrices (Nambu_Ec, impurity_class.f90:1018-1037; Nambu_Eb / Nambu_Vbc,
+// bath_class.f90:956-1035), row-major:
+//   Ec_N  = [[Ec_up, 0], [0, -Ec_dn^T]]                       (2 nc x 2 nc)
+//   Eb_N  = [[Eb_up, Pb], [Pb^T, -Eb_dn^T]]                   (2 nb x 2 nb)
+//   Vbc_N = [[Vbc_up, PVbc_dn], [PVbc_up, -Vbc_dn]]           (2 nb x 2 nc; the PVbc
+//           blocks only with pairing_imp_to_bath, PAIRING_IMP_TO_BATH)
+static void nambu_matrices(int nc, int nb, const double *ec, const double *eb, const double *vbc, const double *pb,
+                           const double *pvbc, int pair_ib, std::vector<double> &ecn, std::vector<double> &ebn,
+                           std::vector<double> &vbn)
+{
+    const int n2 = 2 * nc, b2 = 2 * nb;
+    ecn.assign((size_t)n2 * n2, 0.0);
+    ebn.assign((size_t)b2 * b2, 0.0);
+    vbn.assign((size_t)b2 * n2, 0.0);
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            ecn[(size_t)a * n2 + b] = ec[a * nc + b];
+            ecn[(size_t)(nc + a) * n2 + nc + b] = -ec[nc * nc + b * nc + a];
+        }
+    for (int k = 0; k < nb; ++k)
+        for (int l = 0; l < nb; ++l) {
+            ebn[(size_t)k * b2 + l] = eb[k * nb + l];
+            ebn[(size_t)(nb + k) * b2 + nb + l] = -eb[nb * nb + l * nb + k];
+            ebn[(size_t)k * b2 + nb + l] = pb ? pb[k * nb + l] : 0.0;
+            ebn[(size_t)(nb + l) * b2 + k] = pb ? pb[k * nb + l] : 0.0;
+        }
+    for (int k = 0; k < nb; ++k)
+        for (int c = 0; c < nc; ++c) {
+            vbn[(size_t)k * n2 + c] = vbc[k * nc + c];
+            vbn[(size_t)(nb + k) * n2 + nc + c] = -vbc[nb * nc + k * nc + c];
+            if (pair_ib && pvbc) {
+                vbn[(size_t)(nb + k) * n2 + c] = pvbc[k * nc + c];
+                vbn[(size_t)k * n2 + nc + c] = pvbc[nb * nc + k * nc + c];
+            }
+        }
+}
+
+// The S_z sector of Nambu particle number npart = Ns + 2 S_z (new_HAIMsz + tab_HAIMsz,
+// HAIMsz.f90:51-177; the sector is the popcount-npart states of 2 Ns orbitals in
+// increasing order, quantum_fermion_hilbert_class.f90:346-421).
+static int create_sector_sz(const ed_model *m, const double *pb, const double *pvbc, int pair_ib, int npart,
+                            ed_sector **out)
+{
+    if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create_sz: NULL argument");
+    *out = nullptr;
+    const int nc = m->nc, nb = m->nb, ns = nc + nb;
+    if (nc < 1 || nc > 6 || nb < 0 || ns > 13)
+        return fail(ED_ERR_ARG, "ed_sector_create_sz: need 1 <= nc <= 6, nb >= 0, nc + nb <= 13 (2 Ns bits)");
+    if (npart < 0 || npart > 2 * ns) return fail(ED_ERR_ARG, "ed_sector_create_sz: npart outside [0, 2 Ns]");
+    if (!m->ec || !m->u || !m->jj || (nb && (!m->eb || !m->vbc)))
+        return fail(ED_ERR_ARG, "ed_sector_create_sz: NULL model array");
+    if (pair_ib && !pvbc) return fail(ED_ERR_ARG, "ed_sector_create_sz: pairing_imp_to_bath needs pvbc");
+    if (std::fabs(m->jhund) > 1e-5)
+        return fail(ED_ERR_ARG, "ed_sector_create_sz: Jhund and Sz basis not implemented for real H (HAIMsz.f90:473-476)");
+    ed_sector *S = new ed_sector();
+    ED_HIP_CHECK(hipGetDevice(&S->device));
+    S->nc = nc; S->nb = nb; S->ns = ns; S->nup = npart; S->ndn = -1; S->ncfg = 1 << nc;
+    S->sz = 1;
+    S->ec.assign(m->ec, m->ec + 2 * nc * nc);
+    if (nb) {
+        S->eb.assign(m->eb, m->eb + 2 * nb * nb);
+        S->vbc.assign(m->vbc, m->vbc + 2 * nb * nc);
+    }
+    S->u.assign(m->u, m->u + nc * nc);
+    S->jj.assign(m->jj, m->jj + nc * nc);
+    S->jhund = m->jhund;
+    S->cutoff = m->cutoff;
+    // energy_global_shift (impurity_class.f90:1037) + energy_global_shift2 (bath_class.f90:976)
+    {
+        double s1 = 0.0, s2 = 0.0;
+        for (int c = 0; c < nc; ++c) s1 += m->ec[nc * nc + c * nc + c];
+        for (int k = 0; k < nb; ++k) s2 += m->eb[nb * nb + k * nb + k];
+        S->energy_shift = s1 + s2;
+    }
+    int rc = ED_OK;
+    auto bail = [&](int code) { free_sector(S); return code; };
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    hipStream_t st = S->stream;
+    const int nbits = 2 * ns;
+    if ((rc = dalloc(&S->d_rank, (size_t)1 << nbits)) != ED_OK) return bail(rc);
+    hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << nbits)), dim3(kBlock), 0, st, nbits, S->d_rank);
+    SpinTables &T = S->up;
+    T.d = binom(nbits, npart);
+    const int d = T.d;
+    if ((rc = dalloc(&T.states, d)) != ED_OK || (rc = dalloc(&T.diag, d)) != ED_OK ||
+        (rc = dalloc(&T.noff, d)) != ED_OK || (rc = dalloc(&T.ptr, d + 1)) != ED_OK)
+        return bail(rc);
+    hipLaunchKernelGGL(k_enum_states, dim3(nblocks(d)), dim3(kBlock), 0, st, nbits, npart, d, T.states);
+    std::vector<double> ecn, ebn, vbn;
+    nambu_matrices(nc, nb, m->ec, m->eb, m->vbc, pb, pvbc, pair_ib, ecn, ebn, vbn);
+    double *d_mat = nullptr, *d_u = nullptr;
+    const size_t nmat = ecn.size() + std::max<size_t>(1, ebn.size()) + std::max<size_t>(1, vbn.size());
+    if ((rc = dalloc(&d_mat, nmat)) != ED_OK || (rc = dalloc(&d_u, (size_t)nc * nc)) != ED_OK) {
+        (void)hipFree(d_mat);
+        return bail(rc);
+    }
+    struct Tmp {
+        double *&a, *&b;
+        ~Tmp()
+        {
+            (void)hipFree(a);
+            (void)hipFree(b);
+        }
+    } tmp{d_mat, d_u};
+    double *d_ec = d_mat, *d_eb = d_ec + ecn.size(), *d_vbc = d_eb + std::max<size_t>(1, ebn.size());
+    ED_HIP_CHECK(hipMemcpyAsync(d_ec, ecn.data(), sizeof(double) * ecn.size(), hipMemcpyHostToDevice, st));
+    if (nb) {
+        ED_HIP_CHECK(hipMemcpyAsync(d_eb, ebn.data(), sizeof(double) * ebn.size(), hipMemcpyHostToDevice, st));
+        ED_HIP_CHECK(hipMemcpyAsync(d_vbc, vbn.data(), sizeof(double) * vbn.size(), hipMemcpyHostToDevice, st));
+    }
+    ED_HIP_CHECK(hipMemcpyAsync(d_u, m->u, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+    SpinModel sm{};
+    sm.nimp = 2 * nc;
+    sm.nbath = 2 * nb;
+    for (int c = 0; c < nc; ++c) {
+        sm.ibit[c] = nb + c;
+        sm.ibit[nc + c] = ns + nb + c;
+    }
+    for (int k = 0; k < nb; ++k) {
+        sm.bbit[k] = k;
+        sm.bbit[nb + k] = ns + k;
+    }
+    sm.ec = d_ec;
+    sm.eb = d_eb;
+    sm.vbc = d_vbc;
+    sm.cut = m->cutoff;
+    hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
+                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+    int32_t *d_kmax = nullptr;
+    if ((rc = dalloc(&d_kmax, 1)) != ED_OK) return bail(rc);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kBlock), 0, st, T.noff, d, T.ptr, d_kmax);
+    ED_HIP_CHECK(hipGetLastError());
+    int32_t h[2];
+    ED_HIP_CHECK(hipMemcpyAsync(&h[0], T.ptr + d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipMemcpyAsync(&h[1], d_kmax, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(d_kmax);
+    T.nnz = h[0];
+    T.kmax = h[1];
+    T.kpad = std::max(T.kmax, 1);
+    if ((rc = dalloc(&T.rk, (size_t)T.nnz)) != ED_OK || (rc = dalloc(&T.off, (size_t)T.nnz)) != ED_OK) return bail(rc);
+    hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
+                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off);
+    const int kw = T.kpad;
+    if ((rc = dalloc(&T.ell_rk, (size_t)kw * d)) != ED_OK || (rc = dalloc(&T.ell_val, (size_t)kw * d)) != ED_OK)
+        return bail(rc);
+    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk, T.ell_val);
+    if ((rc = dalloc(&S->d_quart, d)) != ED_OK || (rc = dalloc(&S->d_quart32, d)) != ED_OK ||
+        (rc = dalloc(&T.diag32, d)) != ED_OK || (rc = dalloc(&T.off32, (size_t)T.nnz)) != ED_OK ||
+        (rc = dalloc(&T.ell_val32, (size_t)kw * d)) != ED_OK)
+        return bail(rc);
+    hipLaunchKernelGGL(k_quart_sz, dim3(nblocks(d)), dim3(kBlock), 0, st, T.states, d, nc, nb, ns, (const double *)d_u,
+                       S->d_quart);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(d)), dim3(kBlock), 0, st, S->d_quart, S->d_quart32, (int64_t)d);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(d)), dim3(kBlock), 0, st, T.diag, T.diag32, (int64_t)d);
+    if (T.nnz)
+        hipLaunchKernelGGL(k_to_f32, dim3(nblocks(T.nnz)), dim3(kBlock), 0, st, T.off, T.off32, (int64_t)T.nnz);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks((int64_t)kw * d)), dim3(kBlock), 0, st, T.ell_val, T.ell_val32,
+                       (int64_t)kw * d);
+    ED_HIP_CHECK(hipGetLastError());
+    S->dn.d = 1;
+    S->dim = d;
+    if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
+        return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(ED_ERR_HIP, "S_z sector build failed"));
+    *out = S;
+    return ED_OK;
+}
+
 }  // namespace ed
 
 extern "C" {
+
+int ed_sector_create_sz(const ed_model *model, const double *pb, const double *pvbc, int pairing_imp_to_bath,
+                        int npart, ed_sector **out)
+{
+    return ed::create_sector_sz(model, pb, pvbc, pairing_imp_to_bath, npart, out);
+}
+
+int ed_sector_quart_sz(const ed_sector *sec, double *quart)
+{
+    if (!sec || !quart) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: NULL argument");
+    if (!sec->sz) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: not an S_z sector");
+    ED_HIP_CHECK(hipMemcpy(quart, sec->d_quart, sizeof(double) * sec->dim, hipMemcpyDeviceToHost));
+    return ED_OK;
+}
+
+int ed_sector_energy_shift(const ed_sector *sec, double *shift)
+{
+    if (!sec || !shift) return ed::fail(ED_ERR_ARG, "ed_sector_energy_shift: NULL argument");
+    *shift = sec->sz ? sec->energy_shift : 0.0;
+    return ED_OK;
+}
 
 int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out)
 {
@@ -541,6 +780,7 @@ int ed_sector_tables(const ed_sector *sec, int spin, int32_t *states, double *di
                      int32_t *rankoff, double *offdiag)
 {
     if (!sec || (spin != 0 && spin != 1)) return ed::fail(ED_ERR_ARG, "ed_sector_tables: bad argument");
+    if (sec->sz && spin != 0) return ed::fail(ED_ERR_ARG, "ed_sector_tables: an S_z sector has one table (spin 0)");
     const ed::SpinTables &T = spin ? sec->dn : sec->up;
     if (states) ED_HIP_CHECK(hipMemcpy(states, T.states, sizeof(int32_t) * T.d, hipMemcpyDeviceToHost));
     if (diag) ED_HIP_CHECK(hipMemcpy(diag, T.diag, sizeof(double) * T.d, hipMemcpyDeviceToHost));

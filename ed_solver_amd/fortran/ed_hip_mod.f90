@@ -38,6 +38,20 @@ module ed_hip
       integer(c_int), value      :: nup, ndn
       type(c_ptr), intent(out)   :: sec
     end function
+    ! S_z (Nambu) sector of a superconducting bath (new_HAIMsz); pb / pvbc may be c_null_ptr
+    integer(c_int) function ed_sector_create_sz(model, pb, pvbc, pairing_imp_to_bath, npart, sec) &
+        bind(C, name="ed_sector_create_sz")
+      import :: c_int, c_ptr, ed_model
+      type(ed_model), intent(in) :: model
+      type(c_ptr), value         :: pb, pvbc
+      integer(c_int), value      :: pairing_imp_to_bath, npart
+      type(c_ptr), intent(out)   :: sec
+    end function
+    integer(c_int) function ed_sector_energy_shift(sec, shift) bind(C, name="ed_sector_energy_shift")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: sec
+      real(c_double), intent(out) :: shift
+    end function
     integer(c_int) function ed_sector_destroy(sec) bind(C, name="ed_sector_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: sec

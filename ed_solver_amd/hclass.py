@@ -91,6 +91,13 @@ class Sector:
         self.close()
 
     # ----------------------------------------------------------------- tables
+    @property
+    def energy_shift(self) -> float:
+        """Constant the particle-hole transformation removes from an S_z sector's H (0 otherwise)."""
+        v = ctypes.c_double()
+        check(lib().ed_sector_energy_shift(self.handle, ctypes.byref(v)))
+        return v.value
+
     def nnz(self, spin: int) -> int:
         n = ctypes.c_int64()
         check(lib().ed_sector_nnz(self.handle, spin, ctypes.byref(n)))
@@ -323,6 +330,36 @@ def invert_zmtridiag(a, b, wn) -> np.ndarray:
     out = np.zeros(2 * len(wn))
     check(lib().ed_continued_fraction(len(a), dptr(a), dptr(b), len(wn), dptr(wn), dptr(out)))
     return out[0::2] + 1j * out[1::2]
+
+
+class SzSector(Sector):
+    """One S_z (Nambu) sector of an Anderson impurity model with a superconducting bath
+    (new_HAIMsz, HAIMsz.f90:51-177): 2 Ns orbitals, the down spin particle-hole
+    transformed, npart = Ns + 2 S_z.  pb[nb][nb] = bath pairing Pb, pvbc[2][nb][nc] =
+    impurity-bath pairing (used when pairing_imp_to_bath).  All H.v / Lanczos / dense-ED
+    methods of Sector apply; physical energies are eigenvalues + energy_shift."""
+
+    def __init__(self, model: AIMParams, npart: int, pb: np.ndarray | None = None, pvbc: np.ndarray | None = None,
+                 pairing_imp_to_bath: bool = False):
+        self.model = model
+        self.npart = int(npart)
+        self.nup, self.ndn = self.npart, -1
+        self._cm = CModel.from_params(model)
+        self._pb = None if pb is None else np.ascontiguousarray(pb, np.float64).reshape(model.nb, model.nb)
+        self._pvbc = None if pvbc is None else np.ascontiguousarray(pvbc, np.float64).reshape(2, model.nb, model.nc)
+        h = ctypes.c_void_p()
+        check(lib().ed_sector_create_sz(ctypes.byref(self._cm), dptr(self._pb) if self._pb is not None else None,
+                                        dptr(self._pvbc) if self._pvbc is not None else None,
+                                        1 if pairing_imp_to_bath else 0, self.npart, ctypes.byref(h)))
+        self._h = h
+        self.dim = int(lib().ed_sector_dim(h))
+        self.dup, self.ddn = self.dim, 1
+
+    def quart(self) -> np.ndarray:
+        """QUART_INT_SZ per state."""
+        q = np.zeros(self.dim)
+        check(lib().ed_sector_quart_sz(self.handle, dptr(q)))
+        return q
 
 
 def compute_dynamic(sector: Sector, iph: int, energy: float, phi: np.ndarray, nitergreen: int, wn: np.ndarray,

@@ -59,6 +59,27 @@ typedef struct ed_model {
  * Lanczos work space.  Opaque. */
 typedef struct ed_sector ed_sector;
 
+/* S_z (Nambu) sector for a superconducting bath (new_HAIMsz / tab_HAIMsz, HAIMsz.f90:51-177;
+ * the Hmult__ branch of H_class.f90:84-93,123-132).  The down spin is particle-hole
+ * transformed: a state is 2 Ns bits (up: bath k -> bit k, impurity c -> bit nb + c;
+ * down holes at the same positions + Ns) with npart = N_up + (Ns - N_dn) = Ns + 2 S_z set
+ * bits, in increasing order.  pb[nb][nb] = Pb (bath pairing b+_k,up b+_l,dn + h.c.; NULL =
+ * zero), pvbc[2][nb][nc] = PVbc (impurity-bath pairing, used when pairing_imp_to_bath,
+ * PAIRING_IMP_TO_BATH).  Quadratic part: tab_HAIM2 over the Nambu matrices (Nambu_Ec,
+ * Nambu_Eb, Nambu_Vbc); diagonal interaction QUART_INT_SZ.  Real H only: |Jhund| > 1e-5
+ * is rejected (HAIMsz.f90:473-476).  Ns <= 13.  The returned sector works with every
+ * H.v, Lanczos, block Lanczos, ARPACK-mirror and dense-ED entry point; eigenvalues are
+ * those of the Nambu H, the physical energies are E + ed_sector_energy_shift(). */
+int ed_sector_create_sz(const ed_model *model, const double *pb, const double *pvbc, int pairing_imp_to_bath,
+                        int npart, ed_sector **out);
+
+/* QUART_INT_SZ of an S_z sector (HAIMsz.f90:129-177), one value per state [dimen]. */
+int ed_sector_quart_sz(const ed_sector *sec, double *quart);
+
+/* energy_global_shift + energy_global_shift2 (impurity_class.f90:1037, bath_class.f90:976):
+ * tr(Ec_dn) + tr(Eb_dn) for an S_z sector, 0 for an (N_up, N_dn) sector. */
+int ed_sector_energy_shift(const ed_sector *sec, double *shift);
+
 /* Error text of the last failing call on this thread (never NULL). */
 const char *ed_last_error(void);
 

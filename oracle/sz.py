@@ -1,0 +1,218 @@
+"""Oracle for the S_z (Nambu) sectors of a superconducting bath -- test infrastructure only.
+
+Two independent routes:
+
+* a restatement of the reference's S_z path in NumPy/Python loops, for small sectors:
+  the Nambu matrices (Nambu_Ec, ed_solver/impurity_class.f90:1018-1037; Nambu_Eb and
+  Nambu_Vbc, ed_solver/bath_class.f90:956-1035), tab_HAIM2 over the 2 Nc impurity and
+  2 Nb bath Nambu orbitals (ed_solver/HAIM_offdiag_arrays.f90:419-527, the same loops as
+  the (N_up, N_dn) tables), QUART_INT_SZ (ed_solver/HAIMsz.f90:129-177) and the gather
+  H.v of HAIMsz_multr (ed_solver/HAIMsz.f90:453-560);
+* the physical Hamiltonian with pairing written as operator strings on the full Fock
+  space (oracle.brute conventions), restricted to N_up - N_dn = npart - Ns.
+
+The particle-hole transformation is unitary, so the spectrum of the Nambu sector plus
+tr(Ec_dn) + tr(Eb_dn) (energy_global_shift + energy_global_shift2,
+ed_solver/impurity_class.f90:1037, ed_solver/bath_class.f90:976) must equal the physical
+spectrum of that S_z block: that pins the restatement (matrices, signs, shift), and the
+device tables are then compared with the restatement bit for bit.
+"""
+from __future__ import annotations
+
+from math import comb
+
+import numpy as np
+import scipy.sparse as sp
+
+from ed_solver_amd.aim import AIMParams
+
+from . import brute as bf
+
+
+def nambu_matrices(model: AIMParams, pb=None, pvbc=None, pairing_imp_to_bath=False):
+    """(Ec_N [2nc,2nc], Eb_N [2nb,2nb], Vbc_N [2nb,2nc]) as Nambu_Ec / Nambu_Eb / Nambu_Vbc."""
+    nc, nb = model.nc, model.nb
+    ecn = np.zeros((2 * nc, 2 * nc))
+    ecn[:nc, :nc] = model.ec[0]
+    ecn[nc:, nc:] = -model.ec[1].T
+    ebn = np.zeros((2 * nb, 2 * nb))
+    ebn[:nb, :nb] = model.eb[0]
+    ebn[nb:, nb:] = -model.eb[1].T
+    if pb is not None:
+        ebn[:nb, nb:] = pb
+        ebn[nb:, :nb] = np.asarray(pb).T
+    vbn = np.zeros((2 * nb, 2 * nc))
+    vbn[:nb, :nc] = model.vbc[0]
+    vbn[nb:, nc:] = -model.vbc[1]
+    if pairing_imp_to_bath and pvbc is not None:
+        vbn[nb:, :nc] = pvbc[0]
+        vbn[:nb, nc:] = pvbc[1]
+    return ecn, ebn, vbn
+
+
+def orbital_bits(model: AIMParams):
+    """Bit positions of the Nambu impurity and bath orbitals (AIM2%IMPiorb / BATHiorb)."""
+    nc, nb, ns = model.nc, model.nb, model.ns
+    ibit = [nb + c for c in range(nc)] + [ns + nb + c for c in range(nc)]
+    bbit = list(range(nb)) + [ns + k for k in range(nb)]
+    return ibit, bbit
+
+
+def sz_states(ns: int, npart: int) -> np.ndarray:
+    allst = np.arange(1 << (2 * ns), dtype=np.int64)
+    return allst[bf.popcount(allst) == npart]
+
+
+def _sign_above(s: int, bit: int) -> int:
+    return -1 if bin(s >> (bit + 1)).count("1") & 1 else 1
+
+
+def _hop(s: int, to: int, frm: int):
+    """c+_to c_from (quantum_fermion_ket_class.f90:483-492); None if nil."""
+    if not (s >> frm) & 1:
+        return None
+    t = s & ~(1 << frm)
+    g = _sign_above(t, frm)
+    if (t >> to) & 1:
+        return None
+    o = t | (1 << to)
+    return o, g * _sign_above(o, to)
+
+
+def tab_sz(model: AIMParams, npart: int, pb=None, pvbc=None, pairing_imp_to_bath=False) -> dict:
+    """tab_HAIM2 over the Nambu orbitals and QUART_INT_SZ, for every state in order."""
+    ns, nc, nb = model.ns, model.nc, model.nb
+    ecn, ebn, vbn = nambu_matrices(model, pb, pvbc, pairing_imp_to_bath)
+    ibit, bbit = orbital_bits(model)
+    states = sz_states(ns, npart)
+    rank = {int(s): i for i, s in enumerate(states)}
+    cut = model.cutoff
+    diag = np.zeros(len(states)); noff = np.zeros(len(states), np.int32); quart = np.zeros(len(states))
+    rk, off = [], []
+    U = model.u
+    for i, s in enumerate(states):
+        s = int(s)
+        d = 0.0
+        for a in range(2 * nc):
+            if (s >> ibit[a]) & 1:
+                d += ecn[a, a]
+        for b in range(2 * nb):
+            if (s >> bbit[b]) & 1:
+                d += ebn[b, b]
+        diag[i] = d
+        n0 = len(rk)
+
+        def add(t, to, frm):
+            r = _hop(s, to, frm)
+            if r is not None:
+                rk.append(rank[r[0]]); off.append(t * r[1])
+
+        for a in range(2 * nc):
+            for b in range(2 * nc):
+                t = ecn[a, b]
+                if a != b and t != 0.0 and abs(t) > cut:
+                    add(t, ibit[a], ibit[b])
+        for a in range(2 * nb):
+            for b in range(2 * nb):
+                t = ebn[a, b]
+                if a != b and t != 0.0 and abs(t) > cut:
+                    add(t, bbit[a], bbit[b])
+        for b in range(2 * nb):
+            for c in range(2 * nc):
+                t = vbn[b, c]
+                if t != 0.0 and abs(t) > cut:
+                    add(t, bbit[b], ibit[c])
+                    add(t, ibit[c], bbit[b])
+        noff[i] = len(rk) - n0
+        q = 0.0
+        for c1 in range(nc):
+            n1 = ((s >> (nb + c1)) & 1) + (1 - ((s >> (ns + nb + c1)) & 1))
+            if n1 == 2 and U[c1, c1] != 0.0:
+                q += U[c1, c1]
+            if n1 != 0:
+                for c2 in range(c1 + 1, nc):
+                    if U[c1, c2] != 0.0:
+                        n2 = ((s >> (nb + c2)) & 1) + (1 - ((s >> (ns + nb + c2)) & 1))
+                        q += U[c1, c2] * n1 * n2
+        quart[i] = q
+    return {"states": states.astype(np.int32), "diag": diag, "noff": noff, "rankoff": np.array(rk, np.int32),
+            "offdiag": np.array(off), "quart": quart}
+
+
+def hmult_sz(tab: dict, v: np.ndarray) -> np.ndarray:
+    """HAIMsz_multr in gather form: w_i = quart_i v_i + diag_i v_i + sum_k off_ik v_rank."""
+    w = tab["quart"] * v + tab["diag"] * v
+    ptr = np.concatenate([[0], np.cumsum(tab["noff"])])
+    for i in range(len(v)):
+        for k in range(ptr[i], ptr[i + 1]):
+            w[i] += tab["offdiag"][k] * v[tab["rankoff"][k]]
+    return w
+
+
+def dense_sz(tab: dict) -> np.ndarray:
+    d = len(tab["states"])
+    H = np.diag(tab["quart"] + tab["diag"])
+    ptr = np.concatenate([[0], np.cumsum(tab["noff"])])
+    for i in range(d):
+        for k in range(ptr[i], ptr[i + 1]):
+            H[i, tab["rankoff"][k]] += tab["offdiag"][k]
+    return H
+
+
+def energy_shift(model: AIMParams) -> float:
+    """energy_global_shift + energy_global_shift2, each a left-to-right sum."""
+    s1 = 0.0
+    for c in range(model.nc):
+        s1 += float(model.ec[1][c, c])
+    s2 = 0.0
+    for k in range(model.nb):
+        s2 += float(model.eb[1][k, k])
+    return s1 + s2
+
+
+def physical_terms(model: AIMParams, pb=None, pvbc=None, pairing_imp_to_bath=False):
+    """oracle.brute's terms plus the pairing strings, on modes bit(s_up + 2^Ns s_dn)."""
+    ns, nb, nc = model.ns, model.nb, model.nc
+    T = list(bf.terms(model))
+    if pb is not None:
+        for k in range(nb):
+            for l in range(nb):
+                if pb[k, l] != 0.0:
+                    T.append((pb[k, l], [(k, True), (ns + l, True)]))          # b+_k,up b+_l,dn
+                    T.append((pb[k, l], [(ns + l, False), (k, False)]))        # b_l,dn b_k,up
+    if pairing_imp_to_bath and pvbc is not None:
+        for k in range(nb):
+            for c in range(nc):
+                if pvbc[0][k, c] != 0.0:   # b_k,dn c_c,up + c+_c,up b+_k,dn
+                    T.append((pvbc[0][k, c], [(ns + k, False), (nb + c, False)]))
+                    T.append((pvbc[0][k, c], [(nb + c, True), (ns + k, True)]))
+                if pvbc[1][k, c] != 0.0:   # b+_k,up c+_c,dn + c_c,dn b_k,up
+                    T.append((pvbc[1][k, c], [(k, True), (ns + nb + c, True)]))
+                    T.append((pvbc[1][k, c], [(ns + nb + c, False), (k, False)]))
+    return T
+
+
+def physical_spectrum(model: AIMParams, npart: int, pb=None, pvbc=None, pairing_imp_to_bath=False) -> np.ndarray:
+    """Eigenvalues of the physical H on the block N_up - N_dn = npart - Ns."""
+    ns = model.ns
+    allst = np.arange(1 << (2 * ns), dtype=np.int64)
+    up = bf.popcount(allst & ((1 << ns) - 1))
+    dn = bf.popcount(allst >> ns)
+    basis = allst[(up - dn) == npart - ns]
+    index = np.full(1 << (2 * ns), -1, np.int64)
+    index[basis] = np.arange(len(basis))
+    rows, cols, vals = [], [], []
+    for coef, ops in physical_terms(model, pb, pvbc, pairing_imp_to_bath):
+        out, sg, ok = bf.apply_ops(basis, ops)
+        if not ok.any():
+            continue
+        r = index[out[ok]]
+        assert (r >= 0).all()
+        rows.append(r); cols.append(np.arange(len(basis))[ok]); vals.append(coef * sg[ok])
+    n = len(basis)
+    H = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).toarray()
+    return np.linalg.eigvalsh(H)
+
+
+def sz_dim(ns: int, npart: int) -> int:
+    return comb(2 * ns, npart)

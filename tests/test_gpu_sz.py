@@ -1,0 +1,135 @@
+"""S_z (Nambu) sectors on the GPU against the oracle restatement (tables bit-exact, H.v)
+and against the physical spectrum with pairing (dense ED, Lanczos, block Lanczos)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+import oracle.sz as osz
+from ed_solver_amd.aim import AIMParams
+
+pytestmark = pytest.mark.gpu
+
+
+def sc_model(nc, nb, U, seed, u_offdiag=None):
+    r = np.random.default_rng(seed)
+    ec = np.stack([np.eye(nc) * (-U / 2 + 0.05), np.eye(nc) * (-U / 2 - 0.05)])
+    if nc > 1:
+        ec[:, 0, 1] = ec[:, 1, 0] = 0.15
+    eps = r.uniform(-1, 1, nb)
+    eb = np.stack([np.diag(eps), np.diag(eps + 0.1)])
+    if nb > 1:
+        eb[:, 0, 1] = eb[:, 1, 0] = 0.2
+    vbc = np.zeros((2, nb, nc))
+    for k in range(nb):
+        vbc[:, k, k % nc] = r.uniform(0.2, 0.6)
+    u = np.full((nc, nc), u_offdiag if u_offdiag is not None else 0.0)
+    np.fill_diagonal(u, U)
+    m = AIMParams(nc, nb, ec, eb, vbc, u, np.zeros((nc, nc)), 0.0, 1e-10)
+    return m, r.uniform(-0.4, 0.4, (nb, nb)), r.uniform(-0.3, 0.3, (2, nb, nc))
+
+
+CASES = [(1, 3, 4, False, None), (1, 3, 3, True, None), (2, 2, 4, True, 1.2), (2, 3, 5, False, 1.0),
+         (1, 5, 6, True, None)]
+
+
+def sparse_sz(tab):
+    d = len(tab["states"])
+    ptr = np.concatenate([[0], np.cumsum(tab["noff"])])
+    rows = np.repeat(np.arange(d), np.diff(ptr))
+    H = sp.coo_matrix((tab["offdiag"], (rows, tab["rankoff"])), shape=(d, d)).tocsr()
+    return H + sp.diags(tab["quart"] + tab["diag"])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_sz_tables_bit_exact_and_hmult(gpu, case):
+    nc, nb, npart, pair_ib, u_off = case
+    m, pb, pvbc = sc_model(nc, nb, 2.0, seed=20 + nb + npart, u_offdiag=u_off)
+    print(f"S_z sector nc={nc} nb={nb} npart={npart} pairing_imp_to_bath={pair_ib}", flush=True)
+    s = gpu.SzSector(m, npart, pb, pvbc, pair_ib)
+    tab = osz.tab_sz(m, npart, pb, pvbc, pair_ib)
+    assert s.dim == len(tab["states"]) == osz.sz_dim(m.ns, npart)
+    t = s.tables(0)
+    assert np.array_equal(t["states"], tab["states"])
+    assert np.array_equal(t["noff"], tab["noff"])
+    assert np.array_equal(t["rankoff"], tab["rankoff"])
+    assert np.array_equal(t["diag"], tab["diag"])
+    assert np.array_equal(t["offdiag"], tab["offdiag"])
+    assert np.array_equal(s.quart(), tab["quart"])
+    assert s.energy_shift == osz.energy_shift(m)
+    v = np.random.default_rng(3).uniform(-1, 1, s.dim)
+    w = s.hmult(v)
+    ref = osz.hmult_sz(tab, v)
+    assert np.max(np.abs(w - ref)) / np.max(np.abs(ref)) < 1e-13
+    from hipmem import DevBuf
+    x32 = DevBuf.from_array(v.astype(np.float32))
+    y32 = DevBuf(4 * s.dim)
+    s.hmult_dev(y32.addr, x32.addr, "f32")
+    s.synchronize()
+    w32 = y32.to_array(np.float32, s.dim).astype(np.float64)
+    assert np.max(np.abs(w32 - ref)) / np.max(np.abs(ref)) < 1e-5
+    # two vectors at a padded leading dimension (the block H.v of the block Lanczos)
+    ld = s.dim + 5
+    Xh = np.zeros(2 * ld)
+    Xh[:s.dim] = v
+    Xh[ld:ld + s.dim] = v[::-1]
+    X = DevBuf.from_array(Xh)
+    Y = DevBuf(8 * 2 * ld)
+    s.hmult_block_dev(Y.addr, X.addr, 2, ld)
+    s.synchronize()
+    Yh = Y.to_array(np.float64, 2 * ld)
+    assert np.max(np.abs(Yh[:s.dim] - ref)) / np.max(np.abs(ref)) < 1e-13
+    ref2 = osz.hmult_sz(tab, v[::-1].copy())
+    assert np.max(np.abs(Yh[ld:ld + s.dim] - ref2)) / np.max(np.abs(ref2)) < 1e-13
+    for buf in (x32, y32, X, Y):
+        buf.free()
+    s.close()
+
+
+@pytest.mark.parametrize("case", CASES[:4])
+def test_sz_full_ed_matches_physical_spectrum(gpu, case):
+    nc, nb, npart, pair_ib, u_off = case
+    m, pb, pvbc = sc_model(nc, nb, 2.5, seed=40 + npart, u_offdiag=u_off)
+    s = gpu.SzSector(m, npart, pb, pvbc, pair_ib)
+    if s.dim > 1024:
+        pytest.skip("dense ED limit")
+    ev, vec = s.full_ed(s.dim)
+    e_p = osz.physical_spectrum(m, npart, pb, pvbc, pair_ib)
+    assert np.max(np.abs(ev + s.energy_shift - e_p)) < 1e-11
+    s.close()
+
+
+def test_sz_lanczos_and_block_lanczos(gpu):
+    """Ns = 8 (2 x 8 Nambu bits), npart = 8: dimension 12,870."""
+    from ed_solver_amd.block import block_lanczos
+    m, pb, pvbc = sc_model(2, 6, 2.0, seed=7, u_offdiag=1.2)
+    s = gpu.SzSector(m, 8, pb, pvbc, True)
+    print(f"S_z sector dim {s.dim}", flush=True)
+    tab = osz.tab_sz(m, 8, pb, pvbc, True)
+    H = sparse_sz(tab)
+    e_ref = np.sort(sla.eigsh(H, k=4, which="SA", tol=1e-12)[0])
+    r = s.lanczos(nitermax=300, neigen=1, tolerance=1e-12)
+    assert abs(r.evals[0] - e_ref[0]) < 1e-10
+    psi = s.ground_state(r, 0)
+    assert np.linalg.norm(H @ psi - e_ref[0] * psi) < 1e-5
+    for prec in ("f64", "f32"):
+        a, b = s.lanczos_run(30, precision=prec)
+        assert np.all(np.isfinite(a)) and np.all(b[1:] > 0)
+    ev, vec = block_lanczos(s, nblock=4, neig=4, nitermax=200, tolerance=1e-12)
+    assert np.max(np.abs(np.sort(ev)[:4] - e_ref)) < 1e-9
+    s.close()
+
+
+def test_sz_rejections(gpu):
+    from ed_solver_amd import EDError, apply_C
+    m, pb, pvbc = sc_model(2, 2, 2.0, seed=1, u_offdiag=1.0)
+    mj = AIMParams(m.nc, m.nb, m.ec, m.eb, m.vbc, m.u, np.full((2, 2), 0.4), 0.4, 1e-10)
+    with pytest.raises(EDError):
+        gpu.SzSector(mj, 4, pb)
+    s = gpu.SzSector(m, 4, pb)
+    s2 = gpu.SzSector(m, 5, pb)
+    with pytest.raises(EDError):
+        apply_C(s, s2, 0, 0, +1, np.ones(s.dim))
+    with pytest.raises(EDError):
+        s.set_schedule(8)
+    s.close(); s2.close()

@@ -1,0 +1,78 @@
+"""The S_z (Nambu) restatement of the oracle against the physical Hamiltonian with pairing.
+
+The particle-hole transformation is unitary: eigenvalues of the tabulated Nambu sector
+plus tr(Ec_dn) + tr(Eb_dn) must equal the physical spectrum of the block
+N_up - N_dn = npart - Ns.  A dropped pairing term, a wrong sign on a Nambu block, a
+transposed matrix or a missing shift changes the spectrum and fails here.
+"""
+import numpy as np
+import pytest
+
+import oracle.sz as osz
+from ed_solver_amd.aim import AIMParams
+
+
+def sc_model(nc, nb, U, seed, u_offdiag=None, spin_split=0.0):
+    r = np.random.default_rng(seed)
+    ec = np.stack([np.eye(nc) * (-U / 2 + spin_split), np.eye(nc) * (-U / 2 - spin_split)])
+    if nc > 1:
+        ec[:, 0, 1] = ec[:, 1, 0] = 0.15
+    eps = r.uniform(-1, 1, nb)
+    eb = np.stack([np.diag(eps), np.diag(eps + 0.1)])
+    eb[:, 0, 1] = eb[:, 1, 0] = 0.2
+    vbc = np.zeros((2, nb, nc))
+    for k in range(nb):
+        vbc[:, k, k % nc] = r.uniform(0.2, 0.6)
+    u = np.full((nc, nc), u_offdiag if u_offdiag is not None else 0.0)
+    np.fill_diagonal(u, U)
+    m = AIMParams(nc, nb, ec, eb, vbc, u, np.zeros((nc, nc)), 0.0, 1e-10)
+    pb = r.uniform(-0.4, 0.4, (nb, nb))
+    pvbc = r.uniform(-0.3, 0.3, (2, nb, nc))
+    return m, pb, pvbc
+
+
+@pytest.mark.parametrize("nc,nb,npart,pair_ib,u_off", [(1, 3, 4, False, None), (1, 3, 5, True, None),
+                                                      (2, 2, 4, False, 1.3), (2, 2, 3, True, 1.1),
+                                                      (1, 4, 6, True, None)])
+def test_nambu_spectrum_equals_physical(nc, nb, npart, pair_ib, u_off):
+    m, pb, pvbc = sc_model(nc, nb, 2.0, seed=11 + nb + npart, u_offdiag=u_off, spin_split=0.07)
+    tab = osz.tab_sz(m, npart, pb, pvbc, pair_ib)
+    H = osz.dense_sz(tab)
+    assert np.allclose(H, H.T, atol=1e-14)
+    e_n = np.linalg.eigvalsh(H) + osz.energy_shift(m)
+    e_p = osz.physical_spectrum(m, npart, pb, pvbc, pair_ib)
+    assert len(e_n) == len(e_p) == osz.sz_dim(m.ns, npart)
+    assert np.max(np.abs(e_n - e_p)) < 1e-11
+
+
+def test_plant_a_mistake_is_caught():
+    """A Nambu bath down block without the minus (a plausible slip) breaks the spectral
+    equality the test above relies on.  (Flipping the sign of a whole hybridisation
+    block would not: that is a gauge transformation of the down orbitals.)"""
+    m, pb, pvbc = sc_model(2, 2, 2.0, seed=5, u_offdiag=1.2, spin_split=0.05)
+    good = osz.nambu_matrices
+
+    def wrong(model, pb=None, pvbc=None, pairing_imp_to_bath=False):
+        ecn, ebn, vbn = good(model, pb, pvbc, pairing_imp_to_bath)
+        ebn[model.nb:, model.nb:] *= -1.0
+        return ecn, ebn, vbn
+
+    osz.nambu_matrices = wrong
+    try:
+        H = osz.dense_sz(osz.tab_sz(m, 4, pb, pvbc, False))
+    finally:
+        osz.nambu_matrices = good
+    e_n = np.linalg.eigvalsh(H) + osz.energy_shift(m)
+    e_p = osz.physical_spectrum(m, 4, pb, pvbc, False)
+    assert np.max(np.abs(e_n - e_p)) > 1e-6
+
+
+def test_no_pairing_reduces_to_spin_sectors():
+    """Without pairing the S_z block is the direct sum of (N_up, N_dn) sectors with
+    N_up - N_dn fixed: its spectrum is the union of those sectors' spectra."""
+    import oracle.brute as bf
+    m, _, _ = sc_model(1, 3, 2.5, seed=3)
+    npart = 4   # Ns = 4: N_up - N_dn = 0
+    e_n = np.sort(np.linalg.eigvalsh(osz.dense_sz(osz.tab_sz(m, npart))) + osz.energy_shift(m))
+    parts = [np.linalg.eigvalsh(bf.sector_hamiltonian(m, n, n).toarray()) for n in range(m.ns + 1)]
+    assert np.max(np.abs(e_n - np.sort(np.concatenate(parts)))) < 1e-11

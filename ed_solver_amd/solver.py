@@ -128,20 +128,16 @@ def partition(states: list[EigenState], beta: float) -> np.ndarray:
     return w / w.sum()
 
 
-def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
-                  spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
-                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
-    """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
-
-    For every state with Boltzmann weight above weight_cutoff and every orbital: the
-    particle run from c+|psi> in (N+1) and the hole run from c|psi> in (N-1).  As
-    compute_dynamic dispatches (green_class_compute_dynamic.f90:44-62): a target sector
-    of dimension <= full_ed_switch takes the Lehmann sum over the states `states` holds
-    for it (compute_dynamic_full_ed, :77-165; none held gives zero), every other run is
-    an excitation Lanczos, and those all advance together in one batched device call."""
-    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, ops, nitergreen: int,
+               spin: int, weight_cutoff: float, cutoff_dynamic: float, precision, full_ed_switch: int) -> np.ndarray:
+    """sum over states of boltz * (particle + hole) for each operator A = sum_c coef_c c_c
+    of `ops` (a list of [(orbital, coef), ...]): the particle run from A+|psi>, the hole
+    run from A|psi> (green_class_computeAA.h:101-302 with the vectors of
+    symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137).  Small
+    target sectors take the Lehmann sum over the held states, every other run is an
+    excitation Lanczos and all of those advance together in one batched device call."""
     w = partition(states, beta)
-    G = np.zeros((len(orbitals), len(wn)), complex)
+    G = np.zeros((len(ops), len(wn)), complex)
     sectors: dict[tuple[int, int], Sector] = {}
 
     def sector(nu, nd):
@@ -150,7 +146,7 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
             sectors[key] = Sector(model, nu, nd)
         return sectors[key]
 
-    jobs = []   # (orbital index, weight, sector, phi, iph, energy)
+    jobs = []   # (operator index, weight, sector, phi, iph, energy)
     held: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
 
     def held_states(key):
@@ -165,13 +161,16 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
                 continue
             nu, nd = st.sector
             src = sector(nu, nd)
-            for io, c in enumerate(orbitals):
+            for io, op in enumerate(ops):
                 for iph, pm in ((1, +1), (2, -1)):
                     tn = (nu + pm, nd) if spin == 0 else (nu, nd + pm)
                     if not (0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns):
                         continue
                     dst = sector(*tn)
-                    phi = apply_C(src, dst, c, spin, pm, st.vector)
+                    phi = None
+                    for c, coef in op:
+                        t = coef * apply_C(src, dst, c, spin, pm, st.vector)
+                        phi = t if phi is None else phi + t
                     if dst.dim <= full_ed_switch:
                         ev, vec = held_states(tn)
                         G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
@@ -185,4 +184,45 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
     finally:
         for s in sectors.values():
             s.close()
+    return G
+
+
+def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
+                  spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
+                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
+    """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
+
+    For every state with Boltzmann weight above weight_cutoff and every orbital: the
+    particle run from c+|psi> in (N+1) and the hole run from c|psi> in (N-1).  As
+    compute_dynamic dispatches (green_class_compute_dynamic.f90:44-62): a target sector
+    of dimension <= full_ed_switch takes the Lehmann sum over the states `states` holds
+    for it (compute_dynamic_full_ed, :77-165; none held gives zero), every other run is
+    an excitation Lanczos, and those all advance together in one batched device call."""
+    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+    return _green_ops(model, states, beta, wn, [[(c, 1.0)] for c in orbitals], nitergreen, spin, weight_cutoff,
+                      cutoff_dynamic, precision, full_ed_switch)
+
+
+def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray,
+                         nitergreen: int = 36, spin: int = 0, orbitals=None, weight_cutoff: float = 0.0,
+                         cutoff_dynamic: float = 1e-8, precision="f64",
+                         full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
+    """Impurity G_ij(i w_n) of one spin, all orbital pairs, shape (norb, norb, nw).
+
+    Real H (symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137;
+    reshuffle_vecAA, green_class_compute_symmetric_combination_AA.h:10-57): besides the
+    diagonal runs from c_i, one run per pair i < j from (c_i + c_j)|psi> (and its
+    adjoint), then G_ij = G_ji = (G_(i+j) - G_ii - G_jj) / 2.  Every run of every pair
+    shares the one batched device call."""
+    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+    n = len(orbitals)
+    pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    ops = [[(c, 1.0)] for c in orbitals] + [[(orbitals[a], 1.0), (orbitals[b], 1.0)] for a, b in pairs]
+    g = _green_ops(model, states, beta, wn, ops, nitergreen, spin, weight_cutoff, cutoff_dynamic, precision,
+                   full_ed_switch)
+    G = np.zeros((n, n, len(wn)), complex)
+    for a in range(n):
+        G[a, a] = g[a]
+    for k, (a, b) in enumerate(pairs):
+        G[a, b] = G[b, a] = 0.5 * (g[n + k] - g[a] - g[b])
     return G

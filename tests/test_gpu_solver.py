@@ -162,3 +162,45 @@ def test_green_full_ed_green_flag(gpu):
     # without the flag the windows filter the states and the Lehmann sums are partial
     kept = get_eigen_energies(m, neigen=2, dEmax=0.5)
     assert len(kept) < len(states)
+
+
+def test_green_matrix_offdiagonal_vs_lehmann(gpu):
+    """G_ij from the symmetric combinations (c_i + c_j)|psi>: exact Lehmann sum over the
+    full Fock space for a two-orbital model with inter-orbital hopping, through the
+    Lehmann path (all sectors small) and through excitation Lanczos (full_ed_switch = 0,
+    36 steps >= every sector dimension)."""
+    from ed_solver_amd.aim import AIMParams
+    from ed_solver_amd.solver import compute_green_matrix, get_eigen_energies
+    r = np.random.default_rng(12)
+    nc, nb = 2, 2
+    ec = np.stack([np.array([[-1.0, 0.3], [0.3, -0.8]])] * 2)
+    eb = np.stack([np.diag(r.uniform(-1, 1, nb))] * 2)
+    vbc = np.stack([r.uniform(-0.5, 0.5, (nb, nc))] * 2)
+    u = np.array([[2.0, 1.2], [1.2, 2.0]])
+    m = AIMParams(nc, nb, ec, eb, vbc, u, np.zeros((2, 2)), 0.0, 1e-10)
+    beta = 8.0
+    wn = np.pi / beta * (2 * np.arange(1, 25) - 1)
+    states = get_eigen_energies(m, neigen=36)
+    G = compute_green_matrix(m, states, beta, wn, nitergreen=36)
+    G_cf = compute_green_matrix(m, states, beta, wn, nitergreen=36, full_ed_switch=0)
+    ns = m.ns
+    spec = {(nu, nd): np.linalg.eigh(bf.sector_hamiltonian(m, nu, nd).toarray())
+            for nu in range(ns + 1) for nd in range(ns + 1)}
+    e0 = min(v[0].min() for v in spec.values())
+    Z = sum(np.exp(-beta * (v[0] - e0)).sum() for v in spec.values())
+    Gref = np.zeros((nc, nc, len(wn)), complex)
+    for (nu, nd), (em, vm) in spec.items():
+        if nu == ns:
+            continue
+        en, vn = spec[(nu + 1, nd)]
+        for a in range(len(em)):
+            amp = [vn.T @ bf.apply_c_full(m, nu, nd, c, 0, +1, vm[:, a]) for c in range(nc)]
+            wgt = np.exp(-beta * (em[a] - e0)) + np.exp(-beta * (en - e0))
+            den = 1j * wn[None, :] - (en - em[a])[:, None]
+            for i in range(nc):
+                for j in range(nc):
+                    Gref[i, j] += ((amp[i] * amp[j] * wgt)[:, None] / den).sum(0)
+    Gref /= Z
+    assert np.max(np.abs(Gref[0, 1])) > 1e-3   # a real off-diagonal element
+    assert np.max(np.abs(G - Gref)) < 1e-9
+    assert np.max(np.abs(G_cf - Gref)) < 1e-9

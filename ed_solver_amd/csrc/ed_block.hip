@@ -113,15 +113,24 @@ __global__ __launch_bounds__(256) void k_gram_part(int64_t n, int p, int q, cons
     }
 }
 
-// G[i][j] (row-major p x q) = sum over row blocks, in order, of the partial tiles.
-__global__ void k_gram_sum(int nrb, int p, int q, int p16, const double *part, double *G)
+// G[i][j] (row-major p x q) = sum over the nrb row blocks of the partial tiles.  One
+// workgroup per row i of G: thread t takes column j = t % 16 and the row blocks
+// b = t / 16, t / 16 + 16, ... (each step reads one contiguous 128-byte row of a tile
+// per 16 threads), then the 16 partial sums of a column are added in group order: a
+// fixed order, so every call on the same data gives the same bits.
+__global__ __launch_bounds__(256) void k_gram_sum(int nrb, int p, int q, int p16, const double *part, double *G)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p * q) return;
-    const int i = e / q, j = e - i * q;
+    __shared__ double red[256];
+    const int i = blockIdx.x, t = threadIdx.x, j = t & 15, grp = t >> 4;
     double s = 0.0;
-    for (int b = 0; b < nrb; ++b) s += part[((int64_t)b * p16 + i) * 16 + j];
-    G[e] = s;
+    for (int b = grp; b < nrb; b += 16) s += part[((int64_t)b * p16 + i) * 16 + j];
+    red[t] = s;
+    __syncthreads();
+    if (t < q) {
+        double acc = 0.0;
+        for (int g = 0; g < 16; ++g) acc += red[g * 16 + t];
+        G[(int64_t)i * q + t] = acc;
+    }
 }
 
 // Y[:, 0:q] = beta Y + alpha X[:, 0:p] C (C row-major p x q, q <= 16).  Each wave owns
@@ -243,7 +252,7 @@ static int gram_dev(int dev, int64_t n, int p, int q, const double *X, int64_t l
     ED_TRY(bws_get(dev, gram_part_size(n, p), 256, w));
     const dim3 grid((p16 / 16 + kGramTiles - 1) / kGramTiles, nrb);
     hipLaunchKernelGGL(k_gram_part, grid, dim3(256), 0, st, n, p, q, X, ldx, Y, ldy, rows, p16, w->part);
-    hipLaunchKernelGGL(k_gram_sum, dim3((p * q + 255) / 256), dim3(256), 0, st, nrb, p, q, p16,
+    hipLaunchKernelGGL(k_gram_sum, dim3(p), dim3(256), 0, st, nrb, p, q, p16,
                        (const double *)w->part, d_G);
     ED_HIP_CHECK(hipGetLastError());
     return ED_OK;

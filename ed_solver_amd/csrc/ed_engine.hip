@@ -177,7 +177,6 @@ __global__ void k_bar_reset(LzState *st)
     }
     if (threadIdx.x == 0) {
         st->bar_count = 0;
-        st->bar_top = 0;
         st->bar_err = 0;
     }
 }

@@ -60,8 +60,6 @@ struct LzState {
     int pad;
     unsigned bar_count;   // persistent Lanczos kernel: group leaders' arrivals since its launch
     int bar_err;          // a barrier wait exceeded its spin limit
-    unsigned bar_top;     // barriers released since the launch
-    int pad3;
     double alpha[kMaxIter];      // alpha_j        -> alpha[j-1]
     double beta[kMaxIter + 1];   // beta_1 = ||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
     // grid barrier groups of the persistent Lanczos kernel, one 64-byte line per word

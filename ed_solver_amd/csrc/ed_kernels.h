@@ -570,13 +570,13 @@ __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin,
 // data the workgroups exchange inside the kernel (Krylov vector, partial sums) is written
 // and read with device-scope coherent accesses (ld_agent / st_agent), so the barrier
 // needs no cache write-back or invalidation: every wave waits for its own accesses
-// (s_waitcnt 0) and the workgroup meets.  Then a two-level tree on counters that only
+// (s_waitcnt 0) and the workgroup meets.  Then a two-level arrival on counters that only
 // grow during the launch (zeroed before it): thread 0 adds one to its group's counter
 // (kBarGroupSize workgroups per group); the group's last arrival adds one to
 // st->bar_count, and the last of those stores the barrier number `b` (1, 2, ... in the
-// launch) in st->bar_top.  Group leaders poll bar_top and then release their group by
-// storing b in its word; the other workgroups poll that word.  So no word has more than
-// kBarGroupSize writers or readers.  A poll that exceeds `spin_limit` rounds (or sees
+// launch) in every group's release word; every other workgroup polls its group's word.
+// So no word has more than kBarGroupSize writers or readers, and a release reaches the
+// pollers one store after the last arrival.  A poll that exceeds `spin_limit` rounds (or sees
 // another workgroup's timeout) sets st->bar_err and status 3 and leaves, so the kernel
 // always ends; returns true then.
 __device__ inline bool bar_wait(unsigned *word, unsigned b, LzState *st, int spin_limit)
@@ -599,14 +599,16 @@ __device__ inline bool grid_barrier(LzState *st, unsigned b, int spin_limit, int
         const unsigned sz = min((unsigned)kBarGroupSize, nblk - g * kBarGroupSize);
         bool err = false;
         const unsigned c = __hip_atomic_fetch_add(&st->bar_grp[16 * g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (c == sz * b - 1) {   // last of the group
+        bool released = false;
+        if (c == sz * b - 1) {   // last of the group: arrive at the top
             const unsigned t = __hip_atomic_fetch_add(&st->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t == ng * b - 1) __hip_atomic_store(&st->bar_top, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else err = bar_wait(&st->bar_top, b, st, spin_limit);
-            __hip_atomic_store(&st->bar_rel[16 * g], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            err = bar_wait(&st->bar_rel[16 * g], b, st, spin_limit);
+            if (t == ng * b - 1) {   // last of all: release every group directly
+                for (unsigned h = 0; h < ng; ++h)
+                    __hip_atomic_store(&st->bar_rel[16 * h], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
         }
+        if (!released) err = bar_wait(&st->bar_rel[16 * g], b, st, spin_limit);
         if (err) {
             __hip_atomic_store(&st->bar_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->status = 3;

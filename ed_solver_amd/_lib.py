@@ -76,12 +76,14 @@ SIGNATURES = {
     "ed_lanczos_diagonalize": (ctypes.c_int, [_vp, ctypes.POINTER(LanczosOpts), _dp, _c_int_p, _c_int_p, _dp, _dp,
                                               _dp, _dp]),
     "ed_lanczos_info": (ctypes.c_int, [_vp, _c_int_p]),
+    "ed_lanczos_steps_run": (ctypes.c_int, [_vp, _c_int_p]),
     "ed_full_ed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_gs_vector": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_gs_vector_dev": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _vp]),
     "ed_rayleigh_f64": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp]),
     "ed_tridiag_eig": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "ed_tridiag_lowest": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_band_eig": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_block_default_opts": (ctypes.c_int, [ctypes.POINTER(BlockOpts)]),
     "ed_block_lanczos": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _dp, ctypes.c_int64, _c_int_p,

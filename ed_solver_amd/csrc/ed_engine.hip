@@ -27,8 +27,10 @@
 // workgroup obtains the same bits).  This avoids cross-XCD atomics and release fences,
 // which cost more than the reduction itself on MI355X.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -159,14 +161,19 @@ __global__ __launch_bounds__(kBlock) void k_lz_finish(LzArgs<T> lz)
 
 // Persistent Lanczos: nsteps steps of the fast H.v and the update in one cooperative
 // launch (see lz_persist_body).  256 threads, at most 4 workgroups per CU resident.
-template <typename T, int K, int R>
-__global__ __launch_bounds__(kBlock, 4) void k_lz_persist(HvArgs<T> a, LzArgs<T> lz, int nsteps, int spin_limit)
+// MON: the monitored form (LzMonitor), its own instance so that the plain one carries no
+// monitor registers.
+template <typename T, int K, int R, bool MON>
+__global__ __launch_bounds__(kBlock, 4) void k_lz_persist(HvArgs<T> a, LzArgs<T> lz, int nsteps, int spin_limit,
+                                                          LzMonitor mon)
 {
     extern __shared__ double smem_d[];
     __shared__ double red[kRed];
     __shared__ double grp[kSums];
+    __shared__ double pub[2];
     __shared__ int flag;
-    lz_persist_body<T, K, R>(a, lz, nsteps, spin_limit, reinterpret_cast<unsigned char *>(smem_d), red, grp, &flag);
+    lz_persist_body<T, K, R, MON>(a, lz, nsteps, spin_limit, mon, reinterpret_cast<unsigned char *>(smem_d), red,
+                                  grp, pub, &flag);
 }
 
 __global__ void k_bar_reset(LzState *st)
@@ -178,6 +185,7 @@ __global__ void k_bar_reset(LzState *st)
     if (threadIdx.x == 0) {
         st->bar_count = 0;
         st->bar_err = 0;
+        st->stop_at = 0;
     }
 }
 
@@ -627,34 +635,35 @@ constexpr int kBarrierSpins = 1 << 21;
 constexpr int kPersistMinSteps = 16;
 
 template <typename T, int K, int R>
-static const void *persist_fn()
+static const void *persist_fn(bool mon)
 {
-    return reinterpret_cast<const void *>(&k_lz_persist<T, K, R>);
+    return mon ? reinterpret_cast<const void *>(&k_lz_persist<T, K, R, true>)
+               : reinterpret_cast<const void *>(&k_lz_persist<T, K, R, false>);
 }
 
 // The instances compiled: those whose rows fit the 128-register budget of a co-resident
 // grid (4 workgroups of 256 threads per CU) without spilling in the step loop; wider
 // tables with more rows per thread keep the two-kernel step.
 template <typename T>
-static const void *persist_kernel(int k, int rows)
+static const void *persist_kernel(int k, int rows, bool mon)
 {
     constexpr bool f32 = std::is_same<T, float>::value;
     switch (k * 8 + rows) {
-    case 4 * 8 + 1: return persist_fn<T, 4, 1>();
-    case 4 * 8 + 2: return persist_fn<T, 4, 2>();
-    case 4 * 8 + 4: return persist_fn<T, 4, 4>();
-    case 8 * 8 + 1: return persist_fn<T, 8, 1>();
-    case 8 * 8 + 2: return persist_fn<T, 8, 2>();
-    case 8 * 8 + 4: return persist_fn<T, 8, 4>();
-    case 12 * 8 + 1: return persist_fn<T, 12, 1>();
-    case 12 * 8 + 2: return f32 ? persist_fn<float, 12, 2>() : nullptr;
-    case 16 * 8 + 1: return persist_fn<T, 16, 1>();
+    case 4 * 8 + 1: return persist_fn<T, 4, 1>(mon);
+    case 4 * 8 + 2: return persist_fn<T, 4, 2>(mon);
+    case 4 * 8 + 4: return persist_fn<T, 4, 4>(mon);
+    case 8 * 8 + 1: return persist_fn<T, 8, 1>(mon);
+    case 8 * 8 + 2: return persist_fn<T, 8, 2>(mon);
+    case 8 * 8 + 4: return persist_fn<T, 8, 4>(mon);
+    case 12 * 8 + 1: return persist_fn<T, 12, 1>(mon);
+    case 12 * 8 + 2: return f32 ? persist_fn<float, 12, 2>(mon) : nullptr;
+    case 16 * 8 + 1: return persist_fn<T, 16, 1>(mon);
     default: return nullptr;
     }
 }
 
 struct PersistShape {
-    const void *fn = nullptr;
+    const void *fn = nullptr, *fn_mon = nullptr;   // plain and monitored instance
     int nblk = 0, rows = 0;
     size_t lds = 0;
 };
@@ -670,17 +679,20 @@ static bool persist_shape(ed_sector *s, PersistShape &p)
     const HvShape h = hv_shape(s, sched_of<T>(s));
     if (h.kind != 2 || s->up.d > 4 * kBlock) return false;
     p.rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
-    p.fn = persist_kernel<T>(h.k, p.rows);
+    p.fn = persist_kernel<T>(h.k, p.rows, false);
+    p.fn_mon = persist_kernel<T>(h.k, p.rows, true);
     p.nblk = h.nblk;
     p.lds = lz_persist_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(T));
     if (!p.fn) return false;
     if (s->persist_ok[pi] < 0) {
-        int dev = 0, per_cu = 0, ncu = 0, coop = 0;
+        int dev = 0, per_cu = 0, per_cu_mon = 0, ncu = 0, coop = 0;
         bool ok = hipGetDevice(&dev) == hipSuccess &&
                   hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
                   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, kBlock, p.lds) == hipSuccess &&
-                  (int64_t)per_cu * ncu >= p.nblk && p.nblk <= kBarGroups * kBarGroupSize && p.lds <= 64 * 1024;
+                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_mon, p.fn_mon, kBlock, p.lds) == hipSuccess &&
+                  (int64_t)std::min(per_cu, per_cu_mon) * ncu >= p.nblk && p.nblk <= kBarGroups * kBarGroupSize &&
+                  p.lds <= 64 * 1024;
         (void)hipGetLastError();
         s->persist_ok[pi] = ok ? 1 : 0;
     }
@@ -688,7 +700,8 @@ static bool persist_shape(ed_sector *s, PersistShape &p)
 }
 
 template <typename T>
-static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipStream_t st)
+static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipStream_t st,
+                          LzMonitor mon = LzMonitor{nullptr, nullptr})
 {
     HvArgs<T> a = hv_args<T>(s, 1);
     LzArgs<T> lz = lz_args<T>(s);
@@ -696,9 +709,10 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
     if (h.xcd) a.colord = s->d_colord;
     lz.nA = lz.nB = p.nblk;
     int spins = kBarrierSpins;
-    void *args[] = {&a, &lz, &nsteps, &spins};
+    void *args[] = {&a, &lz, &nsteps, &spins, &mon};
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, st, lz.st);
-    ED_HIP_CHECK(hipLaunchCooperativeKernel(p.fn, dim3(p.nblk), dim3(kBlock), args, (unsigned)p.lds, st));
+    ED_HIP_CHECK(hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk), dim3(kBlock), args,
+                                            (unsigned)p.lds, st));
     hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
     return ED_OK;
 }
@@ -944,9 +958,98 @@ struct PhaseOut {
     std::vector<double> alpha, beta;
 };
 
+// The convergence test of Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134), one
+// iteration at a time: iteration `it` ends the loop when the neig lowest eigenvalues of
+// T_it all moved by less than tol since T_{it-1} (it > 2), at it == niter, or when the
+// first step already gives beta_2 = 0.
+struct ConvTest {
+    int neig, niter;
+    double tol;
+    std::vector<double> last, w;
+    ConvTest(int neig_, int niter_, double tol_) : neig(neig_), niter(niter_), tol(tol_), last(neig_, 1e300), w(neig_) {}
+    bool ends(int it, const double *alpha, const double *beta)
+    {
+        tridiag_lowest_warm(it, alpha, beta, neig, last.data(), w.data());
+        const int m = std::min(neig, it);
+        bool allc = true;
+        for (int k = 0; k < neig; ++k)
+            if (!((k < m) && std::fabs(w[k] - last[k]) < tol && it > 2)) allc = false;
+        const bool one_step = (it == 1) && std::fabs(beta[1]) < 1e-24;
+        if (allc || it == niter || one_step) return true;
+        for (int k = 0; k < m; ++k) last[k] = w[k];
+        return false;
+    }
+};
+
+// The pinned host channel of monitored persistent runs (LzMonitor): fine-grained, so the
+// kernel's system-scope stores reach the host while it runs and its loads see the flag.
+static int ensure_monitor(ed_sector *s)
+{
+    if (s->h_mon) return ED_OK;
+    void *h = nullptr, *d = nullptr;
+    ED_HIP_CHECK(hipHostMalloc(&h, sizeof(double) * (16 + 2 * (size_t)kMaxIter),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(ED_ERR_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    }
+    s->h_mon = (double *)h;
+    s->d_mon = (double *)d;
+    return ED_OK;
+}
+
+// Steps done+1 .. done+nsteps in one persistent launch while the host runs the
+// convergence test on the scalars the kernel publishes; once an iteration ends the loop
+// the kernel is told to stop (it does a few steps later; those steps are never used).
+// alpha/beta: host copies, filled from index `done` on as the scalars arrive; `tested`:
+// the last iteration tested (in order, as the chunked loop does).  Returns with the
+// stream drained; the scalars of the steps not yet tested are then read from the device.
+template <typename T>
+static int run_monitored(ed_sector *s, const PersistShape &p, int done, int nsteps, ConvTest &ct,
+                         std::vector<double> &alpha, std::vector<double> &beta, int &tested, int &final_iter)
+{
+    ED_TRY(ensure_monitor(s));
+    hipStream_t st = s->stream;
+    volatile int *stop = reinterpret_cast<volatile int *>(s->h_mon);
+    volatile double *ab = s->h_mon + 16;
+    *stop = 0;
+    for (int i = 2 * done; i < 2 * (done + nsteps); ++i) ab[i] = std::numeric_limits<double>::quiet_NaN();
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    ED_TRY(launch_persist<T>(s, p, nsteps, st, LzMonitor{reinterpret_cast<const int *>(s->d_mon), s->d_mon + 16}));
+    ED_HIP_CHECK(hipGetLastError());
+    int avail = done;   // steps whose alpha and beta are on the host
+    while (!final_iter) {
+        while (avail < done + nsteps) {
+            const double b = ab[2 * avail], a = ab[2 * avail + 1];
+            if (std::isnan(a) || std::isnan(b)) break;
+            alpha[avail] = a;
+            beta[avail] = b;
+            ++avail;
+        }
+        bool progressed = false;
+        // iteration 1 also needs beta_2 (published with step 2)
+        while (!final_iter && tested < avail && (tested > 0 || avail >= 2)) {
+            ++tested;
+            progressed = true;
+            if (ct.ends(tested, alpha.data(), beta.data())) {
+                final_iter = tested;
+                *stop = 1;
+                std::atomic_thread_fence(std::memory_order_seq_cst);
+            }
+        }
+        if (!progressed && hipStreamQuery(st) != hipErrorNotReady) break;   // ended (errors: the sync reports them)
+    }
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
 // One Lanczos_fast_diagonalize loop (Lanczos_plain.f90:33-134) in precision `prec`
-// from a host or device start vector: chunks of `chunk` steps on the device, then the
-// reference's convergence test on the host for every step of the chunk.
+// from a host or device start vector, with the reference's convergence test on the host
+// for every step.  Where the persistent kernel applies, one launch runs up to the whole
+// loop while the host tests the steps as they come (run_monitored); otherwise chunks of
+// `chunk` steps alternate with the tests.  Both give the same scalars, so the same
+// iteration ends the loop.
 static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig, int chunk, int reorth,
                          const double *h_start, const double *d_start, PhaseOut &out)
 {
@@ -954,29 +1057,40 @@ static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig
     ED_TRY(ensure_basis(s, prec, niter + 1));
     s->basis_valid_iters = 0;
     ED_TRY(lz_start(s, prec, h_start, d_start));
-    std::vector<double> last(neig, 1e300), w(neig);
+    ConvTest ct(neig, niter, tol);
+    std::vector<double> alpha(niter + 1), beta(niter + 2);
     HostLz h;
-    int done = 0, final_iter = 0;
+    int done = 0, final_iter = 0, tested = 0;
     while (done < niter && !final_iter) {
-        const int c = std::min(chunk, niter - done);
-        ED_TRY(steps(s, prec, c, chunk, reorth));
+        int c = std::min(chunk, niter - done);
+        PersistShape ps;
+        const bool mon = !reorth && niter - done >= kPersistMinSteps &&
+                         (prec == ED_PREC_F64 ? persist_shape<double>(s, ps) : persist_shape<float>(s, ps));
+        if (mon) {
+            c = niter - done;
+            if (prec == ED_PREC_F64) ED_TRY(run_monitored<double>(s, ps, done, c, ct, alpha, beta, tested, final_iter));
+            else ED_TRY(run_monitored<float>(s, ps, done, c, ct, alpha, beta, tested, final_iter));
+        } else {
+            ED_TRY(steps(s, prec, c, chunk, reorth));
+        }
         ED_TRY(read_state(s, h, done + c));
         if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
+        s->last_steps_run += std::min(h.j - 1, done + c) - done;
+        if (final_iter) break;
         const int completed = h.j - 1;   // steps whose beta_{j+1} is known
-        for (int it = done + 1; it <= done + c; ++it) {
+        for (int i = tested; i < done + c; ++i) {
+            alpha[i] = h.alpha[i];
+            beta[i] = h.beta[i];
+        }
+        beta[done + c] = h.beta[done + c];
+        for (int it = tested + 1; it <= done + c; ++it) {
             if (it > completed)   // the reference STOPs at the start of this step (Lanczos_plain.h:85-92)
                 return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(it));
-            tridiag_lowest(it, h.alpha.data(), h.beta.data(), neig, w.data());
-            const int m = std::min(neig, it);
-            bool allc = true;
-            for (int k = 0; k < neig; ++k)
-                if (!((k < m) && std::fabs(w[k] - last[k]) < tol && it > 2)) allc = false;
-            const bool one_step = (it == 1) && std::fabs(h.beta[1]) < 1e-24;
-            if (allc || it == niter || one_step) {
+            tested = it;
+            if (ct.ends(it, alpha.data(), beta.data())) {
                 final_iter = it;
                 break;
             }
-            for (int k = 0; k < m; ++k) last[k] = w[k];
         }
         done += c;
     }
@@ -1010,6 +1124,7 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
     const int chunk = std::max(1, o->check_every);
     PhaseOut ph;
     s->last_f32_iters = 0;
+    s->last_steps_run = 0;
     if (o->precision == ED_PREC_F32) {
         PhaseOut p32;
         ED_TRY(lanczos_phase(s, ED_PREC_F32, std::max(o->tolerance, 1e-6), niter, 1, chunk, o->reorth, start, nullptr,
@@ -1265,6 +1380,14 @@ int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase)
 {
     ED_TRY(check_common(sec, 0));
     if (iters_f32_phase) *iters_f32_phase = sec->last_f32_iters;
+    return ED_OK;
+}
+
+int ed_lanczos_steps_run(ed_sector *sec, int *steps)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!steps) return fail(ED_ERR_ARG, "ed_lanczos_steps_run: NULL argument");
+    *steps = sec->last_steps_run;
     return ED_OK;
 }
 

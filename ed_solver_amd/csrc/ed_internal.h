@@ -57,7 +57,8 @@ struct LzState {
     int j_k2;          // step index seen by the update phase
     int status;        // 0 ok, 1 zero-norm vector appeared, 2 zero-norm start vector,
                        // 3 grid barrier of the persistent Lanczos kernel timed out
-    int pad;
+    int stop_at;       // monitored persistent run: the step after which every workgroup
+                       // stops (0 = not asked to stop); zeroed before each launch
     unsigned bar_count;   // persistent Lanczos kernel: group leaders' arrivals since its launch
     int bar_err;          // a barrier wait exceeded its spin limit
     double alpha[kMaxIter];      // alpha_j        -> alpha[j-1]
@@ -114,6 +115,15 @@ struct HvArgs {
     int ncoef_up, kdn, nhund;
     const int32_t *hund_tr;        // [nhund] cu' | cd' << 8
     const int32_t *colord;         // fast kernel: workgroup -> column (XCD-grouped order, -1 = idle), or null
+};
+
+// Host-visible channel of a monitored persistent Lanczos run, in fine-grained pinned host
+// memory: the kernel publishes beta_j and alpha_j of every step (ab[2 (j-1)] and
+// ab[2 (j-1) + 1], NaN until written) while the host tests convergence, and stops after
+// the first step in which it reads *stop != 0.  Null pointers: no monitor.
+struct LzMonitor {
+    const int *stop;
+    double *ab;
 };
 
 // Where the Lanczos vectors live: slot(j) = base + ((j-1) % nslots) * ld, j >= 1, and
@@ -188,6 +198,9 @@ struct ed_sector {
     float *d_quart32 = nullptr;
     double energy_shift = 0.0;
     int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
+    double *h_mon = nullptr;       // monitored persistent run: pinned fine-grained host channel
+    double *d_mon = nullptr;       //   (int stop flag at [0], published scalars from [16]), device view
+    int last_steps_run = 0;        // Lanczos steps the device ran in the last diagonalisation
     int persist_ok[2] = {-1, -1};  // per precision (F64, F32): -1 not checked, 0 no, 1 co-resident grid
     // timing of the H.v kernel inside Lanczos chunks
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -200,6 +213,7 @@ namespace ed {
 // tridiagonal eigen-solver (host, FP64): implicit QL with Wilkinson shifts
 int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z);
 void tridiag_lowest(int n, const double *diag, const double *off, int k, double *w);
+void tridiag_lowest_warm(int n, const double *diag, const double *off, int k, const double *prev, double *w);
 int tridiag_lowest_pairs(int n, const double *diag, const double *off, int k, double *w, double *z);
 // nev lowest eigenpairs of a symmetric band matrix (n x n row-major, half-bandwidth b);
 // z (n x nev, column-major) may be null

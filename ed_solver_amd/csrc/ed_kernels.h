@@ -471,7 +471,10 @@ __device__ inline void lz_update_body(const LzArgs<T> &lz, int64_t n, int advanc
     const double bj = sqrt(sh[0]);
     const bool lead = blk == 0 && threadIdx.x == 0;
     if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
-        if (lead) lz.st->status = (j == 1) ? 2 : 1;
+        if (lead) {
+            lz.st->beta[j - 1] = bj;   // the test of iteration j - 1 reads it
+            lz.st->status = (j == 1) ? 2 : 1;
+        }
         return;
     }
     const double alpha = sh[2] / (bj * bj) - (j > 1 ? sh[1] / bprev : 0.0);
@@ -578,7 +581,9 @@ __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin,
 // So no word has more than kBarGroupSize writers or readers, and a release reaches the
 // pollers one store after the last arrival.  A poll that exceeds `spin_limit` rounds (or sees
 // another workgroup's timeout) sets st->bar_err and status 3 and leaves, so the kernel
-// always ends; returns true then.
+// always ends; returns 1 then.  With `stop_word`, thread 0 reads it after the release and
+// the barrier returns 2 when it holds a step <= `jdone` (the word is written before the
+// writer's arrival, so every workgroup reads the same value), else 0.
 __device__ inline bool bar_wait(unsigned *word, unsigned b, LzState *st, int spin_limit)
 {
     for (int spins = 0;; ++spins) {
@@ -589,7 +594,8 @@ __device__ inline bool bar_wait(unsigned *word, unsigned b, LzState *st, int spi
     }
 }
 
-__device__ inline bool grid_barrier(LzState *st, unsigned b, int spin_limit, int *flag)
+__device__ inline int grid_barrier(LzState *st, unsigned b, int spin_limit, int *flag,
+                                   const int *stop_word = nullptr, int jdone = 0)
 {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -613,10 +619,15 @@ __device__ inline bool grid_barrier(LzState *st, unsigned b, int spin_limit, int
             __hip_atomic_store(&st->bar_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->status = 3;
         }
-        *flag = err ? 1 : 0;
+        int stop = 0;
+        if (!err && stop_word) {
+            const int sa = ld_agent(stop_word);
+            stop = sa != 0 && sa <= jdone;
+        }
+        *flag = err ? 1 : stop ? 2 : 0;
     }
     __syncthreads();
-    return *flag != 0;
+    return *flag;
 }
 
 // Bytes of dynamic LDS of the persistent Lanczos kernel: the fast H.v's layout, then
@@ -643,9 +654,17 @@ __host__ __device__ inline size_t lz_persist_lds(int dup, int ncoef, int ncfg, i
 // (the fast kernel at 256 threads and the update kernel), so alpha and beta carry the
 // same bits.  Breakdown (beta_j below kMinLanczosNorm) is seen by every workgroup alike
 // and ends the loop everywhere.  The caller launches k_lz_finish afterwards.
-template <typename T, int K, int R>
+// With a monitor (MON), thread 0 of workgroup 0 publishes beta and alpha of each step
+// (kept in LDS, pub[2]) to the host at the start of the next one (the stores complete
+// during its H.v), loads the host's stop flag at the start of every H.v phase and, once it is set,
+// writes the current step to st->stop_at in the update phase; every workgroup reads that
+// word after the second barrier of the step and leaves the loop together.  The word is
+// written after the first barrier of a step and read before the first barrier of the
+// next, so no workgroup can see a value written in a later step.
+template <typename T, int K, int R, bool MON>
 __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, int nsteps, int spin_limit,
-                                       unsigned char *smem, double *red, double *sh, int *flag)
+                                       LzMonitor mon, unsigned char *smem, double *red, double *sh, double *pub,
+                                       int *flag)
 {
     constexpr int NT = kBlock;
     LzState *st = lz.st;
@@ -713,9 +732,17 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
         for (int e = tid; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
     }
     double bprev = j > 1 ? st->beta[j - 2] : 1.0;
+    int hflag = 0;   // lead: the host's stop flag as read in this step
     __syncthreads();
 
     for (int s = 0; s < nsteps; ++s) {
+        if (MON && lead) {
+            if (s > 0) {
+                __hip_atomic_store(mon.ab + 2 * (j - 2), pub[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(mon.ab + 2 * (j - 2) + 1, pub[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            hflag = __hip_atomic_load(mon.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
         // ---- H.v phase of step j (u_j of the own column is in col)
         const T *__restrict__ x = lz.slot(j);
         T *__restrict__ y = lz.slot(j + 1);
@@ -771,13 +798,21 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
         __syncthreads();
         const double bj = sqrt(sh[0]);
         if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
-            if (lead) st->status = (j == 1) ? 2 : 1;
+            if (lead) {
+                st->beta[j - 1] = bj;   // the test of iteration j - 1 reads it
+                st->status = (j == 1) ? 2 : 1;
+            }
             return;
         }
         const double alpha = sh[2] / (bj * bj) - (j > 1 ? sh[1] / bprev : 0.0);
         if (lead) {
             st->beta[j - 1] = bj;
             st->alpha[j - 1] = alpha;
+            if (MON) {
+                pub[0] = bj;
+                pub[1] = alpha;
+                if (hflag) st_agent(&st->stop_at, j);
+            }
         }
         const double cz = 1.0 / bj, cp = j > 1 ? bj / bprev : 0.0, cx = alpha / bj;
         double nn = 0.0, cr = 0.0;
@@ -801,7 +836,7 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
             st->j_k1 = j;
             st->j_k2 = j - 1;
         }
-        if (grid_barrier(st, ++nbar, spin_limit, flag)) return;
+        if (grid_barrier(st, ++nbar, spin_limit, flag, MON ? &st->stop_at : nullptr, j - 1)) return;
     }
 }
 

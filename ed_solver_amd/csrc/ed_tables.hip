@@ -474,6 +474,7 @@ static void free_sector(ed_sector *S)
         if (G.exec) (void)hipGraphExecDestroy(G.exec);
         if (G.graph) (void)hipGraphDestroy(G.graph);
     }
+    if (S->h_mon) (void)hipHostFree(S->h_mon);
     if (S->ev0) (void)hipEventDestroy(S->ev0);
     if (S->ev1) (void)hipEventDestroy(S->ev1);
     if (S->stream) (void)hipStreamDestroy(S->stream);

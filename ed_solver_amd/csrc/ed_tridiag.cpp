@@ -235,6 +235,195 @@ void tridiag_lowest(int n, const double *diag, const double *off, int k, double 
     for (int i = n; i < k; ++i) w[i] = 1e300;
 }
 
+// Pivots q_i of T - x = L D L^T for (d[n], e2[i] = e[i]^2): *cnt = the number of negative
+// ones (eigenvalues below x) and *s = d/dx log |det(T - x)| = sum_i q_i'/q_i.  False when a
+// pivot vanishes or a value is not finite.
+static bool log_det_slope(int n, const double *d, const double *e2, double x, int *cnt, double *s)
+{
+    double q = d[0] - x, dq = -1.0;
+    if (q == 0.0) return false;
+    int c = q < 0.0;
+    double r = 1.0 / q, acc = dq * r;
+    for (int i = 1; i < n; ++i) {
+        const double t = e2[i - 1] * r;   // e^2 / q_{i-1}
+        dq = -1.0 + t * dq * r;           // q_i' = -1 + e^2 q_{i-1}' / q_{i-1}^2
+        q = (d[i] - x) - t;               // q_i = d_i - x - e^2 / q_{i-1}
+        if (q == 0.0) return false;
+        c += q < 0.0;
+        r = 1.0 / q;
+        acc += dq * r;
+    }
+    *cnt = c;
+    *s = acc;
+    return std::isfinite(acc);
+}
+
+static int count_below(int n, const double *d, const double *e2, double x)
+{
+    int cnt = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+        q = d[i] - x - (i ? e2[i - 1] / q : 0.0);
+        if (q == 0.0) q = -DBL_EPSILON * (std::fabs(x) + 1e-300);
+        if (q < 0.0) ++cnt;
+    }
+    return cnt;
+}
+
+// Eigenvalue number j in [a, b] (count_below(a) <= j < count_below(b)) by bisection to
+// 2 eps |lambda|.
+static double bisect_eig(int n, const double *d, const double *e2, int j, double a, double b)
+{
+    for (int it = 0; it < 200 && b - a > 2.0 * DBL_EPSILON * std::max(std::fabs(a), std::fabs(b)) + 1e-300; ++it) {
+        const double mid = 0.5 * (a + b);
+        if (count_below(n, d, e2, mid) > j) b = mid;
+        else a = mid;
+    }
+    return 0.5 * (a + b);
+}
+
+// Eigenvalue number j by Newton's method on det(T - x) / prod_{i<j} (x - below[i])
+// (Maehly's deflation of the j eigenvalues already found) from an x with exactly j
+// eigenvalues below it: the deflated polynomial has real roots, all at or above lambda_j,
+// so the iterates increase monotonically to lambda_j (quadratically near a simple root).
+// A plain Lanczos run whose Ritz value has converged carries copies of it (lost
+// orthogonality), and at a root of multiplicity m Newton's steps shrink only by (m-1)/m:
+// when two steps shrink by more than 0.3, the step is multiplied by the m that ratio gives
+// (kept only while the point stays below lambda_j).  Stops when the step falls below
+// 2 eps |x|; an iterate that passes the root by rounding leaves a bracket of one step,
+// which bisection finishes.  False when x does not have j eigenvalues below it, a value
+// is not finite or 100 steps do not settle.
+static bool newton_eig(int n, const double *d, const double *e2, int j, const double *below, double x, double &w)
+{
+    int c;
+    double s, last = 0.0;
+    auto slope = [&](double y, double &g) {
+        if (!log_det_slope(n, d, e2, y, &c, &g) || c != j) return false;
+        for (int i = 0; i < j; ++i) g -= 1.0 / (y - below[i]);
+        return std::isfinite(g) && g < 0.0;
+    };
+    if (!slope(x, s)) return false;
+    for (int it = 0; it < 100; ++it) {
+        const double step = -1.0 / s;
+        if (!(step > 2.0 * DBL_EPSILON * std::fabs(x) + 1e-300)) {
+            w = x + step;
+            return std::isfinite(w);
+        }
+        const double ratio = last > 0.0 ? step / last : 0.0;
+        last = step;
+        double sn, xn;
+        if (ratio > 0.3 && ratio < 1.0) {
+            const double m = std::min(8.0, std::floor(1.0 / (1.0 - ratio) + 0.5));
+            xn = x + m * step;
+            if (m > 1.0 && slope(xn, sn)) {
+                x = xn;
+                s = sn;
+                last = 0.0;   // the next ratio compares plain steps again
+                continue;
+            }
+        }
+        xn = x + step;
+        if (!slope(xn, sn)) {
+            if (count_below(n, d, e2, xn) <= j) return false;   // not finite on the way
+            w = bisect_eig(n, d, e2, j, x, xn);
+            return true;
+        }
+        x = xn;
+        s = sn;
+    }
+    return false;
+}
+
+// Eigenvalue number j in [a, b] (count_below(a) <= j < count_below(b)), lambda_{j-1} =
+// below[j-1] for j > 0: bisection until a has exactly j eigenvalues below it and the
+// bracket is small against its distance to lambda_{j-1} (or to the lower end of the
+// spectrum `lo`), then Newton's method from a (newton_eig, quadratic from there); bisection
+// to the end if that fails.
+static double bracketed_eig(int n, const double *d, const double *e2, int j, const double *below, double lo, double a,
+                            double b)
+{
+    const double floor_j = j ? below[j - 1] : lo;
+    bool tried = false;
+    int ca = count_below(n, d, e2, a);
+    for (int it = 0; it < 200 && b - a > 2.0 * DBL_EPSILON * std::max(std::fabs(a), std::fabs(b)) + 1e-300; ++it) {
+        if (!tried && ca == j && a > floor_j && b - a < 0.25 * (a - floor_j)) {
+            tried = true;
+            double w;
+            if (newton_eig(n, d, e2, j, below, a, w) && w >= a && w <= b) return w;
+        }
+        const double mid = 0.5 * (a + b);
+        const int cm = count_below(n, d, e2, mid);
+        if (cm > j) {
+            b = mid;
+        } else {
+            a = mid;
+            ca = cm;
+        }
+    }
+    return 0.5 * (a + b);
+}
+
+// The k lowest eigenvalues of T_n (diag[n], off[1..n-1]) when those of T_{n-1} are known
+// (prev[j]; 1e300 or more where not), as the Lanczos convergence test needs them every
+// step.  Cauchy interlacing gives lambda_j(T_n) <= lambda_j(T_{n-1}): each is found by
+// Newton's method (newton_eig) from just below prev[j], the offset grown fourfold until
+// the Sturm count confirms the start lies between lambda_{j-1} and lambda_j, else in a
+// bracket below prev[j] (bracketed_eig).  A non-finite entry falls back to tridiag_lowest.
+void tridiag_lowest_warm(int n, const double *diag, const double *off, int k, const double *prev, double *w)
+{
+    if (!prev || k > 32 || k >= n || n < 2) {
+        tridiag_lowest(n, diag, off, k, w);
+        return;
+    }
+    std::vector<double> d(diag, diag + n), e2(n, 0.0);
+    bool finite = true;
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double el = i ? std::fabs(off[i]) : 0.0, er = i + 1 < n ? std::fabs(off[i + 1]) : 0.0;
+        finite = finite && std::isfinite(d[i]) && std::isfinite(er);
+        lo = std::min(lo, d[i] - el - er);
+        hi = std::max(hi, d[i] + el + er);
+        if (i + 1 < n) e2[i] = off[i + 1] * off[i + 1];
+    }
+    if (!finite) {
+        tridiag_lowest(n, diag, off, k, w);
+        return;
+    }
+    const double span = std::max(hi - lo, 1e-300);
+    lo -= 1e-12 * span;
+    hi += 1e-12 * span;
+    for (int j = 0; j < k; ++j) {
+        const double floor_j = j ? w[j - 1] : lo;   // the start must lie above lambda_{j-1}
+        bool ok = false;
+        double a = j ? w[j - 1] - 4.0 * DBL_EPSILON * std::fabs(w[j - 1]) - 1e-300 : lo, b = hi;
+        if (prev[j] < 1e299) {
+            double dl = 1e-7 * std::max(1.0, std::fabs(prev[j]));
+            for (int t = 0; t < 24 && !ok; ++t, dl *= 4.0) {
+                const double x0 = prev[j] - dl;
+                if (x0 <= floor_j) break;
+                int c;
+                double g;
+                if (!log_det_slope(n, d.data(), e2.data(), x0, &c, &g)) continue;
+                if (c > j) {   // lambda_j moved below the start: a tighter upper end
+                    b = std::min(b, x0);
+                    continue;
+                }
+                if (c < j) break;   // below lambda_{j-1}: bracket it instead
+                a = std::max(a, x0);
+                ok = newton_eig(n, d.data(), e2.data(), j, w, x0, w[j]);
+            }
+        }
+        if (!ok) {
+            if (count_below(n, d.data(), e2.data(), a) > j) a = lo;
+            if (b == hi && prev[j] < 1e299) {
+                const double bb = prev[j] + 8.0 * DBL_EPSILON * std::fabs(prev[j]) + 1e-300;
+                if (bb < hi && count_below(n, d.data(), e2.data(), bb) > j) b = bb;
+            }
+            w[j] = bracketed_eig(n, d.data(), e2.data(), j, w, lo, a, b);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- band matrices
 // Lowest eigenpairs of the symmetric band matrix of a block Lanczos run (half-bandwidth
 // b = block size), which takes the role of BLZPACK's Ritz computation inside BLZDRD
@@ -468,6 +657,13 @@ int ed_band_eig(int n, int b, const double *a, int nev, double *w, double *z)
 int ed_tridiag_eig(int n, const double *diag, const double *subdiag, double *w, double *z)
 {
     return ed::tridiag_eig(n, diag, subdiag, w, z);
+}
+
+int ed_tridiag_lowest(int n, const double *diag, const double *subdiag, int k, const double *prev, double *w)
+{
+    if (n < 1 || k < 1 || !diag || !subdiag || !w) return ed::fail(ED_ERR_ARG, "ed_tridiag_lowest: bad argument");
+    ed::tridiag_lowest_warm(n, diag, subdiag, k, prev, w);
+    return ED_OK;
 }
 
 }  // extern "C"

@@ -50,6 +50,7 @@ class LanczosResult:
     subdiag: np.ndarray      # ||v1||, beta_2..beta_iters
     precision: str
     iters_f32: int = 0       # FP32 phase of the mixed form (0 for FP64)
+    steps_run: int = 0       # steps the device ran (>= the iterations: a persistent run stops a few steps late)
 
 
 class Sector:
@@ -189,10 +190,11 @@ class Sector:
         check(lib().ed_lanczos_diagonalize(self.handle, ctypes.byref(o), sp, ctypes.byref(it), ctypes.byref(ne),
                                            dptr(ev), dptr(vecp), dptr(dg), dptr(sd)))
         k, m = ne.value, it.value
-        i32 = ctypes.c_int()
+        i32, nrun = ctypes.c_int(), ctypes.c_int()
         check(lib().ed_lanczos_info(self.handle, ctypes.byref(i32)))
+        check(lib().ed_lanczos_steps_run(self.handle, ctypes.byref(nrun)))
         return LanczosResult(m, ev[:k].copy(), vecp[:k * m].reshape(k, m).copy(), dg[:m].copy(), sd[:m].copy(),
-                             "f64" if o.precision == ED_PREC_F64 else "mixed", i32.value)
+                             "f64" if o.precision == ED_PREC_F64 else "mixed", i32.value, nrun.value)
 
     def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
         """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""
@@ -294,6 +296,18 @@ def tridiag_eig(diag, subdiag):
     w = np.zeros(n); z = np.zeros(n * n)
     check(lib().ed_tridiag_eig(n, dptr(d), dptr(b), dptr(w), dptr(z)))
     return w, z.reshape(n, n)
+
+
+def tridiag_lowest(diag, subdiag, k: int, prev=None) -> np.ndarray:
+    """The k lowest eigenvalues of T (the Lanczos convergence test's eigenvalues); prev:
+    those of T with its last row and column removed (interlacing bounds), or None."""
+    d = np.ascontiguousarray(diag, np.float64); b = np.ascontiguousarray(subdiag, np.float64)
+    w = np.zeros(k)
+    pp = None if prev is None else np.ascontiguousarray(prev, np.float64)
+    if pp is not None and pp.shape != (k,):
+        raise ValueError("prev must hold k values")
+    check(lib().ed_tridiag_lowest(len(d), dptr(d), dptr(b), int(k), None if pp is None else dptr(pp), dptr(w)))
+    return w
 
 
 def band_eig(a: np.ndarray, b: int, nev: int, vectors: bool = True):

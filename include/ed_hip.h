@@ -220,6 +220,13 @@ int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const do
  * on this sector (0 after an FP64 call).  No reference counterpart. */
 int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase);
 
+/* Lanczos steps the device ran in the last ed_lanczos_diagonalize on this sector (both
+ * phases of the mixed form).  Where the persistent kernel applies, one launch runs while
+ * the host tests convergence on the scalars it publishes and is stopped a few steps
+ * after the iteration that ends the loop, so this can exceed the iterations returned;
+ * those extra steps change none of the outputs.  No reference counterpart. */
+int ed_lanczos_steps_run(ed_sector *sec, int *steps);
+
 /* ED_diago (ED_ARPACK.f90:590-687), the path of sectors of dimension <= FULL_ED_SWITCH
  * (solver.h:156-180): dense H from H e_i on the device, all eigenpairs on the host
  * (cyclic Jacobi), ascending.  evals[neig]; evecs[neig][dim] (may be NULL).  dim <= 1024. */
@@ -249,6 +256,14 @@ int ed_rayleigh_f64(ed_sector *sec, int64_t n, const double *psi, double *energy
  * component positive).  The DSTEDC call of diagonalize_tridiag
  * (matrices_tridiag.f90:210-259), host FP64. */
 int ed_tridiag_eig(int n, const double *diag, const double *subdiag, double *w, double *z);
+
+/* The k lowest eigenvalues of T_n (diag[n], subdiag[1..n-1]), ascending, as the
+ * Lanczos convergence test computes them every step: the DSTEDC 'N' call of
+ * Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134, matrices_tridiag.f90:210-259).
+ * prev[k] (may be NULL) holds those of T_{n-1} (1e300 where unknown); by interlacing they
+ * bound the new ones from above and seed Newton's method for the lowest.  Each value to
+ * about 2 eps |lambda|; entries k >= n are 1e300. */
+int ed_tridiag_lowest(int n, const double *diag, const double *subdiag, int k, const double *prev, double *w);
 
 /* nev lowest eigenpairs of a symmetric band matrix a (n x n, row-major, both
  * triangles, half-bandwidth b): w[nev] ascending, z (n x nev, column-major, may be

@@ -73,6 +73,70 @@ def test_tridiag_eig_vs_scipy():
         assert np.max(np.abs(z @ z.T - np.eye(n))) < 1e-12
 
 
+def _lanczos_T(H, n, seed=1):
+    """alpha, beta (beta[0] = ||v1||) of n plain Lanczos steps on a dense symmetric H."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(H.shape[0])
+    a, b = np.zeros(n), np.zeros(n)
+    b[0] = np.linalg.norm(v)
+    v /= b[0]
+    vm = np.zeros_like(v)
+    for j in range(n):
+        w = H @ v - (b[j] * vm if j else 0.0)
+        a[j] = w @ v
+        w -= a[j] * v
+        if j + 1 < n:
+            b[j + 1] = np.linalg.norm(w)
+            vm, v = v, w / b[j + 1]
+    return a, b
+
+
+def test_tridiag_lowest_warm_sequence_vs_scipy():
+    """The convergence test's eigenvalues: every T_n of a Lanczos run (n = 1..90, Ritz
+    values converging and then flat to machine precision) from the values of T_{n-1},
+    against scipy's; also with no previous values and with wrong ones (too low, too
+    high), which only cost speed."""
+    import scipy.linalg as la
+    import ed_solver_amd as ed
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((300, 300)))
+    lam = np.concatenate([[-14.7, -14.2, -14.2], np.linspace(-13, 9, 297)])
+    H = (Q * lam) @ Q.T
+    a, b = _lanczos_T(H, 90)
+    for k in (1, 3):
+        prev = np.full(k, 1e300)
+        for n in range(1, 91):
+            w = ed.tridiag_lowest(a[:n], b[:n], k, prev)
+            ref = la.eigvalsh_tridiagonal(a[:n], b[1:n]) if n > 1 else a[:1]
+            m = min(k, n)
+            scale = max(1.0, np.abs(ref).max())
+            assert np.max(np.abs(w[:m] - ref[:m])) < 16 * np.finfo(float).eps * scale * 10, (k, n)
+            assert np.all(w[m:] >= 1e299)
+            assert np.array_equal(w, ed.tridiag_lowest(a[:n], b[:n], k, prev))   # deterministic
+            cold = ed.tridiag_lowest(a[:n], b[:n], k, None)
+            assert np.max(np.abs(w[:m] - cold[:m])) < 16 * np.finfo(float).eps * scale * 10
+            if n > k:
+                for bad in (ref[:k] - 1.0, ref[:k] + 5.0, np.full(k, -1e6)):
+                    wb = ed.tridiag_lowest(a[:n], b[:n], k, bad)
+                    assert np.max(np.abs(wb - ref[:k])) < 16 * np.finfo(float).eps * scale * 10, (k, n, bad)
+            prev = np.where(np.arange(k) < m, w, 1e300)
+
+
+def test_tridiag_lowest_degenerate_and_split():
+    """Exactly degenerate lowest eigenvalues (zero off-diagonals split T into blocks, a
+    repeated level) and a zero pivot on the way (d_0 - x = 0 at the start)."""
+    import ed_solver_amd as ed
+    d = np.array([1.0, 1.0, 3.0, 1.0, 2.0])
+    b = np.array([9.0, 0.0, 0.5, 0.0, 0.0])
+    import scipy.linalg as la
+    ref = la.eigvalsh_tridiagonal(d, b[1:])
+    for prev in (None, np.array([1.0, 1.0, 1.2]), np.array([1.5, 1.5, 1.5])):
+        w = ed.tridiag_lowest(d, b, 3, prev)
+        assert np.max(np.abs(w - ref[:3])) < 1e-14, (prev, w, ref)
+    w = ed.tridiag_lowest(np.array([0.0, 0.0]), np.array([1.0, 1.0]), 1, np.array([0.0]))
+    assert abs(w[0] + 1.0) < 1e-15
+
+
 def test_band_eig_vs_lapack():
     """The host Ritz step of block Lanczos: band reduction + QL + inverse iteration against
     numpy.linalg.eigh, on random band matrices and on a matrix with exactly degenerate

@@ -99,3 +99,53 @@ def test_persistent_zero_start_vector_is_an_error(gpu):
     a0, b0 = s.lanczos_run(40)
     assert np.array_equal(a1, a0) and np.array_equal(b1, b0)
     s.close()
+
+
+@pytest.mark.parametrize("name,sched", [("B", 16), ("C", 20)])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_monitored_diagonalize_matches_chunked(gpu, name, sched, precision):
+    """The ground-state solve with the convergence test: one persistent launch watched
+    from the host (stopped a few steps after the iteration that ends the loop) against
+    chunks of 8 two-kernel steps with the test between them.  Same scalars, so the same
+    iteration count, eigenvalues, Lanczos coefficients and ground-state vector; the mixed
+    form runs both of its phases this way."""
+    m, (nu, nd) = golden(name)
+    s = gpu.Sector(m, nu, nd)
+    s.set_schedule(sched)
+    assert s.persistent_active(precision)
+    print(f"{name} schedule {sched} {precision}: dim {s.dim}", flush=True)
+    s.set_persistent(True)
+    r1 = s.lanczos(nitermax=128, neigen=2, tolerance=1e-12, precision=precision)
+    psi1 = s.ground_state(r1)
+    s.set_persistent(False)
+    r0 = s.lanczos(nitermax=128, neigen=2, tolerance=1e-12, precision=precision)
+    psi0 = s.ground_state(r0)
+    assert r1.iters == r0.iters and r1.iters_f32 == r0.iters_f32
+    assert np.array_equal(r1.diag, r0.diag) and np.array_equal(r1.subdiag, r0.subdiag)
+    assert np.array_equal(r1.evals, r0.evals) and np.array_equal(r1.vecp, r0.vecp)
+    assert np.max(np.abs(psi1 - psi0)) < 1e-13
+    # the chunked run stops at the end of the chunk holding the last iteration; the
+    # monitored one within a few steps of it
+    assert r1.steps_run >= r1.iters + r1.iters_f32
+    assert r1.steps_run <= 128 * (2 if precision == "f32" else 1)
+    print(f"  iters {r1.iters} (+{r1.iters_f32} FP32), steps run {r1.steps_run} vs chunked {r0.steps_run}", flush=True)
+    s.close()
+
+
+def test_monitored_vs_oracle_and_short_limit(gpu):
+    """Monitored solve on workload B against the oracle's Lanczos (E0, iterations), and a
+    limit of 20 iterations that ends the loop at it == nitermax."""
+    m, (nu, nd) = golden("B")
+    s = gpu.Sector(m, nu, nd)
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    s.set_schedule(16)
+    s.set_persistent(True)
+    assert s.persistent_active("f64")
+    r = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+    ro = o.lanczos(nitermax=128, neigen=1, tol=1e-12)
+    assert abs(r.iters - ro["iters"]) <= 1   # the test compares Ritz values 1e-12 apart
+    assert abs(r.evals[0] - ro["evals"][0]) < 1e-10
+    r20 = s.lanczos(nitermax=20, neigen=1, tolerance=1e-12)
+    assert r20.iters == 20 and r20.steps_run == 20
+    assert np.max(np.abs(r20.diag - ro["alpha"][:20])) < 1e-10
+    s.close()

@@ -59,6 +59,10 @@ SIGNATURES = {
     "ed_sector_energy_shift": (ctypes.c_int, [_vp, _dp]),
     "ed_sector_quart_sz": (ctypes.c_int, [_vp, _dp]),
     "ed_sector_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ed_sector_create_c": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ed_sector_tables_im": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "ed_hmult_c128": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp]),
+    "ed_hmult_c128_dev": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
     "ed_sector_destroy": (ctypes.c_int, [_vp]),
     "ed_sector_dim": (ctypes.c_int64, [_vp]),
     "ed_sector_spin_dims": (ctypes.c_int, [_vp, _i32p, _i32p]),

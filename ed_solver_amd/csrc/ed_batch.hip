@@ -133,7 +133,7 @@ static int ws_reserve(size_t bytes, int device)
     return ED_OK;
 }
 
-static bool rows_capable(const ed_sector *s) { return !s->sz && s->up.d >= 128 && s->up.kpad <= 32; }
+static bool rows_capable(const ed_sector *s) { return !s->sz && !s->cplx && s->up.d >= 128 && s->up.kpad <= 32; }
 
 template <typename T>
 static HvArgs<T> hv_args_of(const ed_sector *s)
@@ -289,7 +289,7 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
     std::vector<int> batch;
     for (int k = 0; k < njobs; ++k) {
         if (nrm2[k] < cutoff_dynamic) continue;   // dyn = 0 (green_class_compute_dynamic.f90:263-265)
-        const int niter = (int)std::min<int64_t>(nitergreen, secs[k]->dim);
+        const int niter = (int)std::min<int64_t>(nitergreen, secs[k]->cplx ? secs[k]->cdim : secs[k]->dim);
         if (rows_capable(secs[k]) && niter == nitergreen) {
             batch.push_back(k);
         } else {

@@ -543,6 +543,9 @@ int ed_block_default_opts(ed_block_opts *o)
 static int block_check(ed_sector *sec, const ed_block_opts *o)
 {
     if (!sec || !o) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: NULL argument");
+    // the real embedding of a complex H doubles every eigenvalue: a block method would
+    // return each state twice (psi and i psi)
+    if (sec->cplx) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: real sectors only");
     if (o->block_size < 1 || o->block_size > 16) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: block_size must be 1..16");
     if (o->nev < 1 || o->nev > 16 || o->nev > sec->dim)
         return ed::fail(ED_ERR_ARG, "ed_block_lanczos: nev must be 1..16 and at most the sector dimension");

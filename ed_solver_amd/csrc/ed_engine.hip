@@ -123,6 +123,35 @@ __global__ __launch_bounds__(kBlock) void k_hv_generic(HvArgs<T> a, const T *__r
     if (MODE == 1) block_partial(dot, red, lz.partA + blockIdx.x);
 }
 
+// Interleaved complex (re, im pairs: Fortran COMPLEX(8)) <-> planar (Re[n], Im[n]).
+__global__ void k_planar(const double *in, double *out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double re = in[2 * i], im = in[2 * i + 1];
+        out[i] = re;
+        out[n + i] = im;
+    }
+}
+
+__global__ void k_interleaved(const double *in, double *out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double re = in[i], im = in[n + i];
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+}
+
+// Complex sectors: HAIMupdo_multc, one thread per amplitude (FP64).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_cplx(CplxArgs a, const double *__restrict__ xin,
+                                                     double *__restrict__ yout, int64_t ld, LzArgs<double> lz)
+{
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_cplx_body<MODE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
+}
+
 // S_z sectors: HAIMsz_multr, one thread per row.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_hv_sz(SzArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
@@ -332,12 +361,27 @@ static HvArgs<T> hv_args(const ed_sector *s, int cpb_rows)
     return a;
 }
 
+static CplxArgs cplx_args(const ed_sector *s)
+{
+    CplxArgs a;
+    a.dup = s->up.d; a.ddn = s->dn.d; a.ncfg = s->ncfg; a.kup = s->up.kpad; a.kdn = s->dn.kpad;
+    a.hund = s->hund;
+    a.D = s->cdim;
+    a.rk_up = s->up.ell_rk; a.rk_dn = s->dn.ell_rk;
+    a.vr_up = s->up.ell_val; a.vi_up = s->up.ell_vim; a.vr_dn = s->dn.ell_val; a.vi_dn = s->dn.ell_vim;
+    a.diag_up = s->up.diag; a.diag_dn = s->dn.diag; a.uimp = s->d_uimp;
+    a.imp_up = s->up.imp; a.imp_dn = s->dn.imp;
+    a.hund_ptr = s->d_hund_ptr; a.hund_tr = s->d_hund_tr; a.hund_val = s->d_hund_val;
+    a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    return a;
+}
+
 HvArgs<double> hv_args_f64(const ed_sector *s) { return hv_args<double>(s, 1); }
 HvArgs<float> hv_args_f32(const ed_sector *s) { return hv_args<float>(s, 1); }
 
 // Launch shape of one H.v: which kernel, columns per workgroup, threads, grid.
 struct HvShape {
-    int kind;    // 0 generic, 1 row kernel, 2 fast kernel
+    int kind;    // 0 generic, 1 row kernel, 2 fast kernel, 3 S_z sector, 4 complex sector
     int cpb;
     int nblk;
     int nt;      // threads per workgroup
@@ -357,7 +401,7 @@ static int width_of(int kmax)
 
 static bool fast_ok(const ed_sector *s)
 {
-    if (s->sz) return false;
+    if (s->sz || s->cplx) return false;
     const int k = width_of(std::max(s->up.kpad, s->dn.kpad));
     // widths above 16 exceed the 128-VGPR budget of a 1024-thread workgroup in FP64
     return s->up.ncoef > 0 && k > 0 && k <= 16 && s->up.d >= 64 && s->up.d <= 65535 && s->dn.d <= 8184 &&
@@ -380,10 +424,17 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     h.k = s->up.kpad;
     h.xcd = 0;
     h.rows = 0;
-    if (s->sz) {   // S_z sector: one row per thread
+    if (s->cplx) {   // complex sector: one amplitude per thread, grid-stride
+        h.kind = 4;
+        h.cpb = 1;
+        h.nblk = (int)std::min<int64_t>(kMaxRowBlocks, (s->cdim + kBlock - 1) / kBlock);
+        h.lds = 0;
+        return h;
+    }
+    if (s->sz) {   // S_z sector: one row per thread, grid-stride beyond kMaxRowBlocks workgroups
         h.kind = 3;
         h.cpb = 1;
-        h.nblk = (int)((s->dim + kBlock - 1) / kBlock);
+        h.nblk = (int)std::min<int64_t>(kMaxRowBlocks, (s->dim + kBlock - 1) / kBlock);
         h.lds = 0;
         return h;
     }
@@ -456,6 +507,12 @@ template <typename T, int MODE>
 static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *y, int64_t ld, int nvec,
                            const LzArgs<T> &lz, hipStream_t st)
 {
+    if (h.kind == 4) {   // complex sectors run FP64 only (check_common)
+        if constexpr (std::is_same<T, double>::value)
+            hipLaunchKernelGGL((k_hv_cplx<MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, cplx_args(s), x,
+                               y, ld, lz);
+        return;
+    }
     if (h.kind == 3) {
         const bool f64 = std::is_same<T, double>::value;
         SzArgs<T> za;
@@ -836,6 +893,7 @@ static int check_common(ed_sector *s, int prec)
 {
     if (!s) return fail(ED_ERR_ARG, "NULL sector");
     if (prec != ED_PREC_F64 && prec != ED_PREC_F32) return fail(ED_ERR_ARG, "precision must be ED_PREC_F64 or ED_PREC_F32");
+    if (s->cplx && prec == ED_PREC_F32) return fail(ED_ERR_ARG, "complex sectors run in FP64 only");
     ED_HIP_CHECK(hipSetDevice(s->device));
     return ED_OK;
 }
@@ -936,7 +994,7 @@ static int tune(ed_sector *s)
     hipStream_t st = s->stream;
     for (double &t : s->tune_ms) t = 0.0;
     for (double &t : s->tune_ms32) t = 0.0;
-    if (s->sz || s->dim < 4096) {
+    if (s->sz || s->cplx || s->dim < 4096) {
         s->cpb = fast_ok(s) ? 8 : 1;
         s->cpb32 = 0;
         return ED_OK;
@@ -1114,7 +1172,7 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
 {
     ED_TRY(check_common(s, o ? o->precision : 0));
     if (!o || !iters_out) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: NULL argument");
-    const int64_t D = s->dim;
+    const int64_t D = s->cplx ? s->cdim : s->dim;   // Niter = min(dimen_H - 1, Nitermax)
     const int niter = (int)std::min<int64_t>(D - 1, o->nitermax);
     if (niter < 1) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: sector dimension 1 has no Lanczos step (full ED case)");
     if (niter > kMaxIter - 2) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: nitermax above " + std::to_string(kMaxIter - 2));
@@ -1259,6 +1317,39 @@ int ed_sector_synchronize(ed_sector *sec)
     return ED_OK;
 }
 
+int ed_hmult_c128_dev(ed_sector *sec, int64_t n, double *d_out, const double *d_in, void *stream)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!sec->cplx) return fail(ED_ERR_ARG, "ed_hmult_c128: not a complex sector (ed_sector_create_c)");
+    if (n != sec->cdim || !d_out || !d_in) return fail(ED_ERR_ARG, "ed_hmult_c128: n differs from dimen_H()");
+    ED_TRY(ensure_io(sec));
+    hipStream_t st = stream ? (hipStream_t)stream : sec->stream;
+    const int g = elem_grid(n);
+    hipLaunchKernelGGL(k_planar, dim3(g), dim3(kBlock), 0, st, d_in, sec->d_io_a, n);
+    ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
+    hipLaunchKernelGGL(k_interleaved, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_b, d_out, n);
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
+int ed_hmult_c128(ed_sector *sec, int64_t n, double *vec_out, const double *vec_in)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!sec->cplx) return fail(ED_ERR_ARG, "ed_hmult_c128: not a complex sector (ed_sector_create_c)");
+    if (n != sec->cdim || !vec_out || !vec_in) return fail(ED_ERR_ARG, "ed_hmult_c128: n differs from dimen_H()");
+    ED_TRY(ensure_io(sec));   // two staging vectors of 2 n doubles
+    hipStream_t st = sec->stream;
+    const int g = elem_grid(n);
+    ED_HIP_CHECK(hipMemcpyAsync(sec->d_io_b, vec_in, sizeof(double) * 2 * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_planar, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_b, sec->d_io_a, n);
+    ED_TRY(launch_hv(sec, ED_PREC_F64, sec->d_io_a, sec->d_io_b, 0, 1, st));
+    hipLaunchKernelGGL(k_interleaved, dim3(g), dim3(kBlock), 0, st, (const double *)sec->d_io_b, sec->d_io_a, n);
+    ED_HIP_CHECK(hipGetLastError());
+    ED_HIP_CHECK(hipMemcpyAsync(vec_out, sec->d_io_a, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
 int ed_hmult_f64(ed_sector *sec, int64_t n, double *vec_out, const double *vec_in)
 {
     ED_TRY(check_common(sec, 0));
@@ -1285,7 +1376,7 @@ int ed_hmult_tune(ed_sector *sec, int *cpb, double *ms_per_choice)
 int ed_hmult_set_schedule(ed_sector *sec, int choice)
 {
     ED_TRY(check_common(sec, 0));
-    if (sec->sz) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: an S_z sector has one H.v schedule");
+    if (sec->sz || sec->cplx) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: S_z and complex sectors have one H.v schedule");
     if (!sched_valid(choice))
         return fail(ED_ERR_ARG, "ed_hmult_set_schedule: choice must be 1, 2, 4, 8, 16, 18, 19 or 20");
     if (choice >= 8 && !fast_ok(sec)) return fail(ED_ERR_ARG, "ed_hmult_set_schedule: fast kernel does not apply here");
@@ -1541,6 +1632,7 @@ extern "C" int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs
 {
     using namespace ed;
     ED_TRY(check_common(sec, 0));
+    if (sec->cplx) return fail(ED_ERR_ARG, "ed_full_ed: real sectors only (a complex H needs a Hermitian solver)");
     const int64_t n = sec->dim;
     if (n < 1 || n > 1024) return fail(ED_ERR_ARG, "ed_full_ed: sector dimension must be in [1, 1024]");
     if (neig < 1 || neig > n || !evals) return fail(ED_ERR_ARG, "ed_full_ed: neig outside [1, dim] or NULL evals");

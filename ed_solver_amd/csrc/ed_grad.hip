@@ -413,7 +413,7 @@ extern "C" {
 int ed_hf_gradient_dev(ed_sector *sec, const double *d_psi, double *d_level, double *d_v, double *d_u)
 {
     if (!sec || !d_psi) return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: NULL argument");
-    if (sec->sz) return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: (N_up, N_dn) sectors only");
+    if (sec->sz || sec->cplx) return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: real (N_up, N_dn) sectors only");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     return ed::hf_gradient(sec, d_psi, d_level, d_v, d_u);
 }
@@ -421,7 +421,7 @@ int ed_hf_gradient_dev(ed_sector *sec, const double *d_psi, double *d_level, dou
 int ed_hf_gradient(ed_sector *sec, int64_t n, const double *psi, double *d_level, double *d_v, double *d_u)
 {
     if (!sec || !psi) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: NULL argument");
-    if (sec->sz) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: (N_up, N_dn) sectors only");
+    if (sec->sz || sec->cplx) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: real (N_up, N_dn) sectors only");
     if (n != sec->dim) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: vector length differs from the sector dimension");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     double *d = nullptr;

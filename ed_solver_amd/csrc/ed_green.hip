@@ -115,6 +115,7 @@ static int check_pair(const ed_sector *src, const ed_sector *dst, int site, int 
 {
     if (!src || !dst) return fail(ED_ERR_ARG, "ed_apply_c: NULL sector");
     if (src->sz || dst->sz) return fail(ED_ERR_ARG, "ed_apply_c: (N_up, N_dn) sectors only (apply_Cupdo)");
+    if (src->cplx != dst->cplx) return fail(ED_ERR_ARG, "ed_apply_c: one complex and one real sector");
     if (src->nc != dst->nc || src->nb != dst->nb) return fail(ED_ERR_ARG, "ed_apply_c: sectors of different models");
     if (site < 0 || site >= src->nc) return fail(ED_ERR_ARG, "ed_apply_c: site outside [0, nc)");
     if ((spin != 0 && spin != 1) || (pm != 1 && pm != -1)) return fail(ED_ERR_ARG, "ed_apply_c: spin 0/1, pm +1/-1");
@@ -133,8 +134,12 @@ int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, 
     ED_HIP_CHECK(hipMallocAsync((void **)&d_map, sizeof(int32_t) * 2 * std::max(1, T.d), st));
     hipLaunchKernelGGL(k_cmap, dim3((T.d + kBlock - 1) / kBlock), dim3(kBlock), 0, st, T.states, T.d,
                        src->nb + site, pm, src->d_rank, d_map, d_map + T.d);
-    hipLaunchKernelGGL(k_apply_c, dim3(grid_for(dst->dim)), dim3(kBlock), 0, st, d_psi, src->up.d, d_phi, dst->up.d,
-                       dst->dim, spin, (const int32_t *)d_map, (const int32_t *)(d_map + T.d));
+    // a complex sector's planar vectors: the same real map on the Re and the Im plane
+    const int planes = dst->cplx ? 2 : 1;
+    const int64_t ns = dst->cplx ? src->cdim : src->dim, nd = dst->cplx ? dst->cdim : dst->dim;
+    for (int p = 0; p < planes; ++p)
+        hipLaunchKernelGGL(k_apply_c, dim3(grid_for(nd)), dim3(kBlock), 0, st, d_psi + p * ns, src->up.d,
+                           d_phi + p * nd, dst->up.d, nd, spin, (const int32_t *)d_map, (const int32_t *)(d_map + T.d));
     ED_HIP_CHECK(hipGetLastError());
     ED_HIP_CHECK(hipFreeAsync(d_map, st));
     return ED_OK;
@@ -164,7 +169,7 @@ static int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_p
 {
     if (iph != 1 && iph != 2) return fail(ED_ERR_ARG, "ed_dynamic: iph must be 1 (particle) or 2 (hole)");
     if (nitergreen < 1) return fail(ED_ERR_ARG, "ed_dynamic: nitergreen < 1");
-    const int niter = (int)std::min<int64_t>(nitergreen, sec->dim);
+    const int niter = (int)std::min<int64_t>(nitergreen, sec->cplx ? sec->cdim : sec->dim);
     if (nrm2 < cutoff_dynamic) {   // green_class_compute_dynamic.f90:263-265
         std::fill(dyn, dyn + 2 * (size_t)nw, 0.0);
         if (a_out) std::fill(a_out, a_out + niter, 0.0);

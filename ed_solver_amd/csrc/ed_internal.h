@@ -39,6 +39,7 @@ constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_so
 // Layout of the per-sector scratch array of doubles (offsets in doubles).
 constexpr int kPartA = 0;                          // partial sums written by the H.v phase
 constexpr int kPartB = kPartA + 8192;              // update phase, ||u_j||^2, [parity of j][8192]
+constexpr int kMaxRowBlocks = 8192;                // workgroups of a row-per-thread H.v (one kPartA partial each)
 constexpr int kPartC = kPartB + 2 * 8192;          // update phase, <u_j, u_{j-1}>, [parity of j][8192]
 constexpr int kPartR = kPartC + 2 * 8192;          // reorthogonalisation dots [k][workgroup]
 constexpr int kReorthGrid = 256;                   // workgroups of the reorthogonalisation dots
@@ -86,6 +87,8 @@ struct SpinTables {
     int32_t *swap = nullptr;     // [d][ncfg] rank of the state with its impurity bits replaced
     // FP32 copies for the mixed-precision kernels
     float *diag32 = nullptr, *off32 = nullptr, *ell_val32 = nullptr;
+    // complex sector: imaginary parts of off (CSR) and ell_val (ELL)
+    double *off_im = nullptr, *ell_vim = nullptr;
     // packed ELL: entry = target rank (low 16 bits) | coefficient index (high 16 bits)
     // into the table of distinct coefficients coef[ncoef] (coef[0] = 0 pads short rows)
     uint32_t *ell_pk = nullptr;  // [max(kpad of both spins)][d]
@@ -194,6 +197,11 @@ struct ed_sector {
     // diagonal QUART_INT_SZ in d_quart, and the constant tr(Ec_dn) + tr(Eb_dn) that the
     // particle-hole transformation moves out of H
     int sz = 0;
+    // complex sector (Hmultc): Ec, Eb, Vbc with imaginary parts; vectors are planar,
+    // (Re[cdim], Im[cdim]), so dim = 2 cdim is the length of every real-typed vector
+    int cplx = 0;
+    int64_t cdim = 0;
+    std::vector<double> ec_im, eb_im, vbc_im;
     double *d_quart = nullptr;
     float *d_quart32 = nullptr;
     double energy_shift = 0.0;

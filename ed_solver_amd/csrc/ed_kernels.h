@@ -554,15 +554,94 @@ __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin,
         x = xin;
         y = yout;
     }
-    const int64_t i = (int64_t)blk * blockDim.x + threadIdx.x, d = a.d;
+    // grid-stride over the rows: the grid stays within the kPartA partials of a Lanczos step
+    const int64_t d = a.d, stride = (int64_t)gridDim.x * blockDim.x;
     double dot = 0.0;
-    if (i < d) {
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < d; i += stride) {
         const T xv = x[i];
         T acc = a.quart[i] * xv;
         acc += a.diag[i] * xv;
         for (int k = 0; k < a.kw; ++k) acc += a.val[k * d + i] * x[a.rk[k * d + i]];
         y[i] = acc;
-        dot = (double)acc * (double)xv;
+        dot += (double)acc * (double)xv;
+    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+}
+
+// ------------------------------------------------------------- complex sectors
+
+// Tables of a complex (N_up, N_dn) sector: the ELL of both spins with the real and
+// imaginary parts of <target|H|state>, the diagonals, the interaction per impurity pair
+// and the Hund transitions (real).
+struct CplxArgs {
+    int dup, ddn, ncfg, kup, kdn, hund;
+    int64_t D;                        // complex dimension; vectors are (Re[D], Im[D])
+    const int32_t *rk_up, *rk_dn;     // ELL [k][d]
+    const double *vr_up, *vi_up, *vr_dn, *vi_dn;
+    const double *diag_up, *diag_dn, *uimp;
+    const int32_t *imp_up, *imp_dn;
+    const int32_t *hund_ptr, *hund_tr;
+    const double *hund_val;
+    const int32_t *swap_up, *swap_dn;
+};
+
+// HAIMupdo_multc (HAIMupdo.f90:150-656, USE_CC gather form) on planar complex vectors,
+// one thread per amplitude (grid-stride beyond the grid), i = iup + ido d_up:
+//   w_i = (U terms + diag_up + diag_dn) v_i + sum_up conj(h) v(jup, ido)
+//         + sum_dn conj(h) v(iup, jdo) + sum_Hund J v(target)
+// with conj(h) v = (hr vr + hi vi) + i (hr vi - hi vr).  MODE 1: Lanczos phase 1 on the
+// Krylov slots with the partial Re <u_j, z> = sum zr ur + zi ui per workgroup (the real
+// embedding: Lanczos on (Re, Im) with the symmetric [[A, -B], [B, A]] of H = A + iB has
+// the complex Lanczos' alpha and beta, which are real).
+template <int MODE>
+__device__ inline void hv_cplx_body(const CplxArgs &a, const double *__restrict__ xin, double *__restrict__ yout,
+                                    const LzArgs<double> &lz, int blk, double *red)
+{
+    const double *__restrict__ x;
+    double *__restrict__ y;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int64_t D = a.D, stride = (int64_t)gridDim.x * blockDim.x;
+    const int dup = a.dup, ddn = a.ddn, ncfg = a.ncfg;
+    double dot = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < D; i += stride) {
+        const int iup = (int)(i % dup), ido = (int)(i / dup);
+        const int64_t cb = (int64_t)ido * dup;
+        const double xr = x[i], xi = x[D + i];
+        const int cu = a.imp_up[iup], cd = a.imp_dn[ido];
+        const double dg = a.uimp[cu * ncfg + cd] + (a.diag_up[iup] + a.diag_dn[ido]);
+        double yr = dg * xr, yi = dg * xi;
+        for (int k = 0; k < a.kup; ++k) {
+            const int64_t e = (int64_t)k * dup + iup, t = a.rk_up[e] + cb;
+            const double hr = a.vr_up[e], hi = a.vi_up[e], vr = x[t], vi = x[D + t];
+            yr += hr * vr + hi * vi;
+            yi += hr * vi - hi * vr;
+        }
+        for (int k = 0; k < a.kdn; ++k) {
+            const int64_t e = (int64_t)k * ddn + ido, t = iup + (int64_t)a.rk_dn[e] * dup;
+            const double hr = a.vr_dn[e], hi = a.vi_dn[e], vr = x[t], vi = x[D + t];
+            yr += hr * vr + hi * vi;
+            yi += hr * vi - hi * vr;
+        }
+        if (a.hund) {
+            const int h0 = a.hund_ptr[cu * ncfg + cd], h1 = a.hund_ptr[cu * ncfg + cd + 1];
+            for (int t = h0; t < h1; ++t) {
+                const int tr = a.hund_tr[t];
+                const int64_t g = a.swap_up[iup * ncfg + (tr & 0xFF)] + (int64_t)a.swap_dn[ido * ncfg + (tr >> 8)] * dup;
+                yr += a.hund_val[t] * x[g];
+                yi += a.hund_val[t] * x[D + g];
+            }
+        }
+        y[i] = yr;
+        y[D + i] = yi;
+        dot += yr * xr + yi * xi;
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }

@@ -88,6 +88,7 @@ struct SpinModel {
     int nimp, nbath;
     int ibit[kMaxImpOrb], bbit[kMaxBathOrb];
     const double *ec, *eb, *vbc;
+    const double *ec_im, *eb_im, *vbc_im;   // imaginary parts of a complex model (null: real)
     double cut;
 };
 
@@ -101,17 +102,30 @@ static SpinModel spin_model(int nc, int nb, const double *ec, const double *eb, 
     m.ec = ec;
     m.eb = eb;
     m.vbc = vbc;
+    m.ec_im = m.eb_im = m.vbc_im = nullptr;
     m.cut = cut;
     return m;
 }
 
+// The mask and cutoff of tab_HAIM2 for one coefficient: kept when nonzero and
+// |t| > cutoff (the modulus for a complex t).
+__device__ inline bool keep_coef(double re, double im, double cut)
+{
+    return !(re == 0.0 && im == 0.0) && hypot(re, im) > cut;
+}
+
+// tab_HAIM2 (HAIM_offdiag_arrays.f90:419-527) for one spin, one thread per state: the
+// diagonal and the entries (target rank, <target|H|state>) in the reference order.  For a
+// complex model off_im receives the imaginary parts: Ec(a,b) sign, Eb(a,b) sign,
+// Vbc(b,c) sign (impurity -> bath) and conj(Vbc(b,c)) sign (bath -> impurity).
 template <bool FILL>
 __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *rank,
-                      const int32_t *ptr, int32_t *noff, double *diag, int32_t *rk, double *off)
+                      const int32_t *ptr, int32_t *noff, double *diag, int32_t *rk, double *off, double *off_im)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
     const int nc = m.nimp, nb = m.nbath;
+    const bool cx = m.ec_im != nullptr;
     const uint32_t s = (uint32_t)states[i];
     int k = FILL ? ptr[i] : 0;
     const int k0 = k;
@@ -125,36 +139,32 @@ __global__ void k_tab(SpinModel m, const int32_t *states, int d, const int32_t *
             if ((s >> m.bbit[b]) & 1u) dg += m.eb[b * nb + b];
         diag[i] = dg;
     }
+    auto put = [&](double tr, double ti) {
+        if (FILL) {
+            rk[k] = rank[o];
+            off[k] = tr * sg;
+            if (cx) off_im[k] = ti * sg;
+        }
+        ++k;
+    };
     for (int a = 0; a < nc; ++a)
         for (int b = 0; b < nc; ++b) {
-            double t = m.ec[a * nc + b];
-            if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, m.ibit[a], m.ibit[b], &o, &sg)) {
-                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
-                ++k;
-            }
+            const double t = m.ec[a * nc + b], ti = cx ? m.ec_im[a * nc + b] : 0.0;
+            if (a == b || !(cx ? keep_coef(t, ti, m.cut) : (t != 0.0 && fabs(t) > m.cut))) continue;
+            if (hop(s, m.ibit[a], m.ibit[b], &o, &sg)) put(t, ti);
         }
     for (int a = 0; a < nb; ++a)
         for (int b = 0; b < nb; ++b) {
-            double t = m.eb[a * nb + b];
-            if (a == b || t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, m.bbit[a], m.bbit[b], &o, &sg)) {
-                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
-                ++k;
-            }
+            const double t = m.eb[a * nb + b], ti = cx ? m.eb_im[a * nb + b] : 0.0;
+            if (a == b || !(cx ? keep_coef(t, ti, m.cut) : (t != 0.0 && fabs(t) > m.cut))) continue;
+            if (hop(s, m.bbit[a], m.bbit[b], &o, &sg)) put(t, ti);
         }
     for (int b = 0; b < nb; ++b)
         for (int c = 0; c < nc; ++c) {
-            double t = m.vbc[b * nc + c];
-            if (t == 0.0 || !(fabs(t) > m.cut)) continue;
-            if (hop(s, m.bbit[b], m.ibit[c], &o, &sg)) {
-                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
-                ++k;
-            }
-            if (hop(s, m.ibit[c], m.bbit[b], &o, &sg)) {
-                if (FILL) { rk[k] = rank[o]; off[k] = t * sg; }
-                ++k;
-            }
+            const double t = m.vbc[b * nc + c], ti = cx ? m.vbc_im[b * nc + c] : 0.0;
+            if (!(cx ? keep_coef(t, ti, m.cut) : (t != 0.0 && fabs(t) > m.cut))) continue;
+            if (hop(s, m.bbit[b], m.ibit[c], &o, &sg)) put(t, ti);
+            if (hop(s, m.ibit[c], m.bbit[b], &o, &sg)) put(t, -ti);
         }
     if (!FILL) noff[i] = k - k0;
 }
@@ -183,7 +193,7 @@ __global__ void k_scan(const int32_t *noff, int d, int32_t *ptr, int32_t *kmax)
 }
 
 __global__ void k_ell(int d, int kmax, const int32_t *ptr, const int32_t *rk, const double *off,
-                      int32_t *ell_rk, double *ell_val)
+                      int32_t *ell_rk, double *ell_val, const double *off_im, double *ell_vim)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
@@ -191,6 +201,7 @@ __global__ void k_ell(int d, int kmax, const int32_t *ptr, const int32_t *rk, co
     for (int k = 0; k < kmax; ++k) {
         ell_rk[(int64_t)k * d + i] = k < n ? rk[p + k] : i;
         ell_val[(int64_t)k * d + i] = k < n ? off[p + k] : 0.0;
+        if (off_im) ell_vim[(int64_t)k * d + i] = k < n ? off_im[p + k] : 0.0;
     }
 }
 
@@ -229,7 +240,7 @@ static int dalloc(T **p, size_t n)
 static void free_spin(SpinTables &t)
 {
     void *ps[] = {t.states, t.diag, t.noff, t.ptr, t.rk, t.off, t.ell_rk, t.ell_val, t.imp, t.swap,
-                  t.diag32, t.off32, t.ell_val32, t.ell_pk, t.coef, t.coef32};
+                  t.diag32, t.off32, t.ell_val32, t.ell_pk, t.coef, t.coef32, t.off_im, t.ell_vim};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     t = SpinTables();
@@ -302,21 +313,33 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
     hipLaunchKernelGGL(k_enum_states, dim3(nblocks(d)), dim3(kBlock), 0, st, ns, n, d, T.states);
     ED_HIP_CHECK(hipGetLastError());
 
+    // model matrices of this spin on the device: re parts, then (complex model) im parts
+    const int nmat = S->cplx ? 2 : 1;
     double *d_ec = nullptr, *d_eb = nullptr, *d_vbc = nullptr;
-    ED_TRY(dalloc(&d_ec, (size_t)nc * nc));
-    ED_TRY(dalloc(&d_eb, (size_t)nb * nb));
-    ED_TRY(dalloc(&d_vbc, (size_t)nb * nc));
-    ED_HIP_CHECK(hipMemcpyAsync(d_ec, S->ec.data() + (size_t)spin * nc * nc, sizeof(double) * nc * nc,
-                                hipMemcpyHostToDevice, st));
-    if (nb) {
-        ED_HIP_CHECK(hipMemcpyAsync(d_eb, S->eb.data() + (size_t)spin * nb * nb, sizeof(double) * nb * nb,
-                                    hipMemcpyHostToDevice, st));
-        ED_HIP_CHECK(hipMemcpyAsync(d_vbc, S->vbc.data() + (size_t)spin * nb * nc, sizeof(double) * nb * nc,
-                                    hipMemcpyHostToDevice, st));
+    ED_TRY(dalloc(&d_ec, (size_t)nmat * nc * nc));
+    ED_TRY(dalloc(&d_eb, (size_t)nmat * nb * nb));
+    ED_TRY(dalloc(&d_vbc, (size_t)nmat * nb * nc));
+    for (int part = 0; part < nmat; ++part) {
+        const std::vector<double> &ec = part ? S->ec_im : S->ec, &eb = part ? S->eb_im : S->eb,
+                                  &vbc = part ? S->vbc_im : S->vbc;
+        ED_HIP_CHECK(hipMemcpyAsync(d_ec + (size_t)part * nc * nc, ec.data() + (size_t)spin * nc * nc,
+                                    sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+        if (nb) {
+            ED_HIP_CHECK(hipMemcpyAsync(d_eb + (size_t)part * nb * nb, eb.data() + (size_t)spin * nb * nb,
+                                        sizeof(double) * nb * nb, hipMemcpyHostToDevice, st));
+            ED_HIP_CHECK(hipMemcpyAsync(d_vbc + (size_t)part * nb * nc, vbc.data() + (size_t)spin * nb * nc,
+                                        sizeof(double) * nb * nc, hipMemcpyHostToDevice, st));
+        }
     }
-    const SpinModel m = spin_model(nc, nb, d_ec, d_eb, d_vbc, S->cutoff);
+    SpinModel m = spin_model(nc, nb, d_ec, d_eb, d_vbc, S->cutoff);
+    if (S->cplx) {
+        m.ec_im = d_ec + (size_t)nc * nc;
+        m.eb_im = d_eb + (size_t)nb * nb;
+        m.vbc_im = d_vbc + (size_t)nb * nc;
+    }
     hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
-                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr,
+                       (double *)nullptr);
     int32_t *d_kmax = nullptr;
     ED_TRY(dalloc(&d_kmax, 1));
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kBlock), 0, st, T.noff, d, T.ptr, d_kmax);
@@ -329,8 +352,9 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
     T.kmax = h[1];
     ED_TRY(dalloc(&T.rk, (size_t)T.nnz));
     ED_TRY(dalloc(&T.off, (size_t)T.nnz));
+    if (S->cplx) ED_TRY(dalloc(&T.off_im, (size_t)T.nnz));
     hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, m, T.states, d, S->d_rank,
-                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off);
+                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off, T.off_im);
     ED_HIP_CHECK(hipGetLastError());
     // ELL width: the next compiled width of the unrolled H.v kernel, or kmax itself
     // above the largest one (the generic kernel then loops over it)
@@ -341,8 +365,9 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
     const int kw = T.kpad;
     ED_TRY(dalloc(&T.ell_rk, (size_t)kw * d));
     ED_TRY(dalloc(&T.ell_val, (size_t)kw * d));
+    if (S->cplx) ED_TRY(dalloc(&T.ell_vim, (size_t)kw * d));
     hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk,
-                       T.ell_val);
+                       T.ell_val, (const double *)T.off_im, T.ell_vim);
     ED_TRY(dalloc(&T.imp, d));
     ED_TRY(dalloc(&T.swap, (size_t)d * S->ncfg));
     hipLaunchKernelGGL(k_imp_swap, dim3(nblocks((int64_t)d * S->ncfg)), dim3(kBlock), 0, st, T.states, d, nb,
@@ -481,7 +506,10 @@ static void free_sector(ed_sector *S)
     delete S;
 }
 
-static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
+// (N_up, N_dn) sector; with ec_im (and eb_im, vbc_im when nb > 0) a complex model, whose
+// tables carry complex entries and whose vectors are planar complex (dim = 2 D).
+static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, const double *ec_im = nullptr,
+                         const double *eb_im = nullptr, const double *vbc_im = nullptr)
 {
     if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create: NULL argument");
     *out = nullptr;
@@ -504,6 +532,14 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
     S->jj.assign(m->jj, m->jj + nc * nc);
     S->jhund = m->jhund;
     S->cutoff = m->cutoff;
+    if (ec_im) {
+        S->cplx = 1;
+        S->ec_im.assign(ec_im, ec_im + 2 * nc * nc);
+        S->eb_im.assign((size_t)2 * nb * nb, 0.0);
+        S->vbc_im.assign((size_t)2 * nb * nc, 0.0);
+        if (nb && eb_im) S->eb_im.assign(eb_im, eb_im + 2 * nb * nb);
+        if (nb && vbc_im) S->vbc_im.assign(vbc_im, vbc_im + 2 * nb * nc);
+    }
     int rc = ED_OK;
     auto bail = [&](int code) { free_sector(S); return code; };
     if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
@@ -512,13 +548,17 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out)
     hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
     if ((rc = build_spin(S, 0, nup, S->up)) != ED_OK) return bail(rc);
     if ((rc = build_spin(S, 1, ndn, S->dn)) != ED_OK) return bail(rc);
-    {
+    if (!S->cplx) {   // packed coefficients of the fast real kernels
         const int rows = std::max(S->up.kpad, S->dn.kpad);
         if ((rc = build_packed(S->up, rows, S->stream)) != ED_OK) return bail(rc);
         if ((rc = build_packed(S->dn, rows, S->stream)) != ED_OK) return bail(rc);
     }
     if ((rc = build_interaction(S)) != ED_OK) return bail(rc);
     S->dim = (int64_t)S->up.d * S->dn.d;
+    if (S->cplx) {
+        S->cdim = S->dim;
+        S->dim = 2 * S->cdim;
+    }
     if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
         return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
     if (hipStreamSynchronize(S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "sector build failed"));
@@ -680,7 +720,8 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     sm.vbc = d_vbc;
     sm.cut = m->cutoff;
     hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
-                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr);
+                       (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr,
+                       (double *)nullptr);
     int32_t *d_kmax = nullptr;
     if ((rc = dalloc(&d_kmax, 1)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kBlock), 0, st, T.noff, d, T.ptr, d_kmax);
@@ -695,11 +736,12 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     T.kpad = std::max(T.kmax, 1);
     if ((rc = dalloc(&T.rk, (size_t)T.nnz)) != ED_OK || (rc = dalloc(&T.off, (size_t)T.nnz)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
-                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off);
+                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off, (double *)nullptr);
     const int kw = T.kpad;
     if ((rc = dalloc(&T.ell_rk, (size_t)kw * d)) != ED_OK || (rc = dalloc(&T.ell_val, (size_t)kw * d)) != ED_OK)
         return bail(rc);
-    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk, T.ell_val);
+    hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk, T.ell_val,
+                       (const double *)nullptr, (double *)nullptr);
     if ((rc = dalloc(&S->d_quart, d)) != ED_OK || (rc = dalloc(&S->d_quart32, d)) != ED_OK ||
         (rc = dalloc(&T.diag32, d)) != ED_OK || (rc = dalloc(&T.off32, (size_t)T.nnz)) != ED_OK ||
         (rc = dalloc(&T.ell_val32, (size_t)kw * d)) != ED_OK)
@@ -747,6 +789,13 @@ int ed_sector_energy_shift(const ed_sector *sec, double *shift)
     return ED_OK;
 }
 
+int ed_sector_create_c(const ed_model *model, const double *ec_im, const double *eb_im, const double *vbc_im,
+                       int nup, int ndn, ed_sector **out)
+{
+    if (!ec_im) return ed::fail(ED_ERR_ARG, "ed_sector_create_c: NULL ec_im");
+    return ed::create_sector(model, nup, ndn, out, ec_im, eb_im, vbc_im);
+}
+
 int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out)
 {
     return ed::create_sector(model, nup, ndn, out);
@@ -760,7 +809,7 @@ int ed_sector_destroy(ed_sector *sec)
     return ED_OK;
 }
 
-int64_t ed_sector_dim(const ed_sector *sec) { return sec ? sec->dim : 0; }
+int64_t ed_sector_dim(const ed_sector *sec) { return sec ? (sec->cplx ? sec->cdim : sec->dim) : 0; }
 
 int ed_sector_spin_dims(const ed_sector *sec, int32_t *d_up, int32_t *d_dn)
 {
@@ -788,6 +837,15 @@ int ed_sector_tables(const ed_sector *sec, int spin, int32_t *states, double *di
     if (noff) ED_HIP_CHECK(hipMemcpy(noff, T.noff, sizeof(int32_t) * T.d, hipMemcpyDeviceToHost));
     if (rankoff && T.nnz) ED_HIP_CHECK(hipMemcpy(rankoff, T.rk, sizeof(int32_t) * T.nnz, hipMemcpyDeviceToHost));
     if (offdiag && T.nnz) ED_HIP_CHECK(hipMemcpy(offdiag, T.off, sizeof(double) * T.nnz, hipMemcpyDeviceToHost));
+    return ED_OK;
+}
+
+int ed_sector_tables_im(const ed_sector *sec, int spin, double *offdiag_im)
+{
+    if (!sec || (spin != 0 && spin != 1) || !offdiag_im) return ed::fail(ED_ERR_ARG, "ed_sector_tables_im: bad argument");
+    if (!sec->cplx) return ed::fail(ED_ERR_ARG, "ed_sector_tables_im: not a complex sector");
+    const ed::SpinTables &T = spin ? sec->dn : sec->up;
+    if (T.nnz) ED_HIP_CHECK(hipMemcpy(offdiag_im, T.off_im, sizeof(double) * T.nnz, hipMemcpyDeviceToHost));
     return ED_OK;
 }
 

@@ -38,6 +38,23 @@ module ed_hip
       integer(c_int), value      :: nup, ndn
       type(c_ptr), intent(out)   :: sec
     end function
+    ! complex AIM (COMPLEX(8) branch of new_H / HAIMupdo_multc): imaginary parts of Ec, Eb,
+    ! Vbc in the layout of the model's arrays; eb_im / vbc_im may be c_null_ptr (= 0)
+    integer(c_int) function ed_sector_create_c(model, ec_im, eb_im, vbc_im, nup, ndn, sec) &
+        bind(C, name="ed_sector_create_c")
+      import :: c_int, c_ptr, ed_model
+      type(ed_model), intent(in) :: model
+      type(c_ptr), value         :: ec_im, eb_im, vbc_im
+      integer(c_int), value      :: nup, ndn
+      type(c_ptr), intent(out)   :: sec
+    end function
+    integer(c_int) function ed_hmult_c128(sec, n, vec_out, vec_in) bind(C, name="ed_hmult_c128")
+      import :: c_int, c_int64_t, c_ptr, c_double_complex
+      type(c_ptr), value                   :: sec
+      integer(c_int64_t), value            :: n
+      complex(c_double_complex)            :: vec_out(*)
+      complex(c_double_complex), intent(in):: vec_in(*)
+    end function
     ! S_z (Nambu) sector of a superconducting bath (new_HAIMsz); pb / pvbc may be c_null_ptr
     integer(c_int) function ed_sector_create_sz(model, pb, pvbc, pairing_imp_to_bath, npart, sec) &
         bind(C, name="ed_sector_create_sz")

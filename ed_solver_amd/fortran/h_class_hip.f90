@@ -20,12 +20,13 @@ module H_class_hip
 
   type(c_ptr), save :: sector_h = c_null_ptr        ! H_class.f90:11
   real(c_double), allocatable, target, save :: m_ec(:), m_eb(:), m_vbc(:), m_u(:), m_jj(:)
+  real(c_double), allocatable, target, save :: m_ec_im(:), m_eb_im(:), m_vbc_im(:)
 
-  interface Hmult__
-    module procedure Hmultr
+  interface Hmult__                                   ! H_class.f90:13-15
+    module procedure Hmultr, Hmultc
   end interface
 
-  public :: new_H, delete_H, Hmult__, Hmultr, dimen_H, sector_handle
+  public :: new_H, new_H_complex, delete_H, Hmult__, Hmultr, Hmultc, dimen_H, sector_handle
   public :: Lanczos_GPU_updo, Lanczos_GPU_get_GS_updo, Lanczos_dynamic_GPU_updo
 
 contains
@@ -41,6 +42,24 @@ contains
     m%ec = c_loc(m_ec); m%eb = c_loc(m_eb); m%vbc = c_loc(m_vbc); m%u = c_loc(m_u); m%jj = c_loc(m_jj)
     m%jhund = jhund; m%cutoff = cutoff
     call ed_check(ed_sector_create(m, int(nup, c_int), int(ndn, c_int), sector_h), 'new_H')
+  end subroutine
+
+  ! new_H for a complex AIM: the imaginary parts of Ec, Eb, Vbc beside the real ones;
+  ! the sector's vectors are then COMPLEX(8) (Hmultc).
+  subroutine new_H_complex(nc, nb, ec, eb, vbc, ec_im, eb_im, vbc_im, u, jj, jhund, cutoff, nup, ndn)
+    integer, intent(in)        :: nc, nb, nup, ndn
+    real(8), intent(in)        :: ec(2*nc*nc), eb(max(1, 2*nb*nb)), vbc(max(1, 2*nb*nc)), u(nc*nc), jj(nc*nc)
+    real(8), intent(in)        :: ec_im(2*nc*nc), eb_im(max(1, 2*nb*nb)), vbc_im(max(1, 2*nb*nc))
+    real(8), intent(in)        :: jhund, cutoff
+    type(ed_model) :: m
+    call delete_H()
+    m_ec = ec; m_eb = eb; m_vbc = vbc; m_u = u; m_jj = jj
+    m_ec_im = ec_im; m_eb_im = eb_im; m_vbc_im = vbc_im
+    m%nc = nc; m%nb = nb
+    m%ec = c_loc(m_ec); m%eb = c_loc(m_eb); m%vbc = c_loc(m_vbc); m%u = c_loc(m_u); m%jj = c_loc(m_jj)
+    m%jhund = jhund; m%cutoff = cutoff
+    call ed_check(ed_sector_create_c(m, c_loc(m_ec_im), c_loc(m_eb_im), c_loc(m_vbc_im), int(nup, c_int), &
+                                     int(ndn, c_int), sector_h), 'new_H_complex')
   end subroutine
 
   subroutine delete_H()
@@ -62,6 +81,15 @@ contains
     real(8), intent(in)    :: vec_in(n)
     if (.not. c_associated(sector_h)) stop 'Hmultr, no array are associated!'
     call ed_check(ed_hmult_f64(sector_h, int(n, c_int64_t), vec_out, vec_in), 'Hmultr')
+  end subroutine
+
+  ! Hmultc (H_class.f90:69-98) -> HAIMupdo_multc on a sector made by new_H_complex
+  subroutine Hmultc(n, vec_out, vec_in)
+    integer                   :: n
+    complex(8), intent(inout) :: vec_out(n)
+    complex(8), intent(in)    :: vec_in(n)
+    if (.not. c_associated(sector_h)) stop 'Hmultc, no array are associated!'
+    call ed_check(ed_hmult_c128(sector_h, int(n, c_int64_t), vec_out, vec_in), 'Hmultc')
   end subroutine
 
   pure integer function dimen_H()

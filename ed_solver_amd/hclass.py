@@ -74,6 +74,21 @@ class Sector:
             raise EDError(-5, "sector already destroyed (delete_H)")
         return self._h
 
+    # host vectors of the Lanczos / apply_C / dynamic entry points: dim doubles here,
+    # the planar (Re, Im) form of dim complex amplitudes for a ComplexSector
+    @property
+    def vlen(self) -> int:
+        return self.dim
+
+    def _pack(self, v: np.ndarray, what: str = "vector") -> np.ndarray:
+        v = np.ascontiguousarray(v, np.float64)
+        if v.shape != (self.dim,):
+            raise EDError(-1, f"{what} of length {v.size}, sector dimension {self.dim}")
+        return v
+
+    def _unpack(self, v: np.ndarray) -> np.ndarray:
+        return v
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().ed_sector_destroy(self._h)
@@ -183,9 +198,7 @@ class Sector:
         it, ne = ctypes.c_int(), ctypes.c_int()
         sp = None
         if start is not None:
-            start = np.ascontiguousarray(start, np.float64)
-            if start.shape != (self.dim,):
-                raise EDError(-1, "start vector length differs from the sector dimension")
+            start = self._pack(start, "start vector")
             sp = dptr(start)
         check(lib().ed_lanczos_diagonalize(self.handle, ctypes.byref(o), sp, ctypes.byref(it), ctypes.byref(ne),
                                            dptr(ev), dptr(vecp), dptr(dg), dptr(sd)))
@@ -198,10 +211,10 @@ class Sector:
 
     def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
         """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""
-        psi = np.zeros(self.dim)
+        psi = np.zeros(self.vlen)
         row = np.ascontiguousarray(res.vecp[k], np.float64)
         check(lib().ed_lanczos_gs_vector(self.handle, res.iters, dptr(row), dptr(psi)))
-        return psi
+        return self._unpack(psi)
 
     def full_ed(self, neig: int = 1):
         """ED_diago: lowest neig eigenpairs of the dense sector H (dim <= 1024)."""
@@ -215,15 +228,15 @@ class Sector:
         dg = np.zeros(niter); sd = np.zeros(niter)
         sp = None
         if start is not None:
-            start = np.ascontiguousarray(start, np.float64)
+            start = self._pack(start, "start vector")
             sp = dptr(start)
         check(lib().ed_lanczos_run(self.handle, niter, sp, _prec(precision), dptr(dg), dptr(sd)))
         return dg, sd
 
     def rayleigh(self, psi: np.ndarray) -> float:
-        psi = np.ascontiguousarray(psi, np.float64)
+        psi = self._pack(psi)
         e = ctypes.c_double()
-        check(lib().ed_rayleigh_f64(self.handle, self.dim, dptr(psi), ctypes.byref(e)))
+        check(lib().ed_rayleigh_f64(self.handle, self.vlen, dptr(psi), ctypes.byref(e)))
         return e.value
 
 
@@ -323,12 +336,10 @@ def band_eig(a: np.ndarray, b: int, nev: int, vectors: bool = True):
 
 def apply_C(src: Sector, dst: Sector, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
     """apply_Cupdo (apply_C.f90:497-598): c+ (pm=+1) / c (pm=-1) on impurity orbital `site`."""
-    psi = np.ascontiguousarray(psi, np.float64)
-    if psi.shape != (src.dim,):
-        raise EDError(-1, "apply_C: psi length differs from the source sector dimension")
-    phi = np.zeros(dst.dim)
+    psi = src._pack(psi, "apply_C: psi")
+    phi = np.zeros(dst.vlen)
     check(lib().ed_apply_c(src.handle, dst.handle, site, spin, pm, dptr(psi), dptr(phi)))
-    return phi
+    return dst._unpack(phi)
 
 
 def matsubara(beta: float, nw: int) -> np.ndarray:
@@ -376,13 +387,75 @@ class SzSector(Sector):
         return q
 
 
+class ComplexSector(Sector):
+    """One (N_up, N_dn) sector of a complex AIM (tab_HAIM2 with COMPLEX(8) entries and
+    HAIMupdo_multc, HAIMupdo.f90:150-656).  model holds the real parts of Ec, Eb, Vbc;
+    ec_im[2, nc, nc], eb_im[2, nb, nb], vbc_im[2, nb, nc] the imaginary parts.  Vectors
+    are complex numpy arrays of dim amplitudes; the device holds them planar (Re, Im),
+    where H acts as [[A, -B], [B, A]] (H = A + iB) and the plain Lanczos has the complex
+    Lanczos' real alpha and beta.  FP64 only; no full ED, block Lanczos or gradients."""
+
+    def __init__(self, model: AIMParams, nup: int, ndn: int, ec_im, eb_im=None, vbc_im=None):
+        self.model = model
+        self.nup, self.ndn = int(nup), int(ndn)
+        self._cm = CModel.from_params(model)
+        nc, nb = model.nc, model.nb
+        self._ec_im = np.ascontiguousarray(np.broadcast_to(ec_im, (2, nc, nc)), np.float64)
+        self._eb_im = np.ascontiguousarray(np.zeros((2, nb, nb)) if eb_im is None
+                                           else np.broadcast_to(eb_im, (2, nb, nb)), np.float64)
+        self._vbc_im = np.ascontiguousarray(np.zeros((2, nb, nc)) if vbc_im is None
+                                            else np.broadcast_to(vbc_im, (2, nb, nc)), np.float64)
+        h = ctypes.c_void_p()
+        check(lib().ed_sector_create_c(ctypes.byref(self._cm), dptr(self._ec_im), dptr(self._eb_im),
+                                       dptr(self._vbc_im), self.nup, self.ndn, ctypes.byref(h)))
+        self._h = h
+        self.dim = int(lib().ed_sector_dim(h))
+        du, dd = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ed_sector_spin_dims(h, ctypes.byref(du), ctypes.byref(dd)))
+        self.dup, self.ddn = du.value, dd.value
+
+    @property
+    def vlen(self) -> int:
+        return 2 * self.dim
+
+    def _pack(self, v: np.ndarray, what: str = "vector") -> np.ndarray:
+        v = np.asarray(v)
+        if v.shape != (self.dim,):
+            raise EDError(-1, f"{what} of length {v.size}, sector dimension {self.dim}")
+        return np.ascontiguousarray(np.concatenate([v.real, v.imag]), np.float64)
+
+    def _unpack(self, v: np.ndarray) -> np.ndarray:
+        return v[:self.dim] + 1j * v[self.dim:]
+
+    def tables_im(self, spin: int) -> np.ndarray:
+        """Imaginary parts of one spin's offdiag (in the order of tables(spin))."""
+        nnz = self.nnz(spin)
+        im = np.zeros(max(nnz, 1))
+        check(lib().ed_sector_tables_im(self.handle, spin, dptr(im)))
+        return im[:nnz]
+
+    def hmult(self, v: np.ndarray) -> np.ndarray:
+        """Hmultc: complex host vector in, complex host vector out."""
+        v = np.ascontiguousarray(v, np.complex128)
+        if v.shape != (self.dim,):
+            raise EDError(-1, f"Hmult: vector of length {v.size}, sector dimension {self.dim}")
+        out = np.empty_like(v)
+        check(lib().ed_hmult_c128(self.handle, self.dim, dptr(out.view(np.float64)), dptr(v.view(np.float64))))
+        return out
+
+    def hmult_c128_dev(self, out_ptr: int, in_ptr: int, stream: int | None = None) -> None:
+        """Hmultc on device COMPLEX(8) vectors (e.g. torch.complex128 data_ptr()); asynchronous."""
+        check(lib().ed_hmult_c128_dev(self.handle, self.dim, ctypes.c_void_p(out_ptr), ctypes.c_void_p(in_ptr),
+                                      ctypes.c_void_p(stream) if stream else None))
+
+
 def compute_dynamic(sector: Sector, iph: int, energy: float, phi: np.ndarray, nitergreen: int, wn: np.ndarray,
                     cutoff_dynamic: float = 1e-8, precision="f64"):
     """compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407): returns (dyn, a, b)."""
-    phi = np.ascontiguousarray(phi, np.float64)
-    wn = np.ascontiguousarray(wn, np.float64)
-    if phi.shape != (sector.dim,):
+    if np.shape(phi) != (sector.dim,):
         raise EDError(-1, "compute_dynamic inconsistent dimensions, CRITICAL")
+    phi = sector._pack(phi)
+    wn = np.ascontiguousarray(wn, np.float64)
     n = max(1, min(nitergreen, sector.dim))
     dyn = np.zeros(2 * len(wn)); a = np.zeros(n); b = np.zeros(n)
     check(lib().ed_dynamic(sector.handle, iph, energy, dptr(phi), nitergreen, cutoff_dynamic, _prec(precision),
@@ -410,10 +483,7 @@ def compute_dynamic_batched(sectors, phis, iphs, energies, nitergreen: int, wn: 
                             cutoff_dynamic: float = 1e-8, precision="f64"):
     """All excitation runs in one call; returns dyn (njobs, nw) complex."""
     nj = len(sectors)
-    phis = [np.ascontiguousarray(p, np.float64) for p in phis]
-    for s, p in zip(sectors, phis):
-        if p.shape != (s.dim,):
-            raise EDError(-1, "compute_dynamic_batched: phi length differs from its sector dimension")
+    phis = [s._pack(p, "compute_dynamic_batched: phi") for s, p in zip(sectors, phis)]
     wn = np.ascontiguousarray(wn, np.float64)
     hs = (ctypes.c_void_p * nj)(*[s.handle.value for s in sectors])
     ps = (ctypes.POINTER(ctypes.c_double) * nj)(*[dptr(p) for p in phis])

@@ -97,10 +97,26 @@ const char *ed_version(void);
  * (HAIM_offdiag_arrays.f90:89-148, 210-606) with integer kernels on the device. */
 int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out);
 
+/* new_H for a complex AIM (the COMPLEX(8) branch of tab_HAIM2 and HAIMupdo_multc,
+ * HAIMupdo.f90:150-656): the model's ec/eb/vbc are the real parts and ec_im[2][nc][nc],
+ * eb_im[2][nb][nb], vbc_im[2][nb][nc] the imaginary parts (eb_im, vbc_im may be NULL =
+ * 0).  Ec and Eb must be Hermitian per spin.  Entries are kept when nonzero and
+ * |t| > cutoff (complex modulus); hybridisation entries are Vbc (impurity -> bath) and
+ * conj(Vbc) (bath -> impurity).  ed_hmult_c128 applies H to COMPLEX(8) vectors.  Every
+ * real-typed vector of the other entry points (Lanczos start and ground state, apply_C,
+ * dynamic, ed_hmult_f64) has length 2 dimen_H() and holds the planar form
+ * (Re[dimen_H], Im[dimen_H]): H acts there as the symmetric [[A, -B], [B, A]] of
+ * H = A + iB, whose plain Lanczos has the complex Lanczos' (real) alpha and beta.  FP64
+ * only; full ED, block Lanczos, batched dynamic and Hellmann-Feynman gradients take real
+ * sectors only. */
+int ed_sector_create_c(const ed_model *model, const double *ec_im, const double *eb_im, const double *vbc_im,
+                       int nup, int ndn, ed_sector **out);
+
 /* delete_H (H_class.f90:48-58).  NULL is accepted. */
 int ed_sector_destroy(ed_sector *sec);
 
-/* dimen_H (H_class.f90:149-152) = d_up * d_dn. */
+/* dimen_H (H_class.f90:149-152) = d_up * d_dn (the number of complex amplitudes of a
+ * complex sector). */
 int64_t ed_sector_dim(const ed_sector *sec);
 
 /* Spin-sector dimensions d_up, d_dn (fermion_sector2 up%dimen, down%dimen). */
@@ -116,6 +132,10 @@ int ed_sector_nnz(const ed_sector *sec, int spin, int64_t *nnz);
 int ed_sector_tables(const ed_sector *sec, int spin, int32_t *states, double *diag,
                      int32_t *noff, int32_t *rankoff, double *offdiag);
 
+/* Imaginary parts of one spin's offdiag[nnz] (complex sectors; offdiagup is COMPLEX(8)
+ * there, HAIM_offdiag_arrays.f90:20-44). */
+int ed_sector_tables_im(const ed_sector *sec, int spin, double *offdiag_im);
+
 /* ---------------------------------------------------------------------- H.v */
 
 /* Hmultr (H_class.f90:109-136) -> HAIMupdo_multr (HAIMupdo.f90:677-1195):
@@ -129,6 +149,15 @@ int ed_hmult_f64_dev(ed_sector *sec, int64_t n, double *d_out, const double *d_i
 
 /* FP32 H.v on device vectors (tables rounded to FP32). */
 int ed_hmult_f32_dev(ed_sector *sec, int64_t n, float *d_out, const float *d_in, void *stream);
+
+/* Hmultc (H_class.f90:69-98) -> HAIMupdo_multc (HAIMupdo.f90:150-656) on a complex
+ * sector: vec_out = H vec_in for COMPLEX(8) arrays (re, im interleaved) of n = dimen_H()
+ * amplitudes; w_i = sum_j conj(<j|H|i>) v_j.  Host arrays, synchronous. */
+int ed_hmult_c128(ed_sector *sec, int64_t n, double *vec_out, const double *vec_in);
+
+/* Hmultc on device COMPLEX(8) vectors (2 n doubles each; d_out may equal d_in);
+ * `stream` as for ed_hmult_f64_dev.  Asynchronous. */
+int ed_hmult_c128_dev(ed_sector *sec, int64_t n, double *d_out, const double *d_in, void *stream);
 
 /* Waits for all work queued on the sector's own stream (the asynchronous device
  * entry points above use it when called with stream == NULL). */

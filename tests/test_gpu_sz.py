@@ -133,3 +133,26 @@ def test_sz_rejections(gpu):
     with pytest.raises(EDError):
         s.set_schedule(8)
     s.close(); s2.close()
+
+
+def test_sz_sector_above_two_million_matches_updo_sectors(gpu):
+    """Ns = 12 without pairing, npart = 12: dimension C(24, 12) = 2,704,156, above the
+    8192 x 256 rows of one H.v launch's partial sums (the kernel strides over the rest).
+    Without pairing the block N_up - N_dn = 0 is the sum of the (k, k) sectors, so
+    E0 + energy_shift = min_k E0(k, k)."""
+    m, _, _ = sc_model(3, 9, 2.0, seed=21, u_offdiag=1.0)
+    s = gpu.SzSector(m, 12)
+    assert s.dim == 2704156
+    print(f"S_z sector dim {s.dim}", flush=True)
+    r = s.lanczos(nitermax=400, neigen=1, tolerance=1e-12)
+    e_sz = r.evals[0] + s.energy_shift
+    s.close()
+    e_updo = []
+    for k in range(13):
+        sk = gpu.Sector(m, k, k)
+        if sk.dim == 1:
+            e_updo.append(float(sk.full_ed(1)[0][0]))
+        else:
+            e_updo.append(float(sk.lanczos(nitermax=400, neigen=1, tolerance=1e-12).evals[0]))
+        sk.close()
+    assert abs(e_sz - min(e_updo)) < 1e-9, (e_sz, min(e_updo))

@@ -522,6 +522,9 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
         za.val = (const T *)(f64 ? (const void *)s->up.ell_val : (const void *)s->up.ell_val32);
         za.diag = (const T *)(f64 ? (const void *)s->up.diag : (const void *)s->up.diag32);
         za.quart = (const T *)(f64 ? (const void *)s->d_quart : (const void *)s->d_quart32);
+        za.kh = s->sz_kh;
+        za.hrk = s->d_sz_hrk;
+        za.hval = (const T *)(f64 ? (const void *)s->d_sz_hval : (const void *)s->d_sz_hval32);
         hipLaunchKernelGGL((k_hv_sz<T, MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za, x, y, ld, lz);
         return;
     }

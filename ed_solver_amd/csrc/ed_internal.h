@@ -204,6 +204,12 @@ struct ed_sector {
     std::vector<double> ec_im, eb_im, vbc_im;
     double *d_quart = nullptr;
     float *d_quart32 = nullptr;
+    // S_z sector with Hund terms (HAIMsz_multc): ELL [sz_kh][dim] of the spin-flip and
+    // pair-hopping transitions (target rank, -J sign), padded with the row itself and 0
+    int sz_kh = 0;
+    int32_t *d_sz_hrk = nullptr;
+    double *d_sz_hval = nullptr;
+    float *d_sz_hval32 = nullptr;
     double energy_shift = 0.0;
     int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
     double *h_mon = nullptr;       // monitored persistent run: pinned fine-grained host channel

@@ -534,10 +534,15 @@ struct SzArgs {
     const T *val;
     const T *diag;
     const T *quart;
+    int kh;                 // Hund transitions per row (0: none)
+    const int32_t *hrk;     // [kh][d]
+    const T *hval;
 };
 
 // HAIMsz_multr (HAIMsz.f90:453-560, USE_CC gather form): one thread per row,
-//   w_i = QUART_i v_i + diag_i v_i + sum_k off_ik v_{rank_ik}   in this order.
+//   w_i = QUART_i v_i + diag_i v_i + sum_k off_ik v_{rank_ik}   in this order,
+// then (|Jhund| > 1e-5) the spin-flip and pair-hopping terms of HAIMsz_multc
+// (HAIMsz.f90:300-406) from their ELL, for real vectors.
 // MODE 1: Lanczos phase 1 on the Krylov slots with the partial <z, u_j> per workgroup.
 template <typename T, int MODE>
 __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
@@ -562,6 +567,7 @@ __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin,
         T acc = a.quart[i] * xv;
         acc += a.diag[i] * xv;
         for (int k = 0; k < a.kw; ++k) acc += a.val[k * d + i] * x[a.rk[k * d + i]];
+        for (int k = 0; k < a.kh; ++k) acc += a.hval[k * d + i] * x[a.hrk[k * d + i]];
         y[i] = acc;
         dot += (double)acc * (double)xv;
     }

@@ -492,7 +492,7 @@ static void free_sector(ed_sector *S)
     free_spin(S->dn);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
                   S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord,
-                  S->d_quart, S->d_quart32};
+                  S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (ed::GraphCache &G : S->graphs) {
@@ -573,14 +573,19 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
 // diagonal mask, HAIMsz.f90:75-79; every term it can add has U_cc' != 0, so the sum over
 // the nonzero off-diagonal U is the same result.)  J = 0 here: the real-valued S_z
 // H.v stops for |Jhund| > 1e-5 (HAIMsz.f90:473-476).
-__global__ void k_quart_sz(const int32_t *states, int d, int nc, int nb, int ns, const double *u, double *quart)
+// jj (non-null when |Jhund| > 1e-5, Jhund_Slater_type): - J_c1c2 for every pair c1 < c2
+// whose physical up orbitals are both occupied, and again for the down orbitals
+// (HAIMsz.f90:161-170).
+__global__ void k_quart_sz(const int32_t *states, int d, int nc, int nb, int ns, const double *u, const double *jj,
+                           double *quart)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
     const uint32_t s = (uint32_t)states[i];
     double q = 0.0;
     for (int c1 = 0; c1 < nc; ++c1) {
-        const int n1 = (int)((s >> (nb + c1)) & 1u) + (int)(((s >> (ns + nb + c1)) & 1u) ^ 1u);
+        const int u1 = (int)((s >> (nb + c1)) & 1u), p1 = (int)(((s >> (ns + nb + c1)) & 1u) ^ 1u);
+        const int n1 = u1 + p1;
         if (n1 == 2 && u[c1 * nc + c1] != 0.0) q += u[c1 * nc + c1];
         if (n1 != 0)
             for (int c2 = c1 + 1; c2 < nc; ++c2) {
@@ -588,8 +593,67 @@ __global__ void k_quart_sz(const int32_t *states, int d, int nc, int nb, int ns,
                 const int n2 = (int)((s >> (nb + c2)) & 1u) + (int)(((s >> (ns + nb + c2)) & 1u) ^ 1u);
                 q += u[c1 * nc + c2] * n1 * n2;
             }
+        if (jj)
+            for (int c2 = c1 + 1; c2 < nc; ++c2) {
+                const int u2 = (int)((s >> (nb + c2)) & 1u), p2 = (int)(((s >> (ns + nb + c2)) & 1u) ^ 1u);
+                if (u1 && u2) q -= jj[c1 * nc + c2];
+                if (p1 && p2) q -= jj[c1 * nc + c2];
+            }
     }
     quart[i] = q;
+}
+
+// The Hund terms of HAIMsz_multc (HAIMsz.f90:300-406) for every S_z state, in the
+// reference's order, as an ELL [kh][d] (padded with the row itself and 0).  Raw Nambu
+// bits: u_c = bit nb + c (up), p_c = bit ns + nb + c (particle-hole down orbital).
+//   spin flip, c1 < c2: (u, p) = (1, 1) at c1 and (0, 0) at c2 swap places, then the
+//     reverse; pair hopping, c1 != c2: (1, 0) at c1 and (0, 1) at c2 become (0, 1), (1, 0);
+// element -J_c1c2 times the parities of the target's bits strictly between u_c1 and u_c2
+// and between p_c1 and p_c2.  FILL = false counts the entries per state.
+template <bool FILL>
+__global__ void k_hund_sz(const int32_t *states, int d, int nc, int nb, int ns, const double *jj, const int32_t *rank,
+                          int kh, int32_t *cnt, int32_t *hrk, double *hval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    const uint32_t s = (uint32_t)states[i];
+    auto between = [](uint32_t o, int a, int b) {
+        const int lo = min(a, b), hi = max(a, b);
+        const uint32_t mask = hi - lo > 1 ? ((1u << (hi - lo - 1)) - 1u) << (lo + 1) : 0u;
+        return (__popc(o & mask) & 1) ? -1 : 1;
+    };
+    int k = 0;
+    auto put = [&](uint32_t o, int c1, int c2) {
+        if (FILL) {
+            const int sg = between(o, nb + c1, nb + c2) * between(o, ns + nb + c1, ns + nb + c2);
+            hrk[(int64_t)k * d + i] = rank[o];
+            hval[(int64_t)k * d + i] = -jj[c1 * nc + c2] * sg;
+        }
+        ++k;
+    };
+    auto ub = [&](int c) { return 1u << (nb + c); };
+    auto pb = [&](int c) { return 1u << (ns + nb + c); };
+    for (int c1 = 0; c1 < nc; ++c1) {
+        const bool u1 = s & ub(c1), p1 = s & pb(c1);
+        for (int c2 = c1 + 1; c2 < nc; ++c2) {
+            const bool u2 = s & ub(c2), p2 = s & pb(c2);
+            if (u1 && p1 && !u2 && !p2) put((s & ~ub(c1) & ~pb(c1)) | pb(c2) | ub(c2), c1, c2);
+            if (!u1 && !p1 && u2 && p2) put((s & ~ub(c2) & ~pb(c2)) | pb(c1) | ub(c1), c1, c2);
+        }
+    }
+    for (int c1 = 0; c1 < nc; ++c1) {
+        if (!((s & ub(c1)) && !(s & pb(c1)))) continue;
+        for (int c2 = 0; c2 < nc; ++c2)
+            if (c2 != c1 && (s & pb(c2)) && !(s & ub(c2))) put((s & ~ub(c1) & ~pb(c2)) | pb(c1) | ub(c2), c1, c2);
+    }
+    if (FILL) {
+        for (; k < kh; ++k) {
+            hrk[(int64_t)k * d + i] = i;
+            hval[(int64_t)k * d + i] = 0.0;
+        }
+    } else {
+        cnt[i] = k;
+    }
 }
 
 // Nambu matrices (Nambu_Ec, impurity_class.f90:1018-1037; Nambu_Eb / Nambu_Vbc,
@@ -644,8 +708,6 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     if (!m->ec || !m->u || !m->jj || (nb && (!m->eb || !m->vbc)))
         return fail(ED_ERR_ARG, "ed_sector_create_sz: NULL model array");
     if (pair_ib && !pvbc) return fail(ED_ERR_ARG, "ed_sector_create_sz: pairing_imp_to_bath needs pvbc");
-    if (std::fabs(m->jhund) > 1e-5)
-        return fail(ED_ERR_ARG, "ed_sector_create_sz: Jhund and Sz basis not implemented for real H (HAIMsz.f90:473-476)");
     ed_sector *S = new ed_sector();
     ED_HIP_CHECK(hipGetDevice(&S->device));
     S->nc = nc; S->nb = nb; S->ns = ns; S->nup = npart; S->ndn = -1; S->ncfg = 1 << nc;
@@ -683,9 +745,9 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     hipLaunchKernelGGL(k_enum_states, dim3(nblocks(d)), dim3(kBlock), 0, st, nbits, npart, d, T.states);
     std::vector<double> ecn, ebn, vbn;
     nambu_matrices(nc, nb, m->ec, m->eb, m->vbc, pb, pvbc, pair_ib, ecn, ebn, vbn);
-    double *d_mat = nullptr, *d_u = nullptr;
+    double *d_mat = nullptr, *d_u = nullptr;   // d_u: U[nc][nc], then JJ[nc][nc]
     const size_t nmat = ecn.size() + std::max<size_t>(1, ebn.size()) + std::max<size_t>(1, vbn.size());
-    if ((rc = dalloc(&d_mat, nmat)) != ED_OK || (rc = dalloc(&d_u, (size_t)nc * nc)) != ED_OK) {
+    if ((rc = dalloc(&d_mat, nmat)) != ED_OK || (rc = dalloc(&d_u, (size_t)2 * nc * nc)) != ED_OK) {
         (void)hipFree(d_mat);
         return bail(rc);
     }
@@ -704,6 +766,7 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
         ED_HIP_CHECK(hipMemcpyAsync(d_vbc, vbn.data(), sizeof(double) * vbn.size(), hipMemcpyHostToDevice, st));
     }
     ED_HIP_CHECK(hipMemcpyAsync(d_u, m->u, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+    ED_HIP_CHECK(hipMemcpyAsync(d_u + nc * nc, m->jj, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
     SpinModel sm{};
     sm.nimp = 2 * nc;
     sm.nbath = 2 * nb;
@@ -746,9 +809,43 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
         (rc = dalloc(&T.diag32, d)) != ED_OK || (rc = dalloc(&T.off32, (size_t)T.nnz)) != ED_OK ||
         (rc = dalloc(&T.ell_val32, (size_t)kw * d)) != ED_OK)
         return bail(rc);
+    // Hund terms of HAIMsz_multc when |Jhund| > 1e-5 and some U_cc' != 0 (offdiag_coulomb)
+    bool offc = false;
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b)
+            if (a != b && m->u[a * nc + b] != 0.0) offc = true;
+    const bool jterm = std::fabs(m->jhund) > 1e-5;
+    const double *d_jj = jterm ? (const double *)(d_u + nc * nc) : nullptr;
     hipLaunchKernelGGL(k_quart_sz, dim3(nblocks(d)), dim3(kBlock), 0, st, T.states, d, nc, nb, ns, (const double *)d_u,
-                       S->d_quart);
+                       d_jj, S->d_quart);
     hipLaunchKernelGGL(k_to_f32, dim3(nblocks(d)), dim3(kBlock), 0, st, S->d_quart, S->d_quart32, (int64_t)d);
+    S->sz_kh = 0;
+    if (jterm && offc) {
+        int32_t *d_cnt = nullptr, *d_ptr = nullptr, *d_kh = nullptr;
+        if ((rc = dalloc(&d_cnt, d)) != ED_OK || (rc = dalloc(&d_ptr, (size_t)d + 1)) != ED_OK ||
+            (rc = dalloc(&d_kh, 1)) != ED_OK)
+            return bail(rc);
+        hipLaunchKernelGGL(k_hund_sz<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, T.states, d, nc, nb, ns, d_jj,
+                           (const int32_t *)S->d_rank, 0, d_cnt, (int32_t *)nullptr, (double *)nullptr);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kBlock), 0, st, (const int32_t *)d_cnt, d, d_ptr, d_kh);
+        int32_t kh = 0;
+        ED_HIP_CHECK(hipMemcpyAsync(&kh, d_kh, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipStreamSynchronize(st));
+        (void)hipFree(d_cnt);
+        (void)hipFree(d_ptr);
+        (void)hipFree(d_kh);
+        if (kh > 0) {
+            if ((rc = dalloc(&S->d_sz_hrk, (size_t)kh * d)) != ED_OK ||
+                (rc = dalloc(&S->d_sz_hval, (size_t)kh * d)) != ED_OK ||
+                (rc = dalloc(&S->d_sz_hval32, (size_t)kh * d)) != ED_OK)
+                return bail(rc);
+            hipLaunchKernelGGL(k_hund_sz<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, T.states, d, nc, nb, ns, d_jj,
+                               (const int32_t *)S->d_rank, (int)kh, (int32_t *)nullptr, S->d_sz_hrk, S->d_sz_hval);
+            hipLaunchKernelGGL(k_to_f32, dim3(nblocks((int64_t)kh * d)), dim3(kBlock), 0, st, S->d_sz_hval,
+                               S->d_sz_hval32, (int64_t)kh * d);
+            S->sz_kh = kh;
+        }
+    }
     hipLaunchKernelGGL(k_to_f32, dim3(nblocks(d)), dim3(kBlock), 0, st, T.diag, T.diag32, (int64_t)d);
     if (T.nnz)
         hipLaunchKernelGGL(k_to_f32, dim3(nblocks(T.nnz)), dim3(kBlock), 0, st, T.off, T.off32, (int64_t)T.nnz);
@@ -779,6 +876,17 @@ int ed_sector_quart_sz(const ed_sector *sec, double *quart)
     if (!sec || !quart) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: NULL argument");
     if (!sec->sz) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: not an S_z sector");
     ED_HIP_CHECK(hipMemcpy(quart, sec->d_quart, sizeof(double) * sec->dim, hipMemcpyDeviceToHost));
+    return ED_OK;
+}
+
+int ed_sector_hund_sz(const ed_sector *sec, int *kh, int32_t *rk, double *val)
+{
+    if (!sec || !kh) return ed::fail(ED_ERR_ARG, "ed_sector_hund_sz: NULL argument");
+    if (!sec->sz) return ed::fail(ED_ERR_ARG, "ed_sector_hund_sz: not an S_z sector");
+    *kh = sec->sz_kh;
+    const size_t n = (size_t)sec->sz_kh * sec->dim;
+    if (rk && n) ED_HIP_CHECK(hipMemcpy(rk, sec->d_sz_hrk, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (val && n) ED_HIP_CHECK(hipMemcpy(val, sec->d_sz_hval, sizeof(double) * n, hipMemcpyDeviceToHost));
     return ED_OK;
 }
 

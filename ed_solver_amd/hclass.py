@@ -361,8 +361,9 @@ class SzSector(Sector):
     """One S_z (Nambu) sector of an Anderson impurity model with a superconducting bath
     (new_HAIMsz, HAIMsz.f90:51-177): 2 Ns orbitals, the down spin particle-hole
     transformed, npart = Ns + 2 S_z.  pb[nb][nb] = bath pairing Pb, pvbc[2][nb][nc] =
-    impurity-bath pairing (used when pairing_imp_to_bath).  All H.v / Lanczos / dense-ED
-    methods of Sector apply; physical energies are eigenvalues + energy_shift."""
+    impurity-bath pairing (used when pairing_imp_to_bath).  With J, the Hund terms of
+    HAIMsz_multc are included.  All H.v / Lanczos / dense-ED methods of Sector apply;
+    physical energies are eigenvalues + energy_shift."""
 
     def __init__(self, model: AIMParams, npart: int, pb: np.ndarray | None = None, pvbc: np.ndarray | None = None,
                  pairing_imp_to_bath: bool = False):
@@ -385,6 +386,15 @@ class SzSector(Sector):
         q = np.zeros(self.dim)
         check(lib().ed_sector_quart_sz(self.handle, dptr(q)))
         return q
+
+    def hund(self):
+        """Hund transitions of HAIMsz_multc as (rk[kh, dim], val[kh, dim]) (kh = 0 without J)."""
+        kh = ctypes.c_int()
+        check(lib().ed_sector_hund_sz(self.handle, ctypes.byref(kh), None, None))
+        rk = np.zeros((kh.value, self.dim), np.int32); val = np.zeros((kh.value, self.dim))
+        if kh.value:
+            check(lib().ed_sector_hund_sz(self.handle, ctypes.byref(kh), iptr(rk), dptr(val)))
+        return rk, val
 
 
 class ComplexSector(Sector):

@@ -66,12 +66,20 @@ typedef struct ed_sector ed_sector;
  * bits, in increasing order.  pb[nb][nb] = Pb (bath pairing b+_k,up b+_l,dn + h.c.; NULL =
  * zero), pvbc[2][nb][nc] = PVbc (impurity-bath pairing, used when pairing_imp_to_bath,
  * PAIRING_IMP_TO_BATH).  Quadratic part: tab_HAIM2 over the Nambu matrices (Nambu_Ec,
- * Nambu_Eb, Nambu_Vbc); diagonal interaction QUART_INT_SZ.  Real H only: |Jhund| > 1e-5
- * is rejected (HAIMsz.f90:473-476).  Ns <= 13.  The returned sector works with every
+ * Nambu_Eb, Nambu_Vbc); diagonal interaction QUART_INT_SZ (with -J for parallel spins
+ * when |Jhund| > 1e-5).  With |Jhund| > 1e-5 and some U_cc' != 0 the spin-flip and
+ * pair-hopping terms of HAIMsz_multc (HAIMsz.f90:300-406) are added as well, so a real
+ * model with J runs the algebra the reference only runs through its COMPLEX(8) path
+ * (its real HAIMsz_multr stops on J, :473-476).  Ns <= 13.  The returned sector works with every
  * H.v, Lanczos, block Lanczos, ARPACK-mirror and dense-ED entry point; eigenvalues are
  * those of the Nambu H, the physical energies are E + ed_sector_energy_shift(). */
 int ed_sector_create_sz(const ed_model *model, const double *pb, const double *pvbc, int pairing_imp_to_bath,
                         int npart, ed_sector **out);
+
+/* Hund transitions of an S_z sector as an ELL: *kh per state (0 without J), rk[kh][dim]
+ * target ranks and val[kh][dim] = -J sign in the order of HAIMsz_multc (HAIMsz.f90:300-406),
+ * rows padded with the row itself and 0.  rk / val may be NULL (query *kh). */
+int ed_sector_hund_sz(const ed_sector *sec, int *kh, int32_t *rk, double *val);
 
 /* QUART_INT_SZ of an S_z sector (HAIMsz.f90:129-177), one value per state [dimen]. */
 int ed_sector_quart_sz(const ed_sector *sec, double *quart);

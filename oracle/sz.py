@@ -125,37 +125,103 @@ def tab_sz(model: AIMParams, npart: int, pb=None, pvbc=None, pairing_imp_to_bath
                     add(t, ibit[c], bbit[b])
         noff[i] = len(rk) - n0
         q = 0.0
+        offc = any(U[a, b] != 0.0 for a in range(nc) for b in range(nc) if a != b)
         for c1 in range(nc):
-            n1 = ((s >> (nb + c1)) & 1) + (1 - ((s >> (ns + nb + c1)) & 1))
+            up1, dn1 = (s >> (nb + c1)) & 1, 1 - ((s >> (ns + nb + c1)) & 1)
+            n1 = up1 + dn1
             if n1 == 2 and U[c1, c1] != 0.0:
                 q += U[c1, c1]
-            if n1 != 0:
+            if offc and n1 != 0:
                 for c2 in range(c1 + 1, nc):
                     if U[c1, c2] != 0.0:
                         n2 = ((s >> (nb + c2)) & 1) + (1 - ((s >> (ns + nb + c2)) & 1))
                         q += U[c1, c2] * n1 * n2
+            if abs(model.jhund) > 1e-5:   # HAIMsz.f90:161-170, parallel spins, every c2 > c1
+                for c2 in range(c1 + 1, nc):
+                    up2, dn2 = (s >> (nb + c2)) & 1, 1 - ((s >> (ns + nb + c2)) & 1)
+                    if up1 and up2:
+                        q -= model.jj[c1, c2]
+                    if dn1 and dn2:
+                        q -= model.jj[c1, c2]
         quart[i] = q
     return {"states": states.astype(np.int32), "diag": diag, "noff": noff, "rankoff": np.array(rk, np.int32),
             "offdiag": np.array(off), "quart": quart}
 
 
-def hmult_sz(tab: dict, v: np.ndarray) -> np.ndarray:
-    """HAIMsz_multr in gather form: w_i = quart_i v_i + diag_i v_i + sum_k off_ik v_rank."""
+def _between(s: int, a: int, b: int) -> int:
+    """(-1)^(occupied bits strictly between bits a and b of s)."""
+    lo, hi = min(a, b), max(a, b)
+    return -1 if bin((s >> (lo + 1)) & ((1 << max(0, hi - lo - 1)) - 1)).count("1") & 1 else 1
+
+
+def hund_sz(model: AIMParams, npart: int) -> list:
+    """The Hund terms of HAIMsz_multc (HAIMsz.f90:300-406) as (row, column, value) in the
+    S_z basis, |Jhund| > 1e-5 and some U_cc' != 0 (c != c').  Raw Nambu bits: up bit u_c,
+    particle-hole bit p_c of the down orbital.
+    * spin flip, c1 < c2: the site with both raw bits set (physical up only) and the site
+      with none (physical down only) exchange them;
+    * pair hopping, c1 != c2: raw (u, p) = (1, 0) at c1 (doubly occupied) and (0, 1) at c2
+      (empty) become (0, 1) and (1, 0);
+    each with -J_c1c2 times the parities of the bits strictly between the two sites' up
+    bits and between their down bits, in the target state."""
+    nc, nb, ns, U = model.nc, model.nb, model.ns, model.u
+    offc = any(U[a, b] != 0.0 for a in range(nc) for b in range(nc) if a != b)
+    if not (abs(model.jhund) > 1e-5 and offc):
+        return []
+    states = sz_states(ns, npart)
+    rank = {int(s): i for i, s in enumerate(states)}
+    ub = lambda c: nb + c
+    pb = lambda c: ns + nb + c
+    out = []
+
+    def sign(o, c1, c2):
+        return _between(o, ub(c1), ub(c2)) * _between(o, pb(c1), pb(c2))
+
+    for i, s in enumerate(states):
+        s = int(s)
+        for c1 in range(nc):
+            b1 = ((s >> ub(c1)) & 1, (s >> pb(c1)) & 1)
+            for c2 in range(c1 + 1, nc):
+                b2 = ((s >> ub(c2)) & 1, (s >> pb(c2)) & 1)
+                if sum(b1) == 2 and sum(b2) == 0:
+                    o = (s & ~(1 << ub(c1)) & ~(1 << pb(c1))) | (1 << pb(c2)) | (1 << ub(c2))
+                    out.append((i, rank[o], -model.jj[c1, c2] * sign(o, c1, c2)))
+                if sum(b1) == 0 and sum(b2) == 2:
+                    o = (s & ~(1 << ub(c2)) & ~(1 << pb(c2))) | (1 << pb(c1)) | (1 << ub(c1))
+                    out.append((i, rank[o], -model.jj[c1, c2] * sign(o, c1, c2)))
+        for c1 in range(nc):
+            if not (((s >> ub(c1)) & 1) and not ((s >> pb(c1)) & 1)):
+                continue
+            for c2 in range(nc):
+                if c2 == c1 or not (((s >> pb(c2)) & 1) and not ((s >> ub(c2)) & 1)):
+                    continue
+                o = (s & ~(1 << ub(c1)) & ~(1 << pb(c2))) | (1 << pb(c1)) | (1 << ub(c2))
+                out.append((i, rank[o], -model.jj[c1, c2] * sign(o, c1, c2)))
+    return out
+
+
+def hmult_sz(tab: dict, v: np.ndarray, hund: list | None = None) -> np.ndarray:
+    """HAIMsz_multr / _multc in gather form: w_i = quart_i v_i + diag_i v_i + sum_k off_ik
+    v_rank (+ the Hund terms of hund_sz, HAIMsz_multc only)."""
     w = tab["quart"] * v + tab["diag"] * v
     ptr = np.concatenate([[0], np.cumsum(tab["noff"])])
     for i in range(len(v)):
         for k in range(ptr[i], ptr[i + 1]):
             w[i] += tab["offdiag"][k] * v[tab["rankoff"][k]]
+    for i, j, h in hund or []:
+        w[i] += h * v[j]
     return w
 
 
-def dense_sz(tab: dict) -> np.ndarray:
+def dense_sz(tab: dict, hund: list | None = None) -> np.ndarray:
     d = len(tab["states"])
     H = np.diag(tab["quart"] + tab["diag"])
     ptr = np.concatenate([[0], np.cumsum(tab["noff"])])
     for i in range(d):
         for k in range(ptr[i], ptr[i + 1]):
             H[i, tab["rankoff"][k]] += tab["offdiag"][k]
+    for i, j, h in hund or []:
+        H[i, j] += h
     return H
 
 

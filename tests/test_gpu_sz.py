@@ -120,12 +120,58 @@ def test_sz_lanczos_and_block_lanczos(gpu):
     s.close()
 
 
+def with_hund(m, J):
+    jj = np.full((m.nc, m.nc), J)
+    np.fill_diagonal(jj, 0.0)
+    return AIMParams(m.nc, m.nb, m.ec, m.eb, m.vbc, m.u, jj, J, m.cutoff)
+
+
+@pytest.mark.parametrize("case", [(2, 2, 4, True), (3, 2, 5, True), (2, 3, 5, False), (3, 3, 6, True)])
+def test_sz_hund_tables_hmult_and_spectrum(gpu, case):
+    """S_z sectors with Hund coupling (the terms of HAIMsz_multc): QUART_INT_SZ and the Hund
+    ELL bit-exact against the oracle, H.v to 1e-12, and the dense spectrum + shift against
+    the physical spectrum of the Kanamori impurity with pairing."""
+    nc, nb, npart, pair_ib = case
+    m0, pb, pvbc = sc_model(nc, nb, 3.0, seed=30 + nc + nb, u_offdiag=1.6)
+    m = with_hund(m0, 0.6)
+    s = gpu.SzSector(m, npart, pb, pvbc, pair_ib)
+    print(f"S_z with J {case}: dim {s.dim}", flush=True)
+    tab = osz.tab_sz(m, npart, pb, pvbc, pair_ib)
+    assert np.array_equal(s.quart(), tab["quart"])
+    hund = osz.hund_sz(m, npart)
+    rk, val = s.hund()
+    rows = [[] for _ in range(s.dim)]
+    for i, j, h in hund:
+        rows[i].append((j, h))
+    assert rk.shape[0] == max(len(r) for r in rows)
+    for i, r in enumerate(rows):
+        for k in range(rk.shape[0]):
+            want = r[k] if k < len(r) else (i, 0.0)
+            assert rk[k, i] == want[0] and val[k, i] == want[1], (i, k)
+    v = np.random.default_rng(3).uniform(-1, 1, s.dim)
+    ref = osz.hmult_sz(tab, v, hund)
+    assert np.max(np.abs(s.hmult(v) - ref)) < 1e-12 * np.max(np.abs(ref))
+    if s.dim <= 1024:
+        ev, _ = s.full_ed(min(s.dim, 8))
+        ep = np.sort(osz.physical_spectrum(m, npart, pb, pvbc if pair_ib else None, pair_ib))
+        assert np.max(np.abs(np.sort(ev) + s.energy_shift - ep[:len(ev)])) < 1e-10
+    r = s.lanczos(nitermax=300, neigen=1, tolerance=1e-12)
+    e_ref = np.sort(sla.eigsh(sparse_sz(tab) + sparse_hund(hund, s.dim), k=2, which="SA", tol=1e-12)[0])[0] \
+        if s.dim > 64 else np.linalg.eigvalsh(osz.dense_sz(tab, hund))[0]
+    assert abs(r.evals[0] - e_ref) < 1e-10
+    s.close()
+
+
+def sparse_hund(hund, d):
+    if not hund:
+        return sp.csr_matrix((d, d))
+    i, j, h = zip(*hund)
+    return sp.coo_matrix((h, (i, j)), shape=(d, d)).tocsr()
+
+
 def test_sz_rejections(gpu):
     from ed_solver_amd import EDError, apply_C
     m, pb, pvbc = sc_model(2, 2, 2.0, seed=1, u_offdiag=1.0)
-    mj = AIMParams(m.nc, m.nb, m.ec, m.eb, m.vbc, m.u, np.full((2, 2), 0.4), 0.4, 1e-10)
-    with pytest.raises(EDError):
-        gpu.SzSector(mj, 4, pb)
     s = gpu.SzSector(m, 4, pb)
     s2 = gpu.SzSector(m, 5, pb)
     with pytest.raises(EDError):

@@ -76,3 +76,26 @@ def test_no_pairing_reduces_to_spin_sectors():
     e_n = np.sort(np.linalg.eigvalsh(osz.dense_sz(osz.tab_sz(m, npart))) + osz.energy_shift(m))
     parts = [np.linalg.eigvalsh(bf.sector_hamiltonian(m, n, n).toarray()) for n in range(m.ns + 1)]
     assert np.max(np.abs(e_n - np.sort(np.concatenate(parts)))) < 1e-11
+
+
+@pytest.mark.parametrize("nc,nb,npart,pair_ib", [(2, 2, 4, False), (2, 2, 4, True), (3, 2, 4, True), (2, 3, 5, True),
+                                                 (3, 2, 5, False)])
+def test_hund_terms_match_physical_spectrum(nc, nb, npart, pair_ib):
+    """The spin-flip and pair-hopping terms of HAIMsz_multc and the -J parallel-spin terms
+    of QUART_INT_SZ: with them the Nambu spectrum + shift equals the physical spectrum of
+    a Kanamori impurity (U, U' = U - 2J, J) with pairing; without them, or with their sign
+    flipped, it does not."""
+    m0, pb, pvbc = sc_model(nc, nb, 3.0, seed=7 + nb + npart, u_offdiag=1.6, spin_split=0.05)
+    jj = np.full((nc, nc), 0.6)
+    np.fill_diagonal(jj, 0.0)
+    m = AIMParams(nc, nb, m0.ec, m0.eb, m0.vbc, m0.u, jj, 0.6, m0.cutoff)
+    P = pb
+    tab = osz.tab_sz(m, npart, P, pvbc if pair_ib else None, pair_ib)
+    hund = osz.hund_sz(m, npart)
+    assert len(hund) > 0
+    ep = np.sort(osz.physical_spectrum(m, npart, P, pvbc if pair_ib else None, pair_ib))
+    e = np.sort(np.linalg.eigvalsh(osz.dense_sz(tab, hund))) + osz.energy_shift(m)
+    assert np.max(np.abs(e - ep)) < 1e-12
+    for bad in ([], [(i, j, -h) for i, j, h in hund]):
+        eb = np.sort(np.linalg.eigvalsh(osz.dense_sz(tab, bad))) + osz.energy_shift(m)
+        assert np.max(np.abs(eb - ep)) > 1e-3

@@ -152,6 +152,17 @@ __global__ __launch_bounds__(kBlock) void k_hv_cplx(CplxArgs a, const double *__
     hv_cplx_body<MODE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
 }
 
+// S_z sectors, on-the-fly entries (HAIMsz_mult_fly), one thread per row.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_sz_fly(SzArgs<T> a, SpinModel m, const int32_t *states,
+                                                       const int32_t *rank, const T *__restrict__ xin,
+                                                       T *__restrict__ yout, int64_t ld, LzArgs<T> lz)
+{
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_sz_fly_body<T, MODE>(a, m, states, rank, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
+}
+
 // S_z sectors: HAIMsz_multr, one thread per row.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_hv_sz(SzArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
@@ -525,6 +536,11 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
         za.kh = s->sz_kh;
         za.hrk = s->d_sz_hrk;
         za.hval = (const T *)(f64 ? (const void *)s->d_sz_hval : (const void *)s->d_sz_hval32);
+        if (s->sz_fly) {
+            hipLaunchKernelGGL((k_hv_sz_fly<T, MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za,
+                               s->sz_model, (const int32_t *)s->up.states, (const int32_t *)s->d_rank, x, y, ld, lz);
+            return;
+        }
         hipLaunchKernelGGL((k_hv_sz<T, MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za, x, y, ld, lz);
         return;
     }
@@ -1389,6 +1405,15 @@ int ed_hmult_set_schedule(ed_sector *sec, int choice)
     sec->cpb = choice;
     sec->cpb32 = 0;   // FP32 follows the forced schedule too
     sec->persist_ok[0] = sec->persist_ok[1] = -1;
+    drop_graphs(sec);
+    return ED_OK;
+}
+
+int ed_sector_set_fly(ed_sector *sec, int on)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!sec->sz) return fail(ED_ERR_ARG, "ed_sector_set_fly: S_z sectors only (HAIMsz_mult_fly)");
+    sec->sz_fly = on ? 1 : 0;
     drop_graphs(sec);
     return ED_OK;
 }

@@ -69,6 +69,21 @@ struct LzState {
     unsigned bar_rel[kBarGroups * 16];   // barriers released in group g: bar_rel[16 g]
 };
 
+// Quadratic model tabulated by tab_HAIM2: nimp impurity and nbath bath orbitals, their
+// bit positions in the state word, and device copies of the row-major matrices
+// ec[nimp][nimp], eb[nbath][nbath], vbc[nbath][nimp].  One spin of the (N_up, N_dn)
+// basis: nimp = Nc at bits nb + c, nbath = Nb at bits b.  The Nambu basis of the S_z
+// sectors: 2 Nc and 2 Nb orbitals over 2 Ns bits (AIM2%IMPiorb / BATHiorb,
+// quantum_algebra_Anderson_Impurity_class2.f90:83-89).
+constexpr int kMaxImpOrb = 12, kMaxBathOrb = 28;
+struct SpinModel {
+    int nimp, nbath;
+    int ibit[kMaxImpOrb], bbit[kMaxBathOrb];
+    const double *ec, *eb, *vbc;
+    const double *ec_im, *eb_im, *vbc_im;   // imaginary parts of a complex model (null: real)
+    double cut;
+};
+
 // One spin's tables on the device (tab_HAIM2 output plus derived layouts).
 struct SpinTables {
     int d = 0;              // spin-sector dimension
@@ -210,6 +225,11 @@ struct ed_sector {
     int32_t *d_sz_hrk = nullptr;
     double *d_sz_hval = nullptr;
     float *d_sz_hval32 = nullptr;
+    // HAIMsz_mult_fly: the Nambu matrices on the device (d_sz_mat) and the sector's
+    // tab_HAIM2 model over them; sz_fly = 1 generates the entries in the H.v
+    ed::SpinModel sz_model{};
+    double *d_sz_mat = nullptr;
+    int sz_fly = 0;
     double energy_shift = 0.0;
     int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
     double *h_mon = nullptr;       // monitored persistent run: pinned fine-grained host channel

@@ -522,6 +522,31 @@ __device__ inline void lz_finish_body(const LzArgs<T> &lz, double *sh)
     }
 }
 
+// ------------------------------------------------------------- Fock-space hops
+
+// Sign of c+/c on `bit` = (-1)^{# occupied orbitals above bit} of the resulting state
+// (Fermion_sign, quantum_fermion_ket_class.f90:605-618).  Bits above `bit` do not
+// change under the operator, so the input state can be used.
+__device__ inline int sign_above(uint32_t s, int bit)
+{
+    return (__popc(s >> (bit + 1)) & 1) ? -1 : 1;
+}
+
+// hop(out, to, from, in) = c+_to c_from: destroy then create (ket_class.f90:483-492).
+__device__ inline bool hop(uint32_t s, int to, int from, uint32_t *out, int *sg)
+{
+    if (!((s >> from) & 1u)) return false;
+    uint32_t t = s & ~(1u << from);
+    int g = sign_above(t, from);
+    if ((t >> to) & 1u) return false;
+    uint32_t o = t | (1u << to);
+    g *= sign_above(o, to);
+    *out = o;
+    *sg = g;
+    return true;
+}
+
+
 // ------------------------------------------------------------- S_z (Nambu) sectors
 
 // Tables of an S_z sector: ELL [kw][d] of tab_HAIM2 over the Nambu orbitals, the
@@ -648,6 +673,64 @@ __device__ inline void hv_cplx_body(const CplxArgs &a, const double *__restrict_
         y[i] = yr;
         y[D + i] = yi;
         dot += yr * xr + yi * xi;
+    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+}
+
+// HAIMsz_mult_fly (HAIMsz.f90:615-1027): the S_z H.v with each row's tab_HAIM2 entries
+// generated on the fly from the Nambu matrices (k_tab's loops: impurity hops, bath hops,
+// the hybridisation both ways, each kept when nonzero and above the cutoff) instead of
+// read from the ELL.  Per row it reads the state word and a rank lookup per target
+// instead of kw (rank, value) pairs.  The entries come in the table's order with the
+// table's values (t sign, rounded to T), so the result has the table path's bits.
+template <typename T, int MODE>
+__device__ inline void hv_sz_fly_body(const SzArgs<T> &a, const SpinModel &m, const int32_t *__restrict__ states,
+                                      const int32_t *__restrict__ rank, const T *__restrict__ xin,
+                                      T *__restrict__ yout, const LzArgs<T> &lz, int blk, double *red)
+{
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int64_t d = a.d, stride = (int64_t)gridDim.x * blockDim.x;
+    const int ni = m.nimp, nbt = m.nbath;
+    double dot = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < d; i += stride) {
+        const uint32_t s = (uint32_t)states[i];
+        const T xv = x[i];
+        T acc = a.quart[i] * xv;
+        acc += a.diag[i] * xv;
+        uint32_t o;
+        int sg;
+        for (int p = 0; p < ni; ++p)
+            for (int q = 0; q < ni; ++q) {
+                const double t = m.ec[p * ni + q];
+                if (p == q || t == 0.0 || !(fabs(t) > m.cut)) continue;
+                if (hop(s, m.ibit[p], m.ibit[q], &o, &sg)) acc += (T)(t * sg) * x[rank[o]];
+            }
+        for (int p = 0; p < nbt; ++p)
+            for (int q = 0; q < nbt; ++q) {
+                const double t = m.eb[p * nbt + q];
+                if (p == q || t == 0.0 || !(fabs(t) > m.cut)) continue;
+                if (hop(s, m.bbit[p], m.bbit[q], &o, &sg)) acc += (T)(t * sg) * x[rank[o]];
+            }
+        for (int b = 0; b < nbt; ++b)
+            for (int c = 0; c < ni; ++c) {
+                const double t = m.vbc[b * ni + c];
+                if (t == 0.0 || !(fabs(t) > m.cut)) continue;
+                if (hop(s, m.bbit[b], m.ibit[c], &o, &sg)) acc += (T)(t * sg) * x[rank[o]];
+                if (hop(s, m.ibit[c], m.bbit[b], &o, &sg)) acc += (T)(t * sg) * x[rank[o]];
+            }
+        for (int k = 0; k < a.kh; ++k) acc += a.hval[k * d + i] * x[a.hrk[k * d + i]];
+        y[i] = acc;
+        dot += (double)acc * (double)xv;
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }

@@ -16,7 +16,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "ed_internal.h"
+#include "ed_kernels.h"
 
 namespace ed {
 
@@ -27,28 +27,6 @@ __host__ __device__ inline int binom(int n, int k)
     int r = 1;
     for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
     return r;
-}
-
-// Sign of c+/c on `bit` = (-1)^{# occupied orbitals above bit} of the resulting state
-// (Fermion_sign, quantum_fermion_ket_class.f90:605-618).  Bits above `bit` do not
-// change under the operator, so the input state can be used.
-__device__ inline int sign_above(uint32_t s, int bit)
-{
-    return (__popc(s >> (bit + 1)) & 1) ? -1 : 1;
-}
-
-// hop(out, to, from, in) = c+_to c_from: destroy then create (ket_class.f90:483-492).
-__device__ inline bool hop(uint32_t s, int to, int from, uint32_t *out, int *sg)
-{
-    if (!((s >> from) & 1u)) return false;
-    uint32_t t = s & ~(1u << from);
-    int g = sign_above(t, from);
-    if ((t >> to) & 1u) return false;
-    uint32_t o = t | (1u << to);
-    g *= sign_above(o, to);
-    *out = o;
-    *sg = g;
-    return true;
 }
 
 __global__ void k_enum_states(int ns, int n, int d, int32_t *states)
@@ -76,21 +54,6 @@ __global__ void k_rank_table(int ns, int32_t *rank)
         if ((s >> c) & 1) r += binom(c, ++i);
     rank[s] = r;
 }
-
-// Quadratic model tabulated by tab_HAIM2: nimp impurity and nbath bath orbitals, their
-// bit positions in the state word, and device copies of the row-major matrices
-// ec[nimp][nimp], eb[nbath][nbath], vbc[nbath][nimp].  One spin of the (N_up, N_dn)
-// basis: nimp = Nc at bits nb + c, nbath = Nb at bits b.  The Nambu basis of the S_z
-// sectors: 2 Nc and 2 Nb orbitals over 2 Ns bits (AIM2%IMPiorb / BATHiorb,
-// quantum_algebra_Anderson_Impurity_class2.f90:83-89).
-constexpr int kMaxImpOrb = 12, kMaxBathOrb = 28;
-struct SpinModel {
-    int nimp, nbath;
-    int ibit[kMaxImpOrb], bbit[kMaxBathOrb];
-    const double *ec, *eb, *vbc;
-    const double *ec_im, *eb_im, *vbc_im;   // imaginary parts of a complex model (null: real)
-    double cut;
-};
 
 static SpinModel spin_model(int nc, int nb, const double *ec, const double *eb, const double *vbc, double cut)
 {
@@ -492,7 +455,7 @@ static void free_sector(ed_sector *S)
     free_spin(S->dn);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
                   S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord,
-                  S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32};
+                  S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32, S->d_sz_mat};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (ed::GraphCache &G : S->graphs) {
@@ -782,6 +745,10 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     sm.eb = d_eb;
     sm.vbc = d_vbc;
     sm.cut = m->cutoff;
+    // the sector keeps the Nambu matrices for the on-the-fly H.v (HAIMsz_mult_fly)
+    S->sz_model = sm;
+    S->d_sz_mat = d_mat;
+    d_mat = nullptr;
     hipLaunchKernelGGL(k_tab<false>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
                        (const int32_t *)nullptr, T.noff, (double *)nullptr, (int32_t *)nullptr, (double *)nullptr,
                        (double *)nullptr);

@@ -387,6 +387,10 @@ class SzSector(Sector):
         check(lib().ed_sector_quart_sz(self.handle, dptr(q)))
         return q
 
+    def set_fly(self, on: bool) -> None:
+        """ON_FLY: generate the H.v entries on the fly (HAIMsz_mult_fly) instead of the table."""
+        check(lib().ed_sector_set_fly(self.handle, 1 if on else 0))
+
     def hund(self):
         """Hund transitions of HAIMsz_multc as (rk[kh, dim], val[kh, dim]) (kh = 0 without J)."""
         kh = ctypes.c_int()

@@ -76,6 +76,12 @@ typedef struct ed_sector ed_sector;
 int ed_sector_create_sz(const ed_model *model, const double *pb, const double *pvbc, int pairing_imp_to_bath,
                         int npart, ed_sector **out);
 
+/* ON_FLY (H_class.f90:84-93 -> HAIMsz_mult_fly, HAIMsz.f90:615-1027) for an S_z sector:
+ * on = 1 makes every later H.v generate each row's tab_HAIM2 entries from the Nambu
+ * matrices instead of reading the stored table (same entries, same order, same bits);
+ * on = 0 returns to the table. */
+int ed_sector_set_fly(ed_sector *sec, int on);
+
 /* Hund transitions of an S_z sector as an ELL: *kh per state (0 without J), rk[kh][dim]
  * target ranks and val[kh][dim] = -J sign in the order of HAIMsz_multc (HAIMsz.f90:300-406),
  * rows padded with the row itself and 0.  rk / val may be NULL (query *kh). */

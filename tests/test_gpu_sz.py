@@ -202,3 +202,40 @@ def test_sz_sector_above_two_million_matches_updo_sectors(gpu):
             e_updo.append(float(sk.lanczos(nitermax=400, neigen=1, tolerance=1e-12).evals[0]))
         sk.close()
     assert abs(e_sz - min(e_updo)) < 1e-9, (e_sz, min(e_updo))
+
+
+@pytest.mark.parametrize("case", [(2, 3, 5, True, 0.6), (3, 3, 6, True, 0.0), (2, 6, 8, False, 0.5)])
+def test_sz_on_the_fly_matches_table(gpu, case):
+    """HAIMsz_mult_fly: the entries generated in the H.v give the table path's bits (H.v in
+    FP64, fixed-length Lanczos runs in FP64 and FP32)."""
+    nc, nb, npart, pair_ib, J = case
+    m0, pb, pvbc = sc_model(nc, nb, 3.0, seed=40 + nb, u_offdiag=1.6)
+    m = with_hund(m0, J) if J else m0
+    s = gpu.SzSector(m, npart, pb, pvbc, pair_ib)
+    print(f"S_z fly {case}: dim {s.dim}", flush=True)
+    v = np.random.default_rng(8).uniform(-1, 1, s.dim)
+    w0 = s.hmult(v)
+    a0 = [s.lanczos_run(40, precision=p) for p in ("f64", "f32")]
+    s.set_fly(True)
+    assert np.array_equal(s.hmult(v), w0)
+    a1 = [s.lanczos_run(40, precision=p) for p in ("f64", "f32")]
+    for (x0, y0), (x1, y1) in zip(a0, a1):
+        assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
+    s.set_fly(False)
+    assert np.array_equal(s.hmult(v), w0)
+    s.close()
+
+
+def test_sz_on_the_fly_large_sector_rate(gpu):
+    """The 2.7M-state S_z sector of Ns = 12: table and on-the-fly H.v agree bitwise; both
+    times printed."""
+    m, _, _ = sc_model(3, 9, 2.0, seed=21, u_offdiag=1.0)
+    s = gpu.SzSector(m, 12)
+    v = np.random.default_rng(9).uniform(-1, 1, s.dim)
+    w0 = s.hmult(v)
+    t0, _ = s.hmult_benchmark("f64", 20)
+    s.set_fly(True)
+    assert np.array_equal(s.hmult(v), w0)
+    t1, _ = s.hmult_benchmark("f64", 20)
+    print(f"S_z dim {s.dim}: H.v table {t0 * 1e3:.1f} us, on the fly {t1 * 1e3:.1f} us", flush=True)
+    s.close()

@@ -718,8 +718,9 @@ static const void *persist_fn(bool mon)
 }
 
 // The instances compiled: those whose rows fit the 128-register budget of a co-resident
-// grid (4 workgroups of 256 threads per CU) without spilling in the step loop; wider
-// tables with more rows per thread keep the two-kernel step.
+// grid (4 workgroups of 256 threads per CU); width 12 with 4 rows per thread (the excited
+// sectors of the G(i w_n) runs at Ns = 12) reloads its packed entries every step and
+// keeps its down-hop entries in LDS to get there.  Wider tables keep the two-kernel step.
 template <typename T>
 static const void *persist_kernel(int k, int rows, bool mon)
 {
@@ -733,6 +734,7 @@ static const void *persist_kernel(int k, int rows, bool mon)
     case 8 * 8 + 4: return persist_fn<T, 8, 4>(mon);
     case 12 * 8 + 1: return persist_fn<T, 12, 1>(mon);
     case 12 * 8 + 2: return f32 ? persist_fn<float, 12, 2>(mon) : nullptr;
+    case 12 * 8 + 4: return persist_fn<T, 12, 4>(mon);
     case 16 * 8 + 1: return persist_fn<T, 16, 1>(mon);
     default: return nullptr;
     }

@@ -870,7 +870,21 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
         jd[k] = (act && k < nd) ? a.rk_dn[a.ptr_dn[ido] + k] * dup : 0;
     }
     const int base = act ? ido * dup : 0;
-    uint32_t pk[R][K];
+    // Wide tables with many rows per thread (K R > 40) do not keep the packed up-hop
+    // entries in registers across the steps: each step reloads them (read-only, L2
+    // resident) with the neighbour loads of its row batch; the column's down-hop entries
+    // sit in LDS instead of scalar registers.
+    constexpr bool PKG = K * R > 40;
+    __shared__ T s_od[PKG ? K : 1];
+    __shared__ int s_jd[PKG ? K : 1];
+    if constexpr (PKG) {
+        if (tid < K) {
+            s_od[tid] = (act && tid < nd) ? a.off_dn[a.ptr_dn[ido] + tid] : (T)0;
+            s_jd[tid] = (act && tid < nd) ? a.rk_dn[a.ptr_dn[ido] + tid] * dup : 0;
+        }
+    }
+    constexpr int RP = PKG ? 1 : R;
+    uint32_t pk[RP][K];
     int cu[R];
     {
         const T *x0 = lz.slot(j);
@@ -880,8 +894,10 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
             const int r = tid + i * NT;
             const bool ok = act && r < dup;
             cu[i] = ok ? a.imp_up[r] : 0;
+            if constexpr (!PKG) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) pk[i][k] = ok ? a.ell_pk_up[k * dup + r] : 0u;
+                for (int k = 0; k < K; ++k) pk[i][k] = ok ? a.ell_pk_up[k * dup + r] : 0u;
+            }
             if (ok) {
                 dgl[r] = (a.diag_up[r] + ddg) + a.uimp[cu[i] * ncfg + cd];
                 col[r] = x0[base + r];
@@ -920,31 +936,40 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
             // neighbour columns through uniform per-hop pointers: one 32-bit row offset
             // per row serves every hop.  Rows go in batches of RB (the loads of a batch
             // in flight together) to stay within the register budget of a co-resident grid.
-            const T *xk[K];
+            const T *xk[PKG ? 1 : K];
+            if constexpr (!PKG) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) xk[k] = x + jd[k];
-            constexpr int RB = R < 2 ? R : 2;
+                for (int k = 0; k < K; ++k) xk[k] = x + jd[k];
+            }
+            constexpr int RB = (R < 2 || PKG) ? 1 : 2;
 #pragma unroll
             for (int i0 = 0; i0 < R; i0 += RB) {
                 T xd[RB][K];
+                uint32_t pkb[PKG ? RB : 1][K];
 #pragma unroll
                 for (int b = 0; b < RB; ++b) {
                     const unsigned r = (unsigned)(tid + (i0 + b) * NT);
                     const bool ok = act && (int)r < dup;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) xd[b][k] = (ok && k < nd) ? ld_agent(xk[k] + r) : (T)0;
+                    for (int k = 0; k < K; ++k)
+                        xd[b][k] = (ok && k < nd) ? ld_agent((PKG ? x + s_jd[k] : xk[PKG ? 0 : k]) + r) : (T)0;
+                    if constexpr (PKG) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k) pkb[b][k] = ok ? a.ell_pk_up[k * dup + r] : 0u;
+                    }
                 }
 #pragma unroll
                 for (int b = 0; b < RB; ++b) {
                     const int i = i0 + b, r = tid + i * NT;
+                    const uint32_t *pr = PKG ? pkb[PKG ? b : 0] : pk[PKG ? 0 : i];
                     z[i] = (T)0;
                     if (act && r < dup) {
                         const T xv = col[r];
                         T acc = dgl[r] * xv;
 #pragma unroll
-                        for (int k = 0; k < K; ++k) acc += coef[pk[i][k] >> 16] * col[pk[i][k] & 0xFFFFu];
+                        for (int k = 0; k < K; ++k) acc += coef[pr[k] >> 16] * col[pr[k] & 0xFFFFu];
 #pragma unroll
-                        for (int k = 0; k < K; ++k) acc += od[k] * xd[b][k];
+                        for (int k = 0; k < K; ++k) acc += (PKG ? s_od[k] : od[k]) * xd[b][k];
                         if (a.hund) {
                             const int h0 = hptr[cu[i] * ncfg + cd], h1 = hptr[cu[i] * ncfg + cd + 1];
                             for (int t = h0; t < h1; ++t) {
@@ -956,6 +981,9 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
                         dot += (double)acc * (double)xv;
                     }
                 }
+                // keep the scheduler from hoisting the next batch's loads into this one
+                // (it would hold every batch's loads at once and spill)
+                if constexpr (PKG) __builtin_amdgcn_sched_barrier(0);
             }
         }
         block_partial2<true>(dot, 0.0, red, lz.partA + blk, nullptr);

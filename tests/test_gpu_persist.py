@@ -66,10 +66,10 @@ def test_persistent_vs_oracle(gpu, name, sched):
     s.close()
 
 
-def test_persistent_long_run_and_excluded_width(gpu):
-    """150 steps in one launch on workload C; the excited sector (7,6) (d_up 792, ELL
-    width 12, four rows per thread) is outside the compiled instances and keeps the
-    two-kernel step."""
+def test_persistent_long_run_and_wide_table(gpu):
+    """150 steps in one launch on workload C; the excited sector (7,6) of its G(i w_n) runs
+    (d_up 792, ELL width 12, four rows per thread: the instance that reloads its packed
+    up-hop entries every step) against the two-kernel step, FP64 and FP32."""
     m, (nu, nd) = golden("C")
     s = gpu.Sector(m, nu, nd)
     s.set_schedule(20)
@@ -80,7 +80,13 @@ def test_persistent_long_run_and_excluded_width(gpu):
     s.close()
     s = gpu.Sector(m, 7, 6)
     s.set_schedule(20)
-    assert not s.persistent_active("f64") and not s.persistent_active("f32")
+    print(f"(7,6): dim {s.dim}", flush=True)
+    for prec in ("f64", "f32"):
+        s.set_persistent(True)
+        assert s.persistent_active(prec)
+        a1, b1 = run(s, 60, True, prec)
+        a0, b0 = run(s, 60, False, prec)
+        assert np.array_equal(a1, a0) and np.array_equal(b1, b0), prec
     s.close()
 
 

@@ -149,23 +149,35 @@ def main():
             secs.append(s_p); phis.append(ed.apply_C(sec, s_p, site, 0, +1, psi)); iphs.append(1)
             secs.append(s_h); phis.append(ed.apply_C(sec, s_h, site, 0, -1, psi)); iphs.append(2)
         ens = [E0] * len(secs)
+        jobs = [(secs[k], iphs[k], [(k // 2, 1.0)]) for k in range(len(secs))]
         ed.compute_dynamic_batched(secs, phis, iphs, ens, 36, wn)   # warm-up
+        ed.dynamic_from_state(sec, psi, E0, jobs, 0, 36, wn)
         reps = 3
+        # one by one from host excitation vectors (the reference's call pattern)
         t0 = time.perf_counter()
         for _ in range(reps):
             for k in range(len(secs)):
                 ed.compute_dynamic(secs[k], iphs[k], E0, phis[k], 36, wn)
         t_plain = (time.perf_counter() - t0) / reps
+        # batched from host excitation vectors
         t0 = time.perf_counter()
         for _ in range(reps):
-            dyn_b, _, _ = ed.compute_dynamic_batched(secs, phis, iphs, ens, 36, wn)
+            dyn_h, _, _ = ed.compute_dynamic_batched(secs, phis, iphs, ens, 36, wn)
+        t_host = (time.perf_counter() - t0) / reps
+        # batched from the ground state (psi copied once, c / c+ on the device): the solver's path
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dyn_b = ed.dynamic_from_state(sec, psi, E0, jobs, 0, 36, wn)
         t_batch = (time.perf_counter() - t0) / reps
         g = dyn_b.sum(axis=0) / model.nc
         green = {"solves": len(secs), "nitergreen": 36, "n_matsubara": 2048, "beta": 100.0,
                  "excited_dim": s_p.dim, "plain_solves_per_s": len(secs) / t_plain,
                  "batched_solves_per_s": len(secs) / t_batch, "batched_vs_plain": t_plain / t_batch,
+                 "batched_host_vectors_solves_per_s": len(secs) / t_host,
+                 "max_abs_diff_device_vs_host_vectors": float(np.max(np.abs(dyn_b - dyn_h))),
                  "G_iw0": [float(g[0].real), float(g[0].imag)]}
-        log(f"G(iw): plain {len(secs)/t_plain:.1f} solves/s, batched {len(secs)/t_batch:.1f} solves/s")
+        log(f"G(iw): plain {len(secs)/t_plain:.1f} solves/s, batched from host vectors {len(secs)/t_host:.1f}, "
+            f"batched from the state on the device {len(secs)/t_batch:.1f} solves/s")
         s_p.close(); s_h.close()
 
     # mixed precision at the accuracy the target is stated for (|dE0| < 1e-8): both

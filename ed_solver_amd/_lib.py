@@ -113,6 +113,9 @@ SIGNATURES = {
     "ed_dynamic_batched": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_dp), _c_int_p, _dp,
                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                           _dp]),
+    "ed_dynamic_from_state": (ctypes.c_int, [_vp, _dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp), _c_int_p,
+                                             _c_int_p, _c_int_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_hmult_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ed_lanczos_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,

@@ -96,6 +96,17 @@ __global__ __launch_bounds__(kBlock) void k_finish_batch(const JobDesc<T> *jobs)
 }
 
 // phi_k / ||phi_k|| for every job into its start area.
+// Partial sums of ||x||^2 per workgroup (grid-stride, fixed order); the host adds them.
+__global__ __launch_bounds__(kBlock) void k_norm2_part(const double *x, int64_t n, double *part)
+{
+    __shared__ double red[kRed];
+    double v = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += x[i] * x[i];
+    v = block_sum_all(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 __global__ void k_scale_into(const double *x, double *y, int64_t n, double s)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -163,7 +174,7 @@ static void launch_hv_batch(int total, size_t lds, hipStream_t st, const JobDesc
 template <typename T>
 static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<const double *> &h_phi,
                      const std::vector<double> &nrm2, int niter, hipStream_t st, std::vector<double> &a_out,
-                     std::vector<double> &b_out)
+                     std::vector<double> &b_out, bool phi_on_device)
 {
     const int nj = (int)secs.size();
     // ---- layout of the work space
@@ -212,7 +223,8 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         J.lz.dup = s->up.d;
         ku = std::max(ku, s->up.kpad);
         lds = std::max(lds, (size_t)s->up.d * sizeof(T));
-        ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k] + 0, h_phi[k], sizeof(double) * s->dim, hipMemcpyHostToDevice, st));
+        ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k] + 0, h_phi[k], sizeof(double) * s->dim,
+                                    phi_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_scale_into, dim3(egrid(s->dim)), dim3(kBlock), 0, st, (const double *)(base + o_start[k]),
                            (double *)(base + o_start[k]), s->dim, 1.0 / std::sqrt(nrm2[k]));
     }
@@ -261,9 +273,35 @@ extern "C" int ed_dynamic(ed_sector *sec, int iph, double energy, const double *
                           double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn, double *a,
                           double *b);
 
-extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
-                                  const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
-                                  const double *wn, double *dyn, double *a, double *b)
+namespace ed {
+int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_phi, double nrm2, int nitergreen,
+                double cutoff_dynamic, int prec, int nw, const double *d_wn, double *dyn, double *a_out, double *b_out);
+}
+
+// ||phi_k||^2 of device vectors: partial sums on the device, added on the host in order.
+static int device_norms(int njobs, ed_sector *const *secs, const double *const *phis, std::vector<double> &nrm2)
+{
+    constexpr int kParts = 256;
+    hipStream_t st = secs[0]->stream;
+    double *d_part = nullptr;
+    ED_HIP_CHECK(hipMallocAsync((void **)&d_part, sizeof(double) * kParts * njobs, st));
+    for (int k = 0; k < njobs; ++k)
+        hipLaunchKernelGGL(k_norm2_part, dim3(kParts), dim3(kBlock), 0, st, phis[k], secs[k]->dim, d_part + k * kParts);
+    std::vector<double> part((size_t)kParts * njobs);
+    ED_HIP_CHECK(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFreeAsync(d_part, st);
+    for (int k = 0; k < njobs; ++k) {
+        double v = 0.0;
+        for (int q = 0; q < kParts; ++q) v += part[(size_t)k * kParts + q];
+        nrm2[k] = v;
+    }
+    return ED_OK;
+}
+
+static int dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
+                           const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
+                           const double *wn, double *dyn, double *a, double *b, bool phi_on_device)
 {
     if (njobs < 1 || !secs || !phis || !iph || !energy || !wn || !dyn || nw < 1 || nitergreen < 1)
         return fail(ED_ERR_ARG, "ed_dynamic_batched: bad argument");
@@ -281,17 +319,32 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
     // Group: all jobs with the same Lanczos length that the batched kernel can take run
     // together; the others (short columns, very wide tables) run one by one.
     std::vector<double> nrm2(njobs, 0.0);
-    for (int k = 0; k < njobs; ++k) {
-        double s = 0.0;
-        for (int64_t i = 0; i < secs[k]->dim; ++i) s += phis[k][i] * phis[k][i];
-        nrm2[k] = s;
+    if (phi_on_device) {
+        ED_TRY(device_norms(njobs, secs, phis, nrm2));
+    } else {
+        for (int k = 0; k < njobs; ++k) nrm2[k] = host_norm2(phis[k], secs[k]->dim);
     }
     std::vector<int> batch;
+    double *d_wn1 = nullptr;   // frequencies on the device for the one-by-one runs of device vectors
     for (int k = 0; k < njobs; ++k) {
         if (nrm2[k] < cutoff_dynamic) continue;   // dyn = 0 (green_class_compute_dynamic.f90:263-265)
         const int niter = (int)std::min<int64_t>(nitergreen, secs[k]->cplx ? secs[k]->cdim : secs[k]->dim);
         if (rows_capable(secs[k]) && niter == nitergreen) {
             batch.push_back(k);
+        } else if (phi_on_device) {
+            hipStream_t sk = secs[k]->stream;
+            if (!d_wn1) {
+                ED_HIP_CHECK(hipMalloc((void **)&d_wn1, sizeof(double) * nw));
+                ED_HIP_CHECK(hipMemcpy(d_wn1, wn, sizeof(double) * nw, hipMemcpyHostToDevice));
+            }
+            ED_HIP_CHECK(hipStreamSynchronize(sk));
+            int rc = dynamic_dev(secs[k], iph[k], energy[k], phis[k], nrm2[k], nitergreen, cutoff_dynamic, precision, nw,
+                                 d_wn1, dyn + (size_t)k * 2 * nw, a ? a + (size_t)k * nitergreen : nullptr,
+                                 b ? b + (size_t)k * nitergreen : nullptr);
+            if (rc != ED_OK) {
+                (void)hipFree(d_wn1);
+                return rc;
+            }
         } else {
             int rc = ed_dynamic(secs[k], iph[k], energy[k], phis[k], nitergreen, cutoff_dynamic, precision, nw, wn,
                                 dyn + (size_t)k * 2 * nw, a ? a + (size_t)k * nitergreen : nullptr,
@@ -299,6 +352,7 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
             if (rc != ED_OK) return rc;
         }
     }
+    if (d_wn1) (void)hipFree(d_wn1);
     if (batch.empty()) return ED_OK;
     const int nb = (int)batch.size();
     std::vector<ed_sector *> bs(nb);
@@ -312,8 +366,8 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
     hipStream_t st = bs[0]->stream;
     const int niter = nitergreen;
     std::vector<double> ab, bb;
-    int rc = precision == ED_PREC_F64 ? run_batch<double>(bs, bp, bn, niter, st, ab, bb)
-                                      : run_batch<float>(bs, bp, bn, niter, st, ab, bb);
+    int rc = precision == ED_PREC_F64 ? run_batch<double>(bs, bp, bn, niter, st, ab, bb, phi_on_device)
+                                      : run_batch<float>(bs, bp, bn, niter, st, ab, bb, phi_on_device);
     if (rc != ED_OK) return rc;
     // shifted tridiagonals: ph (alpha - E) (green_class_compute_dynamic.f90:310-311)
     std::vector<double> sa((size_t)nb * niter), scale(nb);
@@ -347,3 +401,86 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
     (void)hipStreamSynchronize(st);
     return rc;
 }
+
+extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
+                                  const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
+                                  const double *wn, double *dyn, double *a, double *b)
+{
+    return dynamic_batched(njobs, secs, phis, iph, energy, nitergreen, cutoff_dynamic, precision, nw, wn, dyn, a, b,
+                           false);
+}
+
+namespace ed {
+int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *d_psi,
+                double *d_phi, hipStream_t st);
+}
+
+// y += s x
+__global__ void k_axpy(double s, const double *x, double *y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] += s * x[i];
+}
+
+extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs, ed_sector *const *dst,
+                                     const int *iph, const int *nterm, const int *sites, const double *coefs, int spin,
+                                     int nitergreen, double cutoff_dynamic, int precision, int nw, const double *wn,
+                                     double *dyn)
+{
+    if (!gs || !psi || njobs < 1 || !dst || !iph || !nterm || !sites || !coefs || !dyn)
+        return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad argument");
+    if (gs->sz || gs->cplx) return fail(ED_ERR_ARG, "ed_dynamic_from_state: real (N_up, N_dn) sectors only");
+    ED_HIP_CHECK(hipSetDevice(gs->device));
+    hipStream_t st = gs->stream;
+    // psi once on the device, then every excitation vector built there
+    size_t tot = (size_t)gs->dim, maxd = 1;
+    for (int k = 0; k < njobs; ++k) {
+        if (!dst[k] || nterm[k] < 1) return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad job");
+        tot += (size_t)dst[k]->dim;
+        maxd = std::max<size_t>(maxd, (size_t)dst[k]->dim);
+    }
+    double *d = nullptr;
+    ED_HIP_CHECK(hipMalloc((void **)&d, sizeof(double) * (tot + maxd)));
+    double *d_psi = d, *d_tmp = d + tot;
+    std::vector<const double *> phis(njobs);
+    int rc = ED_OK;
+    if (hipMemcpyAsync(d_psi, psi, sizeof(double) * gs->dim, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = fail(ED_ERR_HIP, "ed_dynamic_from_state: copy of psi");
+    size_t off = gs->dim;
+    int t0 = 0;
+    for (int k = 0; k < njobs && rc == ED_OK; ++k) {
+        double *phi = d + off;
+        off += dst[k]->dim;
+        phis[k] = phi;
+        const int pm = iph[k] == 1 ? +1 : -1;
+        if (iph[k] != 1 && iph[k] != 2) rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: iph must be 1 or 2");
+        // sum_t coef_t c^(pm)_{site_t} psi (the vectors of symmetric_combineAA,
+        // green_class_compute_symmetric_combination.f90:39-137)
+        for (int t = 0; t < nterm[k] && rc == ED_OK; ++t) {
+            const int site = sites[t0 + t];
+            const double c = coefs[t0 + t];
+            if (t == 0 && c == 1.0) {
+                rc = apply_c_dev(gs, dst[k], site, spin, pm, d_psi, phi, st);
+            } else {
+                rc = apply_c_dev(gs, dst[k], site, spin, pm, d_psi, d_tmp, st);
+                if (rc == ED_OK) {
+                    if (t == 0) ED_HIP_CHECK(hipMemsetAsync(phi, 0, sizeof(double) * dst[k]->dim, st));
+                    hipLaunchKernelGGL(k_axpy, dim3(egrid(dst[k]->dim)), dim3(kBlock), 0, st, c, (const double *)d_tmp,
+                                       phi, dst[k]->dim);
+                }
+            }
+        }
+        t0 += nterm[k];
+    }
+    if (rc == ED_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ED_ERR_HIP, "ed_dynamic_from_state: apply_C");
+    if (rc == ED_OK) {
+        std::vector<double> en(njobs, energy);
+        std::vector<ed_sector *> ds(dst, dst + njobs);
+        rc = dynamic_batched(njobs, ds.data(), phis.data(), iph, en.data(), nitergreen, cutoff_dynamic, precision, nw,
+                             wn, dyn, nullptr, nullptr, true);
+    }
+    (void)hipDeviceSynchronize();
+    (void)hipFree(d);
+    return rc;
+}
+

@@ -163,7 +163,7 @@ int cfrac_batch_dev(int n, const double *d_a, const double *d_b, int lda, int nj
 }
 
 // compute_dynamic_cpu for one excitation vector already on the device.
-static int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_phi, double nrm2, int nitergreen,
+int dynamic_dev(ed_sector *sec, int iph, double energy, const double *d_phi, double nrm2, int nitergreen,
                        double cutoff_dynamic, int prec, int nw, const double *d_wn, double *dyn, double *a_out,
                        double *b_out)
 {
@@ -323,7 +323,7 @@ int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int ni
     if (precision != ED_PREC_F64 && precision != ED_PREC_F32) return fail(ED_ERR_ARG, "ed_dynamic: bad precision");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     double nrm2 = 0.0;
-    for (int64_t i = 0; i < sec->dim; ++i) nrm2 += phi[i] * phi[i];
+    nrm2 = host_norm2(phi, sec->dim);
     hipStream_t st = sec->stream;
     double *d_phi = nullptr, *d_wn = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_phi, sizeof(double) * sec->dim, st));

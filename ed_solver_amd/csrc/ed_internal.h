@@ -246,6 +246,18 @@ struct ed_sector {
 namespace ed {
 // tridiagonal eigen-solver (host, FP64): implicit QL with Wilkinson shifts
 int tridiag_eig(int n, const double *diag, const double *off, double *w, double *z);
+// ||v||^2 of a host vector in eight interleaved partial sums (added in a fixed order), so
+// the loop vectorises: one pass at memory speed instead of a chain of dependent adds.
+inline double host_norm2(const double *v, int64_t n)
+{
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+        for (int q = 0; q < 8; ++q) p[q] += v[i + q] * v[i + q];
+    for (int q = 0; i < n; ++i, ++q) p[q] += v[i] * v[i];
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+}
+
 void tridiag_lowest(int n, const double *diag, const double *off, int k, double *w);
 void tridiag_lowest_warm(int n, const double *diag, const double *off, int k, const double *prev, double *w);
 int tridiag_lowest_pairs(int n, const double *diag, const double *off, int k, double *w, double *z);

@@ -493,6 +493,26 @@ def dynamic_full_ed(iph: int, energy: float, phi: np.ndarray, evals: np.ndarray,
     return out[0::2] + 1j * out[1::2]
 
 
+def dynamic_from_state(gs: Sector, psi: np.ndarray, energy: float, jobs, spin: int, nitergreen: int,
+                       wn: np.ndarray, cutoff_dynamic: float = 1e-8, precision="f64") -> np.ndarray:
+    """The excitation runs of one eigenstate with every vector on the device: jobs is a list
+    of (dst_sector, iph, [(site, coef), ...]); returns dyn (njobs, nw) complex."""
+    psi = gs._pack(psi, "dynamic_from_state: psi")
+    nj = len(jobs)
+    wn = np.ascontiguousarray(wn, np.float64)
+    ds = (ctypes.c_void_p * nj)(*[j[0].handle.value for j in jobs])
+    iph = np.array([int(j[1]) for j in jobs], np.int32)
+    nterm = np.array([len(j[2]) for j in jobs], np.int32)
+    sites = np.array([int(c) for j in jobs for c, _ in j[2]], np.int32)
+    coefs = np.array([float(a) for j in jobs for _, a in j[2]], np.float64)
+    dyn = np.zeros(2 * nj * len(wn))
+    check(lib().ed_dynamic_from_state(gs.handle, dptr(psi), float(energy), nj, ds, iptr(iph), iptr(nterm), iptr(sites),
+                                      dptr(coefs), int(spin), int(nitergreen), float(cutoff_dynamic), _prec(precision),
+                                      len(wn), dptr(wn), dptr(dyn)))
+    d = dyn.reshape(nj, len(wn), 2)
+    return d[..., 0] + 1j * d[..., 1]
+
+
 def compute_dynamic_batched(sectors, phis, iphs, energies, nitergreen: int, wn: np.ndarray,
                             cutoff_dynamic: float = 1e-8, precision="f64"):
     """All excitation runs in one call; returns dyn (njobs, nw) complex."""

@@ -12,8 +12,8 @@ libed_hip.so):
   (eigen_sector_class.f90:431-444, green_class_computeAA.h:107);
 * ``compute_green``       compute_greenAA (green_class_computeAA.f90:40-185,
   green_class_computeAA.h:101-302) for the diagonal impurity Green's function:
-  G(i w_n) = sum_states boltz * (particle + hole), all excitation runs of a state set
-  in one batched device call.
+  G(i w_n) = sum_states boltz * (particle + hole), each state's excitation vectors
+  built on the device from one copy of the state and its runs batched.
 """
 from __future__ import annotations
 
@@ -23,7 +23,7 @@ from math import comb
 import numpy as np
 
 from .aim import AIMParams
-from .hclass import Sector, apply_C, compute_dynamic_batched, dynamic_full_ed
+from .hclass import Sector, apply_C, dynamic_from_state, dynamic_full_ed
 
 FULL_ED_SWITCH = 40   # Global_Variables.f90:23
 
@@ -135,7 +135,8 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
     run from A|psi> (green_class_computeAA.h:101-302 with the vectors of
     symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137).  Small
     target sectors take the Lehmann sum over the held states, every other run is an
-    excitation Lanczos and all of those advance together in one batched device call."""
+    excitation Lanczos; a state's runs start from excitation vectors built on the device
+    and advance together in one batched device call."""
     w = partition(states, beta)
     G = np.zeros((len(ops), len(wn)), complex)
     sectors: dict[tuple[int, int], Sector] = {}
@@ -146,7 +147,6 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
             sectors[key] = Sector(model, nu, nd)
         return sectors[key]
 
-    jobs = []   # (operator index, weight, sector, phi, iph, energy)
     held: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
 
     def held_states(key):
@@ -161,26 +161,29 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
                 continue
             nu, nd = st.sector
             src = sector(nu, nd)
+            jobs = []   # (operator index, target sector, iph, operator) of the excitation runs
             for io, op in enumerate(ops):
                 for iph, pm in ((1, +1), (2, -1)):
                     tn = (nu + pm, nd) if spin == 0 else (nu, nd + pm)
                     if not (0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns):
                         continue
                     dst = sector(*tn)
-                    phi = None
-                    for c, coef in op:
-                        t = coef * apply_C(src, dst, c, spin, pm, st.vector)
-                        phi = t if phi is None else phi + t
                     if dst.dim <= full_ed_switch:
+                        phi = None
+                        for c, coef in op:
+                            t = coef * apply_C(src, dst, c, spin, pm, st.vector)
+                            phi = t if phi is None else phi + t
                         ev, vec = held_states(tn)
                         G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
                     else:
-                        jobs.append((io, wt, dst, phi, iph, st.energy))
-        if jobs:
-            dyn, _, _ = compute_dynamic_batched([j[2] for j in jobs], [j[3] for j in jobs], [j[4] for j in jobs],
-                                                [j[5] for j in jobs], nitergreen, wn, cutoff_dynamic, precision)
-            for k, (io, wt, *_rest) in enumerate(jobs):
-                G[io] += wt * dyn[k]
+                        jobs.append((io, dst, iph, op))
+            if jobs:
+                # the state's vector goes to the device once; its excitation vectors are
+                # built there and all of its runs advance together
+                dyn = dynamic_from_state(src, st.vector, st.energy, [(d, iph, op) for _, d, iph, op in jobs], spin,
+                                         nitergreen, wn, cutoff_dynamic, precision)
+                for k, (io, *_rest) in enumerate(jobs):
+                    G[io] += wt * dyn[k]
     finally:
         for s in sectors.values():
             s.close()

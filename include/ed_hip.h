@@ -430,6 +430,20 @@ int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *p
                        double cutoff_dynamic, int precision, int nw, const double *wn,
                        double *dyn, double *a, double *b);
 
+/* The excitation runs of one eigenstate in compute_greenAA (loop_over_states /
+ * apply_creat / compute_dynamic, green_class_computeAA.h:101-302) with every vector on
+ * the device: psi (host, dimen of gs) is copied once; job k builds
+ * phi_k = sum_t coefs[t] c^(pm)_{sites[t], spin} psi in sector dst[k] (pm = +1 for
+ * iph[k] = 1, -1 for iph[k] = 2; nterm[k] consecutive (site, coef) terms, the vectors of
+ * symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137), and all
+ * runs advance together as in ed_dynamic_batched from energy = E_psi.  Output
+ * dyn[k][2*nw].  Real (N_up, N_dn) sectors. */
+int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs,
+                          ed_sector *const *dst, const int *iph, const int *nterm,
+                          const int *sites, const double *coefs, int spin, int nitergreen,
+                          double cutoff_dynamic, int precision, int nw, const double *wn,
+                          double *dyn);
+
 /* ----------------------------------------------------------- diagnostics */
 
 /* Starts an FP64 (ED_PREC_F64) or FP32 (ED_PREC_F32) Lanczos recurrence from the

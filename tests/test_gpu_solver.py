@@ -204,3 +204,37 @@ def test_green_matrix_offdiagonal_vs_lehmann(gpu):
     assert np.max(np.abs(Gref[0, 1])) > 1e-3   # a real off-diagonal element
     assert np.max(np.abs(G - Gref)) < 1e-9
     assert np.max(np.abs(G_cf - Gref)) < 1e-9
+
+
+def test_dynamic_from_state_matches_host_vectors(gpu):
+    """The device-resident excitation runs of one state (psi copied once, c / c+ and the
+    (c_i + c_j) combinations built on the device, norms on the device) against the same
+    runs from host excitation vectors; a small target sector (one-by-one path on device
+    vectors) and a batched one."""
+    import json, os
+    from ed_solver_amd.aim import AIMParams
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")
+    with open(gold) as f:
+        d = json.load(f)["B"]
+    m, (nu, nd) = AIMParams.from_json(d["model"]), tuple(d["sector"])
+    wn = gpu.matsubara(50.0, 64)
+    for (su, sd) in ((nu, nd), (1, 1)):
+        src = gpu.Sector(m, su, sd)
+        r = src.lanczos(nitermax=200, neigen=1, tolerance=1e-12) if src.dim > 40 else None
+        psi = src.ground_state(r) if r else src.full_ed(1)[1][0]
+        e0 = float(r.evals[0]) if r else float(src.full_ed(1)[0][0])
+        dp, dh = gpu.Sector(m, su + 1, sd), gpu.Sector(m, su - 1, sd) if su > 0 else None
+        jobs, phis, secs, iphs = [], [], [], []
+        for op in ([(0, 1.0)], [(1, 1.0)], [(0, 1.0), (1, 1.0)], [(0, 0.5), (1, -2.0)]):
+            for iph, pm, dst in ((1, +1, dp), (2, -1, dh)):
+                if dst is None:
+                    continue
+                phi = sum(c * gpu.apply_C(src, dst, s, 0, pm, psi) for s, c in op)
+                jobs.append((dst, iph, op)); phis.append(phi); secs.append(dst); iphs.append(iph)
+        print(f"source ({su},{sd}) dim {src.dim}: {len(jobs)} runs", flush=True)
+        ref, _, _ = gpu.compute_dynamic_batched(secs, phis, iphs, [e0] * len(jobs), 30, wn)
+        got = gpu.dynamic_from_state(src, psi, e0, jobs, 0, 30, wn)
+        assert np.max(np.abs(got - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+        for s in (src, dp, dh):
+            if s is not None:
+                s.close()

@@ -107,6 +107,8 @@ SIGNATURES = {
     "ed_apply_c": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_dynamic_full_ed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp, _dp,
                                           ctypes.c_double, ctypes.c_int, _dp, _dp]),
+    "ed_dynamic_full_ed_c": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp, _dp,
+                                          ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "ed_continued_fraction": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_dynamic": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),

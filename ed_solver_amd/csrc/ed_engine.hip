@@ -1658,14 +1658,19 @@ static int jacobi_eig(int n, std::vector<double> &A, std::vector<double> &w, std
 
 }  // namespace ed
 
+namespace ed {
+int hermitian_from_embedding(int m, const std::vector<double> &w, const std::vector<double> &V, int neig,
+                             double *evals, double *evecs);
+}
+
 extern "C" int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs)
 {
     using namespace ed;
     ED_TRY(check_common(sec, 0));
-    if (sec->cplx) return fail(ED_ERR_ARG, "ed_full_ed: real sectors only (a complex H needs a Hermitian solver)");
-    const int64_t n = sec->dim;
-    if (n < 1 || n > 1024) return fail(ED_ERR_ARG, "ed_full_ed: sector dimension must be in [1, 1024]");
-    if (neig < 1 || neig > n || !evals) return fail(ED_ERR_ARG, "ed_full_ed: neig outside [1, dim] or NULL evals");
+    const int64_t n = sec->dim;   // a complex sector: the planar length 2 dimen_H
+    if (n < 1 || n > 1024) return fail(ED_ERR_ARG, "ed_full_ed: sector dimension must be in [1, 1024] (512 complex)");
+    const int64_t nphys = sec->cplx ? n / 2 : n;
+    if (neig < 1 || neig > nphys || !evals) return fail(ED_ERR_ARG, "ed_full_ed: neig outside [1, dim] or NULL evals");
     hipStream_t st = sec->stream;
     std::vector<double> eye((size_t)n * n, 0.0), H((size_t)n * n);
     for (int64_t i = 0; i < n; ++i) eye[(size_t)i * n + i] = 1.0;
@@ -1679,14 +1684,72 @@ extern "C" int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs
     if (rc != ED_OK) return rc;
     std::vector<double> w, V;
     ED_TRY(jacobi_eig((int)n, H, w, V));
-    for (int k = 0; k < neig; ++k) {
-        evals[k] = w[k];
-        if (evecs) std::memcpy(evecs + (size_t)k * n, V.data() + (size_t)k * n, sizeof(double) * n);
+    if (!sec->cplx) {
+        for (int k = 0; k < neig; ++k) {
+            evals[k] = w[k];
+            if (evecs) std::memcpy(evecs + (size_t)k * n, V.data() + (size_t)k * n, sizeof(double) * n);
+        }
+        return ED_OK;
     }
-    return ED_OK;
+    return ed::hermitian_from_embedding((int)nphys, w, V, neig, evals, evecs);
 }
 
 namespace ed {
+
+// Eigenpairs of a Hermitian H = A + iB of dimension m from those of its real embedding
+// [[A, -B], [B, A]] (2m x 2m, eigenvalues w ascending, eigenvectors rows of V): every
+// eigenvalue of H appears twice there, with the vectors (x; y) and (-y; x) of the one
+// complex vector x + iy.  Within each cluster of equal eigenvalues (2q of them) the
+// complex vectors x + iy of the cluster's real vectors span H's q-dimensional
+// eigenspace; q orthonormal ones are kept by complex Gram-Schmidt.  evecs rows are
+// planar (Re[m], Im[m]).
+int hermitian_from_embedding(int m, const std::vector<double> &w, const std::vector<double> &V, int neig,
+                             double *evals, double *evecs)
+{
+    const int n = 2 * m;
+    double scale = 1.0;
+    for (double x : w) scale = std::max(scale, std::fabs(x));
+    const double tol = 1e-10 * scale;
+    int got = 0;
+    for (int k = 0; k < n && got < neig;) {
+        int k2 = k + 1;
+        while (k2 < n && w[k2] - w[k] <= tol) ++k2;
+        const int q = (k2 - k + 1) / 2;
+        double lam = 0.0;
+        for (int t = k; t < k2; ++t) lam += w[t];
+        lam /= (k2 - k);
+        std::vector<std::vector<double>> acc;   // accepted, planar
+        for (int t = k; t < k2 && (int)acc.size() < q; ++t) {
+            std::vector<double> z(V.begin() + (size_t)t * n, V.begin() + (size_t)(t + 1) * n);
+            for (const auto &a : acc) {   // z -= <a, z> a  (complex inner product on planar vectors)
+                double re = 0.0, im = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    re += a[i] * z[i] + a[m + i] * z[m + i];
+                    im += a[i] * z[m + i] - a[m + i] * z[i];
+                }
+                for (int i = 0; i < m; ++i) {
+                    const double ar = a[i], ai = a[m + i];
+                    z[i] -= re * ar - im * ai;
+                    z[m + i] -= re * ai + im * ar;
+                }
+            }
+            double nrm = 0.0;
+            for (double v : z) nrm += v * v;
+            nrm = std::sqrt(nrm);
+            if (nrm < 0.5) continue;   // i times a vector already kept
+            for (double &v : z) v /= nrm;
+            acc.push_back(std::move(z));
+        }
+        for (const auto &z : acc) {
+            if (got >= neig) break;
+            evals[got] = lam;
+            if (evecs) std::memcpy(evecs + (size_t)got * n, z.data(), sizeof(double) * n);
+            ++got;
+        }
+        k = k2;
+    }
+    return got == neig ? ED_OK : fail(ED_ERR_HIP, "ed_full_ed: fewer complex eigenvectors than requested");
+}
 
 int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, double *diag, double *subdiag,
                     int *breakdown_at)

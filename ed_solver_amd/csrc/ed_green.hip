@@ -220,6 +220,22 @@ __global__ __launch_bounds__(kBlock) void k_overlap2(int dim, const double *vec,
     if (threadIdx.x == 0) w[blockIdx.x] = s * s;
 }
 
+// |<n|phi>|^2 for planar complex vectors (Re[dim], Im[dim]): the real and imaginary
+// parts of sum conj(v) phi by one fixed-order block sum each.
+__global__ __launch_bounds__(kBlock) void k_overlap2_c(int dim, const double *vec, const double *phi, double *w)
+{
+    __shared__ double red[kRed];
+    const double *v = vec + (size_t)blockIdx.x * 2 * dim;
+    double re = 0.0, im = 0.0;
+    for (int i = threadIdx.x; i < dim; i += kBlock) {
+        re += v[i] * phi[i] + v[dim + i] * phi[dim + i];
+        im += v[i] * phi[dim + i] - v[dim + i] * phi[i];
+    }
+    re = block_sum_all(re, red);
+    im = block_sum_all(im, red);
+    if (threadIdx.x == 0) w[blockIdx.x] = re * re + im * im;
+}
+
 // Lehmann sum of compute_dynamic_full_ed: dyn(i w_k) = sum_n w_n / (i w_k - ph (E_n - E)),
 // poles in index order, one thread per frequency.
 __global__ void k_lehmann(int nst, const double *e, const double *w, double e0, double ph, int nw, const double *wn,
@@ -262,33 +278,43 @@ int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, i
     return rc;
 }
 
-int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
-                       const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn)
+}  // extern "C"
+
+// compute_dynamic_full_ed for real (planes = 1) or planar complex (planes = 2) vectors of
+// dim amplitudes.
+static int dynamic_full_ed(int dim, int planes, int iph, double energy, const double *phi, int nst,
+                           const double *evals, const double *evecs, double cutoff_dynamic, int nw, const double *wn,
+                           double *dyn)
 {
     if (dim < 1 || nst < 0 || !phi || (nst > 0 && (!evals || !evecs)) || nw < 1 || !wn || !dyn)
         return fail(ED_ERR_ARG, "ed_dynamic_full_ed: bad argument");
     if (iph != 1 && iph != 2) return fail(ED_ERR_ARG, "ed_dynamic_full_ed: iph must be 1 (particle) or 2 (hole)");
     std::fill(dyn, dyn + 2 * (size_t)nw, 0.0);
     double nrm2 = 0.0;
-    for (int i = 0; i < dim; ++i) nrm2 += phi[i] * phi[i];
+    for (int i = 0; i < planes * dim; ++i) nrm2 += phi[i] * phi[i];
     // green_class_compute_dynamic.f90:117-119: below the cutoff, or no stored state of the
     // target sector (iisector = 0, :135), the contribution is zero
     if (nrm2 < cutoff_dynamic || nst == 0) return ED_OK;
     hipStream_t st = nullptr;
     double *d = nullptr;
-    const size_t tot = (size_t)nst * dim + dim + 2 * (size_t)nst + nw + 2 * (size_t)nw;
+    const size_t len = (size_t)planes * dim;
+    const size_t tot = (size_t)nst * len + len + 2 * (size_t)nst + nw + 2 * (size_t)nw;
     ED_HIP_CHECK(hipMalloc((void **)&d, sizeof(double) * tot));
-    double *d_vec = d, *d_phi = d_vec + (size_t)nst * dim, *d_e = d_phi + dim, *d_w = d_e + nst, *d_wn = d_w + nst,
+    double *d_vec = d, *d_phi = d_vec + (size_t)nst * len, *d_e = d_phi + len, *d_w = d_e + nst, *d_wn = d_w + nst,
            *d_out = d_wn + nw;
     int rc = ED_OK;
-    if (hipMemcpy(d_vec, evecs, sizeof(double) * nst * dim, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_phi, phi, sizeof(double) * dim, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(d_vec, evecs, sizeof(double) * nst * len, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_phi, phi, sizeof(double) * len, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_e, evals, sizeof(double) * nst, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_wn, wn, sizeof(double) * nw, hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(ED_ERR_HIP, "ed_dynamic_full_ed: copy to the device failed");
     } else {
-        hipLaunchKernelGGL(k_overlap2, dim3(nst), dim3(kBlock), 0, st, dim, (const double *)d_vec,
-                           (const double *)d_phi, d_w);
+        if (planes == 1)
+            hipLaunchKernelGGL(k_overlap2, dim3(nst), dim3(kBlock), 0, st, dim, (const double *)d_vec,
+                               (const double *)d_phi, d_w);
+        else
+            hipLaunchKernelGGL(k_overlap2_c, dim3(nst), dim3(kBlock), 0, st, dim, (const double *)d_vec,
+                               (const double *)d_phi, d_w);
         hipLaunchKernelGGL(k_lehmann, dim3((nw + kBlock - 1) / kBlock), dim3(kBlock), 0, st, nst, (const double *)d_e,
                            (const double *)d_w, energy, iph == 1 ? 1.0 : -1.0, nw, (const double *)d_wn, d_out);
         if (hipGetLastError() != hipSuccess ||
@@ -297,6 +323,20 @@ int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int n
     }
     (void)hipFree(d);
     return rc;
+}
+
+extern "C" {
+
+int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
+                       const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn)
+{
+    return dynamic_full_ed(dim, 1, iph, energy, phi, nst, evals, evecs, cutoff_dynamic, nw, wn, dyn);
+}
+
+int ed_dynamic_full_ed_c(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
+                         const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn)
+{
+    return dynamic_full_ed(dim, 2, iph, energy, phi, nst, evals, evecs, cutoff_dynamic, nw, wn, dyn);
 }
 
 int ed_continued_fraction(int n, const double *a, const double *b, int nw, const double *wn, double *out)

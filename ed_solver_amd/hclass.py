@@ -217,11 +217,12 @@ class Sector:
         return self._unpack(psi)
 
     def full_ed(self, neig: int = 1):
-        """ED_diago: lowest neig eigenpairs of the dense sector H (dim <= 1024)."""
+        """ED_diago: lowest neig eigenpairs of the dense sector H (dim <= 1024; 512 complex)."""
         neig = int(min(neig, self.dim))
-        ev = np.zeros(neig); vec = np.zeros(neig * self.dim)
+        ev = np.zeros(neig); vec = np.zeros(neig * self.vlen)
         check(lib().ed_full_ed(self.handle, neig, dptr(ev), dptr(vec)))
-        return ev, vec.reshape(neig, self.dim)
+        vec = vec.reshape(neig, self.vlen)
+        return ev, np.array([self._unpack(v) for v in vec]).reshape(neig, self.dim)
 
     def lanczos_run(self, niter: int, start: np.ndarray | None = None, precision="f64"):
         """Lanczos_GPU_updo: exactly niter steps, returns (diag, subdiag)."""
@@ -407,7 +408,8 @@ class ComplexSector(Sector):
     ec_im[2, nc, nc], eb_im[2, nb, nb], vbc_im[2, nb, nc] the imaginary parts.  Vectors
     are complex numpy arrays of dim amplitudes; the device holds them planar (Re, Im),
     where H acts as [[A, -B], [B, A]] (H = A + iB) and the plain Lanczos has the complex
-    Lanczos' real alpha and beta.  FP64 only; no full ED, block Lanczos or gradients."""
+    Lanczos' real alpha and beta.  FP64 only; dense ED (Hermitian, from the real
+    embedding) up to dimension 512; no block Lanczos or gradients."""
 
     def __init__(self, model: AIMParams, nup: int, ndn: int, ec_im, eb_im=None, vbc_im=None):
         self.model = model
@@ -481,15 +483,23 @@ def dynamic_full_ed(iph: int, energy: float, phi: np.ndarray, evals: np.ndarray,
                     cutoff_dynamic: float = 1e-8) -> np.ndarray:
     """compute_dynamic_full_ed: Lehmann sum over the given eigenpairs of a small target
     sector (evecs one state per row); returns dyn(i w_n) complex."""
-    phi = np.ascontiguousarray(phi, np.float64)
+    cplx = np.iscomplexobj(phi) or np.iscomplexobj(evecs)
+    phi = np.asarray(phi)
+    dim = phi.size
     ev = np.ascontiguousarray(evals, np.float64).reshape(-1)
-    vec = np.ascontiguousarray(evecs, np.float64).reshape(len(ev), -1) if len(ev) else np.zeros((0, phi.size))
-    if len(ev) and vec.shape[1] != phi.size:
+    vec = np.asarray(evecs).reshape(len(ev), -1) if len(ev) else np.zeros((0, dim))
+    if len(ev) and vec.shape[1] != dim:
         raise EDError(-1, "dynamic_full_ed: eigenvector length differs from phi")
+    if cplx:   # planar (Re, Im) rows for the complex entry point
+        phi = np.concatenate([phi.real, phi.imag])
+        vec = np.concatenate([vec.real, vec.imag], axis=1)
+    phi = np.ascontiguousarray(phi, np.float64)
+    vec = np.ascontiguousarray(vec, np.float64)
     wn = np.ascontiguousarray(wn, np.float64)
     out = np.zeros(2 * len(wn))
-    check(lib().ed_dynamic_full_ed(phi.size, int(iph), float(energy), dptr(phi), len(ev), dptr(ev) if len(ev) else None,
-                                   dptr(vec) if len(ev) else None, float(cutoff_dynamic), len(wn), dptr(wn), dptr(out)))
+    f = lib().ed_dynamic_full_ed_c if cplx else lib().ed_dynamic_full_ed
+    check(f(dim, int(iph), float(energy), dptr(phi), len(ev), dptr(ev) if len(ev) else None,
+            dptr(vec) if len(ev) else None, float(cutoff_dynamic), len(wn), dptr(wn), dptr(out)))
     return out[0::2] + 1j * out[1::2]
 
 

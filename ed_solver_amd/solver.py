@@ -23,7 +23,8 @@ from math import comb
 import numpy as np
 
 from .aim import AIMParams
-from .hclass import Sector, apply_C, dynamic_from_state, dynamic_full_ed
+from .hclass import (ComplexSector, Sector, apply_C, compute_dynamic_batched, dynamic_from_state,
+                     dynamic_full_ed)
 
 FULL_ED_SWITCH = 40   # Global_Variables.f90:23
 
@@ -39,10 +40,19 @@ def all_sectors(ns: int) -> list[tuple[int, int]]:
     return [(nu, nd) for nu in range(ns + 1) for nd in range(ns + 1)]
 
 
+def make_sector(model: AIMParams, nu: int, nd: int, im: dict | None = None) -> Sector:
+    """The (N_up, N_dn) sector: real (H_class) or, when im holds the imaginary parts of
+    Ec / Eb / Vbc ({"ec": [2, nc, nc], "eb": [2, nb, nb], "vbc": [2, nb, nc]}), complex
+    (Hmultc, HAIMupdo.f90:150-656)."""
+    if im is None:
+        return Sector(model, nu, nd)
+    return ComplexSector(model, nu, nd, im["ec"], im.get("eb"), im.get("vbc"))
+
+
 def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128,
                        tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
                        precision="f64", which_lanczos: str = "GPU", neigen_arpack: int | None = None,
-                       arpack_ncv: int | None = None, full_ed_green: bool = False):
+                       arpack_ncv: int | None = None, full_ed_green: bool = False, im: dict | None = None):
     """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax].
 
     Per sector (solver.h:150-200): block Lanczos when block_size > 0, unless N_up or N_dn
@@ -50,14 +60,18 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
     which_lanczos = 'NORMAL' / 'GPU' (the device Lanczos), 'FULL_ED' or 'ARPACK'.
     full_ed_green (FLAG_FULL_ED_GREEN): dense-ED sectors keep all their states
     (ED_ARPACK.f90:652-665) and neither window filter runs (solver.h:211,
-    solver.f90:93), so compute_green's Lehmann sums over small sectors are complete."""
+    solver.f90:93), so compute_green's Lehmann sums over small sectors are complete.
+    im (imaginary parts of Ec / Eb / Vbc, see make_sector): complex sectors, where dense
+    ED and the plain device Lanczos run; block Lanczos and ARPACK are real only."""
     sectors = all_sectors(model.ns) if sectors is None else sectors
+    if im is not None and (block_size > 0 or which_lanczos == "ARPACK"):
+        raise ValueError("complex sectors: block Lanczos and ARPACK are real only")
     states: list[EigenState] = []
     ns = model.ns
     for (nu, nd) in sectors:
         dim = comb(ns, nu) * comb(ns, nd)
         plain = block_size == 0 or nu in (0, 1) or nd in (0, 1) or (nu == ns and nd == ns)
-        with Sector(model, nu, nd) as sec:
+        with make_sector(model, nu, nd, im) as sec:
             if dim <= full_ed_switch or (plain and which_lanczos == "FULL_ED"):
                 ev, vec = sec.full_ed(dim if full_ed_green else min(neigen, dim))
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
@@ -128,15 +142,27 @@ def partition(states: list[EigenState], beta: float) -> np.ndarray:
     return w / w.sum()
 
 
+def _excite(src: Sector, dst: Sector, op, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
+    """A+|psi> (pm = +1) or A|psi> (pm = -1) for A = sum_c coef_c c_c: the creation side
+    carries conj(coef_c)."""
+    phi = None
+    for c, coef in op:
+        t = (np.conj(coef) if pm > 0 else coef) * apply_C(src, dst, c, spin, pm, psi)
+        phi = t if phi is None else phi + t
+    return phi
+
+
 def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, ops, nitergreen: int,
-               spin: int, weight_cutoff: float, cutoff_dynamic: float, precision, full_ed_switch: int) -> np.ndarray:
+               spin: int, weight_cutoff: float, cutoff_dynamic: float, precision, full_ed_switch: int,
+               im: dict | None = None) -> np.ndarray:
     """sum over states of boltz * (particle + hole) for each operator A = sum_c coef_c c_c
     of `ops` (a list of [(orbital, coef), ...]): the particle run from A+|psi>, the hole
     run from A|psi> (green_class_computeAA.h:101-302 with the vectors of
     symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137).  Small
     target sectors take the Lehmann sum over the held states, every other run is an
     excitation Lanczos; a state's runs start from excitation vectors built on the device
-    and advance together in one batched device call."""
+    and advance together in one batched device call (complex sectors: the excitation
+    vectors are built by apply_C per operator term and the runs batched the same way)."""
     w = partition(states, beta)
     G = np.zeros((len(ops), len(wn)), complex)
     sectors: dict[tuple[int, int], Sector] = {}
@@ -144,7 +170,7 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
     def sector(nu, nd):
         key = (nu, nd)
         if key not in sectors:
-            sectors[key] = Sector(model, nu, nd)
+            sectors[key] = make_sector(model, nu, nd, im)
         return sectors[key]
 
     held: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
@@ -169,15 +195,19 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
                         continue
                     dst = sector(*tn)
                     if dst.dim <= full_ed_switch:
-                        phi = None
-                        for c, coef in op:
-                            t = coef * apply_C(src, dst, c, spin, pm, st.vector)
-                            phi = t if phi is None else phi + t
+                        phi = _excite(src, dst, op, spin, pm, st.vector)
                         ev, vec = held_states(tn)
                         G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
                     else:
                         jobs.append((io, dst, iph, op))
-            if jobs:
+            if jobs and im is not None:
+                phis = [_excite(src, dst, op, spin, +1 if iph == 1 else -1, st.vector) for _, dst, iph, op in jobs]
+                dyn, _, _ = compute_dynamic_batched([j[1] for j in jobs], phis, [j[2] for j in jobs],
+                                                    [st.energy] * len(jobs), nitergreen, wn, cutoff_dynamic,
+                                                    precision)
+                for k, (io, *_rest) in enumerate(jobs):
+                    G[io] += wt * dyn[k]
+            elif jobs:
                 # the state's vector goes to the device once; its excitation vectors are
                 # built there and all of its runs advance together
                 dyn = dynamic_from_state(src, st.vector, st.energy, [(d, iph, op) for _, d, iph, op in jobs], spin,
@@ -192,7 +222,7 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
 
 def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
                   spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
-                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
+                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None) -> np.ndarray:
     """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
 
     For every state with Boltzmann weight above weight_cutoff and every orbital: the
@@ -200,32 +230,43 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
     compute_dynamic dispatches (green_class_compute_dynamic.f90:44-62): a target sector
     of dimension <= full_ed_switch takes the Lehmann sum over the states `states` holds
     for it (compute_dynamic_full_ed, :77-165; none held gives zero), every other run is
-    an excitation Lanczos, and those all advance together in one batched device call."""
+    an excitation Lanczos, and those all advance together in one batched device call.
+    im: the states are of the complex model (get_eigen_energies with the same im)."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     return _green_ops(model, states, beta, wn, [[(c, 1.0)] for c in orbitals], nitergreen, spin, weight_cutoff,
-                      cutoff_dynamic, precision, full_ed_switch)
+                      cutoff_dynamic, precision, full_ed_switch, im)
 
 
 def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray,
                          nitergreen: int = 36, spin: int = 0, orbitals=None, weight_cutoff: float = 0.0,
                          cutoff_dynamic: float = 1e-8, precision="f64",
-                         full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
-    """Impurity G_ij(i w_n) of one spin, all orbital pairs, shape (norb, norb, nw).
+                         full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None) -> np.ndarray:
+    """Impurity G_ij(i w_n) = <<c_i ; c_j+>> of one spin, all orbital pairs, shape (norb, norb, nw).
 
-    Real H (symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137;
-    reshuffle_vecAA, green_class_compute_symmetric_combination_AA.h:10-57): besides the
-    diagonal runs from c_i, one run per pair i < j from (c_i + c_j)|psi> (and its
-    adjoint), then G_ij = G_ji = (G_(i+j) - G_ii - G_jj) / 2.  Every run of every pair
-    shares the one batched device call."""
+    symmetric_combineAA (green_class_compute_symmetric_combination.f90:39-137) and
+    reshuffle_vecAA (green_class_compute_symmetric_combination_AA.h:10-57): besides the
+    diagonal runs from c_i, one run per pair i < j from A = c_i + c_j (particle run from
+    A+|psi>, hole run from A|psi>), whose G_A = G_ii + G_jj + G_ij + G_ji.  Real H:
+    G_ij = G_ji = (G_A - G_ii - G_jj) / 2.  Complex H (im given, states of the complex
+    model): one more run per pair from B = c_i - i c_j, G_B = G_ii + G_jj + i (G_ij - G_ji),
+    so G_ij, G_ji = (S +- D) / 2 with S = G_A - G_ii - G_jj, D = -i (G_B - G_ii - G_jj).
+    Every run of every pair shares the one batched device call per state."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     n = len(orbitals)
     pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
     ops = [[(c, 1.0)] for c in orbitals] + [[(orbitals[a], 1.0), (orbitals[b], 1.0)] for a, b in pairs]
+    if im is not None:
+        ops += [[(orbitals[a], 1.0), (orbitals[b], -1j)] for a, b in pairs]
     g = _green_ops(model, states, beta, wn, ops, nitergreen, spin, weight_cutoff, cutoff_dynamic, precision,
-                   full_ed_switch)
+                   full_ed_switch, im)
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]
     for k, (a, b) in enumerate(pairs):
-        G[a, b] = G[b, a] = 0.5 * (g[n + k] - g[a] - g[b])
+        S = g[n + k] - g[a] - g[b]
+        if im is None:
+            G[a, b] = G[b, a] = 0.5 * S
+        else:
+            D = -1j * (g[n + len(pairs) + k] - g[a] - g[b])
+            G[a, b], G[b, a] = 0.5 * (S + D), 0.5 * (S - D)
     return G

@@ -272,7 +272,9 @@ int ed_lanczos_steps_run(ed_sector *sec, int *steps);
 
 /* ED_diago (ED_ARPACK.f90:590-687), the path of sectors of dimension <= FULL_ED_SWITCH
  * (solver.h:156-180): dense H from H e_i on the device, all eigenpairs on the host
- * (cyclic Jacobi), ascending.  evals[neig]; evecs[neig][dim] (may be NULL).  dim <= 1024. */
+ * (cyclic Jacobi), ascending.  evals[neig]; evecs[neig][dim] (may be NULL).  dim <= 1024.
+ * A complex sector (dimen_H <= 512): the real embedding of H is diagonalised, each
+ * doubled eigenvalue taken once, and evecs rows are planar [neig][2 dimen_H]. */
 int ed_full_ed(ed_sector *sec, int neig, double *evals, double *evecs);
 
 /* Lanczos_GPU_updo (Lanczos_gpu.f90:106): exactly `niter` Lanczos steps (no
@@ -418,6 +420,11 @@ int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int ni
  * the device; dyn complex interleaved [2*nw]. */
 int ed_dynamic_full_ed(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
                        const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn);
+
+/* The same for a complex sector: phi[2 dim] and the rows of evecs[nst][2 dim] planar
+ * (Re[dim], Im[dim]); the weights are |<n|phi>|^2 with the complex inner product. */
+int ed_dynamic_full_ed_c(int dim, int iph, double energy, const double *phi, int nst, const double *evals,
+                         const double *evecs, double cutoff_dynamic, int nw, const double *wn, double *dyn);
 
 /* Batched excitation Lanczos (no reference counterpart as one call: it fuses the
  * per-(state, orbital, iph) calls of compute_dynamic made by

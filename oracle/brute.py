@@ -156,7 +156,7 @@ def apply_c_full(model: AIMParams, nup: int, ndn: int, site: int, spin: int, pm:
     du2 = len(ups2)
     r_up2 = np.full(1 << ns, -1); r_up2[ups2] = np.arange(du2)
     r_dn2 = np.full(1 << ns, -1); r_dn2[dns2] = np.arange(len(dns2))
-    phi = np.zeros(du2 * len(dns2))
+    phi = np.zeros(du2 * len(dns2), np.result_type(psi, np.float64))
     mode = model.nb + site
     src = ups if spin == 0 else dns
     out, sg, ok = apply_ops(src, [(mode, pm > 0)])

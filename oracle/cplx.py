@@ -159,6 +159,8 @@ def sector_hamiltonian_c(model: AIMParams, im: dict, nup: int, ndn: int) -> sp.c
         r = rank[o & ((1 << ns) - 1)] + rank_dn[o >> ns] * du
         rows.append(r); cols.append(col[ok]); vals.append(coef * sg[ok])
     D = du * dd
+    if not vals:   # the empty and the full sector hold no one-body or interaction matrix element
+        return sp.csr_matrix((D, D), dtype=np.complex128)
     H = sp.coo_matrix((np.concatenate(vals).astype(np.complex128), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(D, D))
     return H.tocsr()

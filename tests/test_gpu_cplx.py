@@ -141,11 +141,78 @@ def test_cplx_rejections(gpu):
     with pytest.raises(EDError):
         s.lanczos(precision="f32")
     with pytest.raises(EDError):
-        s.full_ed(1)
-    with pytest.raises(EDError):
         block_lanczos(s, nblock=2, neig=2)
     with pytest.raises(EDError):
         s.set_schedule(8)
     with pytest.raises(EDError):   # one complex and one real sector
         gpu.apply_C(s, sr, 0, 0, +1, np.ones(s.dim, complex))
     s.close(); sr.close()
+
+
+def test_cplx_full_ed_vs_dense(gpu):
+    """Hermitian dense ED (from the real embedding): eigenvalues against eigvalsh of the
+    brute-force complex H, residuals, and orthonormal complex eigenvectors, also inside
+    the degenerate multiplets of a phase-transformed model of two identical impurity +
+    bath pairs."""
+    m, im = oc.random_model(2, 3, seed=21, jhund=0.3)
+    # two identical impurity + bath pairs without inter-orbital terms: exact degeneracies,
+    # kept by a phase on every orbital (J = 0)
+    diag2 = lambda a, b: np.stack([np.diag([a, b])] * 2)
+    mdeg = AIMParams(2, 2, diag2(-1.0, -1.0), diag2(0.3, 0.3), diag2(0.5, 0.5), np.diag([2.5, 2.5]),
+                     np.zeros((2, 2)), 0.0, 1e-8)
+    theta = np.random.default_rng(8).uniform(0, 2 * np.pi, (2, mdeg.ns))
+    m2, im2 = oc.phase_model(mdeg, theta)
+    for (mm, ii, sec) in ((m, im, (2, 2)), (m, im, (3, 2)), (m2, im2, (2, 2)), (m2, im2, (3, 2))):
+        s = csector(gpu, mm, ii, *sec)
+        H = oc.sector_hamiltonian_c(mm, ii, *sec).toarray()
+        e_ref = np.linalg.eigvalsh(H)
+        ev, vec = s.full_ed(s.dim)
+        print(f"complex full ED {sec}: dim {s.dim}, min gap {np.min(np.diff(e_ref)):.2e}", flush=True)
+        assert vec.dtype == np.complex128 and vec.shape == (s.dim, s.dim)
+        assert np.max(np.abs(ev - e_ref)) < 1e-12
+        assert np.max(np.abs(H @ vec.T - vec.T * ev)) < 1e-11
+        assert np.max(np.abs(vec.conj() @ vec.T - np.eye(s.dim))) < 1e-11
+        ev3, vec3 = s.full_ed(3)
+        assert np.max(np.abs(ev3 - e_ref[:3])) < 1e-12
+        s.close()
+    assert np.min(np.diff(np.linalg.eigvalsh(oc.sector_hamiltonian_c(m2, im2, 2, 2).toarray()))) < 1e-10
+
+
+def test_cplx_green_matrix_vs_lehmann(gpu):
+    """The solver on a complex model (im): every sector by dense ED, G_ij = <<c_i; c_j+>>
+    from the (c_i + c_j) and (c_i - i c_j) runs against the exact Lehmann sum over the
+    full Fock space (G_ij != G_ji), through the Lehmann path and through excitation
+    Lanczos (full_ed_switch = 0, 36 steps >= every sector dimension)."""
+    from ed_solver_amd.solver import compute_green, compute_green_matrix, get_eigen_energies
+    m, im = oc.random_model(2, 2, seed=31, jhund=0.0)
+    beta = 8.0
+    wn = np.pi / beta * (2 * np.arange(1, 25) - 1)
+    states = get_eigen_energies(m, neigen=36, im=im)
+    assert len(states) == 4 ** 4
+    G = compute_green_matrix(m, states, beta, wn, nitergreen=36, im=im)
+    G_cf = compute_green_matrix(m, states, beta, wn, nitergreen=36, full_ed_switch=0, im=im)
+    g_diag = compute_green(m, states, beta, wn, nitergreen=36, im=im)
+    ns, nc = m.ns, m.nc
+    spec = {(nu, nd): np.linalg.eigh(oc.sector_hamiltonian_c(m, im, nu, nd).toarray())
+            for nu in range(ns + 1) for nd in range(ns + 1)}
+    e0 = min(v[0].min() for v in spec.values())
+    Z = sum(np.exp(-beta * (v[0] - e0)).sum() for v in spec.values())
+    Gref = np.zeros((nc, nc, len(wn)), complex)
+    for (nu, nd), (em, vm) in spec.items():
+        if nu == ns:
+            continue
+        en, vn = spec[(nu + 1, nd)]
+        for a in range(len(em)):
+            amp = [vn.conj().T @ bf.apply_c_full(m, nu, nd, c, 0, +1, vm[:, a]) for c in range(nc)]
+            wgt = np.exp(-beta * (em[a] - e0)) + np.exp(-beta * (en - e0))
+            den = 1j * wn[None, :] - (en - em[a])[:, None]
+            for i in range(nc):
+                for j in range(nc):
+                    Gref[i, j] += ((np.conj(amp[i]) * amp[j] * wgt)[:, None] / den).sum(0)
+    Gref /= Z
+    assert np.max(np.abs(Gref[0, 1] - Gref[1, 0])) > 1e-3   # G_ij != G_ji for complex H
+    assert np.max(np.abs(G - Gref)) < 1e-9
+    assert np.max(np.abs(G_cf - Gref)) < 1e-9
+    assert np.max(np.abs(g_diag - np.array([Gref[c, c] for c in range(nc)]))) < 1e-9
+    with pytest.raises(ValueError):
+        get_eigen_energies(m, block_size=4, im=im)

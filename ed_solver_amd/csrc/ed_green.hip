@@ -4,7 +4,9 @@
 //   k_cmap / k_apply_c   apply_Cupdo (apply_C.f90:497-598).  c+/c on one impurity
 //                        orbital of one spin maps each spin state to at most one state
 //                        of the N+-1 spin sector, so phi is a signed gather of psi:
-//                        one thread per phi element, coalesced writes.
+//                        one thread per phi element, coalesced writes.  An S_z
+//                        sector (apply_Csz, apply_C.f90:277-327) is one string of
+//                        2 Ns Nambu orbitals: the same map over that string.
 //   k_cfrac              invert_zmtridiag (matrices_tridiag.f90:163-198), one thread
 //                        per Matsubara frequency (and per job when batched).
 //   ed_dynamic           compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407).
@@ -114,11 +116,16 @@ static int grid_for(int64_t n)
 static int check_pair(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm)
 {
     if (!src || !dst) return fail(ED_ERR_ARG, "ed_apply_c: NULL sector");
-    if (src->sz || dst->sz) return fail(ED_ERR_ARG, "ed_apply_c: (N_up, N_dn) sectors only (apply_Cupdo)");
+    if (src->sz != dst->sz) return fail(ED_ERR_ARG, "ed_apply_c: one S_z and one (N_up, N_dn) sector");
     if (src->cplx != dst->cplx) return fail(ED_ERR_ARG, "ed_apply_c: one complex and one real sector");
     if (src->nc != dst->nc || src->nb != dst->nb) return fail(ED_ERR_ARG, "ed_apply_c: sectors of different models");
     if (site < 0 || site >= src->nc) return fail(ED_ERR_ARG, "ed_apply_c: site outside [0, nc)");
     if ((spin != 0 && spin != 1) || (pm != 1 && pm != -1)) return fail(ED_ERR_ARG, "ed_apply_c: spin 0/1, pm +1/-1");
+    if (src->sz) {   // Nambu orbital (site, spin): npart -> npart + pm (Csz_sector_updo, apply_C.f90:214-228)
+        if (dst->nup != src->nup + pm)
+            return fail(ED_ERR_ARG, "ed_apply_c: destination S_z sector is not npart + pm");
+        return ED_OK;
+    }
     const int nu = src->nup + (spin == 0 ? pm : 0), nd = src->ndn + (spin == 1 ? pm : 0);
     if (dst->nup != nu || dst->ndn != nd)
         return fail(ED_ERR_ARG, "ed_apply_c: destination sector is not (nup, ndn) shifted by the operator (Cupdo_sector)");
@@ -129,17 +136,22 @@ int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, 
                 double *d_phi, hipStream_t st)
 {
     ED_TRY(check_pair(src, dst, site, spin, pm));
-    const SpinTables &T = spin == 0 ? dst->up : dst->dn;
+    // An S_z sector is one string of 2 Ns Nambu orbitals held as the "up" table (ddn = 1):
+    // the operator creates (pm > 0) or destroys the Nambu orbital of (site, spin), bit
+    // spin Ns + nb + site (apply_Csz, apply_C.f90:277-327).
+    const int bit = src->sz ? spin * src->ns + src->nb + site : src->nb + site;
+    const int kspin = src->sz ? 0 : spin;
+    const SpinTables &T = kspin == 0 ? dst->up : dst->dn;
     int32_t *d_map = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_map, sizeof(int32_t) * 2 * std::max(1, T.d), st));
     hipLaunchKernelGGL(k_cmap, dim3((T.d + kBlock - 1) / kBlock), dim3(kBlock), 0, st, T.states, T.d,
-                       src->nb + site, pm, src->d_rank, d_map, d_map + T.d);
+                       bit, pm, src->d_rank, d_map, d_map + T.d);
     // a complex sector's planar vectors: the same real map on the Re and the Im plane
     const int planes = dst->cplx ? 2 : 1;
     const int64_t ns = dst->cplx ? src->cdim : src->dim, nd = dst->cplx ? dst->cdim : dst->dim;
     for (int p = 0; p < planes; ++p)
         hipLaunchKernelGGL(k_apply_c, dim3(grid_for(nd)), dim3(kBlock), 0, st, d_psi + p * ns, src->up.d,
-                           d_phi + p * nd, dst->up.d, nd, spin, (const int32_t *)d_map, (const int32_t *)(d_map + T.d));
+                           d_phi + p * nd, dst->up.d, nd, kspin, (const int32_t *)d_map, (const int32_t *)(d_map + T.d));
     ED_HIP_CHECK(hipGetLastError());
     ED_HIP_CHECK(hipFreeAsync(d_map, st));
     return ED_OK;

@@ -336,7 +336,10 @@ def band_eig(a: np.ndarray, b: int, nev: int, vectors: bool = True):
 
 
 def apply_C(src: Sector, dst: Sector, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
-    """apply_Cupdo (apply_C.f90:497-598): c+ (pm=+1) / c (pm=-1) on impurity orbital `site`."""
+    """apply_Cupdo (apply_C.f90:497-598): c+ (pm=+1) / c (pm=-1) on impurity orbital `site`.
+    S_z sectors (apply_Csz, apply_C.f90:277-327): creates (pm=+1) or destroys the Nambu
+    orbital (site, spin), npart -> npart + pm; the Nambu orbital of spin 1 is the
+    particle-hole transformed down orbital, so pm=+1 there applies the physical c_dn."""
     psi = src._pack(psi, "apply_C: psi")
     phi = np.zeros(dst.vlen)
     check(lib().ed_apply_c(src.handle, dst.handle, site, spin, pm, dptr(psi), dptr(phi)))

@@ -142,38 +142,39 @@ def partition(states: list[EigenState], beta: float) -> np.ndarray:
     return w / w.sum()
 
 
-def _excite(src: Sector, dst: Sector, op, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
-    """A+|psi> (pm = +1) or A|psi> (pm = -1) for A = sum_c coef_c c_c: the creation side
-    carries conj(coef_c)."""
+def _excite(src: Sector, dst: Sector, op, pm: int, psi: np.ndarray) -> np.ndarray:
+    """A+|psi> (pm = +1) or A|psi> (pm = -1) for A = sum coef c_(site, spin) over the
+    entries (site, spin, coef) of op: the creation side carries conj(coef)."""
     phi = None
-    for c, coef in op:
-        t = (np.conj(coef) if pm > 0 else coef) * apply_C(src, dst, c, spin, pm, psi)
+    for c, sp, coef in op:
+        t = (np.conj(coef) if pm > 0 else coef) * apply_C(src, dst, c, sp, pm, psi)
         phi = t if phi is None else phi + t
     return phi
 
 
-def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, ops, nitergreen: int,
-               spin: int, weight_cutoff: float, cutoff_dynamic: float, precision, full_ed_switch: int,
-               im: dict | None = None) -> np.ndarray:
-    """sum over states of boltz * (particle + hole) for each operator A = sum_c coef_c c_c
-    of `ops` (a list of [(orbital, coef), ...]): the particle run from A+|psi>, the hole
+def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, nitergreen: int, weight_cutoff: float,
+               cutoff_dynamic: float, precision, full_ed_switch: int, make, target,
+               device_vectors: bool) -> np.ndarray:
+    """sum over states of boltz * (particle + hole) for each operator A = sum coef c_(site, spin)
+    of `ops` (a list of [(site, spin, coef), ...]): the particle run from A+|psi>, the hole
     run from A|psi> (green_class_computeAA.h:101-302 with the vectors of
-    symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137).  Small
-    target sectors take the Lehmann sum over the held states, every other run is an
-    excitation Lanczos; a state's runs start from excitation vectors built on the device
-    and advance together in one batched device call (complex sectors: the excitation
-    vectors are built by apply_C per operator term and the runs batched the same way)."""
+    symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137).
+    make(key) opens the sector of a state's key, target(key, spin, pm) gives the key of
+    the sector A+ (pm = +1) or A (pm = -1) leads to, or None.  Small target sectors take
+    the Lehmann sum over the held states, every other run is an excitation Lanczos.  With
+    device_vectors (real (N_up, N_dn) sectors) a state's excitation vectors are built on
+    the device from one copy of the state; otherwise apply_C builds them per operator
+    term.  Either way a state's runs advance together in one batched device call."""
     w = partition(states, beta)
     G = np.zeros((len(ops), len(wn)), complex)
-    sectors: dict[tuple[int, int], Sector] = {}
+    sectors: dict[tuple, Sector] = {}
 
-    def sector(nu, nd):
-        key = (nu, nd)
+    def sector(key):
         if key not in sectors:
-            sectors[key] = make_sector(model, nu, nd, im)
+            sectors[key] = make(key)
         return sectors[key]
 
-    held: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
+    held: dict[tuple, tuple[np.ndarray, np.ndarray]] = {}
 
     def held_states(key):
         if key not in held:
@@ -185,39 +186,46 @@ def _green_ops(model: AIMParams, states: list[EigenState], beta: float, wn: np.n
         for st, wt in zip(states, w):
             if wt <= weight_cutoff:
                 continue
-            nu, nd = st.sector
-            src = sector(nu, nd)
+            src = sector(st.sector)
             jobs = []   # (operator index, target sector, iph, operator) of the excitation runs
             for io, op in enumerate(ops):
                 for iph, pm in ((1, +1), (2, -1)):
-                    tn = (nu + pm, nd) if spin == 0 else (nu, nd + pm)
-                    if not (0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns):
+                    tkey = target(st.sector, op[0][1], pm)
+                    if tkey is None:
                         continue
-                    dst = sector(*tn)
+                    dst = sector(tkey)
                     if dst.dim <= full_ed_switch:
-                        phi = _excite(src, dst, op, spin, pm, st.vector)
-                        ev, vec = held_states(tn)
+                        phi = _excite(src, dst, op, pm, st.vector)
+                        ev, vec = held_states(tkey)
                         G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
                     else:
                         jobs.append((io, dst, iph, op))
-            if jobs and im is not None:
-                phis = [_excite(src, dst, op, spin, +1 if iph == 1 else -1, st.vector) for _, dst, iph, op in jobs]
+            if jobs and device_vectors:
+                # the state's vector goes to the device once; its excitation vectors are
+                # built there and all of its runs advance together
+                spin = jobs[0][3][0][1]
+                dyn = dynamic_from_state(src, st.vector, st.energy,
+                                         [(d, iph, [(c, coef) for c, _, coef in op]) for _, d, iph, op in jobs], spin,
+                                         nitergreen, wn, cutoff_dynamic, precision)
+            elif jobs:
+                phis = [_excite(src, dst, op, +1 if iph == 1 else -1, st.vector) for _, dst, iph, op in jobs]
                 dyn, _, _ = compute_dynamic_batched([j[1] for j in jobs], phis, [j[2] for j in jobs],
                                                     [st.energy] * len(jobs), nitergreen, wn, cutoff_dynamic,
                                                     precision)
-                for k, (io, *_rest) in enumerate(jobs):
-                    G[io] += wt * dyn[k]
-            elif jobs:
-                # the state's vector goes to the device once; its excitation vectors are
-                # built there and all of its runs advance together
-                dyn = dynamic_from_state(src, st.vector, st.energy, [(d, iph, op) for _, d, iph, op in jobs], spin,
-                                         nitergreen, wn, cutoff_dynamic, precision)
-                for k, (io, *_rest) in enumerate(jobs):
-                    G[io] += wt * dyn[k]
+            for k, (io, *_rest) in enumerate(jobs):
+                G[io] += wt * dyn[k]
     finally:
         for s in sectors.values():
             s.close()
     return G
+
+
+def _updo_ops(model: AIMParams, spin: int, im: dict | None):
+    """make / target / device_vectors of _green_ops for the (N_up, N_dn) sectors."""
+    def target(key, sp, pm):
+        tn = (key[0] + pm, key[1]) if sp == 0 else (key[0], key[1] + pm)
+        return tn if 0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns else None
+    return (lambda key: make_sector(model, key[0], key[1], im)), target, im is None
 
 
 def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
@@ -233,8 +241,8 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
     an excitation Lanczos, and those all advance together in one batched device call.
     im: the states are of the complex model (get_eigen_energies with the same im)."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
-    return _green_ops(model, states, beta, wn, [[(c, 1.0)] for c in orbitals], nitergreen, spin, weight_cutoff,
-                      cutoff_dynamic, precision, full_ed_switch, im)
+    return _green_ops(states, beta, wn, [[(c, spin, 1.0)] for c in orbitals], nitergreen, weight_cutoff,
+                      cutoff_dynamic, precision, full_ed_switch, *_updo_ops(model, spin, im))
 
 
 def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray,
@@ -254,11 +262,12 @@ def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     n = len(orbitals)
     pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
-    ops = [[(c, 1.0)] for c in orbitals] + [[(orbitals[a], 1.0), (orbitals[b], 1.0)] for a, b in pairs]
+    ops = [[(c, spin, 1.0)] for c in orbitals] + [[(orbitals[a], spin, 1.0), (orbitals[b], spin, 1.0)]
+                                                  for a, b in pairs]
     if im is not None:
-        ops += [[(orbitals[a], 1.0), (orbitals[b], -1j)] for a, b in pairs]
-    g = _green_ops(model, states, beta, wn, ops, nitergreen, spin, weight_cutoff, cutoff_dynamic, precision,
-                   full_ed_switch, im)
+        ops += [[(orbitals[a], spin, 1.0), (orbitals[b], spin, -1j)] for a, b in pairs]
+    g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
+                   *_updo_ops(model, spin, im))
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]
@@ -269,4 +278,70 @@ def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float
         else:
             D = -1j * (g[n + len(pairs) + k] - g[a] - g[b])
             G[a, b], G[b, a] = 0.5 * (S + D), 0.5 * (S - D)
+    return G
+
+
+# --------------------------------------------------------------------------------
+# S_z (Nambu) basis: superconducting baths (HAIMsz; apply_Csz; compute_greenAB with Cup / Cdo)
+
+def get_eigen_energies_sz(model: AIMParams, pb=None, pvbc=None, pairing_imp_to_bath: bool = False, sectors=None,
+                          neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128, tolerance: float = 1e-12,
+                          full_ed_switch: int = FULL_ED_SWITCH, precision="f64", full_ed_green: bool = False):
+    """Lowest states of the S_z sectors npart = 0 .. 2 Ns (or `sectors`), filtered to
+    [E0, E0 + dEmax] as get_eigen_energies (solver.h:98-237 with the sz branch of new_H):
+    dense ED at dimension <= full_ed_switch, else the device Lanczos.  States carry the
+    key (npart,) and the eigenvalue of the Nambu H; the physical energy adds the
+    sector's energy_shift (the same for every npart, so weights and G do not change)."""
+    from .hclass import SzSector
+    sectors = [(n,) for n in range(2 * model.ns + 1)] if sectors is None else [tuple(np.atleast_1d(k)) for k in sectors]
+    states: list[EigenState] = []
+    for key in sectors:
+        with SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath) as sec:
+            if sec.dim <= full_ed_switch:
+                ev, vec = sec.full_ed(sec.dim if full_ed_green else min(neigen, sec.dim))
+                found = [(ev[k], vec[k]) for k in range(len(ev))]
+            else:
+                res = sec.lanczos(nitermax=nitermax, neigen=neigen, tolerance=tolerance, precision=precision)
+                found = [(res.evals[k], sec.ground_state(res, k)) for k in range(len(res.evals))]
+        if not found:
+            continue
+        e_min = min(e for e, _ in found)
+        states += [EigenState(key, float(e), v) for e, v in found if full_ed_green or e <= e_min + dEmax]
+    if not states or full_ed_green:
+        return states
+    e0 = min(s.energy for s in states)
+    return [s for s in states if s.energy <= e0 + dEmax]
+
+
+def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, pb=None,
+                        pvbc=None, pairing_imp_to_bath: bool = False, nitergreen: int = 36, orbitals=None,
+                        weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8, precision="f64",
+                        full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
+    """Nambu impurity Green's function G^N_ij(i w_n) = <<d_i ; d_j+>>, shape (2n, 2n, nw), over
+    the Nambu orbitals d_(c, 0) = c_(c, up) (index c) and d_(c, 1) = c+_(c, dn) (index n + c)
+    of the S_z sectors (states from get_eigen_energies_sz with the same pairing):
+    * G^N[c, c'] = G_up,cc'; G^N[n + c, n + c'] = <<c_dn+ ; c_dn>> = -conj(G_dn,cc'(i w_n));
+    * G^N[c, n + c'] = <<c_up ; c_dn>>, the anomalous F (compute_greenAB with apply_Cup /
+      apply_Cdo, green_class_computeAB.f90:23-197; correlations_dyn_stat.h:115).
+    Every d_i+ raises npart by one (apply_Csz, apply_C.f90:277-327: the Nambu orbital of
+    the down spin is created by the physical c_dn), so the off-diagonal elements come
+    from the symmetric combinations (d_i + d_j) of the real H as in compute_green_matrix."""
+    from .hclass import SzSector
+    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+    nambu = [(c, 0) for c in orbitals] + [(c, 1) for c in orbitals]
+    n = len(nambu)
+    pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    ops = [[(c, sp, 1.0)] for c, sp in nambu] + [[(*nambu[a], 1.0), (*nambu[b], 1.0)] for a, b in pairs]
+
+    def target(key, sp, pm):
+        t = key[0] + pm
+        return (t,) if 0 <= t <= 2 * model.ns else None
+
+    g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
+                   lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath), target, False)
+    G = np.zeros((n, n, len(wn)), complex)
+    for a in range(n):
+        G[a, a] = g[a]
+    for k, (a, b) in enumerate(pairs):
+        G[a, b] = G[b, a] = 0.5 * (g[n + k] - g[a] - g[b])
     return G

@@ -390,7 +390,12 @@ int ed_bath_distance(int nc, int nb, const double *eps, const double *vbc, int n
 
 /* apply_Cupdo (apply_C.f90:497-598): phi = c+ (pm = +1) or c (pm = -1) of impurity
  * orbital `site` (0-based), spin 0 (up) / 1 (down), applied to psi of sector `src`;
- * phi lives in sector `dst` ((nup+-1, ndn) or (nup, ndn+-1)).  Host arrays. */
+ * phi lives in sector `dst` ((nup+-1, ndn) or (nup, ndn+-1)).  Host arrays.
+ * S_z sectors (apply_Csz / apply_Csz_updo, apply_C.f90:277-388): creates (pm = +1) or
+ * destroys the Nambu orbital (site, spin), bit spin*Ns + nb + site, with the sign of the
+ * Nambu orbitals occupied above it; dst is the npart + pm sector.  The Nambu orbital of
+ * spin 1 is the particle-hole transformed down orbital: pm = +1 there is the physical
+ * c_dn (the reference's apply_Cdo with '-'), pm = -1 the physical c+_dn. */
 int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm,
                const double *psi, double *phi);
 

@@ -282,3 +282,90 @@ def physical_spectrum(model: AIMParams, npart: int, pb=None, pvbc=None, pairing_
 
 def sz_dim(ns: int, npart: int) -> int:
     return comb(2 * ns, npart)
+
+
+def apply_c_sz(model: AIMParams, npart: int, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
+    """c+ (pm = +1) / c (pm = -1) of the Nambu impurity orbital (site, spin) on a vector of
+    the npart sector, as apply_Csz (ed_solver/apply_C.f90:277-327): over every source
+    state, the image in the npart + pm sector with the sign of the orbitals occupied above
+    the bit.  The Nambu orbital of spin 1 is the particle-hole transformed down orbital,
+    so its creation is the physical c_dn (apply_Cdo with '-', apply_C.f90:393-394)."""
+    ibit, _ = orbital_bits(model)
+    bit = ibit[spin * model.nc + site]
+    src = sz_states(model.ns, npart)
+    dst = sz_states(model.ns, npart + pm)
+    rank = {int(s): i for i, s in enumerate(dst)}
+    phi = np.zeros(len(dst), np.result_type(psi, np.float64))
+    for i, s in enumerate(src):
+        s = int(s)
+        if psi[i] == 0:
+            continue
+        has = (s >> bit) & 1
+        if (pm > 0) == bool(has):
+            continue
+        o = s | (1 << bit) if pm > 0 else s & ~(1 << bit)
+        phi[rank[o]] += _sign_above(s, bit) * psi[i]
+    return phi
+
+
+def nambu_green_restated(model: AIMParams, beta: float, wn: np.ndarray, pb=None, pvbc=None,
+                         pairing_imp_to_bath=False) -> np.ndarray:
+    """G^N_ij(i w_n) = <<d_i ; d_j+>> over the Nambu impurity orbitals d_(c, s) (index
+    s nc + c) from the restated S_z sectors: every npart sector densely diagonalised
+    (tab_sz + hund_sz), d_j+ applied by apply_c_sz, the Lehmann sum over all pairs of
+    eigenstates with Boltzmann weights."""
+    ns, nc = model.ns, model.nc
+    spec = {}
+    for npart in range(2 * ns + 1):
+        t = tab_sz(model, npart, pb, pvbc, pairing_imp_to_bath)
+        spec[npart] = np.linalg.eigh(dense_sz(t, hund_sz(model, npart)))
+    e0 = min(v[0].min() for v in spec.values())
+    Z = sum(np.exp(-beta * (v[0] - e0)).sum() for v in spec.values())
+    n = 2 * nc
+    G = np.zeros((n, n, len(wn)), complex)
+    for npart in range(2 * ns):
+        em, vm = spec[npart]
+        en, vn = spec[npart + 1]
+        X = []
+        for a in range(n):
+            cols = [apply_c_sz(model, npart, a % nc, a // nc, +1, vm[:, m]) for m in range(len(em))]
+            X.append(vn.T @ np.array(cols).T)   # X[n, m] = <n|d_a+|m>
+        W = np.exp(-beta * (em[None, :] - e0)) + np.exp(-beta * (en[:, None] - e0))
+        den = 1j * wn[None, None, :] - (en[:, None] - em[None, :])[:, :, None]
+        for i in range(n):
+            for j in range(n):
+                G[i, j] += ((X[i] * X[j] * W)[:, :, None] / den).sum((0, 1))
+    return G / Z
+
+
+def nambu_green_physical(model: AIMParams, beta: float, wn: np.ndarray, pb=None, pvbc=None,
+                         pairing_imp_to_bath=False) -> np.ndarray:
+    """The same G^N from the physical Hamiltonian (physical_terms) on the whole Fock space
+    of 2 Ns modes, with d_(c, 0) = c_(c, up) and d_(c, 1) = c+_(c, dn): G_up on the
+    up block, <<c_dn+ ; c_dn>> on the down block and the anomalous <<c_up ; c_dn>> off it."""
+    ns, nb, nc = model.ns, model.nb, model.nc
+    allst = np.arange(1 << (2 * ns), dtype=np.int64)
+    N = len(allst)
+    H = np.zeros((N, N))
+    for coef, ops in physical_terms(model, pb, pvbc, pairing_imp_to_bath):
+        out, sg, ok = bf.apply_ops(allst, ops)
+        np.add.at(H, (out[ok], allst[ok]), coef * sg[ok])
+    E, V = np.linalg.eigh(H)
+    e0 = E.min()
+    Z = np.exp(-beta * (E - e0)).sum()
+    X = []
+    for a in range(2 * nc):
+        mode, create = (nb + a, True) if a < nc else (ns + nb + a - nc, False)
+        D = np.zeros((N, N))
+        out, sg, ok = bf.apply_ops(allst, [(mode, create)])
+        D[out[ok], allst[ok]] = sg[ok]
+        X.append(V.T @ D @ V)   # <n|d_a+|m>
+    W = np.exp(-beta * (E[None, :] - e0)) + np.exp(-beta * (E[:, None] - e0))
+    den = 1j * wn[None, None, :] - (E[:, None] - E[None, :])[:, :, None]
+    G = np.zeros((2 * nc, 2 * nc, len(wn)), complex)
+    for i in range(2 * nc):
+        for j in range(2 * nc):
+            M = X[i] * X[j] * W
+            nz = np.abs(M) > 0
+            G[i, j] = (M[nz][:, None] / den[nz]).sum(0)
+    return G / Z

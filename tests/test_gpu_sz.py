@@ -174,8 +174,10 @@ def test_sz_rejections(gpu):
     m, pb, pvbc = sc_model(2, 2, 2.0, seed=1, u_offdiag=1.0)
     s = gpu.SzSector(m, 4, pb)
     s2 = gpu.SzSector(m, 5, pb)
-    with pytest.raises(EDError):
-        apply_C(s, s2, 0, 0, +1, np.ones(s.dim))
+    with pytest.raises(EDError):   # npart 4 -> 5 is c+, not c
+        apply_C(s, s2, 0, 0, -1, np.ones(s.dim))
+    with pytest.raises(EDError):   # one S_z and one (N_up, N_dn) sector
+        apply_C(s, gpu.Sector(m, 2, 2), 0, 0, +1, np.ones(s.dim))
     with pytest.raises(EDError):
         s.set_schedule(8)
     s.close(); s2.close()
@@ -239,3 +241,54 @@ def test_sz_on_the_fly_large_sector_rate(gpu):
     t1, _ = s.hmult_benchmark("f64", 20)
     print(f"S_z dim {s.dim}: H.v table {t0 * 1e3:.1f} us, on the fly {t1 * 1e3:.1f} us", flush=True)
     s.close()
+
+
+def with_hund(m, jh):
+    jj = np.full((m.nc, m.nc), jh)
+    np.fill_diagonal(jj, 0.0)
+    return AIMParams(m.nc, m.nb, m.ec, m.eb, m.vbc, m.u, jj, jh, m.cutoff)
+
+
+@pytest.mark.parametrize("case", [(1, 3, 4, False, None), (2, 2, 4, True, 1.2), (2, 3, 5, False, 1.0)])
+def test_sz_apply_c_bit_exact(gpu, case):
+    """c+ / c of every Nambu impurity orbital (apply_Csz) against the oracle, bit for bit."""
+    nc, nb, npart, pair, uoff = case
+    m, pb, pvbc = sc_model(nc, nb, 2.0, seed=nc + nb, u_offdiag=uoff)
+    src = gpu.SzSector(m, npart, pb, pvbc, pair)
+    r = np.random.default_rng(npart)
+    psi = r.standard_normal(src.dim)
+    for pm in (+1, -1):
+        dst = gpu.SzSector(m, npart + pm, pb, pvbc, pair)
+        for site in range(nc):
+            for spin in (0, 1):
+                got = gpu.apply_C(src, dst, site, spin, pm, psi)
+                ref = osz.apply_c_sz(m, npart, site, spin, pm, psi)
+                assert np.array_equal(got, ref), (site, spin, pm)
+        dst.close()
+    src.close()
+
+
+@pytest.mark.parametrize("case", [(1, 2, 0.0, False), (2, 2, 0.0, True), (2, 2, 0.4, False)])
+def test_sz_nambu_green_vs_physical(gpu, case):
+    """The solver in the S_z basis: every npart sector's states (dense ED, all kept), then
+    the Nambu G^N (normal up, <<c_dn+ ; c_dn>>, anomalous <<c_up ; c_dn>>) against the
+    Lehmann sum of the physical Hamiltonian with pairing over the whole Fock space,
+    through the Lehmann path and through excitation Lanczos (full_ed_switch = 0, as many
+    steps as the largest sector dimension)."""
+    from ed_solver_amd.solver import compute_green_nambu, get_eigen_energies_sz
+    nc, nb, jh, pair = case
+    m, pb, pvbc = sc_model(nc, nb, 2.0, seed=5 + nc + nb, u_offdiag=1.0 if nc > 1 else None)
+    if jh:
+        m = with_hund(m, jh)
+    beta = 6.0
+    wn = np.pi / beta * (2 * np.arange(1, 17) - 1)
+    dmax = max(osz.sz_dim(m.ns, n) for n in range(2 * m.ns + 1))
+    states = get_eigen_energies_sz(m, pb, pvbc, pair, full_ed_switch=dmax, full_ed_green=True)
+    assert len(states) == 4 ** m.ns
+    print(f"Nambu G case {case}: {len(states)} states, largest sector {dmax}", flush=True)
+    G = compute_green_nambu(m, states, beta, wn, pb, pvbc, pair, full_ed_switch=dmax)
+    G_cf = compute_green_nambu(m, states, beta, wn, pb, pvbc, pair, nitergreen=dmax, full_ed_switch=0)
+    ref = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair)
+    assert np.max(np.abs(ref[:nc, nc:])) > 1e-3   # the anomalous part is there
+    assert np.max(np.abs(G - ref)) < 1e-10
+    assert np.max(np.abs(G_cf - ref)) < 1e-9

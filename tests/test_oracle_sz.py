@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import oracle.sz as osz
-from ed_solver_amd.aim import AIMParams
+from ed_solver_amd.aim import AIMParams, kanamori
+import oracle.brute as bf
 
 
 def sc_model(nc, nb, U, seed, u_offdiag=None, spin_split=0.0):
@@ -99,3 +100,73 @@ def test_hund_terms_match_physical_spectrum(nc, nb, npart, pair_ib):
     for bad in ([], [(i, j, -h) for i, j, h in hund]):
         eb = np.sort(np.linalg.eigvalsh(osz.dense_sz(tab, bad))) + osz.energy_shift(m)
         assert np.max(np.abs(eb - ep)) > 1e-3
+
+
+NAMBU_G_CASES = [(1, 2, 0.0, False), (2, 1, 0.5, False), (1, 2, 0.0, True), (2, 1, 0.4, True)]
+
+
+@pytest.mark.parametrize("nc,nb,jh,pair_ib", NAMBU_G_CASES)
+def test_nambu_green_restated_equals_physical(nc, nb, jh, pair_ib):
+    """The Nambu G^N_ij = <<d_i ; d_j+>> from the restated S_z sectors (apply_Csz on every
+    npart sector) equals the same Lehmann sum of the physical Hamiltonian on the whole
+    Fock space: normal blocks, and the anomalous <<c_up ; c_dn>> of the pairing."""
+    r = np.random.default_rng(40 + nc * 3 + nb)
+    m = kanamori(nc, nb, 2.2, jh, r.uniform(-1, 1, nb), r.uniform(0.2, 0.6, nb))
+    pb = r.uniform(-0.3, 0.3, (nb, nb)); pb = (pb + pb.T) / 2
+    pvbc = r.uniform(-0.3, 0.3, (2, nb, nc)) if pair_ib else None
+    beta = 6.0
+    wn = np.pi / beta * (2 * np.arange(1, 9) - 1)
+    Gr = osz.nambu_green_restated(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    Gp = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    assert np.max(np.abs(Gp[:nc, nc:])) > 1e-3   # an anomalous part is present
+    assert np.max(np.abs(Gr - Gp)) < 1e-12
+
+
+def test_nambu_green_without_pairing_is_spin_green():
+    """No pairing: the up block is G_up of the (N_up, N_dn) route, the down block is
+    <<c_dn+ ; c_dn>>(i w) = -conj(G_dn(i w)), the anomalous block vanishes; and a planted
+    mistake in apply_c_sz (the sign counted below the bit) changes G."""
+    r = np.random.default_rng(7)
+    m = kanamori(1, 2, 2.0, 0.0, r.uniform(-1, 1, 2), r.uniform(0.2, 0.6, 2))
+    beta = 5.0
+    wn = np.pi / beta * (2 * np.arange(1, 9) - 1)
+    Gr = osz.nambu_green_restated(m, beta, wn)
+    ns = m.ns
+    spec = {(u, d): np.linalg.eigh(bf.sector_hamiltonian(m, u, d).toarray()) for u in range(ns + 1)
+            for d in range(ns + 1)}
+    e0 = min(v[0].min() for v in spec.values())
+    Z = sum(np.exp(-beta * (v[0] - e0)).sum() for v in spec.values())
+    g = {0: np.zeros(len(wn), complex), 1: np.zeros(len(wn), complex)}
+    for spin in (0, 1):
+        for (u, d), (em, vm) in spec.items():
+            tu, td = (u + 1, d) if spin == 0 else (u, d + 1)
+            if tu > ns or td > ns:
+                continue
+            en, vn = spec[(tu, td)]
+            for a in range(len(em)):
+                amp = vn.T @ bf.apply_c_full(m, u, d, 0, spin, +1, vm[:, a])
+                w = np.exp(-beta * (em[a] - e0)) + np.exp(-beta * (en - e0))
+                g[spin] += (amp ** 2 * w)[:, None].__truediv__(1j * wn[None, :] - (en - em[a])[:, None]).sum(0)
+    assert np.max(np.abs(Gr[0, 0] - g[0] / Z)) < 1e-12
+    assert np.max(np.abs(Gr[1, 1] + np.conj(g[1] / Z))) < 1e-12
+    assert np.max(np.abs(Gr[0, 1])) < 1e-14
+    # the planted sign reaches apply_c_sz only; the tables keep their own hop signs
+    good, good_hop = osz._sign_above, osz._hop
+
+    def hop_with_good_sign(s, to, frm):
+        planted, osz._sign_above = osz._sign_above, good
+        try:
+            return good_hop(s, to, frm)
+        finally:
+            osz._sign_above = planted
+    try:
+        # the (N_up, N_dn) convention: only the orbitals above the bit in its own spin string
+        top = lambda bit: m.ns if bit < m.ns else 2 * m.ns
+        osz._sign_above = lambda s, bit: -1 if bin((s >> (bit + 1)) & ((1 << (top(bit) - bit - 1)) - 1)).count(
+            "1") & 1 else 1
+        osz._hop = hop_with_good_sign
+        bad = osz.nambu_green_restated(m, beta, wn, pb=np.full((2, 2), 0.2))
+    finally:
+        osz._sign_above, osz._hop = good, good_hop
+    ok = osz.nambu_green_restated(m, beta, wn, pb=np.full((2, 2), 0.2))
+    assert np.max(np.abs(bad - ok)) > 1e-4

@@ -61,6 +61,7 @@ SIGNATURES = {
     "ed_sector_hund_sz": (ctypes.c_int, [_vp, _c_int_p, _i32p, _dp]),
     "ed_sector_set_fly": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ed_sector_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ed_sector_create_slater": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "ed_sector_create_c": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "ed_sector_tables_im": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "ed_hmult_c128": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp]),

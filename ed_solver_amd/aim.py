@@ -110,6 +110,24 @@ WORKLOADS = {
 }
 
 
+def slater_tensor(model: AIMParams) -> np.ndarray:
+    """The model's density, Hund exchange and pair-hopping terms as a Slater tensor
+    U(s1, s2, s3, s4) of flag_slater_int (HAIMupdo.f90:905-999): U_aa / 2 at (a, a, a, a),
+    U_ab / 2 at (a, b, b, a), J_ab / 2 at (b, a, b, a) and (b, b, a, a), a != b.  A model
+    with U = J = 0 plus this tensor has the spectrum of the model itself (spin flip and
+    pair hopping come from the exchange and pair entries)."""
+    nc = model.nc
+    u4 = np.zeros((nc, nc, nc, nc))
+    for a in range(nc):
+        u4[a, a, a, a] = model.u[a, a] / 2
+        for b in range(nc):
+            if a != b:
+                u4[a, b, b, a] = model.u[a, b] / 2
+                u4[b, a, b, a] = model.jj[a, b] / 2
+                u4[b, b, a, a] = model.jj[a, b] / 2
+    return u4
+
+
 def synthetic(name: str) -> tuple[AIMParams, tuple[int, int]]:
     """The seeded model of workload `name` and its headline (N_up, N_dn) sector."""
     w = WORKLOADS[name]

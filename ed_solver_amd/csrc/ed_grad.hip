@@ -414,6 +414,8 @@ int ed_hf_gradient_dev(ed_sector *sec, const double *d_psi, double *d_level, dou
 {
     if (!sec || !d_psi) return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: NULL argument");
     if (sec->sz || sec->cplx) return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: real (N_up, N_dn) sectors only");
+    if (!sec->slater.empty())
+        return ed::fail(ED_ERR_ARG, "ed_hf_gradient_dev: the U derivative is of the Kanamori form, not a Slater tensor");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     return ed::hf_gradient(sec, d_psi, d_level, d_v, d_u);
 }
@@ -422,6 +424,8 @@ int ed_hf_gradient(ed_sector *sec, int64_t n, const double *psi, double *d_level
 {
     if (!sec || !psi) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: NULL argument");
     if (sec->sz || sec->cplx) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: real (N_up, N_dn) sectors only");
+    if (!sec->slater.empty())
+        return ed::fail(ED_ERR_ARG, "ed_hf_gradient: the U derivative is of the Kanamori form, not a Slater tensor");
     if (n != sec->dim) return ed::fail(ED_ERR_ARG, "ed_hf_gradient: vector length differs from the sector dimension");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     double *d = nullptr;

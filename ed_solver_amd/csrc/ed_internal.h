@@ -181,6 +181,8 @@ struct ed_sector {
     int nc = 0, nb = 0, ns = 0, nup = 0, ndn = 0, ncfg = 0;
     int64_t dim = 0;
     std::vector<double> ec, eb, vbc, u, jj;
+    // Slater 4-index impurity interaction U(s1, s2, s3, s4) [nc^4] (flag_slater_int); empty = off
+    std::vector<double> slater;
     double jhund = 0.0, cutoff = 0.0;
     int32_t *d_rank = nullptr;       // [2^ns] 0-based rank inside the popcount bucket
     ed::SpinTables up, dn;

@@ -352,10 +352,64 @@ static int build_spin(ed_sector *S, int spin, int n, SpinTables &T)
     return ED_OK;
 }
 
+// The Slater 4-index term of impurity configuration (cu, cd) (flag_slater_int,
+// HAIMupdo.f90:905-999): destroy site4 in spin is2, destroy site3 in spin is1, create
+// site2 in spin is1, create site1 in spin is2; the result adds U(s1, s2, s3, s4) * sign
+// times v(result) to the row.  Signs: orbitals occupied below site4 and below site1 in
+// the is2 string, strictly between site2 and site3 in the is1 string.  The bath orbitals
+// lie below every impurity orbital and are counted twice in the is2 string, so the sign
+// is a function of the impurity bits alone.  Returns (target cu | cd << 8, value), one
+// entry per target.
+static std::vector<std::pair<int32_t, double>> slater_row(const double *u4, int nc, uint32_t cu, uint32_t cd)
+{
+    auto below = [](uint32_t x, int c) { return (__builtin_popcount(x & ((1u << c) - 1u)) & 1) ? -1 : 1; };
+    auto between = [](uint32_t x, int a, int b) {
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        const uint32_t mask = (hi - lo > 1) ? (((1u << (hi - lo - 1)) - 1u) << (lo + 1)) : 0u;
+        return (__builtin_popcount(x & mask) & 1) ? -1 : 1;
+    };
+    std::vector<std::pair<int32_t, double>> out;
+    const uint32_t in[2] = {cu, cd};
+    for (int s4 = 0; s4 < nc; ++s4)
+        for (int is2 = 0; is2 < 2; ++is2) {
+            if (!((in[is2] >> s4) & 1u)) continue;
+            uint32_t a[2] = {in[0], in[1]};
+            const int g1 = below(a[is2], s4);
+            a[is2] &= ~(1u << s4);
+            for (int s3 = 0; s3 < nc; ++s3)
+                for (int is1 = 0; is1 < 2; ++is1) {
+                    if (!((a[is1] >> s3) & 1u)) continue;
+                    uint32_t b[2] = {a[0], a[1]};
+                    b[is1] &= ~(1u << s3);
+                    for (int s2 = 0; s2 < nc; ++s2) {
+                        if ((b[is1] >> s2) & 1u) continue;
+                        const int g2 = g1 * between(b[is1], s2, s3);
+                        uint32_t c[2] = {b[0], b[1]};
+                        c[is1] |= 1u << s2;
+                        for (int s1 = 0; s1 < nc; ++s1) {
+                            if ((c[is2] >> s1) & 1u) continue;
+                            const double u = u4[((s1 * nc + s2) * nc + s3) * nc + s4];
+                            if (u == 0.0) continue;
+                            const int g = g2 * below(c[is2], s1);
+                            uint32_t d[2] = {c[0], c[1]};
+                            d[is2] |= 1u << s1;
+                            const int32_t key = (int32_t)(d[0] | (d[1] << 8));
+                            size_t q = 0;
+                            while (q < out.size() && out[q].first != key) ++q;
+                            if (q == out.size()) out.push_back({key, 0.0});
+                            out[q].second += u * g;
+                        }
+                    }
+                }
+        }
+    return out;
+}
+
 // Interaction diagonal and Hund transitions per impurity configuration pair.  Both act
 // on impurity bits only, and the Hund signs are parities of impurity bits strictly
 // between two impurity orbitals, so they tabulate over (cu, cd) in [0, 2^nc)^2
-// (HAIMupdo.f90:785-829 density terms, :1017-1129 spin flip and pair hopping).
+// (HAIMupdo.f90:785-829 density terms, :1017-1129 spin flip and pair hopping).  A Slater
+// tensor adds its diagonal to the first and its other transitions to the second list.
 static int build_interaction(ed_sector *S)
 {
     const int nc = S->nc, ncfg = S->ncfg;
@@ -364,7 +418,7 @@ static int build_interaction(ed_sector *S)
     for (int a = 0; a < nc; ++a)
         for (int b = 0; b < nc; ++b)
             if (a != b && U[a * nc + b] != 0.0) offc = true;
-    S->hund = (std::fabs(S->jhund) > 1e-5 && offc) ? 1 : 0;
+    const bool hund_j = std::fabs(S->jhund) > 1e-5 && offc;
     std::vector<double> uimp((size_t)ncfg * ncfg, 0.0);
     std::vector<int32_t> hptr((size_t)ncfg * ncfg + 1, 0), hcu, hcd, htr;
     std::vector<double> hval;
@@ -391,8 +445,14 @@ static int build_interaction(ed_sector *S)
                         if (d1 && d2) e -= JJ[c1 * nc + c2];
                     }
                 }
+            std::vector<std::pair<int32_t, double>> sl;
+            if (!S->slater.empty()) {
+                sl = slater_row(S->slater.data(), nc, (uint32_t)cu, (uint32_t)cd);
+                for (const auto &t : sl)
+                    if (t.first == (int32_t)(cu | (cd << 8))) e += t.second;
+            }
             uimp[(size_t)cu * ncfg + cd] = e;
-            if (S->hund) {
+            if (hund_j) {
                 for (int c1 = 0; c1 < nc; ++c1) {
                     int u1 = (cu >> c1) & 1, d1 = (cd >> c1) & 1;
                     if (u1 + d1 != 1) continue;
@@ -423,9 +483,15 @@ static int build_interaction(ed_sector *S)
                     }
                 }
             }
+            for (const auto &t : sl)
+                if (t.first != (int32_t)(cu | (cd << 8))) {
+                    hcu.push_back(t.first & 0xFF); hcd.push_back(t.first >> 8);
+                    hval.push_back(t.second);
+                }
             hptr[(size_t)cu * ncfg + cd + 1] = (int32_t)hcu.size();
         }
     S->nhund = (int)hcu.size();
+    S->hund = (hund_j || S->nhund > 0) ? 1 : 0;
     for (size_t t = 0; t < hcu.size(); ++t) htr.push_back(hcu[t] | (hcd[t] << 8));
     std::vector<float> uimp32(uimp.begin(), uimp.end()), hval32(hval.begin(), hval.end());
     ED_TRY(dalloc(&S->d_uimp, uimp.size()));
@@ -472,7 +538,7 @@ static void free_sector(ed_sector *S)
 // (N_up, N_dn) sector; with ec_im (and eb_im, vbc_im when nb > 0) a complex model, whose
 // tables carry complex entries and whose vectors are planar complex (dim = 2 D).
 static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, const double *ec_im = nullptr,
-                         const double *eb_im = nullptr, const double *vbc_im = nullptr)
+                         const double *eb_im = nullptr, const double *vbc_im = nullptr, const double *u4 = nullptr)
 {
     if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create: NULL argument");
     *out = nullptr;
@@ -495,6 +561,7 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
     S->jj.assign(m->jj, m->jj + nc * nc);
     S->jhund = m->jhund;
     S->cutoff = m->cutoff;
+    if (u4) S->slater.assign(u4, u4 + (size_t)nc * nc * nc * nc);
     if (ec_im) {
         S->cplx = 1;
         S->ec_im.assign(ec_im, ec_im + 2 * nc * nc);
@@ -874,6 +941,12 @@ int ed_sector_create_c(const ed_model *model, const double *ec_im, const double 
 int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out)
 {
     return ed::create_sector(model, nup, ndn, out);
+}
+
+int ed_sector_create_slater(const ed_model *model, const double *u4, int nup, int ndn, ed_sector **out)
+{
+    if (!u4) return ed::fail(ED_ERR_ARG, "ed_sector_create_slater: NULL u4");
+    return ed::create_sector(model, nup, ndn, out, nullptr, nullptr, nullptr, u4);
 }
 
 int ed_sector_destroy(ed_sector *sec)

@@ -56,12 +56,20 @@ class LanczosResult:
 class Sector:
     """One (N_up, N_dn) sector of an Anderson impurity model on the current GPU."""
 
-    def __init__(self, model: AIMParams, nup: int, ndn: int):
+    def __init__(self, model: AIMParams, nup: int, ndn: int, slater: np.ndarray | None = None):
+        """slater: the 4-index impurity interaction U(s1, s2, s3, s4) [nc]^4 of
+        flag_slater_int (HAIMupdo.f90:905-999), added to the model's U / J terms."""
         self.model = model
         self.nup, self.ndn = int(nup), int(ndn)
         self._cm = CModel.from_params(model)
         h = ctypes.c_void_p()
-        check(lib().ed_sector_create(ctypes.byref(self._cm), self.nup, self.ndn, ctypes.byref(h)))
+        if slater is None:
+            check(lib().ed_sector_create(ctypes.byref(self._cm), self.nup, self.ndn, ctypes.byref(h)))
+        else:
+            nc = model.nc
+            self._u4 = np.ascontiguousarray(slater, np.float64).reshape(nc, nc, nc, nc)
+            check(lib().ed_sector_create_slater(ctypes.byref(self._cm), dptr(self._u4), self.nup, self.ndn,
+                                                ctypes.byref(h)))
         self._h = h
         self.dim = int(lib().ed_sector_dim(h))
         du, dd = ctypes.c_int32(), ctypes.c_int32()

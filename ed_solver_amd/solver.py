@@ -40,19 +40,23 @@ def all_sectors(ns: int) -> list[tuple[int, int]]:
     return [(nu, nd) for nu in range(ns + 1) for nd in range(ns + 1)]
 
 
-def make_sector(model: AIMParams, nu: int, nd: int, im: dict | None = None) -> Sector:
+def make_sector(model: AIMParams, nu: int, nd: int, im: dict | None = None, slater=None) -> Sector:
     """The (N_up, N_dn) sector: real (H_class) or, when im holds the imaginary parts of
     Ec / Eb / Vbc ({"ec": [2, nc, nc], "eb": [2, nb, nb], "vbc": [2, nb, nc]}), complex
-    (Hmultc, HAIMupdo.f90:150-656)."""
+    (Hmultc, HAIMupdo.f90:150-656); slater adds the 4-index impurity interaction
+    (flag_slater_int) to a real sector."""
     if im is None:
-        return Sector(model, nu, nd)
+        return Sector(model, nu, nd, slater)
+    if slater is not None:
+        raise ValueError("the Slater interaction is built for real sectors")
     return ComplexSector(model, nu, nd, im["ec"], im.get("eb"), im.get("vbc"))
 
 
 def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128,
                        tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
                        precision="f64", which_lanczos: str = "GPU", neigen_arpack: int | None = None,
-                       arpack_ncv: int | None = None, full_ed_green: bool = False, im: dict | None = None):
+                       arpack_ncv: int | None = None, full_ed_green: bool = False, im: dict | None = None,
+                       slater=None):
     """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax].
 
     Per sector (solver.h:150-200): block Lanczos when block_size > 0, unless N_up or N_dn
@@ -62,7 +66,8 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
     (ED_ARPACK.f90:652-665) and neither window filter runs (solver.h:211,
     solver.f90:93), so compute_green's Lehmann sums over small sectors are complete.
     im (imaginary parts of Ec / Eb / Vbc, see make_sector): complex sectors, where dense
-    ED and the plain device Lanczos run; block Lanczos and ARPACK are real only."""
+    ED and the plain device Lanczos run; block Lanczos and ARPACK are real only.  slater:
+    the 4-index impurity interaction U(s1, s2, s3, s4) added to every sector."""
     sectors = all_sectors(model.ns) if sectors is None else sectors
     if im is not None and (block_size > 0 or which_lanczos == "ARPACK"):
         raise ValueError("complex sectors: block Lanczos and ARPACK are real only")
@@ -71,7 +76,7 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
     for (nu, nd) in sectors:
         dim = comb(ns, nu) * comb(ns, nd)
         plain = block_size == 0 or nu in (0, 1) or nd in (0, 1) or (nu == ns and nd == ns)
-        with make_sector(model, nu, nd, im) as sec:
+        with make_sector(model, nu, nd, im, slater) as sec:
             if dim <= full_ed_switch or (plain and which_lanczos == "FULL_ED"):
                 ev, vec = sec.full_ed(dim if full_ed_green else min(neigen, dim))
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
@@ -220,17 +225,18 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
     return G
 
 
-def _updo_ops(model: AIMParams, spin: int, im: dict | None):
+def _updo_ops(model: AIMParams, spin: int, im: dict | None, slater=None):
     """make / target / device_vectors of _green_ops for the (N_up, N_dn) sectors."""
     def target(key, sp, pm):
         tn = (key[0] + pm, key[1]) if sp == 0 else (key[0], key[1] + pm)
         return tn if 0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns else None
-    return (lambda key: make_sector(model, key[0], key[1], im)), target, im is None
+    return (lambda key: make_sector(model, key[0], key[1], im, slater)), target, im is None
 
 
 def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
                   spin: int = 0, orbitals=None, weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8,
-                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None) -> np.ndarray:
+                  precision="f64", full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None,
+                  slater=None) -> np.ndarray:
     """Diagonal impurity G_cc(i w_n), one row per orbital c (green_class_computeAA.h:101-302).
 
     For every state with Boltzmann weight above weight_cutoff and every orbital: the
@@ -239,16 +245,17 @@ def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: n
     of dimension <= full_ed_switch takes the Lehmann sum over the states `states` holds
     for it (compute_dynamic_full_ed, :77-165; none held gives zero), every other run is
     an excitation Lanczos, and those all advance together in one batched device call.
-    im: the states are of the complex model (get_eigen_energies with the same im)."""
+    im / slater: the states are of that model (get_eigen_energies with the same im / slater)."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     return _green_ops(states, beta, wn, [[(c, spin, 1.0)] for c in orbitals], nitergreen, weight_cutoff,
-                      cutoff_dynamic, precision, full_ed_switch, *_updo_ops(model, spin, im))
+                      cutoff_dynamic, precision, full_ed_switch, *_updo_ops(model, spin, im, slater))
 
 
 def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray,
                          nitergreen: int = 36, spin: int = 0, orbitals=None, weight_cutoff: float = 0.0,
                          cutoff_dynamic: float = 1e-8, precision="f64",
-                         full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None) -> np.ndarray:
+                         full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None,
+                         slater=None) -> np.ndarray:
     """Impurity G_ij(i w_n) = <<c_i ; c_j+>> of one spin, all orbital pairs, shape (norb, norb, nw).
 
     symmetric_combineAA (green_class_compute_symmetric_combination.f90:39-137) and
@@ -267,7 +274,7 @@ def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float
     if im is not None:
         ops += [[(orbitals[a], spin, 1.0), (orbitals[b], spin, -1j)] for a, b in pairs]
     g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
-                   *_updo_ops(model, spin, im))
+                   *_updo_ops(model, spin, im, slater))
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]

@@ -111,6 +111,19 @@ const char *ed_version(void);
  * (HAIM_offdiag_arrays.f90:89-148, 210-606) with integer kernels on the device. */
 int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out);
 
+/* new_H with the Slater 4-index impurity interaction (flag_slater_int, HAIMupdo.f90:
+ * 905-999; Slater_Coulomb_r): u4[nc][nc][nc][nc] row-major, U(s1, s2, s3, s4) at
+ * ((s1 nc + s2) nc + s3) nc + s4, added to the model's U / J terms.  The H.v gathers
+ * w_i += U(s1, s2, s3, s4) sign v_j over the states j reached from i by destroying site4
+ * (spin is2) and site3 (spin is1), then creating site2 (spin is1) and site1 (spin is2),
+ * for both spins is1, is2, with the reference's signs (orbitals occupied below site4 /
+ * site1 in the is2 string, strictly between site2 and site3 in the is1 string).  The term
+ * acts on impurity bits only and is tabulated per impurity configuration pair: its
+ * diagonal joins the density terms, its transitions the Hund list, so every H.v and
+ * Lanczos kernel takes it.  ed_hf_gradient rejects such sectors (its U derivative is of
+ * the Kanamori form). */
+int ed_sector_create_slater(const ed_model *model, const double *u4, int nup, int ndn, ed_sector **out);
+
 /* new_H for a complex AIM (the COMPLEX(8) branch of tab_HAIM2 and HAIMupdo_multc,
  * HAIMupdo.f90:150-656): the model's ec/eb/vbc are the real parts and ec_im[2][nc][nc],
  * eb_im[2][nb][nb], vbc_im[2][nb][nc] the imaginary parts (eb_im, vbc_im may be NULL =
@@ -121,8 +134,9 @@ int ed_sector_create(const ed_model *model, int nup, int ndn, ed_sector **out);
  * dynamic, ed_hmult_f64) has length 2 dimen_H() and holds the planar form
  * (Re[dimen_H], Im[dimen_H]): H acts there as the symmetric [[A, -B], [B, A]] of
  * H = A + iB, whose plain Lanczos has the complex Lanczos' (real) alpha and beta.  FP64
- * only; full ED, block Lanczos, batched dynamic and Hellmann-Feynman gradients take real
- * sectors only. */
+ * only; ed_full_ed gives the complex eigenvectors (planar) up to 512 amplitudes; block
+ * Lanczos, ed_dynamic_from_state and the Hellmann-Feynman gradients take real sectors
+ * only. */
 int ed_sector_create_c(const ed_model *model, const double *ec_im, const double *eb_im, const double *vbc_im,
                        int nup, int ndn, ed_sector **out);
 

@@ -429,7 +429,7 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
 {
     if (!gs || !psi || njobs < 1 || !dst || !iph || !nterm || !sites || !coefs || !dyn)
         return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad argument");
-    if (gs->sz || gs->cplx) return fail(ED_ERR_ARG, "ed_dynamic_from_state: real (N_up, N_dn) sectors only");
+    if (gs->cplx) return fail(ED_ERR_ARG, "ed_dynamic_from_state: real sectors only");
     ED_HIP_CHECK(hipSetDevice(gs->device));
     hipStream_t st = gs->stream;
     // psi once on the device, then every excitation vector built there
@@ -457,12 +457,21 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
         // sum_t coef_t c^(pm)_{site_t} psi (the vectors of symmetric_combineAA,
         // green_class_compute_symmetric_combination.f90:39-137)
         for (int t = 0; t < nterm[k] && rc == ED_OK; ++t) {
-            const int site = sites[t0 + t];
+            // S_z sectors: sites[] are Nambu orbitals spin * nc + site (apply_Csz), `spin` unused
+            int site = sites[t0 + t], tspin = spin;
+            if (gs->sz) {
+                if (site < 0 || site >= 2 * gs->nc) {
+                    rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: Nambu orbital outside [0, 2 nc)");
+                    break;
+                }
+                tspin = site / gs->nc;
+                site %= gs->nc;
+            }
             const double c = coefs[t0 + t];
             if (t == 0 && c == 1.0) {
-                rc = apply_c_dev(gs, dst[k], site, spin, pm, d_psi, phi, st);
+                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, phi, st);
             } else {
-                rc = apply_c_dev(gs, dst[k], site, spin, pm, d_psi, d_tmp, st);
+                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, d_tmp, st);
                 if (rc == ED_OK) {
                     if (t == 0) ED_HIP_CHECK(hipMemsetAsync(phi, 0, sizeof(double) * dst[k]->dim, st));
                     hipLaunchKernelGGL(k_axpy, dim3(egrid(dst[k]->dim)), dim3(kBlock), 0, st, c, (const double *)d_tmp,

@@ -517,7 +517,8 @@ def dynamic_full_ed(iph: int, energy: float, phi: np.ndarray, evals: np.ndarray,
 def dynamic_from_state(gs: Sector, psi: np.ndarray, energy: float, jobs, spin: int, nitergreen: int,
                        wn: np.ndarray, cutoff_dynamic: float = 1e-8, precision="f64") -> np.ndarray:
     """The excitation runs of one eigenstate with every vector on the device: jobs is a list
-    of (dst_sector, iph, [(site, coef), ...]); returns dyn (njobs, nw) complex."""
+    of (dst_sector, iph, [(site, coef), ...]); returns dyn (njobs, nw) complex.  For an
+    S_z sector a site is the Nambu orbital spin * nc + site and `spin` is unused."""
     psi = gs._pack(psi, "dynamic_from_state: psi")
     nj = len(jobs)
     wn = np.ascontiguousarray(wn, np.float64)

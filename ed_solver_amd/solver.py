@@ -158,8 +158,7 @@ def _excite(src: Sector, dst: Sector, op, pm: int, psi: np.ndarray) -> np.ndarra
 
 
 def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, nitergreen: int, weight_cutoff: float,
-               cutoff_dynamic: float, precision, full_ed_switch: int, make, target,
-               device_vectors: bool) -> np.ndarray:
+               cutoff_dynamic: float, precision, full_ed_switch: int, make, target, device_terms) -> np.ndarray:
     """sum over states of boltz * (particle + hole) for each operator A = sum coef c_(site, spin)
     of `ops` (a list of [(site, spin, coef), ...]): the particle run from A+|psi>, the hole
     run from A|psi> (green_class_computeAA.h:101-302 with the vectors of
@@ -167,9 +166,10 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
     make(key) opens the sector of a state's key, target(key, spin, pm) gives the key of
     the sector A+ (pm = +1) or A (pm = -1) leads to, or None.  Small target sectors take
     the Lehmann sum over the held states, every other run is an excitation Lanczos.  With
-    device_vectors (real (N_up, N_dn) sectors) a state's excitation vectors are built on
-    the device from one copy of the state; otherwise apply_C builds them per operator
-    term.  Either way a state's runs advance together in one batched device call."""
+    device_terms (real sectors: op -> (spin, [(site index, coef), ...]) of
+    ed_dynamic_from_state) a state's excitation vectors are built on the device from one
+    copy of the state; otherwise (complex sectors) apply_C builds them per operator term.
+    Either way a state's runs advance together in one batched device call."""
     w = partition(states, beta)
     G = np.zeros((len(ops), len(wn)), complex)
     sectors: dict[tuple, Sector] = {}
@@ -205,12 +205,12 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
                         G[io] += wt * dynamic_full_ed(iph, st.energy, phi, ev, vec, wn, cutoff_dynamic)
                     else:
                         jobs.append((io, dst, iph, op))
-            if jobs and device_vectors:
+            if jobs and device_terms is not None:
                 # the state's vector goes to the device once; its excitation vectors are
                 # built there and all of its runs advance together
-                spin = jobs[0][3][0][1]
+                dev = [device_terms(op) for _, _, _, op in jobs]
                 dyn = dynamic_from_state(src, st.vector, st.energy,
-                                         [(d, iph, [(c, coef) for c, _, coef in op]) for _, d, iph, op in jobs], spin,
+                                         [(d, iph, t) for (_, d, iph, _), (_, t) in zip(jobs, dev)], dev[0][0],
                                          nitergreen, wn, cutoff_dynamic, precision)
             elif jobs:
                 phis = [_excite(src, dst, op, +1 if iph == 1 else -1, st.vector) for _, dst, iph, op in jobs]
@@ -226,11 +226,12 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
 
 
 def _updo_ops(model: AIMParams, spin: int, im: dict | None, slater=None):
-    """make / target / device_vectors of _green_ops for the (N_up, N_dn) sectors."""
+    """make / target / device_terms of _green_ops for the (N_up, N_dn) sectors."""
     def target(key, sp, pm):
         tn = (key[0] + pm, key[1]) if sp == 0 else (key[0], key[1] + pm)
         return tn if 0 <= tn[0] <= model.ns and 0 <= tn[1] <= model.ns else None
-    return (lambda key: make_sector(model, key[0], key[1], im, slater)), target, im is None
+    terms = (lambda op: (op[0][1], [(c, coef) for c, _, coef in op])) if im is None else None
+    return (lambda key: make_sector(model, key[0], key[1], im, slater)), target, terms
 
 
 def compute_green(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, nitergreen: int = 36,
@@ -332,7 +333,9 @@ def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float,
       apply_Cdo, green_class_computeAB.f90:23-197; correlations_dyn_stat.h:115).
     Every d_i+ raises npart by one (apply_Csz, apply_C.f90:277-327: the Nambu orbital of
     the down spin is created by the physical c_dn), so the off-diagonal elements come
-    from the symmetric combinations (d_i + d_j) of the real H as in compute_green_matrix."""
+    from the symmetric combinations (d_i + d_j) of the real H as in compute_green_matrix.
+    A state's excitation vectors are built on the device (ed_dynamic_from_state with Nambu
+    orbital indices) and its runs batched."""
     from .hclass import SzSector
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     nambu = [(c, 0) for c in orbitals] + [(c, 1) for c in orbitals]
@@ -344,8 +347,10 @@ def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float,
         t = key[0] + pm
         return (t,) if 0 <= t <= 2 * model.ns else None
 
+    nc = model.nc
     g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
-                   lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath), target, False)
+                   lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath), target,
+                   lambda op: (0, [(sp * nc + c, coef) for c, sp, coef in op]))
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]

@@ -463,7 +463,8 @@ int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *p
  * iph[k] = 1, -1 for iph[k] = 2; nterm[k] consecutive (site, coef) terms, the vectors of
  * symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137), and all
  * runs advance together as in ed_dynamic_batched from energy = E_psi.  Output
- * dyn[k][2*nw].  Real (N_up, N_dn) sectors. */
+ * dyn[k][2*nw].  Real (N_up, N_dn) and S_z sectors; for S_z sectors sites[t] is the
+ * Nambu orbital spin * nc + site of apply_Csz (see ed_apply_c) and `spin` is unused. */
 int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs,
                           ed_sector *const *dst, const int *iph, const int *nterm,
                           const int *sites, const double *coefs, int spin, int nitergreen,

@@ -161,6 +161,11 @@ static HvArgs<T> hv_args_of(const ed_sector *s)
     a.hund_ptr = s->d_hund_ptr; a.hund_cu = s->d_hund_cu; a.hund_cd = s->d_hund_cd;
     a.hund_val = (const T *)(f64 ? (void *)s->d_hund_val : (void *)s->d_hund_val32);
     a.swap_up = s->up.swap; a.swap_dn = s->dn.swap;
+    // packed up hops (see hv_rows_body) when the coefficient table fits the LDS budget
+    a.ncoef_up = s->up.ncoef;
+    a.coef_up = (const T *)(f64 ? (void *)s->up.coef : (void *)s->up.coef32);
+    a.ell_pk_up = (s->up.ell_pk && s->up.ncoef <= kMaxRowsCoef) ? s->up.ell_pk : nullptr;
+    a.kdn = s->dn.kpad; a.nhund = s->nhund; a.hund_tr = s->d_hund_tr; a.colord = nullptr;
     return a;
 }
 
@@ -222,7 +227,7 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         J.lz.colord = nullptr;
         J.lz.dup = s->up.d;
         ku = std::max(ku, s->up.kpad);
-        lds = std::max(lds, (size_t)s->up.d * sizeof(T));
+        lds = std::max(lds, ((size_t)s->up.d + (J.a.ell_pk_up ? (size_t)J.a.ncoef_up : 0)) * sizeof(T));
         ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k] + 0, h_phi[k], sizeof(double) * s->dim,
                                     phi_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_scale_into, dim3(egrid(s->dim)), dim3(kBlock), 0, st, (const double *)(base + o_start[k]),

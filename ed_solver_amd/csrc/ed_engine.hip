@@ -465,8 +465,9 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     h.kind = (dup >= 128 && s->up.kpad <= 32) ? 1 : 0;
     if (h.kind == 1) {
         h.cpb = (sched == 2 || sched == 4) ? sched : 1;
-        // LDS holds CPB FP64 columns; stay within 64 KB
-        while (h.cpb > 1 && (size_t)h.cpb * dup * sizeof(double) > 56 * 1024) h.cpb /= 2;
+        // LDS holds CPB FP64 columns and the packed coefficients; stay within 64 KB
+        const size_t nco = (s->up.ell_pk && s->up.ncoef <= kMaxRowsCoef) ? (size_t)s->up.ncoef : 0;
+        while (h.cpb > 1 && ((size_t)h.cpb * dup + nco) * sizeof(double) > 56 * 1024) h.cpb /= 2;
     } else {
         h.cpb = dup >= kBlock ? 1 : std::max(1, kBlock / std::max(1, dup));
     }
@@ -552,7 +553,9 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
         launch_fast<T, MODE>(a, h.k, h.rows, grid, h.nt, lds, st, x, y, ld, lz);
         return;
     }
-    const size_t lds = (size_t)h.cpb * a.dup * sizeof(T);
+    // row kernels: packed up hops when the coefficient table is small enough for LDS
+    if (a.ncoef_up > kMaxRowsCoef) a.ell_pk_up = nullptr;
+    const size_t lds = ((size_t)h.cpb * a.dup + (a.ell_pk_up ? a.ncoef_up : 0)) * sizeof(T);
     if (h.kind == 0) {
         hipLaunchKernelGGL((k_hv_generic<T, MODE>), grid, dim3(kBlock), lds, st, a, x, y, ld, lz);
         return;

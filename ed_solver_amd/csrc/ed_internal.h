@@ -29,6 +29,7 @@ int fail(int code, const std::string &msg);
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kBarGroups = 64;        // grid-barrier groups of the persistent Lanczos kernel
+constexpr int kMaxRowsCoef = 2048;   // packed up-hop coefficients the row kernels keep in LDS
 constexpr int kBarGroupSize = 32;     // workgroups per group
 constexpr int kRed = 34;             // reduction scratch: two slots per wave of a 1024-thread workgroup + 2
 constexpr int kSums = 4;             // LDS slots for the reduced recurrence sums

@@ -185,6 +185,10 @@ __device__ inline void block_partial_last(double v0, double v1, double *red, int
 // MODE 0: y = H x.  MODE 1: Lanczos phase 1 (x, y from the Krylov slots).
 // CLAMP: the ELL width of this job may be below KU (batched kernels); entries k >=
 // a.kup reread entry a.kup-1 with a zero coefficient, so no load leaves the table.
+// With a.ell_pk_up set, the up hops come from the packed table (rank | coefficient
+// index << 16, 4 bytes an entry instead of 12) and the coefficients from an LDS copy
+// placed after the CPB columns (dynamic LDS: col[CPB dup], coef[ncoef_up]); same entries
+// in the same order, so the same sums.
 template <typename T, int MODE, int CPB, int KU, bool CLAMP>
 __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xin, T *__restrict__ yout,
                                     const LzArgs<T> &lz, int blk, T *col, double *red, double *grp)
@@ -217,18 +221,33 @@ __device__ inline void hv_rows_body(const HvArgs<T> &a, const T *__restrict__ xi
     const int ncol = min(CPB, a.ddn - c0);
     const T *xc = x + (int64_t)c0 * dup;
     for (int e = threadIdx.x; e < ncol * dup; e += blockDim.x) col[e] = xc[e];
+    T *cf = col + CPB * dup;
+    const bool pkd = a.ell_pk_up != nullptr;
+    if (pkd)
+        for (int e = threadIdx.x; e < a.ncoef_up; e += blockDim.x) cf[e] = a.coef_up[e];
     __syncthreads();
 
     double dot = 0.0;
     for (int r = threadIdx.x; r < dup; r += blockDim.x) {
         int rk[KU];
         T v[KU];
+        if (pkd) {
+            uint32_t pw[KU];
 #pragma unroll
-        for (int k = 0; k < KU; ++k) {
-            const int kk = CLAMP ? min(k, a.kup - 1) : k;
-            rk[k] = a.ell_rk_up[kk * dup + r];
-            const T vv = a.ell_val_up[kk * dup + r];
-            v[k] = (CLAMP && k >= a.kup) ? (T)0 : vv;
+            for (int k = 0; k < KU; ++k) pw[k] = a.ell_pk_up[(CLAMP ? min(k, a.kup - 1) : k) * dup + r];
+#pragma unroll
+            for (int k = 0; k < KU; ++k) {
+                rk[k] = (int)(pw[k] & 0xFFFFu);
+                v[k] = (CLAMP && k >= a.kup) ? (T)0 : cf[pw[k] >> 16];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KU; ++k) {
+                const int kk = CLAMP ? min(k, a.kup - 1) : k;
+                rk[k] = a.ell_rk_up[kk * dup + r];
+                const T vv = a.ell_val_up[kk * dup + r];
+                v[k] = (CLAMP && k >= a.kup) ? (T)0 : vv;
+            }
         }
         const T du = a.diag_up[r];
         const int cu = a.imp_up[r];

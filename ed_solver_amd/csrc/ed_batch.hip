@@ -119,6 +119,15 @@ static int egrid(int64_t n)
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
+// Workgroups of a job in the batched element-wise kernels: 8 elements per thread, so the
+// per-step scalar reductions (every workgroup sums the partials of its job) read half as
+// many partials as at 4.
+static int egrid_el(int64_t n)
+{
+    int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
 // Grow-only device work space of the batched runs (one per process; calls serialise).
 struct BatchWS {
     std::mutex mu;
@@ -207,7 +216,7 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
         J.a = hv_args_of<T>(s);
         J.n = s->dim;
         J.nblk_hv = s->dn.d;
-        J.nblk_el = egrid(s->dim);
+        J.nblk_el = egrid_el(s->dim);
         J.blk0_hv = blk_hv;
         J.blk0_el = blk_el;
         blk_hv += J.nblk_hv;

@@ -48,6 +48,14 @@ module ed_hip
       integer(c_int), value      :: nup, ndn
       type(c_ptr), intent(out)   :: sec
     end function
+    integer(c_int) function ed_sector_create_slater(model, u4, nup, ndn, sec) &
+        bind(C, name="ed_sector_create_slater")
+      import :: c_int, c_ptr, ed_model
+      type(ed_model), intent(in) :: model
+      type(c_ptr), value         :: u4
+      integer(c_int), value      :: nup, ndn
+      type(c_ptr), intent(out)   :: sec
+    end function
     integer(c_int) function ed_hmult_c128(sec, n, vec_out, vec_in) bind(C, name="ed_hmult_c128")
       import :: c_int, c_int64_t, c_ptr, c_double_complex
       type(c_ptr), value                   :: sec

@@ -184,13 +184,22 @@ def main():
     # ground-state solves with the convergence tolerance 1e-9 (GS vector included)
     r64b, _, t_e64b, t_gs64b = gs_timed("f64", 1e-9)
     r32b, _, t_e32b, t_gs32b = gs_timed("f32", 1e-9)
+    rrqb, _, t_erqb, t_gsrqb = gs_timed("mixed_rq", 1e-9)
+    rq_acc, rq_r2, rq_gap = sec.rq_info()
     mixed_1e8 = {"tolerance": 1e-9, "gs_ms_fp64": t_gs64b * 1e3, "gs_ms_mixed": t_gs32b * 1e3,
                  "gs_speedup": t_gs64b / t_gs32b, "e0_ms_fp64": t_e64b * 1e3, "e0_ms_mixed": t_e32b * 1e3,
                  "e0_speedup": t_e64b / t_e32b, "abs_dE0_fp64": abs(float(r64b.evals[0]) - float(r64.evals[0])),
                  "abs_dE0_mixed": abs(float(r32b.evals[0]) - float(r64.evals[0])), "iters_fp64": r64b.iters,
-                 "iters_mixed_fp32_phase": r32b.iters_f32, "iters_mixed_fp64_phase": r32b.iters}
+                 "iters_mixed_fp32_phase": r32b.iters_f32, "iters_mixed_fp64_phase": r32b.iters,
+                 # the Rayleigh form: FP32 Lanczos, then E0 from one FP64 H.v of its Ritz vector
+                 "rayleigh_form": {"e0_ms": t_erqb * 1e3, "e0_speedup": t_e64b / t_erqb, "gs_ms": t_gsrqb * 1e3,
+                                   "gs_speedup": t_gs64b / t_gsrqb,
+                                   "abs_dE0": abs(float(rrqb.evals[0]) - float(r64.evals[0])),
+                                   "accepted": rq_acc, "res2": rq_r2, "ritz_gap": rq_gap,
+                                   "iters_fp32": rrqb.iters_f32}}
     log(f"GS at tolerance 1e-9: FP64 E0 {t_e64b*1e3:.1f} ms (+psi {t_gs64b*1e3:.1f}), mixed E0 {t_e32b*1e3:.1f} ms "
-        f"(+psi {t_gs32b*1e3:.1f}) (|dE0| {mixed_1e8['abs_dE0_mixed']:.1e})")
+        f"(+psi {t_gs32b*1e3:.1f}) (|dE0| {mixed_1e8['abs_dE0_mixed']:.1e}); Rayleigh form E0 {t_erqb*1e3:.1f} ms "
+        f"({t_e64b / t_erqb:.2f}x, accepted {rq_acc}, |dE0| {mixed_1e8['rayleigh_form']['abs_dE0']:.1e})")
 
     scan = None
     if not args.no_scan:
@@ -232,7 +241,7 @@ def main():
         # step, 11.8M-element vectors): E0 and psi at tolerance 1e-9, FP64 against mixed,
         # |dE0| against the tolerance-1e-12 E0 above
         mixE = {}
-        for prec in ("f64", "f32"):
+        for prec in ("f64", "f32", "mixed_rq"):
             sE.lanczos(nitermax=300, neigen=1, tolerance=1e-9, precision=prec)   # warm-up
             t0 = time.perf_counter()
             rr = sE.lanczos(nitermax=300, neigen=1, tolerance=1e-9, precision=prec)
@@ -246,6 +255,13 @@ def main():
                    "abs_dE0_mixed": abs(float(mixE["f32"][2].evals[0]) - float(rE.evals[0])),
                    "iters_fp64": mixE["f64"][2].iters, "iters_mixed_fp32_phase": mixE["f32"][2].iters_f32,
                    "iters_mixed_fp64_phase": mixE["f32"][2].iters}
+        acc, r2, gap = sE.rq_info()
+        mixed_e["rayleigh_form"] = {"e0_ms": mixE["mixed_rq"][0] * 1e3,
+                                    "e0_speedup": mixE["f64"][0] / mixE["mixed_rq"][0],
+                                    "gs_ms": mixE["mixed_rq"][1] * 1e3,
+                                    "abs_dE0": abs(float(mixE["mixed_rq"][2].evals[0]) - float(rE.evals[0])),
+                                    "accepted": acc, "res2": r2, "ritz_gap": gap,
+                                    "iters_fp32": mixE["mixed_rq"][2].iters_f32}
         log(f"Ns=14 mixed at tolerance 1e-9: E0 FP64 {mixed_e['e0_ms_fp64']:.1f} ms ({mixed_e['iters_fp64']} it), "
             f"mixed {mixed_e['e0_ms_mixed']:.1f} ms ({mixed_e['iters_mixed_fp32_phase']} FP32 + "
             f"{mixed_e['iters_mixed_fp64_phase']} FP64 it) -> {mixed_e['e0_speedup']:.2f}x, "

@@ -27,6 +27,7 @@ ED_ERR_STATE = -5
 ED_ERR_CF = -6
 ED_PREC_F64 = 0
 ED_PREC_F32 = 1
+ED_PREC_MIXED_RQ = 2
 
 
 class LanczosOpts(ctypes.Structure):
@@ -84,6 +85,7 @@ SIGNATURES = {
                                               _dp, _dp]),
     "ed_lanczos_info": (ctypes.c_int, [_vp, _c_int_p]),
     "ed_lanczos_steps_run": (ctypes.c_int, [_vp, _c_int_p]),
+    "ed_lanczos_rq_info": (ctypes.c_int, [_vp, _c_int_p, _dp, _dp]),
     "ed_full_ed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_gs_vector": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),

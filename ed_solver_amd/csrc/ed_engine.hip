@@ -1047,6 +1047,10 @@ struct PhaseOut {
 struct ConvTest {
     int neig, niter;
     double tol;
+    // > 0 (the FP32 phase of the Rayleigh form): also wait until the Lanczos estimate of the
+    // lowest Ritz pair's residual, r = beta_{n+1} |z_n| of T_n (n = it - 1), satisfies
+    // r^2 <= rq_tol * (theta_2 - theta_1) / 2, the bound the FP64 check then applies
+    double rq_tol = 0.0;
     std::vector<double> last, w;
     ConvTest(int neig_, int niter_, double tol_) : neig(neig_), niter(niter_), tol(tol_), last(neig_, 1e300), w(neig_) {}
     bool ends(int it, const double *alpha, const double *beta)
@@ -1056,6 +1060,16 @@ struct ConvTest {
         bool allc = true;
         for (int k = 0; k < neig; ++k)
             if (!((k < m) && std::fabs(w[k] - last[k]) < tol && it > 2)) allc = false;
+        if (allc && rq_tol > 0.0) {
+            const int n = it - 1;
+            std::vector<double> ev(2), z(2 * (size_t)std::max(n, 2));
+            if (n >= 2 && tridiag_lowest_pairs(n, alpha, beta, 2, ev.data(), z.data()) == ED_OK) {
+                const double r = beta[n] * z[n - 1], gap = ev[1] - ev[0];
+                if (!(gap > 0.0) || r * r > rq_tol * 0.5 * gap) allc = false;
+            } else {
+                allc = false;
+            }
+        }
         const bool one_step = (it == 1) && std::fabs(beta[1]) < 1e-24;
         if (allc || it == niter || one_step) return true;
         for (int k = 0; k < m; ++k) last[k] = w[k];
@@ -1133,13 +1147,14 @@ static int run_monitored(ed_sector *s, const PersistShape &p, int done, int nste
 // `chunk` steps alternate with the tests.  Both give the same scalars, so the same
 // iteration ends the loop.
 static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig, int chunk, int reorth,
-                         const double *h_start, const double *d_start, PhaseOut &out)
+                         const double *h_start, const double *d_start, PhaseOut &out, double rq_tol = 0.0)
 {
     ED_TRY(ensure_state(s));
     ED_TRY(ensure_basis(s, prec, niter + 1));
     s->basis_valid_iters = 0;
     ED_TRY(lz_start(s, prec, h_start, d_start));
     ConvTest ct(neig, niter, tol);
+    ct.rq_tol = rq_tol;
     std::vector<double> alpha(niter + 1), beta(niter + 2);
     HostLz h;
     int done = 0, final_iter = 0, tested = 0;
@@ -1186,16 +1201,68 @@ static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig
 
 static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_psi);
 
+// Per-workgroup partial sums of <x, x>, <x, y>, <y, y> (grid-stride, fixed order).
+__global__ __launch_bounds__(kBlock) void k_dot3_part(const double *x, const double *y, int64_t n, double *part)
+{
+    __shared__ double red[kRed];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = x[i], yi = y[i];
+        a += xi * xi;
+        b += xi * yi;
+        c += yi * yi;
+    }
+    a = block_sum_all(a, red);
+    b = block_sum_all(b, red);
+    c = block_sum_all(c, red);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = a;
+        part[3 * blockIdx.x + 1] = b;
+        part[3 * blockIdx.x + 2] = c;
+    }
+}
+
+// FP64 Rayleigh quotient E = <psi, H psi> / <psi, psi> of the device vector d_psi and the
+// squared residual ||H psi - E psi||^2 / <psi, psi> (one H.v into d_w, one pass of sums).
+static int rayleigh_residual(ed_sector *s, const double *d_psi, double *d_w, double &e, double &res2)
+{
+    hipStream_t st = s->stream;
+    ED_TRY(launch_hv(s, ED_PREC_F64, d_psi, d_w, 0, 1, st));
+    constexpr int kParts = 256;
+    double *d_part = nullptr;
+    ED_HIP_CHECK(hipMallocAsync((void **)&d_part, sizeof(double) * 3 * kParts, st));
+    hipLaunchKernelGGL(k_dot3_part, dim3(kParts), dim3(kBlock), 0, st, d_psi, (const double *)d_w, s->dim, d_part);
+    ED_HIP_CHECK(hipGetLastError());
+    std::vector<double> part(3 * kParts);
+    ED_HIP_CHECK(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFreeAsync(d_part, st);
+    double xx = 0.0, xy = 0.0, yy = 0.0;
+    for (int q = 0; q < kParts; ++q) {
+        xx += part[3 * q];
+        xy += part[3 * q + 1];
+        yy += part[3 * q + 2];
+    }
+    if (!(xx > 0.0)) return fail(ED_ERR_BREAKDOWN, "Rayleigh quotient of a zero vector");
+    e = xy / xx;
+    res2 = std::max(0.0, yy / xx - e * e);
+    return ED_OK;
+}
+
 // Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134).  ED_PREC_F64 is the reference
 // algorithm.  ED_PREC_F32 is the mixed-precision form: the loop runs on FP32 vectors
 // and FP32 H.v (FP64 dots and scalars) to |dlambda| < max(tol, 1e-6), then an FP64
 // Lanczos restarted on the device from that Ritz vector finishes to `tol`; the
-// returned T, eigenpairs and kept basis are those of the FP64 phase.
+// returned T, eigenpairs and kept basis are those of the FP64 phase.  ED_PREC_MIXED_RQ
+// (one eigenvalue): after the FP32 phase, the FP64 Rayleigh quotient of its Ritz vector
+// is E0 when ||r||^2 <= tol * (lambda_2 - E) / 2; the outputs are then those of the FP32
+// phase with evals[0] = E.  Otherwise (and for neigen > 1) it continues as ED_PREC_F32.
 static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const double *start, int *iters_out,
                                int *neig_out, double *evals, double *vecp, double *diag, double *subdiag)
 {
-    ED_TRY(check_common(s, o ? o->precision : 0));
     if (!o || !iters_out) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: NULL argument");
+    const bool rq = o->precision == ED_PREC_MIXED_RQ;
+    ED_TRY(check_common(s, rq ? ED_PREC_F32 : o->precision));
     const int64_t D = s->cplx ? s->cdim : s->dim;   // Niter = min(dimen_H - 1, Nitermax)
     const int niter = (int)std::min<int64_t>(D - 1, o->nitermax);
     if (niter < 1) return fail(ED_ERR_ARG, "ed_lanczos_diagonalize: sector dimension 1 has no Lanczos step (full ED case)");
@@ -1207,15 +1274,40 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
     PhaseOut ph;
     s->last_f32_iters = 0;
     s->last_steps_run = 0;
-    if (o->precision == ED_PREC_F32) {
+    s->last_rq_accepted = 0;
+    s->last_rq_res2 = s->last_rq_gap = 0.0;
+    if (o->precision == ED_PREC_F32 || rq) {
         PhaseOut p32;
         ED_TRY(lanczos_phase(s, ED_PREC_F32, std::max(o->tolerance, 1e-6), niter, 1, chunk, o->reorth, start, nullptr,
-                             p32));
-        std::vector<double> ev(1), z((size_t)p32.iters);
-        ED_TRY(tridiag_lowest_pairs(p32.iters, p32.alpha.data(), p32.beta.data(), 1, ev.data(), z.data()));
+                             p32, rq && neig == 1 ? o->tolerance : 0.0));
+        // the RQ form also takes the second Ritz value of the final T for the gap of its bound
+        const int k32 = std::min(rq ? 2 : 1, p32.iters);
+        std::vector<double> ev(k32), z((size_t)p32.iters * k32);
+        ED_TRY(tridiag_lowest_pairs(p32.iters, p32.alpha.data(), p32.beta.data(), k32, ev.data(), z.data()));
         ED_TRY(ensure_io(s));
         ED_TRY(gs_vector_dev(s, p32.iters, z.data(), s->d_io_b));
         s->last_f32_iters = p32.iters;
+        if (rq && neig == 1 && k32 == 2) {
+            // E0 from the FP64 Rayleigh quotient of the FP32 Ritz vector: its error is at most
+            // ||r||^2 / (lambda_2 - E) (Kato-Temple).  lambda_2 is estimated by the second Ritz
+            // value, an upper bound of it, so the test takes half the estimated gap.
+            double e = 0.0, r2 = 0.0;
+            ED_TRY(rayleigh_residual(s, s->d_io_b, s->d_io_a, e, r2));
+            const double gap = ev[1] - e;
+            s->last_rq_res2 = r2;
+            s->last_rq_gap = gap;
+            if (gap > 0.0 && r2 <= o->tolerance * 0.5 * gap) {
+                s->last_rq_accepted = 1;
+                if (evals) evals[0] = e;
+                if (vecp)
+                    for (int r = 0; r < p32.iters; ++r) vecp[r] = z[r];
+                if (diag) std::memcpy(diag, p32.alpha.data(), sizeof(double) * p32.iters);
+                if (subdiag) std::memcpy(subdiag, p32.beta.data(), sizeof(double) * p32.iters);
+                *iters_out = p32.iters;
+                if (neig_out) *neig_out = 1;
+                return ED_OK;
+            }
+        }
         ED_TRY(lanczos_phase(s, ED_PREC_F64, o->tolerance, niter, neig, chunk, o->reorth, nullptr, s->d_io_b, ph));
     } else {
         ED_TRY(lanczos_phase(s, ED_PREC_F64, o->tolerance, niter, neig, chunk, o->reorth, start, nullptr, ph));
@@ -1504,6 +1596,15 @@ int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase)
 {
     ED_TRY(check_common(sec, 0));
     if (iters_f32_phase) *iters_f32_phase = sec->last_f32_iters;
+    return ED_OK;
+}
+
+int ed_lanczos_rq_info(ed_sector *sec, int *accepted, double *res2, double *gap)
+{
+    ED_TRY(check_common(sec, 0));
+    if (accepted) *accepted = sec->last_rq_accepted;
+    if (res2) *res2 = sec->last_rq_res2;
+    if (gap) *gap = sec->last_rq_gap;
     return ED_OK;
 }
 

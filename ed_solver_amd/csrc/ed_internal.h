@@ -203,6 +203,10 @@ struct ed_sector {
     double *d_io_a = nullptr, *d_io_b = nullptr;   // host-API staging vectors [dim]
     ed::GraphCache graphs[4];      // captured Lanczos chunks, key = precision * 2 + reorth
     int last_f32_iters = 0;
+    // last ED_PREC_MIXED_RQ solve: accepted without the FP64 phase, squared residual of the
+    // Ritz vector, Ritz gap lambda_2 - E
+    int last_rq_accepted = 0;
+    double last_rq_res2 = 0.0, last_rq_gap = 0.0;
     int cpb = 0;                   // H.v schedule chosen by the autotuner: 1/2/4 = row kernel with that many
                                    // columns per workgroup, 8 = fast kernel, 16 = fast kernel with the
                                    // XCD-grouped column order; 0 = not tuned

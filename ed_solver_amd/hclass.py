@@ -23,7 +23,7 @@ import dataclasses
 
 import numpy as np
 
-from ._lib import (ED_PREC_F32, ED_PREC_F64, EDError, LanczosOpts, check, dptr, iptr, lib)
+from ._lib import (ED_PREC_F32, ED_PREC_F64, ED_PREC_MIXED_RQ, EDError, LanczosOpts, check, dptr, iptr, lib)
 from .aim import AIMParams, CModel
 
 __all__ = ["Sector", "LanczosResult", "new_H", "delete_H", "Hmult__", "dimen_H", "Lanczos_fast_diagonalize",
@@ -31,14 +31,15 @@ __all__ = ["Sector", "LanczosResult", "new_H", "delete_H", "Hmult__", "dimen_H",
            "matsubara", "start_vector", "tridiag_eig"]
 
 _PREC = {"f64": ED_PREC_F64, "fp64": ED_PREC_F64, "f32": ED_PREC_F32, "fp32": ED_PREC_F32,
-         ED_PREC_F64: ED_PREC_F64, ED_PREC_F32: ED_PREC_F32}
+         "mixed_rq": ED_PREC_MIXED_RQ, ED_PREC_F64: ED_PREC_F64, ED_PREC_F32: ED_PREC_F32,
+         ED_PREC_MIXED_RQ: ED_PREC_MIXED_RQ}
 
 
 def _prec(p) -> int:
     try:
         return _PREC[p]
     except KeyError:
-        raise ValueError(f"precision must be 'f64' or 'f32', got {p!r}") from None
+        raise ValueError(f"precision must be 'f64', 'f32' or 'mixed_rq', got {p!r}") from None
 
 
 @dataclasses.dataclass
@@ -216,6 +217,12 @@ class Sector:
         check(lib().ed_lanczos_steps_run(self.handle, ctypes.byref(nrun)))
         return LanczosResult(m, ev[:k].copy(), vecp[:k * m].reshape(k, m).copy(), dg[:m].copy(), sd[:m].copy(),
                              "f64" if o.precision == ED_PREC_F64 else "mixed", i32.value, nrun.value)
+
+    def rq_info(self) -> tuple[bool, float, float]:
+        """(accepted, squared residual, Ritz gap) of the last precision='mixed_rq' solve."""
+        a, r2, g = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().ed_lanczos_rq_info(self.handle, ctypes.byref(a), ctypes.byref(r2), ctypes.byref(g)))
+        return bool(a.value), r2.value, g.value
 
     def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
         """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""

@@ -39,6 +39,8 @@ extern "C" {
 
 #define ED_PREC_F64 0         /* vectors and H.v in FP64 */
 #define ED_PREC_F32 1         /* vectors and H.v in FP32, dot products / recurrence scalars in FP64 */
+#define ED_PREC_MIXED_RQ 2    /* ed_lanczos_diagonalize only: ED_PREC_F32, then E0 from the FP64
+                                 Rayleigh quotient of the FP32 Ritz vector when its bound allows */
 
 /* Anderson impurity model, one real (FP64) parameter set.  All arrays row-major.
  * Replaces the fields of AIM_type / AIM2_type / impurity_type the Hamiltonian
@@ -276,6 +278,16 @@ int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const do
 /* Iterations of the FP32 phase of the last mixed-precision ed_lanczos_diagonalize
  * on this sector (0 after an FP64 call).  No reference counterpart. */
 int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase);
+
+/* The last ED_PREC_MIXED_RQ solve on this sector (no reference counterpart): with one
+ * eigenvalue asked for, the FP32 phase's Ritz vector psi gets one FP64 H.v; its Rayleigh
+ * quotient E replaces the FP64 restart when res2 = ||H psi - E psi||^2 / ||psi||^2 <=
+ * tolerance * gap / 2, gap = (second Ritz value of the FP32 T) - E (the Kato-Temple bound
+ * E - E0 <= res2 / (lambda_2 - E), with the Ritz value, an upper bound of lambda_2, as
+ * the estimate).  *accepted = 1 then, and the outputs of ed_lanczos_diagonalize are those
+ * of the FP32 phase with evals[0] = E (ed_lanczos_gs_vector gives psi from the FP32
+ * basis); otherwise the call continued as ED_PREC_F32. */
+int ed_lanczos_rq_info(ed_sector *sec, int *accepted, double *res2, double *gap);
 
 /* Lanczos steps the device ran in the last ed_lanczos_diagonalize on this sector (both
  * phases of the mixed form).  Where the persistent kernel applies, one launch runs while

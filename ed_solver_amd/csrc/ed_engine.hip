@@ -780,6 +780,9 @@ static bool persist_shape(ed_sector *s, PersistShape &p)
     return s->persist_ok[pi] == 1;
 }
 
+// launch_persist's return when the runtime refuses the cooperative launch (not an ED_ code)
+constexpr int kPersistRefused = 1;
+
 template <typename T>
 static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipStream_t st,
                           LzMonitor mon = LzMonitor{nullptr, nullptr})
@@ -792,8 +795,15 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
     int spins = kBarrierSpins;
     void *args[] = {&a, &lz, &nsteps, &spins, &mon};
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, st, lz.st);
-    ED_HIP_CHECK(hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk), dim3(kBlock), args,
-                                            (unsigned)p.lds, st));
+    const hipError_t e = hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk), dim3(kBlock), args,
+                                                    (unsigned)p.lds, st);
+    if (e != hipSuccess) {
+        // the runtime refused the cooperative launch (nothing ran): this sector leaves the
+        // persistent kernel for this precision and the caller runs the two-kernel step
+        (void)hipGetLastError();
+        s->persist_ok[sizeof(T) == sizeof(double) ? 0 : 1] = 0;
+        return kPersistRefused;
+    }
     hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
     return ED_OK;
 }
@@ -803,13 +813,16 @@ static int run_persist(ed_sector *s, int prec, int nsteps, hipStream_t st, bool 
 {
     PersistShape p;
     done = false;
+    int rc;
     if (prec == ED_PREC_F64) {
         if (!persist_shape<double>(s, p)) return ED_OK;
-        ED_TRY(launch_persist<double>(s, p, nsteps, st));
+        rc = launch_persist<double>(s, p, nsteps, st);
     } else {
         if (!persist_shape<float>(s, p)) return ED_OK;
-        ED_TRY(launch_persist<float>(s, p, nsteps, st));
+        rc = launch_persist<float>(s, p, nsteps, st);
     }
+    if (rc == kPersistRefused) return ED_OK;
+    ED_TRY(rc);
     done = true;
     return ED_OK;
 }
@@ -1165,8 +1178,11 @@ static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig
                          (prec == ED_PREC_F64 ? persist_shape<double>(s, ps) : persist_shape<float>(s, ps));
         if (mon) {
             c = niter - done;
-            if (prec == ED_PREC_F64) ED_TRY(run_monitored<double>(s, ps, done, c, ct, alpha, beta, tested, final_iter));
-            else ED_TRY(run_monitored<float>(s, ps, done, c, ct, alpha, beta, tested, final_iter));
+            const int rc = prec == ED_PREC_F64
+                               ? run_monitored<double>(s, ps, done, c, ct, alpha, beta, tested, final_iter)
+                               : run_monitored<float>(s, ps, done, c, ct, alpha, beta, tested, final_iter);
+            if (rc == kPersistRefused) continue;   // nothing ran; the two-kernel step takes over
+            ED_TRY(rc);
         } else {
             ED_TRY(steps(s, prec, c, chunk, reorth));
         }

@@ -42,7 +42,7 @@ def test_rayleigh_form_ground_energy(gpu, name):
     r_strict = s.lanczos(nitermax=300, neigen=1, tolerance=1e-14, precision="mixed_rq")
     acc2, r2b, gapb = s.rq_info()
     print(f"{name}: accepted res2 {r2:.2e} gap {gap:.3f} ({r.iters_f32} FP32 it); strict res2 {r2b:.2e}", flush=True)
-    assert not acc2 and r_strict.iters != r_strict.iters_f32 or r_strict.iters_f32 == 0 or not acc2
+    assert not acc2 and r2b > 1e-14 * gapb / 2
     assert abs(float(r_strict.evals[0]) - e_ref) < 1e-11
     # two eigenvalues: the FP64 restart as well
     s.lanczos(nitermax=300, neigen=2, tolerance=1e-9, precision="mixed_rq")

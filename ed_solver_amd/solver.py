@@ -136,7 +136,7 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     t_pass2 = time.perf_counter() - t0
     states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
     info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,
-            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2}
+            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2, "e_low": e_low}
     return states, info
 
 

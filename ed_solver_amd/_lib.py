@@ -115,6 +115,8 @@ SIGNATURES = {
     "ed_continued_fraction": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_dynamic": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "ed_dynamic_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
     "ed_dynamic_batched": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_dp), _c_int_p, _dp,
                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                           _dp]),

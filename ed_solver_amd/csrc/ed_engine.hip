@@ -660,19 +660,21 @@ static int ensure_basis(ed_sector *s, int prec, int nslots)
     return ED_OK;
 }
 
+// Krylov slots, leading dimension and slot count, recurrence scalars and partial sums
+// of a recurrence in work space given by the caller (lz_args_at), or the sector's own.
 template <typename T>
-static LzArgs<T> lz_args(ed_sector *s)
+static LzArgs<T> lz_args_at(const ed_sector *s, const LzWork &w, int sched)
 {
     LzArgs<T> lz;
-    lz.base = (T *)s->d_basis;
-    lz.ld = s->ld;
-    lz.nslots = s->basis_nslots;
-    lz.st = s->d_state;
-    lz.partA = s->d_partials + kPartA;
-    lz.partB = s->d_partials + kPartB;
-    lz.partC = s->d_partials + kPartC;
+    lz.base = (T *)w.basis;
+    lz.ld = w.ld;
+    lz.nslots = w.nslots;
+    lz.st = w.st;
+    lz.partA = w.partials + kPartA;
+    lz.partB = w.partials + kPartB;
+    lz.partC = w.partials + kPartC;
     lz.pstride = 8192;
-    const HvShape h = hv_shape(s, sched_of<T>(s));
+    const HvShape h = hv_shape(s, sched);
     lz.nA = h.nblk;
     lz.dup = s->up.d;
     lz.colmap = h.kind == 2 ? 1 : 0;
@@ -682,14 +684,21 @@ static LzArgs<T> lz_args(ed_sector *s)
 }
 
 template <typename T>
-static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t st)
+static LzArgs<T> lz_args(ed_sector *s)
 {
-    const LzArgs<T> lz = lz_args<T>(s);
-    const HvShape h = hv_shape(s, sched_of<T>(s));
+    return lz_args_at<T>(s, LzWork{s->d_basis, s->ld, s->basis_nslots, s->d_state, s->d_partials}, sched_of<T>(s));
+}
+
+// nsteps two-kernel Lanczos steps (H.v phase, update phase, optional CGS2 with its dots
+// in `partials`), then the finishing kernel, as plain launches on stream st.
+template <typename T>
+static void launch_steps_at(ed_sector *s, const LzArgs<T> &lz, const HvShape &h, double *partials, int nsteps,
+                            int reorth, hipStream_t st)
+{
     const int64_t n = s->dim;
     const int g = lz.nB;   // element-wise kernels: one workgroup per partial of partB
     const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(kReorthGrid, (n + kBlock * 16 - 1) / (kBlock * 16)));
-    double *partR = s->d_partials + kPartR, *coef = s->d_partials + kCoef;
+    double *partR = partials + kPartR, *coef = partials + kCoef;
     for (int i = 0; i < nsteps; ++i) {
         hv_launch_mode<T, 1>(s, h, (const T *)nullptr, (T *)nullptr, 0, 1, lz, st);
         hipLaunchKernelGGL(k_lz_update<T>, dim3(g), dim3(kBlock), 0, st, lz, n, reorth ? 0 : 1);
@@ -703,6 +712,12 @@ static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t 
         }
     }
     hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
+}
+
+template <typename T>
+static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t st)
+{
+    launch_steps_at<T>(s, lz_args<T>(s), hv_shape(s, sched_of<T>(s)), s->d_partials, nsteps, reorth, st);
 }
 
 // ---------------------------------------------------------- persistent Lanczos
@@ -1875,5 +1890,45 @@ int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, do
                     int *breakdown_at)
 {
     return lanczos_run(s, niter, nullptr, d_start, prec, 8, diag, subdiag, breakdown_at);
+}
+
+template <typename T>
+static void run_ws_impl(ed_sector *s, const LzWork &w, int sched, int niter, const double *d_start)
+{
+    const LzArgs<T> lz = lz_args_at<T>(s, w, sched);
+    hipLaunchKernelGGL(k_lz_init<T>, dim3(lz.nB), dim3(kBlock), 0, s->stream, lz, d_start, s->dim);
+    launch_steps_at<T>(s, lz, hv_shape(s, sched), w.partials, niter, 0, s->stream);
+}
+
+int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_start, int prec, double *diag,
+                   double *subdiag)
+{
+    ED_TRY(check_common(s, prec));
+    if (niter < 1 || niter > kMaxIter - 2) return fail(ED_ERR_ARG, "lanczos run: niter outside [1, 2046]");
+    if (w.nslots < 3 || w.ld < s->dim) return fail(ED_ERR_ARG, "lanczos run: work space below 3 vectors of dimen_H");
+    // the tuned schedule of that precision, or the default of an untuned sector (the fast
+    // kernel in natural column order where it applies): neither needs an allocation
+    const int tuned = prec == ED_PREC_F32 ? sched_of<float>(s) : sched_of<double>(s);
+    const int sched = tuned ? tuned : (fast_ok(s) ? 8 : 1);
+    if (prec == ED_PREC_F64) run_ws_impl<double>(s, w, sched, niter, d_start);
+    else run_ws_impl<float>(s, w, sched, niter, d_start);
+    ED_HIP_CHECK(hipGetLastError());
+    int hdr[3];
+    ED_HIP_CHECK(hipMemcpyAsync(hdr, w.st, 3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipMemcpyAsync(diag, (char *)w.st + offsetof(LzState, alpha), sizeof(double) * niter,
+                                hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipMemcpyAsync(subdiag, (char *)w.st + offsetof(LzState, beta), sizeof(double) * niter,
+                                hipMemcpyDeviceToHost, s->stream));
+    ED_HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (hdr[2] == 2) return fail(ED_ERR_BREAKDOWN, "Lanczos start vector has zero norm (Lanczos_plain.h:70-76)");
+    const int reached = hdr[0] - 1;   // completed steps
+    for (int i = 0; i < niter; ++i) {
+        if (i >= reached) diag[i] = 0.0;
+        if (i > reached) subdiag[i] = 0.0;
+    }
+    if (hdr[2] == 1 && reached < niter)
+        return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(reached + 1) +
+                                          " (Lanczos_plain.h:85-92)");
+    return ED_OK;
 }
 }  // namespace ed

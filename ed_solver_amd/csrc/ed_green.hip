@@ -157,15 +157,24 @@ int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, 
     return ED_OK;
 }
 
+// k_cfrac over njobs tridiagonals on stream st; *d_err (cleared by the caller) is set
+// when a denominator falls below epsilon.
+int cfrac_launch_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
+                     const double *d_scale, double *d_out, int *d_err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cfrac, dim3((nw + kBlock - 1) / kBlock, njobs), dim3(kBlock), 0, st, n, d_a, d_b, lda, nw,
+                       d_wn, d_scale, d_out, d_err);
+    ED_HIP_CHECK(hipGetLastError());
+    return ED_OK;
+}
+
 int cfrac_batch_dev(int n, const double *d_a, const double *d_b, int lda, int njobs, int nw, const double *d_wn,
                     const double *d_scale, double *d_out, hipStream_t st)
 {
     int *d_err = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_err, sizeof(int), st));
     ED_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_cfrac, dim3((nw + kBlock - 1) / kBlock, njobs), dim3(kBlock), 0, st, n, d_a, d_b, lda, nw,
-                       d_wn, d_scale, d_out, d_err);
-    ED_HIP_CHECK(hipGetLastError());
+    ED_TRY(cfrac_launch_dev(n, d_a, d_b, lda, njobs, nw, d_wn, d_scale, d_out, d_err, st));
     int h_err = 0;
     ED_HIP_CHECK(hipMemcpyAsync(&h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     ED_HIP_CHECK(hipStreamSynchronize(st));

@@ -36,6 +36,10 @@ constexpr int kSums = 4;             // LDS slots for the reduced recurrence sum
 constexpr int kSchedules = 8;        // H.v schedules the autotuner times
 constexpr int kMaxIter = 2048;       // capacity of the on-device alpha/beta arrays
 constexpr double kMinLanczosNorm = 1e-25;  // cutoff_min_lanczos_vec (dmft_ed_solver_variables.h)
+// limits of ed_dynamic_noalloc's static scratch (ed_noalloc.hip)
+constexpr int64_t kNoAllocMaxDim = ED_NOALLOC_MAX_DIM;
+constexpr int kNoAllocMaxIter = ED_NOALLOC_MAX_ITER;
+constexpr int kNoAllocMaxNw = ED_NOALLOC_MAX_NW;
 
 // Layout of the per-sector scratch array of doubles (offsets in doubles).
 constexpr int kPartA = 0;                          // partial sums written by the H.v phase
@@ -167,6 +171,17 @@ struct LzArgs {
     __host__ __device__ double *crs(int j) const { return partC + (j & 1) * pstride; }
 };
 
+// Device work space of one Lanczos recurrence: nslots Krylov slots of ld elements
+// (sized for FP64) at `basis`, the scalars `st` and kScratchDoubles-layout partial sums
+// (only kPartA .. kPartC + 2 * 8192 are used without reorthogonalisation).
+struct LzWork {
+    void *basis;
+    int64_t ld;
+    int nslots;
+    LzState *st;
+    double *partials;
+};
+
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -276,4 +291,13 @@ int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nv
 HvArgs<double> hv_args_f64(const ed_sector *s);
 HvArgs<float> hv_args_f32(const ed_sector *s);
 int start_vector_dev(int64_t n, double *d_out, hipStream_t st);
+// the no-allocation scratch of this device (ed_noalloc.hip); the device must be current
+int noalloc_setup(int device);
+// Fixed-length Lanczos run (as ed_lanczos_run) entirely in the caller's work space w
+// (w.nslots >= 3) with plain two-kernel steps on the sector's stream: no device memory is
+// allocated and the sector's own Krylov basis is left alone.  The H.v schedule is the
+// tuned one, or the untuned default when the sector was never tuned.  diag/subdiag
+// (capacity niter, host) as ed_lanczos_run; ED_ERR_BREAKDOWN as there.
+int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_start, int prec, double *diag,
+                   double *subdiag);
 }  // namespace ed

@@ -31,7 +31,7 @@ int ed_init(int device)
     if (device < 0 || device >= n)
         return ed::fail(ED_ERR_ARG, "ed_init: device " + std::to_string(device) + " out of range");
     ED_HIP_CHECK(hipSetDevice(device));
-    return ED_OK;
+    return ed::noalloc_setup(device);
 }
 
 int ed_lanczos_default_opts(ed_lanczos_opts *o)

@@ -150,6 +150,16 @@ module ed_hip
       real(c_double), intent(out) :: dist
       real(c_double)              :: grad_eps(*), grad_v(*)
     end function
+    ! ed_dynamic without device allocations (static scratch; limits ED_NOALLOC_MAX_* of ed_hip.h)
+    integer(c_int) function ed_dynamic_noalloc(sec, iph, energy, phi, nitergreen, cutoff_dynamic, precision, nw, wn, &
+        dyn, a, b) bind(C, name="ed_dynamic_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: sec
+      integer(c_int), value       :: iph, nitergreen, precision, nw
+      real(c_double), value       :: energy, cutoff_dynamic
+      real(c_double), intent(in)  :: phi(*), wn(*)
+      real(c_double)              :: dyn(*), a(*), b(*)
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

@@ -441,6 +441,23 @@ int ed_dynamic(ed_sector *sec, int iph, double energy, const double *phi, int ni
                double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn,
                double *a, double *b);
 
+/* ed_dynamic for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, meaning, outputs and errors as ed_dynamic; the excitation
+ * Lanczos runs as plain two-kernel steps in one static device array of about 34 MB per
+ * GPU (held until the process ends) instead of the sector's own work space, which it
+ * leaves untouched.  It calls no allocating routine.  Limits: dimen of `sec` (2 dimen_H
+ * for a complex sector) <= ED_NOALLOC_MAX_DIM, min(nitergreen, dimen) <=
+ * ED_NOALLOC_MAX_ITER, nw <= ED_NOALLOC_MAX_NW; beyond them it returns ED_ERR_ARG.  Every
+ * call on one GPU shares that GPU's array (the last call's contents stay there): calls
+ * that overlap on one GPU race. */
+#define ED_NOALLOC_MAX_DIM 1048576
+#define ED_NOALLOC_MAX_ITER 1024
+#define ED_NOALLOC_MAX_NW 16384
+int ed_dynamic_noalloc(ed_sector *sec, int iph, double energy, const double *phi, int nitergreen,
+                       double cutoff_dynamic, int precision, int nw, const double *wn, double *dyn,
+                       double *a, double *b);
+
 /* compute_dynamic_full_ed (green_class_compute_dynamic.f90:77-165), the dynamic
  * correlation of a target sector of dimension dim <= FULL_ED_SWITCH (40,
  * Global_Variables.f90:23) from the nst eigenpairs stored for that sector (evals[nst],

@@ -1408,9 +1408,16 @@ static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_
 {
     ED_TRY(check_common(s, 0));
     if (!vecp || !d_psi || iters < 1) return fail(ED_ERR_ARG, "ed_lanczos_gs_vector: bad argument");
-    if (iters > s->basis_valid_iters || !s->d_basis)
-        return fail(ED_ERR_STATE, "ed_lanczos_gs_vector: no Krylov basis of " + std::to_string(iters) +
-                                      " vectors kept on this sector (run a Lanczos call first)");
+    if (iters > kMaxIter - 1) return fail(ED_ERR_ARG, "ed_lanczos_gs_vector: iters above 2047");
+    if (iters > s->basis_valid_iters || !s->d_basis) {
+        // No kept basis (a fresh handle, as after the reference's second new_H, solver.h:43):
+        // the second pass of Lanczos_get_GS_sector (Lanczos_plain.f90:156-229), the FP64
+        // recurrence again from the deterministic start vector (create_fix_initial_vector,
+        // :196), here with its iters vectors kept for the combination below.
+        const int n = std::max(1, iters - 1);
+        ED_TRY(lanczos_run(s, n, nullptr, nullptr, ED_PREC_F64, 8, nullptr, nullptr, nullptr));
+        s->basis_valid_iters = std::max(s->basis_valid_iters, iters);   // u_1 .. u_{n+1}, beta_1 .. beta_{n+1}
+    }
     hipStream_t st = s->stream;
     double *d_coef = s->d_partials + kMisc;
     ED_HIP_CHECK(hipMemcpyAsync(d_coef, vecp, sizeof(double) * iters, hipMemcpyHostToDevice, st));

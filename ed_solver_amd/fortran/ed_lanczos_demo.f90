@@ -8,7 +8,13 @@
 !      H.v -- the reference's own solver shape on top of the drop-in Hmult__;
 !   2. the coarse device path Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo;
 !   3. Block_Lanczos_diagonalize through ed_block_lanczos (block size 4, 4 states) when
-!      the sector has at least 16 states.
+!      the sector has at least 16 states;
+!   4. the reference's get_GS order (solver.h:43-60): delete_H, new_H again, then
+!      Lanczos_GPU_get_GS_updo on the fresh handle, which runs the second pass of
+!      Lanczos_get_GS_sector (Lanczos_plain.f90:156-229); the vector is written to the
+!      file named by the second argument, when given (raw REAL(8) stream);
+!   5. the S_z branch of new_H (new_H_sz, no pairing) with the host Lanczos loop over
+!      Hmult__, when that sector has at most 10^6 states.
 ! Prints E0 from each, the iteration count and the Rayleigh quotient of the device
 ! ground state.
 program ed_lanczos_demo
@@ -16,16 +22,16 @@ program ed_lanczos_demo
   use ed_hip
   use H_class_hip
   implicit none
-  integer :: nc, nb, nup, ndn, nitermax, n, iter, niter, k
-  real(8), allocatable :: ec(:), eb(:), vbc(:), u(:), jj(:)
+  integer :: nc, nb, nup, ndn, nitermax, n, niter, k, npart, nsz, u_out
+  real(8), allocatable :: ec(:), eb(:), vbc(:), u(:), jj(:), pb(:), pvbc(:)
   real(8) :: jhund, cutoff
-  real(8), allocatable :: vin(:), vout(:), tmp(:), diag(:), subdiag(:), w(:), z(:), psi(:), hpsi(:)
-  real(8) :: normv_in, normv_out, aa, ddiag, last, e0_host, e0_dev, rq
+  real(8), allocatable :: diag(:), subdiag(:), w(:), z(:), psi(:), hpsi(:), psi2(:)
+  real(8) :: e0_host, e0_dev, rq, e0_sz
   type(ed_block_opts_t) :: bo
   real(8) :: valp(4, 2)
   real(8), allocatable :: vecp(:, :)
   integer(c_int) :: nconv, nsteps
-  character(len=512) :: fname
+  character(len=512) :: fname, fout
   integer :: u_in
 
   call get_command_argument(1, fname)
@@ -45,36 +51,10 @@ program ed_lanczos_demo
   call new_H(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, nup, ndn)
   n = dimen_H()
   niter = min(n - 1, nitermax)
-  allocate(vin(n), vout(n), tmp(n), diag(niter), subdiag(niter), w(niter), z(niter*niter))
+  allocate(diag(niter), subdiag(niter), w(niter), z(niter*niter))
 
   ! ---- 1. host Lanczos loop over Hmult__ (one_step_Lanczos_fast + lanc_opt)
-  call ed_check(ed_start_vector(int(n, c_int64_t), vin), 'ed_start_vector')
-  tmp = 0; diag = 0; subdiag = 0
-  normv_in = sqrt(sum(vin*vin)); normv_out = 0; subdiag(1) = normv_in
-  last = huge(1d0); e0_host = 0
-  do iter = 1, niter
-    if (iter == 1) then
-      aa = 1d0 / normv_in
-    else
-      if (abs(normv_out) < 1d-25) stop '0 norm vector in lanczos, critical'
-      aa = 1d0 / normv_out
-    end if
-    vin = vin * aa
-    call Hmult__(n, vout, vin)
-    if (iter > 1) vout = vout - subdiag(iter) * tmp
-    ddiag = sum(vout * vin)
-    diag(iter) = ddiag
-    vout = vout - ddiag * vin
-    normv_out = sqrt(sum(vout * vout))
-    if (iter < niter) subdiag(iter + 1) = normv_out
-    tmp = vin
-    vin = vout
-    call ed_check(ed_tridiag_eig(int(iter, c_int), diag, subdiag, w, z), 'ed_tridiag_eig')
-    e0_host = w(1)
-    if ((abs(w(1) - last) < 1d-12 .and. iter > 2) .or. iter == niter .or. (iter == 1 .and. normv_out < 1d-24)) exit
-    last = w(1)
-  end do
-  k = min(iter, niter)
+  call host_lanczos(n, niter, diag, subdiag, e0_host, k)
   write(*, '(a,f22.14,a,i0)') 'E0_host_loop ', e0_host, ' iters ', k
 
   ! ---- 2. coarse device path: Lanczos_GPU_updo + Lanczos_GPU_get_GS_updo
@@ -97,5 +77,80 @@ program ed_lanczos_demo
                                    nconv, nsteps), 'ed_block_lanczos')
     write(*, '(a,4f22.14,a,i0,a,i0)') 'E_block ', valp(1:4, 1), ' nconv ', nconv, ' steps ', nsteps
   end if
+
+  ! ---- 4. get_GS order: the handle is rebuilt before the ground-state vector is formed
   call delete_H()
+  call new_H(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, nup, ndn)
+  allocate(psi2(n))
+  call Lanczos_GPU_get_GS_updo(k, n, z(1:k), psi2)
+  write(*, '(a,es12.4)') 'GS_second_pass_overlap ', abs(sum(psi * psi2))
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, fout)
+    open(newunit=u_out, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+    write(u_out) psi2
+    close(u_out)
+  end if
+  call delete_H()
+
+  ! ---- 5. the S_z branch of new_H (no pairing): the Nambu sector npart = N_up + (Ns - N_dn)
+  npart = nup + (nc + nb - ndn)
+  nsz = 1
+  do k = 1, npart
+    nsz = nsz * (2 * (nc + nb) - k + 1) / k
+  end do
+  if (nsz <= 1000000) then
+    allocate(pb(max(1, nb * nb)), pvbc(max(1, 2 * nb * nc)))
+    pb = 0; pvbc = 0
+    call new_H_sz(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, pb, pvbc, 0, npart)
+    n = dimen_H()
+    niter = min(n - 1, nitermax)
+    deallocate(diag, subdiag)
+    allocate(diag(niter), subdiag(niter))
+    call host_lanczos(n, niter, diag, subdiag, e0_sz, k)
+    write(*, '(a,f22.14,a,i0,a,i0)') 'E0_sz_host_loop ', e0_sz + energy_shift_H(), ' iters ', k, ' dim ', n
+    call delete_H()
+  end if
+
+contains
+
+  ! Lanczos_fast_diagonalize with one_step_Lanczos_fast / lanc_opt (Lanczos_plain.f90:33-134,
+  ! Lanczos_plain.h:66-173) on host vectors, every H.v through Hmult__; stops on the
+  ! reference's test for the lowest eigenvalue at tolerance 1e-12.
+  subroutine host_lanczos(n, niter, diag, subdiag, e0, iters)
+    integer, intent(in)  :: n, niter
+    real(8), intent(out) :: diag(niter), subdiag(niter), e0
+    integer, intent(out) :: iters
+    real(8), allocatable :: vin(:), vout(:), tmp(:), w(:), z(:)
+    real(8) :: normv_in, normv_out, aa, ddiag, last
+    integer :: iter
+    allocate(vin(n), vout(n), tmp(n), w(niter), z(niter * niter))
+    call ed_check(ed_start_vector(int(n, c_int64_t), vin), 'ed_start_vector')
+    tmp = 0; diag = 0; subdiag = 0
+    normv_in = sqrt(sum(vin*vin)); normv_out = 0; subdiag(1) = normv_in
+    last = huge(1d0); e0 = 0
+    do iter = 1, niter
+      if (iter == 1) then
+        aa = 1d0 / normv_in
+      else
+        if (abs(normv_out) < 1d-25) stop '0 norm vector in lanczos, critical'
+        aa = 1d0 / normv_out
+      end if
+      vin = vin * aa
+      call Hmult__(n, vout, vin)
+      if (iter > 1) vout = vout - subdiag(iter) * tmp
+      ddiag = sum(vout * vin)
+      diag(iter) = ddiag
+      vout = vout - ddiag * vin
+      normv_out = sqrt(sum(vout * vout))
+      if (iter < niter) subdiag(iter + 1) = normv_out
+      tmp = vin
+      vin = vout
+      call ed_check(ed_tridiag_eig(int(iter, c_int), diag, subdiag, w, z), 'ed_tridiag_eig')
+      e0 = w(1)
+      if ((abs(w(1) - last) < 1d-12 .and. iter > 2) .or. iter == niter .or. (iter == 1 .and. normv_out < 1d-24)) exit
+      last = w(1)
+    end do
+    iters = min(iter, niter)
+  end subroutine
+
 end program ed_lanczos_demo

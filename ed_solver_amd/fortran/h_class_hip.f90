@@ -21,12 +21,14 @@ module H_class_hip
   type(c_ptr), save :: sector_h = c_null_ptr        ! H_class.f90:11
   real(c_double), allocatable, target, save :: m_ec(:), m_eb(:), m_vbc(:), m_u(:), m_jj(:)
   real(c_double), allocatable, target, save :: m_ec_im(:), m_eb_im(:), m_vbc_im(:)
+  real(c_double), allocatable, target, save :: m_pb(:), m_pvbc(:), m_u4(:)
 
   interface Hmult__                                   ! H_class.f90:13-15
     module procedure Hmultr, Hmultc
   end interface
 
-  public :: new_H, new_H_complex, delete_H, Hmult__, Hmultr, Hmultc, dimen_H, sector_handle
+  public :: new_H, new_H_complex, new_H_sz, new_H_slater, delete_H, Hmult__, Hmultr, Hmultc, dimen_H, &
+            sector_handle, energy_shift_H
   public :: Lanczos_GPU_updo, Lanczos_GPU_get_GS_updo, Lanczos_dynamic_GPU_updo
 
 contains
@@ -61,6 +63,51 @@ contains
     call ed_check(ed_sector_create_c(m, c_loc(m_ec_im), c_loc(m_eb_im), c_loc(m_vbc_im), int(nup, c_int), &
                                      int(ndn, c_int), sector_h), 'new_H_complex')
   end subroutine
+
+  ! The S_z branch of new_H (H_class.f90:28-37 -> new_HAIMsz, HAIMsz.f90:51-177): the Nambu
+  ! sector with npart = N_up + (Ns - N_dn) set bits, bath pairing pb(nb*nb) (Pb) and
+  ! impurity-bath pairing pvbc(2*nb*nc) (PVbc, used when pairing_imp_to_bath /= 0), both in
+  ! C row-major order.  Hmultr then applies HAIMsz_mult; the reference's reported energies
+  ! add energy_shift_H() to the eigenvalues of this H (eigen_sector_class.f90:329).
+  subroutine new_H_sz(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, pb, pvbc, pairing_imp_to_bath, npart)
+    integer, intent(in)        :: nc, nb, npart, pairing_imp_to_bath
+    real(8), intent(in)        :: ec(2*nc*nc), eb(max(1, 2*nb*nb)), vbc(max(1, 2*nb*nc)), u(nc*nc), jj(nc*nc)
+    real(8), intent(in)        :: pb(max(1, nb*nb)), pvbc(max(1, 2*nb*nc))
+    real(8), intent(in)        :: jhund, cutoff
+    type(ed_model) :: m
+    call delete_H()
+    m_ec = ec; m_eb = eb; m_vbc = vbc; m_u = u; m_jj = jj; m_pb = pb; m_pvbc = pvbc
+    m%nc = nc; m%nb = nb
+    m%ec = c_loc(m_ec); m%eb = c_loc(m_eb); m%vbc = c_loc(m_vbc); m%u = c_loc(m_u); m%jj = c_loc(m_jj)
+    m%jhund = jhund; m%cutoff = cutoff
+    call ed_check(ed_sector_create_sz(m, c_loc(m_pb), c_loc(m_pvbc), int(pairing_imp_to_bath, c_int), &
+                                      int(npart, c_int), sector_h), 'new_H_sz')
+  end subroutine
+
+  ! new_H with the Slater 4-index impurity interaction of flag_slater_int (HAIMupdo.f90:905-999):
+  ! u4(nc**4) = Slater_Coulomb_r(site1, site2, site3, site4) in C order (site4 fastest).
+  subroutine new_H_slater(nc, nb, ec, eb, vbc, u, jj, jhund, cutoff, u4, nup, ndn)
+    integer, intent(in)        :: nc, nb, nup, ndn
+    real(8), intent(in)        :: ec(2*nc*nc), eb(max(1, 2*nb*nb)), vbc(max(1, 2*nb*nc)), u(nc*nc), jj(nc*nc)
+    real(8), intent(in)        :: u4(nc*nc*nc*nc)
+    real(8), intent(in)        :: jhund, cutoff
+    type(ed_model) :: m
+    call delete_H()
+    m_ec = ec; m_eb = eb; m_vbc = vbc; m_u = u; m_jj = jj; m_u4 = u4
+    m%nc = nc; m%nb = nb
+    m%ec = c_loc(m_ec); m%eb = c_loc(m_eb); m%vbc = c_loc(m_vbc); m%u = c_loc(m_u); m%jj = c_loc(m_jj)
+    m%jhund = jhund; m%cutoff = cutoff
+    call ed_check(ed_sector_create_slater(m, c_loc(m_u4), int(nup, c_int), int(ndn, c_int), sector_h), &
+                  'new_H_slater')
+  end subroutine
+
+  ! The constant the S_z basis moves out of H (tr Ec_dn + tr Eb_dn); 0 for other sectors.
+  real(8) function energy_shift_H()
+    real(c_double) :: sh
+    if (.not. c_associated(sector_h)) stop 'energy_shift_H: no sector'
+    call ed_check(ed_sector_energy_shift(sector_h, sh), 'energy_shift_H')
+    energy_shift_H = sh
+  end function
 
   subroutine delete_H()
     integer(c_int) :: rc

@@ -225,7 +225,9 @@ class Sector:
         return bool(a.value), r2.value, g.value
 
     def ground_state(self, res: LanczosResult, k: int = 0) -> np.ndarray:
-        """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j (basis kept on the device)."""
+        """Lanczos_get_GS_sector: psi_k = normalised sum_j vecp[k, j] v_j over the basis kept on
+        the device, or, on a sector that keeps none (a fresh one, as after the reference's
+        second new_H), from the FP64 recurrence run again from the default start vector."""
         psi = np.zeros(self.vlen)
         row = np.ascontiguousarray(res.vecp[k], np.float64)
         check(lib().ed_lanczos_gs_vector(self.handle, res.iters, dptr(row), dptr(psi)))

@@ -311,8 +311,12 @@ int ed_lanczos_run(ed_sector *sec, int niter, const double *start, int precision
 
 /* Lanczos_GPU_get_GS_updo (Lanczos_gpu.f90:48) / Lanczos_get_GS_sector
  * (Lanczos_plain.f90:156-229): psi = normalised sum_j vecp[j] v_j over the Krylov
- * basis kept by the last Lanczos call on this sector.  psi is a host array of
- * length dimen_H() (FP64). */
+ * basis kept by the last Lanczos call on this sector.  A sector that keeps no basis of
+ * `iters` vectors (a fresh handle: the reference calls new_H again before this step,
+ * solver.h:43-49) runs the reference's second pass: the FP64 recurrence again from
+ * ed_start_vector (create_fix_initial_vector, Lanczos_plain.f90:196), so vecp must come
+ * from an FP64 ed_lanczos_diagonalize / ed_lanczos_run started from the default vector.
+ * psi is a host array of length dimen_H() (FP64). */
 int ed_lanczos_gs_vector(ed_sector *sec, int iters, const double *vecp, double *psi);
 
 /* Same, result left on the device (d_psi, FP64). */

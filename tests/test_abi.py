@@ -36,6 +36,13 @@ def test_python_binding_covers_header():
     assert sorted(SIGNATURES) == declared_symbols()
 
 
+def test_fortran_binding_covers_header():
+    """module ed_hip (ISO_C_BINDING) has one bind(C) interface per C-ABI entry point."""
+    src = open(os.path.join(ROOT, "ed_solver_amd", "fortran", "ed_hip_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(ed_[a-z0-9_]+)"\)', src))
+    assert sorted(bound) == declared_symbols()
+
+
 def test_version_and_defaults():
     import ed_solver_amd as ed
     from ed_solver_amd._lib import LanczosOpts

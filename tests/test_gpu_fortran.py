@@ -1,6 +1,10 @@
 """The Fortran host path: H_class_hip (ISO_C_BINDING over libed_hip.so) driven by a
 Lanczos loop written like Lanczos_fast_diagonalize + lanc_opt, and by the coarse
-Lanczos_GPU_updo / Lanczos_GPU_get_GS_updo entry points."""
+Lanczos_GPU_updo / Lanczos_GPU_get_GS_updo entry points, in the reference's own call
+order (solver.h:43-60: new_H, the Lanczos pass, delete_H, new_H again, then the
+ground-state vector from the second pass of Lanczos_get_GS_sector on the fresh handle,
+checked against the oracle's psi to 1e-9 in overlap), and the S_z branch of new_H
+(new_H_sz) under the same host Lanczos loop."""
 import json
 import os
 import subprocess
@@ -25,7 +29,7 @@ def write_model(path, m, nup, ndn, nitermax=128):
         f.write(f"{m.jhund!r} {m.cutoff!r}\n")
 
 
-@pytest.mark.parametrize("name", ["A", "B"])
+@pytest.mark.parametrize("name", ["A", "B", "C"])
 def test_fortran_demo_matches_oracle(tmp_path, name):
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} not built (amdflang build step of __graft_entry__.build())")
@@ -36,14 +40,16 @@ def test_fortran_demo_matches_oracle(tmp_path, name):
     path = tmp_path / "model.txt"
     write_model(path, m, nu, nd)
     print(f"running {EXE} on workload {name}", flush=True)
-    r = subprocess.run([EXE, str(path)], capture_output=True, text=True, timeout=120)
+    psi_path = tmp_path / "psi.bin"
+    r = subprocess.run([EXE, str(path), str(psi_path)], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = {}
     for line in r.stdout.splitlines():
         parts = line.split()
         if parts and parts[0].startswith("E0_"):
             vals[parts[0]] = float(parts[1])
-    ro = oracle.OracleSector(m, nu, nd, nthreads=8).lanczos(nitermax=128)
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    ro = o.lanczos(nitermax=128)
     e0 = ro["evals"][0]
     assert abs(vals["E0_host_loop"] - e0) < 1e-10
     assert abs(vals["E0_gpu_updo"] - e0) < 1e-10
@@ -51,5 +57,33 @@ def test_fortran_demo_matches_oracle(tmp_path, name):
     # Block_Lanczos_diagonalize through ed_block_lanczos (4 lowest states)
     line = [l for l in r.stdout.splitlines() if l.startswith("E_block")][0].split()
     eb = np.array([float(x) for x in line[1:5]])
-    ref, _ = oracle.lowest_eigs(oracle.OracleSector(m, nu, nd, nthreads=8), 4)
+    ref, _ = oracle.lowest_eigs(o, 4)
     assert np.max(np.abs(eb - ref)) < 1e-10
+    # get_GS order: psi from the second pass on the rebuilt handle equals the kept-basis psi
+    # and the oracle's psi (Lanczos_get_GS_sector restated)
+    line = [l for l in r.stdout.splitlines() if l.startswith("GS_second_pass_overlap")][0].split()
+    assert abs(float(line[1]) - 1.0) < 1e-9
+    psi2 = np.fromfile(psi_path, dtype=np.float64)
+    assert psi2.shape == (o.dim,)
+    psi_o = o.ground_state(ro["iters"], ro["vecp"][0])
+    ov = abs(psi2 @ psi_o) / (np.linalg.norm(psi2) * np.linalg.norm(psi_o))
+    print(f"  second-pass psi overlap with the oracle: 1 - {1 - ov:.2e}", flush=True)
+    assert abs(ov - 1.0) < 1e-9
+    o.close()
+    # the S_z branch (no pairing): its lowest state is the lowest over the (N_up, N_dn)
+    # sectors with N_up - N_dn = nup - nd (checked where the Nambu sector is small)
+    sz = [l for l in r.stdout.splitlines() if l.startswith("E0_sz_host_loop")]
+    if name in ("A", "B"):
+        assert sz, r.stdout
+        e_sz = float(sz[0].split()[1])
+        best = np.inf
+        for n_up in range(m.ns + 1):
+            n_dn = n_up - (nu - nd)
+            if not 0 <= n_dn <= m.ns:
+                continue
+            os_ = oracle.OracleSector(m, n_up, n_dn, nthreads=16)
+            e = oracle.lowest_eigs(os_, 1)[0][0] if os_.dim <= 3000 else os_.lanczos(nitermax=128)["evals"][0]
+            best = min(best, float(e))
+            os_.close()
+        print(f"  S_z sector E0 {e_sz:.12f} vs min over (N, N) sectors {best:.12f}", flush=True)
+        assert abs(e_sz - best) < 1e-10

@@ -375,3 +375,32 @@ def test_workload_e_hmult(gpu):
     v = np.random.default_rng(9).uniform(-1, 1, s.dim)
     assert rel(s.hmult(v), o.hmult(v)) < 1e-13
     s.close()
+
+
+@pytest.mark.parametrize("name", ["B", "C"])
+def test_gs_vector_in_the_reference_call_order(gpu, name):
+    """get_GS (solver.h:43-60): new_H, Lanczos, delete_H, new_H again, Lanczos_get_GS_sector.
+    On the fresh sector the vector comes from the second pass (the recurrence again from
+    the default start vector, Lanczos_plain.f90:156-229): psi overlaps the oracle's psi to
+    1e-9 and equals the kept-basis psi of the first sector to 1e-12."""
+    from ed_solver_amd.hclass import Lanczos_fast_diagonalize, Lanczos_get_GS_sector, delete_H, new_H
+    m, (nu, nd) = golden(name)
+    s1 = new_H(m, nu, nd)
+    r = Lanczos_fast_diagonalize(neigen=2, nitermax=128, tolerance=1e-12)
+    psi_kept = s1.ground_state(r, 0)
+    delete_H()
+    new_H(m, nu, nd)
+    psi = Lanczos_get_GS_sector(r, 0)
+    psi1 = Lanczos_get_GS_sector(r, 1)
+    delete_H()
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    ro = o.lanczos(nitermax=128, neigen=2, tol=1e-12)
+    assert r.iters == ro["iters"]
+    psi_o = o.ground_state(ro["iters"], ro["vecp"][0])
+    psi_o /= np.linalg.norm(psi_o)
+    o.close()
+    print(f"{name}: second pass vs oracle overlap 1 - {1 - abs(psi @ psi_o):.2e}, vs kept basis "
+          f"{np.max(np.abs(psi - psi_kept)):.2e}", flush=True)
+    assert abs(abs(psi @ psi_o) - 1) < 1e-9
+    assert np.max(np.abs(psi - psi_kept)) < 1e-12
+    assert abs(psi @ psi1) < 1e-8   # the second Ritz vector of the same pass

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -810,13 +811,22 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
     int spins = kBarrierSpins;
     void *args[] = {&a, &lz, &nsteps, &spins, &mon};
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, st, lz.st);
-    const hipError_t e = hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk), dim3(kBlock), args,
-                                                    (unsigned)p.lds, st);
+    // persist == 2 (a test hook) takes the refusal path without launching
+    const hipError_t e = s->persist == 2 ? hipErrorCooperativeLaunchTooLarge
+                                         : hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk),
+                                                                      dim3(kBlock), args, (unsigned)p.lds, st);
     if (e != hipSuccess) {
-        // the runtime refused the cooperative launch (nothing ran): this sector leaves the
-        // persistent kernel for this precision and the caller runs the two-kernel step
         (void)hipGetLastError();
+        // A refusal of the cooperative launch itself (nothing ran): the grid does not fit at
+        // once (hipErrorCooperativeLaunchTooLarge), or the runtime reported it cannot launch
+        // one in its present state (hipErrorIllegalState, seen once in a fixed-length run
+        // at dim 63,504 between runs that launched).  The sector then leaves the persistent
+        // kernel for this precision, the refusal is counted (ed_lanczos_persistent_refusals)
+        // and the caller runs the two-kernel step.  Every other error is an error.
+        if (e != hipErrorCooperativeLaunchTooLarge && e != hipErrorIllegalState)
+            return fail(ED_ERR_HIP, std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(e));
         s->persist_ok[sizeof(T) == sizeof(double) ? 0 : 1] = 0;
+        ++s->persist_refused;
         return kPersistRefused;
     }
     hipLaunchKernelGGL(k_lz_finish<T>, dim3(1), dim3(kBlock), 0, st, lz);
@@ -850,6 +860,18 @@ static void run_steps(ed_sector *s, int prec, int nsteps, int reorth, hipStream_
 
 static int tune(ed_sector *s);
 
+// Chunks of two-kernel steps replay captured graphs unless the environment sets
+// ED_LANCZOS_GRAPHS=0: then every chunk is launched kernel by kernel (the same kernels, so
+// the same results), for tools that trace kernel launches and do not follow graph capture.
+static bool graphs_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("ED_LANCZOS_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // Runs nsteps Lanczos steps; chunks of exactly `chunk` steps replay a captured graph.
 static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
 {
@@ -862,7 +884,7 @@ static int steps(ed_sector *s, int prec, int nsteps, int chunk, int reorth)
             return ED_OK;
         }
     }
-    if (nsteps == chunk && chunk > 1) {
+    if (nsteps == chunk && chunk > 1 && graphs_enabled()) {
         const int key = prec * 2 + (reorth ? 1 : 0);
         GraphCache &G = s->graphs[key];
         if (!(G.exec && G.chunk == chunk && G.nslots == s->basis_nslots && G.base == s->d_basis)) {
@@ -1066,6 +1088,10 @@ static int tune(ed_sector *s)
 struct PhaseOut {
     int iters = 0;
     std::vector<double> alpha, beta;
+    // Rayleigh-form FP32 phase: the target was not met (gave up, or the run ended first);
+    // second Ritz value and its residual estimate at the last target test
+    bool rq_gave_up = true;
+    double rq_theta2 = 0.0, rq_rho2 = 0.0;
 };
 
 // The convergence test of Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134), one
@@ -1075,12 +1101,48 @@ struct PhaseOut {
 struct ConvTest {
     int neig, niter;
     double tol;
-    // > 0 (the FP32 phase of the Rayleigh form): also wait until the Lanczos estimate of the
-    // lowest Ritz pair's residual, r = beta_{n+1} |z_n| of T_n (n = it - 1), satisfies
-    // r^2 <= rq_tol * (theta_2 - theta_1) / 2, the bound the FP64 check then applies
-    double rq_tol = 0.0;
+    // > 0 (the FP32 phase of the Rayleigh form): once the eigenvalue test is met, also wait
+    // until the Lanczos estimate of the lowest Ritz pair's residual, rho_1 = beta_{n+1} |z_n1|
+    // of T_n (n = it - 1), meets the Kato-Temple target rho_1^2 <= rq_tol (l_2 - theta_1) / 2
+    // the FP64 check applies, with l_2 = theta_2 - beta_{n+1} |z_n2| the lower estimate of
+    // lambda_2 from the second Ritz pair.  The wait ends without the target (rq_gave_up)
+    // when the target is below what vectors of unit roundoff rq_eps can reach
+    // ((4 rq_eps ||T||)^2 > rq_tol gap / 2), when theta_2 collapses onto theta_1 (a Lanczos
+    // copy of theta_1, or a splitting the phase cannot resolve), or after
+    // max(16, n_conv) further steps (n_conv: the step that met the eigenvalue test).
+    double rq_tol = 0.0, rq_eps = 0.0;
+    bool rq_gave_up = false;
+    int rq_conv_at = 0;             // iteration at which the eigenvalue test was first met
+    double rq_theta2 = 0.0, rq_rho2 = 0.0;   // second Ritz pair of the last target test
     std::vector<double> last, w;
     ConvTest(int neig_, int niter_, double tol_) : neig(neig_), niter(niter_), tol(tol_), last(neig_, 1e300), w(neig_) {}
+    // false: keep iterating for the Rayleigh-form target
+    bool rq_target(int it, const double *alpha, const double *beta)
+    {
+        const int n = it - 1;
+        if (!rq_conv_at) rq_conv_at = it;
+        if (it - rq_conv_at > std::max(16, rq_conv_at)) {
+            rq_gave_up = true;
+            return true;
+        }
+        double ev[2];
+        std::vector<double> z(2 * (size_t)std::max(n, 2));
+        if (n < 2 || tridiag_lowest_pairs(n, alpha, beta, 2, ev, z.data()) != ED_OK) return false;
+        const double rho1 = std::fabs(beta[n] * z[n - 1]), rho2 = std::fabs(beta[n] * z[2 * (size_t)n - 1]);
+        double tn = 0.0;   // Gershgorin bound of ||T_n||
+        for (int i = 0; i < n; ++i)
+            tn = std::max(tn, std::fabs(alpha[i]) + (i > 0 ? std::fabs(beta[i]) : 0.0) +
+                                  (i + 1 < n ? std::fabs(beta[i + 1]) : 0.0));
+        rq_theta2 = ev[1];
+        rq_rho2 = rho2;
+        const double gap = (ev[1] - rho2) - ev[0];
+        const double fl = 4.0 * rq_eps * tn;
+        if (!(ev[1] - ev[0] > 1e-7 * std::max(1.0, std::fabs(ev[0]))) || (gap > 0.0 && fl * fl > rq_tol * 0.5 * gap)) {
+            rq_gave_up = true;
+            return true;
+        }
+        return gap > 0.0 && rho1 * rho1 <= rq_tol * 0.5 * gap;
+    }
     bool ends(int it, const double *alpha, const double *beta)
     {
         tridiag_lowest_warm(it, alpha, beta, neig, last.data(), w.data());
@@ -1088,16 +1150,7 @@ struct ConvTest {
         bool allc = true;
         for (int k = 0; k < neig; ++k)
             if (!((k < m) && std::fabs(w[k] - last[k]) < tol && it > 2)) allc = false;
-        if (allc && rq_tol > 0.0) {
-            const int n = it - 1;
-            std::vector<double> ev(2), z(2 * (size_t)std::max(n, 2));
-            if (n >= 2 && tridiag_lowest_pairs(n, alpha, beta, 2, ev.data(), z.data()) == ED_OK) {
-                const double r = beta[n] * z[n - 1], gap = ev[1] - ev[0];
-                if (!(gap > 0.0) || r * r > rq_tol * 0.5 * gap) allc = false;
-            } else {
-                allc = false;
-            }
-        }
+        if (allc && rq_tol > 0.0 && !rq_gave_up) allc = rq_target(it, alpha, beta);
         const bool one_step = (it == 1) && std::fabs(beta[1]) < 1e-24;
         if (allc || it == niter || one_step) return true;
         for (int k = 0; k < m; ++k) last[k] = w[k];
@@ -1177,12 +1230,14 @@ static int run_monitored(ed_sector *s, const PersistShape &p, int done, int nste
 static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig, int chunk, int reorth,
                          const double *h_start, const double *d_start, PhaseOut &out, double rq_tol = 0.0)
 {
+    out.rq_gave_up = rq_tol > 0.0;
     ED_TRY(ensure_state(s));
     ED_TRY(ensure_basis(s, prec, niter + 1));
     s->basis_valid_iters = 0;
     ED_TRY(lz_start(s, prec, h_start, d_start));
     ConvTest ct(neig, niter, tol);
     ct.rq_tol = rq_tol;
+    ct.rq_eps = prec == ED_PREC_F32 ? 5.9604644775390625e-8 : 1.1102230246251565e-16;   // 2^-24, 2^-53
     std::vector<double> alpha(niter + 1), beta(niter + 2);
     HostLz h;
     int done = 0, final_iter = 0, tested = 0;
@@ -1223,6 +1278,9 @@ static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig
         done += c;
     }
     out.iters = final_iter;
+    out.rq_gave_up = ct.rq_gave_up || !ct.rq_conv_at;
+    out.rq_theta2 = ct.rq_theta2;
+    out.rq_rho2 = ct.rq_rho2;
     out.alpha.assign(h.alpha.begin(), h.alpha.begin() + final_iter);
     out.beta.assign(h.beta.begin(), h.beta.begin() + final_iter);
     s->basis_valid_iters = final_iter;
@@ -1232,51 +1290,55 @@ static int lanczos_phase(ed_sector *s, int prec, double tol, int niter, int neig
 
 static int gs_vector_dev(ed_sector *s, int iters, const double *vecp, double *d_psi);
 
-// Per-workgroup partial sums of <x, x>, <x, y>, <y, y> (grid-stride, fixed order).
-__global__ __launch_bounds__(kBlock) void k_dot3_part(const double *x, const double *y, int64_t n, double *part)
+// Per-workgroup partial sums of <x, x>, <x, y> and ||y - c x||^2 (grid-stride, fixed order).
+__global__ __launch_bounds__(kBlock) void k_dot3_part(const double *x, const double *y, int64_t n, double c,
+                                                      double *part)
 {
     __shared__ double red[kRed];
-    double a = 0.0, b = 0.0, c = 0.0;
+    double a = 0.0, b = 0.0, r = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double xi = x[i], yi = y[i];
+        const double xi = x[i], yi = y[i], di = yi - c * xi;
         a += xi * xi;
         b += xi * yi;
-        c += yi * yi;
+        r += di * di;
     }
     a = block_sum_all(a, red);
     b = block_sum_all(b, red);
-    c = block_sum_all(c, red);
+    r = block_sum_all(r, red);
     if (threadIdx.x == 0) {
         part[3 * blockIdx.x] = a;
         part[3 * blockIdx.x + 1] = b;
-        part[3 * blockIdx.x + 2] = c;
+        part[3 * blockIdx.x + 2] = r;
     }
 }
 
 // FP64 Rayleigh quotient E = <psi, H psi> / <psi, psi> of the device vector d_psi and the
 // squared residual ||H psi - E psi||^2 / <psi, psi> (one H.v into d_w, one pass of sums).
-static int rayleigh_residual(ed_sector *s, const double *d_psi, double *d_w, double &e, double &res2)
+// The residual is summed about the shift c (an estimate of E) and corrected by
+// ||Hpsi - E psi||^2 = ||H psi - c psi||^2 - (E - c)^2 ||psi||^2, which cancels only
+// (E - c)^2 instead of E^2.
+static int rayleigh_residual(ed_sector *s, const double *d_psi, double *d_w, double c, double &e, double &res2)
 {
     hipStream_t st = s->stream;
     ED_TRY(launch_hv(s, ED_PREC_F64, d_psi, d_w, 0, 1, st));
     constexpr int kParts = 256;
     double *d_part = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_part, sizeof(double) * 3 * kParts, st));
-    hipLaunchKernelGGL(k_dot3_part, dim3(kParts), dim3(kBlock), 0, st, d_psi, (const double *)d_w, s->dim, d_part);
+    hipLaunchKernelGGL(k_dot3_part, dim3(kParts), dim3(kBlock), 0, st, d_psi, (const double *)d_w, s->dim, c, d_part);
     ED_HIP_CHECK(hipGetLastError());
     std::vector<double> part(3 * kParts);
     ED_HIP_CHECK(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
     ED_HIP_CHECK(hipStreamSynchronize(st));
     (void)hipFreeAsync(d_part, st);
-    double xx = 0.0, xy = 0.0, yy = 0.0;
+    double xx = 0.0, xy = 0.0, rr = 0.0;
     for (int q = 0; q < kParts; ++q) {
         xx += part[3 * q];
         xy += part[3 * q + 1];
-        yy += part[3 * q + 2];
+        rr += part[3 * q + 2];
     }
     if (!(xx > 0.0)) return fail(ED_ERR_BREAKDOWN, "Rayleigh quotient of a zero vector");
     e = xy / xx;
-    res2 = std::max(0.0, yy / xx - e * e);
+    res2 = std::max(0.0, rr / xx - (e - c) * (e - c));
     return ED_OK;
 }
 
@@ -1318,13 +1380,15 @@ static int lanczos_diagonalize(ed_sector *s, const ed_lanczos_opts *o, const dou
         ED_TRY(ensure_io(s));
         ED_TRY(gs_vector_dev(s, p32.iters, z.data(), s->d_io_b));
         s->last_f32_iters = p32.iters;
-        if (rq && neig == 1 && k32 == 2) {
-            // E0 from the FP64 Rayleigh quotient of the FP32 Ritz vector: its error is at most
-            // ||r||^2 / (lambda_2 - E) (Kato-Temple).  lambda_2 is estimated by the second Ritz
-            // value, an upper bound of it, so the test takes half the estimated gap.
+        if (rq && neig == 1 && k32 == 2 && !p32.rq_gave_up) {
+            // E0 from the FP64 Rayleigh quotient E of the FP32 Ritz vector, with the
+            // Kato-Temple bound E - lambda_1 <= ||r||^2 / (l_2 - E) for a lower bound l_2 of
+            // lambda_2.  l_2 = theta_2 - rho_2 (the second Ritz value less its residual
+            // estimate) stands in for it: Lanczos gives no guaranteed lower bound (a missed
+            // eigenvalue), so the test keeps a factor 2 of margin on the gap.
             double e = 0.0, r2 = 0.0;
-            ED_TRY(rayleigh_residual(s, s->d_io_b, s->d_io_a, e, r2));
-            const double gap = ev[1] - e;
+            ED_TRY(rayleigh_residual(s, s->d_io_b, s->d_io_a, ev[0], e, r2));
+            const double gap = (p32.rq_theta2 - p32.rq_rho2) - e;
             s->last_rq_res2 = r2;
             s->last_rq_gap = gap;
             if (gap > 0.0 && r2 <= o->tolerance * 0.5 * gap) {
@@ -1594,7 +1658,7 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
 int ed_lanczos_set_persistent(ed_sector *sec, int enable)
 {
     ED_TRY(check_common(sec, 0));
-    if (enable != 0 && enable != 1) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0 or 1");
+    if (enable < 0 || enable > 2) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0, 1 or 2");
     sec->persist = enable;
     sec->persist_ok[0] = sec->persist_ok[1] = -1;
     return ED_OK;
@@ -1607,6 +1671,14 @@ int ed_lanczos_persistent_active(ed_sector *sec, int precision, int *active)
     if (!sec->cpb) ED_TRY(tune(sec));
     PersistShape p;
     *active = precision == ED_PREC_F64 ? persist_shape<double>(sec, p) : persist_shape<float>(sec, p);
+    return ED_OK;
+}
+
+int ed_lanczos_persistent_refusals(ed_sector *sec, int *refused)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!refused) return fail(ED_ERR_ARG, "ed_lanczos_persistent_refusals: NULL output");
+    *refused = sec->persist_refused;
     return ED_OK;
 }
 

@@ -253,7 +253,9 @@ struct ed_sector {
     double *d_sz_mat = nullptr;
     int sz_fly = 0;
     double energy_shift = 0.0;
-    int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0)
+    int persist = 1;               // use the persistent Lanczos kernel where it applies (1) or not (0);
+                                   // 2: as 1, but every cooperative launch takes the refusal path (test hook)
+    int persist_refused = 0;       // cooperative launches the runtime refused on this sector
     double *h_mon = nullptr;       // monitored persistent run: pinned fine-grained host channel
     double *d_mon = nullptr;       //   (int stop flag at [0], published scalars from [16]), device view
     int last_steps_run = 0;        // Lanczos steps the device ran in the last diagonalisation

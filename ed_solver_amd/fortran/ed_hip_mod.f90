@@ -502,6 +502,12 @@ module ed_hip
       real(c_double), intent(out) :: bytes_per_launch
     end function
 
+    integer(c_int) function ed_lanczos_persistent_refusals(sec, refused) &
+        bind(C, name="ed_lanczos_persistent_refusals")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sec
+      integer(c_int), intent(out) :: refused
+    end function
     ! ed_dynamic without device allocations (static scratch; limits ED_NOALLOC_MAX_* of ed_hip.h)
     integer(c_int) function ed_dynamic_noalloc(sec, iph, energy, phi, nitergreen, cutoff_dynamic, precision, nw, wn, &
         dyn, a, b) bind(C, name="ed_dynamic_noalloc")

@@ -184,6 +184,12 @@ class Sector:
         """Use the persistent Lanczos kernel where it applies (True, the default) or never."""
         check(lib().ed_lanczos_set_persistent(self.handle, 1 if enable else 0))
 
+    def persistent_refusals(self) -> int:
+        """Cooperative launches of the persistent kernel the runtime refused on this sector."""
+        v = ctypes.c_int()
+        check(lib().ed_lanczos_persistent_refusals(self.handle, ctypes.byref(v)))
+        return v.value
+
     def persistent_active(self, precision="f64") -> bool:
         """Whether the next Lanczos run in `precision` uses the persistent kernel."""
         v = ctypes.c_int()

@@ -233,10 +233,18 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
  * launch: H.v and update separated by grid barriers, the static row data in registers
  * and LDS, the vectors exchanged with device-scope coherent loads and stores.  The
  * arithmetic and the order of every sum are those of the two-kernel step, so alpha and
- * beta are the same numbers.  enable = 1 (default): use it where it applies; 0: never.
- * A barrier wait that exceeds a few seconds stops the run with ED_ERR_HIP and turns
- * the kernel off for the sector. */
+ * beta are the same numbers.  enable = 1 (default): use it where it applies; 0: never;
+ * 2 (a test hook): as 1, but every cooperative launch takes the path of a launch the
+ * runtime refuses.  A barrier wait that exceeds a few seconds stops the run with
+ * ED_ERR_HIP and turns the kernel off for the sector.  A cooperative launch the runtime
+ * refuses (hipErrorCooperativeLaunchTooLarge, hipErrorIllegalState: nothing ran) turns
+ * it off for that sector and precision, is counted, and the run continues with the
+ * two-kernel step; any other launch error returns ED_ERR_HIP. */
 int ed_lanczos_set_persistent(ed_sector *sec, int enable);
+
+/* Cooperative launches of the persistent kernel the runtime refused on this sector (no
+ * reference counterpart; lets a caller tell a persistent run from its fallback). */
+int ed_lanczos_persistent_refusals(ed_sector *sec, int *refused);
 
 /* *active = 1 when a fixed-length Lanczos run of at least 16 steps in `precision` on
  * this sector uses the persistent kernel (tunes the sector first if needed), else 0. */
@@ -280,13 +288,18 @@ int ed_lanczos_diagonalize(ed_sector *sec, const ed_lanczos_opts *opts, const do
 int ed_lanczos_info(ed_sector *sec, int *iters_f32_phase);
 
 /* The last ED_PREC_MIXED_RQ solve on this sector (no reference counterpart): with one
- * eigenvalue asked for, the FP32 phase's Ritz vector psi gets one FP64 H.v; its Rayleigh
- * quotient E replaces the FP64 restart when res2 = ||H psi - E psi||^2 / ||psi||^2 <=
- * tolerance * gap / 2, gap = (second Ritz value of the FP32 T) - E (the Kato-Temple bound
- * E - E0 <= res2 / (lambda_2 - E), with the Ritz value, an upper bound of lambda_2, as
- * the estimate).  *accepted = 1 then, and the outputs of ed_lanczos_diagonalize are those
- * of the FP32 phase with evals[0] = E (ed_lanczos_gs_vector gives psi from the FP32
- * basis); otherwise the call continued as ED_PREC_F32. */
+ * eigenvalue asked for, the FP32 phase runs until the Lanczos estimate of its Ritz
+ * residual meets the target below, then its Ritz vector psi gets one FP64 H.v; the
+ * Rayleigh quotient E replaces the FP64 restart when res2 = ||H psi - E psi||^2 /
+ * ||psi||^2 <= tolerance * gap / 2, gap = l_2 - E with l_2 = theta_2 - rho_2 (the second
+ * Ritz value of the FP32 T less its residual estimate), the lower estimate of lambda_2 in
+ * the Kato-Temple bound E - E0 <= res2 / (l_2 - E).  The FP32 phase stops waiting for
+ * the target, and the FP64 restart runs, when the target lies below what FP32 vectors
+ * reach ((4 eps32 ||T||)^2 > tolerance * gap / 2), when theta_2 collapses onto theta_1, or
+ * after max(16, n) steps beyond its eigenvalue convergence at step n.  *accepted = 1
+ * when E was taken, and the outputs of ed_lanczos_diagonalize are those of the FP32 phase
+ * with evals[0] = E (ed_lanczos_gs_vector gives psi from the FP32 basis); otherwise the
+ * call continued as ED_PREC_F32 (res2 and gap are 0 when the test did not run). */
 int ed_lanczos_rq_info(ed_sector *sec, int *accepted, double *res2, double *gap);
 
 /* Lanczos steps the device ran in the last ed_lanczos_diagonalize on this sector (both

@@ -155,3 +155,26 @@ def test_monitored_vs_oracle_and_short_limit(gpu):
     assert r20.iters == 20 and r20.steps_run == 20
     assert np.max(np.abs(r20.diag - ro["alpha"][:20])) < 1e-10
     s.close()
+
+
+def test_refused_cooperative_launch_takes_the_two_kernel_step(gpu):
+    """The refusal path of the persistent kernel (forced by the test hook enable = 2, which
+    takes it without launching): the run continues with the two-kernel step and its
+    alpha / beta are those of that step; the refusal is counted and the kernel is off for
+    the sector afterwards.  The monitored ground-state solve takes the same path."""
+    from ed_solver_amd._lib import check, lib
+    m, (nu, nd) = golden("C")
+    s = gpu.Sector(m, nu, nd)
+    s.set_schedule(20)
+    a0, b0 = run(s, 40, False)
+    r0 = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+    check(lib().ed_lanczos_set_persistent(s.handle, 2))
+    assert s.persistent_active("f64")
+    a1, b1 = s.lanczos_run(40)
+    assert s.persistent_refusals() == 1 and not s.persistent_active("f64")
+    assert np.array_equal(a1, a0) and np.array_equal(b1, b0)
+    check(lib().ed_lanczos_set_persistent(s.handle, 2))   # resets the applicability check
+    r1 = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+    assert s.persistent_refusals() == 2
+    assert r1.iters == r0.iters and np.array_equal(r1.evals, r0.evals)
+    s.close()

@@ -217,6 +217,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_lz_persist(HvArgs<T> a, LzArgs<T>
                                   grp, pub, &flag);
 }
 
+// Persistent Lanczos with pm.cpw columns per workgroup of 256 pm.cpw threads (see
+// lz_persist_cols_body); one workgroup per CU at cpw = 4.
+template <typename T, int K, int R, bool MON>
+__global__ __launch_bounds__(1024, 1) void k_lz_persist_cols(HvArgs<T> a, LzArgs<T> lz, PersistMap pm, int nsteps,
+                                                             int spin_limit, LzMonitor mon)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double red[kRed];
+    __shared__ double grp[kSums];
+    __shared__ double pub[2];
+    __shared__ int flag;
+    lz_persist_cols_body<T, K, R, MON>(a, lz, pm, nsteps, spin_limit, mon, reinterpret_cast<unsigned char *>(smem_d),
+                                       red, grp, pub, &flag);
+}
+
 __global__ void k_bar_reset(LzState *st)
 {
     for (int g = threadIdx.x; g < kBarGroups; g += blockDim.x) {
@@ -759,15 +774,121 @@ static const void *persist_kernel(int k, int rows, bool mon)
     }
 }
 
+template <typename T, int K, int R>
+static const void *persist_cols_fn(bool mon)
+{
+    return mon ? reinterpret_cast<const void *>(&k_lz_persist_cols<T, K, R, true>)
+               : reinterpret_cast<const void *>(&k_lz_persist_cols<T, K, R, false>);
+}
+
+// The multi-column instances: table widths 4, 8 and 12, 1, 2 or 4 rows per thread.
+template <typename T>
+static const void *persist_cols_kernel(int k, int rows, bool mon)
+{
+    switch (k * 8 + rows) {
+    case 4 * 8 + 1: return persist_cols_fn<T, 4, 1>(mon);
+    case 4 * 8 + 2: return persist_cols_fn<T, 4, 2>(mon);
+    case 4 * 8 + 4: return persist_cols_fn<T, 4, 4>(mon);
+    case 8 * 8 + 1: return persist_cols_fn<T, 8, 1>(mon);
+    case 8 * 8 + 2: return persist_cols_fn<T, 8, 2>(mon);
+    case 8 * 8 + 4: return persist_cols_fn<T, 8, 4>(mon);
+    case 12 * 8 + 1: return persist_cols_fn<T, 12, 1>(mon);
+    case 12 * 8 + 2: return persist_cols_fn<T, 12, 2>(mon);
+    case 12 * 8 + 4: return persist_cols_fn<T, 12, 4>(mon);
+    default: return nullptr;
+    }
+}
+
 struct PersistShape {
     const void *fn = nullptr, *fn_mon = nullptr;   // plain and monitored instance
     int nblk = 0, rows = 0;
     size_t lds = 0;
+    // the multi-column kernel: cpw columns per workgroup of 256 cpw threads (0: the one-column kernel)
+    int cpw = 0, nthreads = kBlock, nA = 0;
+    PersistMap map{};
 };
+
+// The map of the multi-column kernel for `cpw` columns per workgroup over the column
+// order of the sector's H.v schedule (XCD-grouped: the breadth-first order of the down-hop
+// graph cut into 8 chunks, workgroup b taking cpw consecutive columns of chunk b mod 8, so
+// they sit on one XCD; natural: the same over 0 .. d_dn - 1), with each column's partial
+// slot in the two-kernel step's layout.
+static int ensure_pmap(ed_sector *s, int cpw, bool xcd)
+{
+    if (s->d_pmap && s->pmap_cpw == cpw && s->pmap_xcd == (int)xcd) return ED_OK;
+    const int d = s->dn.d;
+    std::vector<int32_t> order(d), pslot(d), idle;
+    if (xcd) {
+        if (!s->d_colord || (int)s->col_bfs.size() != d) return fail(ED_ERR_STATE, "persistent map: no column order");
+        order = s->col_bfs;
+        std::vector<int32_t> cmap(s->colord_blocks);
+        ED_HIP_CHECK(hipMemcpy(cmap.data(), s->d_colord, sizeof(int32_t) * cmap.size(), hipMemcpyDeviceToHost));
+        for (int b = 0; b < (int)cmap.size(); ++b) {
+            if (cmap[b] >= 0) pslot[cmap[b]] = b;
+            else idle.push_back(b);
+        }
+    } else {
+        for (int c = 0; c < d; ++c) order[c] = pslot[c] = c;
+    }
+    const int q = (d + 7) / 8, per = (q + cpw - 1) / cpw, nwg = 8 * per;
+    std::vector<int32_t> map((size_t)nwg * cpw + d + idle.size() + 1);
+    for (int b = 0; b < nwg; ++b)
+        for (int c = 0; c < cpw; ++c) {
+            const int pos = (b / 8) * cpw + c, at = (b % 8) * q + pos;
+            map[(size_t)b * cpw + c] = (pos < q && at < d) ? order[at] : -1;
+        }
+    std::copy(pslot.begin(), pslot.end(), map.begin() + (size_t)nwg * cpw);
+    std::copy(idle.begin(), idle.end(), map.begin() + (size_t)nwg * cpw + d);
+    if (s->d_pmap) (void)hipFree(s->d_pmap);
+    s->d_pmap = nullptr;
+    ED_HIP_CHECK(hipMalloc((void **)&s->d_pmap, sizeof(int32_t) * map.size()));
+    ED_HIP_CHECK(hipMemcpy(s->d_pmap, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
+    s->pmap_cpw = cpw;
+    s->pmap_xcd = xcd ? 1 : 0;
+    s->pmap_nwg = nwg;
+    s->pmap_nidle = (int)idle.size();
+    return ED_OK;
+}
 
 // The persistent kernel applies when the sector's schedule is a fast-kernel one (one
 // column per workgroup, so its partial sums have the fast kernel's layout), d_up fits 4
 // rows per thread of 256, and every workgroup of the grid can be resident at once.
+// The multi-column kernel applies to fast-kernel schedules with tables of width <= 12, at
+// most kPersistHund Hund transitions per row, when its LDS fits and the grid is resident.
+template <typename T>
+static bool persist_cols_shape(ed_sector *s, const HvShape &h, PersistShape &p, int ncu)
+{
+    if (!s->persist_cols || h.k > 12 || (s->hund && s->hund_maxrow > kPersistHund)) return false;
+    const int rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
+    const void *fn = persist_cols_kernel<T>(h.k, rows, false), *fn_mon = persist_cols_kernel<T>(h.k, rows, true);
+    if (!fn || !fn_mon) return false;
+    int cpw = std::min(4, std::max(1, (s->dn.d + ncu - 1) / ncu));
+    for (; cpw >= 1; cpw /= 2) {
+        const size_t lds =
+            persist_cols_lds(s->up.d, h.k, s->up.ncoef, s->ncfg, s->nhund, s->hund, cpw, sizeof(T)).total;
+        if (lds > 160 * 1024) continue;
+        if (ensure_pmap(s, cpw, h.xcd != 0) != ED_OK) return false;
+        int per_cu = 0, per_cu_mon = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock * cpw, lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_mon, fn_mon, kBlock * cpw, lds) != hipSuccess)
+            return false;
+        const int nwg = s->pmap_nwg;
+        if ((int64_t)std::min(per_cu, per_cu_mon) * ncu < nwg || nwg > kBarGroups * kBarGroupSize) continue;
+        p.fn = fn;
+        p.fn_mon = fn_mon;
+        p.rows = rows;
+        p.cpw = cpw;
+        p.nthreads = kBlock * cpw;
+        p.nblk = nwg;
+        p.lds = lds;
+        p.nA = h.nblk;
+        const int32_t *m = s->d_pmap;
+        p.map = PersistMap{m, m + (size_t)nwg * cpw, m + (size_t)nwg * cpw + s->dn.d, s->pmap_nidle, cpw};
+        return true;
+    }
+    return false;
+}
+
 template <typename T>
 static bool persist_shape(ed_sector *s, PersistShape &p)
 {
@@ -775,10 +896,24 @@ static bool persist_shape(ed_sector *s, PersistShape &p)
     if (!s->persist || s->persist_ok[pi] == 0) return false;
     const HvShape h = hv_shape(s, sched_of<T>(s));
     if (h.kind != 2 || s->up.d > 4 * kBlock) return false;
+    {
+        int dev = 0, ncu = 0, coop = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            persist_cols_shape<T>(s, h, p, ncu)) {
+            (void)hipGetLastError();
+            if (s->persist_ok[pi] < 0) s->persist_ok[pi] = 1;
+            return s->persist_ok[pi] == 1;
+        }
+        (void)hipGetLastError();
+        p = PersistShape();
+    }
     p.rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
     p.fn = persist_kernel<T>(h.k, p.rows, false);
     p.fn_mon = persist_kernel<T>(h.k, p.rows, true);
     p.nblk = h.nblk;
+    p.nA = h.nblk;
     p.lds = lz_persist_lds(s->up.d, s->up.ncoef, s->ncfg, s->nhund, sizeof(T));
     if (!p.fn) return false;
     if (s->persist_ok[pi] < 0) {
@@ -807,14 +942,17 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
     LzArgs<T> lz = lz_args<T>(s);
     const HvShape h = hv_shape(s, sched_of<T>(s));
     if (h.xcd) a.colord = s->d_colord;
-    lz.nA = lz.nB = p.nblk;
+    lz.nA = lz.nB = p.nA;
     int spins = kBarrierSpins;
+    PersistMap pm = p.map;
     void *args[] = {&a, &lz, &nsteps, &spins, &mon};
-    hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, st, lz.st);
+    void *args_cols[] = {&a, &lz, &pm, &nsteps, &spins, &mon};
+    hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(256), 0, st, lz.st);
     // persist == 2 (a test hook) takes the refusal path without launching
     const hipError_t e = s->persist == 2 ? hipErrorCooperativeLaunchTooLarge
                                          : hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk),
-                                                                      dim3(kBlock), args, (unsigned)p.lds, st);
+                                                                      dim3(p.nthreads), p.cpw ? args_cols : args,
+                                                                      (unsigned)p.lds, st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         // A refusal of the cooperative launch itself (nothing ran): the grid does not fit at
@@ -955,7 +1093,7 @@ static int read_state(ed_sector *s, HostLz &h, int upto)
     h.j = hdr[0];
     h.status = hdr[2];
     if (h.status == 3) {   // the persistent kernel's barrier timed out: never use it here again
-        hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s->stream, d);
+        hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(256), 0, s->stream, d);
         (void)hipStreamSynchronize(s->stream);
         s->persist = 0;
         return fail(ED_ERR_HIP, "persistent Lanczos kernel: grid barrier wait timed out (run stopped)");
@@ -1008,6 +1146,7 @@ static int ensure_colord(ed_sector *s)
             }
         }
     }
+    s->col_bfs.assign(order.begin(), order.end());
     const int q = (d + 7) / 8, nblk = 8 * q;
     std::vector<int32_t> map(nblk);
     for (int b = 0; b < nblk; ++b) {
@@ -1658,8 +1797,9 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
 int ed_lanczos_set_persistent(ed_sector *sec, int enable)
 {
     ED_TRY(check_common(sec, 0));
-    if (enable < 0 || enable > 2) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0, 1 or 2");
-    sec->persist = enable;
+    if (enable < 0 || enable > 3) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0, 1, 2 or 3");
+    sec->persist = enable == 3 ? 1 : enable;
+    sec->persist_cols = enable == 3 ? 0 : 1;
     sec->persist_ok[0] = sec->persist_ok[1] = -1;
     return ED_OK;
 }

@@ -230,6 +230,13 @@ struct ed_sector {
     double tune_ms32[ed::kSchedules] = {};
     int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
     int colord_blocks = 0;
+    std::vector<int32_t> col_bfs;  // down columns in the breadth-first order of the down-hop graph
+    int hund_maxrow = 0;           // most Hund transitions of one (impurity up, impurity down) pair
+    // map of the multi-column persistent kernel (PersistMap): [cols][pslot][idle] for the
+    // column order of pmap_xcd (1: XCD-grouped, 0: natural) and pmap_cpw columns per workgroup
+    int32_t *d_pmap = nullptr;
+    int pmap_cpw = 0, pmap_xcd = -1, pmap_nwg = 0, pmap_nidle = 0;
+    int persist_cols = 1;          // the multi-column persistent kernel where it applies (1) or never (0)
     // S_z (Nambu) sector: one table over 2 Ns bits in `up` (dn.d = 1), the quartic
     // diagonal QUART_INT_SZ in d_quart, and the constant tr(Ec_dn) + tr(Eb_dn) that the
     // particle-hole transformation moves out of H

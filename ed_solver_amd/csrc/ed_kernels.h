@@ -817,6 +817,53 @@ __device__ inline int grid_barrier(LzState *st, unsigned b, int spin_limit, int 
     return *flag;
 }
 
+// grid_barrier in two halves, for work that needs no other workgroup between arriving and
+// leaving: grid_arrive (every wave waits for its memory accesses, the workgroup meets,
+// thread 0 arrives; true when it was the last arrival and has released barrier b), then
+// grid_wait (thread 0 waits for the release unless it made it; the rest as grid_barrier).
+__device__ inline bool grid_arrive(LzState *st, unsigned b)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    bool released = false;
+    if (threadIdx.x == 0) {
+        const unsigned nblk = gridDim.x, g = blockIdx.x / kBarGroupSize;
+        const unsigned ng = (nblk + kBarGroupSize - 1) / kBarGroupSize;
+        const unsigned sz = min((unsigned)kBarGroupSize, nblk - g * kBarGroupSize);
+        const unsigned c = __hip_atomic_fetch_add(&st->bar_grp[16 * g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == sz * b - 1) {
+            const unsigned t = __hip_atomic_fetch_add(&st->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t == ng * b - 1) {
+                for (unsigned h = 0; h < ng; ++h)
+                    __hip_atomic_store(&st->bar_rel[16 * h], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+    }
+    return released;
+}
+
+__device__ inline int grid_wait(LzState *st, unsigned b, bool released, int spin_limit, int *flag,
+                                const int *stop_word = nullptr, int jdone = 0)
+{
+    if (threadIdx.x == 0) {
+        const unsigned g = blockIdx.x / kBarGroupSize;
+        const bool err = released ? false : bar_wait(&st->bar_rel[16 * g], b, st, spin_limit);
+        if (err) {
+            __hip_atomic_store(&st->bar_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st->status = 3;
+        }
+        int stop = 0;
+        if (!err && stop_word) {
+            const int sa = ld_agent(stop_word);
+            stop = sa != 0 && sa <= jdone;
+        }
+        *flag = err ? 1 : stop ? 2 : 0;
+    }
+    __syncthreads();
+    return *flag;
+}
+
 // Bytes of dynamic LDS of the persistent Lanczos kernel: the fast H.v's layout, then
 // u_{j-1} and the diagonal of the workgroup's rows (T[dup] each, 16-byte aligned).
 __host__ __device__ inline size_t lz_persist_lds(int dup, int ncoef, int ncfg, int nhund, size_t tsize)
@@ -1052,6 +1099,349 @@ __device__ inline void lz_persist_body(const HvArgs<T> &a, const LzArgs<T> &lz, 
             st->j_k2 = j - 1;
         }
         if (grid_barrier(st, ++nbar, spin_limit, flag, MON ? &st->stop_at : nullptr, j - 1)) return;
+    }
+}
+
+
+// ------------------------------------ persistent Lanczos, several columns per workgroup
+
+// Where the columns of the multi-column persistent kernel live and where their partial
+// sums go: workgroup b owns the columns cols[b * cpw + c] (-1 = idle), column ido writes
+// its partial sums at index pslot[ido] of partA / partB / partC, the index the two-kernel
+// step's workgroup for that column uses, so both sum the same partials in the same order;
+// the nidle indices idle[] of that layout hold no column and are zeroed at launch.
+struct PersistMap {
+    const int32_t *cols, *pslot, *idle;
+    int nidle, cpw;
+};
+
+// Hund transitions a row of this kernel gathers in the same batch as its down-hop
+// neighbours; sectors with more per row keep the one-column kernel.
+constexpr int kPersistHund = 2;
+
+// Bytes of dynamic LDS of the multi-column persistent kernel (offsets 16-byte aligned):
+//   shared by the workgroup's columns: pk[K][dup] (u32), du[dup] (T), cu[dup] (i32),
+//   swap[dup][ncfg] (u16, Hund only), coef[ncoef], uimp[ncfg^2], hval[nhund] (T),
+//   hptr[ncfg^2 + 1], htr[nhund] (i32);
+//   per column: col[dup], colm[dup] (T), swdn[ncfg] (i32), then (read by tables of width
+//   >= 12, whose down-hop entries stay out of scalar registers) cod[K] (T), cjd[K] (i32).
+struct PersistLds {
+    size_t pk, du, cu, sw, coef, uimp, hval, hptr, htr, cols, col_stride, total;
+};
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) / 16 * 16; }
+__host__ __device__ inline PersistLds persist_cols_lds(int dup, int K, int ncoef, int ncfg, int nhund, int hund,
+                                                       int cpw, size_t tsize)
+{
+    PersistLds L;
+    const size_t nq = (size_t)ncfg * ncfg;
+    L.pk = 0;
+    L.du = al16(L.pk + 4 * (size_t)K * dup);
+    L.cu = al16(L.du + tsize * dup);
+    L.sw = al16(L.cu + 4 * (size_t)dup);
+    L.coef = al16(L.sw + (hund ? 2 * (size_t)dup * ncfg : 0));
+    L.uimp = al16(L.coef + tsize * ncoef);
+    L.hval = al16(L.uimp + tsize * nq);
+    L.hptr = al16(L.hval + tsize * nhund);
+    L.htr = al16(L.hptr + 4 * (nq + 1));
+    L.cols = al16(L.htr + 4 * (size_t)nhund);
+    L.col_stride = al16(al16(2 * tsize * dup + 4 * (size_t)ncfg) + (tsize + 4) * (size_t)K);
+    L.total = L.cols + L.col_stride * cpw;
+    return L;
+}
+
+// `nsteps` Lanczos steps in one cooperative launch, as lz_persist_body, with map.cpw
+// columns per workgroup of 256 * cpw threads: threads 256 c .. 256 c + 255 own column c of
+// the workgroup, their rows r = t + 256 i (i < R) exactly as one workgroup of the two-kernel
+// step's fast H.v owns its column, so every partial sum is formed in the same order.  What
+// the columns of a workgroup share sits in LDS once: the rows' packed up-hop entries, their
+// diagonal and impurity bits, their Hund swap table.  The Hund targets of a row are found in
+// LDS, so its down-hop neighbours and its Hund targets (at most kPersistHund) are gathered
+// in one batch of device-coherent loads, RB rows at a time.  The grid has a quarter as many
+// workgroups as the one-column kernel at cpw = 4, so each grid barrier has fewer arrivals
+// and each workgroup's reading of the partial sums is shared by four columns.
+// The second grid barrier of a step (after u_{j+1} is stored) is split: a workgroup
+// arrives, computes the part of the next H.v that needs only its own columns (diagonal and
+// up hops, from LDS, the first terms of every row's sum), then waits for the release
+// before it gathers its neighbours.
+
+template <typename T, int K, int R, bool MON>
+__device__ inline void lz_persist_cols_body(const HvArgs<T> &a, const LzArgs<T> &lz, const PersistMap &pm,
+                                            int nsteps, int spin_limit, LzMonitor mon, unsigned char *smem,
+                                            double *red, double *sh, double *pub, int *flag)
+{
+    constexpr int NT = kBlock;                           // threads per column
+    constexpr int RB = (K * R <= 16) ? R : (K * R >= 48 ? 1 : 2);   // rows per batch of loads
+    LzState *st = lz.st;
+    if (st->status) return;   // written by an earlier kernel: the same for every workgroup
+    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund, cpw = pm.cpw;
+    const PersistLds L = persist_cols_lds(dup, K, a.ncoef_up, ncfg, nh, a.hund, cpw, sizeof(T));
+    uint32_t *pk = reinterpret_cast<uint32_t *>(smem + L.pk);
+    T *du = reinterpret_cast<T *>(smem + L.du);
+    int *cup = reinterpret_cast<int *>(smem + L.cu);
+    uint16_t *swp = reinterpret_cast<uint16_t *>(smem + L.sw);
+    T *coef = reinterpret_cast<T *>(smem + L.coef);
+    T *uimp = reinterpret_cast<T *>(smem + L.uimp);
+    T *hval = reinterpret_cast<T *>(smem + L.hval);
+    int *hptr = reinterpret_cast<int *>(smem + L.hptr);
+    int *htr = reinterpret_cast<int *>(smem + L.htr);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // the column group of this wave (uniform per wave)
+    const int cg = __builtin_amdgcn_readfirstlane(tid >> 8), lt = tid & (NT - 1);
+    T *col = reinterpret_cast<T *>(smem + L.cols + L.col_stride * cg);
+    T *colm = col + dup;
+    int *swdn = reinterpret_cast<int *>(colm + dup);
+    T *cod = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(col) + al16(2 * sizeof(T) * dup + 4 * ncfg));
+    int *cjd = reinterpret_cast<int *>(cod + K);
+    constexpr bool DL = K >= 12;   // down-hop entries of the column read from LDS
+    const bool lead = blockIdx.x == 0 && tid == 0;
+    unsigned nbar = 0;
+    int j = st->j_k1;
+    const int ido = __builtin_amdgcn_readfirstlane(pm.cols[blockIdx.x * cpw + cg]);
+    const bool act = ido >= 0;
+    const int ps = act ? pm.pslot[ido] : 0;
+    int cd = 0, nd = 0;
+    T ddg = (T)0;
+    T od[K];
+    int jd[K];
+    if (act) {
+        cd = a.imp_dn[ido];
+        nd = a.ptr_dn[ido + 1] - a.ptr_dn[ido];
+        ddg = a.diag_dn[ido];
+    }
+    if constexpr (DL) {
+        if (lt < K) {
+            cod[lt] = (act && lt < nd) ? a.off_dn[a.ptr_dn[ido] + lt] : (T)0;
+            cjd[lt] = (act && lt < nd) ? a.rk_dn[a.ptr_dn[ido] + lt] * dup : 0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            od[k] = (act && k < nd) ? a.off_dn[a.ptr_dn[ido] + k] : (T)0;
+            jd[k] = (act && k < nd) ? a.rk_dn[a.ptr_dn[ido] + k] * dup : 0;
+        }
+    }
+    const int base = act ? ido * dup : 0;
+    // ---- tables shared by the workgroup's columns, and this column's vectors
+    for (int e = tid; e < K * dup; e += nt) pk[e] = a.ell_pk_up[e];
+    for (int e = tid; e < dup; e += nt) {
+        du[e] = a.diag_up[e];
+        cup[e] = a.imp_up[e];
+    }
+    for (int e = tid; e < a.ncoef_up; e += nt) coef[e] = a.coef_up[e];
+    for (int e = tid; e < nq; e += nt) uimp[e] = a.uimp[e];
+    if (a.hund) {
+        for (int e = tid; e < dup * ncfg; e += nt) swp[e] = (uint16_t)a.swap_up[e];
+        for (int e = tid; e <= nq; e += nt) hptr[e] = a.hund_ptr[e];
+        for (int e = tid; e < nh; e += nt) {
+            htr[e] = a.hund_tr[e];
+            hval[e] = a.hund_val[e];
+        }
+        if (act)
+            for (int e = lt; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
+    }
+    {
+        const T *x0 = lz.slot(j);
+        const T *xm = lz.slot(j > 1 ? j - 1 : j);
+        for (int r = lt; r < dup; r += NT) {
+            col[r] = act ? x0[base + r] : (T)0;
+            colm[r] = (act && j > 1) ? xm[base + r] : (T)0;
+        }
+    }
+    // partial-sum slots no column writes hold zeros (read by every sum of a step): the H.v
+    // partials and both parities of the update partials
+    if (blockIdx.x == 0)
+        for (int e = tid; e < pm.nidle; e += nt) {
+            const int b = pm.idle[e];
+            st_agent(lz.partA + b, 0.0);
+            st_agent(lz.nrm(0) + b, 0.0);
+            st_agent(lz.nrm(1) + b, 0.0);
+            st_agent(lz.crs(0) + b, 0.0);
+            st_agent(lz.crs(1) + b, 0.0);
+        }
+    double bprev = j > 1 ? st->beta[j - 2] : 1.0;
+    int hflag = 0;
+    T zl[R];   // diagonal and up-hop part of the next H.v of each row
+    __syncthreads();
+    for (int s = 0; s < nsteps; ++s) {
+        if (MON && lead) {
+            if (s > 0) {
+                __hip_atomic_store(mon.ab + 2 * (j - 2), pub[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(mon.ab + 2 * (j - 2) + 1, pub[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            hflag = __hip_atomic_load(mon.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        // ---- H.v phase of step j (u_j of the own column is in col)
+        const T *__restrict__ x = lz.slot(j);
+        T *__restrict__ y = lz.slot(j + 1);
+        double *partA = lz.partA;
+        T z[R];
+        double dot = 0.0;
+        if (s == 0) {   // the local part of the first step (later steps form it during the barrier)
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = lt + i * NT;
+                zl[i] = (T)0;
+                if (act && r < dup) {
+                    const T xv = col[r];
+                    T acc = ((du[r] + ddg) + uimp[cup[r] * ncfg + cd]) * xv;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const uint32_t w = pk[k * dup + r];
+                        acc += coef[w >> 16] * col[w & 0xFFFFu];
+                    }
+                    zl[i] = acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int i0 = 0; i0 < R; i0 += RB) {
+            T xd[RB][K], xh[RB][kPersistHund];
+            int h0[RB], nht[RB];
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int r = lt + (i0 + b) * NT;
+                const bool ok = act && r < dup;
+#pragma unroll
+                for (int k = 0; k < K; ++k) xd[b][k] = (ok && k < nd) ? ld_agent(x + (DL ? cjd[k] : jd[k]) + r) : (T)0;
+                h0[b] = 0;
+                nht[b] = 0;
+                if (a.hund && ok) {
+                    const int q = cup[r] * ncfg + cd;
+                    h0[b] = hptr[q];
+                    nht[b] = hptr[q + 1] - h0[b];
+                }
+#pragma unroll
+                for (int h = 0; h < kPersistHund; ++h) {
+                    xh[b][h] = (T)0;
+                    if (h < nht[b]) {
+                        const int tr = htr[h0[b] + h];
+                        xh[b][h] = ld_agent(x + (int)swp[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup);
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int i = i0 + b, r = lt + i * NT;
+                z[i] = (T)0;
+                if (act && r < dup) {
+                    const T xv = col[r];
+                    T acc = zl[i];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc += (DL ? cod[k] : od[k]) * xd[b][k];
+#pragma unroll
+                    for (int h = 0; h < kPersistHund; ++h)
+                        if (h < nht[b]) acc += hval[h0[b] + h] * xh[b][h];
+                    z[i] = acc;
+                    dot += (double)acc * (double)xv;
+                }
+            }
+            if constexpr (RB < R) __builtin_amdgcn_sched_barrier(0);
+        }
+        // column partial <z, u_j>: the four wave sums of the column in wave order
+        {
+            const double v = wave_sum(dot);
+            if ((tid & 63) == 0) red[2 * (tid >> 6)] = v;
+            __syncthreads();
+            if (lt == 0 && act) {
+                double t = 0.0;
+                for (int w = 0; w < NT / 64; ++w) t += red[2 * (cg * (NT / 64) + w)];
+                st_agent(partA + ps, t);
+            }
+        }
+        if (grid_barrier(st, ++nbar, spin_limit, flag)) return;
+
+        // ---- update phase of step j: N, P, Q as lz_scalars
+        {
+            const int w = tid >> 6;
+            if (w == 0) {
+                const double t = wave_part_sum<true>(lz.nrm(j), lz.nB);
+                if (tid == 0) sh[0] = t;
+            } else if (w == 1) {
+                const double t = j > 1 ? wave_part_sum<true>(lz.crs(j), lz.nB) : 0.0;
+                if (tid == 64) sh[1] = t;
+            } else if (w == 2) {
+                const double t = wave_part_sum<true>(partA, lz.nA);
+                if (tid == 128) sh[2] = t;
+            }
+        }
+        __syncthreads();
+        const double bj = sqrt(sh[0]);
+        if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+            if (lead) {
+                st->beta[j - 1] = bj;
+                st->status = (j == 1) ? 2 : 1;
+            }
+            return;
+        }
+        const double alpha = sh[2] / (bj * bj) - (j > 1 ? sh[1] / bprev : 0.0);
+        if (lead) {
+            st->beta[j - 1] = bj;
+            st->alpha[j - 1] = alpha;
+            if (MON) {
+                pub[0] = bj;
+                pub[1] = alpha;
+                if (hflag) st_agent(&st->stop_at, j);
+            }
+        }
+        const double cz = 1.0 / bj, cp = j > 1 ? bj / bprev : 0.0, cx = alpha / bj;
+        double nn = 0.0, cr = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = lt + i * NT;
+            if (act && r < dup) {
+                const T xv = col[r];
+                const T v = (T)(cz * (double)z[i] - cp * (double)colm[r] - cx * (double)xv);
+                st_agent(y + base + r, v);
+                nn += (double)v * (double)v;
+                cr += (double)v * (double)xv;
+                colm[r] = xv;
+                col[r] = v;
+            }
+        }
+        {
+            const double v0 = wave_sum(nn), v1 = wave_sum(cr);
+            __syncthreads();   // red of the H.v phase is no longer read
+            if ((tid & 63) == 0) {
+                red[2 * (tid >> 6)] = v0;
+                red[2 * (tid >> 6) + 1] = v1;
+            }
+            __syncthreads();
+            if (lt == 0 && act) {
+                double t0 = 0.0, t1 = 0.0;
+                for (int w = 0; w < NT / 64; ++w) {
+                    t0 += red[2 * (cg * (NT / 64) + w)];
+                    t1 += red[2 * (cg * (NT / 64) + w) + 1];
+                }
+                st_agent(lz.nrm(j + 1) + ps, t0);
+                st_agent(lz.crs(j + 1) + ps, t1);
+            }
+        }
+        bprev = bj;
+        ++j;
+        if (lead) {
+            st->j_k1 = j;
+            st->j_k2 = j - 1;
+        }
+        // arrive at the exchange barrier, then the local part of the next H.v while it fills
+        const bool last = s + 1 == nsteps;
+        bool rel = false;
+        if (!last) rel = grid_arrive(st, ++nbar);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = lt + i * NT;
+            zl[i] = (T)0;
+            if (!last && act && r < dup) {
+                const T xv = col[r];
+                T acc = ((du[r] + ddg) + uimp[cup[r] * ncfg + cd]) * xv;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t w = pk[k * dup + r];
+                    acc += coef[w >> 16] * col[w & 0xFFFFu];
+                }
+                zl[i] = acc;
+            }
+        }
+        if (!last && grid_wait(st, nbar, rel, spin_limit, flag, MON ? &st->stop_at : nullptr, j - 1)) return;
     }
 }
 

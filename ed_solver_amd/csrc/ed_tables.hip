@@ -491,6 +491,8 @@ static int build_interaction(ed_sector *S)
             hptr[(size_t)cu * ncfg + cd + 1] = (int32_t)hcu.size();
         }
     S->nhund = (int)hcu.size();
+    S->hund_maxrow = 0;
+    for (size_t q = 0; q + 1 < hptr.size(); ++q) S->hund_maxrow = std::max(S->hund_maxrow, hptr[q + 1] - hptr[q]);
     S->hund = (hund_j || S->nhund > 0) ? 1 : 0;
     for (size_t t = 0; t < hcu.size(); ++t) htr.push_back(hcu[t] | (hcd[t] << 8));
     std::vector<float> uimp32(uimp.begin(), uimp.end()), hval32(hval.begin(), hval.end());
@@ -520,7 +522,7 @@ static void free_sector(ed_sector *S)
     free_spin(S->up);
     free_spin(S->dn);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
-                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord,
+                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord, S->d_pmap,
                   S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32, S->d_sz_mat};
     for (void *p : ps)
         if (p) (void)hipFree(p);

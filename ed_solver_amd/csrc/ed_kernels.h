@@ -8,10 +8,35 @@
 
 namespace ed {
 
+// x from lane (lane - k) of the same 16-lane row (row_shr:k), or 0 when that lane is
+// outside the row; dpp_ctrl as the DPP encoding of the instruction.
+template <int CTRL, int ROW_MASK = 0xF, int BANK_MASK = 0xF>
+__device__ inline double dpp_shift(double x)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, BANK_MASK, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, BANK_MASK, true);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed order, returned in every lane with the same
+// bits: within each 16-lane row, lane i adds the partial sums of lanes i-1, i-2, i-4, i-8
+// (DPP row shifts), so lane 15 of a row holds the row's sum; row_bcast15 adds row 0 into
+// row 1 and row 2 into row 3, row_bcast31 adds lane 31 into rows 2 and 3, leaving the wave's
+// sum in lane 63, which is broadcast.  DPP moves stay in the vector ALU, off the LDS
+// crossbar a shuffle would use.
 __device__ inline double wave_sum(double v)
 {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_shift<0x111>(v);               // row_shr:1
+    v += dpp_shift<0x112>(v);               // row_shr:2
+    v += dpp_shift<0x114, 0xF, 0xE>(v);     // row_shr:4, banks 1-3
+    v += dpp_shift<0x118, 0xF, 0xC>(v);     // row_shr:8, banks 2-3
+    v += dpp_shift<0x142, 0xA>(v);          // row_bcast:15 into rows 1, 3
+    v += dpp_shift<0x143, 0xC>(v);          // row_bcast:31 into rows 2, 3
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // Block sum in a fixed order, returned to every thread.  `red` holds blockDim/64 + 1.
@@ -1170,7 +1195,10 @@ __device__ inline void lz_persist_cols_body(const HvArgs<T> &a, const LzArgs<T> 
                                             double *red, double *sh, double *pub, int *flag)
 {
     constexpr int NT = kBlock;                           // threads per column
-    constexpr int RB = (K * R <= 16) ? R : (K * R >= 48 ? 1 : 2);   // rows per batch of loads
+    // rows per batch of loads: all of them, or 2 (1 for the widest FP64 rows) where more
+    // would spill registers or measured slower
+    constexpr int LB = K * R * (int)sizeof(T);
+    constexpr int RB = (K * R <= 16) ? R : (LB >= 384 ? 1 : 2);
     LzState *st = lz.st;
     if (st->status) return;   // written by an earlier kernel: the same for every workgroup
     const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund, cpw = pm.cpw;

@@ -32,9 +32,10 @@ HBM_PEAK_GBS = 8000.0
 # KiB), WRITE_SIZE by 1.0 (it reads 65,536 KiB); summary in profiles/lanczos_ns12_66_pmc.md.
 # FP64 fused H.v per launch, by H.v schedule (8 and 16 share one kernel instance and were
 # not separated: null):
-HV_TRAFFIC_F64 = {18: 22816400, 19: 22834620, 20: 22827147}
-# persistent Lanczos kernel k_lz_persist<double, 8, 4>, per step
-PERSIST_TRAFFIC_F64_STEP = 24405937
+HV_TRAFFIC_F64 = {18: 22816400, 19: 22834620, 20: 22855556}
+# multi-column persistent Lanczos kernel k_lz_persist_cols<double, 8, 4> (4 columns per
+# workgroup), per step; same passes with --schedule 20
+PERSIST_TRAFFIC_F64_STEP = 21923560
 
 
 def log(msg):
@@ -124,6 +125,9 @@ def main():
     ap.add_argument("--no-green", action="store_true")
     ap.add_argument("--no-scan", action="store_true", help="skip the whole-Fock-space sector scan (block Lanczos)")
     ap.add_argument("--no-grad", action="store_true", help="skip the Ns=14 gradient workload")
+    ap.add_argument("--schedule", type=int, default=0,
+                    help="force this H.v schedule instead of the autotuner's choice (counter runs, where the "
+                         "profiler's serialisation would make the tuner's timings meaningless)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one sector (or one batch of sectors) per GPU; "
@@ -142,6 +146,9 @@ def main():
     log(f"sector (6,6) dim {sec.dim} built in {t_build*1e3:.1f} ms")
     D = sec.dim
     sched, sched_ms = sec.tune()
+    if args.schedule:
+        sec.set_schedule(args.schedule)
+        sched = args.schedule
     log(f"H.v schedule {sched} (ms for schedules 1/2/4 row kernel, 8 fast, 16 fast XCD-grouped, 18/19/20 fast XCD-grouped with 2/3/4 rows per thread: {sched_ms})")
 
     def rate(prec):
@@ -357,11 +364,17 @@ def main():
     rows = {8: 1, 16: 1, 18: 2, 19: 3, 20: 4}.get(sched)
     kname_hv = (f"k_hv_fast<double, 1, 8, {rows}> (fused Lanczos H.v, schedule {sched})" if rows
                 else f"k_hv_rows<double, 1, ...> (fused Lanczos H.v, schedule {sched})")
-    d_up = comb(model.ns, nu)
-    pk_rows = 1 if d_up <= 256 else 2 if d_up <= 512 else 4
-    kname = (f"k_lz_persist<double, 8, {pk_rows}> (persistent Lanczos, {args.niter} steps per launch)" if persistent
-             else f"{kname_hv} + k_lz_update<double>")
-    traffic = PERSIST_TRAFFIC_F64_STEP * args.niter if (persistent and pk_rows == 4) else None
+    ps = sec.persistent_shape("f64")
+    if persistent and ps["cols_per_wg"] > 1:
+        kname = (f"k_lz_persist_cols<double, {ps['width']}, {ps['rows']}, false> (persistent Lanczos, "
+                 f"{ps['cols_per_wg']} columns per workgroup, {ps['workgroups']} workgroups, {args.niter} steps "
+                 "per launch)")
+    elif persistent:
+        kname = f"k_lz_persist<double, {ps['width']}, {ps['rows']}, false> (persistent Lanczos, {args.niter} steps per launch)"
+    else:
+        kname = f"{kname_hv} + k_lz_update<double>"
+    traffic = (PERSIST_TRAFFIC_F64_STEP * args.niter
+               if (persistent and ps["cols_per_wg"] == 4 and ps["width"] == 8 and ps["rows"] == 4) else None)
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel": kname, "avg_us": run_ms * 1e3, "bytes_per_launch": run_bytes,
                 "steps_per_launch": args.niter,

@@ -127,6 +127,9 @@ SIGNATURES = {
     "ed_lanczos_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ed_lanczos_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                             ctypes.POINTER(ctypes.c_int)]),
+    "ed_lanczos_persistent_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(ctypes.c_int)]),
     "ed_lanczos_persistent_refusals": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "ed_lanczos_persistent_active": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }

@@ -1814,6 +1814,21 @@ int ed_lanczos_persistent_active(ed_sector *sec, int precision, int *active)
     return ED_OK;
 }
 
+int ed_lanczos_persistent_shape(ed_sector *sec, int precision, int *cols_per_wg, int *workgroups, int *rows,
+                                int *width)
+{
+    ED_TRY(check_common(sec, precision));
+    if (!sec->cpb) ED_TRY(tune(sec));
+    PersistShape p;
+    const bool on = precision == ED_PREC_F64 ? persist_shape<double>(sec, p) : persist_shape<float>(sec, p);
+    const HvShape h = hv_shape(sec, precision == ED_PREC_F64 ? sched_of<double>(sec) : sched_of<float>(sec));
+    if (cols_per_wg) *cols_per_wg = on ? std::max(1, p.cpw) : 0;
+    if (workgroups) *workgroups = on ? p.nblk : 0;
+    if (rows) *rows = on ? p.rows : 0;
+    if (width) *width = on ? h.k : 0;
+    return ED_OK;
+}
+
 int ed_lanczos_persistent_refusals(ed_sector *sec, int *refused)
 {
     ED_TRY(check_common(sec, 0));

@@ -502,6 +502,13 @@ module ed_hip
       real(c_double), intent(out) :: bytes_per_launch
     end function
 
+    integer(c_int) function ed_lanczos_persistent_shape(sec, precision, cols_per_wg, workgroups, rows, width) &
+        bind(C, name="ed_lanczos_persistent_shape")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sec
+      integer(c_int), value :: precision
+      integer(c_int), intent(out) :: cols_per_wg, workgroups, rows, width
+    end function
     integer(c_int) function ed_lanczos_persistent_refusals(sec, refused) &
         bind(C, name="ed_lanczos_persistent_refusals")
       import :: c_int, c_ptr

@@ -184,6 +184,13 @@ class Sector:
         """Use the persistent Lanczos kernel where it applies (True, the default) or never."""
         check(lib().ed_lanczos_set_persistent(self.handle, 1 if enable else 0))
 
+    def persistent_shape(self, precision="f64") -> dict:
+        """Launch shape of the persistent kernel: columns per workgroup (0: not used),
+        workgroups, rows per thread, table width."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().ed_lanczos_persistent_shape(self.handle, _prec(precision), *[ctypes.byref(x) for x in v]))
+        return {"cols_per_wg": v[0].value, "workgroups": v[1].value, "rows": v[2].value, "width": v[3].value}
+
     def persistent_refusals(self) -> int:
         """Cooperative launches of the persistent kernel the runtime refused on this sector."""
         v = ctypes.c_int()

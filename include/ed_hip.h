@@ -229,18 +229,29 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
 /* Persistent Lanczos kernel (no reference counterpart; an engine option).  Where the
  * sector's H.v schedule is a fast-kernel one (8, 16, 18-20), d_up <= 1024 and every
  * workgroup of the grid fits on the device at once, fixed-length runs of at least 16
- * steps (ed_lanczos_run, the excitation runs of ed_dynamic) execute in one cooperative
- * launch: H.v and update separated by grid barriers, the static row data in registers
- * and LDS, the vectors exchanged with device-scope coherent loads and stores.  The
- * arithmetic and the order of every sum are those of the two-kernel step, so alpha and
- * beta are the same numbers.  enable = 1 (default): use it where it applies; 0: never;
- * 2 (a test hook): as 1, but every cooperative launch takes the path of a launch the
- * runtime refuses.  A barrier wait that exceeds a few seconds stops the run with
- * ED_ERR_HIP and turns the kernel off for the sector.  A cooperative launch the runtime
- * refuses (hipErrorCooperativeLaunchTooLarge, hipErrorIllegalState: nothing ran) turns
- * it off for that sector and precision, is counted, and the run continues with the
- * two-kernel step; any other launch error returns ED_ERR_HIP. */
+ * steps (ed_lanczos_run, the excitation runs of ed_dynamic) and the ground-state solves
+ * of ed_lanczos_diagonalize execute in one cooperative launch: H.v and update separated
+ * by grid barriers, the static row data in registers and LDS, the vectors exchanged with
+ * device-coherent loads and stores.  Tables of width <= 12 with at most 2 Hund
+ * transitions per row use the multi-column form (up to 4 columns per workgroup of 1024
+ * threads, the row tables shared in LDS); others the one-column form.  Either way the
+ * arithmetic and the order of every sum are those of the two-kernel step at the same
+ * thread layout, so alpha and beta are the same numbers.  enable = 1 (default): use it
+ * where it applies; 0: never; 2 (a test hook): as 1, but every cooperative launch takes
+ * the path of a launch the runtime refuses; 3: the one-column form only.  A barrier wait
+ * that exceeds a few seconds stops the run with ED_ERR_HIP and turns the kernel off for
+ * the sector.  A cooperative launch the runtime refuses (hipErrorCooperativeLaunchTooLarge,
+ * hipErrorIllegalState: nothing ran) turns it off for that sector and precision, is
+ * counted, and the run continues with the two-kernel step; any other launch error
+ * returns ED_ERR_HIP. */
 int ed_lanczos_set_persistent(ed_sector *sec, int enable);
+
+/* Launch shape of the persistent kernel for `precision` on this sector (no reference
+ * counterpart; for reports): columns per workgroup (0: the kernel does not apply; 1 with
+ * the one-column kernel), workgroups, rows per thread, compiled table width.  Any output
+ * may be NULL. */
+int ed_lanczos_persistent_shape(ed_sector *sec, int precision, int *cols_per_wg, int *workgroups, int *rows,
+                                int *width);
 
 /* Cooperative launches of the persistent kernel the runtime refused on this sector (no
  * reference counterpart; lets a caller tell a persistent run from its fallback). */

@@ -36,6 +36,9 @@ HV_TRAFFIC_F64 = {18: 22816400, 19: 22834620, 20: 22855556}
 # multi-column persistent Lanczos kernel k_lz_persist_cols<double, 8, 4> (4 columns per
 # workgroup), per step; same passes with --schedule 20
 PERSIST_TRAFFIC_F64_STEP = 21923560
+# multi-run persistent kernel k_lz_persist_runs<double, 12, 4> of the batched G(i w_n) runs
+# (6 runs of dim 731,808), per job-step (one run advanced one step)
+RUNS_TRAFFIC_F64_JOB_STEP = 45275093
 
 
 def log(msg):
@@ -220,13 +223,41 @@ def main():
         for _ in range(reps):
             dyn_b = ed.dynamic_from_state(sec, psi, E0, jobs, 0, 36, wn)
         t_batch = (time.perf_counter() - t0) / reps
+        # the batch kernel of that last call: one launch for all runs, HIP event pair
+        n_launch, _, lastk = ed.batched_stats(last=True)
+        batch_kernel = None
+        if n_launch and lastk["job_steps"]:
+            ach_b = lastk["bytes"] / (lastk["ms"] * 1e-3) / 1e9
+            batch_kernel = {"kernel": "k_lz_persist_runs<double, 12, 4> (all excitation runs in one cooperative "
+                                      "launch)", "avg_us": lastk["ms"] * 1e3, "job_steps": lastk["job_steps"],
+                            "us_per_job_step": lastk["ms"] * 1e3 / lastk["job_steps"],
+                            "bytes_per_launch": lastk["bytes"], "achieved": ach_b, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": ach_b / HBM_PEAK_GBS,
+                            "traffic": RUNS_TRAFFIC_F64_JOB_STEP * lastk["job_steps"]}
+            log(f"batch kernel: {lastk['ms']*1e3:.0f} us for {lastk['job_steps']:.0f} job-steps "
+                f"({lastk['ms']*1e3/lastk['job_steps']:.2f} us per job-step) -> {ach_b:.0f} GB/s")
+        # the same runs with FP32 vectors and H.v (FP64 scalars and continued fraction): rate,
+        # and the relative error at each w_n against the FP64 device result (which the GPU
+        # tests hold to 1e-10 of the oracle)
+        ed.dynamic_from_state(sec, psi, E0, jobs, 0, 36, wn, precision="f32")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dyn_32 = ed.dynamic_from_state(sec, psi, E0, jobs, 0, 36, wn, precision="f32")
+        t_32 = (time.perf_counter() - t0) / reps
+        per_w = np.max(np.abs(dyn_32 - dyn_b) / np.abs(dyn_b), axis=0)
+        green_f32 = {"batched_solves_per_s": len(secs) / t_32, "vs_fp64": t_batch / t_32,
+                     "per_w_rel_err_max": float(per_w.max()), "per_w_rel_err_w1": float(per_w[0]),
+                     "per_w_rel_err_median": float(np.median(per_w)), "per_w_rel_err_last": float(per_w[-1])}
+        log(f"G(iw) FP32: {len(secs)/t_32:.1f} solves/s; per-w relative error max {per_w.max():.2e} "
+            f"(w_1 {per_w[0]:.2e}, median {np.median(per_w):.2e})")
         g = dyn_b.sum(axis=0) / model.nc
         green = {"solves": len(secs), "nitergreen": 36, "n_matsubara": 2048, "beta": 100.0,
                  "excited_dim": s_p.dim, "plain_solves_per_s": len(secs) / t_plain,
                  "batched_solves_per_s": len(secs) / t_batch, "batched_vs_plain": t_plain / t_batch,
                  "batched_host_vectors_solves_per_s": len(secs) / t_host,
                  "max_abs_diff_device_vs_host_vectors": float(np.max(np.abs(dyn_b - dyn_h))),
-                 "G_iw0": [float(g[0].real), float(g[0].imag)]}
+                 "G_iw0": [float(g[0].real), float(g[0].imag)], "batch_kernel": batch_kernel,
+                 "fp32": green_f32}
         log(f"G(iw): plain {len(secs)/t_plain:.1f} solves/s, batched from host vectors {len(secs)/t_host:.1f}, "
             f"batched from the state on the device {len(secs)/t_batch:.1f} solves/s")
         s_p.close(); s_h.close()

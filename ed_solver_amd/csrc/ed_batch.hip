@@ -136,6 +136,15 @@ struct BatchWS {
     int device = -1;
 };
 static BatchWS g_ws;
+// launches of the multi-run persistent kernel, and launches the runtime refused (then the
+// two-launch batch ran), since the library was loaded
+static int g_runs_launched = 0, g_runs_refused = 0;
+// the last launch: its duration (HIP event pair on the stream), job-steps and algorithmic
+// bytes (5 s D per job-step: H.v reads u_j and writes H u_j, the recurrence reads u_{j-1},
+// u_j and writes u_{j+1})
+static double g_runs_ms = 0.0, g_runs_job_steps = 0.0, g_runs_bytes = 0.0;
+// wait rounds of a grid-barrier poll of the multi-run kernel before it gives up (seconds)
+constexpr int kRunsBarrierSpins = 1 << 21;
 
 static int ws_reserve(size_t bytes, int device)
 {
@@ -279,6 +288,297 @@ static int run_batch(const std::vector<ed_sector *> &secs, const std::vector<con
     return ED_OK;
 }
 
+// ------------------------------------------------- every run in one persistent launch
+
+template <typename T, int K, int R>
+__global__ __launch_bounds__(1024, 1) void k_lz_persist_runs(const PersistRun<T> *runs, int nruns, LzState *bar,
+                                                             int nsteps, int spin_limit)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double sh[kSums];
+    __shared__ int flag;
+    lz_persist_runs_body<T, K, R>(runs, nruns, bar, nsteps, spin_limit, reinterpret_cast<unsigned char *>(smem_d), sh,
+                                  &flag);
+}
+
+// beta_{n+1} of every run after the launch (one workgroup per run)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finish_runs(const PersistRun<T> *runs)
+{
+    __shared__ double grp[kSums];
+    lz_finish_body(runs[blockIdx.x].lz, grp);
+}
+
+// Zeroes n words (a barrier state before a launch; a kernel, so a captured graph holds no
+// memset node).
+__global__ void k_zero_words(unsigned *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+template <typename T, int K, int R>
+static const void *runs_fn()
+{
+    return reinterpret_cast<const void *>(&k_lz_persist_runs<T, K, R>);
+}
+
+// The instances: table widths 4, 8 and 12, 1, 2 or 4 rows per thread (d_up <= 1024).
+template <typename T>
+static const void *runs_kernel(int k, int rows)
+{
+    switch (k * 8 + rows) {
+    case 4 * 8 + 1: return runs_fn<T, 4, 1>();
+    case 4 * 8 + 2: return runs_fn<T, 4, 2>();
+    case 4 * 8 + 4: return runs_fn<T, 4, 4>();
+    case 8 * 8 + 1: return runs_fn<T, 8, 1>();
+    case 8 * 8 + 2: return runs_fn<T, 8, 2>();
+    case 8 * 8 + 4: return runs_fn<T, 8, 4>();
+    case 12 * 8 + 1: return runs_fn<T, 12, 1>();
+    case 12 * 8 + 2: return runs_fn<T, 12, 2>();
+    case 12 * 8 + 4: return runs_fn<T, 12, 4>();
+    default: return nullptr;
+    }
+}
+
+static int runs_width(const ed_sector *s) { return std::max(s->up.kpad, s->dn.kpad); }
+static int runs_rows(const ed_sector *s) { return s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4; }
+
+// Sectors the multi-run kernel takes: real, not S_z, with the packed up-hop table,
+// 64 <= d_up <= 1024, table width <= 12, at most kPersistHund Hund transitions per row,
+// and the persistent kernel not switched off (ed_lanczos_set_persistent).
+static bool runs_capable(const ed_sector *s)
+{
+    return !s->sz && !s->cplx && s->persist != 0 && s->up.ell_pk && s->up.ncoef > 0 && s->up.d >= 64 &&
+           s->up.d <= 4 * kBlock && runs_width(s) <= 12 && (!s->hund || s->hund_maxrow <= kPersistHund);
+}
+
+// Return of run_batch_persist when the kernel does not take the batch (not an ED_ code):
+// the caller runs the two-launch batch instead.
+constexpr int kRunsNotUsed = 1;
+
+// Workgroups per run.  A run's columns go round by round to its workgroups' 4 column
+// groups and a step lasts about as long as the most rounds any run needs, so run J gets
+// ceil(d_dn,J / (4 nr)) workgroups for the least round count nr whose total fits the grid
+// (gmax), each then raised to a multiple of 8 (the XCD count, so every XCD serves every
+// run) while the grid has room.  False when not even one workgroup per run fits.
+static bool runs_table(const std::vector<int> &ddn, int gmax, std::vector<int> &nwg)
+{
+    const int n = (int)ddn.size();
+    const int maxd = *std::max_element(ddn.begin(), ddn.end());
+    nwg.assign(n, 0);
+    int tot = 0;
+    for (int nr = 1;; ++nr) {
+        tot = 0;
+        for (int k = 0; k < n; ++k) {
+            nwg[k] = (ddn[k] + kRunCpw * nr - 1) / (kRunCpw * nr);
+            tot += nwg[k];
+        }
+        if (tot <= gmax) break;
+        if (nr >= maxd) return false;
+    }
+    for (int k = 0; k < n; ++k) {
+        const int up = std::min((nwg[k] + 7) / 8 * 8, (ddn[k] + kRunCpw - 1) / kRunCpw);
+        if (tot - nwg[k] + up <= gmax) {
+            tot += up - nwg[k];
+            nwg[k] = up;
+        }
+    }
+    return true;
+}
+
+// Column map of one run: workgroup w of the run (grid index wg0 + w, on XCD (wg0 + w) mod 8)
+// takes in round rd, group g the column cmap[(w * nround + rd) * 4 + g].  The breadth-first
+// column order of the down-hop graph is cut into one contiguous share per XCD, in
+// proportion to the XCD's workgroups, so the columns an XCD holds are neighbours of each
+// other; within a share, each round gives the XCD's column groups consecutive columns.
+static int runs_cmap(const std::vector<int32_t> &order, int ddn, int wg0, int nwg, int nround,
+                     std::vector<int32_t> &cmap)
+{
+    cmap.assign((size_t)nwg * nround * kRunCpw, -1);
+    std::vector<int> members[8];
+    for (int w = 0; w < nwg; ++w) members[(wg0 + w) % 8].push_back(w);
+    int off = 0, seen = 0;
+    for (int x = 0; x < 8; ++x) {
+        const int nx = (int)members[x].size();
+        if (!nx) continue;
+        seen += nx;
+        const int end = (int)((int64_t)ddn * seen / nwg);
+        for (int p = 0; p < end - off; ++p) {
+            const int rd = p / (kRunCpw * nx), q = p % (kRunCpw * nx);
+            if (rd >= nround) return fail(ED_ERR_STATE, "multi-run Lanczos: column map overflow");
+            cmap[((size_t)members[x][q / kRunCpw] * nround + rd) * kRunCpw + q % kRunCpw] = order[off + p];
+        }
+        off = end;
+    }
+    std::vector<char> hit(ddn, 0);
+    for (int32_t c : cmap)
+        if (c >= 0) {
+            if (c >= ddn || hit[c]) return fail(ED_ERR_STATE, "multi-run Lanczos: column map is not a permutation");
+            hit[c] = 1;
+        }
+    if (off != ddn) return fail(ED_ERR_STATE, "multi-run Lanczos: column map misses columns");
+    return ED_OK;
+}
+
+// Runs `niter` Lanczos steps of every job in one cooperative launch of k_lz_persist_runs
+// (same contract as run_batch); kRunsNotUsed when the kernel does not take the batch or the
+// runtime refuses the launch.
+template <typename T>
+static int run_batch_persist(const std::vector<ed_sector *> &secs, const std::vector<const double *> &h_phi,
+                             const std::vector<double> &nrm2, int niter, hipStream_t st, std::vector<double> &a_out,
+                             std::vector<double> &b_out, bool phi_on_device)
+{
+    const int nj = (int)secs.size();
+    int K = 4, R = 1;
+    for (ed_sector *s : secs) {
+        if (!runs_capable(s)) return kRunsNotUsed;
+        K = std::max(K, runs_width(s));
+        R = std::max(R, runs_rows(s));
+    }
+    const void *fn = runs_kernel<T>(K, R);
+    if (!fn) return kRunsNotUsed;
+    // one workgroup per CU (1024 threads, LDS-heavy): the grid is the CU count
+    int dev = 0, ncu = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return kRunsNotUsed;
+    }
+    std::vector<int> ddn(nj), nwg;
+    for (int k = 0; k < nj; ++k) ddn[k] = secs[k]->dn.d;
+    if (!runs_table(ddn, std::min(ncu, kBarGroups * kBarGroupSize), nwg)) return kRunsNotUsed;
+    size_t lds = 0;
+    for (int k = 0; k < nj; ++k) {
+        const ed_sector *s = secs[k];
+        const int nr = (ddn[k] + kRunCpw * nwg[k] - 1) / (kRunCpw * nwg[k]);
+        if (nr > kRunMaxRounds) return kRunsNotUsed;
+        lds = std::max(lds, persist_runs_lds(s->up.d, runs_width(s), K, s->up.ncoef, s->ncfg, s->nhund, s->hund, nr,
+                                             sizeof(T)).total);
+    }
+    int per_cu = 0;
+    if (lds > 159 * 1024 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock * kRunCpw, lds) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        return kRunsNotUsed;
+    }
+    // ---- work space: barrier state, run table, then per run its state, 3 Krylov slots,
+    //      start vector, partial sums and column map
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t o_bar = take(sizeof(LzState));
+    const size_t o_runs = take(sizeof(PersistRun<T>) * nj);
+    std::vector<size_t> o_state(nj), o_basis(nj), o_start(nj), o_part(nj), o_map(nj);
+    std::vector<int64_t> ld(nj);
+    std::vector<int> ps(nj), nround(nj), wg0(nj);
+    std::vector<std::vector<int32_t>> cmaps(nj);
+    int g = 0;
+    for (int k = 0; k < nj; ++k) {
+        ed_sector *s = secs[k];
+        ED_TRY(ensure_col_bfs(s));
+        wg0[k] = g;
+        g += nwg[k];
+        nround[k] = (ddn[k] + kRunCpw * nwg[k] - 1) / (kRunCpw * nwg[k]);
+        ED_TRY(runs_cmap(s->col_bfs, ddn[k], wg0[k], nwg[k], nround[k], cmaps[k]));
+        ld[k] = (s->dim + 63) / 64 * 64;
+        ps[k] = (ddn[k] + 63) / 64 * 64;
+        o_state[k] = take(sizeof(LzState));
+        o_basis[k] = take(sizeof(T) * (size_t)ld[k] * 3);
+        o_start[k] = take(sizeof(double) * (size_t)s->dim);
+        o_part[k] = take(sizeof(double) * 5 * (size_t)ps[k]);
+        o_map[k] = take(sizeof(int32_t) * cmaps[k].size());
+    }
+    ED_TRY(ws_reserve(off, secs[0]->device));
+    char *base = (char *)g_ws.ptr;
+    std::vector<PersistRun<T>> runs(nj);
+    for (int k = 0; k < nj; ++k) {
+        ed_sector *s = secs[k];
+        PersistRun<T> &P = runs[k];
+        P.a = hv_args_of<T>(s);
+        P.a.ell_pk_up = s->up.ell_pk;   // every row of the packed table (hv_args_of drops it for wide coefficient tables)
+        P.lz.base = (T *)(base + o_basis[k]);
+        P.lz.ld = ld[k];
+        P.lz.nslots = 3;
+        P.lz.st = (LzState *)(base + o_state[k]);
+        P.lz.partA = (double *)(base + o_part[k]);
+        P.lz.partB = P.lz.partA + ps[k];
+        P.lz.partC = P.lz.partB + 2 * ps[k];
+        P.lz.pstride = ps[k];
+        P.lz.nA = P.lz.nB = ddn[k];
+        P.lz.colmap = 0;
+        P.lz.dup = s->up.d;
+        P.lz.colord = nullptr;
+        P.start = (const double *)(base + o_start[k]);
+        P.cmap = (const int32_t *)(base + o_map[k]);
+        P.wg0 = wg0[k];
+        P.nwg = nwg[k];
+        P.nround = nround[k];
+        P.kk = runs_width(s);
+        ED_HIP_CHECK(hipMemcpyAsync(base + o_start[k], h_phi[k], sizeof(double) * s->dim,
+                                    phi_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scale_into, dim3(egrid(s->dim)), dim3(kBlock), 0, st, (const double *)(base + o_start[k]),
+                           (double *)(base + o_start[k]), s->dim, 1.0 / std::sqrt(nrm2[k]));
+        ED_HIP_CHECK(hipMemcpyAsync(base + o_map[k], cmaps[k].data(), sizeof(int32_t) * cmaps[k].size(),
+                                    hipMemcpyHostToDevice, st));
+    }
+    const PersistRun<T> *d_runs = (const PersistRun<T> *)(base + o_runs);
+    LzState *bar = (LzState *)(base + o_bar);
+    ED_HIP_CHECK(hipMemcpyAsync(base + o_runs, runs.data(), sizeof(PersistRun<T>) * nj, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_words, dim3(16), dim3(kBlock), 0, st, (unsigned *)bar, (int)(sizeof(LzState) / 4));
+    int nruns = nj, nsteps = niter, spins = kRunsBarrierSpins;
+    void *args[] = {&d_runs, &nruns, &bar, &nsteps, &spins};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ED_HIP_CHECK(hipEventCreate(&ev[0]));
+    ED_HIP_CHECK(hipEventCreate(&ev[1]));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
+    } ev_guard{ev};
+    ED_HIP_CHECK(hipEventRecord(ev[0], st));
+    const hipError_t e = secs[0]->persist == 2 ? hipErrorCooperativeLaunchTooLarge
+                                               : hipLaunchCooperativeKernel(fn, dim3(g), dim3(kBlock * kRunCpw), args,
+                                                                            (unsigned)lds, st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        // the refusals launch_persist also takes (nothing ran); every other code is an error
+        if (e != hipErrorCooperativeLaunchTooLarge && e != hipErrorIllegalState)
+            return fail(ED_ERR_HIP, std::string("multi-run Lanczos: hipLaunchCooperativeKernel: ") + hipGetErrorString(e));
+        ++g_runs_refused;
+        return kRunsNotUsed;
+    }
+    ED_HIP_CHECK(hipEventRecord(ev[1], st));
+    hipLaunchKernelGGL(k_finish_runs<T>, dim3(nj), dim3(kBlock), 0, st, d_runs);
+    ED_HIP_CHECK(hipGetLastError());
+    ++g_runs_launched;
+    std::vector<LzState> hs(nj + 1);
+    ED_HIP_CHECK(hipMemcpyAsync(&hs[nj], bar, sizeof(LzState), hipMemcpyDeviceToHost, st));
+    for (int k = 0; k < nj; ++k)
+        ED_HIP_CHECK(hipMemcpyAsync(&hs[k], base + o_state[k], sizeof(LzState), hipMemcpyDeviceToHost, st));
+    ED_HIP_CHECK(hipStreamSynchronize(st));
+    if (hs[nj].status == 3) return fail(ED_ERR_HIP, "multi-run Lanczos: grid barrier wait timed out (run stopped)");
+    float ms = 0.0f;
+    ED_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    g_runs_ms = ms;
+    g_runs_job_steps = (double)nj * niter;
+    g_runs_bytes = 0.0;
+    for (ed_sector *s : secs) g_runs_bytes += 5.0 * sizeof(T) * (double)s->dim * niter;
+    a_out.assign((size_t)nj * niter, 0.0);
+    b_out.assign((size_t)nj * niter, 0.0);
+    for (int k = 0; k < nj; ++k) {
+        const LzState &h = hs[k];
+        const int reached = h.j_k1 - 1;
+        if (h.status == 2) return fail(ED_ERR_BREAKDOWN, "batched Lanczos: zero-norm start vector in job " + std::to_string(k));
+        if (h.status == 1 && reached < niter)
+            return fail(ED_ERR_BREAKDOWN, "batched Lanczos: zero-norm vector at iteration " + std::to_string(reached + 1) +
+                                              " of job " + std::to_string(k) + " (Lanczos_plain.h:85-92)");
+        for (int i = 0; i < niter; ++i) {
+            a_out[(size_t)k * niter + i] = h.alpha[i];
+            b_out[(size_t)k * niter + i] = h.beta[i];
+        }
+    }
+    return ED_OK;
+}
+
 }  // namespace ed
 
 using namespace ed;
@@ -379,10 +679,35 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
     }
     hipStream_t st = bs[0]->stream;
     const int niter = nitergreen;
-    std::vector<double> ab, bb;
-    int rc = precision == ED_PREC_F64 ? run_batch<double>(bs, bp, bn, niter, st, ab, bb, phi_on_device)
-                                      : run_batch<float>(bs, bp, bn, niter, st, ab, bb, phi_on_device);
-    if (rc != ED_OK) return rc;
+    // the jobs the multi-run persistent kernel takes run in one launch of it; the rest (or
+    // all, when it does not take them or the runtime refuses it) in the two-launch batch
+    std::vector<double> ab((size_t)nb * niter), bb((size_t)nb * niter);
+    std::vector<int> grp[2];
+    for (int i = 0; i < nb; ++i) grp[runs_capable(bs[i]) ? 0 : 1].push_back(i);
+    int rc = ED_OK;
+    for (int gi = 0; gi < 2; ++gi) {
+        if (grp[gi].empty()) continue;
+        std::vector<ed_sector *> gs;
+        std::vector<const double *> gp;
+        std::vector<double> gn, ga, gb;
+        for (int i : grp[gi]) {
+            gs.push_back(bs[i]);
+            gp.push_back(bp[i]);
+            gn.push_back(bn[i]);
+        }
+        rc = kRunsNotUsed;
+        if (gi == 0)
+            rc = precision == ED_PREC_F64 ? run_batch_persist<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
+                                          : run_batch_persist<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
+        if (rc == kRunsNotUsed)
+            rc = precision == ED_PREC_F64 ? run_batch<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
+                                          : run_batch<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
+        if (rc != ED_OK) return rc;
+        for (size_t q = 0; q < grp[gi].size(); ++q) {
+            std::copy(ga.begin() + q * niter, ga.begin() + (q + 1) * niter, ab.begin() + (size_t)grp[gi][q] * niter);
+            std::copy(gb.begin() + q * niter, gb.begin() + (q + 1) * niter, bb.begin() + (size_t)grp[gi][q] * niter);
+        }
+    }
     // shifted tridiagonals: ph (alpha - E) (green_class_compute_dynamic.f90:310-311)
     std::vector<double> sa((size_t)nb * niter), scale(nb);
     for (int i = 0; i < nb; ++i) {
@@ -414,6 +739,18 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
     (void)hipFreeAsync(d, st);
     (void)hipStreamSynchronize(st);
     return rc;
+}
+
+extern "C" int ed_dynamic_batched_stats(int *persistent_launches, int *refusals, double *last_ms,
+                                        double *last_job_steps, double *last_bytes)
+{
+    std::lock_guard<std::mutex> lock(g_ws.mu);
+    if (persistent_launches) *persistent_launches = g_runs_launched;
+    if (refusals) *refusals = g_runs_refused;
+    if (last_ms) *last_ms = g_runs_ms;
+    if (last_job_steps) *last_job_steps = g_runs_job_steps;
+    if (last_bytes) *last_bytes = g_runs_bytes;
+    return ED_OK;
 }
 
 extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,

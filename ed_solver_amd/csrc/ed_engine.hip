@@ -1159,6 +1159,11 @@ static int ensure_colord(ed_sector *s)
     return ED_OK;
 }
 
+int ensure_col_bfs(ed_sector *s)
+{
+    return ensure_colord(s);
+}
+
 // Autotuner: times the FP64 H.v under every applicable schedule (row kernel with 1, 2
 // or 4 columns per workgroup; fast kernel in natural and in XCD-grouped column order),
 // 3 launches each after one warm-up, HIP events on the sector's stream, and keeps the

@@ -309,4 +309,6 @@ int noalloc_setup(int device);
 // (capacity niter, host) as ed_lanczos_run; ED_ERR_BREAKDOWN as there.
 int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_start, int prec, double *diag,
                    double *subdiag);
+// s->col_bfs: the sector's down columns in the breadth-first order of its down-hop graph
+int ensure_col_bfs(ed_sector *s);
 }  // namespace ed

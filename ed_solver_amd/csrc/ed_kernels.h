@@ -1473,4 +1473,365 @@ __device__ inline void lz_persist_cols_body(const HvArgs<T> &a, const LzArgs<T> 
     }
 }
 
+// ------------------------------------ persistent Lanczos, several runs in one launch
+
+// One Lanczos run of the multi-run persistent kernel: its sector's tables, its recurrence
+// (Krylov slots, scalars, one partial sum per column of the sector: partA[d_dn] and, by
+// parity of j, nrm / crs with pstride >= d_dn), its start vector (FP64, unit norm), the
+// workgroups [wg0, wg0 + nwg) of the grid that serve it, their column map
+// cmap[(w * nround + round) * 4 + group] (-1 = idle) and the sector's table width kk.
+template <typename T>
+struct PersistRun {
+    HvArgs<T> a;
+    LzArgs<T> lz;
+    const double *start;
+    const int32_t *cmap;
+    int wg0, nwg, nround, kk;
+};
+
+// Column groups (of 256 threads) per workgroup of the multi-run kernel, and the most
+// rounds of columns a workgroup takes.
+constexpr int kRunCpw = 4;
+constexpr int kRunMaxRounds = 16;
+
+// Dynamic LDS of the multi-run kernel for one run's sector (offsets 16-byte aligned):
+//   the shared tables of persist_cols_lds for kk table rows;
+//   per (round, group) column record: cod[K] (T), cjd[K], swdn[ncfg], ci, cd, nd (i32),
+//   ddg (T) -- the column's down-hop entries and what the diagonal and Hund terms need;
+//   per group two column buffers col[2][dup] (T): u_j of the round being computed and of
+//   the next one, staged while this one computes;
+//   wave sums sums[round][16][2] (double).
+struct RunsLds {
+    PersistLds t;
+    size_t rec, rec_stride, col, col_stride, sums, total;
+};
+__host__ __device__ inline RunsLds persist_runs_lds(int dup, int kk, int K, int ncoef, int ncfg, int nhund, int hund,
+                                                    int nround, size_t tsize)
+{
+    RunsLds L;
+    L.t = persist_cols_lds(dup, kk, ncoef, ncfg, nhund, hund, 0, tsize);
+    L.rec = L.t.cols;
+    L.rec_stride = al16(tsize * (K + 1) + 4 * (size_t)(K + ncfg + 3));
+    L.col = al16(L.rec + L.rec_stride * (size_t)nround * kRunCpw);
+    L.col_stride = al16(2 * tsize * dup);
+    L.sums = L.col + L.col_stride * kRunCpw;
+    L.total = L.sums + sizeof(double) * 2 * 16 * (size_t)nround;
+    return L;
+}
+
+// One column record of the multi-run kernel (views into LDS).
+template <typename T, int K>
+struct RunCol {
+    T *od;        // [K] down-hop coefficients (0 beyond nd)
+    int *jd;      // [K] neighbour column offsets rk * dup
+    int *swdn;    // [ncfg] down state with its impurity bits replaced (Hund)
+    int *hdr;     // ci, cd, nd
+    T *ddg;       // diagonal of the down state
+    __device__ RunCol(unsigned char *p, int ncfg)
+    {
+        od = reinterpret_cast<T *>(p);
+        ddg = od + K;
+        jd = reinterpret_cast<int *>(ddg + 1);
+        swdn = jd + K;
+        hdr = swdn + ncfg;
+    }
+};
+
+// `nsteps` Lanczos steps of every run of `runs` in ONE cooperative launch (the excitation
+// runs of a Green's function, green_class_computeAA.h:101-302, each a compute_dynamic
+// recurrence, green_class_compute_dynamic.f90:279-334, in sectors of uneven dimension).
+// Workgroup b serves the run whose range holds it; its four column groups take the columns
+// of the run's column map round by round.  What a column needs besides u_j (down-hop
+// entries, diagonal, impurity bits, Hund swap row) is read once, at launch, into LDS.
+// H.v phase, per round: a group gathers the down-hop and Hund neighbours of its column from
+// the other columns with device-coherent loads while the next round's column of u_j is
+// loaded, computes z = H u_j with the up hops and the diagonal from the staged column in
+// LDS, stores z in the slot of u_{j+1} and keeps the wave sums of <z, u_j>; then it stages
+// the next column (one workgroup barrier per round).  The per-column partial sums are
+// formed after the last round, in the order lz_persist_cols_body uses.  One grid barrier
+// for all runs; then every workgroup forms its run's scalars from the fixed-order sums
+// and updates its columns (u_{j+1} over z; wave sums of ||u_{j+1}||^2 and <u_{j+1}, u_j>,
+// added per column after the last round); a second grid barrier ends the step.  The
+// partial sums are indexed by column, so the sums do not depend on the map.  A run whose
+// beta falls below kMinLanczosNorm records it (status 1, or 2 at j = 1) and idles; the
+// others go on.  Every workgroup reaches every barrier, so the launch always drains; a
+// barrier wait past spin_limit ends it with bar->status = 3.  u_1 = start is written in
+// the prologue.
+template <typename T, int K, int R>
+__device__ inline void lz_persist_runs_body(const PersistRun<T> *__restrict__ runs, int nruns, LzState *bar,
+                                            int nsteps, int spin_limit, unsigned char *smem, double *sh, int *flag)
+{
+    constexpr int NT = kBlock;
+    constexpr int LB = K * R * (int)sizeof(T);
+    constexpr int RB = (K * R <= 16) ? R : (LB >= 384 ? 1 : 2);
+    int jr = 0;
+    while (jr + 1 < nruns && (int)blockIdx.x >= runs[jr + 1].wg0) ++jr;
+    const PersistRun<T> &P = runs[jr];
+    const HvArgs<T> &a = P.a;
+    const LzArgs<T> lz = P.lz;
+    LzState *st = lz.st;
+    const int lw = blockIdx.x - P.wg0, nround = P.nround, kk = P.kk;
+    const int dup = a.dup, ddn = a.ddn, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund, hund = a.hund;
+    const RunsLds L = persist_runs_lds(dup, kk, K, a.ncoef_up, ncfg, nh, hund, nround, sizeof(T));
+    uint32_t *pk = reinterpret_cast<uint32_t *>(smem + L.t.pk);
+    T *du = reinterpret_cast<T *>(smem + L.t.du);
+    int *cup = reinterpret_cast<int *>(smem + L.t.cu);
+    uint16_t *swp = reinterpret_cast<uint16_t *>(smem + L.t.sw);
+    T *coef = reinterpret_cast<T *>(smem + L.t.coef);
+    T *uimp = reinterpret_cast<T *>(smem + L.t.uimp);
+    T *hval = reinterpret_cast<T *>(smem + L.t.hval);
+    int *hptr = reinterpret_cast<int *>(smem + L.t.hptr);
+    int *htr = reinterpret_cast<int *>(smem + L.t.htr);
+    double *sums = reinterpret_cast<double *>(smem + L.sums);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int cg = __builtin_amdgcn_readfirstlane(tid >> 8), lt = tid & (NT - 1), wv = tid >> 6;
+    T *colb = reinterpret_cast<T *>(smem + L.col + L.col_stride * cg);   // [2][dup]
+    auto rec = [&](int rd) { return RunCol<T, K>(smem + L.rec + L.rec_stride * ((size_t)rd * kRunCpw + cg), ncfg); };
+    const bool lead = lw == 0 && tid == 0;   // the run's first workgroup records its scalars
+    const int32_t *cmap = P.cmap + (size_t)lw * nround * kRunCpw + cg;
+    // ---- tables of the run's sector, shared by the column groups
+    for (int e = tid; e < kk * dup; e += nt) pk[e] = a.ell_pk_up[e];
+    for (int e = tid; e < dup; e += nt) {
+        du[e] = a.diag_up[e];
+        cup[e] = a.imp_up[e];
+    }
+    for (int e = tid; e < a.ncoef_up; e += nt) coef[e] = a.coef_up[e];
+    for (int e = tid; e < nq; e += nt) uimp[e] = a.uimp[e];
+    if (hund) {
+        for (int e = tid; e < dup * ncfg; e += nt) swp[e] = (uint16_t)a.swap_up[e];
+        for (int e = tid; e <= nq; e += nt) hptr[e] = a.hund_ptr[e];
+        for (int e = tid; e < nh; e += nt) {
+            htr[e] = a.hund_tr[e];
+            hval[e] = a.hund_val[e];
+        }
+    }
+    // ---- the group's column records, every round
+    for (int rd = 0; rd < nround; ++rd) {
+        RunCol<T, K> c = rec(rd);
+        const int ci = cmap[rd * kRunCpw];
+        const int p0 = ci >= 0 ? a.ptr_dn[ci] : 0, nd = ci >= 0 ? a.ptr_dn[ci + 1] - p0 : 0;
+        if (lt < K) {
+            c.od[lt] = lt < nd ? a.off_dn[p0 + lt] : (T)0;
+            c.jd[lt] = lt < nd ? a.rk_dn[p0 + lt] * dup : 0;
+        }
+        if (hund && ci >= 0)
+            for (int e = lt; e < ncfg; e += NT) c.swdn[e] = a.swap_dn[ci * ncfg + e];
+        if (lt == 0) {
+            c.hdr[0] = ci;
+            c.hdr[1] = ci >= 0 ? a.imp_dn[ci] : 0;
+            c.hdr[2] = nd;
+            *c.ddg = ci >= 0 ? a.diag_dn[ci] : (T)0;
+        }
+    }
+    // per-column sums of the wave sums sums[rd][16][slot] of the last phase, in wave order
+    auto column_sums = [&](double *out0, double *out1) {
+        __syncthreads();
+        if (lt == 0)
+            for (int rd = 0; rd < nround; ++rd) {
+                const int ci = cmap[rd * kRunCpw];
+                if (ci < 0) continue;
+                double t0 = 0.0, t1 = 0.0;
+                for (int q = 0; q < NT / 64; ++q) {
+                    t0 += sums[(rd * 16 + cg * (NT / 64) + q) * 2];
+                    t1 += sums[(rd * 16 + cg * (NT / 64) + q) * 2 + 1];
+                }
+                st_agent(out0 + ci, t0);
+                if (out1) st_agent(out1 + ci, t1);
+            }
+    };
+    // ---- u_1 = start (lz_init_body per column): ||u_1||^2 partials, <u_1, u_0> = 0
+    {
+        T *u1 = lz.slot(1);
+        for (int rd = 0; rd < nround; ++rd) {
+            const int ci = __builtin_amdgcn_readfirstlane(cmap[rd * kRunCpw]);
+            double nn = 0.0;
+            if (ci >= 0)
+                for (int r = lt; r < dup; r += NT) {
+                    const int64_t i = (int64_t)ci * dup + r;
+                    const T v = (T)P.start[i];
+                    st_agent(u1 + i, v);
+                    nn += (double)v * (double)v;
+                }
+            const double w0 = wave_sum(nn);
+            if ((tid & 63) == 0) {
+                sums[(rd * 16 + wv) * 2] = w0;
+                sums[(rd * 16 + wv) * 2 + 1] = 0.0;
+            }
+        }
+        column_sums(lz.nrm(1), lz.crs(1));
+        if (lead) {
+            st->j_k1 = 1;
+            st->j_k2 = 0;
+            st->status = 0;
+        }
+    }
+    unsigned nbar = 0;
+    if (grid_barrier(bar, ++nbar, spin_limit, flag)) return;
+    int j = 1;
+    bool live = true;
+    double bprev = 1.0;
+    for (int s = 0; s < nsteps; ++s) {
+        const T *__restrict__ x = lz.slot(j);
+        T *__restrict__ y = lz.slot(j + 1);
+        // ---- H.v of step j, round by round
+        if (live) {
+            {   // stage round 0
+                const RunCol<T, K> c = rec(0);
+                const int ci = __builtin_amdgcn_readfirstlane(c.hdr[0]);
+                for (int r = lt; r < dup; r += NT) colb[r] = ci >= 0 ? ld_agent(x + (size_t)ci * dup + r) : (T)0;
+                __syncthreads();
+            }
+            for (int rd = 0; rd < nround; ++rd) {
+                const RunCol<T, K> c = rec(rd);
+                const int ci = __builtin_amdgcn_readfirstlane(c.hdr[0]);
+                const bool act = ci >= 0;
+                const int base = act ? ci * dup : 0, cd = c.hdr[1], nd = __builtin_amdgcn_readfirstlane(c.hdr[2]);
+                const T ddg = *c.ddg;
+                const T *col = colb + (rd & 1) * dup;
+                // the next round's column of u_j, in flight during this round's gathers
+                T nx[R];
+                int nbase = -1;
+                if (rd + 1 < nround) nbase = __builtin_amdgcn_readfirstlane(rec(rd + 1).hdr[0]);
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int r = lt + i * NT;
+                    nx[i] = (nbase >= 0 && r < dup) ? ld_agent(x + (size_t)nbase * dup + r) : (T)0;
+                }
+                double dot = 0.0;
+#pragma unroll
+                for (int i0 = 0; i0 < R; i0 += RB) {
+                    T xd[RB][K], xh[RB][kPersistHund];
+                    int h0[RB], nht[RB];
+#pragma unroll
+                    for (int b = 0; b < RB; ++b) {
+                        const int r = lt + (i0 + b) * NT;
+                        const bool ok = act && r < dup;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) xd[b][k] = (ok && k < nd) ? ld_agent(x + c.jd[k] + r) : (T)0;
+                        h0[b] = 0;
+                        nht[b] = 0;
+                        if (hund && ok) {
+                            const int q = cup[r] * ncfg + cd;
+                            h0[b] = hptr[q];
+                            nht[b] = hptr[q + 1] - h0[b];
+                        }
+#pragma unroll
+                        for (int h = 0; h < kPersistHund; ++h) {
+                            xh[b][h] = (T)0;
+                            if (h < nht[b]) {
+                                const int tr = htr[h0[b] + h];
+                                xh[b][h] = ld_agent(x + (int)swp[r * ncfg + (tr & 0xFF)] + c.swdn[tr >> 8] * dup);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int b = 0; b < RB; ++b) {
+                        const int r = lt + (i0 + b) * NT;
+                        if (act && r < dup) {
+                            const T xv = col[r];
+                            T acc = ((du[r] + ddg) + uimp[cup[r] * ncfg + cd]) * xv;
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                if (k < kk) {
+                                    const uint32_t w = pk[k * dup + r];
+                                    acc += coef[w >> 16] * col[w & 0xFFFFu];
+                                }
+#pragma unroll
+                            for (int k = 0; k < K; ++k) acc += c.od[k] * xd[b][k];
+#pragma unroll
+                            for (int h = 0; h < kPersistHund; ++h)
+                                if (h < nht[b]) acc += hval[h0[b] + h] * xh[b][h];
+                            st_agent(y + base + r, acc);
+                            dot += (double)acc * (double)xv;
+                        }
+                    }
+                    if constexpr (RB < R) __builtin_amdgcn_sched_barrier(0);
+                }
+                const double w0 = wave_sum(dot);
+                if ((tid & 63) == 0) sums[(rd * 16 + wv) * 2] = w0;
+                if (rd + 1 < nround) {
+                    T *nb = colb + ((rd + 1) & 1) * dup;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const int r = lt + i * NT;
+                        if (r < dup) nb[r] = nx[i];
+                    }
+                }
+                __syncthreads();   // the next column staged; this one no longer read
+            }
+            column_sums(lz.partA, nullptr);
+        }
+        if (grid_barrier(bar, ++nbar, spin_limit, flag)) return;
+        // ---- scalars and update of step j (N, P, Q as lz_scalars over the run's columns)
+        if (live) {
+            if (wv == 0) {
+                const double t = wave_part_sum<true>(lz.nrm(j), ddn);
+                if (tid == 0) sh[0] = t;
+            } else if (wv == 1) {
+                const double t = j > 1 ? wave_part_sum<true>(lz.crs(j), ddn) : 0.0;
+                if (tid == 64) sh[1] = t;
+            } else if (wv == 2) {
+                const double t = wave_part_sum<true>(lz.partA, ddn);
+                if (tid == 128) sh[2] = t;
+            }
+            __syncthreads();
+            const double bj = sqrt(sh[0]);
+            const double q = sh[2], pp = sh[1];
+            if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+                if (lead) {
+                    st->beta[j - 1] = bj;
+                    st->status = (j == 1) ? 2 : 1;
+                }
+                live = false;
+            } else {
+                const double alpha = q / (bj * bj) - (j > 1 ? pp / bprev : 0.0);
+                if (lead) {
+                    st->beta[j - 1] = bj;
+                    st->alpha[j - 1] = alpha;
+                }
+                const double cz = 1.0 / bj, cp = j > 1 ? bj / bprev : 0.0, cx = alpha / bj;
+                const T *__restrict__ xm = lz.slot(j > 1 ? j - 1 : j);
+                for (int rd = 0; rd < nround; ++rd) {
+                    const int ci = __builtin_amdgcn_readfirstlane(rec(rd).hdr[0]);
+                    const bool act = ci >= 0;
+                    const int base = act ? ci * dup : 0;
+                    T zv[R], xv[R], pv[R];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const int r = lt + i * NT;
+                        const bool ok = act && r < dup;
+                        zv[i] = ok ? ld_agent(y + base + r) : (T)0;
+                        xv[i] = ok ? ld_agent(x + base + r) : (T)0;
+                        pv[i] = (ok && j > 1) ? ld_agent(xm + base + r) : (T)0;
+                    }
+                    double nn = 0.0, cr = 0.0;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const int r = lt + i * NT;
+                        if (act && r < dup) {
+                            const T v = (T)(cz * (double)zv[i] - cp * (double)pv[i] - cx * (double)xv[i]);
+                            st_agent(y + base + r, v);
+                            nn += (double)v * (double)v;
+                            cr += (double)v * (double)xv[i];
+                        }
+                    }
+                    const double w0 = wave_sum(nn), w1 = wave_sum(cr);
+                    if ((tid & 63) == 0) {
+                        sums[(rd * 16 + wv) * 2] = w0;
+                        sums[(rd * 16 + wv) * 2 + 1] = w1;
+                    }
+                }
+                column_sums(lz.nrm(j + 1), lz.crs(j + 1));
+                bprev = bj;
+            }
+        }
+        ++j;
+        if (lead && live) {
+            st->j_k1 = j;
+            st->j_k2 = j - 1;
+        }
+        if (s + 1 < nsteps && grid_barrier(bar, ++nbar, spin_limit, flag)) return;
+    }
+}
+
 }  // namespace ed

@@ -515,6 +515,12 @@ module ed_hip
       type(c_ptr), value :: sec
       integer(c_int), intent(out) :: refused
     end function
+    integer(c_int) function ed_dynamic_batched_stats(persistent_launches, refusals, last_ms, last_job_steps, &
+        last_bytes) bind(C, name="ed_dynamic_batched_stats")
+      import :: c_int, c_double
+      integer(c_int), intent(out) :: persistent_launches, refusals
+      real(c_double), intent(out) :: last_ms, last_job_steps, last_bytes
+    end function
     ! ed_dynamic without device allocations (static scratch; limits ED_NOALLOC_MAX_* of ed_hip.h)
     integer(c_int) function ed_dynamic_noalloc(sec, iph, energy, phi, nitergreen, cutoff_dynamic, precision, nw, wn, &
         dyn, a, b) bind(C, name="ed_dynamic_noalloc")

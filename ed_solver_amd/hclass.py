@@ -557,6 +557,18 @@ def dynamic_from_state(gs: Sector, psi: np.ndarray, energy: float, jobs, spin: i
     return d[..., 0] + 1j * d[..., 1]
 
 
+def batched_stats(last=False):
+    """(launches of the multi-run persistent kernel, launches the runtime refused) of the
+    batched excitation runs since the library was loaded; with last=True also a dict of the
+    last launch: its duration in ms (HIP events), job-steps and algorithmic bytes."""
+    n, r = ctypes.c_int(), ctypes.c_int()
+    v = [ctypes.c_double() for _ in range(3)]
+    check(lib().ed_dynamic_batched_stats(ctypes.byref(n), ctypes.byref(r), *[ctypes.byref(x) for x in v]))
+    if not last:
+        return n.value, r.value
+    return n.value, r.value, {"ms": v[0].value, "job_steps": v[1].value, "bytes": v[2].value}
+
+
 def compute_dynamic_batched(sectors, phis, iphs, energies, nitergreen: int, wn: np.ndarray,
                             cutoff_dynamic: float = 1e-8, precision="f64"):
     """All excitation runs in one call; returns dyn (njobs, nw) complex."""

@@ -506,7 +506,11 @@ int ed_dynamic_full_ed_c(int dim, int iph, double energy, const double *phi, int
  * per-(state, orbital, iph) calls of compute_dynamic made by
  * green_class_computeAA.h:101-302): njobs independent runs, job k in sector
  * secs[k] from the host vector phis[k] (length dimen of that sector), all advanced
- * together, one batched kernel per recurrence phase.  Per job: iph[k], energy[k];
+ * together.  Runs in real sectors with d_up <= 1024 and table width <= 12 go through
+ * ONE cooperative persistent launch for all of them (a table gives each run workgroups
+ * in proportion to its columns); the others, or all when the persistent kernel is
+ * switched off on a sector (ed_lanczos_set_persistent 0) or the runtime refuses the
+ * launch, take one batched kernel per recurrence phase.  Per job: iph[k], energy[k];
  * output dyn[k][2*nw], a[k][nitergreen], b[k][nitergreen]. */
 int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis,
                        const int *iph, const double *energy, int nitergreen,
@@ -529,6 +533,16 @@ int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int n
                           double *dyn);
 
 /* ----------------------------------------------------------- diagnostics */
+
+/* Launches of the multi-run persistent kernel of ed_dynamic_batched /
+ * ed_dynamic_from_state since the library was loaded, launches of it the runtime refused
+ * (those batches ran the two-launch path), and for the last launch its duration (a HIP
+ * event pair around it on the sector's stream), its job-steps (runs x steps) and its
+ * algorithmic HBM bytes (5 * sizeof(T) * dimension per job-step).  No reference
+ * counterpart: it lets a caller tell which path ran and measure it.  Any output may be
+ * NULL. */
+int ed_dynamic_batched_stats(int *persistent_launches, int *refusals, double *last_ms, double *last_job_steps,
+                             double *last_bytes);
 
 /* Starts an FP64 (ED_PREC_F64) or FP32 (ED_PREC_F32) Lanczos recurrence from the
  * default start vector and launches the fused H.v of its first step (z = H u_1 and

@@ -40,6 +40,14 @@ class BlockOpts(ctypes.Structure):
                 ("check_every", ctypes.c_int), ("tolerance", ctypes.c_double)]
 
 
+class ArpackOpts(ctypes.Structure):
+    _fields_ = [("nev", ctypes.c_int), ("ncv", ctypes.c_int), ("which", ctypes.c_int), ("tol", ctypes.c_double),
+                ("maxitr", ctypes.c_int)]
+
+
+ED_ARPACK_BE, ED_ARPACK_SA, ED_ARPACK_LA = 0, 1, 2
+
+
 class EDError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[ed_hip status {code}] {msg}")
@@ -95,6 +103,9 @@ SIGNATURES = {
     "ed_tridiag_lowest": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_band_eig": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_block_default_opts": (ctypes.c_int, [ctypes.POINTER(BlockOpts)]),
+    "ed_arpack_default_opts": (ctypes.c_int, [ctypes.POINTER(ArpackOpts)]),
+    "ed_arpack": (ctypes.c_int, [_vp, ctypes.POINTER(ArpackOpts), _dp, _dp, _dp, _dp, ctypes.c_int64, _c_int_p,
+                                 _c_int_p, _c_int_p]),
     "ed_block_lanczos": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _dp, ctypes.c_int64, _c_int_p,
                                         _c_int_p]),
     "ed_block_lanczos_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _vp, ctypes.c_int64, _c_int_p,
@@ -108,6 +119,7 @@ SIGNATURES = {
     "ed_bath_distance": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
     "ed_apply_c": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "ed_apply_c_n": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_dynamic_full_ed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp, _dp,
                                           ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "ed_dynamic_full_ed_c": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp, _dp,

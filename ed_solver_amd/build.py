@@ -16,7 +16,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libed_hip.so")
-SOURCES = ["ed_runtime.hip", "ed_tables.hip", "ed_engine.hip", "ed_green.hip", "ed_batch.hip", "ed_block.hip", "ed_grad.hip", "ed_noalloc.hip", "ed_tridiag.cpp"]
+SOURCES = ["ed_runtime.hip", "ed_tables.hip", "ed_engine.hip", "ed_green.hip", "ed_batch.hip", "ed_block.hip", "ed_arpack.hip", "ed_grad.hip", "ed_noalloc.hip", "ed_tridiag.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("ED_FLANG", "/opt/rocm/llvm/bin/amdflang")
 HIP_FLAGS = [

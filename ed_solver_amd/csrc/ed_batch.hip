@@ -763,7 +763,7 @@ extern "C" int ed_dynamic_batched(int njobs, ed_sector *const *secs, const doubl
 
 namespace ed {
 int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *d_psi,
-                double *d_phi, hipStream_t st);
+                double *d_phi, hipStream_t st, int ncond);
 }
 
 // y += s x
@@ -808,8 +808,16 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
         // sum_t coef_t c^(pm)_{site_t} psi (the vectors of symmetric_combineAA,
         // green_class_compute_symmetric_combination.f90:39-137)
         for (int t = 0; t < nterm[k] && rc == ED_OK; ++t) {
-            // S_z sectors: sites[] are Nambu orbitals spin * nc + site (apply_Csz), `spin` unused
+            // S_z sectors: sites[] are Nambu orbitals spin * nc + site (apply_Csz), `spin`
+            // unused; ED_SITE_NCOND on a Nambu orbital asks for the correlated excitation of
+            // apply_Csz_N_updo (apply_C.f90:404-483)
             int site = sites[t0 + t], tspin = spin;
+            const int ncond = (site & ED_SITE_NCOND) ? 1 : 0;
+            site &= ~ED_SITE_NCOND;
+            if (ncond && !gs->sz) {
+                rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: the correlated excitation exists for S_z sectors only");
+                break;
+            }
             if (gs->sz) {
                 if (site < 0 || site >= 2 * gs->nc) {
                     rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: Nambu orbital outside [0, 2 nc)");
@@ -820,9 +828,9 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
             }
             const double c = coefs[t0 + t];
             if (t == 0 && c == 1.0) {
-                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, phi, st);
+                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, phi, st, ncond);
             } else {
-                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, d_tmp, st);
+                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, d_tmp, st, ncond);
                 if (rc == ED_OK) {
                     if (t == 0) ED_HIP_CHECK(hipMemsetAsync(phi, 0, sizeof(double) * dst[k]->dim, st));
                     hipLaunchKernelGGL(k_axpy, dim3(egrid(dst[k]->dim)), dim3(kBlock), 0, st, c, (const double *)d_tmp,

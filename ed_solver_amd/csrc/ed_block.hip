@@ -293,6 +293,27 @@ static int xc(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, co
     return ED_OK;
 }
 
+// The contractions for other drivers (the implicitly restarted Lanczos of ed_arpack):
+// reserve the partial-sum work space for Gram products of up to pmax columns over n rows
+// before queueing several of them, then queue them without host waits.
+int block_ws_reserve(int dev, int64_t n, int pmax)
+{
+    BlockWS *w;
+    return bws_get(dev, gram_part_size(n, pmax), 256, w);
+}
+
+int block_gram_dev(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                   double *d_G, hipStream_t st)
+{
+    return gram_dev(dev, n, p, q, X, ldx, Y, ldy, d_G, st);
+}
+
+void block_xc_dev(int64_t n, int p, int q, const double *X, int64_t ldx, const double *d_C, double alpha, double beta,
+                  double *Y, int64_t ldy, hipStream_t st)
+{
+    xc_dev(n, p, q, X, ldx, d_C, alpha, beta, Y, ldy, st);
+}
+
 // Upper-triangular R with G = R^T R (G b x b row-major, symmetric); false when G is not
 // numerically positive definite relative to its largest diagonal entry.
 static bool cholesky_upper(int b, const std::vector<double> &G, std::vector<double> &R)

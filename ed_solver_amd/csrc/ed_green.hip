@@ -27,15 +27,18 @@ int lanczos_run_dev(ed_sector *s, int niter, const double *d_start, int prec, do
 // For every destination spin state t: src[t] = source rank (or -1), sg[t] = operator sign.
 // c+ (pm > 0): t must hold `bit`, source = t without it; c (pm < 0): t must not hold it,
 // source = t with it.  The sign counts occupied orbitals above `bit` in t
-// (Fermion_sign, quantum_fermion_ket_class.f90:605-618).
+// (Fermion_sign, quantum_fermion_ket_class.f90:605-618).  With cbit >= 0 the operator is
+// conditioned on orbital cbit holding cval (0 or 1) in the source state, which is its
+// value in t because cbit != bit: the correlated excitation of apply_Csz_N_updo
+// (apply_C.f90:404-483).
 __global__ void k_cmap(const int32_t *dst_states, int d, int bit, int pm, const int32_t *rank, int32_t *src,
-                       int32_t *sg)
+                       int32_t *sg, int cbit, int cval)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d) return;
     const uint32_t s = (uint32_t)dst_states[t];
     const bool has = (s >> bit) & 1u;
-    if ((pm > 0) != has) {
+    if ((pm > 0) != has || (cbit >= 0 && (int)((s >> cbit) & 1u) != cval)) {
         src[t] = -1;
         sg[t] = 0;
         return;
@@ -133,9 +136,11 @@ static int check_pair(const ed_sector *src, const ed_sector *dst, int site, int 
 }
 
 int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *d_psi,
-                double *d_phi, hipStream_t st)
+                double *d_phi, hipStream_t st, int ncond)
 {
     ED_TRY(check_pair(src, dst, site, spin, pm));
+    if (ncond && !src->sz)   // apply_N_Cup / apply_N_Cdo STOP there (apply_C.f90:134-145,168-179)
+        return fail(ED_ERR_ARG, "ed_apply_c_n: the correlated excitation exists for S_z sectors only");
     // An S_z sector is one string of 2 Ns Nambu orbitals held as the "up" table (ddn = 1):
     // the operator creates (pm > 0) or destroys the Nambu orbital of (site, spin), bit
     // spin Ns + nb + site (apply_Csz, apply_C.f90:277-327).
@@ -144,8 +149,13 @@ int apply_c_dev(const ed_sector *src, const ed_sector *dst, int site, int spin, 
     const SpinTables &T = kspin == 0 ? dst->up : dst->dn;
     int32_t *d_map = nullptr;
     ED_HIP_CHECK(hipMallocAsync((void **)&d_map, sizeof(int32_t) * 2 * std::max(1, T.d), st));
+    // correlated excitation (apply_Csz_N_updo): the Nambu up orbital of the site acts only
+    // where the Nambu down orbital is empty (NDN == 0: a physical down electron), the
+    // Nambu down orbital only where the Nambu up orbital is occupied (NUP == 1)
+    const int cbit = ncond ? (spin == 0 ? src->ns + src->nb + site : src->nb + site) : -1;
+    const int cval = spin == 0 ? 0 : 1;
     hipLaunchKernelGGL(k_cmap, dim3((T.d + kBlock - 1) / kBlock), dim3(kBlock), 0, st, T.states, T.d,
-                       bit, pm, src->d_rank, d_map, d_map + T.d);
+                       bit, pm, src->d_rank, d_map, d_map + T.d, cbit, cval);
     // a complex sector's planar vectors: the same real map on the Re and the Im plane
     const int planes = dst->cplx ? 2 : 1;
     const int64_t ns = dst->cplx ? src->cdim : src->dim, nd = dst->cplx ? dst->cdim : dst->dim;
@@ -281,7 +291,8 @@ using namespace ed;
 
 extern "C" {
 
-int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *psi, double *phi)
+static int apply_c_host(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *psi,
+                        double *phi, int ncond)
 {
     ED_TRY(check_pair(src, dst, site, spin, pm));
     if (!psi || !phi) return fail(ED_ERR_ARG, "ed_apply_c: NULL vector");
@@ -291,12 +302,23 @@ int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, i
     ED_HIP_CHECK(hipMallocAsync((void **)&d_psi, sizeof(double) * src->dim, st));
     ED_HIP_CHECK(hipMallocAsync((void **)&d_phi, sizeof(double) * dst->dim, st));
     ED_HIP_CHECK(hipMemcpyAsync(d_psi, psi, sizeof(double) * src->dim, hipMemcpyHostToDevice, st));
-    int rc = apply_c_dev(src, dst, site, spin, pm, d_psi, d_phi, st);
+    int rc = apply_c_dev(src, dst, site, spin, pm, d_psi, d_phi, st, ncond);
     if (rc == ED_OK) ED_HIP_CHECK(hipMemcpyAsync(phi, d_phi, sizeof(double) * dst->dim, hipMemcpyDeviceToHost, st));
     (void)hipFreeAsync(d_psi, st);
     (void)hipFreeAsync(d_phi, st);
     ED_HIP_CHECK(hipStreamSynchronize(st));
     return rc;
+}
+
+int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *psi, double *phi)
+{
+    return apply_c_host(src, dst, site, spin, pm, psi, phi, 0);
+}
+
+int ed_apply_c_n(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm, const double *psi,
+                 double *phi)
+{
+    return apply_c_host(src, dst, site, spin, pm, psi, phi, 1);
 }
 
 }  // extern "C"

@@ -311,4 +311,13 @@ int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_sta
                    double *subdiag);
 // s->col_bfs: the sector's down columns in the breadth-first order of its down-hop graph
 int ensure_col_bfs(ed_sector *s);
+// FP64 MFMA contractions of ed_block.hip on device matrices (column-major, 16-byte aligned,
+// even leading dimensions), queued on st: d_G (row-major p x q) = X^T Y and
+// Y = beta Y + alpha X C (d_C row-major p x q), q <= 16; block_ws_reserve sizes the Gram
+// work space for up to pmax columns so that queued products never reallocate it.
+int block_ws_reserve(int dev, int64_t n, int pmax);
+int block_gram_dev(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                   double *d_G, hipStream_t st);
+void block_xc_dev(int64_t n, int p, int q, const double *X, int64_t ldx, const double *d_C, double alpha, double beta,
+                  double *Y, int64_t ldy, hipStream_t st);
 }  // namespace ed

@@ -26,6 +26,14 @@ module ed_hip
     real(c_double)     :: jhund, cutoff
   end type
 
+  ! ed_arpack_opts (arpack_eigenvector_sym_matrix parameters, ED_ARPACK.f90:35-250)
+  integer(c_int), parameter :: ED_ARPACK_BE = 0, ED_ARPACK_SA = 1, ED_ARPACK_LA = 2
+  type, bind(C) :: ed_arpack_opts_t
+    integer(c_int) :: nev, ncv, which
+    real(c_double) :: tol
+    integer(c_int) :: maxitr
+  end type
+
   ! ed_block_opts (Block_Lanczos_diagonalize parameters)
   type, bind(C) :: ed_block_opts_t
     integer(c_int) :: block_size, nev, max_steps, check_every
@@ -141,6 +149,20 @@ module ed_hip
       real(c_double)                    :: evals(*), resid(*), evecs(*)
       integer(c_int64_t), value         :: ldv
       integer(c_int), intent(out)       :: nconv, steps
+    end function
+    integer(c_int) function ed_arpack_default_opts(opts) bind(C, name="ed_arpack_default_opts")
+      import :: c_int, ed_arpack_opts_t
+      type(ed_arpack_opts_t), intent(out) :: opts
+    end function
+    ! ARPACK_diago on the device; start may be c_null_ptr (create_fix_initial_vector's vector)
+    integer(c_int) function ed_arpack(sec, opts, start, evals, resid, evecs, ldv, nconv, iters, matvecs) &
+        bind(C, name="ed_arpack")
+      import :: c_int, c_int64_t, c_ptr, c_double, ed_arpack_opts_t
+      type(c_ptr), value                 :: sec, start
+      type(ed_arpack_opts_t), intent(in) :: opts
+      real(c_double)                     :: evals(*), resid(*), evecs(*)
+      integer(c_int64_t), value          :: ldv
+      integer(c_int), intent(out)        :: nconv, iters, matvecs
     end function
     integer(c_int) function ed_hf_gradient(sec, n, psi, d_level, d_v, d_u) bind(C, name="ed_hf_gradient")
       import :: c_int, c_int64_t, c_ptr, c_double
@@ -401,6 +423,17 @@ module ed_hip
       real(c_double), intent(out) :: d_u
     end function
     integer(c_int) function ed_apply_c(src, dst, site, spin, pm, psi, phi) bind(C, name="ed_apply_c")
+      import :: c_double, c_int, c_ptr
+      type(c_ptr), value :: src
+      type(c_ptr), value :: dst
+      integer(c_int), value :: site
+      integer(c_int), value :: spin
+      integer(c_int), value :: pm
+      real(c_double) :: psi(*)
+      real(c_double) :: phi(*)
+    end function
+    ! apply_Csz_N_updo (apply_C.f90:404-483): the correlated excitation on S_z sectors
+    integer(c_int) function ed_apply_c_n(src, dst, site, spin, pm, psi, phi) bind(C, name="ed_apply_c_n")
       import :: c_double, c_int, c_ptr
       type(c_ptr), value :: src
       type(c_ptr), value :: dst

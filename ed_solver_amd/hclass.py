@@ -376,6 +376,23 @@ def apply_C(src: Sector, dst: Sector, site: int, spin: int, pm: int, psi: np.nda
     return dst._unpack(phi)
 
 
+# flag or-ed into a Nambu orbital of dynamic_from_state's terms: the correlated
+# excitation of apply_C_n for that term (ED_SITE_NCOND of the C-ABI)
+SITE_NCOND = 0x10000
+
+
+def apply_C_n(src: "SzSector", dst: "SzSector", site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
+    """apply_Csz_N_updo (apply_C.f90:404-483): apply_C on an S_z sector restricted by the
+    other Nambu orbital of the site, the excitation of the correlated-hopping (FLAG_NCUP)
+    Green's functions.  spin 0: the Nambu up orbital where the Nambu down orbital is empty
+    (a physical down electron: n_dn c_up, c+_up n_dn); spin 1: the Nambu down orbital where
+    the Nambu up orbital is occupied (n_up c+_dn, n_up c_dn).  pm as in apply_C."""
+    psi = src._pack(psi, "apply_C_n: psi")
+    phi = np.zeros(dst.vlen)
+    check(lib().ed_apply_c_n(src.handle, dst.handle, site, spin, pm, dptr(psi), dptr(phi)))
+    return dst._unpack(phi)
+
+
 def matsubara(beta: float, nw: int) -> np.ndarray:
     """w_n = (2n-1) pi / beta, n = 1..nw (frequency_axis.f90:165-170)."""
     n = np.arange(1, nw + 1, dtype=np.float64)

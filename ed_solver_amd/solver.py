@@ -23,7 +23,7 @@ from math import comb
 import numpy as np
 
 from .aim import AIMParams
-from .hclass import (ComplexSector, Sector, apply_C, compute_dynamic_batched, dynamic_from_state,
+from .hclass import (ComplexSector, Sector, apply_C, apply_C_n, compute_dynamic_batched, dynamic_from_state,
                      dynamic_full_ed)
 
 FULL_ED_SWITCH = 40   # Global_Variables.f90:23
@@ -149,10 +149,12 @@ def partition(states: list[EigenState], beta: float) -> np.ndarray:
 
 def _excite(src: Sector, dst: Sector, op, pm: int, psi: np.ndarray) -> np.ndarray:
     """A+|psi> (pm = +1) or A|psi> (pm = -1) for A = sum coef c_(site, spin) over the
-    entries (site, spin, coef) of op: the creation side carries conj(coef)."""
+    entries (site, spin, coef[, correlated]) of op: the creation side carries conj(coef); an
+    entry marked correlated takes the excitation of apply_Csz_N_updo (apply_C_n)."""
     phi = None
-    for c, sp, coef in op:
-        t = (np.conj(coef) if pm > 0 else coef) * apply_C(src, dst, c, sp, pm, psi)
+    for c, sp, coef, *corr in op:
+        fn = apply_C_n if corr and corr[0] else apply_C
+        t = (np.conj(coef) if pm > 0 else coef) * fn(src, dst, c, sp, pm, psi)
         phi = t if phi is None else phi + t
     return phi
 
@@ -342,18 +344,56 @@ def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float,
     n = len(nambu)
     pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
     ops = [[(c, sp, 1.0)] for c, sp in nambu] + [[(*nambu[a], 1.0), (*nambu[b], 1.0)] for a, b in pairs]
-
-    def target(key, sp, pm):
-        t = key[0] + pm
-        return (t,) if 0 <= t <= 2 * model.ns else None
-
-    nc = model.nc
     g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
-                   lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath), target,
-                   lambda op: (0, [(sp * nc + c, coef) for c, sp, coef in op]))
+                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath))
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]
     for k, (a, b) in enumerate(pairs):
         G[a, b] = G[b, a] = 0.5 * (g[n + k] - g[a] - g[b])
     return G
+
+
+def _sz_ops(model: AIMParams, pb, pvbc, pairing_imp_to_bath):
+    """make / target / device_terms of _green_ops for the S_z sectors: every Nambu operator
+    moves npart by one; device terms are Nambu orbitals spin * nc + site, with SITE_NCOND
+    on the correlated ones."""
+    from .hclass import SITE_NCOND, SzSector
+
+    def target(key, sp, pm):
+        t = key[0] + pm
+        return (t,) if 0 <= t <= 2 * model.ns else None
+
+    nc = model.nc
+
+    def terms(op):
+        return 0, [((sp * nc + c) | (SITE_NCOND if corr and corr[0] else 0), coef) for c, sp, coef, *corr in op]
+
+    return (lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath)), target, terms
+
+
+def compute_green_ncup(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, pb=None,
+                       pvbc=None, pairing_imp_to_bath: bool = False, nitergreen: int = 36, orbitals=None,
+                       weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8, precision="f64",
+                       full_ed_switch: int = FULL_ED_SWITCH):
+    """The correlated-hopping Green's functions of FLAG_NCUP (correlations_dyn_stat.h:69-100)
+    on the S_z sectors, per Nambu impurity orbital a = (c, s) (index s n + c over `orbitals`):
+    * GN[a] = <<D_a ; D_a+>>: compute_greenAA with apply_N_Cup / apply_N_Cdo, the operator
+      D_(c, 0) = n_dn c_up and D_(c, 1) = n_up c+_dn (apply_Csz_N_updo, apply_C.f90:404-483);
+    * GNF[a] = <<D_a ; d_a+>>: compute_greenAB with apply_N_Cup and apply_Cup
+      (green_class_computeAB.f90:23-197), d_(c, 0) = c_up, d_(c, 1) = c+_dn.  For a real H
+      <<D ; d+>> = <<d ; D+>>, and it comes from the symmetric combination D + d as the
+      reference's real build does: GNF = (G_(D+d) - GN - G) / 2.
+    Returns (GN, GNF, G), each of shape (2n, nw), G the diagonal Nambu G^N of the same runs.
+    States from get_eigen_energies_sz with the same pairing; a state's runs are built on the
+    device and batched (ed_dynamic_from_state with ED_SITE_NCOND terms)."""
+    orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
+    nambu = [(c, 0) for c in orbitals] + [(c, 1) for c in orbitals]
+    n = len(nambu)
+    ops = ([[(c, sp, 1.0, True)] for c, sp in nambu] + [[(c, sp, 1.0)] for c, sp in nambu]
+           + [[(c, sp, 1.0, True), (c, sp, 1.0)] for c, sp in nambu])
+    g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
+                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath))
+    GN, G = g[:n], g[n:2 * n]
+    GNF = 0.5 * (g[2 * n:] - GN - G)
+    return GN, GNF, G

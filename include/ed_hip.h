@@ -401,6 +401,36 @@ int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, d
 int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *d_evecs,
                          int64_t ldv, int *nconv, int *steps);
 
+/* ARPACK_diago (ED_ARPACK.f90:475-570) through arpack_eigenvector_sym_matrix
+ * (ED_ARPACK.f90:35-250): the implicitly restarted Lanczos method of dsaupd / dseupd
+ * (bmat = 'I', mode 1, exact shifts) with the basis, the reorthogonalisation and the
+ * restart rotation on the device; only the ncv x ncv tridiagonal work runs on the host.
+ * The reference passes which = 'BE', nev = Neigen_arpack - 1, ncv = Neigen_arpack,
+ * tol = tolerance, maxitr = 300 (ED_ARPACK_BE keeps nev / 2 Ritz values from the low end
+ * of the spectrum and the rest from the high end, as dsgets; _SA the nev smallest; _LA the
+ * nev largest).  Outputs: evals[nconv] ascending (the converged wanted Ritz values, d(:,1)
+ * of dseupd), resid[nconv] = ||H x - E x|| / |E| (d(:,2), may be NULL), evecs = host
+ * nconv x dimen_H() with leading dimension ldv (VECP, may be NULL), *nconv = iparam(5),
+ * *iters = implicit restarts + 1 (iparam(3)), *matvecs = H.v applications (iparam(9)).
+ * Fewer than nev converged within maxitr is not an error (info = 1 of dsaupd): *nconv
+ * says how many.  Real sectors ((N_up, N_dn) and S_z); ED_ERR_ARG unless
+ * 1 <= nev < ncv <= min(dimen_H(), 1024). */
+#define ED_ARPACK_BE 0
+#define ED_ARPACK_SA 1
+#define ED_ARPACK_LA 2
+typedef struct ed_arpack_opts {
+    int nev;          /* wanted Ritz values (NEV)                  */
+    int ncv;          /* basis length (NCV)                         */
+    int which;        /* ED_ARPACK_BE / _SA / _LA                   */
+    double tol;       /* relative accuracy of the Ritz values (TOL) */
+    int maxitr;       /* implicit restarts allowed (iparam(3))      */
+} ed_arpack_opts;
+
+int ed_arpack_default_opts(ed_arpack_opts *opts);   /* 8, 9, BE, 1e-12, 300 */
+
+int ed_arpack(ed_sector *sec, const ed_arpack_opts *opts, const double *start, double *evals, double *resid,
+              double *evecs, int64_t ldv, int *nconv, int *iters, int *matvecs);
+
 /* The contractions of the block orthogonalisation, on device matrices (column-major):
  *   ed_gram_f64_dev:  g (host, row-major p x q) = X^T Y, X n x p, Y n x q, q <= 16
  *                     (16-byte aligned pointers, even leading dimensions);
@@ -452,6 +482,18 @@ int ed_bath_distance(int nc, int nb, const double *eps, const double *vbc, int n
  * c_dn (the reference's apply_Cdo with '-'), pm = -1 the physical c+_dn. */
 int ed_apply_c(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm,
                const double *psi, double *phi);
+
+/* apply_Csz_N_updo (apply_C.f90:404-483; apply_N_Cup / apply_N_Cdo :134-145,168-179),
+ * the correlated-hopping excitation of the FLAG_NCUP Green's functions: ed_apply_c on an
+ * S_z sector restricted to the states whose other Nambu orbital of the same site holds
+ * the reference's condition.  spin 0 acts on the Nambu up orbital where the Nambu down
+ * orbital is empty (NDN == 0, a physical down electron: c+_up n_dn and n_dn c_up); spin 1
+ * acts on the Nambu down orbital where the Nambu up orbital is occupied (NUP == 1).  pm
+ * creates or destroys the Nambu orbital as in ed_apply_c (the reference's spin-2 '+' is
+ * pm = -1 here).  (N_up, N_dn) sectors give ED_ERR_ARG, where the reference STOPs.
+ * Host arrays. */
+int ed_apply_c_n(const ed_sector *src, const ed_sector *dst, int site, int spin, int pm,
+                 const double *psi, double *phi);
 
 /* invert_zmtridiag (matrices_tridiag.f90:163-198) for nw points z = i*wn[k]:
  * out[2k], out[2k+1] = Re, Im of <0|(z - T)^-1|0>; T = (a[0..n-1], b[1..n-1]).
@@ -525,7 +567,10 @@ int ed_dynamic_batched(int njobs, ed_sector *const *secs, const double *const *p
  * symmetric_combineAA, green_class_compute_symmetric_combination.f90:39-137), and all
  * runs advance together as in ed_dynamic_batched from energy = E_psi.  Output
  * dyn[k][2*nw].  Real (N_up, N_dn) and S_z sectors; for S_z sectors sites[t] is the
- * Nambu orbital spin * nc + site of apply_Csz (see ed_apply_c) and `spin` is unused. */
+ * Nambu orbital spin * nc + site of apply_Csz (see ed_apply_c) and `spin` is unused;
+ * ED_SITE_NCOND or-ed into an S_z entry takes the correlated excitation of ed_apply_c_n
+ * for that term. */
+#define ED_SITE_NCOND 0x10000
 int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs,
                           ed_sector *const *dst, const int *iph, const int *nterm,
                           const int *sites, const double *coefs, int spin, int nitergreen,

@@ -308,6 +308,93 @@ def apply_c_sz(model: AIMParams, npart: int, site: int, spin: int, pm: int, psi:
     return phi
 
 
+def apply_c_n_sz(model: AIMParams, npart: int, site: int, spin: int, pm: int, psi: np.ndarray) -> np.ndarray:
+    """apply_Csz_N_updo (ed_solver/apply_C.f90:404-483): apply_c_sz over the source states
+    that pass the reference's test on the other Nambu orbital of the site (goforit):
+    spin 0 (Nambu up acted on) needs NDN = 0, the Nambu down orbital empty; spin 1 (Nambu
+    down acted on) needs NUP = 1, the Nambu up orbital occupied."""
+    ibit, _ = orbital_bits(model)
+    nc = model.nc
+    other = ibit[(1 - spin) * nc + site]
+    need = 0 if spin == 0 else 1
+    src = sz_states(model.ns, npart)
+    keep = np.array([((int(s) >> other) & 1) == need for s in src])
+    return apply_c_sz(model, npart, site, spin, pm, np.where(keep, psi, 0.0))
+
+
+def ncup_green_restated(model: AIMParams, beta: float, wn: np.ndarray, pb=None, pvbc=None,
+                        pairing_imp_to_bath=False):
+    """The correlated-hopping Green's functions of FLAG_NCUP (correlations_dyn_stat.h:69-100)
+    from the restated S_z sectors, per Nambu impurity orbital a = (c, s) (index s nc + c):
+    GN[a] = <<D_a ; D_a+>> (compute_greenAA with apply_N_Cup / apply_N_Cdo) and
+    GNF[a] = <<D_a ; d_a+>> (compute_greenAB with apply_N_Cup, apply_Cup), where d_a+ is
+    apply_c_sz and D_a+ apply_c_n_sz; every npart sector densely diagonalised."""
+    ns, nc = model.ns, model.nc
+    spec = {}
+    for npart in range(2 * ns + 1):
+        t = tab_sz(model, npart, pb, pvbc, pairing_imp_to_bath)
+        spec[npart] = np.linalg.eigh(dense_sz(t, hund_sz(model, npart)))
+    e0 = min(v[0].min() for v in spec.values())
+    Z = sum(np.exp(-beta * (v[0] - e0)).sum() for v in spec.values())
+    n = 2 * nc
+    GN = np.zeros((n, len(wn)), complex)
+    GNF = np.zeros((n, len(wn)), complex)
+    for npart in range(2 * ns):
+        em, vm = spec[npart]
+        en, vn = spec[npart + 1]
+        W = np.exp(-beta * (em[None, :] - e0)) + np.exp(-beta * (en[:, None] - e0))
+        den = 1j * wn[None, None, :] - (en[:, None] - em[None, :])[:, :, None]
+        for a in range(n):
+            c, sp_ = a % nc, a // nc
+            Xd = vn.T @ np.array([apply_c_sz(model, npart, c, sp_, +1, vm[:, m]) for m in range(len(em))]).T
+            XD = vn.T @ np.array([apply_c_n_sz(model, npart, c, sp_, +1, vm[:, m]) for m in range(len(em))]).T
+            GN[a] += ((XD * XD * W)[:, :, None] / den).sum((0, 1))
+            GNF[a] += ((XD * Xd * W)[:, :, None] / den).sum((0, 1))
+    return GN / Z, GNF / Z
+
+
+def ncup_green_physical(model: AIMParams, beta: float, wn: np.ndarray, pb=None, pvbc=None,
+                        pairing_imp_to_bath=False):
+    """GN and GNF of ncup_green_restated from the physical Hamiltonian with pairing on the
+    whole Fock space (operator strings, no Nambu transformation): for orbital c,
+    d_(c, 0) = c_up, D_(c, 0) = n_dn c_up (the reference's n_dn C_up of FLAG_NCUP), and
+    d_(c, 1) = c+_dn, D_(c, 1) = n_up c+_dn; GN = <<D ; D+>>, GNF = <<D ; d+>>."""
+    ns, nb, nc = model.ns, model.nb, model.nc
+    allst = np.arange(1 << (2 * ns), dtype=np.int64)
+    N = len(allst)
+    H = np.zeros((N, N))
+    for coef, ops in physical_terms(model, pb, pvbc, pairing_imp_to_bath):
+        out, sg, ok = bf.apply_ops(allst, ops)
+        np.add.at(H, (out[ok], allst[ok]), coef * sg[ok])
+    E, V = np.linalg.eigh(H)
+    e0 = E.min()
+    Z = np.exp(-beta * (E - e0)).sum()
+
+    def mat(ops):
+        M = np.zeros((N, N))
+        out, sg, ok = bf.apply_ops(allst, ops)
+        M[out[ok], allst[ok]] = sg[ok]
+        return V.T @ M @ V   # <n|op|m>
+
+    W = np.exp(-beta * (E[None, :] - e0)) + np.exp(-beta * (E[:, None] - e0))
+    den = 1j * wn[None, None, :] - (E[:, None] - E[None, :])[:, :, None]
+    GN = np.zeros((2 * nc, len(wn)), complex)
+    GNF = np.zeros((2 * nc, len(wn)), complex)
+    for a in range(2 * nc):
+        c = a % nc
+        up, dn = nb + c, ns + nb + c
+        if a < nc:   # d+ = c+_up, D+ = c+_up n_dn
+            Xd = mat([(up, True)])
+            XD = mat([(up, True), (dn, True), (dn, False)])
+        else:        # d+ = c_dn, D+ = c_dn n_up
+            Xd = mat([(dn, False)])
+            XD = mat([(dn, False), (up, True), (up, False)])
+        for G, M in ((GN, XD * XD * W), (GNF, XD * Xd * W)):
+            nz = np.abs(M) > 0
+            G[a] = (M[nz][:, None] / den[nz]).sum(0)
+    return GN / Z, GNF / Z
+
+
 def nambu_green_restated(model: AIMParams, beta: float, wn: np.ndarray, pb=None, pvbc=None,
                          pairing_imp_to_bath=False) -> np.ndarray:
     """G^N_ij(i w_n) = <<d_i ; d_j+>> over the Nambu impurity orbitals d_(c, s) (index

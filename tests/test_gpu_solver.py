@@ -72,11 +72,13 @@ def test_thermal_green_function_vs_lehmann(gpu):
     assert np.max(np.abs(G - Gref)) < 1e-9
 
 
+@pytest.mark.parametrize("backend", ["device", "scipy"])
 @pytest.mark.parametrize("name,neigen_arpack,ncv", [("A", 9, None), ("B", 7, 24)])
-def test_arpack_diago_with_device_hmult(gpu, name, neigen_arpack, ncv):
-    """ARPACK_diago mirror: ARPACK (which='BE', nev = Neigen_arpack - 1, maxitr 300; ncv =
-    Neigen_arpack as in the reference, or larger) on the device H.v; the low end against
-    LAPACK / ARPACK on the oracle H.v."""
+def test_arpack_diago_with_device_hmult(gpu, name, neigen_arpack, ncv, backend):
+    """ARPACK_diago (which='BE', nev = Neigen_arpack - 1, maxitr 300; ncv = Neigen_arpack as
+    in the reference, or larger): the implicitly restarted Lanczos on the device (ed_arpack)
+    and SciPy's ARPACK over the device H.v; the low end against LAPACK / ARPACK on the
+    oracle H.v, residuals, orthonormal Ritz vectors."""
     import json
     import os
     import oracle
@@ -86,15 +88,56 @@ def test_arpack_diago_with_device_hmult(gpu, name, neigen_arpack, ncv):
         d = json.load(f)[name]
     m, sec = AIMParams.from_json(d["model"]), tuple(d["sector"])
     s = gpu.Sector(m, *sec)
-    r = arpack_diago(s, neigen_arpack, tolerance=1e-12, ncv=ncv)
+    r = arpack_diago(s, neigen_arpack, tolerance=1e-12, ncv=ncv, backend=backend)
     o = oracle.OracleSector(m, *sec, nthreads=16)
     ref, _ = oracle.lowest_eigs(o, 3)
     o.close()
-    print(f"{name}: nconv {r.nconv} matvecs {r.matvecs} E {r.evals[:3]} rel resid {r.rel_resid.max():.1e}", flush=True)
+    print(f"{name} {backend}: nconv {r.nconv} matvecs {r.matvecs} restarts {r.iters} E {r.evals[:3]} "
+          f"rel resid {r.rel_resid.max():.1e}", flush=True)
     assert r.nconv >= 2
     assert abs(r.evals[0] - ref[0]) < 1e-10
     assert np.max(r.rel_resid) < 1e-8
     assert np.max(np.abs(r.vectors @ r.vectors.T - np.eye(r.nconv))) < 1e-8
+    s.close()
+
+
+@pytest.mark.parametrize("name,which,nev,ncv", [("A", "BE", 8, 20), ("B", "SA", 6, 24), ("C", "SA", 4, 24)])
+def test_arpack_device_both_ends(gpu, name, which, nev, ncv):
+    """ed_arpack against library eigenvalues: 'BE' keeps nev / 2 Ritz values from each end of
+    the spectrum (dsgets), checked against LAPACK's lowest and highest of the dense restated
+    H; 'SA' the nev lowest against LAPACK / ARPACK on the oracle H.v (workload C: dimension
+    853,776, 12 eigenvalues of the oracle by eigsh).  Residuals ||H x - E x|| / |E| < 1e-8,
+    Ritz vectors orthonormal, and the eigenvectors satisfy H x = E x with the oracle's H.v."""
+    import json
+    import os
+    import oracle
+    from ed_solver_amd.aim import AIMParams
+    from ed_solver_amd.arpack import arpack_diago
+    if name == "C":
+        m, sec = gpu.synthetic("C")
+    else:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")) as f:
+            d = json.load(f)[name]
+        m, sec = AIMParams.from_json(d["model"]), tuple(d["sector"])
+    s = gpu.Sector(m, *sec)
+    print(f"ed_arpack {name} which={which} nev={nev} ncv={ncv} dim {s.dim}", flush=True)
+    r = arpack_diago(s, nev + 1, tolerance=1e-12, ncv=ncv, which=which)
+    o = oracle.OracleSector(m, *sec, nthreads=16)
+    print(f"  nconv {r.nconv} restarts {r.iters} matvecs {r.matvecs} E {r.evals}", flush=True)
+    assert r.nconv == nev
+    if which == "BE":
+        H = np.stack([o.hmult(e) for e in np.eye(o.dim)], axis=1)
+        w = np.linalg.eigvalsh(0.5 * (H + H.T))
+        ref = np.concatenate([w[:nev // 2], w[-(nev - nev // 2):]])
+    else:
+        ref, _ = oracle.lowest_eigs(o, nev)
+    assert np.max(np.abs(r.evals - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(r.rel_resid) < 1e-8
+    assert np.max(np.abs(r.vectors @ r.vectors.T - np.eye(r.nconv))) < 1e-8
+    for k in range(r.nconv):
+        x = r.vectors[k]
+        assert np.linalg.norm(o.hmult(x) - r.evals[k] * x) < 1e-7 * abs(r.evals[k])
+    o.close()
     s.close()
 
 

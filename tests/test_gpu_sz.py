@@ -292,3 +292,56 @@ def test_sz_nambu_green_vs_physical(gpu, case):
     assert np.max(np.abs(ref[:nc, nc:])) > 1e-3   # the anomalous part is there
     assert np.max(np.abs(G - ref)) < 1e-10
     assert np.max(np.abs(G_cf - ref)) < 1e-9
+
+
+def test_sz_apply_c_n_vs_restatement(gpu):
+    """ed_apply_c_n (apply_Csz_N_updo) on every Nambu impurity orbital, both signs, against
+    the restatement oracle/sz.py apply_c_n_sz: bit-exact; (N_up, N_dn) sectors are refused
+    as the reference STOPs."""
+    m, pb, pvbc = sc_model(2, 2, 2.0, seed=11, u_offdiag=1.0)
+    npart = m.ns
+    src = gpu.SzSector(m, npart, pb, pvbc, True)
+    res = src.lanczos(nitermax=60, neigen=1, tolerance=1e-12)
+    psi = src.ground_state(res)
+    for pm in (+1, -1):
+        dst = gpu.SzSector(m, npart + pm, pb, pvbc, True)
+        for spin in (0, 1):
+            for site in range(m.nc):
+                got = gpu.apply_C_n(src, dst, site, spin, pm, psi)
+                ref = osz.apply_c_n_sz(m, npart, site, spin, pm, psi)
+                assert np.array_equal(got, ref), (site, spin, pm)
+                assert np.any(got != 0) and not np.array_equal(got, gpu.apply_C(src, dst, site, spin, pm, psi))
+        dst.close()
+    src.close()
+    mu, (nu, nd) = gpu.synthetic("A")
+    a, b = gpu.Sector(mu, nu, nd), gpu.Sector(mu, nu + 1, nd)
+    with pytest.raises(gpu.EDError):
+        gpu.apply_C_n(a, b, 0, 0, +1, np.ones(a.dim))
+    a.close(); b.close()
+
+
+@pytest.mark.parametrize("case", [(1, 2, 0.0, True), (2, 2, 0.4, False)])
+def test_sz_ncup_green_vs_physical(gpu, case):
+    """FLAG_NCUP Green's functions of the S_z solver (compute_green_ncup): GN = <<D ; D+>>
+    with D = n_dn c_up, n_up c+_dn and the cross GNF = <<D ; d+>> against the Lehmann sums
+    of those physical operator strings over the whole Fock space (oracle/sz.py
+    ncup_green_physical), through the Lehmann path and through excitation Lanczos (the
+    correlated terms built on the device with ED_SITE_NCOND)."""
+    from ed_solver_amd.solver import compute_green_ncup, get_eigen_energies_sz
+    nc, nb, jh, pair = case
+    m, pb, pvbc = sc_model(nc, nb, 2.0, seed=21 + nc + nb, u_offdiag=1.0 if nc > 1 else None)
+    if jh:
+        m = with_hund(m, jh)
+    beta = 6.0
+    wn = np.pi / beta * (2 * np.arange(1, 17) - 1)
+    dmax = max(osz.sz_dim(m.ns, n) for n in range(2 * m.ns + 1))
+    states = get_eigen_energies_sz(m, pb, pvbc, pair, full_ed_switch=dmax, full_ed_green=True)
+    print(f"NCUP G case {case}: {len(states)} states, largest sector {dmax}", flush=True)
+    gn_ref, gnf_ref = osz.ncup_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair)
+    G_ref = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair)
+    for kw in (dict(full_ed_switch=dmax), dict(nitergreen=dmax, full_ed_switch=0)):
+        GN, GNF, G = compute_green_ncup(m, states, beta, wn, pb, pvbc, pair, **kw)
+        tol = 1e-10 if kw["full_ed_switch"] else 1e-9
+        assert np.max(np.abs(GN - gn_ref)) < tol, kw
+        assert np.max(np.abs(GNF - gnf_ref)) < tol, kw
+        assert np.max(np.abs(G - np.array([G_ref[a, a] for a in range(2 * nc)]))) < tol, kw

@@ -170,3 +170,36 @@ def test_nambu_green_without_pairing_is_spin_green():
         osz._sign_above, osz._hop = good, good_hop
     ok = osz.nambu_green_restated(m, beta, wn, pb=np.full((2, 2), 0.2))
     assert np.max(np.abs(bad - ok)) > 1e-4
+
+
+@pytest.mark.parametrize("nc,nb,jh,pair_ib", [(1, 2, 0.0, False), (2, 1, 0.5, True), (1, 3, 0.0, True)])
+def test_ncup_green_restated_equals_physical(nc, nb, jh, pair_ib):
+    """The correlated-hopping functions of FLAG_NCUP from the restated S_z route
+    (apply_Csz_N_updo's condition on the other Nambu orbital of the site) equal the
+    Lehmann sums of n_dn c_up and n_up c+_dn written as physical operator strings on the
+    whole Fock space: GN = <<D ; D+>> and the cross GNF = <<D ; d+>>.  A planted mistake
+    (the condition inverted) fails."""
+    r = np.random.default_rng(90 + nc * 5 + nb)
+    m = kanamori(nc, nb, 2.4, jh, r.uniform(-1, 1, nb), r.uniform(0.2, 0.6, nb))
+    pb = r.uniform(-0.3, 0.3, (nb, nb)); pb = (pb + pb.T) / 2
+    pvbc = r.uniform(-0.3, 0.3, (2, nb, nc)) if pair_ib else None
+    beta = 7.0
+    wn = np.pi / beta * (2 * np.arange(1, 9) - 1)
+    gn_r, gnf_r = osz.ncup_green_restated(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    gn_p, gnf_p = osz.ncup_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    assert np.max(np.abs(gn_r - gn_p)) < 1e-12
+    assert np.max(np.abs(gnf_r - gnf_p)) < 1e-12
+    G = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    for a in range(2 * nc):   # the projector matters: GN differs from G and from GNF
+        assert np.max(np.abs(gn_p[a] - G[a, a])) > 1e-3
+        assert np.max(np.abs(gn_p[a] - gnf_p[a])) > 1e-4
+    good = osz.apply_c_n_sz
+    try:
+        def inverted(model, npart, site, spin, pm, psi):
+            full = osz.apply_c_sz(model, npart, site, spin, pm, psi)
+            return full - good(model, npart, site, spin, pm, psi)
+        osz.apply_c_n_sz = inverted
+        bad, _ = osz.ncup_green_restated(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
+    finally:
+        osz.apply_c_n_sz = good
+    assert np.max(np.abs(bad - gn_p)) > 1e-3

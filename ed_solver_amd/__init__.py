@@ -8,11 +8,11 @@ from .aim import AIMParams, WORKLOADS, kanamori, sector_dims, synthetic
 from ._lib import BreakdownError, EDError, LIB_PATH, lib
 from .hclass import (LanczosResult, Sector, Hmult__, Lanczos_fast_diagonalize, Lanczos_get_GS_sector, apply_C, apply_C_n,
                      batched_stats, compute_dynamic, compute_dynamic_batched, dynamic_from_state, delete_H, dimen_H, invert_zmtridiag, matsubara,
-                     new_H, start_vector, tridiag_eig, tridiag_lowest, band_eig, SzSector, ComplexSector)
+                     new_H, start_vector, tridiag_eig, tridiag_lowest, band_eig, SzSector, ComplexSector, ComplexSzSector)
 from .block import block_lanczos
 
 __all__ = ["AIMParams", "WORKLOADS", "kanamori", "sector_dims", "synthetic", "BreakdownError", "EDError", "LIB_PATH",
            "lib", "LanczosResult", "Sector", "Hmult__", "Lanczos_fast_diagonalize", "Lanczos_get_GS_sector",
            "apply_C", "apply_C_n", "batched_stats", "compute_dynamic", "compute_dynamic_batched", "dynamic_from_state", "delete_H", "dimen_H", "invert_zmtridiag",
            "matsubara", "new_H", "start_vector", "tridiag_eig", "tridiag_lowest", "band_eig", "block_lanczos", "SzSector",
-           "ComplexSector"]
+           "ComplexSector", "ComplexSzSector"]

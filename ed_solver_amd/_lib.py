@@ -65,6 +65,8 @@ SIGNATURES = {
     "ed_init": (ctypes.c_int, [ctypes.c_int]),
     "ed_sector_create_sz": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(_vp)]),
+    "ed_sector_create_sz_c": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(_vp)]),
     "ed_sector_energy_shift": (ctypes.c_int, [_vp, _dp]),
     "ed_sector_quart_sz": (ctypes.c_int, [_vp, _dp]),
     "ed_sector_hund_sz": (ctypes.c_int, [_vp, _c_int_p, _i32p, _dp]),

@@ -164,6 +164,28 @@ __global__ __launch_bounds__(kBlock) void k_hv_sz_fly(SzArgs<T> a, SpinModel m, 
     hv_sz_fly_body<T, MODE>(a, m, states, rank, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
 }
 
+// Complex S_z sectors: HAIMsz_multc, one thread per amplitude (FP64).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_sz_cplx(SzArgs<double> a, const double *__restrict__ xin,
+                                                        double *__restrict__ yout, int64_t ld, LzArgs<double> lz)
+{
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_sz_cplx_body<MODE>(a, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x, red);
+}
+
+// Complex S_z sectors, on-the-fly entries (HAIMsz_mult_fly_c).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_hv_sz_cplx_fly(SzArgs<double> a, SpinModel m, const int32_t *states,
+                                                            const int32_t *rank, const double *__restrict__ xin,
+                                                            double *__restrict__ yout, int64_t ld, LzArgs<double> lz)
+{
+    __shared__ double red[kRed];
+    const int64_t off = MODE ? 0 : (int64_t)blockIdx.y * ld;
+    hv_sz_cplx_fly_body<MODE>(a, m, states, rank, MODE ? xin : xin + off, MODE ? yout : yout + off, lz, blockIdx.x,
+                              red);
+}
+
 // S_z sectors: HAIMsz_multr, one thread per row.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_hv_sz(SzArgs<T> a, const T *__restrict__ xin, T *__restrict__ yout,
@@ -451,17 +473,17 @@ static HvShape hv_shape(const ed_sector *s, int sched)
     h.k = s->up.kpad;
     h.xcd = 0;
     h.rows = 0;
-    if (s->cplx) {   // complex sector: one amplitude per thread, grid-stride
+    if (s->cplx && !s->sz) {   // complex sector: one amplitude per thread, grid-stride
         h.kind = 4;
         h.cpb = 1;
         h.nblk = (int)std::min<int64_t>(kMaxRowBlocks, (s->cdim + kBlock - 1) / kBlock);
         h.lds = 0;
         return h;
     }
-    if (s->sz) {   // S_z sector: one row per thread, grid-stride beyond kMaxRowBlocks workgroups
+    if (s->sz) {   // S_z sector: one row (complex: amplitude) per thread, grid-stride beyond kMaxRowBlocks
         h.kind = 3;
         h.cpb = 1;
-        h.nblk = (int)std::min<int64_t>(kMaxRowBlocks, (s->dim + kBlock - 1) / kBlock);
+        h.nblk = (int)std::min<int64_t>(kMaxRowBlocks, ((s->cplx ? s->cdim : s->dim) + kBlock - 1) / kBlock);
         h.lds = 0;
         return h;
     }
@@ -553,6 +575,21 @@ static void hv_launch_mode(const ed_sector *s, const HvShape &h, const T *x, T *
         za.kh = s->sz_kh;
         za.hrk = s->d_sz_hrk;
         za.hval = (const T *)(f64 ? (const void *)s->d_sz_hval : (const void *)s->d_sz_hval32);
+        za.vim = nullptr;
+        if (s->cplx) {   // complex S_z sectors run FP64 only (check_common)
+            if constexpr (std::is_same<T, double>::value) {
+                za.d = s->cdim;
+                za.vim = s->up.ell_vim;
+                if (s->sz_fly)
+                    hipLaunchKernelGGL((k_hv_sz_cplx_fly<MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st,
+                                       za, s->sz_model, (const int32_t *)s->up.states, (const int32_t *)s->d_rank, x,
+                                       y, ld, lz);
+                else
+                    hipLaunchKernelGGL((k_hv_sz_cplx<MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za,
+                                       x, y, ld, lz);
+            }
+            return;
+        }
         if (s->sz_fly) {
             hipLaunchKernelGGL((k_hv_sz_fly<T, MODE>), dim3(h.nblk, MODE ? 1 : nvec), dim3(kBlock), 0, st, za,
                                s->sz_model, (const int32_t *)s->up.states, (const int32_t *)s->d_rank, x, y, ld, lz);

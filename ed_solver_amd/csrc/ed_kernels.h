@@ -590,6 +590,13 @@ __device__ inline bool hop(uint32_t s, int to, int from, uint32_t *out, int *sg)
     return true;
 }
 
+// The mask and cutoff of tab_HAIM2 for one coefficient: kept when nonzero and
+// |t| > cutoff (the modulus for a complex t).
+__device__ inline bool keep_coef(double re, double im, double cut)
+{
+    return !(re == 0.0 && im == 0.0) && hypot(re, im) > cut;
+}
+
 
 // ------------------------------------------------------------- S_z (Nambu) sectors
 
@@ -606,6 +613,7 @@ struct SzArgs {
     int kh;                 // Hund transitions per row (0: none)
     const int32_t *hrk;     // [kh][d]
     const T *hval;
+    const T *vim;           // complex sector: imaginary parts of val ([kw][d]); null when real
 };
 
 // HAIMsz_multr (HAIMsz.f90:453-560, USE_CC gather form): one thread per row,
@@ -639,6 +647,51 @@ __device__ inline void hv_sz_body(const SzArgs<T> &a, const T *__restrict__ xin,
         for (int k = 0; k < a.kh; ++k) acc += a.hval[k * d + i] * x[a.hrk[k * d + i]];
         y[i] = acc;
         dot += (double)acc * (double)xv;
+    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+}
+
+// HAIMsz_multc (HAIMsz.f90:188-452, USE_CC gather form) of a complex S_z sector on planar
+// vectors (Re[d], Im[d]), one thread per amplitude (grid-stride):
+//   w_i = (QUART_i + diag_i) v_i + sum_k conj(h_ik) v_{rank_ik} + sum_Hund (-J) v_target
+// with conj(h) v = (hr vr + hi vi) + i (hr vi - hi vr) (HAIMsz.f90:279-289; QUART, the
+// diagonal and the Hund elements are real).  MODE 1: Lanczos phase 1 of the real
+// embedding, partial Re <u_j, z> per workgroup.
+template <int MODE>
+__device__ inline void hv_sz_cplx_body(const SzArgs<double> &a, const double *__restrict__ xin,
+                                       double *__restrict__ yout, const LzArgs<double> &lz, int blk, double *red)
+{
+    const double *__restrict__ x;
+    double *__restrict__ y;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int64_t d = a.d, stride = (int64_t)gridDim.x * blockDim.x;
+    double dot = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < d; i += stride) {
+        const double xr = x[i], xi = x[d + i];
+        const double dg = a.quart[i] + a.diag[i];
+        double yr = dg * xr, yi = dg * xi;
+        for (int k = 0; k < a.kw; ++k) {
+            const int64_t e = (int64_t)k * d + i, t = a.rk[e];
+            const double hr = a.val[e], hi = a.vim[e], vr = x[t], vi = x[d + t];
+            yr += hr * vr + hi * vi;
+            yi += hr * vi - hi * vr;
+        }
+        for (int k = 0; k < a.kh; ++k) {
+            const int64_t e = (int64_t)k * d + i, t = a.hrk[e];
+            yr += a.hval[e] * x[t];
+            yi += a.hval[e] * x[d + t];
+        }
+        y[i] = yr;
+        y[d + i] = yi;
+        dot += yr * xr + yi * xi;
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }
@@ -775,6 +828,74 @@ __device__ inline void hv_sz_fly_body(const SzArgs<T> &a, const SpinModel &m, co
         for (int k = 0; k < a.kh; ++k) acc += a.hval[k * d + i] * x[a.hrk[k * d + i]];
         y[i] = acc;
         dot += (double)acc * (double)xv;
+    }
+    if (MODE == 1) block_partial(dot, red, lz.partA + blk);
+}
+
+// HAIMsz_mult_fly_c (HAIMsz.f90:833-1027): the complex S_z H.v with each row's entries
+// generated on the fly from the complex Nambu matrices in the table's order (k_tab's
+// loops and keep rule; the hybridisation's reverse hop carries the conjugate), then the
+// conj(h) gather of hv_sz_cplx_body on planar vectors.  Same values as the table, so the
+// same bits.
+template <int MODE>
+__device__ inline void hv_sz_cplx_fly_body(const SzArgs<double> &a, const SpinModel &m,
+                                           const int32_t *__restrict__ states, const int32_t *__restrict__ rank,
+                                           const double *__restrict__ xin, double *__restrict__ yout,
+                                           const LzArgs<double> &lz, int blk, double *red)
+{
+    const double *__restrict__ x;
+    double *__restrict__ y;
+    int j = 0;
+    if (MODE == 1) {
+        if (!lz_hv_begin(lz, blk == 0 && threadIdx.x == 0, j)) return;
+        x = lz.slot(j);
+        y = lz.slot(j + 1);
+    } else {
+        x = xin;
+        y = yout;
+    }
+    const int64_t d = a.d, stride = (int64_t)gridDim.x * blockDim.x;
+    const int ni = m.nimp, nbt = m.nbath;
+    double dot = 0.0;
+    for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < d; i += stride) {
+        const uint32_t s = (uint32_t)states[i];
+        const double xr = x[i], xi = x[d + i];
+        const double dg = a.quart[i] + a.diag[i];
+        double yr = dg * xr, yi = dg * xi;
+        uint32_t o;
+        int sg;
+        auto add = [&](double tr, double ti, int to, int frm) {
+            if (!hop(s, to, frm, &o, &sg)) return;
+            const int64_t t = rank[o];
+            const double hr = tr * sg, hi = ti * sg, vr = x[t], vi = x[d + t];
+            yr += hr * vr + hi * vi;
+            yi += hr * vi - hi * vr;
+        };
+        for (int p = 0; p < ni; ++p)
+            for (int q = 0; q < ni; ++q) {
+                const double t = m.ec[p * ni + q], ti = m.ec_im[p * ni + q];
+                if (p != q && keep_coef(t, ti, m.cut)) add(t, ti, m.ibit[p], m.ibit[q]);
+            }
+        for (int p = 0; p < nbt; ++p)
+            for (int q = 0; q < nbt; ++q) {
+                const double t = m.eb[p * nbt + q], ti = m.eb_im[p * nbt + q];
+                if (p != q && keep_coef(t, ti, m.cut)) add(t, ti, m.bbit[p], m.bbit[q]);
+            }
+        for (int b = 0; b < nbt; ++b)
+            for (int c = 0; c < ni; ++c) {
+                const double t = m.vbc[b * ni + c], ti = m.vbc_im[b * ni + c];
+                if (!keep_coef(t, ti, m.cut)) continue;
+                add(t, ti, m.bbit[b], m.ibit[c]);
+                add(t, -ti, m.ibit[c], m.bbit[b]);
+            }
+        for (int k = 0; k < a.kh; ++k) {
+            const int64_t e = (int64_t)k * d + i, t = a.hrk[e];
+            yr += a.hval[e] * x[t];
+            yi += a.hval[e] * x[d + t];
+        }
+        y[i] = yr;
+        y[d + i] = yi;
+        dot += yr * xr + yi * xi;
     }
     if (MODE == 1) block_partial(dot, red, lz.partA + blk);
 }

@@ -70,13 +70,6 @@ static SpinModel spin_model(int nc, int nb, const double *ec, const double *eb, 
     return m;
 }
 
-// The mask and cutoff of tab_HAIM2 for one coefficient: kept when nonzero and
-// |t| > cutoff (the modulus for a complex t).
-__device__ inline bool keep_coef(double re, double im, double cut)
-{
-    return !(re == 0.0 && im == 0.0) && hypot(re, im) > cut;
-}
-
 // tab_HAIM2 (HAIM_offdiag_arrays.f90:419-527) for one spin, one thread per state: the
 // diagonal and the entries (target rank, <target|H|state>) in the reference order.  For a
 // complex model off_im receives the imaginary parts: Ec(a,b) sign, Eb(a,b) sign,
@@ -725,11 +718,61 @@ static void nambu_matrices(int nc, int nb, const double *ec, const double *eb, c
         }
 }
 
+// Imaginary parts of the Nambu matrices of a complex model (the _complex branches of
+// Nambu_Ec / Nambu_Eb / Nambu_Vbc): Im Ec_N = [[Im Ec_up, 0], [0, -Im Ec_dn^T]],
+// Im Eb_N = [[Im Eb_up, Im Pb], [-Im Pb^T, -Im Eb_dn^T]] (conj(Pb)^T below the diagonal),
+// Im Vbc_N = [[Im Vbc_up, Im PVbc_dn], [Im PVbc_up, +Im Vbc_dn]] (-conj(Vbc_dn)).  Any
+// input may be null (zero imaginary part).
+static void nambu_matrices_im(int nc, int nb, const double *ec_im, const double *eb_im, const double *vbc_im,
+                              const double *pb_im, const double *pvbc_im, int pair_ib, std::vector<double> &eci,
+                              std::vector<double> &ebi, std::vector<double> &vbi)
+{
+    const int n2 = 2 * nc, b2 = 2 * nb;
+    eci.assign((size_t)n2 * n2, 0.0);
+    ebi.assign((size_t)b2 * b2, 0.0);
+    vbi.assign((size_t)b2 * n2, 0.0);
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b)
+            if (ec_im) {
+                eci[(size_t)a * n2 + b] = ec_im[a * nc + b];
+                eci[(size_t)(nc + a) * n2 + nc + b] = -ec_im[nc * nc + b * nc + a];
+            }
+    for (int k = 0; k < nb; ++k)
+        for (int l = 0; l < nb; ++l) {
+            if (eb_im) {
+                ebi[(size_t)k * b2 + l] = eb_im[k * nb + l];
+                ebi[(size_t)(nb + k) * b2 + nb + l] = -eb_im[nb * nb + l * nb + k];
+            }
+            if (pb_im) {
+                ebi[(size_t)k * b2 + nb + l] = pb_im[k * nb + l];
+                ebi[(size_t)(nb + l) * b2 + k] = -pb_im[k * nb + l];
+            }
+        }
+    for (int k = 0; k < nb; ++k)
+        for (int c = 0; c < nc; ++c) {
+            if (vbc_im) {
+                vbi[(size_t)k * n2 + c] = vbc_im[k * nc + c];
+                vbi[(size_t)(nb + k) * n2 + nc + c] = vbc_im[nb * nc + k * nc + c];
+            }
+            if (pair_ib && pvbc_im) {
+                vbi[(size_t)(nb + k) * n2 + c] = pvbc_im[k * nc + c];
+                vbi[(size_t)k * n2 + nc + c] = pvbc_im[nb * nc + k * nc + c];
+            }
+        }
+}
+
+// Imaginary parts of a complex S_z model (each may be null).
+struct SzImag {
+    const double *ec = nullptr, *eb = nullptr, *vbc = nullptr, *pb = nullptr, *pvbc = nullptr;
+};
+
 // The S_z sector of Nambu particle number npart = Ns + 2 S_z (new_HAIMsz + tab_HAIMsz,
 // HAIMsz.f90:51-177; the sector is the popcount-npart states of 2 Ns orbitals in
-// increasing order, quantum_fermion_hilbert_class.f90:346-421).
+// increasing order, quantum_fermion_hilbert_class.f90:346-421).  With `im` the model is
+// complex (the COMPLEX(8) tables of HAIMsz_multc): vectors are planar as in every
+// complex sector.
 static int create_sector_sz(const ed_model *m, const double *pb, const double *pvbc, int pair_ib, int npart,
-                            ed_sector **out)
+                            ed_sector **out, const SzImag *im = nullptr)
 {
     if (!m || !out) return fail(ED_ERR_ARG, "ed_sector_create_sz: NULL argument");
     *out = nullptr;
@@ -775,10 +818,13 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
         (rc = dalloc(&T.noff, d)) != ED_OK || (rc = dalloc(&T.ptr, d + 1)) != ED_OK)
         return bail(rc);
     hipLaunchKernelGGL(k_enum_states, dim3(nblocks(d)), dim3(kBlock), 0, st, nbits, npart, d, T.states);
-    std::vector<double> ecn, ebn, vbn;
+    std::vector<double> ecn, ebn, vbn, eci, ebi, vbi;
     nambu_matrices(nc, nb, m->ec, m->eb, m->vbc, pb, pvbc, pair_ib, ecn, ebn, vbn);
+    const bool cx = im != nullptr;
+    if (cx) nambu_matrices_im(nc, nb, im->ec, im->eb, im->vbc, im->pb, im->pvbc, pair_ib, eci, ebi, vbi);
     double *d_mat = nullptr, *d_u = nullptr;   // d_u: U[nc][nc], then JJ[nc][nc]
-    const size_t nmat = ecn.size() + std::max<size_t>(1, ebn.size()) + std::max<size_t>(1, vbn.size());
+    const size_t nre = ecn.size() + std::max<size_t>(1, ebn.size()) + std::max<size_t>(1, vbn.size());
+    const size_t nmat = cx ? 2 * nre : nre;   // real parts, then (complex) the imaginary parts
     if ((rc = dalloc(&d_mat, nmat)) != ED_OK || (rc = dalloc(&d_u, (size_t)2 * nc * nc)) != ED_OK) {
         (void)hipFree(d_mat);
         return bail(rc);
@@ -797,6 +843,15 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
         ED_HIP_CHECK(hipMemcpyAsync(d_eb, ebn.data(), sizeof(double) * ebn.size(), hipMemcpyHostToDevice, st));
         ED_HIP_CHECK(hipMemcpyAsync(d_vbc, vbn.data(), sizeof(double) * vbn.size(), hipMemcpyHostToDevice, st));
     }
+    double *d_eci = cx ? d_mat + nre : nullptr, *d_ebi = cx ? d_eci + ecn.size() : nullptr,
+           *d_vbi = cx ? d_ebi + std::max<size_t>(1, ebn.size()) : nullptr;
+    if (cx) {
+        ED_HIP_CHECK(hipMemcpyAsync(d_eci, eci.data(), sizeof(double) * eci.size(), hipMemcpyHostToDevice, st));
+        if (nb) {
+            ED_HIP_CHECK(hipMemcpyAsync(d_ebi, ebi.data(), sizeof(double) * ebi.size(), hipMemcpyHostToDevice, st));
+            ED_HIP_CHECK(hipMemcpyAsync(d_vbi, vbi.data(), sizeof(double) * vbi.size(), hipMemcpyHostToDevice, st));
+        }
+    }
     ED_HIP_CHECK(hipMemcpyAsync(d_u, m->u, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
     ED_HIP_CHECK(hipMemcpyAsync(d_u + nc * nc, m->jj, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
     SpinModel sm{};
@@ -813,6 +868,9 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     sm.ec = d_ec;
     sm.eb = d_eb;
     sm.vbc = d_vbc;
+    sm.ec_im = d_eci;   // null for a real model
+    sm.eb_im = d_ebi;
+    sm.vbc_im = d_vbi;
     sm.cut = m->cutoff;
     // the sector keeps the Nambu matrices for the on-the-fly H.v (HAIMsz_mult_fly)
     S->sz_model = sm;
@@ -834,13 +892,15 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     T.kmax = h[1];
     T.kpad = std::max(T.kmax, 1);
     if ((rc = dalloc(&T.rk, (size_t)T.nnz)) != ED_OK || (rc = dalloc(&T.off, (size_t)T.nnz)) != ED_OK) return bail(rc);
+    if (cx && (rc = dalloc(&T.off_im, (size_t)std::max<int64_t>(1, T.nnz))) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_tab<true>, dim3(nblocks(d)), dim3(kBlock), 0, st, sm, T.states, d, S->d_rank,
-                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off, (double *)nullptr);
+                       (const int32_t *)T.ptr, T.noff, T.diag, T.rk, T.off, T.off_im);
     const int kw = T.kpad;
     if ((rc = dalloc(&T.ell_rk, (size_t)kw * d)) != ED_OK || (rc = dalloc(&T.ell_val, (size_t)kw * d)) != ED_OK)
         return bail(rc);
+    if (cx && (rc = dalloc(&T.ell_vim, (size_t)kw * d)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_ell, dim3(nblocks(d)), dim3(kBlock), 0, st, d, kw, T.ptr, T.rk, T.off, T.ell_rk, T.ell_val,
-                       (const double *)nullptr, (double *)nullptr);
+                       (const double *)T.off_im, T.ell_vim);
     if ((rc = dalloc(&S->d_quart, d)) != ED_OK || (rc = dalloc(&S->d_quart32, d)) != ED_OK ||
         (rc = dalloc(&T.diag32, d)) != ED_OK || (rc = dalloc(&T.off32, (size_t)T.nnz)) != ED_OK ||
         (rc = dalloc(&T.ell_val32, (size_t)kw * d)) != ED_OK)
@@ -890,6 +950,11 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     ED_HIP_CHECK(hipGetLastError());
     S->dn.d = 1;
     S->dim = d;
+    if (cx) {   // planar vectors (Re[d], Im[d]), as every complex sector
+        S->cplx = 1;
+        S->cdim = d;
+        S->dim = 2 * (int64_t)d;
+    }
     if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
         return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(ED_ERR_HIP, "S_z sector build failed"));
@@ -907,11 +972,27 @@ int ed_sector_create_sz(const ed_model *model, const double *pb, const double *p
     return ed::create_sector_sz(model, pb, pvbc, pairing_imp_to_bath, npart, out);
 }
 
+int ed_sector_create_sz_c(const ed_model *model, const double *ec_im, const double *eb_im, const double *vbc_im,
+                          const double *pb, const double *pb_im, const double *pvbc, const double *pvbc_im,
+                          int pairing_imp_to_bath, int npart, ed_sector **out)
+{
+    if (!ec_im) return ed::fail(ED_ERR_ARG, "ed_sector_create_sz_c: NULL ec_im");
+    if (pb_im && !pb) return ed::fail(ED_ERR_ARG, "ed_sector_create_sz_c: pb_im without pb");
+    if (pvbc_im && !pvbc) return ed::fail(ED_ERR_ARG, "ed_sector_create_sz_c: pvbc_im without pvbc");
+    ed::SzImag im;
+    im.ec = ec_im;
+    im.eb = eb_im;
+    im.vbc = vbc_im;
+    im.pb = pb_im;
+    im.pvbc = pvbc_im;
+    return ed::create_sector_sz(model, pb, pvbc, pairing_imp_to_bath, npart, out, &im);
+}
+
 int ed_sector_quart_sz(const ed_sector *sec, double *quart)
 {
     if (!sec || !quart) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: NULL argument");
     if (!sec->sz) return ed::fail(ED_ERR_ARG, "ed_sector_quart_sz: not an S_z sector");
-    ED_HIP_CHECK(hipMemcpy(quart, sec->d_quart, sizeof(double) * sec->dim, hipMemcpyDeviceToHost));
+    ED_HIP_CHECK(hipMemcpy(quart, sec->d_quart, sizeof(double) * sec->up.d, hipMemcpyDeviceToHost));
     return ED_OK;
 }
 
@@ -920,7 +1001,7 @@ int ed_sector_hund_sz(const ed_sector *sec, int *kh, int32_t *rk, double *val)
     if (!sec || !kh) return ed::fail(ED_ERR_ARG, "ed_sector_hund_sz: NULL argument");
     if (!sec->sz) return ed::fail(ED_ERR_ARG, "ed_sector_hund_sz: not an S_z sector");
     *kh = sec->sz_kh;
-    const size_t n = (size_t)sec->sz_kh * sec->dim;
+    const size_t n = (size_t)sec->sz_kh * sec->up.d;
     if (rk && n) ED_HIP_CHECK(hipMemcpy(rk, sec->d_sz_hrk, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     if (val && n) ED_HIP_CHECK(hipMemcpy(val, sec->d_sz_hval, sizeof(double) * n, hipMemcpyDeviceToHost));
     return ED_OK;

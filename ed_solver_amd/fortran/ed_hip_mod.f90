@@ -89,6 +89,15 @@ module ed_hip
       integer(c_int), value      :: pairing_imp_to_bath, npart
       type(c_ptr), intent(out)   :: sec
     end function
+    ! complex S_z sector (COMPLEX(8) new_HAIMsz); every imaginary part but ec_im may be c_null_ptr
+    integer(c_int) function ed_sector_create_sz_c(model, ec_im, eb_im, vbc_im, pb, pb_im, pvbc, pvbc_im, &
+        pairing_imp_to_bath, npart, sec) bind(C, name="ed_sector_create_sz_c")
+      import :: c_int, c_ptr, ed_model
+      type(ed_model), intent(in) :: model
+      type(c_ptr), value         :: ec_im, eb_im, vbc_im, pb, pb_im, pvbc, pvbc_im
+      integer(c_int), value      :: pairing_imp_to_bath, npart
+      type(c_ptr), intent(out)   :: sec
+    end function
     integer(c_int) function ed_sector_energy_shift(sec, shift) bind(C, name="ed_sector_energy_shift")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value          :: sec

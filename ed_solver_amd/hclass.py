@@ -515,6 +515,49 @@ class ComplexSector(Sector):
                                       ctypes.c_void_p(stream) if stream else None))
 
 
+class ComplexSzSector(ComplexSector):
+    """One S_z (Nambu) sector of a complex model with a superconducting bath (the COMPLEX(8)
+    new_HAIMsz / HAIMsz_multc, HAIMsz.f90:188-452): model holds the real parts, im the
+    imaginary parts as for ComplexSector ({"ec": [2, nc, nc], "eb": [2, nb, nb], "vbc":
+    [2, nb, nc]}); pb[nb][nb] and pvbc[2][nb][nc] may be complex.  Vectors are complex numpy
+    arrays (planar on the device), FP64 only; physical energies are eigenvalues +
+    energy_shift."""
+
+    def __init__(self, model: AIMParams, npart: int, im: dict, pb=None, pvbc=None, pairing_imp_to_bath: bool = False):
+        self.model = model
+        self.npart = int(npart)
+        self.nup, self.ndn = self.npart, -1
+        self._cm = CModel.from_params(model)
+        nc, nb = model.nc, model.nb
+        self._ec_im = np.ascontiguousarray(np.broadcast_to(im["ec"], (2, nc, nc)), np.float64)
+        self._eb_im = np.ascontiguousarray(np.broadcast_to(im.get("eb", np.zeros((2, nb, nb))), (2, nb, nb)),
+                                           np.float64)
+        self._vbc_im = np.ascontiguousarray(np.broadcast_to(im.get("vbc", np.zeros((2, nb, nc))), (2, nb, nc)),
+                                            np.float64)
+        keep = {}
+        for name, a, shape in (("pb", pb, (nb, nb)), ("pvbc", pvbc, (2, nb, nc))):
+            if a is None:
+                keep[name] = keep[name + "_im"] = None
+            else:
+                a = np.asarray(a).reshape(shape)
+                keep[name] = np.ascontiguousarray(a.real, np.float64)
+                keep[name + "_im"] = np.ascontiguousarray(a.imag, np.float64)
+        self._pair = keep
+        p = lambda a: dptr(a) if a is not None else None
+        h = ctypes.c_void_p()
+        check(lib().ed_sector_create_sz_c(ctypes.byref(self._cm), dptr(self._ec_im), dptr(self._eb_im),
+                                          dptr(self._vbc_im), p(keep["pb"]), p(keep["pb_im"]), p(keep["pvbc"]),
+                                          p(keep["pvbc_im"]), 1 if pairing_imp_to_bath else 0, self.npart,
+                                          ctypes.byref(h)))
+        self._h = h
+        self.dim = int(lib().ed_sector_dim(h))
+        self.dup, self.ddn = self.dim, 1
+
+    quart = SzSector.quart
+    set_fly = SzSector.set_fly
+    hund = SzSector.hund
+
+
 def compute_dynamic(sector: Sector, iph: int, energy: float, phi: np.ndarray, nitergreen: int, wn: np.ndarray,
                     cutoff_dynamic: float = 1e-8, precision="f64"):
     """compute_dynamic_cpu (green_class_compute_dynamic.f90:191-407): returns (dyn, a, b)."""

@@ -294,19 +294,37 @@ def compute_green_matrix(model: AIMParams, states: list[EigenState], beta: float
 # --------------------------------------------------------------------------------
 # S_z (Nambu) basis: superconducting baths (HAIMsz; apply_Csz; compute_greenAB with Cup / Cdo)
 
+def _sz_complex(pb, pvbc, im) -> bool:
+    return im is not None or np.iscomplexobj(pb) or np.iscomplexobj(pvbc)
+
+
+def make_sz_sector(model: AIMParams, npart: int, pb=None, pvbc=None, pairing_imp_to_bath: bool = False,
+                   im: dict | None = None):
+    """new_H's sz branch: the real S_z sector, or the complex one (COMPLEX(8) HAIMsz_multc)
+    when im (imaginary parts of Ec, Eb, Vbc) is given or the pairing is complex."""
+    from .hclass import ComplexSzSector, SzSector
+    if _sz_complex(pb, pvbc, im):
+        nc, nb = model.nc, model.nb
+        im = im if im is not None else {"ec": np.zeros((2, nc, nc))}
+        return ComplexSzSector(model, npart, im, pb, pvbc, pairing_imp_to_bath)
+    return SzSector(model, npart, pb, pvbc, pairing_imp_to_bath)
+
+
 def get_eigen_energies_sz(model: AIMParams, pb=None, pvbc=None, pairing_imp_to_bath: bool = False, sectors=None,
                           neigen: int = 1, dEmax: float = np.inf, nitermax: int = 128, tolerance: float = 1e-12,
-                          full_ed_switch: int = FULL_ED_SWITCH, precision="f64", full_ed_green: bool = False):
+                          full_ed_switch: int = FULL_ED_SWITCH, precision="f64", full_ed_green: bool = False,
+                          im: dict | None = None):
     """Lowest states of the S_z sectors npart = 0 .. 2 Ns (or `sectors`), filtered to
     [E0, E0 + dEmax] as get_eigen_energies (solver.h:98-237 with the sz branch of new_H):
     dense ED at dimension <= full_ed_switch, else the device Lanczos.  States carry the
     key (npart,) and the eigenvalue of the Nambu H; the physical energy adds the
-    sector's energy_shift (the same for every npart, so weights and G do not change)."""
-    from .hclass import SzSector
+    sector's energy_shift (the same for every npart, so weights and G do not change).
+    A complex model (im, or complex pb / pvbc) runs the complex S_z sectors; its states
+    carry complex vectors."""
     sectors = [(n,) for n in range(2 * model.ns + 1)] if sectors is None else [tuple(np.atleast_1d(k)) for k in sectors]
     states: list[EigenState] = []
     for key in sectors:
-        with SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath) as sec:
+        with make_sz_sector(model, key[0], pb, pvbc, pairing_imp_to_bath, im) as sec:
             if sec.dim <= full_ed_switch:
                 ev, vec = sec.full_ed(sec.dim if full_ed_green else min(neigen, sec.dim))
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
@@ -326,7 +344,7 @@ def get_eigen_energies_sz(model: AIMParams, pb=None, pvbc=None, pairing_imp_to_b
 def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, pb=None,
                         pvbc=None, pairing_imp_to_bath: bool = False, nitergreen: int = 36, orbitals=None,
                         weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8, precision="f64",
-                        full_ed_switch: int = FULL_ED_SWITCH) -> np.ndarray:
+                        full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None) -> np.ndarray:
     """Nambu impurity Green's function G^N_ij(i w_n) = <<d_i ; d_j+>>, shape (2n, 2n, nw), over
     the Nambu orbitals d_(c, 0) = c_(c, up) (index c) and d_(c, 1) = c+_(c, dn) (index n + c)
     of the S_z sectors (states from get_eigen_energies_sz with the same pairing):
@@ -337,28 +355,38 @@ def compute_green_nambu(model: AIMParams, states: list[EigenState], beta: float,
     the down spin is created by the physical c_dn), so the off-diagonal elements come
     from the symmetric combinations (d_i + d_j) of the real H as in compute_green_matrix.
     A state's excitation vectors are built on the device (ed_dynamic_from_state with Nambu
-    orbital indices) and its runs batched."""
-    from .hclass import SzSector
+    orbital indices) and its runs batched.  A complex model (im, or complex pb / pvbc):
+    one more run per pair from d_i - i d_j, and G_ij, G_ji = (S +- D) / 2 as in
+    compute_green_matrix (the _complex branch of symmetric_combineAA); its excitation
+    vectors are built per term by apply_C and the runs batched."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     nambu = [(c, 0) for c in orbitals] + [(c, 1) for c in orbitals]
     n = len(nambu)
     pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
     ops = [[(c, sp, 1.0)] for c, sp in nambu] + [[(*nambu[a], 1.0), (*nambu[b], 1.0)] for a, b in pairs]
+    cx = _sz_complex(pb, pvbc, im)
+    if cx:
+        ops += [[(*nambu[a], 1.0), (*nambu[b], -1j)] for a, b in pairs]
     g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
-                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath))
+                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath, im))
     G = np.zeros((n, n, len(wn)), complex)
     for a in range(n):
         G[a, a] = g[a]
     for k, (a, b) in enumerate(pairs):
-        G[a, b] = G[b, a] = 0.5 * (g[n + k] - g[a] - g[b])
+        S = g[n + k] - g[a] - g[b]
+        if not cx:
+            G[a, b] = G[b, a] = 0.5 * S
+        else:
+            D = -1j * (g[n + len(pairs) + k] - g[a] - g[b])
+            G[a, b], G[b, a] = 0.5 * (S + D), 0.5 * (S - D)
     return G
 
 
-def _sz_ops(model: AIMParams, pb, pvbc, pairing_imp_to_bath):
+def _sz_ops(model: AIMParams, pb, pvbc, pairing_imp_to_bath, im=None):
     """make / target / device_terms of _green_ops for the S_z sectors: every Nambu operator
     moves npart by one; device terms are Nambu orbitals spin * nc + site, with SITE_NCOND
-    on the correlated ones."""
-    from .hclass import SITE_NCOND, SzSector
+    on the correlated ones (real sectors; a complex model builds its vectors per term)."""
+    from .hclass import SITE_NCOND
 
     def target(key, sp, pm):
         t = key[0] + pm
@@ -369,31 +397,43 @@ def _sz_ops(model: AIMParams, pb, pvbc, pairing_imp_to_bath):
     def terms(op):
         return 0, [((sp * nc + c) | (SITE_NCOND if corr and corr[0] else 0), coef) for c, sp, coef, *corr in op]
 
-    return (lambda key: SzSector(model, key[0], pb, pvbc, pairing_imp_to_bath)), target, terms
+    make = lambda key: make_sz_sector(model, key[0], pb, pvbc, pairing_imp_to_bath, im)
+    return make, target, (None if _sz_complex(pb, pvbc, im) else terms)
 
 
 def compute_green_ncup(model: AIMParams, states: list[EigenState], beta: float, wn: np.ndarray, pb=None,
                        pvbc=None, pairing_imp_to_bath: bool = False, nitergreen: int = 36, orbitals=None,
                        weight_cutoff: float = 0.0, cutoff_dynamic: float = 1e-8, precision="f64",
-                       full_ed_switch: int = FULL_ED_SWITCH):
+                       full_ed_switch: int = FULL_ED_SWITCH, im: dict | None = None):
     """The correlated-hopping Green's functions of FLAG_NCUP (correlations_dyn_stat.h:69-100)
     on the S_z sectors, per Nambu impurity orbital a = (c, s) (index s n + c over `orbitals`):
     * GN[a] = <<D_a ; D_a+>>: compute_greenAA with apply_N_Cup / apply_N_Cdo, the operator
       D_(c, 0) = n_dn c_up and D_(c, 1) = n_up c+_dn (apply_Csz_N_updo, apply_C.f90:404-483);
-    * GNF[a] = <<D_a ; d_a+>>: compute_greenAB with apply_N_Cup and apply_Cup
-      (green_class_computeAB.f90:23-197), d_(c, 0) = c_up, d_(c, 1) = c+_dn.  For a real H
-      <<D ; d+>> = <<d ; D+>>, and it comes from the symmetric combination D + d as the
-      reference's real build does: GNF = (G_(D+d) - GN - G) / 2.
-    Returns (GN, GNF, G), each of shape (2n, nw), G the diagonal Nambu G^N of the same runs.
-    States from get_eigen_energies_sz with the same pairing; a state's runs are built on the
-    device and batched (ed_dynamic_from_state with ED_SITE_NCOND terms)."""
+    * GNF[0, a] = <<D_a ; d_a+>> and GNF[1, a] = <<d_a ; D_a+>>: compute_greenAB with
+      apply_N_Cup and apply_Cup (green_class_computeAB.f90:23-197; its greenAB and greenBA),
+      d_(c, 0) = c_up, d_(c, 1) = c+_dn.  A real H has <<D ; d+>> = <<d ; D+>> = (G_(D+d) -
+      GN - G) / 2 from the symmetric combination D + d, as the reference's real build; a
+      complex model (im, or complex pairing) adds the runs of D - i d and splits the two
+      as compute_green_matrix does.
+    Returns (GN, GNF, G): GN and G of shape (2n, nw), GNF of shape (2, 2n, nw); G is the
+    diagonal Nambu G^N of the same runs.  States from get_eigen_energies_sz with the same
+    model; a real state's runs are built on the device and batched (ed_dynamic_from_state
+    with ED_SITE_NCOND terms)."""
     orbitals = list(range(model.nc)) if orbitals is None else list(orbitals)
     nambu = [(c, 0) for c in orbitals] + [(c, 1) for c in orbitals]
     n = len(nambu)
     ops = ([[(c, sp, 1.0, True)] for c, sp in nambu] + [[(c, sp, 1.0)] for c, sp in nambu]
            + [[(c, sp, 1.0, True), (c, sp, 1.0)] for c, sp in nambu])
+    cx = _sz_complex(pb, pvbc, im)
+    if cx:
+        ops += [[(c, sp, 1.0, True), (c, sp, -1j)] for c, sp in nambu]
     g = _green_ops(states, beta, wn, ops, nitergreen, weight_cutoff, cutoff_dynamic, precision, full_ed_switch,
-                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath))
+                   *_sz_ops(model, pb, pvbc, pairing_imp_to_bath, im))
     GN, G = g[:n], g[n:2 * n]
-    GNF = 0.5 * (g[2 * n:] - GN - G)
+    S = g[2 * n:3 * n] - GN - G
+    if not cx:
+        GNF = np.stack([0.5 * S, 0.5 * S])
+    else:
+        D = -1j * (g[3 * n:] - GN - G)
+        GNF = np.stack([0.5 * (S + D), 0.5 * (S - D)])
     return GN, GNF, G

@@ -78,6 +78,18 @@ typedef struct ed_sector ed_sector;
 int ed_sector_create_sz(const ed_model *model, const double *pb, const double *pvbc, int pairing_imp_to_bath,
                         int npart, ed_sector **out);
 
+/* new_HAIMsz of a complex model (the COMPLEX(8) build: tab_HAIM2 with complex entries over
+ * the Nambu orbitals, HAIMsz_multc, HAIMsz.f90:188-452; the _complex branches of Nambu_Ec /
+ * Nambu_Eb / Nambu_Vbc, impurity_class.f90:1018-1037, bath_class.f90:956-1035: -Ec_dn^T,
+ * -Eb_dn^T, conj(Pb)^T below the diagonal, -conj(Vbc_dn)).  model holds the real parts;
+ * ec_im[2][nc][nc] (required), eb_im[2][nb][nb], vbc_im[2][nb][nc], pb_im[nb][nb],
+ * pvbc_im[2][nb][nc] the imaginary parts (NULL = 0).  Vectors are complex as for
+ * ed_sector_create_c (planar in the real-typed entry points, COMPLEX(8) in
+ * ed_hmult_c128); FP64 only; ed_sector_set_fly selects HAIMsz_mult_fly_c. */
+int ed_sector_create_sz_c(const ed_model *model, const double *ec_im, const double *eb_im, const double *vbc_im,
+                          const double *pb, const double *pb_im, const double *pvbc, const double *pvbc_im,
+                          int pairing_imp_to_bath, int npart, ed_sector **out);
+
 /* ON_FLY (H_class.f90:84-93 -> HAIMsz_mult_fly, HAIMsz.f90:615-1027) for an S_z sector:
  * on = 1 makes every later H.v generate each row's tab_HAIM2 entries from the Nambu
  * matrices instead of reading the stored table (same entries, same order, same bits);

@@ -344,4 +344,96 @@ def test_sz_ncup_green_vs_physical(gpu, case):
         tol = 1e-10 if kw["full_ed_switch"] else 1e-9
         assert np.max(np.abs(GN - gn_ref)) < tol, kw
         assert np.max(np.abs(GNF - gnf_ref)) < tol, kw
+        assert np.array_equal(GNF[0], GNF[1])   # real H: one symmetric combination
         assert np.max(np.abs(G - np.array([G_ref[a, a] for a in range(2 * nc)]))) < tol, kw
+
+
+
+def complex_sc_model(nc, nb, seed, jhund=0.0):
+    """Complex Ec, Eb, Vbc (oracle.cplx.random_model) with complex Pb and PVbc."""
+    import oracle.cplx as oc
+    m, im = oc.random_model(nc, nb, seed, jhund=jhund, u_off=1.1 if nc > 1 else 1.0)
+    r = np.random.default_rng(seed + 100)
+    pb = r.uniform(-0.4, 0.4, (nb, nb)) + 1j * r.uniform(-0.4, 0.4, (nb, nb))
+    pvbc = r.uniform(-0.3, 0.3, (2, nb, nc)) + 1j * r.uniform(-0.3, 0.3, (2, nb, nc))
+    return m, im, pb, pvbc
+
+
+@pytest.mark.parametrize("case", [(1, 3, 4, True, 0.0), (2, 2, 4, False, 0.4), (2, 3, 5, True, 0.3)])
+def test_complex_sz_tables_hmult_spectrum(gpu, case):
+    """Complex S_z sector (ed_sector_create_sz_c, HAIMsz_multc): the complex tables (real and
+    imaginary parts) bit for bit against oracle/sz.py tab_sz with im; QUART and the Hund
+    ELL as the restatement; ed_hmult_c128 against the restated dense H (conj(h) gather)
+    <= 1e-12 relative; the on-the-fly H.v (HAIMsz_mult_fly_c) bitwise equal to the table;
+    dense ED + shift against the physical spectrum of the complex Hamiltonian with complex
+    pairing <= 1e-10; Lanczos E0 against eigvalsh of the restated H <= 1e-10."""
+    nc, nb, npart, pair, jh = case
+    m, im, pb, pvbc = complex_sc_model(nc, nb, seed=7 + nb + npart, jhund=jh)
+    s = gpu.ComplexSzSector(m, npart, im, pb, pvbc, pair)
+    tab = osz.tab_sz(m, npart, pb, pvbc, pair, im=im)
+    print(f"complex S_z case {case}: dim {s.dim}, nnz {len(tab['rankoff'])}", flush=True)
+    t = s.tables(0)
+    assert np.array_equal(t["states"], tab["states"]) and np.array_equal(t["noff"], tab["noff"])
+    assert np.array_equal(t["rankoff"], tab["rankoff"])
+    assert np.array_equal(t["offdiag"], tab["offdiag"].real)
+    assert np.array_equal(s.tables_im(0), tab["offdiag"].imag)
+    assert np.array_equal(t["diag"], tab["diag"])
+    assert np.array_equal(s.quart(), tab["quart"])
+    hund = osz.hund_sz(m, npart)
+    H = osz.dense_sz(tab, hund)
+    v = np.random.default_rng(npart).standard_normal(s.dim) + 1j * np.random.default_rng(npart + 1).standard_normal(s.dim)
+    w = s.hmult(v)
+    assert np.max(np.abs(w - H @ v)) < 1e-12 * np.max(np.abs(H @ v))
+    s.set_fly(True)
+    assert np.array_equal(s.hmult(v), w)
+    s.set_fly(False)
+    ev, vec = s.full_ed(min(6, s.dim))
+    e_phys = osz.physical_spectrum(m, npart, pb, pvbc, pair, im=im)
+    assert np.max(np.abs(ev + s.energy_shift - e_phys[:len(ev)])) < 1e-10
+    for k in range(len(ev)):
+        assert np.linalg.norm(s.hmult(vec[k]) - ev[k] * vec[k]) < 1e-10
+    r = s.lanczos(nitermax=min(s.dim - 1, 200), neigen=1, tolerance=1e-13)
+    w_ref = np.linalg.eigvalsh(H)
+    assert abs(r.evals[0] - w_ref[0]) < 1e-10
+    psi = s.ground_state(r)
+    assert np.linalg.norm(H @ psi - w_ref[0] * psi) < 1e-6
+    s.close()
+
+
+def test_complex_sz_apply_c_and_green(gpu):
+    """Complex S_z solver: c / c+ and the correlated excitation of every Nambu orbital on
+    planar vectors bit for bit against the restatement; the Nambu G^N (G_ij != G_ji) and the
+    FLAG_NCUP GN, GNF (both orders) against the physical Lehmann sums of the complex
+    Hamiltonian with complex pairing, through the Lehmann path and excitation Lanczos."""
+    from ed_solver_amd.solver import compute_green_nambu, compute_green_ncup, get_eigen_energies_sz
+    m, im, pb, pvbc = complex_sc_model(2, 1, seed=41, jhund=0.3)
+    pair = True
+    npart = m.ns
+    src = gpu.ComplexSzSector(m, npart, im, pb, pvbc, pair)
+    r = src.lanczos(nitermax=src.dim - 1, neigen=1, tolerance=1e-13)
+    psi = src.ground_state(r)
+    for pm in (+1, -1):
+        dst = gpu.ComplexSzSector(m, npart + pm, im, pb, pvbc, pair)
+        for spin in (0, 1):
+            for site in range(m.nc):
+                assert np.array_equal(gpu.apply_C(src, dst, site, spin, pm, psi),
+                                      osz.apply_c_sz(m, npart, site, spin, pm, psi))
+                assert np.array_equal(gpu.apply_C_n(src, dst, site, spin, pm, psi),
+                                      osz.apply_c_n_sz(m, npart, site, spin, pm, psi))
+        dst.close()
+    src.close()
+    beta = 5.0
+    wn = np.pi / beta * (2 * np.arange(1, 13) - 1)
+    dmax = max(osz.sz_dim(m.ns, n) for n in range(2 * m.ns + 1))
+    states = get_eigen_energies_sz(m, pb, pvbc, pair, full_ed_switch=dmax, full_ed_green=True, im=im)
+    assert len(states) == 4 ** m.ns
+    G_ref = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair, im=im)
+    gn_ref, gnf_ref = osz.ncup_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair, im=im)
+    assert np.max(np.abs(G_ref[0, 1] - G_ref[1, 0])) > 1e-4
+    for kw in (dict(full_ed_switch=dmax), dict(nitergreen=dmax, full_ed_switch=0)):
+        tol = 1e-10 if kw["full_ed_switch"] else 1e-9
+        G = compute_green_nambu(m, states, beta, wn, pb, pvbc, pair, im=im, **kw)
+        assert np.max(np.abs(G - G_ref)) < tol, kw
+        GN, GNF, Gd = compute_green_ncup(m, states, beta, wn, pb, pvbc, pair, im=im, **kw)
+        assert np.max(np.abs(GN - gn_ref)) < tol, kw
+        assert np.max(np.abs(GNF - gnf_ref)) < tol, kw

@@ -53,8 +53,8 @@ def test_plant_a_mistake_is_caught():
     m, pb, pvbc = sc_model(2, 2, 2.0, seed=5, u_offdiag=1.2, spin_split=0.05)
     good = osz.nambu_matrices
 
-    def wrong(model, pb=None, pvbc=None, pairing_imp_to_bath=False):
-        ecn, ebn, vbn = good(model, pb, pvbc, pairing_imp_to_bath)
+    def wrong(model, pb=None, pvbc=None, pairing_imp_to_bath=False, im=None):
+        ecn, ebn, vbn = good(model, pb, pvbc, pairing_imp_to_bath, im)
         ebn[model.nb:, model.nb:] *= -1.0
         return ecn, ebn, vbn
 
@@ -192,7 +192,8 @@ def test_ncup_green_restated_equals_physical(nc, nb, jh, pair_ib):
     G = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib)
     for a in range(2 * nc):   # the projector matters: GN differs from G and from GNF
         assert np.max(np.abs(gn_p[a] - G[a, a])) > 1e-3
-        assert np.max(np.abs(gn_p[a] - gnf_p[a])) > 1e-4
+        assert np.max(np.abs(gn_p[a] - gnf_p[0, a])) > 1e-4
+    assert np.max(np.abs(gnf_p[0] - gnf_p[1])) < 1e-13   # real H: <<D ; d+>> = <<d ; D+>>
     good = osz.apply_c_n_sz
     try:
         def inverted(model, npart, site, spin, pm, psi):
@@ -203,3 +204,85 @@ def test_ncup_green_restated_equals_physical(nc, nb, jh, pair_ib):
     finally:
         osz.apply_c_n_sz = good
     assert np.max(np.abs(bad - gn_p)) > 1e-3
+
+
+def complex_sc_model(nc, nb, seed, jhund=0.0):
+    """A complex model (Hermitian Ec, Eb, complex Vbc; oracle.cplx.random_model) with complex
+    bath pairing Pb and impurity-bath pairing PVbc."""
+    import oracle.cplx as oc
+    m, im = oc.random_model(nc, nb, seed, jhund=jhund, u_off=1.1 if nc > 1 else 1.0)
+    r = np.random.default_rng(seed + 100)
+    pb = r.uniform(-0.4, 0.4, (nb, nb)) + 1j * r.uniform(-0.4, 0.4, (nb, nb))
+    pvbc = r.uniform(-0.3, 0.3, (2, nb, nc)) + 1j * r.uniform(-0.3, 0.3, (2, nb, nc))
+    return m, im, pb, pvbc
+
+
+@pytest.mark.parametrize("nc,nb,npart,pair_ib,jh", [(1, 3, 4, False, 0.0), (1, 3, 5, True, 0.0),
+                                                     (2, 2, 4, True, 0.0), (2, 2, 3, False, 0.4),
+                                                     (2, 2, 4, True, 0.3)])
+def test_complex_nambu_spectrum_equals_physical(nc, nb, npart, pair_ib, jh):
+    """Complex S_z sectors (HAIMsz_multc with COMPLEX(8) tables): the restated Nambu matrices
+    of the _complex branches, the complex tab_HAIM2 over them and the conj(h) gather
+    give a Hermitian H whose spectrum + shift is the physical spectrum of the complex
+    Hamiltonian with complex pairing, written as operator strings on the full Fock space."""
+    m, im, pb, pvbc = complex_sc_model(nc, nb, seed=3 + nb + npart, jhund=jh)
+    tab = osz.tab_sz(m, npart, pb, pvbc, pair_ib, im=im)
+    assert np.iscomplexobj(tab["offdiag"]) and np.max(np.abs(tab["offdiag"].imag)) > 1e-3
+    H = osz.dense_sz(tab, osz.hund_sz(m, npart))
+    assert np.allclose(H, np.conj(H.T), atol=1e-14)
+    e_n = np.linalg.eigvalsh(H) + osz.energy_shift(m)
+    e_p = osz.physical_spectrum(m, npart, pb, pvbc, pair_ib, im=im)
+    assert np.max(np.abs(e_n - e_p)) < 1e-11
+    v = np.random.default_rng(npart).standard_normal(len(e_n)) * (1 + 0.5j)
+    assert np.max(np.abs(osz.hmult_sz(tab, v, osz.hund_sz(m, npart)) - H @ v)) < 1e-12
+
+
+@pytest.mark.parametrize("plant", ["no_conj_pairing", "conj_dn_ec", "no_conj_vbc_dn"])
+def test_complex_nambu_planted_mistakes(plant):
+    """Plausible slips in the complex Nambu matrices change the spectrum: Pb^T below the
+    diagonal instead of conj(Pb)^T, conj(-Ec_dn^T) instead of -Ec_dn^T, -Vbc_dn
+    instead of -conj(Vbc_dn)."""
+    m, im, pb, pvbc = complex_sc_model(2, 2, seed=17)
+    good = osz.nambu_matrices
+    nc, nb = m.nc, m.nb
+    import oracle.cplx as oc
+
+    def wrong(model, pb=None, pvbc=None, pairing_imp_to_bath=False, im=None):
+        ecn, ebn, vbn = good(model, pb, pvbc, pairing_imp_to_bath, im)
+        if plant == "no_conj_pairing":
+            ebn[nb:, :nb] = np.asarray(pb).T
+        elif plant == "conj_dn_ec":
+            ecn[nc:, nc:] = np.conj(ecn[nc:, nc:])
+        else:
+            vbn[nb:, nc:] = -oc.complex_matrices(model, im, 1)[2]
+        return ecn, ebn, vbn
+
+    e_p = osz.physical_spectrum(m, 4, pb, pvbc, True, im=im)
+    try:
+        osz.nambu_matrices = wrong
+        tab = osz.tab_sz(m, 4, pb, pvbc, True, im=im)
+    finally:
+        osz.nambu_matrices = good
+    H = osz.dense_sz(tab)
+    if np.allclose(H, np.conj(H.T)):
+        e_n = np.linalg.eigvalsh(H) + osz.energy_shift(m)
+        assert np.max(np.abs(e_n - e_p)) > 1e-4
+
+
+@pytest.mark.parametrize("nc,nb,jh,pair_ib", [(1, 2, 0.0, True), (2, 1, 0.4, False)])
+def test_complex_nambu_and_ncup_green_restated_equal_physical(nc, nb, jh, pair_ib):
+    """Complex model with complex pairing: the Nambu G^N and the FLAG_NCUP GN / GNF from the
+    restated complex S_z sectors equal the Lehmann sums of the physical complex Hamiltonian;
+    G_ij != G_ji and <<D ; d+>> != <<d ; D+>> there, so both orders are needed."""
+    m, im, pb, pvbc = complex_sc_model(nc, nb, seed=31 + nc + nb, jhund=jh)
+    beta = 5.0
+    wn = np.pi / beta * (2 * np.arange(1, 9) - 1)
+    Gr = osz.nambu_green_restated(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib, im=im)
+    Gp = osz.nambu_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib, im=im)
+    assert np.max(np.abs(Gr - Gp)) < 1e-12
+    assert np.max(np.abs(Gp[0, 1] - Gp[1, 0])) > 1e-4
+    gn_r, gnf_r = osz.ncup_green_restated(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib, im=im)
+    gn_p, gnf_p = osz.ncup_green_physical(m, beta, wn, pb=pb, pvbc=pvbc, pairing_imp_to_bath=pair_ib, im=im)
+    assert np.max(np.abs(gn_r - gn_p)) < 1e-12
+    assert np.max(np.abs(gnf_r - gnf_p)) < 1e-12
+    assert np.max(np.abs(gnf_p[0] - gnf_p[1])) > 1e-5

@@ -283,6 +283,22 @@ def main():
         f"(+psi {t_gs32b*1e3:.1f}) (|dE0| {mixed_1e8['abs_dE0_mixed']:.1e}); Rayleigh form E0 {t_erqb*1e3:.1f} ms "
         f"({t_e64b / t_erqb:.2f}x, accepted {rq_acc}, |dE0| {mixed_1e8['rayleigh_form']['abs_dE0']:.1e})")
 
+    # ARPACK_diago at (6,6): the implicitly restarted Lanczos on the device (basis, H.v,
+    # reorthogonalisation, restart rotation in HBM) against SciPy's ARPACK driving the
+    # device H.v through host vectors (two PCIe copies per H.v, as Hmultr); 4 lowest, ncv 24
+    from ed_solver_amd.arpack import arpack_diago
+    arp = {}
+    for backend in ("device", "scipy"):
+        arpack_diago(sec, 5, tolerance=1e-12, ncv=24, which="SA", backend=backend)   # warm-up
+        t0 = time.perf_counter()
+        ra = arpack_diago(sec, 5, tolerance=1e-12, ncv=24, which="SA", backend=backend)
+        arp[backend] = {"ms": (time.perf_counter() - t0) * 1e3, "matvecs": ra.matvecs, "nconv": ra.nconv,
+                        "E": [float(x) for x in ra.evals], "max_rel_resid": float(ra.rel_resid.max())}
+    arp["device_vs_scipy"] = arp["scipy"]["ms"] / arp["device"]["ms"]
+    arp["abs_dE0_vs_lanczos"] = abs(arp["device"]["E"][0] - float(r64.evals[0]))
+    log(f"ARPACK_diago (4 lowest, ncv 24): device {arp['device']['ms']:.1f} ms ({arp['device']['matvecs']} H.v), "
+        f"SciPy over host vectors {arp['scipy']['ms']:.1f} ms ({arp['scipy']['matvecs']} H.v)")
+
     scan = None
     if not args.no_scan:
         # every sector of the Ns=12 model with Boltzmann weight > 1e-6 at beta = 100, 8 states
@@ -297,7 +313,8 @@ def main():
         scan = {"sectors": info["sectors"], "kept_sectors": [list(k) for k in info["kept_sectors"]],
                 "states": info["states"], "E0": info["E0"], "t_lowest_all_sectors_s": info["t_pass1_s"],
                 "t_block_lanczos_s": info["t_pass2_s"], "t_total_s": t_scan, "t_green_s": t_g,
-                "G_iw0": [float(G[0, 0].real), float(G[0, 0].imag)]}
+                "block_steps": info.get("block_steps"), "block_full_passes": info.get("block_full_passes"),
+                "green_runs_one_call": True, "G_iw0": [float(G[0, 0].real), float(G[0, 0].imag)]}
         log(f"sector scan: {info['sectors']} sectors in {info['t_pass1_s']:.2f} s, block Lanczos on "
             f"{len(info['kept_sectors'])} sectors ({info['states']} states) in {info['t_pass2_s']:.2f} s, "
             f"G in {t_g:.2f} s")
@@ -441,6 +458,7 @@ def main():
         "green": green,
         "speedup_vs_cpu": (it64 / cpu["value"]) if cpu else None,
         "sector_scan": scan,
+        "arpack": arp,
         "gradients_ns14": grad,
     }
     if cpu and green:

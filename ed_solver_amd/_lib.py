@@ -37,7 +37,7 @@ class LanczosOpts(ctypes.Structure):
 
 class BlockOpts(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int), ("nev", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("tolerance", ctypes.c_double)]
+                ("check_every", ctypes.c_int), ("tolerance", ctypes.c_double), ("reorth", ctypes.c_int)]
 
 
 class ArpackOpts(ctypes.Structure):
@@ -105,6 +105,7 @@ SIGNATURES = {
     "ed_tridiag_lowest": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_band_eig": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_block_default_opts": (ctypes.c_int, [ctypes.POINTER(BlockOpts)]),
+    "ed_block_last_stats": (ctypes.c_int, [_c_int_p, _c_int_p]),
     "ed_arpack_default_opts": (ctypes.c_int, [ctypes.POINTER(ArpackOpts)]),
     "ed_arpack": (ctypes.c_int, [_vp, ctypes.POINTER(ArpackOpts), _dp, _dp, _dp, _dp, ctypes.c_int64, _c_int_p,
                                  _c_int_p, _c_int_p]),
@@ -137,6 +138,10 @@ SIGNATURES = {
     "ed_dynamic_from_state": (ctypes.c_int, [_vp, _dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp), _c_int_p,
                                              _c_int_p, _c_int_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "ed_dynamic_from_states": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_dp), _dp,
+                                              ctypes.c_int, _c_int_p, ctypes.POINTER(_vp), _c_int_p, _c_int_p,
+                                              _c_int_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_hmult_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "ed_lanczos_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ed_lanczos_benchmark": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,

@@ -30,27 +30,33 @@ class BlockResult:
     vectors: np.ndarray    # VECP, shape (nev, dim)
     nconv: int             # ISTOR(67)
     steps: int             # ISTOR(19)
+    full_passes: int = 0   # steps that projected against every earlier block
 
 
 def block_opts(nblock: int = 8, neig: int = 8, nitermax: int = 128, tolerance: float = 1e-12,
-               check_every: int = 2) -> BlockOpts:
+               check_every: int = 2, reorth: str = "partial") -> BlockOpts:
     o = BlockOpts()
     check(lib().ed_block_default_opts(ctypes.byref(o)))
     o.block_size, o.nev, o.max_steps, o.tolerance, o.check_every = nblock, neig, nitermax, tolerance, check_every
+    o.reorth = {"partial": 1, "full": 0}[reorth]
     return o
 
 
 def block_lanczos(sec: Sector, nblock: int = 8, neig: int = 8, nitermax: int = 128, tolerance: float = 1e-12,
-                  check_every: int = 2, dEmax: float | None = None, vectors: bool = True):
+                  check_every: int = 2, dEmax: float | None = None, vectors: bool = True, reorth: str = "partial"):
     """(evals, vectors) of the lowest `neig` states; with dEmax, only those within dEmax of
-    the lowest (Block_Lanczos.f90:300-320).  The full result is in `.last`."""
+    the lowest (Block_Lanczos.f90:300-320).  reorth: "partial" (the block omega-recurrence
+    decides when to project against every earlier block, as BLZPACK) or "full" (every
+    step).  The full result is in `.last`."""
     nev = min(neig, sec.dim)
-    o = block_opts(min(nblock, sec.dim), nev, nitermax, tolerance, check_every)
+    o = block_opts(min(nblock, sec.dim), nev, nitermax, tolerance, check_every, reorth)
     ev, res = np.zeros(nev), np.zeros(nev)
     vec = np.zeros((nev, sec.dim)) if vectors else None
     nconv, steps = ctypes.c_int(), ctypes.c_int()
     check(lib().ed_block_lanczos(sec.handle, ctypes.byref(o), _dptr(ev), _dptr(res),
                                  _dptr(vec) if vectors else None, sec.dim, ctypes.byref(nconv), ctypes.byref(steps)))
-    block_lanczos.last = BlockResult(ev, res, vec, nconv.value, steps.value)
+    full = ctypes.c_int()
+    check(lib().ed_block_last_stats(None, ctypes.byref(full)))
+    block_lanczos.last = BlockResult(ev, res, vec, nconv.value, steps.value, full.value)
     keep = np.ones(nev, bool) if dEmax is None else (ev - ev.min() <= dEmax)
     return ev[keep], (vec[keep] if vectors else None)

@@ -682,10 +682,36 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
     // the jobs the multi-run persistent kernel takes run in one launch of it; the rest (or
     // all, when it does not take them or the runtime refuses it) in the two-launch batch
     std::vector<double> ab((size_t)nb * niter), bb((size_t)nb * niter);
-    std::vector<int> grp[2];
-    for (int i = 0; i < nb; ++i) grp[runs_capable(bs[i]) ? 0 : 1].push_back(i);
+    // groups: the runs the multi-run kernel takes, cut (in order) into launches that each fit
+    // one workgroup per CU at kRunMaxRounds rounds of kRunCpw columns; then the rest
+    std::vector<std::vector<int>> grp;
+    std::vector<int> rest;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 0;
+        (void)hipGetLastError();
+        const int gmax = std::min(ncu, kBarGroups * kBarGroupSize);
+        int used = 0;
+        for (int i = 0; i < nb; ++i) {
+            const int need = (bs[i]->dn.d + kRunCpw * kRunMaxRounds - 1) / (kRunCpw * kRunMaxRounds);
+            if (!runs_capable(bs[i]) || need > gmax) {
+                rest.push_back(i);
+                continue;
+            }
+            if (grp.empty() || used + need > gmax) {
+                grp.emplace_back();
+                used = 0;
+            }
+            grp.back().push_back(i);
+            used += need;
+        }
+    }
+    const int npersist = (int)grp.size();
+    if (!rest.empty()) grp.push_back(rest);
     int rc = ED_OK;
-    for (int gi = 0; gi < 2; ++gi) {
+    for (int gi = 0; gi < (int)grp.size(); ++gi) {
         if (grp[gi].empty()) continue;
         std::vector<ed_sector *> gs;
         std::vector<const double *> gp;
@@ -696,7 +722,7 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
             gn.push_back(bn[i]);
         }
         rc = kRunsNotUsed;
-        if (gi == 0)
+        if (gi < npersist)
             rc = precision == ED_PREC_F64 ? run_batch_persist<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
                                           : run_batch_persist<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
         if (rc == kRunsNotUsed)
@@ -773,36 +799,57 @@ __global__ void k_axpy(double s, const double *x, double *y, int64_t n)
         y[i] += s * x[i];
 }
 
-extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs, ed_sector *const *dst,
-                                     const int *iph, const int *nterm, const int *sites, const double *coefs, int spin,
-                                     int nitergreen, double cutoff_dynamic, int precision, int nw, const double *wn,
-                                     double *dyn)
+// The excitation runs of several eigenstates at once: every state's vector copied to the
+// device once, job k's vector built there from state state_of[k], every run advanced in
+// one batched call (dynamic_batched: one launch of the multi-run persistent kernel for all
+// the runs it takes).
+static int dynamic_from_states(int nst, ed_sector *const *gs, const double *const *psi, const double *energy,
+                               int njobs, const int *state_of, ed_sector *const *dst, const int *iph, const int *nterm,
+                               const int *sites, const double *coefs, int spin, int nitergreen, double cutoff_dynamic,
+                               int precision, int nw, const double *wn, double *dyn)
 {
-    if (!gs || !psi || njobs < 1 || !dst || !iph || !nterm || !sites || !coefs || !dyn)
-        return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad argument");
-    if (gs->cplx) return fail(ED_ERR_ARG, "ed_dynamic_from_state: real sectors only");
-    ED_HIP_CHECK(hipSetDevice(gs->device));
-    hipStream_t st = gs->stream;
-    // psi once on the device, then every excitation vector built there
-    size_t tot = (size_t)gs->dim, maxd = 1;
+    if (nst < 1 || !gs || !psi || !energy || njobs < 1 || !state_of || !dst || !iph || !nterm || !sites || !coefs ||
+        !dyn)
+        return fail(ED_ERR_ARG, "ed_dynamic_from_states: bad argument");
+    for (int q = 0; q < nst; ++q) {
+        if (!gs[q] || !psi[q]) return fail(ED_ERR_ARG, "ed_dynamic_from_states: NULL state sector or vector");
+        if (gs[q]->cplx) return fail(ED_ERR_ARG, "ed_dynamic_from_state: real sectors only");
+        if (gs[q]->device != gs[0]->device) return fail(ED_ERR_ARG, "ed_dynamic_from_states: sectors on different devices");
+    }
+    ED_HIP_CHECK(hipSetDevice(gs[0]->device));
+    hipStream_t st = gs[0]->stream;
+    // every psi once on the device, then every excitation vector built there
+    size_t tot = 0, maxd = 1;
+    std::vector<size_t> o_psi(nst);
+    for (int q = 0; q < nst; ++q) {
+        o_psi[q] = tot;
+        tot += (size_t)gs[q]->dim;
+    }
     for (int k = 0; k < njobs; ++k) {
-        if (!dst[k] || nterm[k] < 1) return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad job");
+        if (!dst[k] || nterm[k] < 1 || state_of[k] < 0 || state_of[k] >= nst)
+            return fail(ED_ERR_ARG, "ed_dynamic_from_states: bad job");
         tot += (size_t)dst[k]->dim;
         maxd = std::max<size_t>(maxd, (size_t)dst[k]->dim);
     }
     double *d = nullptr;
     ED_HIP_CHECK(hipMalloc((void **)&d, sizeof(double) * (tot + maxd)));
-    double *d_psi = d, *d_tmp = d + tot;
+    double *d_tmp = d + tot;
     std::vector<const double *> phis(njobs);
+    std::vector<double> en(njobs);
     int rc = ED_OK;
-    if (hipMemcpyAsync(d_psi, psi, sizeof(double) * gs->dim, hipMemcpyHostToDevice, st) != hipSuccess)
-        rc = fail(ED_ERR_HIP, "ed_dynamic_from_state: copy of psi");
-    size_t off = gs->dim;
+    for (int q = 0; q < nst && rc == ED_OK; ++q)
+        if (hipMemcpyAsync(d + o_psi[q], psi[q], sizeof(double) * gs[q]->dim, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = fail(ED_ERR_HIP, "ed_dynamic_from_states: copy of psi");
+    size_t off = o_psi[nst - 1] + (size_t)gs[nst - 1]->dim;
     int t0 = 0;
     for (int k = 0; k < njobs && rc == ED_OK; ++k) {
+        const int q = state_of[k];
+        const ed_sector *g = gs[q];
+        const double *d_psi = d + o_psi[q];
         double *phi = d + off;
         off += dst[k]->dim;
         phis[k] = phi;
+        en[k] = energy[q];
         const int pm = iph[k] == 1 ? +1 : -1;
         if (iph[k] != 1 && iph[k] != 2) rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: iph must be 1 or 2");
         // sum_t coef_t c^(pm)_{site_t} psi (the vectors of symmetric_combineAA,
@@ -814,23 +861,23 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
             int site = sites[t0 + t], tspin = spin;
             const int ncond = (site & ED_SITE_NCOND) ? 1 : 0;
             site &= ~ED_SITE_NCOND;
-            if (ncond && !gs->sz) {
+            if (ncond && !g->sz) {
                 rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: the correlated excitation exists for S_z sectors only");
                 break;
             }
-            if (gs->sz) {
-                if (site < 0 || site >= 2 * gs->nc) {
+            if (g->sz) {
+                if (site < 0 || site >= 2 * g->nc) {
                     rc = fail(ED_ERR_ARG, "ed_dynamic_from_state: Nambu orbital outside [0, 2 nc)");
                     break;
                 }
-                tspin = site / gs->nc;
-                site %= gs->nc;
+                tspin = site / g->nc;
+                site %= g->nc;
             }
             const double c = coefs[t0 + t];
             if (t == 0 && c == 1.0) {
-                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, phi, st, ncond);
+                rc = apply_c_dev(g, dst[k], site, tspin, pm, d_psi, phi, st, ncond);
             } else {
-                rc = apply_c_dev(gs, dst[k], site, tspin, pm, d_psi, d_tmp, st, ncond);
+                rc = apply_c_dev(g, dst[k], site, tspin, pm, d_psi, d_tmp, st, ncond);
                 if (rc == ED_OK) {
                     if (t == 0) ED_HIP_CHECK(hipMemsetAsync(phi, 0, sizeof(double) * dst[k]->dim, st));
                     hipLaunchKernelGGL(k_axpy, dim3(egrid(dst[k]->dim)), dim3(kBlock), 0, st, c, (const double *)d_tmp,
@@ -842,7 +889,6 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
     }
     if (rc == ED_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ED_ERR_HIP, "ed_dynamic_from_state: apply_C");
     if (rc == ED_OK) {
-        std::vector<double> en(njobs, energy);
         std::vector<ed_sector *> ds(dst, dst + njobs);
         rc = dynamic_batched(njobs, ds.data(), phis.data(), iph, en.data(), nitergreen, cutoff_dynamic, precision, nw,
                              wn, dyn, nullptr, nullptr, true);
@@ -850,5 +896,26 @@ extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double en
     (void)hipDeviceSynchronize();
     (void)hipFree(d);
     return rc;
+}
+
+extern "C" int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int njobs, ed_sector *const *dst,
+                                     const int *iph, const int *nterm, const int *sites, const double *coefs, int spin,
+                                     int nitergreen, double cutoff_dynamic, int precision, int nw, const double *wn,
+                                     double *dyn)
+{
+    if (!gs || !psi || njobs < 1) return fail(ED_ERR_ARG, "ed_dynamic_from_state: bad argument");
+    std::vector<int> state_of(njobs, 0);
+    return dynamic_from_states(1, &gs, &psi, &energy, njobs, state_of.data(), dst, iph, nterm, sites, coefs, spin,
+                               nitergreen, cutoff_dynamic, precision, nw, wn, dyn);
+}
+
+extern "C" int ed_dynamic_from_states(int nstates, ed_sector *const *gs, const double *const *psi,
+                                      const double *energy, int njobs, const int *state_of, ed_sector *const *dst,
+                                      const int *iph, const int *nterm, const int *sites, const double *coefs,
+                                      int spin, int nitergreen, double cutoff_dynamic, int precision, int nw,
+                                      const double *wn, double *dyn)
+{
+    return dynamic_from_states(nstates, gs, psi, energy, njobs, state_of, dst, iph, nterm, sites, coefs, spin,
+                               nitergreen, cutoff_dynamic, precision, nw, wn, dyn);
 }
 

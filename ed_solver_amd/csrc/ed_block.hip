@@ -380,11 +380,130 @@ int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nv
 
 struct BlockResult {
     std::vector<double> evals, resid;
-    int steps = 0, nconv = 0;
+    int steps = 0, nconv = 0, full_passes = 0;
+};
+
+// The block omega-recurrence of partial reorthogonalisation (Simon's recurrence for the
+// loss of orthogonality of Lanczos vectors, in the block form used by block Lanczos codes
+// such as BLZPACK).  With H Q_j = Q_{j+1} B_{j+1} + Q_j A_j + Q_{j-1} B_j^T and
+// Om_{k,i} = Q_k^T Q_i (b x b), the symmetry of H gives for k < j
+//   Om_{k,j+1} B_{j+1} = B_{k+1}^T Om_{k+1,j} + A_k Om_{k,j} + B_k Om_{k-1,j}
+//                        - Om_{k,j} A_j - Om_{k,j-1} B_j^T,
+// with Om_{j,j} = I, Om_{-1,.} = 0.  Rounding seeds the recurrence: every new estimate gets
+// eps (||T|| estimate) ||B_{j+1}^{-1}|| in every entry, with alternating signs, and a block
+// just projected against everything restarts at that level.  cur holds Om_{k,j} (k < j),
+// old Om_{k,j-1}; A_k, B_k are read from T (B_k = T block (k, k-1), upper triangular).
+struct BlockOmega {
+    int b;
+    std::vector<std::vector<double>> cur, old;
+    double tnorm = 0.0;
+    explicit BlockOmega(int b_) : b(b_) {}
+    static double fro(const std::vector<double> &m)
+    {
+        double t = 0.0;
+        for (double x : m) t += x * x;
+        return std::sqrt(t);
+    }
+    double seed = 1e-16;
+    // Om_{k,jn}, k < jn, back to the rounding level: the block was just projected against
+    // every block (Om_{k,jn-1} is kept)
+    void reset(int jn)
+    {
+        cur.assign(jn, std::vector<double>((size_t)b * b, 0.0));
+        for (auto &m : cur)
+            for (int e = 0; e < b * b; ++e) m[e] = (e & 1) ? -seed : seed;
+    }
+    // Om_{k,j+1} for k < j from the recurrence; Om_{j,j+1} at the seed level (the step
+    // projected the block against Q_j).  Returns whether some |Om_{k,j+1}| > sqrt(eps).
+    bool advance(int j, int bs, int nmax, const std::vector<double> &T, const std::vector<double> &Bn, bool was_full)
+    {
+        const double eps = 2.220446049250313e-16;
+        auto blk = [&](int bi, int bj, std::vector<double> &out) {   // T block (bi, bj)
+            out.resize((size_t)bs * bs);
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) out[(size_t)r * bs + c] = T[(size_t)(bi * bs + r) * nmax + bj * bs + c];
+        };
+        auto mul = [&](const std::vector<double> &x, const std::vector<double> &y, bool tx) {
+            std::vector<double> z((size_t)bs * bs, 0.0);   // z = op(x) y, op = transpose when tx
+            for (int r = 0; r < bs; ++r)
+                for (int k = 0; k < bs; ++k) {
+                    const double a = tx ? x[(size_t)k * bs + r] : x[(size_t)r * bs + k];
+                    if (a == 0.0) continue;
+                    for (int c = 0; c < bs; ++c) z[(size_t)r * bs + c] += a * y[(size_t)k * bs + c];
+                }
+            return z;
+        };
+        auto mulT = [&](const std::vector<double> &x, const std::vector<double> &y) {   // x y^T
+            std::vector<double> z((size_t)bs * bs, 0.0);
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) {
+                    double t = 0.0;
+                    for (int k = 0; k < bs; ++k) t += x[(size_t)r * bs + k] * y[(size_t)c * bs + k];
+                    z[(size_t)r * bs + c] = t;
+                }
+            return z;
+        };
+        std::vector<double> Aj, Bj, Ak, Bk, Bk1, I((size_t)bs * bs, 0.0);
+        for (int r = 0; r < bs; ++r) I[(size_t)r * bs + r] = 1.0;
+        blk(j, j, Aj);
+        if (j > 0) blk(j, j - 1, Bj);
+        // ||T|| estimate and ||B_{j+1}^{-1}|| (B upper triangular, back substitution on I)
+        tnorm = std::max(tnorm, fro(Aj) + (j > 0 ? fro(Bj) : 0.0) + fro(Bn));
+        std::vector<double> Binv((size_t)bs * bs, 0.0);
+        for (int c = 0; c < bs; ++c)
+            for (int r = bs - 1; r >= 0; --r) {
+                double t = (r == c) ? 1.0 : 0.0;
+                for (int k = r + 1; k < bs; ++k) t -= Bn[(size_t)r * bs + k] * Binv[(size_t)k * bs + c];
+                Binv[(size_t)r * bs + c] = t / Bn[(size_t)r * bs + r];
+            }
+        seed = eps * tnorm * fro(Binv);
+        std::vector<std::vector<double>> nxt(j + 1);
+        bool over = false;
+        if (!was_full)
+            for (int k = 0; k < j; ++k) {
+                blk(k, k, Ak);
+                blk(k + 1, k, Bk1);                   // B_{k+1}
+                const std::vector<double> &Okj = cur[k];
+                const std::vector<double> &Ok1j = (k + 1 == j) ? I : cur[k + 1];
+                std::vector<double> M = mul(Bk1, Ok1j, true);   // B_{k+1}^T Om_{k+1,j}
+                std::vector<double> t = mul(Ak, Okj, false);
+                for (size_t e = 0; e < M.size(); ++e) M[e] += t[e];
+                if (k > 0) {
+                    blk(k, k - 1, Bk);                // B_k
+                    t = mul(Bk, cur[k - 1], false);
+                    for (size_t e = 0; e < M.size(); ++e) M[e] += t[e];
+                }
+                t = mul(Okj, Aj, false);
+                for (size_t e = 0; e < M.size(); ++e) M[e] -= t[e];
+                if (j > 0) {
+                    const std::vector<double> &Okjm = (k == j - 1) ? I : old[k];
+                    t = mulT(Okjm, Bj);              // Om_{k,j-1} B_j^T
+                    for (size_t e = 0; e < M.size(); ++e) M[e] -= t[e];
+                }
+                std::vector<double> O = mul(M, Binv, false);
+                for (int e = 0; e < bs * bs; ++e) {
+                    O[e] += (e & 1) ? -seed : seed;
+                    if (std::fabs(O[e]) > std::sqrt(eps)) over = true;
+                }
+                nxt[k] = std::move(O);
+            }
+        else
+            for (int k = 0; k < j; ++k) {
+                nxt[k].assign((size_t)bs * bs, 0.0);
+                for (int e = 0; e < bs * bs; ++e) nxt[k][e] = (e & 1) ? -seed : seed;
+            }
+        nxt[j].assign((size_t)bs * bs, 0.0);
+        for (int e = 0; e < bs * bs; ++e) nxt[j][e] = (e & 1) ? -seed : seed;
+        old = cur;
+        cur = std::move(nxt);
+        return over;
+    }
 };
 
 // The block Lanczos driver; Ritz vectors go to d_out (column-major n x nev, leading
 // dimension ldo) when d_out is not null.
+static int g_last_steps = 0, g_last_full = 0;   // the last block Lanczos call (ed_block_last_stats)
+
 static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, int64_t ldo, BlockResult &res)
 {
     const int64_t n = s->dim;
@@ -438,28 +557,43 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     int steps = 0, done = 0;
     bool exhausted = false;
     std::vector<double> lastB;
+    // Partial reorthogonalisation (o.reorth == 1): the estimates Om[k] ~ Q_k^T Q_j (b x b,
+    // k < j) of the block omega-recurrence; a step projects its new block against Q_{j-1}
+    // and Q_j only, unless an estimate exceeds sqrt(eps) (then, and at the step after, against
+    // every block).  o.reorth == 0: against every block at every step (full CGS2).
+    const bool partial = o.reorth == 1;
+    BlockOmega om(b);
+    int full_due = 0;
     for (int j = 0; j < maxsteps && !done; ++j) {
         const int p = (j + 1) * b;   // columns of Q_{1..j+1}
-        double *Qj = Q + (size_t)j * b * ld, *W = Q + (size_t)(j + 1) * b * ld;
+        const bool full = !partial || full_due > 0 || j < 2;
+        const int c0 = full ? 0 : (j - 1) * b;   // first column the new block is projected against
+        const int pw = p - c0;
+        double *Qj = Q + (size_t)j * b * ld, *W = Q + (size_t)(j + 1) * b * ld, *Qc = Q + (size_t)c0 * ld;
         ED_TRY(launch_hv(s, ED_PREC_F64, Qj, W, ld, b, st));
-        G1.assign((size_t)p * b, 0.0);
-        G2.assign((size_t)p * b, 0.0);
+        G1.assign((size_t)pw * b, 0.0);
+        G2.assign((size_t)pw * b, 0.0);
         // CGS2 entirely on the device: the projections stay there between the products
         double *dG1 = dG, *dG2 = dG + nvec * b;
-        ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
-        xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
-        ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
-        xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
+        ED_TRY(gram_dev(dev, n, pw, b, Qc, ld, W, ld, dG1, st));
+        xc_dev(n, pw, b, Qc, ld, dG1, -1.0, 1.0, W, ld, st);
+        ED_TRY(gram_dev(dev, n, pw, b, Qc, ld, W, ld, dG2, st));
+        xc_dev(n, pw, b, Qc, ld, dG2, -1.0, 1.0, W, ld, st);
         ED_HIP_CHECK(hipGetLastError());
-        ED_HIP_CHECK(hipMemcpyAsync(G1.data(), dG1, sizeof(double) * p * b, hipMemcpyDeviceToHost, st));
-        ED_HIP_CHECK(hipMemcpyAsync(G2.data(), dG2, sizeof(double) * p * b, hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipMemcpyAsync(G1.data(), dG1, sizeof(double) * pw * b, hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipMemcpyAsync(G2.data(), dG2, sizeof(double) * pw * b, hipMemcpyDeviceToHost, st));
         steps = j + 1;
+        if (full) {
+            ++res.full_passes;
+            if (full_due > 0) --full_due;
+        }
         bool last = (j + 1 == maxsteps) || (int64_t)p + b > n;
         auto diag_block = [&]() {   // A_j, symmetrised (G1, G2 are on the host after a stream wait)
+            const int r0 = j * b - c0;
             for (int r = 0; r < b; ++r)
                 for (int c = 0; c < b; ++c) {
-                    const double a1 = G1[(size_t)(j * b + r) * b + c] + G2[(size_t)(j * b + r) * b + c];
-                    const double a2 = G1[(size_t)(j * b + c) * b + r] + G2[(size_t)(j * b + c) * b + r];
+                    const double a1 = G1[(size_t)(r0 + r) * b + c] + G2[(size_t)(r0 + r) * b + c];
+                    const double a2 = G1[(size_t)(r0 + c) * b + r] + G2[(size_t)(r0 + c) * b + r];
                     Tat(j * b + r, j * b + c) = 0.5 * (a1 + a2);
                 }
         };
@@ -473,7 +607,37 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
             if (!ok) {
                 exhausted = true;   // the Krylov space is invariant: T is exact on it
                 last = true;
-            } else {
+            } else if (partial) {
+                // omega-recurrence for Q_k^T Q_{j+1}, k < j; above sqrt(eps): project the new
+                // block against every block now (and the next block as well)
+                const bool over = om.advance(j, b, nmax, T, Bnext, full);
+                if (over && !full) {
+                    ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
+                    xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
+                    ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
+                    xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
+                    std::vector<double> R2;
+                    ED_TRY(chol_qr2(dev, n, b, W, ld, R2, ok, st));
+                    if (!ok) {
+                        exhausted = true;
+                        last = true;
+                    } else {
+                        // Q_{j+1} B_{j+1} = (Q' R2) B_{j+1}: the new B is R2 B (upper triangular)
+                        std::vector<double> Bn((size_t)b * b, 0.0);
+                        for (int r = 0; r < b; ++r)
+                            for (int c = r; c < b; ++c) {
+                                double t = 0.0;
+                                for (int k = r; k <= c; ++k) t += R2[(size_t)r * b + k] * Bnext[(size_t)k * b + c];
+                                Bn[(size_t)r * b + c] = t;
+                            }
+                        Bnext = Bn;
+                        om.reset(j + 1);
+                    }
+                    ++res.full_passes;
+                    full_due = 1;
+                }
+            }
+            if (!last) {
                 for (int r = 0; r < b; ++r)
                     for (int c = 0; c < b; ++c) {
                         Tat((j + 1) * b + r, j * b + c) = Bnext[(size_t)r * b + c];
@@ -539,6 +703,8 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         if (res.resid[k] < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))) ++nconv;
     }
     res.nconv = nconv;
+    g_last_steps = res.steps;
+    g_last_full = res.full_passes;
     if (d_out)
         ED_HIP_CHECK(hipMemcpy2DAsync(d_out, sizeof(double) * ldo, X, sizeof(double) * ld, sizeof(double) * n, nev,
                                       hipMemcpyDeviceToDevice, st));
@@ -558,6 +724,7 @@ int ed_block_default_opts(ed_block_opts *o)
     o->max_steps = 128;
     o->check_every = 2;
     o->tolerance = 1e-12;
+    o->reorth = 1;
     return ED_OK;
 }
 
@@ -572,6 +739,7 @@ static int block_check(ed_sector *sec, const ed_block_opts *o)
         return ed::fail(ED_ERR_ARG, "ed_block_lanczos: nev must be 1..16 and at most the sector dimension");
     if (o->max_steps < 1) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: max_steps < 1");
     if (!(o->tolerance > 0.0)) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: tolerance must be positive");
+    if (o->reorth != 0 && o->reorth != 1) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: reorth must be 0 or 1");
     if ((int64_t)o->block_size > sec->dim) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: block_size exceeds the dimension");
     ED_HIP_CHECK(hipSetDevice(sec->device));
     return ED_OK;
@@ -619,6 +787,13 @@ int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *eval
     }
     if (nconv) *nconv = r.nconv;
     if (steps) *steps = r.steps;
+    return ED_OK;
+}
+
+int ed_block_last_stats(int *steps, int *full_passes)
+{
+    if (steps) *steps = ed::g_last_steps;
+    if (full_passes) *full_passes = ed::g_last_full;
     return ED_OK;
 }
 

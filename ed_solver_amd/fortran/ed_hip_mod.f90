@@ -38,6 +38,7 @@ module ed_hip
   type, bind(C) :: ed_block_opts_t
     integer(c_int) :: block_size, nev, max_steps, check_every
     real(c_double) :: tolerance
+    integer(c_int) :: reorth
   end type
 
   interface
@@ -158,6 +159,10 @@ module ed_hip
       real(c_double)                    :: evals(*), resid(*), evecs(*)
       integer(c_int64_t), value         :: ldv
       integer(c_int), intent(out)       :: nconv, steps
+    end function
+    integer(c_int) function ed_block_last_stats(steps, full_passes) bind(C, name="ed_block_last_stats")
+      import :: c_int
+      integer(c_int), intent(out) :: steps, full_passes
     end function
     integer(c_int) function ed_arpack_default_opts(opts) bind(C, name="ed_arpack_default_opts")
       import :: c_int, ed_arpack_opts_t
@@ -522,6 +527,29 @@ module ed_hip
       real(c_double) :: psi(*)
       real(c_double), value :: energy
       integer(c_int), value :: njobs
+      type(c_ptr) :: dst(*)
+      integer(c_int) :: iph(*)
+      integer(c_int) :: nterm(*)
+      integer(c_int) :: sites(*)
+      real(c_double) :: coefs(*)
+      integer(c_int), value :: spin
+      integer(c_int), value :: nitergreen
+      real(c_double), value :: cutoff_dynamic
+      integer(c_int), value :: precision
+      integer(c_int), value :: nw
+      real(c_double) :: wn(*)
+      real(c_double) :: dyn(*)
+    end function
+    ! all excitation runs of several states in one call; psi(q) = c_loc of state q's vector
+    integer(c_int) function ed_dynamic_from_states(nstates, gs, psi, energy, njobs, state_of, dst, iph, nterm, &
+        sites, coefs, spin, nitergreen, cutoff_dynamic, precision, nw, wn, dyn) bind(C, name="ed_dynamic_from_states")
+      import :: c_double, c_int, c_ptr
+      integer(c_int), value :: nstates
+      type(c_ptr) :: gs(*)
+      type(c_ptr) :: psi(*)
+      real(c_double) :: energy(*)
+      integer(c_int), value :: njobs
+      integer(c_int) :: state_of(*)
       type(c_ptr) :: dst(*)
       integer(c_int) :: iph(*)
       integer(c_int) :: nterm(*)

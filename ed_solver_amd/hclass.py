@@ -617,6 +617,31 @@ def dynamic_from_state(gs: Sector, psi: np.ndarray, energy: float, jobs, spin: i
     return d[..., 0] + 1j * d[..., 1]
 
 
+def dynamic_from_states(states, jobs, spin: int, nitergreen: int, wn: np.ndarray, cutoff_dynamic: float = 1e-8,
+                        precision="f64") -> np.ndarray:
+    """The excitation runs of several eigenstates in one call: states is a list of
+    (sector, psi, energy), jobs a list of (state index, dst_sector, iph, [(site, coef), ...]);
+    returns dyn (njobs, nw) complex, unweighted."""
+    nst, nj = len(states), len(jobs)
+    psis = [st[0]._pack(st[1], "dynamic_from_states: psi") for st in states]
+    gs = (ctypes.c_void_p * nst)(*[st[0].handle.value for st in states])
+    pp = (ctypes.POINTER(ctypes.c_double) * nst)(*[dptr(p) for p in psis])
+    en = np.array([float(st[2]) for st in states])
+    wn = np.ascontiguousarray(wn, np.float64)
+    sof = np.array([int(j[0]) for j in jobs], np.int32)
+    ds = (ctypes.c_void_p * nj)(*[j[1].handle.value for j in jobs])
+    iph = np.array([int(j[2]) for j in jobs], np.int32)
+    nterm = np.array([len(j[3]) for j in jobs], np.int32)
+    sites = np.array([int(c) for j in jobs for c, _ in j[3]], np.int32)
+    coefs = np.array([float(a) for j in jobs for _, a in j[3]], np.float64)
+    dyn = np.zeros(2 * nj * len(wn))
+    check(lib().ed_dynamic_from_states(nst, gs, pp, dptr(en), nj, iptr(sof), ds, iptr(iph), iptr(nterm), iptr(sites),
+                                       dptr(coefs), int(spin), int(nitergreen), float(cutoff_dynamic),
+                                       _prec(precision), len(wn), dptr(wn), dptr(dyn)))
+    d = dyn.reshape(nj, len(wn), 2)
+    return d[..., 0] + 1j * d[..., 1]
+
+
 def batched_stats(last=False):
     """(launches of the multi-run persistent kernel, launches the runtime refused) of the
     batched excitation runs since the library was loaded; with last=True also a dict of the

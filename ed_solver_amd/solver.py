@@ -23,7 +23,7 @@ from math import comb
 import numpy as np
 
 from .aim import AIMParams
-from .hclass import (ComplexSector, Sector, apply_C, apply_C_n, compute_dynamic_batched, dynamic_from_state,
+from .hclass import (ComplexSector, Sector, apply_C, apply_C_n, compute_dynamic_batched, dynamic_from_states,
                      dynamic_full_ed)
 
 FULL_ED_SWITCH = 40   # Global_Variables.f90:23
@@ -56,7 +56,7 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
                        tolerance: float = 1e-12, block_size: int = 0, full_ed_switch: int = FULL_ED_SWITCH,
                        precision="f64", which_lanczos: str = "GPU", neigen_arpack: int | None = None,
                        arpack_ncv: int | None = None, full_ed_green: bool = False, im: dict | None = None,
-                       slater=None):
+                       slater=None, on_block=None):
     """Lowest states of every sector (solver.h:98-237), filtered to [E0, E0 + dEmax].
 
     Per sector (solver.h:150-200): block Lanczos when block_size > 0, unless N_up or N_dn
@@ -67,7 +67,8 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
     solver.f90:93), so compute_green's Lehmann sums over small sectors are complete.
     im (imaginary parts of Ec / Eb / Vbc, see make_sector): complex sectors, where dense
     ED and the plain device Lanczos run; block Lanczos and ARPACK are real only.  slater:
-    the 4-index impurity interaction U(s1, s2, s3, s4) added to every sector."""
+    the 4-index impurity interaction U(s1, s2, s3, s4) added to every sector.  on_block(sector,
+    result) is called after each block Lanczos solve (its steps and full projections)."""
     sectors = all_sectors(model.ns) if sectors is None else sectors
     if im is not None and (block_size > 0 or which_lanczos == "ARPACK"):
         raise ValueError("complex sectors: block Lanczos and ARPACK are real only")
@@ -84,6 +85,8 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
                 from .block import block_lanczos
                 ev, vec = block_lanczos(sec, nblock=block_size, neig=neigen, nitermax=nitermax, tolerance=tolerance)
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
+                if on_block is not None:
+                    on_block((nu, nd), block_lanczos.last)
             elif which_lanczos == "ARPACK":
                 from .arpack import arpack_diago
                 r = arpack_diago(sec, neigen_arpack or max(2 * neigen, 2) + 1, tolerance, dEmax, ncv=arpack_ncv)
@@ -131,12 +134,19 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     e0 = min(e_low.values())
     kept = sorted(s for s, e in e_low.items() if np.exp(-beta * (e - e0)) > weight_cutoff)
     t0 = time.perf_counter()
+    steps, fulls = [], []
+
+    def count(_sector, result):
+        steps.append(result.steps)
+        fulls.append(result.full_passes)
+
     states = get_eigen_energies(model, sectors=kept, neigen=neigen, block_size=block_size, nitermax=nitermax,
-                                tolerance=tolerance, full_ed_switch=full_ed_switch)
+                                tolerance=tolerance, full_ed_switch=full_ed_switch, on_block=count)
     t_pass2 = time.perf_counter() - t0
     states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
     info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,
-            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2, "e_low": e_low}
+            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2, "e_low": e_low, "block_steps": steps,
+            "block_full_passes": fulls}
     return states, info
 
 
@@ -170,8 +180,9 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
     the Lehmann sum over the held states, every other run is an excitation Lanczos.  With
     device_terms (real sectors: op -> (spin, [(site index, coef), ...]) of
     ed_dynamic_from_state) a state's excitation vectors are built on the device from one
-    copy of the state; otherwise (complex sectors) apply_C builds them per operator term.
-    Either way a state's runs advance together in one batched device call."""
+    copy of the state, and the runs of all states advance together in one batched device
+    call (ed_dynamic_from_states); otherwise (complex sectors) apply_C builds them per
+    operator term and a state's runs advance together."""
     w = partition(states, beta)
     G = np.zeros((len(ops), len(wn)), complex)
     sectors: dict[tuple, Sector] = {}
@@ -189,6 +200,8 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
             held[key] = (np.array([x.energy for x in sel]), np.array([x.vector for x in sel]).reshape(len(sel), -1))
         return held[key]
 
+    dev_states, dev_jobs, dev_meta = [], [], []   # the device-built runs of every state
+    dev_spin = 0
     try:
         for st, wt in zip(states, w):
             if wt <= weight_cutoff:
@@ -208,18 +221,25 @@ def _green_ops(states: list[EigenState], beta: float, wn: np.ndarray, ops, niter
                     else:
                         jobs.append((io, dst, iph, op))
             if jobs and device_terms is not None:
-                # the state's vector goes to the device once; its excitation vectors are
-                # built there and all of its runs advance together
-                dev = [device_terms(op) for _, _, _, op in jobs]
-                dyn = dynamic_from_state(src, st.vector, st.energy,
-                                         [(d, iph, t) for (_, d, iph, _), (_, t) in zip(jobs, dev)], dev[0][0],
-                                         nitergreen, wn, cutoff_dynamic, precision)
+                # the state's vector goes to the device once and its excitation vectors are
+                # built there; the runs of every state go in one call below
+                q = len(dev_states)
+                dev_states.append((src, st.vector, st.energy))
+                for io, dst, iph, op in jobs:
+                    dev_spin, terms = device_terms(op)
+                    dev_jobs.append((q, dst, iph, terms))
+                    dev_meta.append((io, wt))
             elif jobs:
                 phis = [_excite(src, dst, op, +1 if iph == 1 else -1, st.vector) for _, dst, iph, op in jobs]
                 dyn, _, _ = compute_dynamic_batched([j[1] for j in jobs], phis, [j[2] for j in jobs],
                                                     [st.energy] * len(jobs), nitergreen, wn, cutoff_dynamic,
                                                     precision)
-            for k, (io, *_rest) in enumerate(jobs):
+                for k, (io, *_rest) in enumerate(jobs):
+                    G[io] += wt * dyn[k]
+        if dev_jobs:
+            # every excitation run of every state advances together (ed_dynamic_from_states)
+            dyn = dynamic_from_states(dev_states, dev_jobs, dev_spin, nitergreen, wn, cutoff_dynamic, precision)
+            for k, (io, wt) in enumerate(dev_meta):
                 G[io] += wt * dyn[k]
     finally:
         for s in sectors.values():

@@ -399,9 +399,14 @@ typedef struct ed_block_opts {
     int max_steps;       /* block steps                                          */
     int check_every;     /* Ritz test every this many block steps                */
     double tolerance;
+    int reorth;          /* 1: partial reorthogonalisation (block omega-recurrence;
+                            the new block is projected against all earlier blocks only
+                            when the estimated loss of orthogonality exceeds sqrt(eps),
+                            and at the step after), as BLZPACK; 0: against every
+                            earlier block at every step                              */
 } ed_block_opts;
 
-int ed_block_default_opts(ed_block_opts *opts);   /* 8, 8, 128, 2, 1e-12 */
+int ed_block_default_opts(ed_block_opts *opts);   /* 8, 8, 128, 2, 1e-12, 1 */
 
 /* evals[nev] ascending (VALP(:,1)), resid[nev] = ||H x_k - E_k x_k|| (VALP(:,2), may be
  * NULL), evecs = host D x nev column-major with leading dimension ldv (VECP, may be
@@ -442,6 +447,11 @@ int ed_arpack_default_opts(ed_arpack_opts *opts);   /* 8, 9, BE, 1e-12, 300 */
 
 int ed_arpack(ed_sector *sec, const ed_arpack_opts *opts, const double *start, double *evals, double *resid,
               double *evecs, int64_t ldv, int *nconv, int *iters, int *matvecs);
+
+/* The last ed_block_lanczos call: block steps, and the steps that projected the new block
+ * against every earlier block (all of them with reorth = 0).  No reference counterpart:
+ * it tells a caller what partial reorthogonalisation saved.  Either output may be NULL. */
+int ed_block_last_stats(int *steps, int *full_passes);
 
 /* The contractions of the block orthogonalisation, on device matrices (column-major):
  *   ed_gram_f64_dev:  g (host, row-major p x q) = X^T Y, X n x p, Y n x q, q <= 16
@@ -588,6 +598,19 @@ int ed_dynamic_from_state(ed_sector *gs, const double *psi, double energy, int n
                           const int *sites, const double *coefs, int spin, int nitergreen,
                           double cutoff_dynamic, int precision, int nw, const double *wn,
                           double *dyn);
+
+/* ed_dynamic_from_state for several eigenstates at once (the loop_over_states of
+ * compute_greenAA, green_class_computeAA.h:101-302, with every state's runs in one call):
+ * state q is psi[q] (host, dimen of gs[q]) with energy[q]; job k runs from state
+ * state_of[k] as ed_dynamic_from_state's job k.  All runs advance together: the ones the
+ * multi-run persistent kernel takes in as few cooperative launches as fill the GPU, the
+ * others in the two-launch batch.  Output dyn[k][2*nw] (unweighted; the caller applies
+ * the Boltzmann weights). */
+int ed_dynamic_from_states(int nstates, ed_sector *const *gs, const double *const *psi,
+                           const double *energy, int njobs, const int *state_of, ed_sector *const *dst,
+                           const int *iph, const int *nterm, const int *sites, const double *coefs,
+                           int spin, int nitergreen, double cutoff_dynamic, int precision, int nw,
+                           const double *wn, double *dyn);
 
 /* ----------------------------------------------------------- diagnostics */
 

@@ -65,15 +65,17 @@ def test_mfma_gram_and_update(gpu, n, p, q):
     dX.free(); dY.free()
 
 
-def check_block(gpu, m, nu, nd, nev=8, nblock=8, tol_e=1e-10):
+def check_block(gpu, m, nu, nd, nev=8, nblock=8, tol_e=1e-10, reorth="partial"):
     s = gpu.Sector(m, nu, nd)
     o = oracle.OracleSector(m, nu, nd, nthreads=16)
-    print(f"block Lanczos sector ({nu},{nd}) dim {s.dim}", flush=True)
-    ev, vec = gpu.block_lanczos(s, nblock=nblock, neig=nev, nitermax=200, tolerance=1e-12)
+    print(f"block Lanczos sector ({nu},{nd}) dim {s.dim} reorth {reorth}", flush=True)
+    ev, vec = gpu.block_lanczos(s, nblock=nblock, neig=nev, nitermax=200, tolerance=1e-12, reorth=reorth)
     r = gpu.block_lanczos.last
     w_ref, _ = oracle.lowest_eigs(o, nev)
-    print(f"  steps {r.steps} nconv {r.nconv} max|dE| {np.max(np.abs(ev - w_ref)):.2e} max resid {r.resid.max():.2e}",
-          flush=True)
+    print(f"  steps {r.steps} full passes {r.full_passes} nconv {r.nconv} max|dE| {np.max(np.abs(ev - w_ref)):.2e} "
+          f"max resid {r.resid.max():.2e}", flush=True)
+    if reorth == "full":
+        assert r.full_passes == r.steps
     assert np.max(np.abs(ev - w_ref)) < tol_e
     assert r.resid.max() < 1e-6
     assert r.nconv == nev
@@ -86,23 +88,29 @@ def check_block(gpu, m, nu, nd, nev=8, nblock=8, tol_e=1e-10):
     return ev
 
 
-def test_block_lanczos_small_sector(gpu):
+@pytest.mark.parametrize("reorth", ["partial", "full"])
+def test_block_lanczos_small_sector(gpu, reorth):
     m, (nu, nd) = golden("A")   # dim 400
-    check_block(gpu, m, nu, nd)
+    check_block(gpu, m, nu, nd, reorth=reorth)
 
 
-def test_block_lanczos_workload_b(gpu):
+@pytest.mark.parametrize("reorth", ["partial", "full"])
+def test_block_lanczos_workload_b(gpu, reorth):
     m, (nu, nd) = golden("B")   # dim 63,504
-    check_block(gpu, m, nu, nd)
+    check_block(gpu, m, nu, nd, reorth=reorth)
+    if reorth == "partial":   # the omega-recurrence skipped most full projections
+        r = gpu.block_lanczos.last
+        assert r.full_passes < r.steps
 
 
-def test_block_lanczos_finds_degenerate_pairs(gpu):
+@pytest.mark.parametrize("reorth", ["partial", "full"])
+def test_block_lanczos_finds_degenerate_pairs(gpu, reorth):
     """Two orbitals with identical baths: levels come in orbital-exchange pairs, which a
     single-vector Lanczos cannot resolve and a block of 8 does."""
     eps = np.array([-0.7, -0.7, 0.4, 0.4, 1.1, 1.1])
     v = np.array([0.35, 0.35, 0.5, 0.5, 0.25, 0.25])
     m = kanamori(2, 6, 3.0, 0.5, eps, v)   # Ns=8
-    ev = check_block(gpu, m, 4, 4)
+    ev = check_block(gpu, m, 4, 4, reorth=reorth)
     w_all = np.sort(ev)
     assert np.min(np.diff(w_all)) < 1e-9   # at least one exactly degenerate pair found
 

@@ -237,6 +237,35 @@ def test_lanczos_schedules_workload_c(gpu, sched):
     s.close()
 
 
+@pytest.mark.parametrize("sched", [8, 16])
+def test_lanczos_large_columns_workload_e(gpu, sched):
+    """Workload E (d_up = 3432, more rows than a workgroup has threads: each thread loops
+    over its rows) in natural (8) and XCD-grouped (16) column order: the Lanczos
+    recurrence (16 fixed two-kernel steps) against the oracle's, alpha/beta to 1e-10
+    relative in FP64 and 1e-5 with FP32 vectors; the two orders agree to 1e-12."""
+    m, (nu, nd) = golden("E")
+    s = gpu.Sector(m, nu, nd)
+    try:
+        s.set_schedule(sched)
+    except gpu.EDError:
+        pytest.skip(f"schedule {sched} does not apply to workload E")
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    print(f"E schedule {sched}: dim {s.dim}", flush=True)
+    a, b = s.lanczos_run(16)
+    ro = o.lanczos(nitermax=16, neigen=1, tol=0.0)
+    assert ro["iters"] == 16
+    # relative: beta_1 = ||u_1|| is about 2e3 here (a sum of 11.8M squares in another order)
+    assert np.max(np.abs(a - ro["alpha"][:16]) / np.maximum(1.0, np.abs(ro["alpha"][:16]))) < 1e-10
+    assert np.max(np.abs(b - ro["beta"][:16]) / np.maximum(1.0, np.abs(ro["beta"][:16]))) < 1e-10
+    a32, b32 = s.lanczos_run(16, precision="f32")
+    assert np.max(np.abs(a32 - ro["alpha"][:16])) < 1e-5 * np.max(np.abs(ro["alpha"][:16]))
+    assert np.max(np.abs(b32 - ro["beta"][:16])) < 1e-5 * np.max(np.abs(ro["beta"][:16]))
+    s.set_schedule(24 - sched)
+    a8, _ = s.lanczos_run(16)
+    assert np.max(np.abs(a8 - a)) < 1e-12
+    s.close()
+
+
 def test_lanczos_reorth_b(gpu):
     m, (nu, nd) = golden("B")
     s = gpu.Sector(m, nu, nd)

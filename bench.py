@@ -131,6 +131,9 @@ def main():
     ap.add_argument("--schedule", type=int, default=0,
                     help="force this H.v schedule instead of the autotuner's choice (counter runs, where the "
                          "profiler's serialisation would make the tuner's timings meaningless)")
+    ap.add_argument("--only", choices=["ns14"], default=None,
+                    help="ns14: only the Ns=14 (7,7) H.v and Lanczos step in FP64 and FP32 (for profiler runs), "
+                         "one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one sector (or one batch of sectors) per GPU; "
@@ -142,6 +145,27 @@ def main():
     from ed_solver_amd._lib import check
 
     check(ed.lib().ed_init(0))
+    if args.only == "ns14":
+        # Ns=14 (7,7), dim 11,778,624: the fused H.v (256 launches between an event pair) and
+        # fixed-length two-kernel Lanczos runs of 32 steps, FP64 and FP32, under the tuned
+        # H.v schedule (or --schedule)
+        mE, secE = ed.synthetic("E")
+        sE = ed.Sector(mE, *secE)
+        schedE, msE = sE.tune()
+        if args.schedule:
+            sE.set_schedule(args.schedule)
+            schedE = args.schedule
+        out = {"workload": f"Ns=14: 2-orbital Kanamori + 12 bath sites, sector (7,7), dim {sE.dim}",
+               "schedule": schedE, "schedule_ms": msE}
+        for prec in ("f64", "f32"):
+            ms, by = sE.hmult_benchmark(prec, reps=max(1, args.steps) * 8)
+            rms, rby, _ = sE.lanczos_benchmark(prec, niter=32, reps=max(1, args.steps))
+            out[prec] = {"hv_us": ms * 1e3, "hv_GBps": by / (ms * 1e-3) / 1e9, "step_us": rms / 32 * 1e3,
+                         "step_GBps": rby / (rms * 1e-3) / 1e9}
+            log(f"Ns=14 {prec}: H.v {ms * 1e3:.1f} us, Lanczos step {rms / 32 * 1e3:.1f} us")
+        sE.close()
+        print(json.dumps(out), flush=True)
+        return
     model, (nu, nd) = ed.synthetic("C")
     t0 = time.perf_counter()
     sec = ed.Sector(model, nu, nd)

@@ -613,6 +613,73 @@ static int device_norms(int njobs, ed_sector *const *secs, const double *const *
     return ED_OK;
 }
 
+// Fixed-length recurrences of niter steps from the start vectors bp (device when
+// phi_on_device, norms^2 bn) in the sectors bs, all advancing together: the runs the
+// multi-run persistent kernel takes in cooperative launches cut (in order) so that each
+// fills one workgroup per CU at kRunMaxRounds rounds, then the rest (or a group the
+// runtime refuses) in the two-launch batch.  ab, bb: [run][niter] alpha, beta.
+static int run_fixed(const std::vector<ed_sector *> &bs, const std::vector<const double *> &bp,
+                     const std::vector<double> &bn, int niter, int precision, hipStream_t st, std::vector<double> &ab,
+                     std::vector<double> &bb, bool phi_on_device)
+{
+    const int nb = (int)bs.size();
+    ab.assign((size_t)nb * niter, 0.0);
+    bb.assign((size_t)nb * niter, 0.0);
+    // groups: the runs the multi-run kernel takes, cut (in order) into launches that each fit
+    // one workgroup per CU at kRunMaxRounds rounds of kRunCpw columns; then the rest
+    std::vector<std::vector<int>> grp;
+    std::vector<int> rest;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 0;
+        (void)hipGetLastError();
+        const int gmax = std::min(ncu, kBarGroups * kBarGroupSize);
+        int used = 0;
+        for (int i = 0; i < nb; ++i) {
+            const int need = (bs[i]->dn.d + kRunCpw * kRunMaxRounds - 1) / (kRunCpw * kRunMaxRounds);
+            if (!runs_capable(bs[i]) || need > gmax) {
+                rest.push_back(i);
+                continue;
+            }
+            if (grp.empty() || used + need > gmax) {
+                grp.emplace_back();
+                used = 0;
+            }
+            grp.back().push_back(i);
+            used += need;
+        }
+    }
+    const int npersist = (int)grp.size();
+    if (!rest.empty()) grp.push_back(rest);
+    int rc = ED_OK;
+    for (int gi = 0; gi < (int)grp.size(); ++gi) {
+        if (grp[gi].empty()) continue;
+        std::vector<ed_sector *> gs;
+        std::vector<const double *> gp;
+        std::vector<double> gn, ga, gb;
+        for (int i : grp[gi]) {
+            gs.push_back(bs[i]);
+            gp.push_back(bp[i]);
+            gn.push_back(bn[i]);
+        }
+        rc = kRunsNotUsed;
+        if (gi < npersist)
+            rc = precision == ED_PREC_F64 ? run_batch_persist<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
+                                          : run_batch_persist<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
+        if (rc == kRunsNotUsed)
+            rc = precision == ED_PREC_F64 ? run_batch<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
+                                          : run_batch<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
+        if (rc != ED_OK) return rc;
+        for (size_t q = 0; q < grp[gi].size(); ++q) {
+            std::copy(ga.begin() + q * niter, ga.begin() + (q + 1) * niter, ab.begin() + (size_t)grp[gi][q] * niter);
+            std::copy(gb.begin() + q * niter, gb.begin() + (q + 1) * niter, bb.begin() + (size_t)grp[gi][q] * niter);
+        }
+    }
+    return rc;
+}
+
 static int dynamic_batched(int njobs, ed_sector *const *secs, const double *const *phis, const int *iph,
                            const double *energy, int nitergreen, double cutoff_dynamic, int precision, int nw,
                            const double *wn, double *dyn, double *a, double *b, bool phi_on_device)
@@ -679,61 +746,9 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
     }
     hipStream_t st = bs[0]->stream;
     const int niter = nitergreen;
-    // the jobs the multi-run persistent kernel takes run in one launch of it; the rest (or
-    // all, when it does not take them or the runtime refuses it) in the two-launch batch
-    std::vector<double> ab((size_t)nb * niter), bb((size_t)nb * niter);
-    // groups: the runs the multi-run kernel takes, cut (in order) into launches that each fit
-    // one workgroup per CU at kRunMaxRounds rounds of kRunCpw columns; then the rest
-    std::vector<std::vector<int>> grp;
-    std::vector<int> rest;
-    {
-        int dev = 0, ncu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 0;
-        (void)hipGetLastError();
-        const int gmax = std::min(ncu, kBarGroups * kBarGroupSize);
-        int used = 0;
-        for (int i = 0; i < nb; ++i) {
-            const int need = (bs[i]->dn.d + kRunCpw * kRunMaxRounds - 1) / (kRunCpw * kRunMaxRounds);
-            if (!runs_capable(bs[i]) || need > gmax) {
-                rest.push_back(i);
-                continue;
-            }
-            if (grp.empty() || used + need > gmax) {
-                grp.emplace_back();
-                used = 0;
-            }
-            grp.back().push_back(i);
-            used += need;
-        }
-    }
-    const int npersist = (int)grp.size();
-    if (!rest.empty()) grp.push_back(rest);
-    int rc = ED_OK;
-    for (int gi = 0; gi < (int)grp.size(); ++gi) {
-        if (grp[gi].empty()) continue;
-        std::vector<ed_sector *> gs;
-        std::vector<const double *> gp;
-        std::vector<double> gn, ga, gb;
-        for (int i : grp[gi]) {
-            gs.push_back(bs[i]);
-            gp.push_back(bp[i]);
-            gn.push_back(bn[i]);
-        }
-        rc = kRunsNotUsed;
-        if (gi < npersist)
-            rc = precision == ED_PREC_F64 ? run_batch_persist<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
-                                          : run_batch_persist<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
-        if (rc == kRunsNotUsed)
-            rc = precision == ED_PREC_F64 ? run_batch<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
-                                          : run_batch<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
-        if (rc != ED_OK) return rc;
-        for (size_t q = 0; q < grp[gi].size(); ++q) {
-            std::copy(ga.begin() + q * niter, ga.begin() + (q + 1) * niter, ab.begin() + (size_t)grp[gi][q] * niter);
-            std::copy(gb.begin() + q * niter, gb.begin() + (q + 1) * niter, bb.begin() + (size_t)grp[gi][q] * niter);
-        }
-    }
+    std::vector<double> ab, bb;
+    int rc = run_fixed(bs, bp, bn, niter, precision, st, ab, bb, phi_on_device);
+    if (rc != ED_OK) return rc;
     // shifted tridiagonals: ph (alpha - E) (green_class_compute_dynamic.f90:310-311)
     std::vector<double> sa((size_t)nb * niter), scale(nb);
     for (int i = 0; i < nb; ++i) {
@@ -765,6 +780,79 @@ static int dynamic_batched(int njobs, ed_sector *const *secs, const double *cons
     (void)hipFreeAsync(d, st);
     (void)hipStreamSynchronize(st);
     return rc;
+}
+
+// Lanczos_GPU_updo for many sectors at once (ed_lanczos_batched).
+extern "C" int ed_lanczos_batched(int nsec, ed_sector *const *secs, int niter, int precision, double *alpha,
+                                  double *beta, int *steps)
+{
+    if (nsec < 1 || !secs || niter < 1 || niter > kMaxIter - 2 || !alpha || !beta || !steps)
+        return fail(ED_ERR_ARG, "ed_lanczos_batched: bad argument");
+    if (precision != ED_PREC_F64 && precision != ED_PREC_F32)
+        return fail(ED_ERR_ARG, "ed_lanczos_batched: precision must be ED_PREC_F64 or ED_PREC_F32");
+    for (int k = 0; k < nsec; ++k) {
+        if (!secs[k]) return fail(ED_ERR_ARG, "ed_lanczos_batched: NULL sector");
+        if (secs[k]->device != secs[0]->device) return fail(ED_ERR_ARG, "ed_lanczos_batched: sectors on different devices");
+    }
+    ED_HIP_CHECK(hipSetDevice(secs[0]->device));
+    std::fill(alpha, alpha + (size_t)nsec * niter, 0.0);
+    std::fill(beta, beta + (size_t)nsec * niter, 0.0);
+    // one sector alone: the sector's own fixed-length run (its breakdown step reported)
+    auto alone = [&](int k) -> int {
+        int at = 0;
+        int rc = lanczos_run_dev(secs[k], niter, nullptr, precision, alpha + (size_t)k * niter,
+                                 beta + (size_t)k * niter, &at);
+        if (rc == ED_ERR_BREAKDOWN && at > 0) {
+            steps[k] = at;
+            return ED_OK;
+        }
+        steps[k] = niter;
+        return rc;
+    };
+    std::vector<int> batch;
+    for (int k = 0; k < nsec; ++k) {
+        const int64_t d = secs[k]->cplx ? secs[k]->cdim : secs[k]->dim;
+        if (rows_capable(secs[k]) && d > niter) batch.push_back(k);
+        else ED_TRY(alone(k));
+    }
+    if (batch.empty()) return ED_OK;
+    std::lock_guard<std::mutex> lock(g_ws.mu);
+    const int nb = (int)batch.size();
+    std::vector<ed_sector *> bs(nb);
+    std::vector<size_t> off(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) {
+        bs[i] = secs[batch[i]];
+        off[i + 1] = off[i] + (size_t)bs[i]->dim;
+    }
+    hipStream_t st = bs[0]->stream;
+    // every start vector (create_fix_initial_vector's) on the device, then one batched run
+    double *d = nullptr;
+    ED_HIP_CHECK(hipMalloc((void **)&d, sizeof(double) * off[nb]));
+    std::vector<const double *> bp(nb);
+    int rc = ED_OK;
+    for (int i = 0; i < nb && rc == ED_OK; ++i) {
+        bp[i] = d + off[i];
+        rc = start_vector_dev(bs[i]->dim, d + off[i], st);
+    }
+    std::vector<double> bn(nb, 0.0);
+    if (rc == ED_OK) rc = device_norms(nb, bs.data(), bp.data(), bn);
+    std::vector<double> ab, bb;
+    if (rc == ED_OK) rc = run_fixed(bs, bp, bn, niter, precision, st, ab, bb, true);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (rc == ED_ERR_BREAKDOWN) {   // a Krylov space closed before niter: each sector alone
+        for (int i = 0; i < nb; ++i) ED_TRY(alone(batch[i]));
+        return ED_OK;
+    }
+    if (rc != ED_OK) return rc;
+    for (int i = 0; i < nb; ++i) {
+        const int k = batch[i];
+        std::copy(ab.begin() + (size_t)i * niter, ab.begin() + (size_t)(i + 1) * niter, alpha + (size_t)k * niter);
+        std::copy(bb.begin() + (size_t)i * niter, bb.begin() + (size_t)(i + 1) * niter, beta + (size_t)k * niter);
+        beta[(size_t)k * niter] = std::sqrt(bn[i]);   // subdiag(1) = ||v_1||, as ed_lanczos_run
+        steps[k] = niter;
+    }
+    return ED_OK;
 }
 
 extern "C" int ed_dynamic_batched_stats(int *persistent_launches, int *refusals, double *last_ms,

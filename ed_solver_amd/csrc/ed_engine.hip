@@ -1928,6 +1928,27 @@ int ed_lanczos_run(ed_sector *sec, int niter, const double *start, int precision
     return lanczos_run(sec, niter, start, nullptr, precision, 8, diag, subdiag, nullptr);
 }
 
+int ed_lanczos_converged(int n, const double *alpha, const double *beta, int neigen, double tolerance, int nitermax,
+                         int *iters, double *evals)
+{
+    if (n < 1 || !alpha || !beta || neigen < 1 || nitermax < 1 || !iters || !evals)
+        return fail(ED_ERR_ARG, "ed_lanczos_converged: bad argument");
+    // Niter = min(dimen_H - 1, Nitermax) is the caller's n when the run was that long; the
+    // test reads beta one past the current step, so the copies end with a zero
+    ConvTest ct(neigen, std::min(n, nitermax), tolerance);
+    std::vector<double> a(alpha, alpha + n), b(beta, beta + n);
+    a.push_back(0.0);
+    b.push_back(0.0);
+    int it = 1;
+    for (; it <= std::min(n, nitermax); ++it)
+        if (ct.ends(it, a.data(), b.data())) break;
+    it = std::min(it, std::min(n, nitermax));
+    *iters = it;
+    const int m = std::min(neigen, it);
+    for (int k = 0; k < neigen; ++k) evals[k] = k < m ? ct.w[k] : 0.0;
+    return ED_OK;
+}
+
 int ed_lanczos_gs_vector_dev(ed_sector *sec, int iters, const double *vecp, double *d_psi)
 {
     return gs_vector_dev(sec, iters, vecp, d_psi);

@@ -160,6 +160,24 @@ module ed_hip
       integer(c_int64_t), value         :: ldv
       integer(c_int), intent(out)       :: nconv, steps
     end function
+    ! Lanczos_GPU_updo for many sectors at once; secs(k) = sector handles
+    integer(c_int) function ed_lanczos_batched(nsec, secs, niter, precision, alpha, beta, steps) &
+        bind(C, name="ed_lanczos_batched")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: nsec, niter, precision
+      type(c_ptr)           :: secs(*)
+      real(c_double)        :: alpha(*), beta(*)
+      integer(c_int)        :: steps(*)
+    end function
+    ! Lanczos_fast_diagonalize's convergence test on recorded coefficients
+    integer(c_int) function ed_lanczos_converged(n, alpha, beta, neigen, tolerance, nitermax, iters, evals) &
+        bind(C, name="ed_lanczos_converged")
+      import :: c_int, c_double
+      integer(c_int), value       :: n, neigen, nitermax
+      real(c_double)              :: alpha(*), beta(*), evals(*)
+      real(c_double), value       :: tolerance
+      integer(c_int), intent(out) :: iters
+    end function
     integer(c_int) function ed_block_last_stats(steps, full_passes) bind(C, name="ed_block_last_stats")
       import :: c_int
       integer(c_int), intent(out) :: steps, full_passes

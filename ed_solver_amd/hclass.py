@@ -642,6 +642,28 @@ def dynamic_from_states(states, jobs, spin: int, nitergreen: int, wn: np.ndarray
     return d[..., 0] + 1j * d[..., 1]
 
 
+def lanczos_batched(sectors, niter: int, precision="f64"):
+    """Lanczos_GPU_updo for many sectors at once (ed_lanczos_batched): niter fixed steps from
+    each sector's default start vector, all advancing together.  Returns (alpha, beta,
+    steps) with alpha, beta of shape (nsec, niter) and steps[k] the completed steps."""
+    n = len(sectors)
+    hs = (ctypes.c_void_p * n)(*[s.handle.value for s in sectors])
+    a = np.zeros((n, niter)); b = np.zeros((n, niter)); st = np.zeros(n, np.int32)
+    check(lib().ed_lanczos_batched(n, hs, int(niter), _prec(precision), dptr(a), dptr(b), iptr(st)))
+    return a, b, st
+
+
+def lanczos_converged(alpha, beta, neigen: int = 1, tolerance: float = 1e-12, nitermax: int | None = None):
+    """Lanczos_fast_diagonalize's convergence test replayed on recorded coefficients:
+    (iterations, the neigen lowest eigenvalues of T at that iteration)."""
+    alpha = np.ascontiguousarray(alpha, np.float64); beta = np.ascontiguousarray(beta, np.float64)
+    it = ctypes.c_int()
+    ev = np.zeros(neigen)
+    check(lib().ed_lanczos_converged(len(alpha), dptr(alpha), dptr(beta), int(neigen), float(tolerance),
+                                     int(nitermax or len(alpha)), ctypes.byref(it), dptr(ev)))
+    return it.value, ev
+
+
 def batched_stats(last=False):
     """(launches of the multi-run persistent kernel, launches the runtime refused) of the
     batched excitation runs since the library was loaded; with last=True also a dict of the

@@ -111,25 +111,50 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
 
 
 def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, block_size: int = 8, neigen: int = 8,
-                nitermax: int = 128, tolerance: float = 1e-12, full_ed_switch: int = FULL_ED_SWITCH):
+                nitermax: int = 128, tolerance: float = 1e-12, full_ed_switch: int = FULL_ED_SWITCH,
+                batched: bool = True):
     """Every (N_up, N_dn) sector with Boltzmann weight above weight_cutoff, neigen lowest
     states each (solver.h:98-237 over the whole Fock space).
 
-    Pass 1: the lowest energy of every sector (dense ED or device Lanczos).  Pass 2: the
-    sectors with e^{-beta (E_s - E0)} > weight_cutoff get `neigen` states by block Lanczos
-    of width `block_size` (plain Lanczos in the sectors solver.h:152 excludes); states
-    below the weight cutoff are dropped.  Returns (states, info)."""
+    Pass 1: the lowest energy of every sector: dense ED at dimension <= full_ed_switch;
+    the others' Lanczos_fast_diagonalize all at once when `batched` (min(Nitermax, dim - 1)
+    fixed steps of every sector in one batched device call, ed_lanczos_batched, then the
+    reference's convergence test replayed on each sector's coefficients,
+    ed_lanczos_converged: the same iteration and Ritz value as the sector's own loop up to
+    rounding), else sector by sector.  Pass 2: the sectors with e^{-beta (E_s - E0)} >
+    weight_cutoff get `neigen` states by block Lanczos of width `block_size` (plain Lanczos
+    in the sectors solver.h:152 excludes); states below the weight cutoff are dropped.
+    Returns (states, info)."""
     import time
+    from .hclass import lanczos_batched, lanczos_converged
     ns = model.ns
     t0 = time.perf_counter()
     e_low = {}
+    lanczos_keys = []
     for (nu, nd) in all_sectors(ns):
         dim = comb(ns, nu) * comb(ns, nd)
-        with Sector(model, nu, nd) as sec:
-            if dim <= full_ed_switch:
+        if dim <= full_ed_switch:
+            with Sector(model, nu, nd) as sec:
                 e_low[(nu, nd)] = float(sec.full_ed(1)[0][0])
-            else:
+        elif batched:
+            lanczos_keys.append((nu, nd))
+        else:
+            with Sector(model, nu, nd) as sec:
                 e_low[(nu, nd)] = float(sec.lanczos(nitermax=nitermax, neigen=1, tolerance=tolerance).evals[0])
+    # batched pass 1, grouped by run length Niter = min(Nitermax, dimen - 1)
+    groups: dict[int, list] = {}
+    for key in lanczos_keys:
+        groups.setdefault(min(nitermax, comb(ns, key[0]) * comb(ns, key[1]) - 1), []).append(key)
+    for niter, keys in groups.items():
+        secs = [Sector(model, *k) for k in keys]
+        try:
+            a, b, steps = lanczos_batched(secs, niter)
+        finally:
+            for sec in secs:
+                sec.close()
+        for i, key in enumerate(keys):
+            _, ev = lanczos_converged(a[i, :steps[i]], b[i, :steps[i]], 1, tolerance, nitermax)
+            e_low[key] = float(ev[0])
     t_pass1 = time.perf_counter() - t0
     e0 = min(e_low.values())
     kept = sorted(s for s, e in e_low.items() if np.exp(-beta * (e - e0)) > weight_cutoff)

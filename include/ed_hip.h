@@ -612,6 +612,25 @@ int ed_dynamic_from_states(int nstates, ed_sector *const *gs, const double *cons
                            int spin, int nitergreen, double cutoff_dynamic, int precision, int nw,
                            const double *wn, double *dyn);
 
+/* Lanczos_GPU_updo (Lanczos_gpu.f90:106) for many sectors at once: niter fixed steps of
+ * the recurrence from each sector's create_fix_initial_vector start (rcvector_class.f90:
+ * 43-60, as ed_start_vector), all sectors advancing together (the multi-run persistent
+ * kernel where the sectors fit it, cut into launches that fill the GPU; the two-launch
+ * batch otherwise; a sector of dimension <= niter, or one neither batch kernel takes,
+ * runs alone).  alpha[k][niter], beta[k][niter] as ed_lanczos_run (beta[k][0] = ||v_1||);
+ * steps[k] = niter, or the step at which sector k's Krylov space closed (beta below the
+ * breakdown threshold; the coefficients after it are zero).  FP64 or FP32 vectors. */
+int ed_lanczos_batched(int nsec, ed_sector *const *secs, int niter, int precision, double *alpha,
+                       double *beta, int *steps);
+
+/* The convergence test of Lanczos_fast_diagonalize (Lanczos_plain.f90:33-134) replayed on
+ * recorded coefficients alpha[n], beta[n] (as ed_lanczos_run returns them): the first
+ * iteration j <= min(n, nitermax) at which every one of the neigen lowest eigenvalues of T_j
+ * moved by less than tolerance since T_{j-1} (j > 2), or min(n, nitermax); *iters = j and
+ * evals[neigen] = those eigenvalues of T_j (ascending).  Host only. */
+int ed_lanczos_converged(int n, const double *alpha, const double *beta, int neigen, double tolerance,
+                         int nitermax, int *iters, double *evals);
+
 /* ----------------------------------------------------------- diagnostics */
 
 /* Launches of the multi-run persistent kernel of ed_dynamic_batched /

@@ -164,3 +164,70 @@ def test_runs_kernel_breakdown_is_an_error_and_drains(gpu):
     o.close()
     assert rel(dyn[0], dyn_o) < 1e-10
     dst.close()
+
+
+def test_lanczos_batched_matches_single_runs(gpu):
+    """ed_lanczos_batched (Lanczos_GPU_updo for many sectors at once) against each sector's
+    own ed_lanczos_run from the same default start vector: sectors of workload C of uneven
+    dimension (the multi-run kernel) and one with d_up < 128 (run alone); alpha/beta to 1e-10
+    relative.  The replayed convergence test gives the per-sector Lanczos E0 to 1e-10, and
+    on a 66-state sector run for Niter = dim - 1 steps its exact lowest eigenvalue."""
+    import json, os
+    from ed_solver_amd.aim import AIMParams
+    from ed_solver_amd.hclass import lanczos_batched, lanczos_converged
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")) as f:
+        m = AIMParams.from_json(json.load(f)["C"]["model"])
+    keys = [(6, 6), (5, 6), (4, 7), (3, 5), (1, 6)]
+    secs = [gpu.Sector(m, *k) for k in keys]
+    niter = 64
+    print(f"batched Lanczos over {[s.dim for s in secs]}", flush=True)
+    a, b, steps = lanczos_batched(secs, niter)
+    assert list(steps) == [niter] * len(keys)
+    for i, s in enumerate(secs):
+        a1, b1 = s.lanczos_run(niter)
+        scale = np.maximum(1.0, np.abs(a1))
+        assert np.max(np.abs(a[i] - a1) / scale) < 1e-10, keys[i]
+        assert np.max(np.abs(b[i] - b1) / np.maximum(1.0, np.abs(b1))) < 1e-10, keys[i]
+        it, ev = lanczos_converged(a[i], b[i], 1, 1e-12, 128)
+        r = s.lanczos(nitermax=niter, neigen=1, tolerance=1e-12)
+        assert abs(ev[0] - r.evals[0]) < 1e-10, keys[i]
+        assert abs(it - r.iters) <= 1, (keys[i], it, r.iters)
+    for s in secs:
+        s.close()
+    # a sector of dimension 66 with d_up = 1 (run alone) and the reference's Niter = dim - 1
+    tiny = gpu.Sector(m, 0, 2)
+    a, b, steps = lanczos_batched([tiny], tiny.dim - 1)
+    assert steps[0] == tiny.dim - 1
+    it, ev = lanczos_converged(a[0, :steps[0]], b[0, :steps[0]], 1, 1e-12, 128)
+    assert abs(ev[0] - tiny.full_ed(1)[0][0]) < 1e-10
+    tiny.close()
+
+
+def test_dynamic_from_states_matches_per_state(gpu):
+    """ed_dynamic_from_states (the runs of several eigenstates in one call) against one
+    ed_dynamic_from_state call per state: two states of workload B in different sectors, all
+    c / c+ runs of every orbital and a (c_0 + c_1) combination; G to 1e-12 relative."""
+    import json, os
+    from ed_solver_amd.aim import AIMParams
+    from ed_solver_amd.hclass import dynamic_from_state, dynamic_from_states
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")) as f:
+        m = AIMParams.from_json(json.load(f)["B"]["model"])
+    wn = gpu.matsubara(50.0, 256)
+    states, jobs = [], []
+    opened = []
+    for q, (nu, nd) in enumerate([(5, 5), (4, 5)]):
+        s = gpu.Sector(m, nu, nd); opened.append(s)
+        r = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+        states.append((s, s.ground_state(r), float(r.evals[0])))
+        sp = gpu.Sector(m, nu + 1, nd); sh = gpu.Sector(m, nu - 1, nd); opened += [sp, sh]
+        for c in range(m.nc):
+            jobs.append((q, sp, 1, [(c, 1.0)]))
+            jobs.append((q, sh, 2, [(c, 1.0)]))
+        jobs.append((q, sp, 1, [(0, 1.0), (1, 1.0)]))
+    dyn = dynamic_from_states(states, jobs, 0, 36, wn)
+    for q, (s, psi, e) in enumerate(states):
+        mine = [k for k, j in enumerate(jobs) if j[0] == q]
+        ref = dynamic_from_state(s, psi, e, [jobs[k][1:] for k in mine], 0, 36, wn)
+        assert np.max(np.abs(dyn[mine] - ref)) < 1e-12 * np.max(np.abs(ref))
+    for s in opened:
+        s.close()

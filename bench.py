@@ -338,6 +338,7 @@ def main():
                 "states": info["states"], "E0": info["E0"], "t_lowest_all_sectors_s": info["t_pass1_s"],
                 "t_block_lanczos_s": info["t_pass2_s"], "t_total_s": t_scan, "t_green_s": t_g,
                 "block_steps": info.get("block_steps"), "block_full_passes": info.get("block_full_passes"),
+                "block_ms": info.get("block_ms"), "block_ritz_ms": info.get("block_ritz_ms"),
                 "green_runs_one_call": True, "G_iw0": [float(G[0, 0].real), float(G[0, 0].imag)]}
         log(f"sector scan: {info['sectors']} sectors in {info['t_pass1_s']:.2f} s, block Lanczos on "
             f"{len(info['kept_sectors'])} sectors ({info['states']} states) in {info['t_pass2_s']:.2f} s, "

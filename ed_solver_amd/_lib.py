@@ -105,7 +105,7 @@ SIGNATURES = {
     "ed_tridiag_lowest": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "ed_band_eig": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]),
     "ed_block_default_opts": (ctypes.c_int, [ctypes.POINTER(BlockOpts)]),
-    "ed_block_last_stats": (ctypes.c_int, [_c_int_p, _c_int_p]),
+    "ed_block_last_stats": (ctypes.c_int, [_c_int_p, _c_int_p, _dp, _dp]),
     "ed_lanczos_batched": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _dp, _dp,
                                           _c_int_p]),
     "ed_lanczos_converged": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,

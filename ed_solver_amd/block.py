@@ -31,6 +31,8 @@ class BlockResult:
     nconv: int             # ISTOR(67)
     steps: int             # ISTOR(19)
     full_passes: int = 0   # steps that projected against every earlier block
+    ms: float = 0.0        # host wall time of the call
+    ritz_ms: float = 0.0   # of which in the Ritz tests (band solver)
 
 
 def block_opts(nblock: int = 8, neig: int = 8, nitermax: int = 128, tolerance: float = 1e-12,
@@ -55,8 +57,8 @@ def block_lanczos(sec: Sector, nblock: int = 8, neig: int = 8, nitermax: int = 1
     nconv, steps = ctypes.c_int(), ctypes.c_int()
     check(lib().ed_block_lanczos(sec.handle, ctypes.byref(o), _dptr(ev), _dptr(res),
                                  _dptr(vec) if vectors else None, sec.dim, ctypes.byref(nconv), ctypes.byref(steps)))
-    full = ctypes.c_int()
-    check(lib().ed_block_last_stats(None, ctypes.byref(full)))
-    block_lanczos.last = BlockResult(ev, res, vec, nconv.value, steps.value, full.value)
+    full, ms, rms = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    check(lib().ed_block_last_stats(None, ctypes.byref(full), ctypes.byref(ms), ctypes.byref(rms)))
+    block_lanczos.last = BlockResult(ev, res, vec, nconv.value, steps.value, full.value, ms.value, rms.value)
     keep = np.ones(nev, bool) if dEmax is None else (ev - ev.min() <= dEmax)
     return ev[keep], (vec[keep] if vectors else None)

@@ -29,6 +29,7 @@
 //           reduction over p in MFMA steps of 4.
 // Both are HBM-bound (about b/4 flop per byte); MFMA keeps the instruction count low.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -502,10 +503,15 @@ struct BlockOmega {
 
 // The block Lanczos driver; Ritz vectors go to d_out (column-major n x nev, leading
 // dimension ldo) when d_out is not null.
-static int g_last_steps = 0, g_last_full = 0;   // the last block Lanczos call (ed_block_last_stats)
+// the last block Lanczos call (ed_block_last_stats): steps, full projections, and host
+// wall time of the whole call and of its Ritz tests (band solver)
+static int g_last_steps = 0, g_last_full = 0;
+static double g_last_ms = 0.0, g_last_ritz_ms = 0.0;
 
 static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, int64_t ldo, BlockResult &res)
 {
+    const auto t_start = std::chrono::steady_clock::now();
+    double ritz_ms = 0.0;
     const int64_t n = s->dim;
     const int b = o.block_size, nev = o.nev, dev = s->device;
     hipStream_t st = s->stream;
@@ -648,20 +654,29 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         }
         const int m = p;
         if (m < nev) continue;
-        // Ritz test every check_every steps, thinned to every steps / 8 as T grows (the
+        // Ritz test every check_every steps, thinned to every steps / 4 as T grows (the
         // band solver costs O(m^2 b) per test)
-        if (!last && (steps % std::max(std::max(1, o.check_every), steps / 8)) != 0) continue;
-        // Ritz values and vectors of T_j (leading m x m block)
+        if (!last && (steps % std::max(std::max(1, o.check_every), steps / 4)) != 0) continue;
+        // Ritz values of T_j (leading m x m block); the vectors (for the residual bounds and
+        // the Ritz vectors) only once the values have settled, or at the last step
         std::vector<double> Tm((size_t)m * m);
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
-        Z.assign((size_t)m * nev, 0.0);
-        ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), Z.data()));
+        const auto t_r = std::chrono::steady_clock::now();
+        ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), nullptr));
+        bool settled = steps > 1;
+        for (int k = 0; k < nev && settled; ++k)
+            if (!(std::fabs(w[k] - prev[k]) < o.tolerance)) settled = false;
+        if (last || settled) {
+            Z.assign((size_t)m * nev, 0.0);
+            ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), Z.data()));
+        }
+        ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_r).count();
         if (last) {
             done = 1;
             break;
         }
-        bool conv = steps > 1;
+        bool conv = settled;
         for (int k = 0; k < nev && conv; ++k) {
             // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
             double rr = 0.0;
@@ -705,6 +720,8 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     res.nconv = nconv;
     g_last_steps = res.steps;
     g_last_full = res.full_passes;
+    g_last_ritz_ms = ritz_ms;
+    g_last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     if (d_out)
         ED_HIP_CHECK(hipMemcpy2DAsync(d_out, sizeof(double) * ldo, X, sizeof(double) * ld, sizeof(double) * n, nev,
                                       hipMemcpyDeviceToDevice, st));
@@ -790,10 +807,12 @@ int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *eval
     return ED_OK;
 }
 
-int ed_block_last_stats(int *steps, int *full_passes)
+int ed_block_last_stats(int *steps, int *full_passes, double *ms, double *ritz_ms)
 {
     if (steps) *steps = ed::g_last_steps;
     if (full_passes) *full_passes = ed::g_last_full;
+    if (ms) *ms = ed::g_last_ms;
+    if (ritz_ms) *ritz_ms = ed::g_last_ritz_ms;
     return ED_OK;
 }
 

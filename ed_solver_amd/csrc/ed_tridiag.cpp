@@ -463,44 +463,46 @@ struct Band {
 };
 
 // A (symmetric band, half-bandwidth b) -> tridiagonal by similarity rotations G on the
-// planes (p, p+1), G = [[c, s], [-s, c]]; the rotations are appended to `log` in order.
-// A rotation mixes rows (and columns) p and q = p + 1 at the columns x within b + 1 of
-// either; the 2 x 2 block transforms as G B G^T.
-void band_to_tridiag(Band &A, std::vector<Rot> &log)
+// planes (p, p+1), G = [[c, s], [-s, c]]; the rotations are appended to `log` in order when
+// it is not null (only the eigenvectors need them).  A rotation mixes rows (and columns) p
+// and q = p + 1 at the columns x within b + 1 of either; the 2 x 2 block transforms as
+// G B G^T.  Storage: L[c w + (r - c)] = A[r][c], 0 <= r - c <= b + 1.
+void band_to_tridiag(Band &A, std::vector<Rot> *log)
 {
     const int n = A.n, b = A.b;
-    double *L = A.L.data();
+    double *__restrict__ L = A.L.data();
     const int w = A.w;
-    auto rotate = [&](int p, double c, double s) {
-        const int q = p + 1, lo = std::max(0, q - b - 1), hi = std::min(n - 1, p + b + 1);
-        for (int x = lo; x < p; ++x) {   // A[p][x], A[q][x]: adjacent entries of column x
-            double *colx = L + (size_t)x * w;
-            const double t1 = colx[p - x], t2 = colx[q - x];
-            colx[p - x] = c * t1 + s * t2;
-            colx[q - x] = -s * t1 + c * t2;
-        }
-        double *colp = L + (size_t)p * w, *colq = L + (size_t)q * w;
-        for (int x = q + 1; x <= hi; ++x) {   // A[x][p], A[x][q]: runs down columns p and q
-            const double t1 = colp[x - p], t2 = colq[x - q];
-            colp[x - p] = c * t1 + s * t2;
-            colq[x - q] = -s * t1 + c * t2;
-        }
-        const double app = A.get(p, p), apq = A.get(p, q), aqq = A.get(q, q);
-        A.set(p, p, c * c * app + 2.0 * c * s * apq + s * s * aqq);
-        A.set(q, q, s * s * app - 2.0 * c * s * apq + c * c * aqq);
-        A.set(p, q, -c * s * app + (c * c - s * s) * apq + c * s * aqq);
-    };
+    auto at = [&](int r, int c) -> double & { return L[(size_t)c * w + (r - c)]; };   // r >= c
     for (int k = 0; k + 2 < n; ++k) {
         for (int i = std::min(n - 1, k + b); i >= k + 2; --i) {
             int q = i, col = k;
             while (true) {
                 const int p = q - 1;
-                const double x = A.get(p, col), y = A.get(q, col);
+                const double x = at(p, col), y = at(q, col);
                 if (y == 0.0) break;
-                const double r = std::hypot(x, y), c = x / r, s = y / r;
-                rotate(p, c, s);
-                A.set(q, col, 0.0);
-                log.push_back({p, c, s});
+                const double r = std::sqrt(x * x + y * y), c = x / r, s = y / r;
+                // rows p, q at the columns left of p (column x holds A[p][x], A[q][x] adjacent)
+                const int lo = std::max(0, q - b - 1);
+                for (int xx = lo; xx < p; ++xx) {
+                    double *colx = L + (size_t)xx * w;
+                    const double t1 = colx[p - xx], t2 = colx[q - xx];
+                    colx[p - xx] = c * t1 + s * t2;
+                    colx[q - xx] = -s * t1 + c * t2;
+                }
+                // columns p, q below q (two contiguous runs)
+                const int hi = std::min(n - 1, p + b + 1);
+                double *colp = L + (size_t)p * w + 1, *colq = L + (size_t)q * w;   // colp[x - q] = A[x][p]
+                for (int d = 1; d <= hi - q; ++d) {
+                    const double t1 = colp[d], t2 = colq[d];
+                    colp[d] = c * t1 + s * t2;
+                    colq[d] = -s * t1 + c * t2;
+                }
+                const double app = at(p, p), apq = at(q, p), aqq = at(q, q);
+                at(p, p) = c * c * app + 2.0 * c * s * apq + s * s * aqq;
+                at(q, q) = s * s * app - 2.0 * c * s * apq + c * c * aqq;
+                at(q, p) = -c * s * app + (c * c - s * s) * apq + c * s * aqq;
+                at(q, col) = 0.0;
+                if (log) log->push_back({p, c, s});
                 // the rotation filled (q + b, p): chase it one band further down
                 if (q + b >= n) break;
                 col = p;
@@ -603,7 +605,7 @@ int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
     for (int c = 0; c < n; ++c)
         for (int r = c; r < std::min(n, c + std::max(b, 1) + 1); ++r) A.set(r, c, a[(size_t)r * n + c]);
     std::vector<Rot> log;
-    if (b > 1) band_to_tridiag(A, log);
+    if (b > 1) band_to_tridiag(A, z ? &log : nullptr);
     std::vector<double> d(n), e(n, 0.0);
     for (int i = 0; i < n; ++i) {
         d[i] = A.get(i, i);

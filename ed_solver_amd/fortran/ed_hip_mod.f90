@@ -178,9 +178,10 @@ module ed_hip
       real(c_double), value       :: tolerance
       integer(c_int), intent(out) :: iters
     end function
-    integer(c_int) function ed_block_last_stats(steps, full_passes) bind(C, name="ed_block_last_stats")
-      import :: c_int
+    integer(c_int) function ed_block_last_stats(steps, full_passes, ms, ritz_ms) bind(C, name="ed_block_last_stats")
+      import :: c_int, c_double
       integer(c_int), intent(out) :: steps, full_passes
+      real(c_double), intent(out) :: ms, ritz_ms
     end function
     integer(c_int) function ed_arpack_default_opts(opts) bind(C, name="ed_arpack_default_opts")
       import :: c_int, ed_arpack_opts_t

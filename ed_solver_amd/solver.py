@@ -159,11 +159,13 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     e0 = min(e_low.values())
     kept = sorted(s for s, e in e_low.items() if np.exp(-beta * (e - e0)) > weight_cutoff)
     t0 = time.perf_counter()
-    steps, fulls = [], []
+    steps, fulls, ms, ritz = [], [], [], []
 
     def count(_sector, result):
         steps.append(result.steps)
         fulls.append(result.full_passes)
+        ms.append(result.ms)
+        ritz.append(result.ritz_ms)
 
     states = get_eigen_energies(model, sectors=kept, neigen=neigen, block_size=block_size, nitermax=nitermax,
                                 tolerance=tolerance, full_ed_switch=full_ed_switch, on_block=count)
@@ -171,7 +173,7 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
     info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,
             "t_pass1_s": t_pass1, "t_pass2_s": t_pass2, "e_low": e_low, "block_steps": steps,
-            "block_full_passes": fulls}
+            "block_full_passes": fulls, "block_ms": ms, "block_ritz_ms": ritz}
     return states, info
 
 

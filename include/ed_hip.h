@@ -448,10 +448,12 @@ int ed_arpack_default_opts(ed_arpack_opts *opts);   /* 8, 9, BE, 1e-12, 300 */
 int ed_arpack(ed_sector *sec, const ed_arpack_opts *opts, const double *start, double *evals, double *resid,
               double *evecs, int64_t ldv, int *nconv, int *iters, int *matvecs);
 
-/* The last ed_block_lanczos call: block steps, and the steps that projected the new block
- * against every earlier block (all of them with reorth = 0).  No reference counterpart:
- * it tells a caller what partial reorthogonalisation saved.  Either output may be NULL. */
-int ed_block_last_stats(int *steps, int *full_passes);
+/* The last ed_block_lanczos call: block steps, the steps that projected the new block
+ * against every earlier block (all of them with reorth = 0), the call's host wall time and
+ * the part of it spent in the Ritz tests (the band solver on T).  No reference
+ * counterpart: it tells a caller what partial reorthogonalisation saved and where the
+ * time goes.  Any output may be NULL. */
+int ed_block_last_stats(int *steps, int *full_passes, double *ms, double *ritz_ms);
 
 /* The contractions of the block orthogonalisation, on device matrices (column-major):
  *   ed_gram_f64_dev:  g (host, row-major p x q) = X^T Y, X n x p, Y n x q, q <= 16

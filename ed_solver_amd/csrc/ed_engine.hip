@@ -254,6 +254,22 @@ __global__ __launch_bounds__(1024, 1) void k_lz_persist_cols(HvArgs<T> a, LzArgs
                                        red, grp, pub, &flag);
 }
 
+// Persistent Lanczos with one grid barrier per step (see lz_persist_1b_body): the
+// multi-column kernel's map and LDS layout, two exchange vectors at qx (leading dimension
+// lz.ld) and the partial sums at p1 (kP1Arrays x 2 x kP1Stride doubles).
+template <typename T, int K, int R, bool MON>
+__global__ __launch_bounds__(1024, 1) void k_lz_persist_1b(HvArgs<T> a, LzArgs<T> lz, PersistMap pm, int nsteps,
+                                                           int spin_limit, LzMonitor mon, T *qx, double *p1,
+                                                           int exact_beta)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double red[kP1Arrays * 16 + 2];
+    __shared__ double sh[8];
+    __shared__ int flag;
+    lz_persist_1b_body<T, K, R, MON>(a, lz, pm, nsteps, spin_limit, mon, qx, p1, exact_beta,
+                                     reinterpret_cast<unsigned char *>(smem_d), red, sh, &flag);
+}
+
 __global__ void k_bar_reset(LzState *st)
 {
     for (int g = threadIdx.x; g < kBarGroups; g += blockDim.x) {
@@ -264,6 +280,7 @@ __global__ void k_bar_reset(LzState *st)
         st->bar_count = 0;
         st->bar_err = 0;
         st->stop_at = 0;
+        st->exact_beta_steps = 0;
     }
 }
 
@@ -836,10 +853,39 @@ static const void *persist_cols_kernel(int k, int rows, bool mon)
     }
 }
 
+template <typename T, int K, int R>
+static const void *persist_1b_fn(bool mon)
+{
+    return mon ? reinterpret_cast<const void *>(&k_lz_persist_1b<T, K, R, true>)
+               : reinterpret_cast<const void *>(&k_lz_persist_1b<T, K, R, false>);
+}
+
+// The one-barrier instances: one row per thread (d_up <= 256).  With more rows a thread
+// cannot hold q and u of all its gathered elements at once (128 registers at 4 waves per
+// SIMD): its rows then take twice the batches of loads of the two-barrier kernel, which
+// costs more than the barrier saves (measured at d_up = 495 and 924).
+template <typename T>
+static const void *persist_1b_kernel(int k, int rows, bool mon)
+{
+    if (rows != 1) return nullptr;
+    switch (k) {
+    case 4: return persist_1b_fn<T, 4, 1>(mon);
+    case 8: return persist_1b_fn<T, 8, 1>(mon);
+    case 12: return persist_1b_fn<T, 12, 1>(mon);
+    default: return nullptr;
+    }
+}
+
+// the one-barrier kernel's partial sums live in the reorthogonalisation dots' space, which
+// a persistent run (never reorthogonalised) does not use
+static_assert((size_t)kP1Arrays * 2 * kP1Stride <= (size_t)kMaxIter * kReorthGrid, "partials of the one-barrier kernel");
+static_assert(kBarGroups * kBarGroupSize <= kP1Stride, "one partial slot per workgroup");
+
 struct PersistShape {
     const void *fn = nullptr, *fn_mon = nullptr;   // plain and monitored instance
     int nblk = 0, rows = 0;
     size_t lds = 0;
+    bool onebar = false;   // the one-barrier form of the multi-column kernel
     // the multi-column kernel: cpw columns per workgroup of 256 cpw threads (0: the one-column kernel)
     int cpw = 0, nthreads = kBlock, nA = 0;
     PersistMap map{};
@@ -897,12 +943,15 @@ static bool persist_cols_shape(ed_sector *s, const HvShape &h, PersistShape &p, 
 {
     if (!s->persist_cols || h.k > 12 || (s->hund && s->hund_maxrow > kPersistHund)) return false;
     const int rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
-    const void *fn = persist_cols_kernel<T>(h.k, rows, false), *fn_mon = persist_cols_kernel<T>(h.k, rows, true);
+    const bool onebar = s->persist_1b != 0 && rows == 1;
+    const void *fn = onebar ? persist_1b_kernel<T>(h.k, rows, false) : persist_cols_kernel<T>(h.k, rows, false);
+    const void *fn_mon = onebar ? persist_1b_kernel<T>(h.k, rows, true) : persist_cols_kernel<T>(h.k, rows, true);
     if (!fn || !fn_mon) return false;
     int cpw = std::min(4, std::max(1, (s->dn.d + ncu - 1) / ncu));
     for (; cpw >= 1; cpw /= 2) {
         const size_t lds =
-            persist_cols_lds(s->up.d, h.k, s->up.ncoef, s->ncfg, s->nhund, s->hund, cpw, sizeof(T)).total;
+            persist_cols_lds(s->up.d, h.k, s->up.ncoef, s->ncfg, s->nhund, s->hund, cpw, sizeof(T), onebar ? 3 : 2)
+                .total;
         if (lds > 160 * 1024) continue;
         if (ensure_pmap(s, cpw, h.xcd != 0) != ED_OK) return false;
         int per_cu = 0, per_cu_mon = 0;
@@ -913,6 +962,7 @@ static bool persist_cols_shape(ed_sector *s, const HvShape &h, PersistShape &p, 
         if ((int64_t)std::min(per_cu, per_cu_mon) * ncu < nwg || nwg > kBarGroups * kBarGroupSize) continue;
         p.fn = fn;
         p.fn_mon = fn_mon;
+        p.onebar = onebar;
         p.rows = rows;
         p.cpw = cpw;
         p.nthreads = kBlock * cpw;
@@ -982,13 +1032,25 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
     lz.nA = lz.nB = p.nA;
     int spins = kBarrierSpins;
     PersistMap pm = p.map;
+    if (p.onebar && !s->d_qx) {
+        const hipError_t ea = hipMalloc(&s->d_qx, sizeof(double) * 2 * (size_t)s->ld);
+        if (ea != hipSuccess) {
+            s->d_qx = nullptr;
+            return fail(ED_ERR_NOMEM, std::string("exchange vectors of the persistent kernel: ") + hipGetErrorString(ea));
+        }
+    }
+    T *qx = (T *)s->d_qx;
+    double *p1 = s->d_partials + kPartR;
+    int exact = s->persist_exact_beta;
     void *args[] = {&a, &lz, &nsteps, &spins, &mon};
     void *args_cols[] = {&a, &lz, &pm, &nsteps, &spins, &mon};
+    void *args_1b[] = {&a, &lz, &pm, &nsteps, &spins, &mon, &qx, &p1, &exact};
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(256), 0, st, lz.st);
     // persist == 2 (a test hook) takes the refusal path without launching
     const hipError_t e = s->persist == 2 ? hipErrorCooperativeLaunchTooLarge
                                          : hipLaunchCooperativeKernel(mon.stop ? p.fn_mon : p.fn, dim3(p.nblk),
-                                                                      dim3(p.nthreads), p.cpw ? args_cols : args,
+                                                                      dim3(p.nthreads),
+                                                                      p.onebar ? args_1b : p.cpw ? args_cols : args,
                                                                       (unsigned)p.lds, st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -1839,9 +1901,11 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
 int ed_lanczos_set_persistent(ed_sector *sec, int enable)
 {
     ED_TRY(check_common(sec, 0));
-    if (enable < 0 || enable > 3) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0, 1, 2 or 3");
-    sec->persist = enable == 3 ? 1 : enable;
+    if (enable < 0 || enable > 5) return fail(ED_ERR_ARG, "ed_lanczos_set_persistent: enable must be 0 .. 5");
+    sec->persist = enable >= 3 ? 1 : enable;
     sec->persist_cols = enable == 3 ? 0 : 1;
+    sec->persist_1b = (enable == 1 || enable == 2 || enable == 5) ? 1 : 0;
+    sec->persist_exact_beta = enable == 5 ? 1 : 0;
     sec->persist_ok[0] = sec->persist_ok[1] = -1;
     return ED_OK;
 }

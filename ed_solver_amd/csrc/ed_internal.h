@@ -67,6 +67,8 @@ struct LzState {
                        // stops (0 = not asked to stop); zeroed before each launch
     unsigned bar_count;   // persistent Lanczos kernel: group leaders' arrivals since its launch
     int bar_err;          // a barrier wait exceeded its spin limit
+    int exact_beta_steps; // one-barrier persistent kernel: steps whose beta took the exact
+                          // norm (one more grid barrier) since its launch
     double alpha[kMaxIter];      // alpha_j        -> alpha[j-1]
     double beta[kMaxIter + 1];   // beta_1 = ||u_1|| -> beta[0]; beta_{j+1} -> beta[j]
     // grid barrier groups of the persistent Lanczos kernel, one 64-byte line per word
@@ -237,6 +239,9 @@ struct ed_sector {
     int32_t *d_pmap = nullptr;
     int pmap_cpw = 0, pmap_xcd = -1, pmap_nwg = 0, pmap_nidle = 0;
     int persist_cols = 1;          // the multi-column persistent kernel where it applies (1) or never (0)
+    int persist_1b = 1;            // its one-grid-barrier-per-step form (1) or the two-barrier form (0)
+    int persist_exact_beta = 0;    // one-barrier form: every beta from the reduced norm (test hook)
+    void *d_qx = nullptr;          // one-barrier form: two exchange vectors of ld FP64 elements
     // S_z (Nambu) sector: one table over 2 Ns bits in `up` (dn.d = 1), the quartic
     // diagonal QUART_INT_SZ in d_quart, and the constant tr(Ec_dn) + tr(Eb_dn) that the
     // particle-hole transformation moves out of H

@@ -1269,14 +1269,15 @@ constexpr int kPersistHund = 2;
 //   shared by the workgroup's columns: pk[K][dup] (u32), du[dup] (T), cu[dup] (i32),
 //   swap[dup][ncfg] (u16, Hund only), coef[ncoef], uimp[ncfg^2], hval[nhund] (T),
 //   hptr[ncfg^2 + 1], htr[nhund] (i32);
-//   per column: col[dup], colm[dup] (T), swdn[ncfg] (i32), then (read by tables of width
-//   >= 12, whose down-hop entries stay out of scalar registers) cod[K] (T), cjd[K] (i32).
+//   per column: nvec vectors of dup (T: col, colm; the one-barrier kernel has a third),
+//   swdn[ncfg] (i32), then (read by tables of width >= 12, whose down-hop entries stay
+//   out of scalar registers, and by the one-barrier kernel) cod[K] (T), cjd[K] (i32).
 struct PersistLds {
     size_t pk, du, cu, sw, coef, uimp, hval, hptr, htr, cols, col_stride, total;
 };
 __host__ __device__ inline size_t al16(size_t x) { return (x + 15) / 16 * 16; }
 __host__ __device__ inline PersistLds persist_cols_lds(int dup, int K, int ncoef, int ncfg, int nhund, int hund,
-                                                       int cpw, size_t tsize)
+                                                       int cpw, size_t tsize, int nvec = 2)
 {
     PersistLds L;
     const size_t nq = (size_t)ncfg * ncfg;
@@ -1290,7 +1291,7 @@ __host__ __device__ inline PersistLds persist_cols_lds(int dup, int K, int ncoef
     L.hptr = al16(L.hval + tsize * nhund);
     L.htr = al16(L.hptr + 4 * (nq + 1));
     L.cols = al16(L.htr + 4 * (size_t)nhund);
-    L.col_stride = al16(al16(2 * tsize * dup + 4 * (size_t)ncfg) + (tsize + 4) * (size_t)K);
+    L.col_stride = al16(al16(nvec * tsize * dup + 4 * (size_t)ncfg) + (tsize + 4) * (size_t)K);
     L.total = L.cols + L.col_stride * cpw;
     return L;
 }
@@ -1591,6 +1592,414 @@ __device__ inline void lz_persist_cols_body(const HvArgs<T> &a, const LzArgs<T> 
             }
         }
         if (!last && grid_wait(st, nbar, rel, spin_limit, flag, MON ? &st->stop_at : nullptr, j - 1)) return;
+    }
+}
+
+// ------------------------- persistent Lanczos with one grid barrier per step
+
+// Partial sums of the one-barrier kernel: array k (0 A = <z_j, u_j>, 1 X = <u_j, u_{j-1}>,
+// 2 E = ||u_j||^2, 3 N = ||q_{j+1}||^2, 4 C = <q_{j+1}, u_j>) of the step j that wrote it,
+// by parity of j, one slot per workgroup (the sum over its columns), kP1Stride slots each.
+constexpr int kP1Arrays = 5;
+constexpr int kP1Stride = 8192;
+__host__ __device__ inline double *p1_arr(double *p1, int k, int j)
+{
+    return p1 + (size_t)(2 * k + (j & 1)) * kP1Stride;
+}
+
+// u_j = q_j - c u_{j-1}: one fused multiply-add in FP64, rounded once to T, so the
+// workgroup that owns an element and every workgroup that gathers it form the same bits.
+template <typename T>
+__device__ inline T lz_unfold(T q, T um, double c)
+{
+    return (T)__builtin_fma(-c, (double)um, (double)q);
+}
+
+// n <= 256 partial sums, 4 per lane, with coherent loads, then their fixed-order wave sum
+// (lane l adds entries l, l + 64, ...): split in two so that other loads can be issued
+// between the loads and the sum.
+__device__ inline void part_load4(const double *p, int n, double *v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = q * 64 + lane;
+        v[q] = i < n ? ld_agent(p + i) : 0.0;
+    }
+}
+__device__ inline double part_sum4(const double *v)
+{
+    return wave_sum(((v[0] + v[1]) + v[2]) + v[3]);
+}
+
+// Squared-norm ratio below which beta_j^2 = N - 2 c C + c^2 E (the expansion of
+// ||q_j - c u_{j-1}||^2) has lost too many digits to cancellation: beta_j is then formed
+// from ||u_j||^2 itself, at the cost of one more grid barrier.
+constexpr double kP1Cancel = 1e-3;
+
+// Gathered elements (q and u of each) a thread of the one-barrier kernel keeps in flight:
+// the row's whole batch of down hops and Hund targets at table width 8 (more spill).
+constexpr int kP1LoadElems = 20;
+
+// `nsteps` Lanczos steps of the recurrence of lz_persist_cols_body (same map, same LDS
+// layout, same scalars up to rounding) with ONE grid barrier per step instead of two.
+// The barrier of the two-barrier kernel that only serves alpha_j is removed by not
+// subtracting alpha_j u_j before the exchange: step j stores
+//     q_{j+1} = z_j / beta_j - (beta_j / beta_{j-1}) u_{j-1},   z_j = H u_j,
+// for the other workgroups (two exchange vectors, by parity), together with one partial
+// sum per workgroup of each of <z_j, u_j>, <u_j, u_{j-1}>, ||u_j||^2, ||q_{j+1}||^2 and
+// <q_{j+1}, u_j>.  After the barrier every workgroup forms
+//     alpha_j = <z_j, u_j> / beta_j^2 - <u_j, u_{j-1}> / beta_{j-1},   c = alpha_j / beta_j,
+//     beta_{j+1}^2 = ||q_{j+1}||^2 - 2 c <q_{j+1}, u_j> + c^2 ||u_j||^2,
+// then u_{j+1} = q_{j+1} - c u_j for its own rows (LDS and the Krylov slot), and every
+// neighbour element it gathers as the same fused multiply-add of q_{j+1} and u_j loaded
+// from memory, so H u_{j+1} is H applied to the stored u_{j+1}: nothing is propagated
+// from one step to the next but the vectors themselves.  beta_{j+1} differs from the
+// norm of the stored vector by the cancellation of the expansion (N / beta^2 is about
+// (alpha^2 + beta^2) / beta^2, a few hundred eps); where that ratio exceeds 1 / kP1Cancel
+// (or always, with exact_beta) the norm is reduced over the grid instead.  The last
+// step (or the one a monitored run stops after) ends with the two-barrier kernel's state:
+// u_{j+1} in its slot, its norm and <u_{j+1}, u_j> partials in nrm / crs, j_k1 = j + 1.
+template <typename T, int K, int R, bool MON>
+__device__ inline void lz_persist_1b_body(const HvArgs<T> &a, const LzArgs<T> &lz, const PersistMap &pm,
+                                          int nsteps, int spin_limit, LzMonitor mon, T *qx, double *p1,
+                                          int exact_beta, unsigned char *smem, double *red, double *sh,
+                                          int *flag)
+{
+    constexpr int NT = kBlock;                 // threads per column
+    constexpr int NL = K + kPersistHund;       // gathered elements per row
+    // rows per batch of gathers (q and u per element)
+    constexpr int RB = (2 * NL * R <= kP1LoadElems) ? R : (2 * NL * 2 <= kP1LoadElems) ? 2 : 1;
+    LzState *st = lz.st;
+    if (st->status) return;
+    const int dup = a.dup, ncfg = a.ncfg, nq = ncfg * ncfg, nh = a.nhund, cpw = pm.cpw;
+    const PersistLds L = persist_cols_lds(dup, K, a.ncoef_up, ncfg, nh, a.hund, cpw, sizeof(T), 3);
+    uint32_t *pk = reinterpret_cast<uint32_t *>(smem + L.pk);
+    T *du = reinterpret_cast<T *>(smem + L.du);
+    int *cup = reinterpret_cast<int *>(smem + L.cu);
+    uint16_t *swp = reinterpret_cast<uint16_t *>(smem + L.sw);
+    T *coef = reinterpret_cast<T *>(smem + L.coef);
+    T *uimp = reinterpret_cast<T *>(smem + L.uimp);
+    T *hval = reinterpret_cast<T *>(smem + L.hval);
+    int *hptr = reinterpret_cast<int *>(smem + L.hptr);
+    int *htr = reinterpret_cast<int *>(smem + L.htr);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int cg = __builtin_amdgcn_readfirstlane(tid >> 8), lt = tid & (NT - 1);
+    // three vectors of the column, by step j: vb[j % 3] holds q_j (u_j once formed),
+    // vb[(j + 2) % 3] u_{j-1}, and q_{j+1} goes to vb[(j + 1) % 3]
+    T *vb = reinterpret_cast<T *>(smem + L.cols + L.col_stride * cg);
+    int *swdn = reinterpret_cast<int *>(vb + 3 * dup);
+    T *cod = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(vb) + al16(3 * sizeof(T) * dup + 4 * ncfg));
+    int *cjd = reinterpret_cast<int *>(cod + K);
+    constexpr bool DL = true;   // down-hop entries from LDS: their registers go to the gathers
+    const bool lead = blockIdx.x == 0 && tid == 0;
+    unsigned nbar = 0;
+    int j = st->j_k1;
+    const int ido = __builtin_amdgcn_readfirstlane(pm.cols[blockIdx.x * cpw + cg]);
+    const bool act = ido >= 0;
+    const int ps = act ? pm.pslot[ido] : 0;
+    int cd = 0, nd = 0;
+    T ddg = (T)0;
+    T od[K];
+    int jd[K];
+    if (act) {
+        cd = a.imp_dn[ido];
+        nd = a.ptr_dn[ido + 1] - a.ptr_dn[ido];
+        ddg = a.diag_dn[ido];
+    }
+    if constexpr (DL) {
+        if (lt < K) {
+            cod[lt] = (act && lt < nd) ? a.off_dn[a.ptr_dn[ido] + lt] : (T)0;
+            cjd[lt] = (act && lt < nd) ? a.rk_dn[a.ptr_dn[ido] + lt] * dup : 0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            od[k] = (act && k < nd) ? a.off_dn[a.ptr_dn[ido] + k] : (T)0;
+            jd[k] = (act && k < nd) ? a.rk_dn[a.ptr_dn[ido] + k] * dup : 0;
+        }
+    }
+    const int base = act ? ido * dup : 0;
+    for (int e = tid; e < K * dup; e += nt) pk[e] = a.ell_pk_up[e];
+    for (int e = tid; e < dup; e += nt) {
+        du[e] = a.diag_up[e];
+        cup[e] = a.imp_up[e];
+    }
+    for (int e = tid; e < a.ncoef_up; e += nt) coef[e] = a.coef_up[e];
+    for (int e = tid; e < nq; e += nt) uimp[e] = a.uimp[e];
+    if (a.hund) {
+        for (int e = tid; e < dup * ncfg; e += nt) swp[e] = (uint16_t)a.swap_up[e];
+        for (int e = tid; e <= nq; e += nt) hptr[e] = a.hund_ptr[e];
+        for (int e = tid; e < nh; e += nt) {
+            htr[e] = a.hund_tr[e];
+            hval[e] = a.hund_val[e];
+        }
+        if (act)
+            for (int e = lt; e < ncfg; e += NT) swdn[e] = a.swap_dn[ido * ncfg + e];
+    }
+    {
+        const T *x0 = lz.slot(j);
+        const T *xm = lz.slot(j > 1 ? j - 1 : j);
+        T *c0 = vb + (j % 3) * dup, *cm = vb + ((j + 2) % 3) * dup;
+        for (int r = lt; r < dup; r += NT) {
+            c0[r] = act ? x0[base + r] : (T)0;
+            cm[r] = (act && j > 1) ? xm[base + r] : (T)0;
+        }
+    }
+    double bj1 = j > 1 ? st->beta[j - 2] : 1.0;   // beta_{j-1}
+    double bj2 = j > 2 ? st->beta[j - 3] : 1.0;   // beta_{j-2}
+    const int nwg = gridDim.x;
+    int hflag = 0;
+    __syncthreads();
+    for (int s = 0;; ++s) {
+        const int w = tid >> 6;
+        T *col = vb + (j % 3) * dup, *colm = vb + ((j + 2) % 3) * dup, *colq = vb + ((j + 1) % 3) * dup;
+        // ---- gather sources of step j: q_j (u_j itself in the first step) and u_{j-1}
+        const T *G = s == 0 ? lz.slot(j) : qx + (int64_t)(j & 1) * lz.ld;
+        const T *U = s == 0 ? G : lz.slot(j - 1);
+        // ---- scalars: beta_j (and alpha_{j-1}, c with u_j = q_j - c u_{j-1})
+        double c = 0.0, bj;
+        if (s == 0) {
+            // u_j is whole in its slot: beta_j from the update partials of the step that made it
+            if (w == 0) {
+                const double t = wave_part_sum<true>(lz.nrm(j), lz.nB);
+                if (tid == 0) sh[0] = t;
+            }
+            if (MON && lead) hflag = __hip_atomic_load(mon.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __syncthreads();
+            bj = sqrt(sh[0]);
+        } else {
+            // wave w reduces arrays w, w + (waves), ... (a one-column workgroup has 4 waves)
+            for (int k = w; k < kP1Arrays; k += nt >> 6) {
+                double t;
+                if (nwg <= 256) {
+                    double pv[4];
+                    part_load4(p1_arr(p1, k, j - 1), nwg, pv);
+                    t = part_sum4(pv);
+                } else {
+                    t = wave_part_sum<true>(p1_arr(p1, k, j - 1), nwg);
+                }
+                if ((tid & 63) == 0) sh[k] = t;
+            }
+            __syncthreads();
+            const double alpha = sh[0] / (bj1 * bj1) - (j > 2 ? sh[1] / bj2 : 0.0);
+            c = alpha / bj1;
+            double b2 = sh[3] - 2.0 * c * sh[4] + c * c * sh[2];
+            if (exact_beta || !(b2 >= kP1Cancel * sh[3])) {
+                // ||u_j||^2 over the grid: one more barrier (the same branch in every workgroup)
+                double nn = 0.0;
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int r = lt + i * NT;
+                    if (act && r < dup) {
+                        const double u = (double)lz_unfold(col[r], colm[r], c);
+                        nn += u * u;
+                    }
+                }
+                const double v = wave_sum(nn);
+                if ((tid & 63) == 0) red[w] = v;
+                __syncthreads();
+                if (tid == 0) {
+                    double t = 0.0;
+                    for (int q = 0; q < (nt >> 6); ++q) t += red[q];
+                    st_agent(p1_arr(p1, 2, j) + blockIdx.x, t);
+                }
+                if (grid_barrier(st, ++nbar, spin_limit, flag)) return;
+                if (w == 0) {
+                    const double t = wave_part_sum<true>(p1_arr(p1, 2, j), nwg);
+                    if (tid == 0) sh[5] = t;
+                }
+                __syncthreads();
+                b2 = sh[5];
+                if (lead) atomicAdd(&st->exact_beta_steps, 1);
+            }
+            bj = sqrt(fmax(b2, 0.0));
+            if (lead) {
+                st->alpha[j - 2] = alpha;
+                if (MON) {
+                    __hip_atomic_store(mon.ab + 2 * (j - 2), bj1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(mon.ab + 2 * (j - 2) + 1, alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    hflag = __hip_atomic_load(mon.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+            }
+        }
+        if (bj < kMinLanczosNorm) {   // the reference STOPs here (Lanczos_plain.h:70-92)
+            if (lead) {
+                st->beta[j - 1] = bj;
+                st->status = (j == 1) ? 2 : 1;
+            }
+            return;
+        }
+        if (lead) st->beta[j - 1] = bj;
+        // ---- u_j of the own rows (LDS and its Krylov slot)
+        if (s > 0) {
+            T *uj = lz.slot(j);
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = lt + i * NT;
+                if (act && r < dup) {
+                    const T u = lz_unfold(col[r], colm[r], c);
+                    col[r] = u;
+                    st_agent(uj + base + r, u);
+                }
+            }
+            __syncthreads();   // the up hops read other rows of col
+        }
+        // ---- z = H u_j (diagonal and up hops from LDS, down hops and Hund terms gathered),
+        // q_{j+1} = z / beta_j - (beta_j / beta_{j-1}) u_{j-1} for the other workgroups
+        const double cz = 1.0 / bj, cp = j > 1 ? bj / bj1 : 0.0;
+        T *Q = qx + (int64_t)((j + 1) & 1) * lz.ld;
+        double dA = 0.0, dX = 0.0, dE = 0.0, dN = 0.0, dC = 0.0;
+#pragma unroll
+        for (int i0 = 0; i0 < R; i0 += RB) {
+            // diagonal and up hops of the batch's rows first (LDS), then its gathers in flight
+            T zl[RB];
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int r = lt + (i0 + b) * NT;
+                zl[b] = (T)0;
+                if (act && r < dup) {
+                    T acc = ((du[r] + ddg) + uimp[cup[r] * ncfg + cd]) * col[r];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const uint32_t e = pk[k * dup + r];
+                        acc += coef[e >> 16] * col[e & 0xFFFFu];
+                    }
+                    zl[b] = acc;
+                }
+            }
+            T xq[RB][NL], xu[RB][NL];
+            int h0[RB], nht[RB];
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int r = lt + (i0 + b) * NT;
+                const bool ok = act && r < dup;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int o = (DL ? cjd[k] : jd[k]) + r;
+                    xq[b][k] = (ok && k < nd) ? ld_agent(G + o) : (T)0;
+                    xu[b][k] = (ok && k < nd) ? ld_agent(U + o) : (T)0;
+                }
+                h0[b] = 0;
+                nht[b] = 0;
+                if (a.hund && ok) {
+                    const int qq = cup[r] * ncfg + cd;
+                    h0[b] = hptr[qq];
+                    nht[b] = hptr[qq + 1] - h0[b];
+                }
+#pragma unroll
+                for (int h = 0; h < kPersistHund; ++h) {
+                    xq[b][K + h] = (T)0;
+                    xu[b][K + h] = (T)0;
+                    if (h < nht[b]) {
+                        const int tr = htr[h0[b] + h];
+                        const int o = (int)swp[r * ncfg + (tr & 0xFF)] + swdn[tr >> 8] * dup;
+                        xq[b][K + h] = ld_agent(G + o);
+                        xu[b][K + h] = ld_agent(U + o);
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int i = i0 + b, r = lt + i * NT;
+                if (act && r < dup) {
+                    const T xv = col[r];
+                    T acc = zl[b];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc += (DL ? cod[k] : od[k]) * lz_unfold(xq[b][k], xu[b][k], c);
+#pragma unroll
+                    for (int h = 0; h < kPersistHund; ++h)
+                        if (h < nht[b]) acc += hval[h0[b] + h] * lz_unfold(xq[b][K + h], xu[b][K + h], c);
+                    const double um = (double)colm[r];
+                    dA += (double)acc * (double)xv;
+                    dX += (double)xv * um;
+                    dE += (double)xv * (double)xv;
+                    const T q = (T)(cz * (double)acc - cp * um);
+                    st_agent(Q + base + r, q);
+                    colq[r] = q;
+                    dN += (double)q * (double)q;
+                    dC += (double)q * (double)xv;
+                }
+            }
+            if constexpr (RB < R) __builtin_amdgcn_sched_barrier(0);
+        }
+        {
+            const double v[kP1Arrays] = {wave_sum(dA), wave_sum(dX), wave_sum(dE), wave_sum(dN), wave_sum(dC)};
+            if ((tid & 63) == 0)
+#pragma unroll
+                for (int k = 0; k < kP1Arrays; ++k) red[kP1Arrays * w + k] = v[k];
+            __syncthreads();
+            if (tid < kP1Arrays) {   // the workgroup's sum: its waves in order
+                double t = 0.0;
+                for (int q = 0; q < (nt >> 6); ++q) t += red[kP1Arrays * q + tid];
+                st_agent(p1_arr(p1, tid, j) + blockIdx.x, t);
+            }
+        }
+        if (MON && lead && hflag) st_agent(&st->stop_at, j);
+        bj2 = bj1;
+        bj1 = bj;
+        ++j;
+        if (lead) {
+            st->j_k1 = j;
+            st->j_k2 = j - 1;
+        }
+        const int fl = grid_barrier(st, ++nbar, spin_limit, flag, MON ? &st->stop_at : nullptr, j - 1);
+        if (fl == 1) return;
+        if (fl == 2 || s + 1 == nsteps) break;
+    }
+    // ---- end of the launch: alpha_{j-1}, u_j = q_j - c u_{j-1} into its slot, and the
+    // partials of ||u_j||^2 and <u_j, u_{j-1}> where the two-barrier kernel leaves them
+    {
+        const int w = tid >> 6;
+        const T *col = vb + (j % 3) * dup, *colm = vb + ((j + 2) % 3) * dup;
+        if (w < 2) {
+            const double t = wave_part_sum<true>(p1_arr(p1, w, j - 1), nwg);
+            if ((tid & 63) == 0) sh[w] = t;
+        }
+        __syncthreads();
+        const double alpha = sh[0] / (bj1 * bj1) - (j > 2 ? sh[1] / bj2 : 0.0);
+        const double c = alpha / bj1;
+        if (lead) {
+            st->alpha[j - 2] = alpha;
+            if (MON) {
+                __hip_atomic_store(mon.ab + 2 * (j - 2), bj1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(mon.ab + 2 * (j - 2) + 1, alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        T *uj = lz.slot(j);
+        double nn = 0.0, cr = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = lt + i * NT;
+            if (act && r < dup) {
+                const T u = lz_unfold(col[r], colm[r], c);
+                st_agent(uj + base + r, u);
+                nn += (double)u * (double)u;
+                cr += (double)u * (double)colm[r];
+            }
+        }
+        const double v0 = wave_sum(nn), v1 = wave_sum(cr);
+        if ((tid & 63) == 0) {
+            red[2 * w] = v0;
+            red[2 * w + 1] = v1;
+        }
+        __syncthreads();
+        if (lt == 0 && act) {
+            double t0 = 0.0, t1 = 0.0;
+            for (int q = 0; q < NT / 64; ++q) {
+                t0 += red[2 * (cg * (NT / 64) + q)];
+                t1 += red[2 * (cg * (NT / 64) + q) + 1];
+            }
+            st_agent(lz.nrm(j) + ps, t0);
+            st_agent(lz.crs(j) + ps, t1);
+        }
+        // canonical slots no column writes hold zeros (the first step of this launch read
+        // nrm of its parity before any barrier, so they are cleared only here)
+        if (blockIdx.x == 0)
+            for (int e = tid; e < pm.nidle; e += nt) {
+                st_agent(lz.nrm(j) + pm.idle[e], 0.0);
+                st_agent(lz.crs(j) + pm.idle[e], 0.0);
+            }
     }
 }
 

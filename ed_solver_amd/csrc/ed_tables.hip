@@ -515,7 +515,7 @@ static void free_sector(ed_sector *S)
     free_spin(S->up);
     free_spin(S->dn);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
-                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord, S->d_pmap,
+                  S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord, S->d_pmap, S->d_qx,
                   S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32, S->d_sz_mat};
     for (void *p : ps)
         if (p) (void)hipFree(p);

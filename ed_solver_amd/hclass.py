@@ -180,9 +180,17 @@ class Sector:
                                          ctypes.byref(by), ctypes.byref(pk)))
         return ms.value, by.value, bool(pk.value)
 
-    def set_persistent(self, enable: bool) -> None:
-        """Use the persistent Lanczos kernel where it applies (True, the default) or never."""
-        check(lib().ed_lanczos_set_persistent(self.handle, 1 if enable else 0))
+    _PERSIST_FORMS = {"one-barrier": 1, "two-barrier": 4, "one-column": 3, "exact-beta": 5}
+
+    def set_persistent(self, enable: bool, form: str = "one-barrier") -> None:
+        """Use the persistent Lanczos kernel where it applies (True, the default) or never.
+        form: "one-barrier" (the default: one grid barrier per step, alpha/beta equal to
+        the two-kernel step's to rounding), "two-barrier" (the multi-column kernel with the
+        two-kernel step's arithmetic, alpha/beta bit for bit), "one-column", or
+        "exact-beta" (one-barrier with every beta from the reduced norm, a test hook)."""
+        if form not in self._PERSIST_FORMS:
+            raise ValueError(f"form must be one of {sorted(self._PERSIST_FORMS)}")
+        check(lib().ed_lanczos_set_persistent(self.handle, self._PERSIST_FORMS[form] if enable else 0))
 
     def persistent_shape(self, precision="f64") -> dict:
         """Launch shape of the persistent kernel: columns per workgroup (0: not used),

@@ -246,11 +246,19 @@ int ed_lanczos_benchmark(ed_sector *sec, int precision, int niter, int reps, dou
  * by grid barriers, the static row data in registers and LDS, the vectors exchanged with
  * device-coherent loads and stores.  Tables of width <= 12 with at most 2 Hund
  * transitions per row use the multi-column form (up to 4 columns per workgroup of 1024
- * threads, the row tables shared in LDS); others the one-column form.  Either way the
- * arithmetic and the order of every sum are those of the two-kernel step at the same
- * thread layout, so alpha and beta are the same numbers.  enable = 1 (default): use it
- * where it applies; 0: never; 2 (a test hook): as 1, but every cooperative launch takes
- * the path of a launch the runtime refuses; 3: the one-column form only.  A barrier wait
+ * threads, the row tables shared in LDS); others the one-column form.  Where each thread
+ * owns one row (d_up <= 256) the multi-column form runs ONE grid barrier per step by
+ * default: it exchanges the vector before the alpha_j u_j subtraction, rebuilds u_{j+1}
+ * bit for bit wherever it is gathered, and forms beta_{j+1} from the expansion of its norm
+ * (the norm itself where the expansion cancels), so alpha and beta agree with the
+ * two-kernel step to rounding (1e-14 over the first 80 steps at d_up = 252; later steps
+ * drift apart as those of any two Lanczos runs whose rounding differs, with the same Ritz
+ * values).  Its two-barrier form and the one-column form keep the two-kernel step's arithmetic and
+ * order of every sum, so alpha and beta are the same numbers.  enable = 1 (default): use
+ * it where it applies, one-barrier form; 0: never; 2 (a test hook): as 1, but every
+ * cooperative launch takes the path of a launch the runtime refuses; 3: the one-column
+ * form only; 4: the two-barrier multi-column form; 5 (a test hook): the one-barrier form
+ * with every beta from the norm (one more barrier per step).  A barrier wait
  * that exceeds a few seconds stops the run with ED_ERR_HIP and turns the kernel off for
  * the sector.  A cooperative launch the runtime refuses (hipErrorCooperativeLaunchTooLarge,
  * hipErrorIllegalState: nothing ran) turns it off for that sector and precision, is

@@ -192,6 +192,21 @@ def main():
     it32, dt32 = rate("f32")
     log(f"FP32-H.v Lanczos: {it32:.0f} iterations/s")
 
+    # the persistent kernel's two forms on a sector whose threads own one row (workload B,
+    # (5,5), d_up = 252): one grid barrier per step (the default there) against two
+    mB, secB = ed.synthetic("B")
+    sB = ed.Sector(mB, *secB)
+    sB.tune()
+    forms_b = {"workload": f"Ns=10 (5,5), dim {sB.dim}, {args.niter}-step FP64 runs"}
+    for form in ("one-barrier", "two-barrier"):
+        sB.set_persistent(True, form)
+        ms_b, _, pk_b = sB.lanczos_benchmark("f64", args.niter, 5)
+        forms_b[form] = {"us_per_step": ms_b / args.niter * 1e3, "persistent": pk_b}
+    forms_b["speedup"] = forms_b["two-barrier"]["us_per_step"] / forms_b["one-barrier"]["us_per_step"]
+    log(f"workload B persistent Lanczos: one barrier {forms_b['one-barrier']['us_per_step']:.2f} us/step, "
+        f"two barriers {forms_b['two-barrier']['us_per_step']:.2f} us/step")
+    sB.close()
+
     def gs_timed(prec, tol, reps=5):
         """Ground state with the reference convergence test: one warm-up, then the median
         over `reps` runs of the eigenvalue solve and of solve + psi (copied to the host)."""
@@ -484,6 +499,7 @@ def main():
         "speedup_vs_cpu": (it64 / cpu["value"]) if cpu else None,
         "sector_scan": scan,
         "arpack": arp,
+        "persistent_forms_b": forms_b,
         "gradients_ns14": grad,
     }
     if cpu and green:

@@ -650,6 +650,15 @@ static int sched_of(const ed_sector *s)
     return (sizeof(T) == sizeof(float) && s->cpb32) ? s->cpb32 : s->cpb;
 }
 
+// Schedule of the Lanczos recurrence (two-kernel step and persistent kernel) in precision T:
+// the H.v schedule unless the tuner chose another one for it (lanczos_sched).
+template <typename T>
+static int sched_lz(const ed_sector *s)
+{
+    const int l = sizeof(T) == sizeof(float) ? s->lzs32 : s->lzs;
+    return l ? l : sched_of<T>(s);
+}
+
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st)
 {
     const HvShape h = hv_shape(s, prec == ED_PREC_F32 ? sched_of<float>(s) : sched_of<double>(s));
@@ -756,7 +765,7 @@ static LzArgs<T> lz_args_at(const ed_sector *s, const LzWork &w, int sched)
 template <typename T>
 static LzArgs<T> lz_args(ed_sector *s)
 {
-    return lz_args_at<T>(s, LzWork{s->d_basis, s->ld, s->basis_nslots, s->d_state, s->d_partials}, sched_of<T>(s));
+    return lz_args_at<T>(s, LzWork{s->d_basis, s->ld, s->basis_nslots, s->d_state, s->d_partials}, sched_lz<T>(s));
 }
 
 // nsteps two-kernel Lanczos steps (H.v phase, update phase, optional CGS2 with its dots
@@ -787,7 +796,7 @@ static void launch_steps_at(ed_sector *s, const LzArgs<T> &lz, const HvShape &h,
 template <typename T>
 static void launch_steps_impl(ed_sector *s, int nsteps, int reorth, hipStream_t st)
 {
-    launch_steps_at<T>(s, lz_args<T>(s), hv_shape(s, sched_of<T>(s)), s->d_partials, nsteps, reorth, st);
+    launch_steps_at<T>(s, lz_args<T>(s), hv_shape(s, sched_lz<T>(s)), s->d_partials, nsteps, reorth, st);
 }
 
 // ---------------------------------------------------------- persistent Lanczos
@@ -981,7 +990,7 @@ static bool persist_shape(ed_sector *s, PersistShape &p)
 {
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     if (!s->persist || s->persist_ok[pi] == 0) return false;
-    const HvShape h = hv_shape(s, sched_of<T>(s));
+    const HvShape h = hv_shape(s, sched_lz<T>(s));
     if (h.kind != 2 || s->up.d > 4 * kBlock) return false;
     {
         int dev = 0, ncu = 0, coop = 0;
@@ -1027,7 +1036,7 @@ static int launch_persist(ed_sector *s, const PersistShape &p, int nsteps, hipSt
 {
     HvArgs<T> a = hv_args<T>(s, 1);
     LzArgs<T> lz = lz_args<T>(s);
-    const HvShape h = hv_shape(s, sched_of<T>(s));
+    const HvShape h = hv_shape(s, sched_lz<T>(s));
     if (h.xcd) a.colord = s->d_colord;
     lz.nA = lz.nB = p.nA;
     int spins = kBarrierSpins;
@@ -1301,6 +1310,44 @@ static int tune_prec(ed_sector *s, double *ms_out, int &choice)
     return ED_OK;
 }
 
+// Whether the persistent Lanczos kernel has an instance for the sector under schedule
+// `sched` (the fast kernel, d_up <= 1024, a compiled width and rows per thread); the
+// co-residency test of persist_shape still applies at launch.
+template <typename T>
+static bool persist_may_apply(ed_sector *s, int sched)
+{
+    const HvShape h = hv_shape(s, sched);
+    if (h.kind != 2 || s->up.d > 4 * kBlock || !s->persist) return false;
+    const int rows = s->up.d <= kBlock ? 1 : s->up.d <= 2 * kBlock ? 2 : 4;
+    const bool cols = s->persist_cols && h.k <= 12 && !(s->hund && s->hund_maxrow > kPersistHund) &&
+                      persist_cols_kernel<T>(h.k, rows, false);
+    return cols || persist_kernel<T>(h.k, rows, false);
+}
+
+// Schedule of the Lanczos recurrence (0: the H.v schedule `hv_choice`).  Where the fastest
+// H.v is the row kernel, which keeps the persistent kernel out, the fastest fast-kernel
+// schedule under which the persistent kernel applies is taken instead if its H.v is within
+// 1.5x: a persistent step (H.v and update in one launch, no launch gaps, one grid barrier
+// where a thread owns one row) is 1.7x the two-kernel step there (6.3 against 10.6 us at
+// (5,5) of Ns = 10 with H.v times 4.91 against 4.85 us; 8.5 against 14.5 us at (3,6) of
+// Ns = 12 with 6.96 against 6.23 us).
+template <typename T>
+static int lanczos_sched(ed_sector *s, const double *ms, int hv_choice)
+{
+    const int cands[kSchedules] = {1, 2, 4, 8, 16, 18, 19, 20};
+    if (hv_shape(s, hv_choice).kind == 2) return 0;
+    double thv = 0.0, tbest = 1e30;
+    int best = 0;
+    for (int i = 0; i < kSchedules; ++i) {
+        if (cands[i] == hv_choice) thv = ms[i];
+        if (cands[i] >= 8 && ms[i] > 0.0 && ms[i] < tbest && persist_may_apply<T>(s, cands[i])) {
+            tbest = ms[i];
+            best = cands[i];
+        }
+    }
+    return (best && thv > 0.0 && tbest <= 1.5 * thv) ? best : 0;
+}
+
 // Autotuner: times the H.v under every applicable schedule (row kernel with 1, 2 or 4
 // columns per workgroup; fast kernel in natural and XCD-grouped column order, with 1-4
 // rows per thread), 3 launches each after one warm-up, HIP events on the sector's
@@ -1314,6 +1361,7 @@ static int tune(ed_sector *s)
     if (s->sz || s->cplx || s->dim < 4096) {
         s->cpb = fast_ok(s) ? 8 : 1;
         s->cpb32 = 0;
+        s->lzs = s->lzs32 = 0;
         return ED_OK;
     }
     if (fast_ok(s)) ED_TRY(ensure_colord(s));
@@ -1323,6 +1371,8 @@ static int tune(ed_sector *s)
     ED_TRY(tune_prec<float>(s, s->tune_ms32, c32));
     s->cpb = c64;
     s->cpb32 = c32;
+    s->lzs = lanczos_sched<double>(s, s->tune_ms, c64);
+    s->lzs32 = lanczos_sched<float>(s, s->tune_ms32, c32);
     s->persist_ok[0] = s->persist_ok[1] = -1;
     drop_graphs(s);
     return ED_OK;
@@ -1845,7 +1895,8 @@ int ed_hmult_set_schedule(ed_sector *sec, int choice)
     if (choice >= 18 && hv_shape(sec, choice).rows != choice - 16)
         return fail(ED_ERR_ARG, "ed_hmult_set_schedule: that many rows per thread do not apply here");
     sec->cpb = choice;
-    sec->cpb32 = 0;   // FP32 follows the forced schedule too
+    sec->cpb32 = 0;   // FP32 and the Lanczos recurrence follow the forced schedule too
+    sec->lzs = sec->lzs32 = 0;
     sec->persist_ok[0] = sec->persist_ok[1] = -1;
     drop_graphs(sec);
     return ED_OK;
@@ -1927,7 +1978,7 @@ int ed_lanczos_persistent_shape(ed_sector *sec, int precision, int *cols_per_wg,
     if (!sec->cpb) ED_TRY(tune(sec));
     PersistShape p;
     const bool on = precision == ED_PREC_F64 ? persist_shape<double>(sec, p) : persist_shape<float>(sec, p);
-    const HvShape h = hv_shape(sec, precision == ED_PREC_F64 ? sched_of<double>(sec) : sched_of<float>(sec));
+    const HvShape h = hv_shape(sec, precision == ED_PREC_F64 ? sched_lz<double>(sec) : sched_lz<float>(sec));
     if (cols_per_wg) *cols_per_wg = on ? std::max(1, p.cpw) : 0;
     if (workgroups) *workgroups = on ? p.nblk : 0;
     if (rows) *rows = on ? p.rows : 0;
@@ -2060,7 +2111,7 @@ int ed_hmult_benchmark(ed_sector *sec, int precision, int reps, double *avg_ms, 
     ED_TRY(lz_start(sec, precision, nullptr, nullptr));
     const int64_t n = sec->dim;
     const bool f64 = precision == ED_PREC_F64;
-    const HvShape h = hv_shape(sec, f64 ? sched_of<double>(sec) : sched_of<float>(sec));
+    const HvShape h = hv_shape(sec, f64 ? sched_lz<double>(sec) : sched_lz<float>(sec));
     // Phase 1 of step 1 (z = H u_1 and the partials of <z, u_1>), launched reps times back
     // to back between one event pair: every launch reads u_1 and writes z and the
     // partials again, the same traffic as a step of the recurrence, so the average is the

@@ -229,6 +229,7 @@ struct ed_sector {
                                    // XCD-grouped column order; 0 = not tuned
     double tune_ms[ed::kSchedules] = {};   // measured FP64 H.v time per schedule (1, 2, 4, 8, 16, 18, 19, 20)
     int cpb32 = 0;                 // schedule of the FP32 kernels (tuned on their own); 0 = as cpb
+    int lzs = 0, lzs32 = 0;        // schedule of the Lanczos recurrence (FP64, FP32); 0 = as the H.v's
     double tune_ms32[ed::kSchedules] = {};
     int32_t *d_colord = nullptr;   // [colord_blocks] workgroup -> down column, -1 = idle
     int colord_blocks = 0;

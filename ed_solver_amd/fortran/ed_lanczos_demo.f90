@@ -14,7 +14,9 @@
 !      Lanczos_get_GS_sector (Lanczos_plain.f90:156-229); the vector is written to the
 !      file named by the second argument, when given (raw REAL(8) stream);
 !   5. the S_z branch of new_H (new_H_sz, no pairing) with the host Lanczos loop over
-!      Hmult__, when that sector has at most 10^6 states.
+!      Hmult__, when that sector has at most 10^6 states;
+!   6. ARPACK_diago's eigensolver on the device (ARPACK_diago_hip: Neigen_ = 5, so 4 Ritz
+!      pairs from both ends of the spectrum, ncv 24) on the (N_up, N_dn) sector.
 ! Prints E0 from each, the iteration count and the Rayleigh quotient of the device
 ! ground state.
 program ed_lanczos_demo
@@ -28,7 +30,8 @@ program ed_lanczos_demo
   real(8), allocatable :: diag(:), subdiag(:), w(:), z(:), psi(:), hpsi(:), psi2(:)
   real(8) :: e0_host, e0_dev, rq, e0_sz
   type(ed_block_opts_t) :: bo
-  real(8) :: valp(4, 2)
+  real(8) :: valp(4, 2), valp_a(4), res_a(4)
+  integer :: nconv_a
   real(8), allocatable :: vecp(:, :)
   integer(c_int) :: nconv, nsteps
   character(len=512) :: fname, fout
@@ -76,6 +79,17 @@ program ed_lanczos_demo
     call ed_check(ed_block_lanczos(sector_handle(), bo, valp(:, 1), valp(:, 2), vecp, int(n, c_int64_t), &
                                    nconv, nsteps), 'ed_block_lanczos')
     write(*, '(a,4f22.14,a,i0,a,i0)') 'E_block ', valp(1:4, 1), ' nconv ', nconv, ' steps ', nsteps
+  end if
+
+  ! ---- 6. ARPACK_diago on the device (before the handle is rebuilt: any H will do)
+  if (n >= 30) then
+    block
+      real(8), allocatable :: vecp_a(:, :)
+      allocate(vecp_a(n, 4))
+      call ARPACK_diago_hip(5, 1d-12, valp_a, vecp_a, res_a, nconv_a, ncv=24)
+      write(*, '(a,4f22.14,a,i0,a,es10.2)') 'E_arpack ', valp_a, ' nconv ', nconv_a, ' maxres ', &
+        maxval(res_a(1:max(1, nconv_a)))
+    end block
   end if
 
   ! ---- 4. get_GS order: the handle is rebuilt before the ground-state vector is formed

@@ -29,7 +29,7 @@ module H_class_hip
 
   public :: new_H, new_H_complex, new_H_sz, new_H_slater, delete_H, Hmult__, Hmultr, Hmultc, dimen_H, &
             sector_handle, energy_shift_H
-  public :: Lanczos_GPU_updo, Lanczos_GPU_get_GS_updo, Lanczos_dynamic_GPU_updo
+  public :: Lanczos_GPU_updo, Lanczos_GPU_get_GS_updo, Lanczos_dynamic_GPU_updo, ARPACK_diago_hip
 
 contains
 
@@ -178,6 +178,37 @@ contains
     if (sizevec /= dimen_H()) stop 'Lanczos_dynamic_GPU_updo: sizevec /= dimen_H()'
     call ed_check(ed_lanczos_run(sector_h, int(Niter, c_int), c_loc(vec), ED_PREC_F64, diag, subdiag), &
                   'Lanczos_dynamic_GPU_updo')
+  end subroutine
+
+  ! The arpack_eigenvector_sym_matrix call of ARPACK_diago (ED_ARPACK.f90:498-525) on the
+  ! device for the current H: nev = Neigen_ - 1 Ritz pairs from both ends ('BE'),
+  ! ncv = Neigen_ (or ncv when given), tol = tolerance, maxitr 300, start vector
+  ! create_fix_initial_vector's.  VALP(1:nconv) ascending, VECP(:, i) the vectors,
+  ! rel_resid(i) = ||H x - VALP(i) x|| / |VALP(i)|; the dEmax window stays with the caller
+  ! (:527-545).
+  subroutine ARPACK_diago_hip(Neigen_, tolerance, VALP, VECP, rel_resid, nconv, ncv)
+    integer, intent(in)           :: Neigen_
+    real(8), intent(in)           :: tolerance
+    real(8), intent(out)          :: VALP(:), rel_resid(:)
+    real(8), intent(out)          :: VECP(:, :)
+    integer, intent(out)          :: nconv
+    integer, intent(in), optional :: ncv
+    type(ed_arpack_opts_t) :: ao
+    integer(c_int) :: nc, iters, mv
+    if (Neigen_ < 2) stop 'ARPACK_diago_hip: Neigen_ < 2'
+    if (size(VALP) < Neigen_ - 1 .or. size(rel_resid) < Neigen_ - 1 .or. size(VECP, 2) < Neigen_ - 1) &
+      stop 'ARPACK_diago_hip: VALP, VECP or rel_resid shorter than Neigen_ - 1'
+    if (size(VECP, 1) /= dimen_H()) stop 'ARPACK_diago_hip: size(VECP, 1) /= dimen_H()'
+    call ed_check(ed_arpack_default_opts(ao), 'ed_arpack_default_opts')
+    ao%nev = Neigen_ - 1
+    ao%ncv = Neigen_
+    if (present(ncv)) ao%ncv = max(Neigen_, ncv)
+    ao%which = ED_ARPACK_BE
+    ao%tol = tolerance
+    ao%maxitr = 300
+    call ed_check(ed_arpack(sector_h, ao, c_null_ptr, VALP, rel_resid, VECP, int(size(VECP, 1), c_int64_t), &
+                            nc, iters, mv), 'ARPACK_diago_hip')
+    nconv = nc
   end subroutine
 
 end module H_class_hip

@@ -59,6 +59,13 @@ def test_fortran_demo_matches_oracle(tmp_path, name):
     eb = np.array([float(x) for x in line[1:5]])
     ref, _ = oracle.lowest_eigs(o, 4)
     assert np.max(np.abs(eb - ref)) < 1e-10
+    # ARPACK_diago's eigensolver through ARPACK_diago_hip: 4 Ritz pairs from both ends ('BE'):
+    # the two lowest against the oracle's, all four converged with relative residual < 1e-10
+    line = [l for l in r.stdout.splitlines() if l.startswith("E_arpack")][0].split()
+    ea = np.array([float(x) for x in line[1:5]])
+    assert int(line[6]) == 4 and float(line[8]) < 1e-10, line
+    assert abs(ea[0] - e0) < 1e-10 and abs(ea[1] - ref[1]) < 1e-10
+    assert ea[1] < ea[2] <= ea[3]
     # get_GS order: psi from the second pass on the rebuilt handle equals the kept-basis psi
     # and the oracle's psi (Lanczos_get_GS_sector restated)
     line = [l for l in r.stdout.splitlines() if l.startswith("GS_second_pass_overlap")][0].split()

@@ -282,3 +282,24 @@ def test_one_barrier_breakdown_takes_the_exact_norm(gpu):
     a0, b0 = s.lanczos_run(30, start=good)
     assert np.max(np.abs(a1 - a0)) < 1e-12 and np.max(np.abs(b1 - b0) / b0) < 1e-12
     s.close()
+
+
+@pytest.mark.parametrize("name,sector", [("B", None), ("C", (3, 6))])
+def test_tuned_lanczos_schedule_takes_the_persistent_kernel(gpu, name, sector):
+    """After the autotuner, the Lanczos recurrence runs the persistent kernel even where the
+    row kernel gives the fastest standalone H.v (within 1.5x of a fast-kernel schedule): the
+    tuned sector reports the persistent kernel active and its ground-state solve matches the
+    oracle (E0 to 1e-10, iterations within one)."""
+    m, (nu, nd) = golden(name)
+    if sector:
+        nu, nd = sector
+    s = gpu.Sector(m, nu, nd)
+    choice, ms = s.tune()
+    print(f"{name} ({nu},{nd}): dim {s.dim}, H.v schedule {choice}, ms {np.round(np.array(ms) * 1e3, 2)}", flush=True)
+    assert s.persistent_active("f64")
+    o = oracle.OracleSector(m, nu, nd, nthreads=16)
+    r = s.lanczos(nitermax=128, neigen=1, tolerance=1e-12)
+    ro = o.lanczos(nitermax=128, neigen=1, tol=1e-12)
+    assert abs(r.evals[0] - ro["evals"][0]) < 1e-10 and abs(r.iters - ro["iters"]) <= 1
+    o.close()
+    s.close()

@@ -136,6 +136,9 @@ struct BatchWS {
     int device = -1;
 };
 static BatchWS g_ws;
+// work space of the sectors ed_lanczos_batched runs one by one (recurrence state, partial
+// sums, 3 Krylov slots, start vector), its own lock: those runs also happen while g_ws is held
+static BatchWS g_alone;
 // launches of the multi-run persistent kernel, and launches the runtime refused (then the
 // two-launch batch ran), since the library was loaded
 static int g_runs_launched = 0, g_runs_refused = 0;
@@ -146,21 +149,25 @@ static double g_runs_ms = 0.0, g_runs_job_steps = 0.0, g_runs_bytes = 0.0;
 // wait rounds of a grid-barrier poll of the multi-run kernel before it gives up (seconds)
 constexpr int kRunsBarrierSpins = 1 << 21;
 
-static int ws_reserve(size_t bytes, int device)
+static constexpr size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int ws_reserve_in(BatchWS &w, size_t bytes, int device)
 {
-    if (g_ws.ptr && g_ws.bytes >= bytes && g_ws.device == device) return ED_OK;
-    if (g_ws.ptr) {
+    if (w.ptr && w.bytes >= bytes && w.device == device) return ED_OK;
+    if (w.ptr) {
         (void)hipDeviceSynchronize();
-        (void)hipFree(g_ws.ptr);
-        g_ws.ptr = nullptr;
-        g_ws.bytes = 0;
+        (void)hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
     }
-    hipError_t e = hipMalloc(&g_ws.ptr, bytes);
+    hipError_t e = hipMalloc(&w.ptr, bytes);
     if (e != hipSuccess) return fail(ED_ERR_NOMEM, "batched Lanczos work space: " + std::string(hipGetErrorString(e)));
-    g_ws.bytes = bytes;
-    g_ws.device = device;
+    w.bytes = bytes;
+    w.device = device;
     return ED_OK;
 }
+
+static int ws_reserve(size_t bytes, int device) { return ws_reserve_in(g_ws, bytes, device); }
 
 static bool rows_capable(const ed_sector *s) { return !s->sz && !s->cplx && s->up.d >= 128 && s->up.kpad <= 32; }
 
@@ -355,6 +362,9 @@ static bool runs_capable(const ed_sector *s)
 // Return of run_batch_persist when the kernel does not take the batch (not an ED_ code):
 // the caller runs the two-launch batch instead.
 constexpr int kRunsNotUsed = 1;
+// Return of run_fixed when a refused launch held a run that only the multi-run kernel takes
+// (d_up below the two-launch batch's 128): the caller runs those sectors one by one.
+constexpr int kRunsNeedAlone = 2;
 
 // Workgroups per run.  A run's columns go round by round to its workgroups' 4 column
 // groups and a step lasts about as long as the most rounds any run needs, so run J gets
@@ -668,9 +678,12 @@ static int run_fixed(const std::vector<ed_sector *> &bs, const std::vector<const
         if (gi < npersist)
             rc = precision == ED_PREC_F64 ? run_batch_persist<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
                                           : run_batch_persist<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
-        if (rc == kRunsNotUsed)
+        if (rc == kRunsNotUsed) {
+            for (ed_sector *x : gs)
+                if (!rows_capable(x)) return kRunsNeedAlone;
             rc = precision == ED_PREC_F64 ? run_batch<double>(gs, gp, gn, niter, st, ga, gb, phi_on_device)
                                           : run_batch<float>(gs, gp, gn, niter, st, ga, gb, phi_on_device);
+        }
         if (rc != ED_OK) return rc;
         for (size_t q = 0; q < grp[gi].size(); ++q) {
             std::copy(ga.begin() + q * niter, ga.begin() + (q + 1) * niter, ab.begin() + (size_t)grp[gi][q] * niter);
@@ -797,11 +810,24 @@ extern "C" int ed_lanczos_batched(int nsec, ed_sector *const *secs, int niter, i
     ED_HIP_CHECK(hipSetDevice(secs[0]->device));
     std::fill(alpha, alpha + (size_t)nsec * niter, 0.0);
     std::fill(beta, beta + (size_t)nsec * niter, 0.0);
-    // one sector alone: the sector's own fixed-length run (its breakdown step reported)
+    // one sector alone: a fixed-length run from create_fix_initial_vector's vector in a
+    // shared work space (no Krylov basis, scalars or partials of the sector's own, no
+    // autotuning), its breakdown step reported
     auto alone = [&](int k) -> int {
+        ed_sector *x = secs[k];
+        std::lock_guard<std::mutex> lk(g_alone.mu);
+        const int64_t ld = (x->dim + 63) / 64 * 64;
+        const size_t o_part = al256(sizeof(LzState)), n_part = (size_t)kPartC + 2 * 8192;
+        const size_t o_basis = al256(o_part + sizeof(double) * n_part);
+        const size_t o_start = al256(o_basis + sizeof(double) * 3 * (size_t)ld);
+        ED_TRY(ws_reserve_in(g_alone, o_start + sizeof(double) * (size_t)x->dim, x->device));
+        char *base = (char *)g_alone.ptr;
+        double *d_start = (double *)(base + o_start);
+        ED_TRY(start_vector_dev(x->dim, d_start, x->stream));
+        const LzWork w{base + o_basis, ld, 3, (LzState *)base, (double *)(base + o_part)};
         int at = 0;
-        int rc = lanczos_run_dev(secs[k], niter, nullptr, precision, alpha + (size_t)k * niter,
-                                 beta + (size_t)k * niter, &at);
+        int rc = lanczos_run_ws(x, w, niter, d_start, precision, alpha + (size_t)k * niter, beta + (size_t)k * niter,
+                                &at);
         if (rc == ED_ERR_BREAKDOWN && at > 0) {
             steps[k] = at;
             return ED_OK;
@@ -812,7 +838,7 @@ extern "C" int ed_lanczos_batched(int nsec, ed_sector *const *secs, int niter, i
     std::vector<int> batch;
     for (int k = 0; k < nsec; ++k) {
         const int64_t d = secs[k]->cplx ? secs[k]->cdim : secs[k]->dim;
-        if (rows_capable(secs[k]) && d > niter) batch.push_back(k);
+        if ((rows_capable(secs[k]) || runs_capable(secs[k])) && d > niter) batch.push_back(k);
         else ED_TRY(alone(k));
     }
     if (batch.empty()) return ED_OK;
@@ -840,7 +866,9 @@ extern "C" int ed_lanczos_batched(int nsec, ed_sector *const *secs, int niter, i
     if (rc == ED_OK) rc = run_fixed(bs, bp, bn, niter, precision, st, ab, bb, true);
     (void)hipStreamSynchronize(st);
     (void)hipFree(d);
-    if (rc == ED_ERR_BREAKDOWN) {   // a Krylov space closed before niter: each sector alone
+    if (rc == ED_ERR_BREAKDOWN || rc == kRunsNeedAlone) {
+        // a Krylov space closed before niter, or a refused launch held a run only the
+        // multi-run kernel takes: each sector alone
         for (int i = 0; i < nb; ++i) ED_TRY(alone(batch[i]));
         return ED_OK;
     }

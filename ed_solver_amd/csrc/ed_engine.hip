@@ -2313,8 +2313,9 @@ static void run_ws_impl(ed_sector *s, const LzWork &w, int sched, int niter, con
 }
 
 int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_start, int prec, double *diag,
-                   double *subdiag)
+                   double *subdiag, int *breakdown_at)
 {
+    if (breakdown_at) *breakdown_at = 0;
     ED_TRY(check_common(s, prec));
     if (niter < 1 || niter > kMaxIter - 2) return fail(ED_ERR_ARG, "lanczos run: niter outside [1, 2046]");
     if (w.nslots < 3 || w.ld < s->dim) return fail(ED_ERR_ARG, "lanczos run: work space below 3 vectors of dimen_H");
@@ -2338,6 +2339,7 @@ int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_sta
         if (i >= reached) diag[i] = 0.0;
         if (i > reached) subdiag[i] = 0.0;
     }
+    if (breakdown_at && hdr[2] == 1) *breakdown_at = reached;
     if (hdr[2] == 1 && reached < niter)
         return fail(ED_ERR_BREAKDOWN, "zero-norm Lanczos vector at iteration " + std::to_string(reached + 1) +
                                           " (Lanczos_plain.h:85-92)");

@@ -312,9 +312,10 @@ int noalloc_setup(int device);
 // (w.nslots >= 3) with plain two-kernel steps on the sector's stream: no device memory is
 // allocated and the sector's own Krylov basis is left alone.  The H.v schedule is the
 // tuned one, or the untuned default when the sector was never tuned.  diag/subdiag
-// (capacity niter, host) as ed_lanczos_run; ED_ERR_BREAKDOWN as there.
+// (capacity niter, host) as ed_lanczos_run; ED_ERR_BREAKDOWN as there, with the completed
+// steps in *breakdown_at when given (0 otherwise).
 int lanczos_run_ws(ed_sector *s, const LzWork &w, int niter, const double *d_start, int prec, double *diag,
-                   double *subdiag);
+                   double *subdiag, int *breakdown_at = nullptr);
 // s->col_bfs: the sector's down columns in the breadth-first order of its down-hop graph
 int ensure_col_bfs(ed_sector *s);
 // FP64 MFMA contractions of ed_block.hip on device matrices (column-major, 16-byte aligned,

@@ -169,15 +169,15 @@ def test_runs_kernel_breakdown_is_an_error_and_drains(gpu):
 def test_lanczos_batched_matches_single_runs(gpu):
     """ed_lanczos_batched (Lanczos_GPU_updo for many sectors at once) against each sector's
     own ed_lanczos_run from the same default start vector: sectors of workload C of uneven
-    dimension (the multi-run kernel) and one with d_up < 128 (run alone); alpha/beta to 1e-10
-    relative.  The replayed convergence test gives the per-sector Lanczos E0 to 1e-10, and
+    dimension (the multi-run kernel, d_up = 66 included), one with d_up = 12 (run alone, in the
+    shared work space); alpha/beta to 1e-10 relative.  The replayed convergence test gives the per-sector Lanczos E0 to 1e-10, and
     on a 66-state sector run for Niter = dim - 1 steps its exact lowest eigenvalue."""
     import json, os
     from ed_solver_amd.aim import AIMParams
     from ed_solver_amd.hclass import lanczos_batched, lanczos_converged
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models.json")) as f:
         m = AIMParams.from_json(json.load(f)["C"]["model"])
-    keys = [(6, 6), (5, 6), (4, 7), (3, 5), (1, 6)]
+    keys = [(6, 6), (5, 6), (4, 7), (3, 5), (2, 6), (10, 5), (1, 6)]
     secs = [gpu.Sector(m, *k) for k in keys]
     niter = 64
     print(f"batched Lanczos over {[s.dim for s in secs]}", flush=True)

@@ -23,6 +23,9 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-result",
     # packed FP32 ops stay disabled (hand-written or compiler-made v_pk_*_f32 are avoided)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # host code (the band and tridiagonal eigen-solvers of the Ritz tests) with AVX2 and FMA:
+    # every x86-64 host an MI355X sits in has them; the block Lanczos Ritz test runs 1.7x faster
+    "-Xarch_host", "-mavx2", "-Xarch_host", "-mfma",
     "-I", os.path.join(ROOT, "include"),
 ]
 

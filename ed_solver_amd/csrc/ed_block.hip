@@ -561,6 +561,7 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     auto Tat = [&](int i, int j) -> double & { return T[(size_t)i * nmax + j]; };
     std::vector<double> G1, G2, Bnext, prev(nev, 1e300), w(nev), Z;
     int steps = 0, done = 0;
+    int next_test = std::max(1, o.check_every);   // block steps at the next Ritz test
     bool exhausted = false;
     std::vector<double> lastB;
     // Partial reorthogonalisation (o.reorth == 1): the estimates Om[k] ~ Q_k^T Q_j (b x b,
@@ -654,22 +655,34 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         }
         const int m = p;
         if (m < nev) continue;
-        // Ritz test every check_every steps, thinned to every steps / 4 as T grows (the
-        // band solver costs O(m^2 b) per test)
-        if (!last && (steps % std::max(std::max(1, o.check_every), steps / 4)) != 0) continue;
+        // Ritz test after check_every steps, then at intervals that grow with T (steps / 4:
+        // the band solver costs O(m^2 b) per test) while the Ritz values still move by more
+        // than sqrt(tolerance) between tests, every check_every steps once they move less
+        // (so convergence is seen within a few steps of the test that would see it), and at
+        // the last step
+        if (!last && steps < next_test) continue;
         // Ritz values of T_j (leading m x m block); the vectors (for the residual bounds and
         // the Ritz vectors) only once the values have settled, or at the last step
         std::vector<double> Tm((size_t)m * m);
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
         const auto t_r = std::chrono::steady_clock::now();
-        ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), nullptr));
+        // one reduction (rotations logged) serves the values and, once they settle, the vectors
+        BandTri bt;
+        ED_TRY(band_reduce(m, b, Tm.data(), true, bt));
+        ED_TRY(band_tri_values(bt, nev, w.data()));
         bool settled = steps > 1;
-        for (int k = 0; k < nev && settled; ++k)
-            if (!(std::fabs(w[k] - prev[k]) < o.tolerance)) settled = false;
+        double moved = 0.0;
+        for (int k = 0; k < nev; ++k) {
+            const double dk = std::fabs(w[k] - prev[k]);
+            if (!(dk < o.tolerance)) settled = false;
+            moved = std::max(moved, std::isfinite(dk) ? dk : 1e300);
+        }
+        const int every = std::max(1, o.check_every);
+        next_test = steps + (moved < std::sqrt(o.tolerance) ? every : std::max(every, steps / 4));
         if (last || settled) {
             Z.assign((size_t)m * nev, 0.0);
-            ED_TRY(band_lowest(m, b, Tm.data(), nev, w.data(), Z.data()));
+            ED_TRY(band_tri_vectors(bt, nev, w.data(), Z.data()));
         }
         ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_r).count();
         if (last) {

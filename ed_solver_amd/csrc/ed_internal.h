@@ -301,6 +301,21 @@ int tridiag_lowest_pairs(int n, const double *diag, const double *off, int k, do
 // nev lowest eigenpairs of a symmetric band matrix (n x n row-major, half-bandwidth b);
 // z (n x nev, column-major) may be null
 int band_lowest(int n, int b, const double *a, int nev, double *w, double *z);
+// The same in two parts: band_reduce brings the band matrix to the tridiagonal (d, e) by
+// plane rotations, logged when `keep_log` (the vectors need them); band_tri_values gives
+// the nev lowest eigenvalues and band_tri_vectors their vectors from one reduction.
+struct BandRot {
+    int p;
+    double c, s;
+};
+struct BandTri {
+    int n = 0;
+    std::vector<double> d, e;
+    std::vector<BandRot> log;
+};
+int band_reduce(int n, int b, const double *a, bool keep_log, BandTri &t);
+int band_tri_values(const BandTri &t, int nev, double *w);
+int band_tri_vectors(const BandTri &t, int nev, const double *w, double *z);
 // launch helpers implemented in ed_engine.hip
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st);
 HvArgs<double> hv_args_f64(const ed_sector *s);

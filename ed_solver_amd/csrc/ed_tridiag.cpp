@@ -438,10 +438,7 @@ void tridiag_lowest_warm(int n, const double *diag, const double *off, int k, co
 
 namespace {
 
-struct Rot {
-    int p;
-    double c, s;
-};
+using Rot = BandRot;
 
 // Symmetric band storage with room for the bulge: L[c * (b + 2) + (r - c)] = A[r][c] for
 // 0 <= r - c <= b + 1 (the band plus the one diagonal a bulge occupies while it is chased).
@@ -598,33 +595,49 @@ static void tri_vectors(int n, const double *d, const double *e, int nev, const 
     }
 }
 
-int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
+int band_reduce(int n, int b, const double *a, bool keep_log, BandTri &t)
 {
-    if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
+    if (n < 1 || b < 0 || !a) return fail(ED_ERR_ARG, "band_reduce: bad argument");
     Band A(n, std::max(b, 1));
     for (int c = 0; c < n; ++c)
         for (int r = c; r < std::min(n, c + std::max(b, 1) + 1); ++r) A.set(r, c, a[(size_t)r * n + c]);
-    std::vector<Rot> log;
-    if (b > 1) band_to_tridiag(A, z ? &log : nullptr);
-    std::vector<double> d(n), e(n, 0.0);
+    t.n = n;
+    t.log.clear();
+    if (keep_log && b > 1) t.log.reserve((size_t)n * n / 2);
+    if (b > 1) band_to_tridiag(A, keep_log ? &t.log : nullptr);
+    t.d.assign(n, 0.0);
+    t.e.assign(n, 0.0);
     for (int i = 0; i < n; ++i) {
-        d[i] = A.get(i, i);
-        if (i + 1 < n) e[i] = A.get(i + 1, i);
-        if (!std::isfinite(d[i]) || !std::isfinite(e[i])) return fail(ED_ERR_ARG, "band_lowest: non-finite entry");
+        t.d[i] = A.get(i, i);
+        if (i + 1 < n) t.e[i] = A.get(i + 1, i);
+        if (!std::isfinite(t.d[i]) || !std::isfinite(t.e[i])) return fail(ED_ERR_ARG, "band_lowest: non-finite entry");
     }
+    return ED_OK;
+}
+
+int band_tri_values(const BandTri &t, int nev, double *w)
+{
+    const int n = t.n;
+    if (nev < 1 || nev > n || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
     if (nev <= 32 && nev < n) {
-        sturm_lowest(n, d.data(), e.data(), nev, w);
+        sturm_lowest(n, t.d.data(), t.e.data(), nev, w);
     } else {
-        std::vector<double> dq(d), eq(e);
-        if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0) sturm_all(n, d.data(), e.data(), dq.data());
+        std::vector<double> dq(t.d), eq(t.e), ds(t.d), es(t.e);
+        if (ql_implicit(n, dq.data(), eq.data(), nullptr) != 0) sturm_all(n, ds.data(), es.data(), dq.data());
         std::sort(dq.begin(), dq.end());
         for (int k = 0; k < nev; ++k) w[k] = dq[k];
     }
-    if (!z) return ED_OK;
-    tri_vectors(n, d.data(), e.data(), nev, w, z);
+    return ED_OK;
+}
+
+int band_tri_vectors(const BandTri &t, int nev, const double *w, double *z)
+{
+    const int n = t.n;
+    if (nev < 1 || nev > n || !w || !z) return fail(ED_ERR_ARG, "band_lowest: bad argument");
+    tri_vectors(n, t.d.data(), t.e.data(), nev, w, z);
     // back through the rotations: x = G_1^T ... G_N^T y
-    for (size_t r = log.size(); r-- > 0;) {
-        const Rot &g = log[r];
+    for (size_t r = t.log.size(); r-- > 0;) {
+        const Rot &g = t.log[r];
         for (int k = 0; k < nev; ++k) {
             double *x = z + (size_t)k * n;
             const double yp = x[g.p], yq = x[g.p + 1];
@@ -641,6 +654,15 @@ int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
             for (int i = 0; i < n; ++i) x[i] = -x[i];
     }
     return ED_OK;
+}
+
+int band_lowest(int n, int b, const double *a, int nev, double *w, double *z)
+{
+    if (n < 1 || b < 0 || nev < 1 || nev > n || !a || !w) return fail(ED_ERR_ARG, "band_lowest: bad argument");
+    BandTri t;
+    ED_TRY(band_reduce(n, b, a, z != nullptr, t));
+    ED_TRY(band_tri_values(t, nev, w));
+    return z ? band_tri_vectors(t, nev, w, z) : ED_OK;
 }
 
 }  // namespace ed

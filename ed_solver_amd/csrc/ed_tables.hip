@@ -12,8 +12,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "ed_kernels.h"
@@ -510,6 +512,45 @@ static int build_interaction(ed_sector *S)
     return ED_OK;
 }
 
+// Streams of released sectors, kept for the next sectors created on the same device:
+// creating and destroying a HIP stream costs about a millisecond each (844 and 615 us on
+// average over the 157 Lanczos sectors of an Ns = 12 scan), more than building a small
+// sector's tables.  A stream goes back to the pool only once it is idle; the pool holds at
+// most kStreamPool streams (default priority, non-blocking, as created).
+constexpr size_t kStreamPool = 256;
+static std::mutex g_stream_mu;
+static std::vector<std::pair<int, hipStream_t>> g_stream_pool;
+
+static hipError_t stream_acquire(hipStream_t *s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (size_t i = g_stream_pool.size(); i-- > 0;)
+            if (g_stream_pool[i].first == dev) {
+                *s = g_stream_pool[i].second;
+                g_stream_pool.erase(g_stream_pool.begin() + (std::ptrdiff_t)i);
+                return hipSuccess;
+            }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+static void stream_release(int device, hipStream_t s)
+{
+    if (!s) return;
+    if (hipStreamSynchronize(s) != hipSuccess) {   // not idle or broken: not reused
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if (g_stream_pool.size() < kStreamPool) g_stream_pool.emplace_back(device, s);
+    else (void)hipStreamDestroy(s);
+}
+
 static void free_sector(ed_sector *S)
 {
     free_spin(S->up);
@@ -526,7 +567,7 @@ static void free_sector(ed_sector *S)
     if (S->h_mon) (void)hipHostFree(S->h_mon);
     if (S->ev0) (void)hipEventDestroy(S->ev0);
     if (S->ev1) (void)hipEventDestroy(S->ev1);
-    if (S->stream) (void)hipStreamDestroy(S->stream);
+    stream_release(S->device, S->stream);
     delete S;
 }
 
@@ -567,8 +608,7 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
     }
     int rc = ED_OK;
     auto bail = [&](int code) { free_sector(S); return code; };
-    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    if (stream_acquire(&S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
     if ((rc = dalloc(&S->d_rank, (size_t)1 << ns)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
     if ((rc = build_spin(S, 0, nup, S->up)) != ED_OK) return bail(rc);
@@ -805,8 +845,7 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     }
     int rc = ED_OK;
     auto bail = [&](int code) { free_sector(S); return code; };
-    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    if (stream_acquire(&S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
     hipStream_t st = S->stream;
     const int nbits = 2 * ns;
     if ((rc = dalloc(&S->d_rank, (size_t)1 << nbits)) != ED_OK) return bail(rc);

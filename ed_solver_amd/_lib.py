@@ -154,6 +154,7 @@ SIGNATURES = {
                                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                                    ctypes.POINTER(ctypes.c_int)]),
     "ed_lanczos_persistent_refusals": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "ed_lanczos_exact_beta_steps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "ed_lanczos_persistent_active": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "ed_dynamic_batched_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp,
                                                 _dp]),

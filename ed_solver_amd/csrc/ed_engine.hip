@@ -1994,6 +1994,18 @@ int ed_lanczos_persistent_refusals(ed_sector *sec, int *refused)
     return ED_OK;
 }
 
+int ed_lanczos_exact_beta_steps(ed_sector *sec, int *steps)
+{
+    ED_TRY(check_common(sec, 0));
+    if (!steps) return fail(ED_ERR_ARG, "ed_lanczos_exact_beta_steps: NULL output");
+    *steps = 0;
+    if (!sec->d_state) return ED_OK;
+    ED_HIP_CHECK(hipStreamSynchronize(sec->stream));
+    ED_HIP_CHECK(hipMemcpy(steps, (const char *)sec->d_state + offsetof(LzState, exact_beta_steps), sizeof(int),
+                           hipMemcpyDeviceToHost));
+    return ED_OK;
+}
+
 int ed_start_vector(int64_t n, double *out)
 {
     if (n < 1 || !out) return fail(ED_ERR_ARG, "ed_start_vector: bad argument");

@@ -604,6 +604,11 @@ module ed_hip
       type(c_ptr), value :: sec
       integer(c_int), intent(out) :: refused
     end function
+    integer(c_int) function ed_lanczos_exact_beta_steps(sec, steps) bind(C, name="ed_lanczos_exact_beta_steps")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sec
+      integer(c_int), intent(out) :: steps
+    end function
     integer(c_int) function ed_dynamic_batched_stats(persistent_launches, refusals, last_ms, last_job_steps, &
         last_bytes) bind(C, name="ed_dynamic_batched_stats")
       import :: c_int, c_double

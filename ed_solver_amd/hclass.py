@@ -205,6 +205,13 @@ class Sector:
         check(lib().ed_lanczos_persistent_refusals(self.handle, ctypes.byref(v)))
         return v.value
 
+    def exact_beta_steps(self) -> int:
+        """Steps of the last one-barrier persistent launch whose beta came from the reduced
+        norm (the expansion of the norm had cancelled)."""
+        v = ctypes.c_int()
+        check(lib().ed_lanczos_exact_beta_steps(self.handle, ctypes.byref(v)))
+        return v.value
+
     def persistent_active(self, precision="f64") -> bool:
         """Whether the next Lanczos run in `precision` uses the persistent kernel."""
         v = ctypes.c_int()

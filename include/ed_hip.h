@@ -277,6 +277,12 @@ int ed_lanczos_persistent_shape(ed_sector *sec, int precision, int *cols_per_wg,
  * reference counterpart; lets a caller tell a persistent run from its fallback). */
 int ed_lanczos_persistent_refusals(ed_sector *sec, int *refused);
 
+/* Steps of the last launch of the one-barrier persistent kernel on this sector whose
+ * beta_{j+1} came from the reduced norm of u_{j+1} (one more grid barrier) rather than
+ * from the expansion of that norm, which had cancelled (no reference counterpart; a
+ * measurement hook).  0 when the sector never ran that kernel. */
+int ed_lanczos_exact_beta_steps(ed_sector *sec, int *steps);
+
 /* *active = 1 when a fixed-length Lanczos run of at least 16 steps in `precision` on
  * this sector uses the persistent kernel (tunes the sector first if needed), else 0. */
 int ed_lanczos_persistent_active(ed_sector *sec, int precision, int *active);

@@ -276,8 +276,10 @@ def test_one_barrier_breakdown_takes_the_exact_norm(gpu):
     phi[iu + idn * len(up)] = 1.0
     with pytest.raises(BreakdownError):
         s.lanczos_run(20, start=phi)
+    assert s.exact_beta_steps() == 1   # beta_2 from the reduced norm, then the stop
     good = np.random.default_rng(3).standard_normal(s.dim)
     a1, b1 = s.lanczos_run(30, start=good)
+    print(f"  random start: {s.exact_beta_steps()} of 30 steps took the reduced norm", flush=True)
     s.set_persistent(False)
     a0, b0 = s.lanczos_run(30, start=good)
     assert np.max(np.abs(a1 - a0)) < 1e-12 and np.max(np.abs(b1 - b0) / b0) < 1e-12

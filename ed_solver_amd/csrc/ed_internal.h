@@ -204,6 +204,8 @@ struct ed_sector {
     double jhund = 0.0, cutoff = 0.0;
     int32_t *d_rank = nullptr;       // [2^ns] 0-based rank inside the popcount bucket
     ed::SpinTables up, dn;
+    // keys of up / dn in the process's cache of spin tables (empty: the sector owns them)
+    std::string up_key, dn_key;
     double *d_uimp = nullptr;
     float *d_uimp32 = nullptr;
     int hund = 0, nhund = 0;

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -512,6 +513,86 @@ static int build_interaction(ed_sector *S)
     return ED_OK;
 }
 
+// ------------------------------------------------------------ spin-table cache
+// The tables of one spin depend only on that spin's model matrices, the cutoff, the orbital
+// counts, the electron count and the device, not on the other spin: the 157 Lanczos sectors
+// of an Ns = 12 scan have 13 distinct tables per spin.  Sectors of the same model share
+// them: a table is built by the first sector that needs it, kept while any sector holds it
+// (reference count) and freed with the last one.  The tables are never written after they
+// are built, and the builder synchronises its stream before it publishes them.  The packed
+// up-hop table has kPkRows rows (at least the spin's own width), enough for every
+// compiled width of the fast kernels, so it does not depend on the other spin either.
+constexpr int kPkRows = 16;
+struct SpinCacheEntry {
+    SpinTables T;
+    int refs = 0;
+};
+static std::mutex g_spin_mu;
+static std::map<std::string, SpinCacheEntry> g_spin_cache;
+
+static std::string spin_key(const ed_sector *S, int spin, int n)
+{
+    std::string k;
+    auto put = [&](const void *p, size_t bytes) { k.append(reinterpret_cast<const char *>(p), bytes); };
+    const int hdr[7] = {S->device, S->ns, S->nc, S->nb, n, S->cplx, S->ncfg};
+    put(hdr, sizeof hdr);
+    put(&S->cutoff, sizeof(double));
+    const size_t nc2 = (size_t)S->nc * S->nc, nb2 = (size_t)S->nb * S->nb, nbc = (size_t)S->nb * S->nc;
+    put(S->ec.data() + spin * nc2, sizeof(double) * nc2);
+    put(S->eb.data() + spin * nb2, sizeof(double) * nb2);
+    put(S->vbc.data() + spin * nbc, sizeof(double) * nbc);
+    if (S->cplx) {
+        put(S->ec_im.data() + spin * nc2, sizeof(double) * nc2);
+        put(S->eb_im.data() + spin * nb2, sizeof(double) * nb2);
+        put(S->vbc_im.data() + spin * nbc, sizeof(double) * nbc);
+    }
+    return k;
+}
+
+// The tables of `spin` with n electrons for sector S: from the cache, or built (with the
+// packed table of a real sector) and entered in it.  `key` is set once S holds a reference.
+static int spin_acquire(ed_sector *S, int spin, int n, SpinTables &T, std::string &key)
+{
+    const std::string k = spin_key(S, spin, n);
+    {
+        std::lock_guard<std::mutex> lk(g_spin_mu);
+        auto it = g_spin_cache.find(k);
+        if (it != g_spin_cache.end()) {
+            T = it->second.T;
+            ++it->second.refs;
+            key = k;
+            return ED_OK;
+        }
+    }
+    ED_TRY(build_spin(S, spin, n, T));
+    if (!S->cplx) ED_TRY(build_packed(T, std::max(kPkRows, T.kpad), S->stream));
+    std::lock_guard<std::mutex> lk(g_spin_mu);
+    auto ins = g_spin_cache.emplace(k, SpinCacheEntry{T, 1});
+    if (!ins.second) {   // another thread built the same tables meanwhile: use those
+        free_spin(T);
+        T = ins.first->second.T;
+        ++ins.first->second.refs;
+    }
+    key = k;
+    return ED_OK;
+}
+
+static void spin_release(SpinTables &T, std::string &key)
+{
+    if (key.empty()) {   // owned by the sector (S_z tables, or a build that failed half-way)
+        free_spin(T);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_spin_mu);
+    auto it = g_spin_cache.find(key);
+    if (it != g_spin_cache.end() && --it->second.refs == 0) {
+        free_spin(it->second.T);
+        g_spin_cache.erase(it);
+    }
+    T = SpinTables();
+    key.clear();
+}
+
 // Streams of released sectors, kept for the next sectors created on the same device:
 // creating and destroying a HIP stream costs about a millisecond each (844 and 615 us on
 // average over the 157 Lanczos sectors of an Ns = 12 scan), more than building a small
@@ -553,8 +634,8 @@ static void stream_release(int device, hipStream_t s)
 
 static void free_sector(ed_sector *S)
 {
-    free_spin(S->up);
-    free_spin(S->dn);
+    spin_release(S->up, S->up_key);
+    spin_release(S->dn, S->dn_key);
     void *ps[] = {S->d_rank, S->d_uimp, S->d_uimp32, S->d_hund_ptr, S->d_hund_cu, S->d_hund_cd, S->d_hund_val, S->d_hund_tr,
                   S->d_hund_val32, S->d_basis, S->d_state, S->d_partials, S->d_io_a, S->d_io_b, S->d_colord, S->d_pmap, S->d_qx,
                   S->d_quart, S->d_quart32, S->d_sz_hrk, S->d_sz_hval, S->d_sz_hval32, S->d_sz_mat};
@@ -611,13 +692,10 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
     if (stream_acquire(&S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
     if ((rc = dalloc(&S->d_rank, (size_t)1 << ns)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
-    if ((rc = build_spin(S, 0, nup, S->up)) != ED_OK) return bail(rc);
-    if ((rc = build_spin(S, 1, ndn, S->dn)) != ED_OK) return bail(rc);
-    if (!S->cplx) {   // packed coefficients of the fast real kernels
-        const int rows = std::max(S->up.kpad, S->dn.kpad);
-        if ((rc = build_packed(S->up, rows, S->stream)) != ED_OK) return bail(rc);
-        if ((rc = build_packed(S->dn, rows, S->stream)) != ED_OK) return bail(rc);
-    }
+    // the spin tables, shared with the other sectors of the model (with the packed
+    // coefficients of the fast real kernels)
+    if ((rc = spin_acquire(S, 0, nup, S->up, S->up_key)) != ED_OK) return bail(rc);
+    if ((rc = spin_acquire(S, 1, ndn, S->dn, S->dn_key)) != ED_OK) return bail(rc);
     if ((rc = build_interaction(S)) != ED_OK) return bail(rc);
     S->dim = (int64_t)S->up.d * S->dn.d;
     if (S->cplx) {

@@ -107,6 +107,38 @@ def test_hmult_workloads_fp64(gpu, name):
     s.close()
 
 
+def test_shared_spin_tables(gpu):
+    """Sectors of one model share the tables of each (spin, N): (5,5), (5,4) and (4,5) of
+    workload B hold the N = 5 tables of both spins, and a second model with other bath
+    levels gets its own.  Each H.v matches the oracle (1e-13) while the others are alive,
+    and after the sectors that built the shared tables are released."""
+    m, _ = golden("B")
+    m2, _ = golden("B")
+    m2.eb = np.array(m2.eb, dtype=np.float64)
+    m2.eb[:, 0, 0] += 0.25   # another model: its own tables
+    keys = [(5, 5), (5, 4), (4, 5)]
+    secs = {k: gpu.Sector(m, *k) for k in keys}
+    other = gpu.Sector(m2, 5, 5)
+    rng = np.random.default_rng(11)
+
+    def check(sec, mm, k):
+        o = oracle.OracleSector(mm, *k, nthreads=8)
+        v = rng.uniform(-1, 1, sec.dim)
+        assert rel(sec.hmult(v), o.hmult(v)) < 1e-13, k
+        o.close()
+
+    for k in keys:
+        check(secs[k], m, k)
+    check(other, m2, (5, 5))
+    secs.pop((5, 5)).close()   # the first builder of both N = 5 tables
+    check(secs[(5, 4)], m, (5, 4))
+    check(secs[(4, 5)], m, (4, 5))
+    again = gpu.Sector(m, 5, 5)   # from the tables still held
+    check(again, m, (5, 5))
+    for x in list(secs.values()) + [again, other]:
+        x.close()
+
+
 @pytest.mark.parametrize("name", ["B", "C", "E"])
 def test_hmult_every_schedule(gpu, name):
     """Every H.v schedule the autotuner can pick gives the oracle's H.v (FP64 1e-13)."""

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C-ABI) against the oracle, on the GPU.
 
 Bars: integer tables / states / ranks / apply_C bit-exact; FP64 H.v within 1e-13
-relative (max |diff| / max |w|); E0 within 1e-10 (north-star tolerance for FP64);
+relative (max |diff| / max |w|); E0 within 1e-10 (the FP64 tolerance of every E0 test here);
 G(i w_n) within 1e-10 relative; FP32 H.v within 1e-5 relative.
 """
 import json

@@ -36,6 +36,19 @@ class EigenState:
     vector: np.ndarray
 
 
+# Least up-spin dimension the multi-run Lanczos kernel takes (ed_lanczos_batched runs
+# smaller ones one by one).
+_RUNS_MIN_DUP = 64
+
+
+def spin_reversed(model: AIMParams) -> AIMParams:
+    """The model with the spin labels exchanged (Ec, Eb, Vbc of spin up and down swapped;
+    U, J and Jhund are spin-symmetric): its sector (N_dn, N_up) has the Hamiltonian of the
+    original's (N_up, N_dn) in a relabelled basis."""
+    return AIMParams(model.nc, model.nb, model.ec[::-1].copy(), model.eb[::-1].copy(), model.vbc[::-1].copy(),
+                     model.u, model.jj, model.jhund, model.cutoff)
+
+
 def all_sectors(ns: int) -> list[tuple[int, int]]:
     return [(nu, nd) for nu in range(ns + 1) for nd in range(ns + 1)]
 
@@ -121,7 +134,12 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     fixed steps of every sector in one batched device call, ed_lanczos_batched, then the
     reference's convergence test replayed on each sector's coefficients,
     ed_lanczos_converged: the same iteration and Ritz value as the sector's own loop up to
-    rounding), else sector by sector.  Pass 2: the sectors with e^{-beta (E_s - E0)} >
+    rounding), else sector by sector.  In the batched form a sector whose up spin has fewer
+    than 64 states (the multi-run kernel's minimum) and whose down spin has more runs as the
+    spin-reversed sector (N_dn, N_up) of the spin-reversed model: the same Hamiltonian up to
+    a relabelling of the basis (the spins' strings are signed independently and every
+    interaction term is symmetric under the exchange), so the same spectrum, reached from
+    another start vector.  Pass 2: the sectors with e^{-beta (E_s - E0)} >
     weight_cutoff get `neigen` states by block Lanczos of width `block_size` (plain Lanczos
     in the sectors solver.h:152 excludes); states below the weight cutoff are dropped.
     Returns (states, info)."""
@@ -131,6 +149,7 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     t0 = time.perf_counter()
     e_low = {}
     lanczos_keys = []
+    reversed_model = None
     for (nu, nd) in all_sectors(ns):
         dim = comb(ns, nu) * comb(ns, nd)
         if dim <= full_ed_switch:
@@ -145,8 +164,17 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     groups: dict[int, list] = {}
     for key in lanczos_keys:
         groups.setdefault(min(nitermax, comb(ns, key[0]) * comb(ns, key[1]) - 1), []).append(key)
+    def pass1_sector(k):
+        nonlocal reversed_model
+        nu, nd = k
+        if comb(ns, nu) < _RUNS_MIN_DUP <= comb(ns, nd):
+            if reversed_model is None:
+                reversed_model = spin_reversed(model)
+            return Sector(reversed_model, nd, nu)
+        return Sector(model, nu, nd)
+
     for niter, keys in groups.items():
-        secs = [Sector(model, *k) for k in keys]
+        secs = [pass1_sector(k) for k in keys]
         try:
             a, b, steps = lanczos_batched(secs, niter)
         finally:

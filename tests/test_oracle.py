@@ -337,3 +337,28 @@ def test_hubbard_atom_green_function():
 def test_matsubara_axis():
     w = oracle.matsubara(50.0, 4)
     assert np.allclose(w, [(2 * n - 1) * np.pi / 50.0 for n in range(1, 5)], rtol=0, atol=1e-15)
+
+
+def test_spin_reversed_model_has_the_same_sector_spectrum():
+    """The sector scan's first pass runs a sector whose up spin is small as the spin-reversed
+    sector of the spin-reversed model: the brute-force Hamiltonians of (N_up, N_dn) and of the
+    reversed (N_dn, N_up) have the same spectrum, for spin-dependent levels, bath hoppings,
+    hybridisations and Hund terms (spin flip and pair hopping); a model whose spins differ
+    changes the spectrum when only the sector is reversed."""
+    from ed_solver_amd.solver import spin_reversed
+    rng = np.random.default_rng(21)
+    nc, nb = 2, 3
+    ec = np.stack([np.array([[-1.0, 0.2], [0.2, -0.6]]), np.array([[-0.7, 0.1], [0.1, -0.9]])])
+    eb = rng.uniform(-1, 1, (2, nb, nb))
+    eb = 0.5 * (eb + eb.transpose(0, 2, 1))
+    vbc = rng.uniform(-0.6, 0.6, (2, nb, nc))
+    u = np.array([[3.0, 2.0], [2.0, 3.0]])
+    jj = np.full((2, 2), 0.5)
+    m = AIMParams(nc, nb, ec, eb, vbc, u, jj, 0.5, 1e-8)
+    r = spin_reversed(m)
+    for nu, nd in [(1, 3), (2, 4), (0, 2)]:
+        w = la.eigvalsh(bf.sector_hamiltonian(m, nu, nd).toarray())
+        wr = la.eigvalsh(bf.sector_hamiltonian(r, nd, nu).toarray())
+        assert np.max(np.abs(w - wr)) < 1e-12, (nu, nd)
+    w_wrong = la.eigvalsh(bf.sector_hamiltonian(m, 3, 1).toarray())
+    assert np.max(np.abs(la.eigvalsh(bf.sector_hamiltonian(m, 1, 3).toarray()) - w_wrong)) > 1e-3

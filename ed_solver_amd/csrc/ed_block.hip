@@ -194,6 +194,58 @@ __global__ void k_block_start(int64_t n, int b, double *Q, int64_t ld)
         }
 }
 
+// Cholesky factor of a b x b Gram matrix G (row-major, b <= 16) on the device: one
+// workgroup, thread t owning entry (t / 16, t % 16).  Right-looking elimination writes
+// R (upper triangular, G = R^T R) and Ri = R^{-1} by back substitution, and *ok = 1; when
+// G is not numerically positive definite relative to its largest diagonal entry (a pivot
+// <= 1e-26 max_i G_ii, or none positive) it writes R = 0, Ri = I and *ok = 0, so that
+// W Ri leaves W as it is.  With a_out set it also copies the b x b blocks a1 and a2 to
+// a_out[0, b^2) and a_out[b^2, 2 b^2), for the host to read with the factors in one copy.
+__global__ __launch_bounds__(256) void k_chol_upper(int b, const double *G, double *R, double *Ri, double *ok,
+                                                    const double *a1, const double *a2, double *a_out)
+{
+    __shared__ double A[16][17], Rs[16][17], Is[16][17];
+    const int t = threadIdx.x, i = t >> 4, j = t & 15;
+    const bool in = i < b && j < b;
+    A[i][j] = in ? G[i * b + j] : 0.0;
+    Rs[i][j] = 0.0;
+    Is[i][j] = 0.0;
+    if (a_out && t < b * b) {
+        a_out[t] = a1[t];
+        a_out[b * b + t] = a2[t];
+    }
+    __syncthreads();
+    double gmax = 0.0;
+    for (int k = 0; k < b; ++k) gmax = fmax(gmax, A[k][k]);
+    bool good = gmax > 0.0;
+    for (int k = 0; k < b && good; ++k) {   // every thread reads the same pivot: uniform exit
+        const double piv = A[k][k];
+        if (!(piv > 1e-26 * gmax)) {
+            good = false;
+            break;
+        }
+        const double rkk = sqrt(piv);
+        if (i == k && j >= k && j < b) Rs[k][j] = j == k ? rkk : A[k][j] / rkk;
+        __syncthreads();
+        if (i > k && j >= i && j < b) A[i][j] -= Rs[k][i] * Rs[k][j];
+        __syncthreads();
+    }
+    if (good)
+        for (int r = b - 1; r >= 0; --r) {
+            if (i == r && j >= r && j < b) {
+                double v = j == r ? 1.0 : 0.0;
+                for (int k = r + 1; k <= j; ++k) v -= Rs[r][k] * Is[k][j];
+                Is[r][j] = v / Rs[r][r];
+            }
+            __syncthreads();
+        }
+    if (in) {
+        R[i * b + j] = good ? Rs[i][j] : 0.0;
+        Ri[i * b + j] = good ? Is[i][j] : (i == j ? 1.0 : 0.0);
+    }
+    if (t == 0) *ok = good ? 1.0 : 0.0;
+}
+
 // ---------------------------------------------------------------- host side
 
 // Work space of the contractions: Gram partials and the small matrices, per device,
@@ -315,66 +367,50 @@ void block_xc_dev(int64_t n, int p, int q, const double *X, int64_t ldx, const d
     xc_dev(n, p, q, X, ldx, d_C, alpha, beta, Y, ldy, st);
 }
 
-// Upper-triangular R with G = R^T R (G b x b row-major, symmetric); false when G is not
-// numerically positive definite relative to its largest diagonal entry.
-static bool cholesky_upper(int b, const std::vector<double> &G, std::vector<double> &R)
+// Small device buffers of the Cholesky-QR2 below: the Gram matrix, the inverse factor, and
+// the read-back block [R1 | R2 | A1 | A2 | ok1 | ok2] (b x b each, two flags).
+struct CholBufs {
+    double *G, *Ri, *out;
+    static size_t out_size(int b) { return 4 * (size_t)b * b + 2; }
+};
+
+// In place W <- W R1^{-1} R2^{-1} with W^T W = R1^T R1, then (W R1^{-1})^T (W R1^{-1}) =
+// R2^T R2 (Cholesky-QR, twice), queued on st with no host wait.  The factors and flags
+// land in c.out (R1, R2, ok1, ok2; W Rtot with Rtot = R2 R1 is the original block); with
+// a1, a2 set the last factor kernel copies those b x b projection blocks to
+// c.out + 2 b^2.  A block that is numerically rank deficient gives ok1 or ok2 = 0 (W is
+// then left as the identity factor found it).
+static int chol_qr2_dev(int dev, int64_t n, int b, double *W, int64_t ld, const CholBufs &c, const double *a1,
+                        const double *a2, hipStream_t st)
 {
-    R.assign((size_t)b * b, 0.0);
-    double gmax = 0.0;
-    for (int i = 0; i < b; ++i) gmax = std::max(gmax, G[(size_t)i * b + i]);
-    if (!(gmax > 0.0)) return false;
-    for (int i = 0; i < b; ++i) {
-        double s = G[(size_t)i * b + i];
-        for (int k = 0; k < i; ++k) s -= R[(size_t)k * b + i] * R[(size_t)k * b + i];
-        if (!(s > 1e-26 * gmax)) return false;
-        const double rii = std::sqrt(s);
-        R[(size_t)i * b + i] = rii;
-        for (int j = i + 1; j < b; ++j) {
-            double t = G[(size_t)i * b + j];
-            for (int k = 0; k < i; ++k) t -= R[(size_t)k * b + i] * R[(size_t)k * b + j];
-            R[(size_t)i * b + j] = t / rii;
-        }
+    const size_t bb = (size_t)b * b;
+    for (int pass = 0; pass < 2; ++pass) {
+        ED_TRY(gram_dev(dev, n, b, b, W, ld, W, ld, c.G, st));
+        const bool last = pass == 1 && a1;
+        hipLaunchKernelGGL(k_chol_upper, dim3(1), dim3(256), 0, st, b, (const double *)c.G, c.out + pass * bb, c.Ri,
+                           c.out + 4 * bb + pass, last ? a1 : nullptr, last ? a2 : nullptr,
+                           last ? c.out + 2 * bb : nullptr);
+        xc_dev(n, b, b, W, ld, c.Ri, 1.0, 0.0, W, ld, st);
+        ED_HIP_CHECK(hipGetLastError());
     }
-    return true;
+    return ED_OK;
 }
 
-// Inverse of an upper-triangular b x b matrix (row-major).
-static std::vector<double> upper_inverse(int b, const std::vector<double> &R)
+// Rtot = R2 R1 (upper triangular, row-major) from a read-back block of chol_qr2_dev;
+// false when either factorisation failed.
+static bool chol_result(int b, const double *h, std::vector<double> &Rtot)
 {
-    std::vector<double> Ri((size_t)b * b, 0.0);
-    for (int j = 0; j < b; ++j) {
-        Ri[(size_t)j * b + j] = 1.0 / R[(size_t)j * b + j];
-        for (int i = j - 1; i >= 0; --i) {
-            double s = 0.0;
-            for (int k = i + 1; k <= j; ++k) s += R[(size_t)i * b + k] * Ri[(size_t)k * b + j];
-            Ri[(size_t)i * b + j] = -s / R[(size_t)i * b + i];
-        }
-    }
-    return Ri;
-}
-
-// In place W <- W R^{-1} with W^T W = R^T R (Cholesky-QR, twice); Rtot = R2 R1.
-// Returns false when the block is numerically rank deficient.
-static int chol_qr2(int dev, int64_t n, int b, double *W, int64_t ld, std::vector<double> &Rtot, bool &ok,
-                    hipStream_t st)
-{
-    std::vector<double> G((size_t)b * b), R1, R2;
-    ED_TRY(gram(dev, n, b, b, W, ld, W, ld, G.data(), st));
-    ok = cholesky_upper(b, G, R1);
-    if (!ok) return ED_OK;
-    ED_TRY(xc(dev, n, b, b, W, ld, upper_inverse(b, R1).data(), 1.0, 0.0, W, ld, st));
-    ED_TRY(gram(dev, n, b, b, W, ld, W, ld, G.data(), st));
-    ok = cholesky_upper(b, G, R2);
-    if (!ok) return ED_OK;
-    ED_TRY(xc(dev, n, b, b, W, ld, upper_inverse(b, R2).data(), 1.0, 0.0, W, ld, st));
-    Rtot.assign((size_t)b * b, 0.0);
+    const size_t bb = (size_t)b * b;
+    if (!(h[4 * bb] == 1.0 && h[4 * bb + 1] == 1.0)) return false;
+    const double *R1 = h, *R2 = h + bb;
+    Rtot.assign(bb, 0.0);
     for (int i = 0; i < b; ++i)
         for (int j = i; j < b; ++j) {
             double s = 0.0;
             for (int k = i; k <= j; ++k) s += R2[(size_t)i * b + k] * R1[(size_t)k * b + j];
             Rtot[(size_t)i * b + j] = s;
         }
-    return ED_OK;
+    return true;
 }
 
 int launch_hv(ed_sector *s, int prec, const void *x, void *y, int64_t ld, int nvec, hipStream_t st);
@@ -524,43 +560,61 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     const int maxsteps = (int)std::max<int64_t>(1, std::min<int64_t>(o.max_steps, (n + b - 1) / b));
     if ((int64_t)maxsteps * b < nev) return fail(ED_ERR_ARG, "block Lanczos: max_steps * block_size < nev");
     const size_t nvec = (size_t)(maxsteps + 1) * b;
-    double *Q = nullptr, *HX = nullptr, *dG = nullptr;
+    const size_t bb = (size_t)b * b, nout = CholBufs::out_size(b);
+    double *Q = nullptr, *HX = nullptr, *dG = nullptr, *hb = nullptr;
     std::string err;
     {
         hipError_t e = hipMalloc((void **)&Q, sizeof(double) * ld * nvec);
         if (e != hipSuccess)
             return fail(ED_ERR_NOMEM, "block Lanczos basis of " + std::to_string(nvec) + " vectors: " + hipGetErrorString(e));
-        e = hipMalloc((void **)&dG, sizeof(double) * 2 * nvec * b);
+        // projections of the two CGS passes, then the Cholesky-QR2 buffers: the Gram matrix,
+        // the inverse factor, and two read-back blocks (block steps; start block and
+        // re-projections)
+        e = hipMalloc((void **)&dG, sizeof(double) * (2 * nvec * b + 2 * bb + 2 * nout));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&hb, sizeof(double) * 2 * nout, hipHostMallocDefault);
         if (e != hipSuccess) {
             (void)hipFree(Q);
+            if (dG) (void)hipFree(dG);
             return fail(ED_ERR_NOMEM, std::string("block Lanczos projections: ") + hipGetErrorString(e));
         }
     }
     struct Guard {
-        double *&a, *&b, *&c;
+        double *&a, *&b, *&c, *&h;
+        hipStream_t st;
         ~Guard()
         {
+            (void)hipStreamSynchronize(st);   // queued steps may still use the buffers
             if (a) (void)hipFree(a);
             if (b) (void)hipFree(b);
             if (c) (void)hipFree(c);
+            if (h) (void)hipHostFree(h);
         }
-    } guard{Q, HX, dG};
+    } guard{Q, HX, dG, hb, st};
     {   // size the Gram workspace once for the widest product, so queued Grams never move it
         BlockWS *w;
         ED_TRY(bws_get(dev, gram_part_size(n, (int)nvec), 256, w));
     }
+    double *dG1 = dG, *dG2 = dG + nvec * b;
+    const CholBufs cb_step{dG + 2 * nvec * b, dG + 2 * nvec * b + bb, dG + 2 * nvec * b + 2 * bb};
+    const CholBufs cb_aux{cb_step.G, cb_step.Ri, cb_step.out + nout};
+    double *hb_step = hb, *hb_aux = hb + nout;
+    auto read_back = [&](const CholBufs &c, double *h) -> int {
+        ED_HIP_CHECK(hipMemcpyAsync(h, c.out, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+        ED_HIP_CHECK(hipStreamSynchronize(st));
+        return ED_OK;
+    };
     hipLaunchKernelGGL(k_block_start, dim3(1024), dim3(256), 0, st, n, b, Q, ld);
     ED_HIP_CHECK(hipGetLastError());
     std::vector<double> R0;
-    bool ok = true;
-    ED_TRY(chol_qr2(dev, n, b, Q, ld, R0, ok, st));
-    if (!ok) return fail(ED_ERR_BREAKDOWN, "block Lanczos: start block is rank deficient");
+    ED_TRY(chol_qr2_dev(dev, n, b, Q, ld, cb_aux, nullptr, nullptr, st));
+    ED_TRY(read_back(cb_aux, hb_aux));
+    if (!chol_result(b, hb_aux, R0)) return fail(ED_ERR_BREAKDOWN, "block Lanczos: start block is rank deficient");
 
     const int nmax = maxsteps * b;
     std::vector<double> T((size_t)nmax * nmax, 0.0);
     auto Tat = [&](int i, int j) -> double & { return T[(size_t)i * nmax + j]; };
-    std::vector<double> G1, G2, Bnext, prev(nev, 1e300), w(nev), Z;
-    int steps = 0, done = 0;
+    std::vector<double> Bnext, prev(nev, 1e300), w(nev), Z;
+    int steps = 0;
     int next_test = std::max(1, o.check_every);   // block steps at the next Ritz test
     bool exhausted = false;
     std::vector<double> lastB;
@@ -571,66 +625,87 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     const bool partial = o.reorth == 1;
     BlockOmega om(b);
     int full_due = 0;
-    for (int j = 0; j < maxsteps && !done; ++j) {
+    // One block step on the device, queued with no host wait: W = H Q_j, CGS2 against
+    // Q_{c0..j}, Cholesky-QR2 of W (not at the last step), and the read-back of A_j's
+    // projections and the factors into hb_step.  The host takes the step's results after
+    // one stream wait, and issues the next step before its own work on them (the omega
+    // recurrence, the Ritz test), so that the device runs step j + 1 meanwhile.  When the
+    // recurrence then asks for a full projection of Q_{j+1}, that projection is queued
+    // behind the step already issued (which read Q_{j+1} first) and the step is issued
+    // again: the results are those of the sequential order.
+    struct Step {
+        int j = 0, c0 = 0;
+        bool full = false, last = false;
+    };
+    auto issue = [&](int j, Step &sp) -> int {
         const int p = (j + 1) * b;   // columns of Q_{1..j+1}
-        const bool full = !partial || full_due > 0 || j < 2;
-        const int c0 = full ? 0 : (j - 1) * b;   // first column the new block is projected against
-        const int pw = p - c0;
-        double *Qj = Q + (size_t)j * b * ld, *W = Q + (size_t)(j + 1) * b * ld, *Qc = Q + (size_t)c0 * ld;
+        sp.j = j;
+        sp.full = !partial || full_due > 0 || j < 2;
+        sp.c0 = sp.full ? 0 : (j - 1) * b;   // first column the new block is projected against
+        sp.last = (j + 1 == maxsteps) || (int64_t)p + b > n;
+        const int pw = p - sp.c0;
+        double *Qj = Q + (size_t)j * b * ld, *W = Q + (size_t)(j + 1) * b * ld, *Qc = Q + (size_t)sp.c0 * ld;
         ED_TRY(launch_hv(s, ED_PREC_F64, Qj, W, ld, b, st));
-        G1.assign((size_t)pw * b, 0.0);
-        G2.assign((size_t)pw * b, 0.0);
-        // CGS2 entirely on the device: the projections stay there between the products
-        double *dG1 = dG, *dG2 = dG + nvec * b;
         ED_TRY(gram_dev(dev, n, pw, b, Qc, ld, W, ld, dG1, st));
         xc_dev(n, pw, b, Qc, ld, dG1, -1.0, 1.0, W, ld, st);
         ED_TRY(gram_dev(dev, n, pw, b, Qc, ld, W, ld, dG2, st));
         xc_dev(n, pw, b, Qc, ld, dG2, -1.0, 1.0, W, ld, st);
         ED_HIP_CHECK(hipGetLastError());
-        ED_HIP_CHECK(hipMemcpyAsync(G1.data(), dG1, sizeof(double) * pw * b, hipMemcpyDeviceToHost, st));
-        ED_HIP_CHECK(hipMemcpyAsync(G2.data(), dG2, sizeof(double) * pw * b, hipMemcpyDeviceToHost, st));
+        const double *a1 = dG1 + (size_t)(j * b - sp.c0) * b, *a2 = dG2 + (size_t)(j * b - sp.c0) * b;
+        if (sp.last) {
+            ED_HIP_CHECK(hipMemcpyAsync(hb_step + 2 * bb, a1, sizeof(double) * bb, hipMemcpyDeviceToHost, st));
+            ED_HIP_CHECK(hipMemcpyAsync(hb_step + 3 * bb, a2, sizeof(double) * bb, hipMemcpyDeviceToHost, st));
+        } else {
+            ED_TRY(chol_qr2_dev(dev, n, b, W, ld, cb_step, a1, a2, st));
+            ED_HIP_CHECK(hipMemcpyAsync(hb_step, cb_step.out, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+        }
+        return ED_OK;
+    };
+    Step cur, nxt;
+    ED_TRY(issue(0, cur));
+    for (;;) {
+        ED_HIP_CHECK(hipStreamSynchronize(st));
+        const int j = cur.j, p = (j + 1) * b;
         steps = j + 1;
-        if (full) {
+        if (cur.full) {
             ++res.full_passes;
             if (full_due > 0) --full_due;
         }
-        bool last = (j + 1 == maxsteps) || (int64_t)p + b > n;
-        auto diag_block = [&]() {   // A_j, symmetrised (G1, G2 are on the host after a stream wait)
-            const int r0 = j * b - c0;
+        {   // A_j, symmetrised: the Q_j rows of the two projections
+            const double *a1 = hb_step + 2 * bb, *a2 = hb_step + 3 * bb;
             for (int r = 0; r < b; ++r)
                 for (int c = 0; c < b; ++c) {
-                    const double a1 = G1[(size_t)(r0 + r) * b + c] + G2[(size_t)(r0 + r) * b + c];
-                    const double a2 = G1[(size_t)(r0 + c) * b + r] + G2[(size_t)(r0 + c) * b + r];
-                    Tat(j * b + r, j * b + c) = 0.5 * (a1 + a2);
+                    const double x = a1[(size_t)r * b + c] + a2[(size_t)r * b + c];
+                    const double y = a1[(size_t)c * b + r] + a2[(size_t)c * b + r];
+                    Tat(j * b + r, j * b + c) = 0.5 * (x + y);
                 }
-        };
-        if (last) {
-            ED_HIP_CHECK(hipStreamSynchronize(st));
-            diag_block();
+        }
+        bool last = cur.last;
+        if (!last && !chol_result(b, hb_step, Bnext)) {
+            exhausted = true;   // the Krylov space is invariant: T is exact on it
+            last = true;
         }
         if (!last) {
-            ED_TRY(chol_qr2(dev, n, b, W, ld, Bnext, ok, st));   // waits on the stream: G1, G2 are in
-            diag_block();
-            if (!ok) {
-                exhausted = true;   // the Krylov space is invariant: T is exact on it
-                last = true;
-            } else if (partial) {
+            ED_TRY(issue(j + 1, nxt));
+            if (partial) {
                 // omega-recurrence for Q_k^T Q_{j+1}, k < j; above sqrt(eps): project the new
                 // block against every block now (and the next block as well)
-                const bool over = om.advance(j, b, nmax, T, Bnext, full);
-                if (over && !full) {
+                const bool over = om.advance(j, b, nmax, T, Bnext, cur.full);
+                if (over && !cur.full) {
+                    double *W = Q + (size_t)(j + 1) * b * ld;
                     ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
                     xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
                     ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
                     xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
+                    ED_TRY(chol_qr2_dev(dev, n, b, W, ld, cb_aux, nullptr, nullptr, st));
+                    ED_TRY(read_back(cb_aux, hb_aux));
                     std::vector<double> R2;
-                    ED_TRY(chol_qr2(dev, n, b, W, ld, R2, ok, st));
-                    if (!ok) {
+                    if (!chol_result(b, hb_aux, R2)) {
                         exhausted = true;
                         last = true;
                     } else {
                         // Q_{j+1} B_{j+1} = (Q' R2) B_{j+1}: the new B is R2 B (upper triangular)
-                        std::vector<double> Bn((size_t)b * b, 0.0);
+                        std::vector<double> Bn(bb, 0.0);
                         for (int r = 0; r < b; ++r)
                             for (int c = r; c < b; ++c) {
                                 double t = 0.0;
@@ -642,6 +717,7 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
                     }
                     ++res.full_passes;
                     full_due = 1;
+                    if (!last) ED_TRY(issue(j + 1, nxt));   // the step issued above used the unprojected block
                 }
             }
             if (!last) {
@@ -654,59 +730,62 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
             }
         }
         const int m = p;
-        if (m < nev) continue;
         // Ritz test after check_every steps, then at intervals that grow with T (steps / 4:
         // the band solver costs O(m^2 b) per test) while the Ritz values still move by more
         // than sqrt(tolerance) between tests, every check_every steps once they move less
         // (so convergence is seen within a few steps of the test that would see it), and at
         // the last step
-        if (!last && steps < next_test) continue;
-        // Ritz values of T_j (leading m x m block); the vectors (for the residual bounds and
-        // the Ritz vectors) only once the values have settled, or at the last step
-        std::vector<double> Tm((size_t)m * m);
-        for (int r = 0; r < m; ++r)
-            for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
-        const auto t_r = std::chrono::steady_clock::now();
-        // one reduction (rotations logged) serves the values and, once they settle, the vectors
-        BandTri bt;
-        ED_TRY(band_reduce(m, b, Tm.data(), true, bt));
-        ED_TRY(band_tri_values(bt, nev, w.data()));
-        bool settled = steps > 1;
-        double moved = 0.0;
-        for (int k = 0; k < nev; ++k) {
-            const double dk = std::fabs(w[k] - prev[k]);
-            if (!(dk < o.tolerance)) settled = false;
-            moved = std::max(moved, std::isfinite(dk) ? dk : 1e300);
-        }
-        const int every = std::max(1, o.check_every);
-        next_test = steps + (moved < std::sqrt(o.tolerance) ? every : std::max(every, steps / 4));
-        if (last || settled) {
-            Z.assign((size_t)m * nev, 0.0);
-            ED_TRY(band_tri_vectors(bt, nev, w.data(), Z.data()));
-        }
-        ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_r).count();
-        if (last) {
-            done = 1;
-            break;
-        }
-        bool conv = settled;
-        for (int k = 0; k < nev && conv; ++k) {
-            // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
-            double rr = 0.0;
-            for (int r = 0; r < b; ++r) {
-                double t = 0.0;
-                for (int c = r; c < b; ++c) t += lastB[(size_t)r * b + c] * Z[(size_t)k * m + (m - b + c)];
-                rr += t * t;
+        if (m >= nev && (last || steps >= next_test)) {
+            // Ritz values of T_j (leading m x m block); the vectors (for the residual bounds
+            // and the Ritz vectors) only once the values have settled, or at the last step
+            std::vector<double> Tm((size_t)m * m);
+            for (int r = 0; r < m; ++r)
+                for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
+            const auto t_r = std::chrono::steady_clock::now();
+            // one reduction (rotations logged) serves the values and, once they settle, the vectors
+            BandTri bt;
+            ED_TRY(band_reduce(m, b, Tm.data(), true, bt));
+            ED_TRY(band_tri_values(bt, nev, w.data()));
+            bool settled = steps > 1;
+            double moved = 0.0;
+            for (int k = 0; k < nev; ++k) {
+                const double dk = std::fabs(w[k] - prev[k]);
+                if (!(dk < o.tolerance)) settled = false;
+                moved = std::max(moved, std::isfinite(dk) ? dk : 1e300);
             }
-            const double rnorm = std::sqrt(rr);
-            if (!(std::fabs(w[k] - prev[k]) < o.tolerance && rnorm < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))))
-                conv = false;
+            const int every = std::max(1, o.check_every);
+            next_test = steps + (moved < std::sqrt(o.tolerance) ? every : std::max(every, steps / 4));
+            if (last || settled) {
+                Z.assign((size_t)m * nev, 0.0);
+                ED_TRY(band_tri_vectors(bt, nev, w.data(), Z.data()));
+            }
+            ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_r).count();
+            if (last) break;
+            bool conv = settled;
+            for (int k = 0; k < nev && conv; ++k) {
+                // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
+                double rr = 0.0;
+                for (int r = 0; r < b; ++r) {
+                    double t = 0.0;
+                    for (int c = r; c < b; ++c) t += lastB[(size_t)r * b + c] * Z[(size_t)k * m + (m - b + c)];
+                    rr += t * t;
+                }
+                const double rnorm = std::sqrt(rr);
+                if (!(std::fabs(w[k] - prev[k]) < o.tolerance &&
+                      rnorm < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))))
+                    conv = false;
+            }
+            for (int k = 0; k < nev; ++k) prev[k] = w[k];
+            if (conv) break;   // a step issued meanwhile only wrote Q_{j+2} and the read-back
         }
-        for (int k = 0; k < nev; ++k) prev[k] = w[k];
-        if (conv) done = 1;
+        if (last) break;   // m < nev at the last step: no Ritz pairs (rejected by the argument checks)
+        cur = nxt;
     }
     const int m = steps * b;
     (void)exhausted;
+    if (Z.size() < (size_t)m * nev)
+        return fail(ED_ERR_BREAKDOWN, "block Lanczos: the Krylov space (dimension " + std::to_string(m) +
+                                          ") is smaller than the number of states asked for");
     // Ritz vectors X = Q_{1..m} S and the true residuals
     res.evals.assign(w.begin(), w.end());
     res.steps = steps;

@@ -117,6 +117,8 @@ SIGNATURES = {
                                         _c_int_p]),
     "ed_block_lanczos_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), _dp, _dp, _vp, ctypes.c_int64, _c_int_p,
                                             _c_int_p]),
+    "ed_block_lanczos_window": (ctypes.c_int, [_vp, ctypes.POINTER(BlockOpts), ctypes.c_double, _dp, _dp, _dp,
+                                               ctypes.c_int64, _c_int_p, _c_int_p, _c_int_p]),
     "ed_gram_f64_dev": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp,
                                        ctypes.c_int64, _dp, _vp]),
     "ed_xc_f64_dev": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _dp,

@@ -55,10 +55,19 @@ def block_lanczos(sec: Sector, nblock: int = 8, neig: int = 8, nitermax: int = 1
     ev, res = np.zeros(nev), np.zeros(nev)
     vec = np.zeros((nev, sec.dim)) if vectors else None
     nconv, steps = ctypes.c_int(), ctypes.c_int()
-    check(lib().ed_block_lanczos(sec.handle, ctypes.byref(o), _dptr(ev), _dptr(res),
-                                 _dptr(vec) if vectors else None, sec.dim, ctypes.byref(nconv), ctypes.byref(steps)))
+    if dEmax is not None and vectors:
+        # only the vectors within dEmax of the lowest state cross to the host
+        # (ed_block_lanczos_window); the other rows of .last.vectors stay zero
+        nkept = ctypes.c_int()
+        check(lib().ed_block_lanczos_window(sec.handle, ctypes.byref(o), float(dEmax), _dptr(ev), _dptr(res),
+                                            _dptr(vec), sec.dim, ctypes.byref(nconv), ctypes.byref(steps),
+                                            ctypes.byref(nkept)))
+    else:
+        check(lib().ed_block_lanczos(sec.handle, ctypes.byref(o), _dptr(ev), _dptr(res),
+                                     _dptr(vec) if vectors else None, sec.dim, ctypes.byref(nconv),
+                                     ctypes.byref(steps)))
     full, ms, rms = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
     check(lib().ed_block_last_stats(None, ctypes.byref(full), ctypes.byref(ms), ctypes.byref(rms)))
     block_lanczos.last = BlockResult(ev, res, vec, nconv.value, steps.value, full.value, ms.value, rms.value)
-    keep = np.ones(nev, bool) if dEmax is None else (ev - ev.min() <= dEmax)
+    keep = np.ones(nev, bool) if dEmax is None else (ev - ev[0] <= dEmax)   # ascending: a leading run
     return ev[keep], (vec[keep] if vectors else None)

@@ -31,8 +31,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ed_kernels.h"
@@ -537,6 +543,145 @@ struct BlockOmega {
     }
 };
 
+// Ritz tests of the block driver on worker threads.  A test after `step` block steps
+// reduces the leading m x m block of T (m = step * b, row-major with leading dimension nmax)
+// to tridiagonal form, logging the rotations, and finds its nev lowest eigenvalues.  Later
+// block steps write T only outside that block, so a test may run while the driver goes on.
+constexpr int kRitzWorkers = 4;
+
+struct RitzJob {
+    int step = 0, rc = ED_OK;
+    bool ready = false;
+    std::string err;
+    std::vector<double> w;
+    BandTri bt;
+    double ms = 0.0;
+};
+
+class RitzPool {
+  public:
+    RitzPool(const double *T, int nmax, int b, int nev, int nthreads) : T_(T), nmax_(nmax), b_(b), nev_(nev)
+    {
+        for (int i = 0; i < nthreads; ++i) {
+            try {
+                th_.emplace_back([this] { run(); });
+            } catch (...) {
+                break;   // fewer threads; with none, get() runs the tests itself
+            }
+        }
+    }
+    ~RitzPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            queue_.clear();
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    // queue the test after `step` block steps, unless it is queued already
+    void submit(int step)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (jobs_.count(step)) return;
+        auto jb = std::make_shared<RitzJob>();
+        jb->step = step;
+        jobs_[step] = jb;
+        queue_.push_back(jb);
+        cv_.notify_one();
+    }
+    // tests queued or running
+    int busy()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        int k = 0;
+        for (auto &e : jobs_) k += e.second->ready ? 0 : 1;
+        return k;
+    }
+    // the finished test after `step` steps, or null (not queued, or with wait == false not
+    // finished); with wait, blocks until it has finished
+    std::shared_ptr<RitzJob> get(int step, bool wait)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = jobs_.find(step);
+        if (it == jobs_.end()) return nullptr;
+        auto jb = it->second;
+        if (wait && !jb->ready && th_.empty()) {   // no workers: run it here
+            for (auto q = queue_.begin(); q != queue_.end(); ++q)
+                if (*q == jb) {
+                    queue_.erase(q);
+                    break;
+                }
+            lk.unlock();
+            const int rc = compute(*jb);
+            lk.lock();
+            jb->rc = rc;
+            jb->ready = true;
+        }
+        if (wait) done_.wait(lk, [&] { return jb->ready; });
+        return jb->ready ? jb : nullptr;
+    }
+    // forget the tests at or below `step`, and those queued below `keep` that have not
+    // started (off the schedule now); a running one finishes on its own
+    void forget(int step, int keep)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = jobs_.begin(); it != jobs_.end() && it->first <= step;) it = jobs_.erase(it);
+        for (auto q = queue_.begin(); q != queue_.end();) {
+            if ((*q)->step < keep) {
+                jobs_.erase((*q)->step);
+                q = queue_.erase(q);
+            } else {
+                ++q;
+            }
+        }
+    }
+
+  private:
+    int compute(RitzJob &jb)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int m = jb.step * b_;
+        std::vector<double> Tm((size_t)m * m);
+        for (int r = 0; r < m; ++r) std::memcpy(&Tm[(size_t)r * m], T_ + (size_t)r * nmax_, sizeof(double) * m);
+        jb.w.assign(nev_, 0.0);
+        int rc = band_reduce(m, b_, Tm.data(), true, jb.bt);
+        if (rc == ED_OK) rc = band_tri_values(jb.bt, nev_, jb.w.data());
+        if (rc != ED_OK) jb.err = ed_last_error();
+        jb.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    }
+    void run()
+    {
+        for (;;) {
+            std::shared_ptr<RitzJob> jb;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+                if (stop_) return;
+                jb = queue_.front();
+                queue_.pop_front();
+            }
+            const int rc = compute(*jb);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                jb->rc = rc;
+                jb->ready = true;
+            }
+            done_.notify_all();
+        }
+    }
+    const double *T_;
+    const int nmax_, b_, nev_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::deque<std::shared_ptr<RitzJob>> queue_;
+    std::map<int, std::shared_ptr<RitzJob>> jobs_;
+    std::vector<std::thread> th_;
+    bool stop_ = false;
+};
+
 // The block Lanczos driver; Ritz vectors go to d_out (column-major n x nev, leading
 // dimension ldo) when d_out is not null.
 // the last block Lanczos call (ed_block_last_stats): steps, full projections, and host
@@ -615,9 +760,11 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     auto Tat = [&](int i, int j) -> double & { return T[(size_t)i * nmax + j]; };
     std::vector<double> Bnext, prev(nev, 1e300), w(nev), Z;
     int steps = 0;
-    int next_test = std::max(1, o.check_every);   // block steps at the next Ritz test
     bool exhausted = false;
-    std::vector<double> lastB;
+    // per completed step j: B_{j+1} (for the residual bounds of a Ritz test after j + 1
+    // steps) and the full projections so far
+    std::vector<std::vector<double>> Bhist;
+    std::vector<int> full_hist;
     // Partial reorthogonalisation (o.reorth == 1): the estimates Om[k] ~ Q_k^T Q_j (b x b,
     // k < j) of the block omega-recurrence; a step projects its new block against Q_{j-1}
     // and Q_j only, unless an estimate exceeds sqrt(eps) (then, and at the step after, against
@@ -661,12 +808,114 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
         }
         return ED_OK;
     };
+    // Ritz tests.  Their schedule is that of a sequential loop: the first after
+    // max(check_every, nev / b) steps; then after max(check_every, steps / 4) more steps while
+    // the wanted values still move by more than sqrt(tolerance) between tests (the band solver
+    // costs O(m^2 b) per test), after check_every steps once they move less (so convergence is
+    // seen within a few steps of the test that would see it); and at the last step.  The
+    // values come from one band reduction with its rotations logged, the vectors (residual
+    // bounds, Ritz vectors) from the same reduction once every wanted value has moved by less
+    // than `tolerance`, or at the last step.  Converged: that, and every residual bound
+    // ||B_{j+1} s_k(last b rows)|| below sqrt(tolerance) max(1, |theta_k|).
+    //
+    // A test reads only the leading block of T, so it runs on a worker thread as soon as its
+    // steps are done while the device runs further steps; once the values move by less than
+    // sqrt(tolerance), the next tests of the check_every cadence are started as well.  The
+    // results are taken in schedule order (a test off the schedule is never used), so the
+    // solve stops at the same step with the same values and vectors as the sequential loop;
+    // the device runs at most `lookahead` steps past the next scheduled test, into blocks
+    // outside the basis that is used.
+    const int every = std::max(1, o.check_every);
+    int sched = std::max(every, (nev + b - 1) / b);   // steps of the next scheduled test
+    bool cadence = false;   // the last test moved by < sqrt(tolerance): tests every `every` steps
+    int L = -1;             // steps at the last step, once reached
+    int avail = 0;          // completed steps
+    bool finished = false;
+    RitzPool pool(T.data(), nmax, b, nev, kRitzWorkers);
+    auto eff = [&]() { return (L >= 0 && sched >= L) ? L : sched; };
+    auto allow = [&](int jn) {   // may step jn (0-based) be issued now
+        const int lookahead = std::max(every * kRitzWorkers, sched / 4) + every;
+        return jn + 1 <= sched + lookahead;
+    };
+    auto dispatch = [&]() {
+        const int e = eff();
+        if (e > avail || e * b < nev) return;
+        pool.submit(e);
+        if (cadence)
+            for (int k = 1; k < kRitzWorkers; ++k) {
+                const int s2 = e + k * every;
+                if (s2 > avail || (L >= 0 && s2 > L) || pool.busy() >= kRitzWorkers) break;
+                pool.submit(s2);
+            }
+    };
+    // take the finished tests in schedule order (wait: block on the next one)
+    auto process = [&](bool wait) -> int {
+        for (;;) {
+            const int e = eff();
+            if (e > avail) return ED_OK;
+            if (e * b < nev) {   // the last step came before the first test: no Ritz pairs
+                finished = true;
+                steps = e;
+                return ED_OK;
+            }
+            dispatch();
+            auto jb = pool.get(e, wait);
+            if (!jb) return ED_OK;
+            wait = false;
+            if (jb->rc != ED_OK) return fail(jb->rc, jb->err);
+            const bool is_last = e == L;
+            bool settled = e > 1;
+            double moved = 0.0;
+            for (int k = 0; k < nev; ++k) {
+                const double dk = std::fabs(jb->w[k] - prev[k]);
+                if (!(dk < o.tolerance)) settled = false;
+                moved = std::max(moved, std::isfinite(dk) ? dk : 1e300);
+            }
+            w = jb->w;
+            ritz_ms += jb->ms;
+            const int m = e * b;
+            if (is_last || settled) {
+                const auto t_v = std::chrono::steady_clock::now();
+                Z.assign((size_t)m * nev, 0.0);
+                ED_TRY(band_tri_vectors(jb->bt, nev, w.data(), Z.data()));
+                ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_v).count();
+            }
+            if (is_last) {
+                finished = true;
+                steps = e;
+                return ED_OK;
+            }
+            bool conv = settled;
+            const std::vector<double> &lastB = Bhist[e - 1];
+            for (int k = 0; k < nev && conv; ++k) {
+                // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
+                double rr = 0.0;
+                for (int r = 0; r < b; ++r) {
+                    double t = 0.0;
+                    for (int c = r; c < b; ++c) t += lastB[(size_t)r * b + c] * Z[(size_t)k * m + (m - b + c)];
+                    rr += t * t;
+                }
+                const double rnorm = std::sqrt(rr);
+                if (!(std::fabs(w[k] - prev[k]) < o.tolerance &&
+                      rnorm < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))))
+                    conv = false;
+            }
+            for (int k = 0; k < nev; ++k) prev[k] = w[k];
+            if (conv) {
+                finished = true;
+                steps = e;
+                return ED_OK;
+            }
+            sched = e + (moved < std::sqrt(o.tolerance) ? every : std::max(every, e / 4));
+            cadence = moved < std::sqrt(o.tolerance);
+            pool.forget(e, sched);
+        }
+    };
     Step cur, nxt;
     ED_TRY(issue(0, cur));
     for (;;) {
         ED_HIP_CHECK(hipStreamSynchronize(st));
         const int j = cur.j, p = (j + 1) * b;
-        steps = j + 1;
         if (cur.full) {
             ++res.full_passes;
             if (full_due > 0) --full_due;
@@ -685,102 +934,68 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
             exhausted = true;   // the Krylov space is invariant: T is exact on it
             last = true;
         }
-        if (!last) {
-            ED_TRY(issue(j + 1, nxt));
-            if (partial) {
-                // omega-recurrence for Q_k^T Q_{j+1}, k < j; above sqrt(eps): project the new
-                // block against every block now (and the next block as well)
-                const bool over = om.advance(j, b, nmax, T, Bnext, cur.full);
-                if (over && !cur.full) {
-                    double *W = Q + (size_t)(j + 1) * b * ld;
-                    ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
-                    xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
-                    ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
-                    xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
-                    ED_TRY(chol_qr2_dev(dev, n, b, W, ld, cb_aux, nullptr, nullptr, st));
-                    ED_TRY(read_back(cb_aux, hb_aux));
-                    std::vector<double> R2;
-                    if (!chol_result(b, hb_aux, R2)) {
-                        exhausted = true;
-                        last = true;
-                    } else {
-                        // Q_{j+1} B_{j+1} = (Q' R2) B_{j+1}: the new B is R2 B (upper triangular)
-                        std::vector<double> Bn(bb, 0.0);
-                        for (int r = 0; r < b; ++r)
-                            for (int c = r; c < b; ++c) {
-                                double t = 0.0;
-                                for (int k = r; k <= c; ++k) t += R2[(size_t)r * b + k] * Bnext[(size_t)k * b + c];
-                                Bn[(size_t)r * b + c] = t;
-                            }
-                        Bnext = Bn;
-                        om.reset(j + 1);
-                    }
-                    ++res.full_passes;
-                    full_due = 1;
-                    if (!last) ED_TRY(issue(j + 1, nxt));   // the step issued above used the unprojected block
+        const bool ahead = !last && allow(j + 1);
+        if (ahead) ED_TRY(issue(j + 1, nxt));
+        if (!last && partial) {
+            // omega-recurrence for Q_k^T Q_{j+1}, k < j; above sqrt(eps): project the new
+            // block against every block now (and the next block as well)
+            const bool over = om.advance(j, b, nmax, T, Bnext, cur.full);
+            if (over && !cur.full) {
+                double *W = Q + (size_t)(j + 1) * b * ld;
+                ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG1, st));
+                xc_dev(n, p, b, Q, ld, dG1, -1.0, 1.0, W, ld, st);
+                ED_TRY(gram_dev(dev, n, p, b, Q, ld, W, ld, dG2, st));
+                xc_dev(n, p, b, Q, ld, dG2, -1.0, 1.0, W, ld, st);
+                ED_TRY(chol_qr2_dev(dev, n, b, W, ld, cb_aux, nullptr, nullptr, st));
+                ED_TRY(read_back(cb_aux, hb_aux));
+                std::vector<double> R2;
+                if (!chol_result(b, hb_aux, R2)) {
+                    exhausted = true;
+                    last = true;
+                } else {
+                    // Q_{j+1} B_{j+1} = (Q' R2) B_{j+1}: the new B is R2 B (upper triangular)
+                    std::vector<double> Bn(bb, 0.0);
+                    for (int r = 0; r < b; ++r)
+                        for (int c = r; c < b; ++c) {
+                            double t = 0.0;
+                            for (int k = r; k <= c; ++k) t += R2[(size_t)r * b + k] * Bnext[(size_t)k * b + c];
+                            Bn[(size_t)r * b + c] = t;
+                        }
+                    Bnext = Bn;
+                    om.reset(j + 1);
                 }
-            }
-            if (!last) {
-                for (int r = 0; r < b; ++r)
-                    for (int c = 0; c < b; ++c) {
-                        Tat((j + 1) * b + r, j * b + c) = Bnext[(size_t)r * b + c];
-                        Tat(j * b + c, (j + 1) * b + r) = Bnext[(size_t)r * b + c];
-                    }
-                lastB = Bnext;
+                ++res.full_passes;
+                full_due = 1;
+                // the step issued above used the unprojected block
+                if (!last && ahead) ED_TRY(issue(j + 1, nxt));
             }
         }
-        const int m = p;
-        // Ritz test after check_every steps, then at intervals that grow with T (steps / 4:
-        // the band solver costs O(m^2 b) per test) while the Ritz values still move by more
-        // than sqrt(tolerance) between tests, every check_every steps once they move less
-        // (so convergence is seen within a few steps of the test that would see it), and at
-        // the last step
-        if (m >= nev && (last || steps >= next_test)) {
-            // Ritz values of T_j (leading m x m block); the vectors (for the residual bounds
-            // and the Ritz vectors) only once the values have settled, or at the last step
-            std::vector<double> Tm((size_t)m * m);
-            for (int r = 0; r < m; ++r)
-                for (int c = 0; c < m; ++c) Tm[(size_t)r * m + c] = Tat(r, c);
-            const auto t_r = std::chrono::steady_clock::now();
-            // one reduction (rotations logged) serves the values and, once they settle, the vectors
-            BandTri bt;
-            ED_TRY(band_reduce(m, b, Tm.data(), true, bt));
-            ED_TRY(band_tri_values(bt, nev, w.data()));
-            bool settled = steps > 1;
-            double moved = 0.0;
-            for (int k = 0; k < nev; ++k) {
-                const double dk = std::fabs(w[k] - prev[k]);
-                if (!(dk < o.tolerance)) settled = false;
-                moved = std::max(moved, std::isfinite(dk) ? dk : 1e300);
-            }
-            const int every = std::max(1, o.check_every);
-            next_test = steps + (moved < std::sqrt(o.tolerance) ? every : std::max(every, steps / 4));
-            if (last || settled) {
-                Z.assign((size_t)m * nev, 0.0);
-                ED_TRY(band_tri_vectors(bt, nev, w.data(), Z.data()));
-            }
-            ritz_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_r).count();
-            if (last) break;
-            bool conv = settled;
-            for (int k = 0; k < nev && conv; ++k) {
-                // ||H x_k - theta_k x_k|| = ||B_{j+1} s_k(last b rows)||
-                double rr = 0.0;
-                for (int r = 0; r < b; ++r) {
-                    double t = 0.0;
-                    for (int c = r; c < b; ++c) t += lastB[(size_t)r * b + c] * Z[(size_t)k * m + (m - b + c)];
-                    rr += t * t;
+        if (!last)
+            for (int r = 0; r < b; ++r)
+                for (int c = 0; c < b; ++c) {
+                    Tat((j + 1) * b + r, j * b + c) = Bnext[(size_t)r * b + c];
+                    Tat(j * b + c, (j + 1) * b + r) = Bnext[(size_t)r * b + c];
                 }
-                const double rnorm = std::sqrt(rr);
-                if (!(std::fabs(w[k] - prev[k]) < o.tolerance &&
-                      rnorm < std::sqrt(o.tolerance) * std::max(1.0, std::fabs(w[k]))))
-                    conv = false;
+        Bhist.push_back(last ? std::vector<double>() : Bnext);
+        full_hist.push_back(res.full_passes);
+        avail = j + 1;
+        if (last) L = avail;
+        ED_TRY(process(false));
+        // with no step in flight: wait for tests until one may be issued or the solve ends
+        // (a step issued before the last one turned out last is dropped)
+        bool have_next = ahead && !last;
+        while (!finished && !have_next) {
+            if (L < 0 && allow(j + 1)) {
+                ED_TRY(issue(j + 1, nxt));
+                have_next = true;
+            } else {
+                ED_TRY(process(true));
             }
-            for (int k = 0; k < nev; ++k) prev[k] = w[k];
-            if (conv) break;   // a step issued meanwhile only wrote Q_{j+2} and the read-back
         }
-        if (last) break;   // m < nev at the last step: no Ritz pairs (rejected by the argument checks)
+        if (finished) break;
         cur = nxt;
     }
+    res.full_passes = full_hist[steps - 1];
     const int m = steps * b;
     (void)exhausted;
     if (Z.size() < (size_t)m * nev)
@@ -854,12 +1069,15 @@ static int block_check(ed_sector *sec, const ed_block_opts *o)
     return ED_OK;
 }
 
-int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *evecs,
-                     int64_t ldv, int *nconv, int *steps)
+// the host-vector forms: every eigenvector (window = +inf), or those within `window` of
+// the lowest eigenvalue (a leading run, the values being ascending)
+static int block_lanczos_host(ed_sector *sec, const ed_block_opts *opts, double window, double *evals, double *resid,
+                              double *evecs, int64_t ldv, int *nconv, int *steps, int *nkept)
 {
     ED_TRY(block_check(sec, opts));
     if (!evals) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: NULL evals");
     if (evecs && ldv < sec->dim) return ed::fail(ED_ERR_ARG, "ed_block_lanczos: ldv < dimension");
+    if (!(window >= 0.0)) return ed::fail(ED_ERR_ARG, "ed_block_lanczos_window: dEmax must be >= 0");
     ed::BlockResult r;
     double *d_out = nullptr;
     if (evecs) {
@@ -867,9 +1085,12 @@ int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, d
         if (e != hipSuccess) return ed::fail(ED_ERR_NOMEM, "ed_block_lanczos: eigenvector staging");
     }
     int rc = ed::block_lanczos(sec, *opts, d_out, sec->dim, r);
-    if (rc == ED_OK && evecs) {
+    int keep = 0;
+    if (rc == ED_OK)
+        while (keep < opts->nev && r.evals[keep] - r.evals[0] <= window) ++keep;
+    if (rc == ED_OK && evecs && keep > 0) {
         hipError_t e = hipMemcpy2D(evecs, sizeof(double) * ldv, d_out, sizeof(double) * sec->dim,
-                                   sizeof(double) * sec->dim, opts->nev, hipMemcpyDeviceToHost);
+                                   sizeof(double) * sec->dim, keep, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = ed::fail(ED_ERR_HIP, std::string("ed_block_lanczos: copy-out: ") + hipGetErrorString(e));
     }
     if (d_out) (void)hipFree(d_out);
@@ -880,7 +1101,20 @@ int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, d
     }
     if (nconv) *nconv = r.nconv;
     if (steps) *steps = r.steps;
+    if (nkept) *nkept = keep;
     return ED_OK;
+}
+
+int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *evecs,
+                     int64_t ldv, int *nconv, int *steps)
+{
+    return block_lanczos_host(sec, opts, HUGE_VAL, evals, resid, evecs, ldv, nconv, steps, nullptr);
+}
+
+int ed_block_lanczos_window(ed_sector *sec, const ed_block_opts *opts, double dEmax, double *evals, double *resid,
+                            double *evecs, int64_t ldv, int *nconv, int *steps, int *nkept)
+{
+    return block_lanczos_host(sec, opts, dEmax, evals, resid, evecs, ldv, nconv, steps, nkept);
 }
 
 int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *d_evecs,

@@ -160,6 +160,18 @@ module ed_hip
       integer(c_int64_t), value         :: ldv
       integer(c_int), intent(out)       :: nconv, steps
     end function
+    ! Same, keeping the eigenpairs within dEmax of the lowest (Block_Lanczos.f90:289-300): only
+    ! their vectors (the leading nkept columns of evecs) are copied
+    integer(c_int) function ed_block_lanczos_window(sec, opts, demax, evals, resid, evecs, ldv, nconv, steps, &
+        nkept) bind(C, name="ed_block_lanczos_window")
+      import :: c_int, c_int64_t, c_ptr, c_double, ed_block_opts_t
+      type(c_ptr), value                :: sec
+      type(ed_block_opts_t), intent(in) :: opts
+      real(c_double), value             :: demax
+      real(c_double)                    :: evals(*), resid(*), evecs(*)
+      integer(c_int64_t), value         :: ldv
+      integer(c_int), intent(out)       :: nconv, steps, nkept
+    end function
     ! Lanczos_GPU_updo for many sectors at once; secs(k) = sector handles
     integer(c_int) function ed_lanczos_batched(nsec, secs, niter, precision, alpha, beta, steps) &
         bind(C, name="ed_lanczos_batched")

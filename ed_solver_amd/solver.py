@@ -96,7 +96,8 @@ def get_eigen_energies(model: AIMParams, sectors=None, neigen: int = 1, dEmax: f
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
             elif not plain:
                 from .block import block_lanczos
-                ev, vec = block_lanczos(sec, nblock=block_size, neig=neigen, nitermax=nitermax, tolerance=tolerance)
+                ev, vec = block_lanczos(sec, nblock=block_size, neig=neigen, nitermax=nitermax, tolerance=tolerance,
+                                        dEmax=None if np.isinf(dEmax) else dEmax)
                 found = [(ev[k], vec[k]) for k in range(len(ev))]
                 if on_block is not None:
                     on_block((nu, nd), block_lanczos.last)
@@ -195,8 +196,12 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
         ms.append(result.ms)
         ritz.append(result.ritz_ms)
 
+    # a state is kept when e^{-beta (E - E0)} > weight_cutoff, i.e. E - E0 < ln(1 / weight_cutoff) / beta:
+    # that window, relative to each sector's lowest state and to E0, is get_eigen_energies'
+    # dEmax, so only those states' vectors are copied back from the device
     states = get_eigen_energies(model, sectors=kept, neigen=neigen, block_size=block_size, nitermax=nitermax,
-                                tolerance=tolerance, full_ed_switch=full_ed_switch, on_block=count)
+                                tolerance=tolerance, full_ed_switch=full_ed_switch, on_block=count,
+                                dEmax=max(0.0, float(np.log(1.0 / weight_cutoff) / beta)))
     t_pass2 = time.perf_counter() - t0
     states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
     info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,

@@ -428,6 +428,12 @@ int ed_block_default_opts(ed_block_opts *opts);   /* 8, 8, 128, 2, 1e-12, 1 */
  * steps (ISTOR(19)). */
 int ed_block_lanczos(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *evecs,
                      int64_t ldv, int *nconv, int *steps);
+/* Same, with the selection of Block_Lanczos_diagonalize (Block_Lanczos.f90:289-300): only
+ * the eigenpairs within dEmax of the lowest one are kept, and only their eigenvectors are
+ * copied to the host (the leading *nkept rows of evecs, the values being ascending; the
+ * other rows are not written).  evals / resid still receive all nev values.  dEmax >= 0. */
+int ed_block_lanczos_window(ed_sector *sec, const ed_block_opts *opts, double dEmax, double *evals, double *resid,
+                            double *evecs, int64_t ldv, int *nconv, int *steps, int *nkept);
 /* Same, eigenvectors written to device memory d_evecs (leading dimension ldv). */
 int ed_block_lanczos_dev(ed_sector *sec, const ed_block_opts *opts, double *evals, double *resid, double *d_evecs,
                          int64_t ldv, int *nconv, int *steps);

@@ -115,6 +115,27 @@ def test_block_lanczos_finds_degenerate_pairs(gpu, reorth):
     assert np.min(np.diff(w_all)) < 1e-9   # at least one exactly degenerate pair found
 
 
+def test_block_lanczos_window_copies_the_leading_vectors(gpu):
+    """ed_block_lanczos_window (the dEmax selection of Block_Lanczos.f90:289-300): the same
+    values as ed_block_lanczos, the vectors of the states within dEmax of the lowest (a
+    leading run) copied, the other rows left as they were."""
+    m, (nu, nd) = golden("B")
+    s = gpu.Sector(m, nu, nd)
+    ev, vec = gpu.block_lanczos(s, nblock=8, neig=8, nitermax=200, tolerance=1e-12)
+    d = 0.5 * ((ev[2] - ev[0]) + (ev[3] - ev[0]))   # between the third and the fourth state
+    print(f"window {d:.6f} over E - E0 = {ev - ev[0]}", flush=True)
+    ev_w, vec_w = gpu.block_lanczos(s, nblock=8, neig=8, nitermax=200, tolerance=1e-12, dEmax=d)
+    full = gpu.block_lanczos.last
+    keep = int(np.sum(ev - ev[0] <= d))
+    assert 0 < keep < 8 and len(ev_w) == keep and vec_w.shape == (keep, s.dim)
+    assert np.max(np.abs(ev_w - ev[:keep])) < 1e-12
+    assert np.max(np.abs(full.evals - ev)) < 1e-12
+    for k in range(keep):   # the same vectors up to sign
+        assert min(np.max(np.abs(vec_w[k] - vec[k])), np.max(np.abs(vec_w[k] + vec[k]))) < 1e-10
+    assert not np.any(full.vectors[keep:])
+    s.close()
+
+
 def test_solver_block_path(gpu):
     """get_eigen_energies with block_size=8 (solver.h:152: N=0, 1, Ns sectors keep plain
     Lanczos) against the oracle's eigenvalues, sector by sector."""

@@ -21,13 +21,14 @@
 //     the block steps or the Krylov space run out.
 //   Ritz vectors X = Q_{1..m} S, residuals ||H x_k - theta_k x_k|| on the device.
 //
-// Contractions (FP64, v_mfma_f64_16x16x4f64, one wave per 16 x 16 output tile):
-//   Gram    G = X^T Y   (X n x p, Y n x q, q <= 16): the reduction over n is split in
-//           row blocks; each workgroup writes its partial tile sums, a second kernel adds
-//           them in row-block order (deterministic);
-//   update  Y = beta Y + alpha X C  (C p x q): each wave owns 64 rows of Y and runs the
-//           reduction over p in MFMA steps of 4.
-// Both are HBM-bound (about b/4 flop per byte); MFMA keeps the instruction count low.
+// Contractions (FP64):
+//   Gram    G = X^T Y   (X n x p, Y n x q, q <= 16) on the matrix cores
+//           (v_mfma_f64_16x16x4f64, one wave per 16 x 16 output tile): the reduction over
+//           n is split in row blocks; each workgroup writes its partial tile sums, a second
+//           kernel adds them in row-block order (deterministic);
+//   update  Y = beta Y + alpha X C  (C p x q): one thread per row on the vector unit, so
+//           that every load and store of a wave is 512 contiguous bytes of one column.
+// Both are HBM-bound (about q/4 flop per byte).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -56,7 +57,11 @@ __device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c)
 }
 
 constexpr int kGramTiles = 4;    // 16-column tiles of X per Gram workgroup
-constexpr int kGramRowBlocks = 256;
+// Row blocks of a Gram product: with one tile group (p <= 16 kGramTiles) the grid is the
+// row blocks alone, and 1024 of them put four workgroups on every CU (one wave per SIMD is
+// too few to keep a streaming reduction at HBM speed); wider products have tile groups
+// enough with 256.
+constexpr int kGramRowBlocks = 256, kGramRowBlocksNarrow = 1024;
 
 // 16 consecutive rows [r, r + 16) of column `col` of a column-major matrix, zero
 // outside [0, n) or when !ok.
@@ -90,6 +95,8 @@ __global__ __launch_bounds__(256) void k_gram_part(int64_t n, int p, int q, cons
     dbl4 acc[kGramTiles];
 #pragma unroll
     for (int t = 0; t < kGramTiles; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // X^T X (the Cholesky-QR Gram): the X tile is the Y tile, loaded once
+    const bool same = X == Y && ldx == ldy && p == q;
     for (int64_t c0 = r_begin + 64 * w; c0 < r_end; c0 += 256) {
         const int64_t r = c0 + 16 * g;
         double yv[16];
@@ -100,7 +107,12 @@ __global__ __launch_bounds__(256) void k_gram_part(int64_t n, int p, int q, cons
             if (tile >= ntile) break;
             const int xc = tile * 16 + i16;
             double xv[16];
-            load_rows16(X, ldx, xc < p, xc, r, n, xv);
+            if (same) {
+#pragma unroll
+                for (int s = 0; s < 16; ++s) xv[s] = yv[s];
+            } else {
+                load_rows16(X, ldx, xc < p, xc, r, n, xv);
+            }
 #pragma unroll
             for (int s = 0; s < 16; ++s) acc[t] = mfma_f64_16x16x4(xv[s], yv[s], acc[t]);
         }
@@ -122,15 +134,29 @@ __global__ __launch_bounds__(256) void k_gram_part(int64_t n, int p, int q, cons
 
 // G[i][j] (row-major p x q) = sum over the nrb row blocks of the partial tiles.  One
 // workgroup per row i of G: thread t takes column j = t % 16 and the row blocks
-// b = t / 16, t / 16 + 16, ... (each step reads one contiguous 128-byte row of a tile
+// b = t / 16, t / 16 + 16, ... (each load reads one contiguous 128-byte row of a tile
 // per 16 threads), then the 16 partial sums of a column are added in group order: a
 // fixed order, so every call on the same data gives the same bits.
 __global__ __launch_bounds__(256) void k_gram_sum(int nrb, int p, int q, int p16, const double *part, double *G)
 {
     __shared__ double red[256];
     const int i = blockIdx.x, t = threadIdx.x, j = t & 15, grp = t >> 4;
-    double s = 0.0;
-    for (int b = grp; b < nrb; b += 16) s += part[((int64_t)b * p16 + i) * 16 + j];
+    // eight independent partial sums per thread (row blocks grp + 16 (8 u + v), v = 0..7),
+    // so that eight loads are in flight, added in a fixed order
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+    for (int b0 = grp; b0 < nrb; b0 += 128) {
+        double x[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {   // unconditional loads (clamped index), so none waits on another
+            const int b = min(b0 + 16 * v, nrb - 1);
+            x[v] = part[((int64_t)b * p16 + i) * 16 + j];
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acc[v] += b0 + 16 * v < nrb ? x[v] : 0.0;
+    }
+    const double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     red[t] = s;
     __syncthreads();
     if (t < q) {
@@ -140,51 +166,60 @@ __global__ __launch_bounds__(256) void k_gram_sum(int nrb, int p, int q, int p16
     }
 }
 
-// Y[:, 0:q] = beta Y + alpha X[:, 0:p] C (C row-major p x q, q <= 16).  Each wave owns
-// rows [r0, r0 + 64) as four 16-row tiles; MFMA step k covers columns k0 .. k0 + 3 of X:
-// lane l reads X[r0 + 16 t + l % 16][k0 + l / 16] (16 contiguous rows per lane group)
-// and C[k0 + l / 16][l % 16].  Y may alias X when p == q (every wave reads its rows of X
-// before it writes them).
-__global__ __launch_bounds__(256) void k_xc(int64_t n, int p, int q, const double *X, int64_t ldx, const double *C,
-                                            double alpha, double beta, double *Y, int64_t ldy)
+// Y[:, 0:q] = beta Y + alpha X[:, 0:p] C (C row-major p x q, q <= Q), one thread per row
+// on the vector unit: the product is HBM-bound (2q flop per 8-byte element of X, well below
+// the FP64 FMA rate), so what matters is that every load and store of a wave covers 64
+// consecutive rows of one column (512 contiguous bytes).  C passes through LDS in chunks
+// of 64 of its rows, zero-padded to Q columns; every thread reads the same LDS word
+// (broadcast).  Y may alias X when p == q <= 64: a thread reads its row of X before it
+// writes that row of Y.
+template <int Q>
+__global__ __launch_bounds__(256) void k_xc_rows(int64_t n, int p, int q, const double *X, int64_t ldx,
+                                                 const double *C, double alpha, double beta, double *Y, int64_t ldy)
 {
-    const int lane = threadIdx.x & 63, i16 = lane & 15, g = lane >> 4;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-    if (r0 >= n) return;
-    dbl4 acc[4];
+    __shared__ double sc[64 * Q];
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool ok = row < n;
+    double acc[Q];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    const bool full = r0 + 64 <= n;
-    for (int k0 = 0; k0 < p; k0 += 16) {
-        double a[4][4], bcol[4];
+    for (int c = 0; c < Q; ++c) acc[c] = 0.0;
+    for (int k0 = 0; k0 < p; k0 += 64) {
+        const int kn = min(64, p - k0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * Q; e += 256) {
+            const int k = e / Q, c = e % Q;
+            sc[e] = (k < kn && c < q) ? C[(int64_t)(k0 + k) * q + c] : 0.0;
+        }
+        __syncthreads();
+        if (!ok) continue;
+        const double *x = X + (int64_t)k0 * ldx + row;
+        int k = 0;
+        for (; k + 4 <= kn; k += 4) {
+            const double x0 = x[(int64_t)k * ldx], x1 = x[(int64_t)(k + 1) * ldx];
+            const double x2 = x[(int64_t)(k + 2) * ldx], x3 = x[(int64_t)(k + 3) * ldx];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = k0 + 4 * u + g;
-            const bool kok = k < p;
-            bcol[u] = (kok && i16 < q) ? C[(int64_t)k * q + i16] : 0.0;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int64_t r = r0 + 16 * t + i16;
-                a[u][t] = (kok && (full || r < n)) ? X[(int64_t)k * ldx + r] : 0.0;
+            for (int c = 0; c < Q; ++c) {
+                double t = acc[c];
+                t = __builtin_fma(x0, sc[k * Q + c], t);
+                t = __builtin_fma(x1, sc[(k + 1) * Q + c], t);
+                t = __builtin_fma(x2, sc[(k + 2) * Q + c], t);
+                acc[c] = __builtin_fma(x3, sc[(k + 3) * Q + c], t);
             }
         }
+        for (; k < kn; ++k) {
+            const double x0 = x[(int64_t)k * ldx];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = mfma_f64_16x16x4(a[u][t], bcol[u], acc[t]);
+            for (int c = 0; c < Q; ++c) acc[c] = __builtin_fma(x0, sc[k * Q + c], acc[c]);
+        }
     }
-    if (i16 >= q) return;
+    if (!ok) return;
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t row = r0 + 16 * t + 4 * r + g;
-            if (row < n) {
-                double *y = Y + (int64_t)i16 * ldy + row;
-                const double v = alpha * acc[t][r];
-                *y = beta == 0.0 ? v : beta * *y + v;
-            }
-        }
+    for (int c = 0; c < Q; ++c) {
+        if (c >= q) break;
+        double *y = Y + (int64_t)c * ldy + row;
+        const double v = alpha * acc[c];
+        *y = beta == 0.0 ? v : __builtin_fma(beta, *y, v);
+    }
 }
 
 // Deterministic start block in (-1, 1): the hash of (row mod 10000, column).
@@ -284,18 +319,22 @@ static int bws_get(int dev, size_t part_n, size_t small_n, BlockWS *&w)
     return ED_OK;
 }
 
-static int gram_rows(int64_t n)
+static int gram_rows(int64_t n, int p)
 {
-    const int64_t per = (n + kGramRowBlocks - 1) / kGramRowBlocks;
+    const int nrb = p <= 16 * kGramTiles ? kGramRowBlocksNarrow : kGramRowBlocks;
+    const int64_t per = (n + nrb - 1) / nrb;
     return (int)std::max<int64_t>(256, (per + 255) / 256 * 256);
 }
 
-// Partial-sum workspace of a Gram product with p columns of X over n rows.
+// Partial-sum workspace of the Gram products with up to p columns of X over n rows.
 static size_t gram_part_size(int64_t n, int p)
 {
-    const int p16 = (p + 15) / 16 * 16;
-    const int64_t rows = gram_rows(n);
-    return (size_t)((n + rows - 1) / rows) * p16 * 16;
+    auto size = [n](int pc) {
+        const int p16 = (pc + 15) / 16 * 16;
+        const int64_t rows = gram_rows(n, pc);
+        return (size_t)((n + rows - 1) / rows) * p16 * 16;
+    };
+    return std::max(size(p), size(std::min(p, 16 * kGramTiles)));
 }
 
 // d_G (device, row-major p x q) = X^T Y, queued on st (no host wait).  The partial-sum
@@ -305,7 +344,7 @@ static int gram_dev(int dev, int64_t n, int p, int q, const double *X, int64_t l
                     double *d_G, hipStream_t st)
 {
     const int p16 = (p + 15) / 16 * 16;
-    const int64_t rows = gram_rows(n);
+    const int64_t rows = gram_rows(n, p);
     const int nrb = (int)((n + rows - 1) / rows);
     BlockWS *w;
     ED_TRY(bws_get(dev, gram_part_size(n, p), 256, w));
@@ -333,9 +372,17 @@ static int gram(int dev, int64_t n, int p, int q, const double *X, int64_t ldx, 
 static void xc_dev(int64_t n, int p, int q, const double *X, int64_t ldx, const double *d_C, double alpha,
                    double beta, double *Y, int64_t ldy, hipStream_t st)
 {
-    const int64_t waves = (n + 63) / 64;
-    hipLaunchKernelGGL(k_xc, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta,
-                       Y, ldy);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (q <= 1)
+        hipLaunchKernelGGL(k_xc_rows<1>, grid, dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta, Y, ldy);
+    else if (q <= 2)
+        hipLaunchKernelGGL(k_xc_rows<2>, grid, dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta, Y, ldy);
+    else if (q <= 4)
+        hipLaunchKernelGGL(k_xc_rows<4>, grid, dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta, Y, ldy);
+    else if (q <= 8)
+        hipLaunchKernelGGL(k_xc_rows<8>, grid, dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta, Y, ldy);
+    else
+        hipLaunchKernelGGL(k_xc_rows<16>, grid, dim3(256), 0, st, n, p, q, X, ldx, d_C, alpha, beta, Y, ldy);
 }
 
 // Y = beta Y + alpha X C with C given on the host (row-major p x q).

@@ -33,6 +33,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -596,6 +597,16 @@ struct BlockOmega {
 // block steps write T only outside that block, so a test may run while the driver goes on.
 constexpr int kRitzWorkers = 4;
 
+// Worker threads of the Ritz tests: kRitzWorkers, or ED_BLOCK_RITZ_THREADS (0..16) from the
+// environment; with 0 every test runs on the driving thread when the schedule reaches it
+// (the same tests in the same order, so the same results).
+static int ritz_threads()
+{
+    const char *e = std::getenv("ED_BLOCK_RITZ_THREADS");
+    if (!e || !*e) return kRitzWorkers;
+    return std::max(0, std::min(16, std::atoi(e)));
+}
+
 struct RitzJob {
     int step = 0, rc = ED_OK;
     bool ready = false;
@@ -878,7 +889,7 @@ static int block_lanczos(ed_sector *s, const ed_block_opts &o, double *d_out, in
     int L = -1;             // steps at the last step, once reached
     int avail = 0;          // completed steps
     bool finished = false;
-    RitzPool pool(T.data(), nmax, b, nev, kRitzWorkers);
+    RitzPool pool(T.data(), nmax, b, nev, ritz_threads());
     auto eff = [&]() { return (L >= 0 && sched >= L) ? L : sched; };
     auto allow = [&](int jn) {   // may step jn (0-based) be issued now
         const int lookahead = std::max(every * kRitzWorkers, sched / 4) + every;

@@ -136,6 +136,29 @@ def test_block_lanczos_window_copies_the_leading_vectors(gpu):
     s.close()
 
 
+def test_block_lanczos_ritz_threads_do_not_change_the_result(gpu, monkeypatch):
+    """The Ritz tests run on worker threads beside the device steps (ED_BLOCK_RITZ_THREADS,
+    4 by default) or on the driving thread (0).  The driver takes the scheduled tests in
+    order either way, so the stopping step, the full projections, the values and the vectors
+    are the same; only the number of device steps run past the stopping one differs."""
+    m, (nu, nd) = golden("B")
+    s = gpu.Sector(m, nu, nd)
+    out = {}
+    for th in ("0", "1", "4"):
+        monkeypatch.setenv("ED_BLOCK_RITZ_THREADS", th)
+        ev, vec = gpu.block_lanczos(s, nblock=8, neig=8, nitermax=200, tolerance=1e-12)
+        r = gpu.block_lanczos.last
+        out[th] = (ev.copy(), vec.copy(), r.steps, r.full_passes)
+        print(f"Ritz threads {th}: steps {r.steps} full {r.full_passes} E0 {ev[0]:.15f}", flush=True)
+    ev0, vec0, steps0, full0 = out["0"]
+    for th in ("1", "4"):
+        ev, vec, steps, full = out[th]
+        assert steps == steps0 and full == full0
+        assert np.max(np.abs(ev - ev0)) < 1e-13
+        assert np.max(np.abs(vec - vec0)) < 1e-10
+    s.close()
+
+
 def test_solver_block_path(gpu):
     """get_eigen_energies with block_size=8 (solver.h:152: N=0, 1, Ns sectors keep plain
     Lanczos) against the oracle's eigenvalues, sector by sector."""

@@ -351,6 +351,7 @@ def main():
         t_g = time.perf_counter() - t0
         scan = {"sectors": info["sectors"], "kept_sectors": [list(k) for k in info["kept_sectors"]],
                 "states": info["states"], "E0": info["E0"], "t_lowest_all_sectors_s": info["t_pass1_s"],
+                "t_lowest_parts_s": {k: round(v, 4) for k, v in info["t_pass1_parts"].items()},
                 "t_block_lanczos_s": info["t_pass2_s"], "t_total_s": t_scan, "t_green_s": t_g,
                 "block_steps": info.get("block_steps"), "block_full_passes": info.get("block_full_passes"),
                 "block_ms": info.get("block_ms"), "block_ritz_ms": info.get("block_ritz_ms"),

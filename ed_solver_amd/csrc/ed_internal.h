@@ -191,11 +191,21 @@ struct GraphCache {
     void *base = nullptr;
 };
 
+// The HIP stream of a sector, taken from the per-device stream pool when first used (the
+// conversion below): a sector that only joins batched work, which runs on another sector's
+// stream, never needs one of its own.  The sector build runs on a pooled stream that goes
+// back to the pool when the build is done.
+struct SectorStream {
+    mutable hipStream_t s = nullptr;
+    int device = 0;
+    operator hipStream_t() const;
+};
+
 }  // namespace ed
 
 struct ed_sector {
     int device = 0;
-    hipStream_t stream = nullptr;
+    ed::SectorStream stream;
     int nc = 0, nb = 0, ns = 0, nup = 0, ndn = 0, ncfg = 0;
     int64_t dim = 0;
     std::vector<double> ec, eb, vbc, u, jj;

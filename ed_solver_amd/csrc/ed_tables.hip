@@ -632,6 +632,21 @@ static void stream_release(int device, hipStream_t s)
     else (void)hipStreamDestroy(s);
 }
 
+SectorStream::operator hipStream_t() const
+{
+    if (!s) {
+        int cur = -1;
+        const bool other = hipGetDevice(&cur) == hipSuccess && cur != device;
+        if (other) (void)hipSetDevice(device);
+        if (stream_acquire(&s) != hipSuccess) {
+            (void)hipGetLastError();
+            s = nullptr;
+        }
+        if (other) (void)hipSetDevice(cur);
+    }
+    return s;
+}
+
 static void free_sector(ed_sector *S)
 {
     spin_release(S->up, S->up_key);
@@ -648,7 +663,7 @@ static void free_sector(ed_sector *S)
     if (S->h_mon) (void)hipHostFree(S->h_mon);
     if (S->ev0) (void)hipEventDestroy(S->ev0);
     if (S->ev1) (void)hipEventDestroy(S->ev1);
-    stream_release(S->device, S->stream);
+    stream_release(S->device, S->stream.s);
     delete S;
 }
 
@@ -689,7 +704,8 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
     }
     int rc = ED_OK;
     auto bail = [&](int code) { free_sector(S); return code; };
-    if (stream_acquire(&S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    S->stream.device = S->device;
+    if (stream_acquire(&S->stream.s) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
     if ((rc = dalloc(&S->d_rank, (size_t)1 << ns)) != ED_OK) return bail(rc);
     hipLaunchKernelGGL(k_rank_table, dim3(nblocks((int64_t)1 << ns)), dim3(kBlock), 0, S->stream, ns, S->d_rank);
     // the spin tables, shared with the other sectors of the model (with the packed
@@ -705,6 +721,8 @@ static int create_sector(const ed_model *m, int nup, int ndn, ed_sector **out, c
     if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
         return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
     if (hipStreamSynchronize(S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "sector build failed"));
+    stream_release(S->device, S->stream.s);   // the sector takes a stream again when it first needs one
+    S->stream.s = nullptr;
     *out = S;
     return ED_OK;
 }
@@ -923,7 +941,8 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     }
     int rc = ED_OK;
     auto bail = [&](int code) { free_sector(S); return code; };
-    if (stream_acquire(&S->stream) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
+    S->stream.device = S->device;
+    if (stream_acquire(&S->stream.s) != hipSuccess) return bail(fail(ED_ERR_HIP, "hipStreamCreate failed"));
     hipStream_t st = S->stream;
     const int nbits = 2 * ns;
     if ((rc = dalloc(&S->d_rank, (size_t)1 << nbits)) != ED_OK) return bail(rc);
@@ -1075,6 +1094,8 @@ static int create_sector_sz(const ed_model *m, const double *pb, const double *p
     if (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess)
         return bail(fail(ED_ERR_HIP, "hipEventCreate failed"));
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(ED_ERR_HIP, "S_z sector build failed"));
+    stream_release(S->device, S->stream.s);   // the sector takes a stream again when it first needs one
+    S->stream.s = nullptr;
     *out = S;
     return ED_OK;
 }
@@ -1152,7 +1173,7 @@ int ed_sector_create_slater(const ed_model *model, const double *u4, int nup, in
 int ed_sector_destroy(ed_sector *sec)
 {
     if (!sec) return ED_OK;
-    (void)hipStreamSynchronize(sec->stream);
+    if (sec->stream.s) (void)hipStreamSynchronize(sec->stream.s);   // a sector that never took a stream has none to wait on
     ed::free_sector(sec);
     return ED_OK;
 }

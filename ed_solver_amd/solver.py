@@ -174,16 +174,25 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
             return Sector(reversed_model, nd, nu)
         return Sector(model, nu, nd)
 
+    t_parts = {"dense_s": time.perf_counter() - t0, "create_s": 0.0, "batched_s": 0.0, "release_s": 0.0,
+               "converged_s": 0.0}
     for niter, keys in groups.items():
+        t1 = time.perf_counter()
         secs = [pass1_sector(k) for k in keys]
+        t2 = time.perf_counter()
         try:
             a, b, steps = lanczos_batched(secs, niter)
         finally:
+            t3 = time.perf_counter()
             for sec in secs:
                 sec.close()
+        t4 = time.perf_counter()
         for i, key in enumerate(keys):
             _, ev = lanczos_converged(a[i, :steps[i]], b[i, :steps[i]], 1, tolerance, nitermax)
             e_low[key] = float(ev[0])
+        for name, dt in (("create_s", t2 - t1), ("batched_s", t3 - t2), ("release_s", t4 - t3),
+                         ("converged_s", time.perf_counter() - t4)):
+            t_parts[name] += dt
     t_pass1 = time.perf_counter() - t0
     e0 = min(e_low.values())
     kept = sorted(s for s, e in e_low.items() if np.exp(-beta * (e - e0)) > weight_cutoff)
@@ -205,7 +214,7 @@ def sector_scan(model: AIMParams, beta: float, weight_cutoff: float = 1e-6, bloc
     t_pass2 = time.perf_counter() - t0
     states = [st for st in states if np.exp(-beta * (st.energy - e0)) > weight_cutoff]
     info = {"sectors": len(e_low), "kept_sectors": kept, "states": len(states), "E0": e0,
-            "t_pass1_s": t_pass1, "t_pass2_s": t_pass2, "e_low": e_low, "block_steps": steps,
+            "t_pass1_s": t_pass1, "t_pass1_parts": t_parts, "t_pass2_s": t_pass2, "e_low": e_low, "block_steps": steps,
             "block_full_passes": fulls, "block_ms": ms, "block_ritz_ms": ritz}
     return states, info
 

@@ -33,7 +33,7 @@ def model_c():
     return AIMParams.from_json(d["model"])
 
 
-def test_config_d_sector_selection(gpu):
+def test_ns12_scan_sector_selection(gpu):
     from ed_solver_amd.solver import sector_scan
     m = model_c()
     states, info = sector_scan(m, beta=BETA, weight_cutoff=WCUT, block_size=8, neigen=8)
@@ -61,13 +61,13 @@ def test_config_d_sector_selection(gpu):
 
 
 @pytest.mark.parametrize("sector", KEPT)
-def test_config_d_block_lanczos_sector(gpu, sector):
+def test_ns12_scan_block_lanczos_sector(gpu, sector):
     """The 8 lowest states of a kept sector by block Lanczos of width 8 with FP64 MFMA
     contractions, as the scan's pass 2 computes them."""
     from ed_solver_amd.solver import get_eigen_energies
     m = model_c()
     nu, nd = sector
-    print(f"config D sector {sector} dim {comb(m.ns, nu) * comb(m.ns, nd)}", flush=True)
+    print(f"Ns=12 scan sector {sector} dim {comb(m.ns, nu) * comb(m.ns, nd)}", flush=True)
     states = get_eigen_energies(m, sectors=[sector], neigen=8, block_size=8, nitermax=128, tolerance=1e-12)
     assert len(states) == 8
     ev = np.array([st.energy for st in states])

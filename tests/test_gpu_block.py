@@ -1,8 +1,8 @@
 """Block Lanczos (Block_Lanczos_diagonalize, Block_Lanczos.f90:37-355) and its FP64 MFMA
 contractions on the GPU, through the C-ABI.
 
-Bars: the MFMA Gram / update products against NumPy to 1e-13 relative (this also pins
-the matrix-core lane layout the kernels assume); the block Lanczos eigenvalues against
+Bars: the Gram (matrix cores) and update (vector unit) products against NumPy to 1e-13
+relative (this also pins the matrix-core lane layout the Gram kernel assumes); the block Lanczos eigenvalues against
 LAPACK / ARPACK on the oracle's H.v (oracle.lowest_eigs) to 1e-10, residuals
 ||H x - E x|| below 1e-6, Ritz vectors orthonormal to 1e-10."""
 import ctypes
